@@ -1,0 +1,190 @@
+/*
+ * medimage_amd: MI355X-native Ambrosio-Tortorelli (Mumford-Shah) low-dose
+ * CT reconstruction.  C ABI of libmedimage_amd.so.
+ *
+ * The call surface keeps the one of the reference drivers:
+ *   - a geometry struct whose fields map onto industry3d's Parameter
+ *     (industry3d/include/ct3d.h:17-63) and GoldenC's config.h
+ *     (origin/GoldenC/src/config.h:4-17);
+ *   - sinogram / image / edge buffers laid out as the reference lays them
+ *     out (row-major [view][bin] sinogram, row-major [x][y] image, as
+ *     origin/GoldenC/src/main.c:8-10 and msbeam/CPU_OpenMP/src/main.c:12-15);
+ *   - alpha / beta / epsilon and the iteration counts of GoldenC
+ *     (origin/GoldenC/src/main.c:25-35) plus the ordered-subset controls.
+ *
+ * Conventions.  Every entry point returns 0 on success or a negative
+ * MI_E_* code; mi_last_error() returns a thread-local message for the last
+ * failure.  Host pointers are caller-owned and copied in and out; *_dev
+ * variants take device pointers and do not copy.  Calls on one context are
+ * serialised on that context's HIP stream and are blocking unless named
+ * *_async.  A context is not shared between threads without external
+ * locking.  Nothing here ever falls back to a CPU path: without a usable
+ * gfx950 device, mi_context_create fails with MI_E_DEVICE.
+ */
+#ifndef MEDIMAGE_AMD_H
+#define MEDIMAGE_AMD_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MI_ABI_VERSION 1
+
+enum mi_status {
+    MI_OK = 0,
+    MI_E_ARG = -1,        /* invalid argument or geometry                   */
+    MI_E_DEVICE = -2,     /* no usable device / HIP runtime error           */
+    MI_E_NOMEM = -3,      /* device memory exhausted                        */
+    MI_E_STATE = -4,      /* call not valid in the object's current state   */
+    MI_E_LIMIT = -5       /* size beyond a documented maximum               */
+};
+
+/* Beam / projector kinds. */
+enum mi_beam {
+    /* 2D fan beam, flat equidistant detector, parametric Siddon in double:
+     * ray construction of industry3d/src/tracing.cpp:51-81 (cone(), mid-plane)
+     * and tracer of helical/rebin/src/tracing.cpp:22-147. */
+    MI_BEAM_FAN_FLAT = 0,
+    /* 2D parallel beam over pi, unit detector cells, incremental tracer of
+     * msbeam/CPU_OpenMP/src/proj.c:54-211 (= blockrecon/src/wray.cpp:10-167). */
+    MI_BEAM_PARALLEL = 1,
+    /* GoldenC symmetric tiled DDA, origin/GoldenC/src/proj.c:14-397
+     * (512 x 512 image, 180 views, 767 bins only). */
+    MI_BEAM_GOLDENC = 2
+};
+
+enum mi_precision { MI_FP32 = 0, MI_FP64 = 1 };
+
+/* Geometry.  Field <- reference name (industry3d Parameter / GoldenC config.h). */
+typedef struct mi_geometry {
+    int32_t beam;          /* BEAM ("Parallel"/"Cone") -> enum mi_beam         */
+    int32_t nx, ny;        /* NX, NY: image rows (slow) and columns (fast)     */
+    double  pixel_mm;      /* SAMPLESIZE (vx = vy)                             */
+    int32_t n_views;       /* NPROJ / nAlpha                                   */
+    double  span_deg;      /* angular span; 360 for fan, 180 for parallel      */
+    int32_t n_bins;        /* NDX / projWidth / NRAY                           */
+    double  bin_mm;        /* PIXELSIZE (detector pitch at the detector)       */
+    double  sod_mm;        /* SOD                                              */
+    double  sdd_mm;        /* SDD                                              */
+    double  det_offset;    /* OFF_CENTER, in bins                              */
+    int32_t flavour;       /* MI_BEAM_PARALLEL only: 0 msbeam constants
+                              (double pi/tolerances), 1 blockrecon constants
+                              (float, blockrecon/src/config.h:35-38)           */
+    int32_t reserved;
+} mi_geometry;
+
+/* AT parameters.  alpha/beta/epsilon as in
+ * origin/MATLAB/MumfordShah_reg.m:4-7 and origin/GoldenC/src/config.h:6-8. */
+typedef struct mi_at_params {
+    double  alpha, beta, epsilon;
+    int32_t iter_alt;       /* _IterAlt   (GoldenC form)                       */
+    int32_t iter_image;     /* _IterImage (GoldenC form)                       */
+    int32_t iter_edge;      /* _IterEdge  (both forms; edge SD steps per pass) */
+    int32_t n_subsets;      /* ordered subsets (OS form)                       */
+    int32_t max_iters;      /* AT iterations cap (OS form)                     */
+    int32_t median_every;   /* OS form: 3x3 median of u every k passes, 0 off  */
+    double  rel_tol;        /* OS form: stop when |df_k - df_{k-1}| < rel_tol*df_{k-1} */
+    double  lower, upper;   /* LOWER_X / UPPER_X clamp of u (OS form)          */
+    int32_t precision;      /* enum mi_precision                               */
+    int32_t reserved;
+} mi_at_params;
+
+/* Result report of a reconstruction. */
+typedef struct mi_report {
+    int32_t iterations;     /* AT passes (OS form) / alternations (GoldenC)    */
+    int32_t converged;      /* 1 if the rel_tol test stopped the run           */
+    double  data_fidelity;  /* last pass' sum_s ||g_s - A_s u||^2               */
+    double  rel_change;     /* last relative data-fidelity change              */
+    double  elapsed_ms;     /* device time of the solve (HIP events)           */
+    int64_t nnz;            /* system-matrix entries of the geometry           */
+} mi_report;
+
+typedef struct mi_context mi_context;
+
+/* ---- context ----------------------------------------------------------- */
+int mi_context_create(int device, mi_context **out);
+int mi_context_destroy(mi_context *ctx);
+const char *mi_last_error(void);
+/* Bytes of device memory the context holds (matrices + work buffers). */
+int mi_context_mem(mi_context *ctx, int64_t *bytes);
+/* Drop cached system matrices (all geometries). */
+int mi_context_trim(mi_context *ctx);
+
+/* ---- geometry / system matrix ------------------------------------------ */
+/* Traces the geometry (if not cached) with n_subsets view subsets and
+ * returns its entry count.  Later calls with the same geometry reuse it. */
+int mi_geometry_prepare(mi_context *ctx, const mi_geometry *g, int n_subsets, int64_t *nnz);
+/* Copies the (index, length) sequence of ray (view, bin) as traced on the
+ * device: up to cap entries; *numb receives the full count.  Lengths are the
+ * tracer's own type widened to double (float for MI_BEAM_PARALLEL). */
+int mi_trace_ray(mi_context *ctx, const mi_geometry *g, int view, int bin,
+                 int32_t *ind, double *len, int cap, int *numb);
+
+/* Copies the whole one-subset system matrix of the geometry in the caller's
+ * ray order (ray = view * n_bins + bin): rowptr[n_views*n_bins + 1] (entry
+ * offsets), ind[nnz] pixel indices, len[nnz] lengths widened to double.
+ * nnz must equal the value mi_geometry_prepare reports. */
+int mi_matrix_export(mi_context *ctx, const mi_geometry *g, int64_t *rowptr, int32_t *ind, double *len,
+                     int64_t nnz);
+
+/* ---- projectors (replace A / AStar) --------------------------------------
+ * forward:     sino[view][bin] = sum over the ray of img[x][y] * len
+ *              (origin/GoldenC/src/proj.c:14-204, msbeam proj.c:14-32)
+ * backproject: img[x][y] = sum over rays through the pixel of sino * len,
+ *              gather form, deterministic (proj.c:206-397, msbeam proj.c:34-52)
+ */
+int mi_forward(mi_context *ctx, const mi_geometry *g, const float *img, float *sino);
+int mi_backproject(mi_context *ctx, const mi_geometry *g, const float *sino, float *img);
+int mi_forward_f64(mi_context *ctx, const mi_geometry *g, const double *img, double *sino);
+int mi_backproject_f64(mi_context *ctx, const mi_geometry *g, const double *sino, double *img);
+
+/* ---- AT building blocks (GoldenC, origin/GoldenC/src/main.c) ------------- */
+/* minimize_in_IMAGE_variable (main.c:81-257): CG image step, FP64. */
+int mi_image_step_f64(mi_context *ctx, const mi_geometry *g, double *f_out, const double *sino,
+                      const double *f, const double *v, double alpha, double epsilon,
+                      int iter_image);
+/* minimize_in_EDGE_variable (main.c:259-360): SD edge step, FP64. */
+int mi_edge_step_f64(mi_context *ctx, int nx, int ny, double *v_out, const double *f,
+                     const double *v, double alpha, double beta, double epsilon, int iter_edge);
+/* medfilt2 (main.c:366-390): 3x3 median, zero padding. */
+int mi_medfilt2_f64(mi_context *ctx, int nx, int ny, double *f);
+
+/* ---- drivers ------------------------------------------------------------- */
+/* MumfordShah (main.c:35-49): f = 0, v = 1, iter_alt x (image step, edge
+ * step), FP64 throughout. */
+int mi_mumford_shah(mi_context *ctx, const mi_geometry *g, const mi_at_params *p,
+                    const double *sino, double *f, double *v, mi_report *rep);
+/* Ordered-subset AT reconstruction of one job (the plain per-job GPU form).
+ * u and v are read as the starting point and overwritten. */
+int mi_at_reconstruct(mi_context *ctx, const mi_geometry *g, const mi_at_params *p,
+                      const float *sino, float *u, float *v, mi_report *rep);
+int mi_at_reconstruct_f64(mi_context *ctx, const mi_geometry *g, const mi_at_params *p,
+                          const double *sino, double *u, double *v, mi_report *rep);
+
+/* ---- instrumentation ----------------------------------------------------- */
+enum mi_kernel_id {
+    MI_K_FP = 0,          /* forward projection of a subset (residual form)  */
+    MI_K_BP = 1,          /* gather back projection + image gradient         */
+    MI_K_FPSQ = 2,        /* forward projection of the direction, ||A d||^2  */
+    MI_K_STEP = 3,        /* line-search step + update                       */
+    MI_K_EDGE = 4,        /* edge (v) step kernels                           */
+    MI_K_COUNT = 8
+};
+typedef struct mi_kernel_stats {
+    int64_t launches;
+    double  total_ms;     /* sum of HIP-event durations of the launches      */
+    double  bytes;        /* algorithmic bytes moved by those launches       */
+} mi_kernel_stats;
+/* Enable per-launch HIP-event timing of the kernels above (adds one event
+ * pair per launch on the context's stream). */
+int mi_stats_enable(mi_context *ctx, int on);
+int mi_stats_get(mi_context *ctx, int kernel, mi_kernel_stats *out);
+int mi_stats_reset(mi_context *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,284 @@
+"""medimage_amd: MI355X-native Ambrosio-Tortorelli low-dose CT reconstruction.
+
+Python mirror of the C ABI in include/medimage_amd.h (ctypes).  The compute
+path is libmedimage_amd.so (HIP, gfx950); importing this package without the
+built library raises, and there is no CPU fallback.
+
+Names follow the reference drivers: ``forward`` / ``backproject`` replace
+``A`` / ``AStar`` (origin/GoldenC/src/proj.h:6-7, msbeam/CPU_OpenMP/src/proj.h:7-8),
+``mumford_shah`` replaces ``MumfordShah`` (origin/GoldenC/src/main.c:35-49),
+``image_step`` / ``edge_step`` replace ``minimize_in_IMAGE_variable`` /
+``minimize_in_EDGE_variable`` (main.c:81-360), ``at_reconstruct`` is the
+ordered-subset driver that takes the place of ``MSbeam``
+(msbeam/CPU_OpenMP/src/main.c:364-394) on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from pathlib import Path
+
+import numpy as np
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "libmedimage_amd.so"
+
+MI_OK, MI_E_ARG, MI_E_DEVICE, MI_E_NOMEM, MI_E_STATE, MI_E_LIMIT = 0, -1, -2, -3, -4, -5
+MI_BEAM_FAN_FLAT, MI_BEAM_PARALLEL, MI_BEAM_GOLDENC = 0, 1, 2
+MI_FP32, MI_FP64 = 0, 1
+MI_K_FP, MI_K_BP, MI_K_FPSQ, MI_K_STEP, MI_K_EDGE = 0, 1, 2, 3, 4
+
+
+class Geometry(C.Structure):
+    _fields_ = [
+        ("beam", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("pixel_mm", C.c_double),
+        ("n_views", C.c_int32), ("span_deg", C.c_double), ("n_bins", C.c_int32), ("bin_mm", C.c_double),
+        ("sod_mm", C.c_double), ("sdd_mm", C.c_double), ("det_offset", C.c_double),
+        ("flavour", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+    def key(self):
+        return tuple(getattr(self, f) for f, _ in self._fields_)
+
+    def __repr__(self):
+        return "Geometry(" + ", ".join(f"{f}={getattr(self, f)}" for f, _ in self._fields_[:-1]) + ")"
+
+
+class AtParams(C.Structure):
+    _fields_ = [
+        ("alpha", C.c_double), ("beta", C.c_double), ("epsilon", C.c_double),
+        ("iter_alt", C.c_int32), ("iter_image", C.c_int32), ("iter_edge", C.c_int32),
+        ("n_subsets", C.c_int32), ("max_iters", C.c_int32), ("median_every", C.c_int32),
+        ("rel_tol", C.c_double), ("lower", C.c_double), ("upper", C.c_double),
+        ("precision", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
+class Report(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int32), ("converged", C.c_int32), ("data_fidelity", C.c_double),
+        ("rel_change", C.c_double), ("elapsed_ms", C.c_double), ("nnz", C.c_int64),
+    ]
+
+
+class KernelStats(C.Structure):
+    _fields_ = [("launches", C.c_int64), ("total_ms", C.c_double), ("bytes", C.c_double)]
+
+
+def fan_geometry(n, pixel_mm, views, bins, bin_mm, sod=541.0, sdd=949.0, span_deg=360.0, det_offset=0.0):
+    """2D fan beam, flat equidistant detector (MI_BEAM_FAN_FLAT)."""
+    return Geometry(MI_BEAM_FAN_FLAT, n, n, pixel_mm, views, span_deg, bins, bin_mm, sod, sdd, det_offset, 0, 0)
+
+
+def parallel_geometry(n=512, views=180, bins=768, flavour=0):
+    """msbeam/blockrecon parallel beam (MI_BEAM_PARALLEL); flavour 0 msbeam, 1 blockrecon."""
+    return Geometry(MI_BEAM_PARALLEL, n, n, 1.0, views, 180.0, bins, 1.0, 0.0, 0.0, 0.0, flavour, 0)
+
+
+def goldenc_geometry():
+    """GoldenC tiled DDA: 512 x 512, 180 views, 767 bins (MI_BEAM_GOLDENC)."""
+    return Geometry(MI_BEAM_GOLDENC, 512, 512, 1.0, 180, 180.0, 767, 1.0, 0.0, 0.0, 0.0, 0, 0)
+
+
+def at_params(alpha=1e-3, beta=1e-4, epsilon=1e-2, n_subsets=1, max_iters=50, iter_edge=1, rel_tol=0.0,
+              median_every=0, lower=-math.inf, upper=math.inf, iter_alt=10, iter_image=10, precision=MI_FP32):
+    return AtParams(alpha, beta, epsilon, iter_alt, iter_image, iter_edge, n_subsets, max_iters, median_every,
+                    rel_tol, lower, upper, precision, 0)
+
+
+class MIError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"medimage_amd error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() (there is no CPU fallback)")
+    lib = C.CDLL(str(LIB_PATH))
+    P = C.c_void_p
+    G = C.POINTER(Geometry)
+    sigs = {
+        "mi_context_create": [C.c_int, C.POINTER(C.c_void_p)],
+        "mi_context_destroy": [P],
+        "mi_context_mem": [P, C.POINTER(C.c_int64)],
+        "mi_context_trim": [P],
+        "mi_geometry_prepare": [P, G, C.c_int, C.POINTER(C.c_int64)],
+        "mi_trace_ray": [P, G, C.c_int, C.c_int, P, P, C.c_int, C.POINTER(C.c_int)],
+        "mi_matrix_export": [P, G, P, P, P, C.c_int64],
+        "mi_forward": [P, G, P, P],
+        "mi_backproject": [P, G, P, P],
+        "mi_forward_f64": [P, G, P, P],
+        "mi_backproject_f64": [P, G, P, P],
+        "mi_image_step_f64": [P, G, P, P, P, P, C.c_double, C.c_double, C.c_int],
+        "mi_edge_step_f64": [P, C.c_int, C.c_int, P, P, P, C.c_double, C.c_double, C.c_double, C.c_int],
+        "mi_medfilt2_f64": [P, C.c_int, C.c_int, P],
+        "mi_mumford_shah": [P, G, C.POINTER(AtParams), P, P, P, C.POINTER(Report)],
+        "mi_at_reconstruct": [P, G, C.POINTER(AtParams), P, P, P, C.POINTER(Report)],
+        "mi_at_reconstruct_f64": [P, G, C.POINTER(AtParams), P, P, P, C.POINTER(Report)],
+        "mi_stats_enable": [P, C.c_int],
+        "mi_stats_get": [P, C.c_int, C.POINTER(KernelStats)],
+        "mi_stats_reset": [P],
+    }
+    for name, args in sigs.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    lib.mi_last_error.restype = C.c_char_p
+    lib.mi_last_error.argtypes = []
+    _lib = lib
+    return lib
+
+
+def lib():
+    return _load()
+
+
+def _check(rc):
+    if rc != MI_OK:
+        raise MIError(rc, _load().mi_last_error().decode())
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _arr(a, dtype, shape=None):
+    a = np.ascontiguousarray(a, dtype=dtype)
+    if shape is not None and a.shape != tuple(shape):
+        a = a.reshape(shape)
+    return a
+
+
+class Context:
+    """One device, one HIP stream, a cache of system matrices (mi_context)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = _load()
+        h = C.c_void_p()
+        _check(self._lib.mi_context_create(device, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.mi_context_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- geometry ---------------------------------------------------------
+    def prepare(self, g: Geometry, n_subsets: int = 1) -> int:
+        nnz = C.c_int64()
+        _check(self._lib.mi_geometry_prepare(self._h, C.byref(g), n_subsets, C.byref(nnz)))
+        return nnz.value
+
+    def trace_ray(self, g: Geometry, view: int, bin: int, cap: int = 8192):
+        ind = np.zeros(cap, np.int32)
+        ln = np.zeros(cap, np.float64)
+        n = C.c_int()
+        _check(self._lib.mi_trace_ray(self._h, C.byref(g), view, bin, _ptr(ind), _ptr(ln), cap, C.byref(n)))
+        k = min(n.value, cap)
+        return ind[:k].copy(), ln[:k].copy()
+
+    def matrix(self, g: Geometry):
+        """(rowptr, ind, len) of the one-subset matrix, rays in [view][bin] order."""
+        nnz = self.prepare(g, 1)
+        rp = np.zeros(g.n_views * g.n_bins + 1, np.int64)
+        ind = np.zeros(max(nnz, 1), np.int32)
+        ln = np.zeros(max(nnz, 1), np.float64)
+        _check(self._lib.mi_matrix_export(self._h, C.byref(g), _ptr(rp), _ptr(ind), _ptr(ln), nnz))
+        return rp, ind[:nnz], ln[:nnz]
+
+    def mem_bytes(self) -> int:
+        b = C.c_int64()
+        _check(self._lib.mi_context_mem(self._h, C.byref(b)))
+        return b.value
+
+    def trim(self):
+        _check(self._lib.mi_context_trim(self._h))
+
+    # -- projectors -------------------------------------------------------
+    def forward(self, g: Geometry, img, dtype=np.float32):
+        img = _arr(img, dtype, (g.nx, g.ny))
+        out = np.empty((g.n_views, g.n_bins), dtype)
+        fn = self._lib.mi_forward if dtype == np.float32 else self._lib.mi_forward_f64
+        _check(fn(self._h, C.byref(g), _ptr(img), _ptr(out)))
+        return out
+
+    def backproject(self, g: Geometry, sino, dtype=np.float32):
+        sino = _arr(sino, dtype, (g.n_views, g.n_bins))
+        out = np.empty((g.nx, g.ny), dtype)
+        fn = self._lib.mi_backproject if dtype == np.float32 else self._lib.mi_backproject_f64
+        _check(fn(self._h, C.byref(g), _ptr(sino), _ptr(out)))
+        return out
+
+    # -- GoldenC building blocks -----------------------------------------
+    def image_step(self, g: Geometry, sino, f, v, alpha, epsilon, iter_image):
+        sino = _arr(sino, np.float64, (g.n_views, g.n_bins))
+        f = _arr(f, np.float64, (g.nx, g.ny))
+        v = _arr(v, np.float64, (g.nx, g.ny))
+        out = np.empty_like(f)
+        _check(self._lib.mi_image_step_f64(self._h, C.byref(g), _ptr(out), _ptr(sino), _ptr(f), _ptr(v),
+                                           alpha, epsilon, iter_image))
+        return out
+
+    def edge_step(self, f, v, alpha, beta, epsilon, iter_edge):
+        f = _arr(f, np.float64)
+        v = _arr(v, np.float64, f.shape)
+        out = np.empty_like(v)
+        _check(self._lib.mi_edge_step_f64(self._h, f.shape[0], f.shape[1], _ptr(out), _ptr(f), _ptr(v),
+                                          alpha, beta, epsilon, iter_edge))
+        return out
+
+    def medfilt2(self, f):
+        f = _arr(f, np.float64).copy()
+        _check(self._lib.mi_medfilt2_f64(self._h, f.shape[0], f.shape[1], _ptr(f)))
+        return f
+
+    # -- drivers ----------------------------------------------------------
+    def mumford_shah(self, g: Geometry, sino, p: AtParams):
+        sino = _arr(sino, np.float64, (g.n_views, g.n_bins))
+        f = np.empty((g.nx, g.ny), np.float64)
+        v = np.empty((g.nx, g.ny), np.float64)
+        rep = Report()
+        _check(self._lib.mi_mumford_shah(self._h, C.byref(g), C.byref(p), _ptr(sino), _ptr(f), _ptr(v),
+                                         C.byref(rep)))
+        return f, v, rep
+
+    def at_reconstruct(self, g: Geometry, sino, p: AtParams, u0=None, v0=None):
+        dt = np.float64 if p.precision == MI_FP64 else np.float32
+        sino = _arr(sino, dt, (g.n_views, g.n_bins))
+        u = np.zeros((g.nx, g.ny), dt) if u0 is None else _arr(u0, dt, (g.nx, g.ny)).copy()
+        v = np.ones((g.nx, g.ny), dt) if v0 is None else _arr(v0, dt, (g.nx, g.ny)).copy()
+        rep = Report()
+        fn = self._lib.mi_at_reconstruct_f64 if dt == np.float64 else self._lib.mi_at_reconstruct
+        _check(fn(self._h, C.byref(g), C.byref(p), _ptr(sino), _ptr(u), _ptr(v), C.byref(rep)))
+        return u, v, rep
+
+    # -- instrumentation --------------------------------------------------
+    def stats_enable(self, on=True):
+        _check(self._lib.mi_stats_enable(self._h, int(bool(on))))
+
+    def stats(self, kernel: int) -> KernelStats:
+        s = KernelStats()
+        _check(self._lib.mi_stats_get(self._h, kernel, C.byref(s)))
+        return s
+
+    def stats_reset(self):
+        _check(self._lib.mi_stats_reset(self._h))

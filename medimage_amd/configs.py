@@ -1,0 +1,88 @@
+"""The reconstruction workloads, with every size the descriptions leave open
+fixed here.
+
+* C1 plain: 256 x 256, 1.0 mm pixels, 360 views over 360 degrees, 512 bins of
+  1.0 mm, SOD 541 mm, SDD 949 mm; 10 ellipses; I0 = 1e5; 50 AT iterations;
+  alpha 1e-3, beta 1e-4, eps 1e-2.
+* C2 precision: 512 x 512, 0.7 mm pixels, 720 views, 736 bins of 1.0 mm,
+  same source/detector distances; I0 = 2e4; 12 subsets, 100 AT iterations.
+* C3 batched: 64 slices, sizes from {256, 384, 512}, views from {360, 720,
+  1152}, bins from {512, 736}; field of view 358.4 mm (pixel = 358.4 / N),
+  detector width 736 mm (bin = 736 / bins); I0 uniform in [1e4, 1e5];
+  8 subsets, 50 AT iterations.
+* C5 serving: sizes {256, 384, 512, 768, 1024}, views {360, 720, 1152, 2304},
+  bins = 1.5 N, same field of view and detector width; I0 log-uniform in
+  [1e4, 1e5]; stop at relative data-fidelity change < 1e-4 or 200 iterations.
+
+One AT iteration is one pass over all subsets (per subset: forward
+projection, residual, gather back projection, image update with an exact
+line search), followed by one edge (v) step.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import at_params, fan_geometry, phantom
+
+SOD, SDD = 541.0, 949.0
+FOV_MM = 358.4
+DET_MM = 736.0
+ALPHA, BETA, EPS = 1e-3, 1e-4, 1e-2
+
+
+def c1_geometry():
+    return fan_geometry(256, 1.0, 360, 512, 1.0, SOD, SDD)
+
+
+def c2_geometry():
+    return fan_geometry(512, 0.7, 720, 736, 1.0, SOD, SDD)
+
+
+def c1_params(**kw):
+    return at_params(ALPHA, BETA, EPS, n_subsets=kw.pop("n_subsets", 1), max_iters=kw.pop("max_iters", 50), **kw)
+
+
+def c2_params(**kw):
+    return at_params(ALPHA, BETA, EPS, n_subsets=kw.pop("n_subsets", 12), max_iters=kw.pop("max_iters", 100), **kw)
+
+
+def mixed_geometry(n, views, bins):
+    """C3/C5 geometry: FOV 358.4 mm and a 736 mm wide detector."""
+    return fan_geometry(n, FOV_MM / n, views, bins, DET_MM / bins, SOD, SDD)
+
+
+def c3_jobs(n_jobs=64, seed=3):
+    g = phantom.rng(seed, 77)
+    out = []
+    for j in range(n_jobs):
+        n = int(g.choice([256, 384, 512]))
+        v = int(g.choice([360, 720, 1152]))
+        b = int(g.choice([512, 736]))
+        i0 = float(g.uniform(1e4, 1e5))
+        out.append(dict(job=j, n=n, views=v, bins=b, i0=i0, seed=seed * 1000 + j))
+    return out
+
+
+def make_case(ctx, geom, i0, seed, n_ellipses=10, mu_max=0.02):
+    """Phantom and noisy sinogram (float64) for a geometry."""
+    fov = geom.nx * geom.pixel_mm
+    ell = phantom.random_ellipses(n_ellipses, fov, seed, mu_max=mu_max)
+    img = phantom.rasterize(ell, geom.nx, geom.pixel_mm)
+    p = ctx.forward(geom, img, dtype=np.float64)
+    sino = phantom.noisy_sinogram(p, i0, seed)
+    return img, sino
+
+
+def rays(geom):
+    return geom.n_views * geom.n_bins
+
+
+def subsets_for(views, want):
+    s = min(want, views)
+    return max(1, s)
+
+
+__all__ = ["c1_geometry", "c2_geometry", "c1_params", "c2_params", "mixed_geometry", "c3_jobs", "make_case",
+           "rays", "math"]

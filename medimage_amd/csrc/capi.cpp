@@ -1,0 +1,462 @@
+// C ABI of libmedimage_amd.so (include/medimage_amd.h): argument checks,
+// error translation, context and geometry cache, instrumentation.
+#include "solver.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <string>
+
+namespace mi {
+
+namespace {
+thread_local std::string g_last_error;
+
+template <typename F>
+int guarded(F &&f) {
+    try {
+        f();
+        return MI_OK;
+    } catch (const Error &e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_last_error = "host allocation failed";
+        return MI_E_NOMEM;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return MI_E_STATE;
+    }
+}
+}  // namespace
+
+void fail(int code, const std::string &msg) { throw Error(code, msg); }
+
+void hip_check(hipError_t e, const char *what, const char *file, int line) {
+    if (e == hipSuccess) return;
+    std::string m = std::string(what) + ": " + hipGetErrorString(e) + " (" + file + ":" + std::to_string(line) + ")";
+    throw Error(e == hipErrorOutOfMemory ? MI_E_NOMEM : MI_E_DEVICE, m);
+}
+
+template <typename T>
+void DevBuf<T>::alloc(size_t count) {
+    reset();
+    if (count == 0) return;
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, count * sizeof(T));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        fail(e == hipErrorOutOfMemory ? MI_E_NOMEM : MI_E_DEVICE,
+             "device allocation of " + std::to_string(count * sizeof(T)) + " bytes failed");
+    }
+    p = (T *)q;
+    n = count;
+}
+
+template <typename T>
+void DevBuf<T>::reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+}
+
+template struct DevBuf<float>;
+template struct DevBuf<double>;
+template struct DevBuf<int>;
+template struct DevBuf<int64_t>;
+template struct DevBuf<uint32_t>;
+template struct DevBuf<unsigned char>;
+template struct DevBuf<Scalars>;
+
+void validate_geometry(const mi_geometry &g) {
+    MI_REQUIRE(g.nx > 0 && g.ny > 0 && g.n_views > 0 && g.n_bins > 0, MI_E_ARG, "geometry sizes must be positive");
+    MI_REQUIRE((int64_t)g.nx * g.ny < (int64_t)1 << 31, MI_E_LIMIT, "image too large");
+    switch (g.beam) {
+        case MI_BEAM_FAN_FLAT:
+            MI_REQUIRE(g.pixel_mm > 0 && g.bin_mm > 0 && g.sod_mm > 0 && g.sdd_mm > g.sod_mm, MI_E_ARG,
+                       "fan geometry needs pixel_mm, bin_mm > 0 and sdd > sod > 0");
+            MI_REQUIRE(std::isfinite(g.span_deg) && g.span_deg > 0, MI_E_ARG, "span_deg must be positive");
+            break;
+        case MI_BEAM_PARALLEL:
+            MI_REQUIRE(g.nx == g.ny, MI_E_ARG, "parallel beam needs a square image");
+            MI_REQUIRE(g.flavour == 0 || g.flavour == 1, MI_E_ARG, "flavour must be 0 or 1");
+            break;
+        case MI_BEAM_GOLDENC:
+            MI_REQUIRE(g.nx == 512 && g.ny == 512 && g.n_views == 180 && g.n_bins == 767, MI_E_ARG,
+                       "GoldenC geometry is 512 x 512, 180 views, 767 bins");
+            break;
+        default:
+            fail(MI_E_ARG, "unknown beam kind");
+    }
+}
+
+GeomKey make_key(const mi_geometry &g, int S) {
+    GeomKey k;
+    k.i = {g.beam, g.nx, g.ny, g.n_views, g.n_bins, g.beam == MI_BEAM_PARALLEL ? g.flavour : 0, S};
+    if (g.beam == MI_BEAM_FAN_FLAT) k.d = {g.pixel_mm, g.span_deg, g.bin_mm, g.sod_mm, g.sdd_mm, g.det_offset};
+    else k.d = {0, 0, 0, 0, 0, 0};
+    return k;
+}
+
+Matrix &get_matrix(mi_context *ctx, const mi_geometry &g, int S, bool want64) {
+    const GeomKey key = make_key(g, S);
+    auto it = ctx->matrices.find(key);
+    if (it != ctx->matrices.end() && (!want64 || it->second->has64)) return *it->second;
+    if (it != ctx->matrices.end()) ctx->matrices.erase(it);
+    auto m = build_matrix(g, S, want64, ctx->stream);
+    Matrix &ref = *m;
+    ctx->matrices[key] = std::move(m);
+    return ref;
+}
+
+LaunchTimer::LaunchTimer(mi_context *c, int k, double by) : ctx(c), kernel(k), bytes(by) {
+    if (!ctx->stats.on) return;
+    auto &pool = ctx->stats.pool;
+    auto take = [&]() {
+        hipEvent_t e;
+        if (!pool.empty()) { e = pool.back(); pool.pop_back(); }
+        else MI_HIP(hipEventCreate(&e));
+        return e;
+    };
+    a = take();
+    b = take();
+    MI_HIP(hipEventRecord(a, ctx->stream));
+}
+
+LaunchTimer::~LaunchTimer() {
+    if (!a) return;
+    if (hipEventRecord(b, ctx->stream) != hipSuccess) return;
+    ctx->stats.pending.push_back({kernel, a, b, bytes});
+}
+
+void stats_collect(mi_context *ctx) {
+    auto &S = ctx->stats;
+    if (S.pending.empty()) return;
+    MI_HIP(hipStreamSynchronize(ctx->stream));
+    for (auto &p : S.pending) {
+        float ms = 0.f;
+        MI_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+        auto &k = S.k[p.kernel];
+        k.launches += 1;
+        k.total_ms += ms;
+        k.bytes += p.bytes;
+        S.pool.push_back(p.a);
+        S.pool.push_back(p.b);
+    }
+    S.pending.clear();
+}
+
+}  // namespace mi
+
+using namespace mi;
+
+namespace {
+
+mi_context *check_ctx(mi_context *ctx) {
+    MI_REQUIRE(ctx != nullptr, MI_E_ARG, "null context");
+    MI_HIP(hipSetDevice(ctx->device));
+    return ctx;
+}
+
+template <typename T>
+void forward_impl(mi_context *ctx, const mi_geometry *g, const T *img, T *sino) {
+    MI_REQUIRE(g && img && sino, MI_E_ARG, "null argument");
+    const bool f64 = sizeof(T) == 8;
+    Matrix &m = get_matrix(ctx, *g, 1, f64);
+    const hipStream_t st = ctx->stream;
+    DevBuf<T> x(m.npix), rows(m.nrows), user(m.nrows);
+    DevBuf<double> part(2 * kMaxGrid);
+    DevBuf<unsigned> tick(1);
+    MI_HIP(hipMemsetAsync(tick.p, 0, sizeof(unsigned), st));
+    MI_HIP(hipMemcpyAsync(x.p, img, m.npix * sizeof(T), hipMemcpyHostToDevice, st));
+    {
+        LaunchTimer lt(ctx, MI_K_FP, fp_bytes(m, m.nrows, sizeof(T), FP_WRITE));
+        launch_fp<T>(st, RedBuf{part.p, tick.p}, FP_WRITE, csr_view(m, f64), 0, m.nrows, x.p, nullptr, nullptr,
+                     rows.p, Slot{});
+    }
+    launch_scatter_rows<T>(st, m.nrows, m.perm.p, rows.p, user.p);
+    MI_HIP(hipMemcpyAsync(sino, user.p, m.nrows * sizeof(T), hipMemcpyDeviceToHost, st));
+    MI_HIP(hipStreamSynchronize(st));
+    stats_collect(ctx);
+}
+
+template <typename T>
+void backproject_impl(mi_context *ctx, const mi_geometry *g, const T *sino, T *img) {
+    MI_REQUIRE(g && img && sino, MI_E_ARG, "null argument");
+    const bool f64 = sizeof(T) == 8;
+    Matrix &m = get_matrix(ctx, *g, 1, f64);
+    const hipStream_t st = ctx->stream;
+    DevBuf<T> x(m.npix), rows(m.nrows);
+    DevBuf<double> part(2 * kMaxGrid);
+    DevBuf<unsigned> tick(1);
+    MI_HIP(hipMemsetAsync(tick.p, 0, sizeof(unsigned), st));
+    load_sino<T>(st, m, sino, rows.p);
+    {
+        LaunchTimer lt(ctx, MI_K_BP, bp_bytes(m, m.nrows, sizeof(T), false));
+        launch_bp_grad<T>(st, RedBuf{part.p, tick.p}, csc_view(m, 0, f64), g->nx, g->ny, rows.p, 1.0, nullptr,
+                          nullptr, 0.0, 0.0, x.p, Slot{}, bp_lanes(m));
+    }
+    MI_HIP(hipMemcpyAsync(img, x.p, m.npix * sizeof(T), hipMemcpyDeviceToHost, st));
+    MI_HIP(hipStreamSynchronize(st));
+    stats_collect(ctx);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *mi_last_error(void) { return g_last_error.c_str(); }
+
+int mi_context_create(int device, mi_context **out) {
+    return guarded([&] {
+        MI_REQUIRE(out != nullptr, MI_E_ARG, "null output pointer");
+        *out = nullptr;
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+            (void)hipGetLastError();
+            fail(MI_E_DEVICE, "no HIP device available");
+        }
+        MI_REQUIRE(device >= 0 && device < n, MI_E_ARG, "device index out of range");
+        MI_HIP(hipSetDevice(device));
+        hipDeviceProp_t prop;
+        MI_HIP(hipGetDeviceProperties(&prop, device));
+        MI_REQUIRE(std::string(prop.gcnArchName).rfind("gfx950", 0) == 0, MI_E_DEVICE,
+                   std::string("library is built for gfx950, device is ") + prop.gcnArchName);
+        auto c = new mi_context();
+        c->device = device;
+        hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete c;
+            MI_HIP(e);
+        }
+        *out = c;
+    });
+}
+
+int mi_context_destroy(mi_context *ctx) {
+    return guarded([&] {
+        if (!ctx) return;
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        ctx->matrices.clear();
+        for (auto &p : ctx->stats.pending) {
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
+        }
+        for (auto e : ctx->stats.pool) (void)hipEventDestroy(e);
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+    });
+}
+
+int mi_context_mem(mi_context *ctx, int64_t *bytes) {
+    return guarded([&] {
+        check_ctx(ctx);
+        MI_REQUIRE(bytes, MI_E_ARG, "null output");
+        int64_t b = 0;
+        for (auto &kv : ctx->matrices) b += (int64_t)kv.second->bytes();
+        *bytes = b;
+    });
+}
+
+int mi_context_trim(mi_context *ctx) {
+    return guarded([&] {
+        check_ctx(ctx);
+        MI_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->matrices.clear();
+    });
+}
+
+int mi_geometry_prepare(mi_context *ctx, const mi_geometry *g, int n_subsets, int64_t *nnz) {
+    return guarded([&] {
+        check_ctx(ctx);
+        MI_REQUIRE(g, MI_E_ARG, "null geometry");
+        Matrix &m = get_matrix(ctx, *g, n_subsets, false);
+        if (nnz) *nnz = m.nnz;
+    });
+}
+
+int mi_trace_ray(mi_context *ctx, const mi_geometry *g, int view, int bin, int32_t *ind, double *len, int cap,
+                 int *numb) {
+    return guarded([&] {
+        check_ctx(ctx);
+        MI_REQUIRE(g && numb && cap >= 0 && (cap == 0 || (ind && len)), MI_E_ARG, "bad arguments");
+        MI_REQUIRE(view >= 0 && view < g->n_views && bin >= 0 && bin < g->n_bins, MI_E_ARG, "ray out of range");
+        Matrix &m = get_matrix(ctx, *g, 1, true);
+        const int64_t r = (int64_t)view * g->n_bins + bin;  // one subset: rows in view order
+        int64_t rp[2];
+        MI_HIP(hipMemcpy(rp, m.rowptr.p + r, 2 * sizeof(int64_t), hipMemcpyDeviceToHost));
+        const int n = (int)(rp[1] - rp[0]);
+        const int take = n < cap ? n : cap;
+        if (take > 0) {
+            std::vector<uint32_t> ix(take);
+            MI_HIP(hipMemcpy(ix.data(), m.idx.p + rp[0], take * sizeof(uint32_t), hipMemcpyDeviceToHost));
+            MI_HIP(hipMemcpy(len, m.len64.p + rp[0], take * sizeof(double), hipMemcpyDeviceToHost));
+            for (int i = 0; i < take; ++i) ind[i] = (int32_t)ix[i];
+        }
+        *numb = n;
+    });
+}
+
+int mi_matrix_export(mi_context *ctx, const mi_geometry *g, int64_t *rowptr, int32_t *ind, double *len,
+                     int64_t nnz) {
+    return guarded([&] {
+        check_ctx(ctx);
+        MI_REQUIRE(g && rowptr && (nnz == 0 || (ind && len)), MI_E_ARG, "null argument");
+        Matrix &m = get_matrix(ctx, *g, 1, true);
+        MI_REQUIRE(nnz == m.nnz, MI_E_ARG, "nnz does not match the geometry's matrix");
+        MI_HIP(hipMemcpy(rowptr, m.rowptr.p, (m.nrows + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
+        if (nnz > 0) {
+            static_assert(sizeof(uint32_t) == sizeof(int32_t), "index width");
+            MI_HIP(hipMemcpy(ind, m.idx.p, nnz * sizeof(uint32_t), hipMemcpyDeviceToHost));
+            MI_HIP(hipMemcpy(len, m.len64.p, nnz * sizeof(double), hipMemcpyDeviceToHost));
+        }
+    });
+}
+
+int mi_forward(mi_context *ctx, const mi_geometry *g, const float *img, float *sino) {
+    return guarded([&] { forward_impl<float>(check_ctx(ctx), g, img, sino); });
+}
+int mi_backproject(mi_context *ctx, const mi_geometry *g, const float *sino, float *img) {
+    return guarded([&] { backproject_impl<float>(check_ctx(ctx), g, sino, img); });
+}
+int mi_forward_f64(mi_context *ctx, const mi_geometry *g, const double *img, double *sino) {
+    return guarded([&] { forward_impl<double>(check_ctx(ctx), g, img, sino); });
+}
+int mi_backproject_f64(mi_context *ctx, const mi_geometry *g, const double *sino, double *img) {
+    return guarded([&] { backproject_impl<double>(check_ctx(ctx), g, sino, img); });
+}
+
+int mi_image_step_f64(mi_context *ctx, const mi_geometry *g, double *f_out, const double *sino, const double *f,
+                      const double *v, double alpha, double epsilon, int iter_image) {
+    return guarded([&] {
+        check_ctx(ctx);
+        MI_REQUIRE(g && f_out && sino && f && v && iter_image >= 1, MI_E_ARG, "bad arguments");
+        Matrix &m = get_matrix(ctx, *g, 1, true);
+        JobState<double> js(m);
+        const hipStream_t st = ctx->stream;
+        load_sino<double>(st, m, sino, js.g.p);
+        MI_HIP(hipMemcpyAsync(js.u.p, f, js.npix * sizeof(double), hipMemcpyHostToDevice, st));
+        MI_HIP(hipMemcpyAsync(js.v.p, v, js.npix * sizeof(double), hipMemcpyHostToDevice, st));
+        gc_image_step<double>(ctx, m, js, alpha, epsilon, iter_image, nullptr);
+        MI_HIP(hipMemcpyAsync(f_out, js.u.p, js.npix * sizeof(double), hipMemcpyDeviceToHost, st));
+        MI_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int mi_edge_step_f64(mi_context *ctx, int nx, int ny, double *v_out, const double *f, const double *v,
+                     double alpha, double beta, double epsilon, int iter_edge) {
+    return guarded([&] {
+        check_ctx(ctx);
+        MI_REQUIRE(nx > 0 && ny > 0 && v_out && f && v && iter_edge >= 1, MI_E_ARG, "bad arguments");
+        // An edge step needs no projector; a 1-view dummy matrix is not built.
+        Matrix dummy;
+        dummy.g.nx = nx;
+        dummy.g.ny = ny;
+        dummy.npix = nx * ny;
+        dummy.nrows = 1;
+        JobState<double> js(dummy);
+        const hipStream_t st = ctx->stream;
+        MI_HIP(hipMemcpyAsync(js.u.p, f, js.npix * sizeof(double), hipMemcpyHostToDevice, st));
+        MI_HIP(hipMemcpyAsync(js.v.p, v, js.npix * sizeof(double), hipMemcpyHostToDevice, st));
+        gc_edge_step<double>(ctx, js, alpha, beta, epsilon, iter_edge);
+        MI_HIP(hipMemcpyAsync(v_out, js.v.p, js.npix * sizeof(double), hipMemcpyDeviceToHost, st));
+        MI_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int mi_medfilt2_f64(mi_context *ctx, int nx, int ny, double *f) {
+    return guarded([&] {
+        check_ctx(ctx);
+        MI_REQUIRE(nx > 0 && ny > 0 && f, MI_E_ARG, "bad arguments");
+        const int64_t n = (int64_t)nx * ny;
+        DevBuf<double> a(n), b(n);
+        const hipStream_t st = ctx->stream;
+        MI_HIP(hipMemcpyAsync(a.p, f, n * sizeof(double), hipMemcpyHostToDevice, st));
+        launch_median<double>(st, nx, ny, a.p, b.p);
+        MI_HIP(hipMemcpyAsync(f, b.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
+        MI_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int mi_mumford_shah(mi_context *ctx, const mi_geometry *g, const mi_at_params *p, const double *sino, double *f,
+                    double *v, mi_report *rep) {
+    return guarded([&] {
+        check_ctx(ctx);
+        MI_REQUIRE(g && p && sino && f && v, MI_E_ARG, "null argument");
+        MI_REQUIRE(p->iter_alt >= 0 && p->iter_image >= 1 && p->iter_edge >= 1, MI_E_ARG, "bad iteration counts");
+        Matrix &m = get_matrix(ctx, *g, 1, true);
+        JobState<double> js(m);
+        const hipStream_t st = ctx->stream;
+        load_sino<double>(st, m, sino, js.g.p);
+        fill<double>(st, js.u.p, js.npix, 0.0);
+        fill<double>(st, js.v.p, js.npix, 1.0);
+        hipEvent_t e0, e1;
+        MI_HIP(hipEventCreate(&e0));
+        MI_HIP(hipEventCreate(&e1));
+        MI_HIP(hipEventRecord(e0, st));
+        for (int i = 1; i <= p->iter_alt; ++i) {
+            gc_image_step<double>(ctx, m, js, p->alpha, p->epsilon, p->iter_image, nullptr);
+            gc_edge_step<double>(ctx, js, p->alpha, p->beta, p->epsilon, p->iter_edge);
+        }
+        MI_HIP(hipEventRecord(e1, st));
+        MI_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        MI_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        MI_HIP(hipMemcpyAsync(f, js.u.p, js.npix * sizeof(double), hipMemcpyDeviceToHost, st));
+        MI_HIP(hipMemcpyAsync(v, js.v.p, js.npix * sizeof(double), hipMemcpyDeviceToHost, st));
+        MI_HIP(hipStreamSynchronize(st));
+        if (rep) {
+            std::memset(rep, 0, sizeof(*rep));
+            rep->iterations = p->iter_alt;
+            rep->elapsed_ms = ms;
+            rep->nnz = m.nnz;
+        }
+    });
+}
+
+int mi_at_reconstruct(mi_context *ctx, const mi_geometry *g, const mi_at_params *p, const float *sino, float *u,
+                      float *v, mi_report *rep) {
+    return guarded([&] {
+        check_ctx(ctx);
+        MI_REQUIRE(g && p && sino && u && v, MI_E_ARG, "null argument");
+        os_solve<float>(ctx, *g, *p, sino, u, v, rep);
+    });
+}
+
+int mi_at_reconstruct_f64(mi_context *ctx, const mi_geometry *g, const mi_at_params *p, const double *sino,
+                          double *u, double *v, mi_report *rep) {
+    return guarded([&] {
+        check_ctx(ctx);
+        MI_REQUIRE(g && p && sino && u && v, MI_E_ARG, "null argument");
+        os_solve<double>(ctx, *g, *p, sino, u, v, rep);
+    });
+}
+
+int mi_stats_enable(mi_context *ctx, int on) {
+    return guarded([&] {
+        check_ctx(ctx);
+        ctx->stats.on = on != 0;
+    });
+}
+
+int mi_stats_get(mi_context *ctx, int kernel, mi_kernel_stats *out) {
+    return guarded([&] {
+        check_ctx(ctx);
+        MI_REQUIRE(kernel >= 0 && kernel < MI_K_COUNT && out, MI_E_ARG, "bad kernel id");
+        stats_collect(ctx);
+        *out = ctx->stats.k[kernel];
+    });
+}
+
+int mi_stats_reset(mi_context *ctx) {
+    return guarded([&] {
+        check_ctx(ctx);
+        stats_collect(ctx);
+        for (auto &k : ctx->stats.k) k = mi_kernel_stats{};
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,114 @@
+// Host runtime of medimage_amd: device buffers, system matrices, solver
+// entry points.  Everything here is C++ behind the C ABI of
+// include/medimage_amd.h.
+#pragma once
+
+#include "common.hpp"
+#include "../../include/medimage_amd.h"
+
+#include <array>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+namespace mi {
+
+// Owning device allocation.
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { alloc(count); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept { reset(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; return *this; }
+    ~DevBuf() { reset(); }
+    void alloc(size_t count);
+    void reset();
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+// Geometry + subset count identify one system matrix.
+struct GeomKey {
+    std::array<int32_t, 7> i;
+    std::array<double, 6> d;
+    bool operator<(const GeomKey &o) const { return i != o.i ? i < o.i : d < o.d; }
+};
+GeomKey make_key(const mi_geometry &g, int n_subsets);
+void validate_geometry(const mi_geometry &g);
+
+// System matrix of one geometry, rows (rays) grouped by view subset.
+//   Row order: subsets in order; inside a subset its views ascending; inside
+//   a view the bins ascending.  Row r is ray (vlist[r / nbins], r % nbins).
+//   CSR (ray-major) feeds the forward projector; one CSC per subset
+//   (pixel-major, entries ordered by row) feeds the gather back projector.
+struct Matrix {
+    mi_geometry g{};
+    int S = 1;
+    int npix = 0;
+    int64_t nrows = 0, nnz = 0;
+    std::vector<int> vlist;          // subset-ordered view list
+    std::vector<int64_t> sub_row0;   // S+1 row offsets
+    bool has64 = false;
+
+    DevBuf<int> d_vlist;
+    DevBuf<int64_t> rowptr;          // nrows + 1
+    DevBuf<uint32_t> idx;            // nnz, pixel index
+    DevBuf<float> len32;             // nnz
+    DevBuf<double> len64;            // nnz (FP64 runs only)
+    DevBuf<int64_t> colptr;          // S * npix + 1
+    DevBuf<uint32_t> crow;           // nnz, row index
+    DevBuf<float> clen32;
+    DevBuf<double> clen64;
+    // Row permutation between the caller's [view][bin] sinogram and the
+    // subset-ordered rows: perm[r] = view * nbins + bin of row r.
+    DevBuf<int64_t> perm;
+
+    size_t bytes() const;
+};
+
+// Builds the matrix for (g, S) on the stream; want64 also keeps double
+// lengths.  Host-side GoldenC walk for MI_BEAM_GOLDENC, device tracers
+// otherwise.
+std::unique_ptr<Matrix> build_matrix(const mi_geometry &g, int S, bool want64, hipStream_t st);
+// Subset order: bit-reversed subset indices (values >= S dropped).
+std::vector<int> subset_order(int S);
+
+// Device trace of one ray (for mi_trace_ray).
+int trace_one(const mi_geometry &g, int view, int bin, int32_t *ind, double *len, int cap, hipStream_t st);
+
+// Timing / byte counters of the instrumented kernels.
+struct Stats {
+    bool on = false;
+    std::array<mi_kernel_stats, MI_K_COUNT> k{};
+    struct Pending { int kernel; hipEvent_t a, b; double bytes; };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+};
+
+}  // namespace mi
+
+struct mi_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::map<mi::GeomKey, std::unique_ptr<mi::Matrix>> matrices;
+    mi::Stats stats;
+};
+
+namespace mi {
+// Matrix for (g, S) from the context cache, building it if needed.
+Matrix &get_matrix(mi_context *ctx, const mi_geometry &g, int S, bool want64);
+// Stats helpers: bracket a launch with events when enabled.
+struct LaunchTimer {
+    mi_context *ctx;
+    int kernel;
+    double bytes;
+    hipEvent_t a = nullptr, b = nullptr;
+    LaunchTimer(mi_context *c, int k, double by);
+    ~LaunchTimer();
+};
+void stats_collect(mi_context *ctx);
+}  // namespace mi

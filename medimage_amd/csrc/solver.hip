@@ -1,0 +1,300 @@
+// Solver drivers: the GoldenC CG/SD alternation (origin/GoldenC/src/main.c)
+// and the ordered-subset AT iteration, both on device-resident state with
+// FP64 reductions feeding every step size.
+#include "solver.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+namespace mi {
+
+namespace {
+
+template <typename T> struct LenSel;
+template <> struct LenSel<float> {
+    static const void *csr(const Matrix &m) { return m.len32.p; }
+    static const void *csc(const Matrix &m) { return m.clen32.p; }
+};
+template <> struct LenSel<double> {
+    static const void *csr(const Matrix &m) { return m.len64.p; }
+    static const void *csc(const Matrix &m) { return m.clen64.p; }
+};
+
+__global__ void k_fill_f32(float *p, int64_t n, float v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_f64(double *p, int64_t n, double v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+}  // namespace
+
+template <typename T>
+void fill(hipStream_t st, T *p, int64_t n, T v) {
+    const unsigned grid = (unsigned)((n + kBlock - 1) / kBlock);
+    if (grid == 0) return;
+    if constexpr (sizeof(T) == 4) k_fill_f32<<<grid, kBlock, 0, st>>>((float *)p, n, (float)v);
+    else k_fill_f64<<<grid, kBlock, 0, st>>>((double *)p, n, (double)v);
+    MI_HIP(hipGetLastError());
+}
+template void fill<float>(hipStream_t, float *, int64_t, float);
+template void fill<double>(hipStream_t, double *, int64_t, double);
+
+CsrView csr_view(const Matrix &m, bool f64) {
+    return CsrView{m.rowptr.p, m.idx.p, f64 ? (const void *)m.len64.p : (const void *)m.len32.p};
+}
+CscView csc_view(const Matrix &m, int s, bool f64) {
+    return CscView{m.colptr.p + (int64_t)s * m.npix, m.crow.p, f64 ? (const void *)m.clen64.p : (const void *)m.clen32.p};
+}
+
+// Lanes per pixel column of the gather back projector: about a quarter of the
+// subset's mean column length, within [4, 64].
+int bp_lanes(const Matrix &m) {
+    const double mean = (double)m.nnz / ((double)m.S * (double)m.npix);
+    int l = 4;
+    while (l < 64 && l * 4 < mean) l *= 2;
+    return l;
+}
+
+double fp_bytes(const Matrix &m, int64_t nrows, size_t tsz, int mode) {
+    // Mean entries per row of the geometry times the rows of the launch.
+    const double nnz = (double)m.nnz * (double)nrows / (double)m.nrows;
+    double b = nnz * (4.0 + tsz) + (double)(nrows + 1) * 8.0 + (double)m.npix * tsz;
+    if (mode == FP_RESID || mode == FP_ERR) b += 2.0 * nrows * tsz;
+    else if (mode == FP_WRITE || mode == FP_DOT) b += (double)nrows * tsz;
+    return b;
+}
+
+double bp_bytes(const Matrix &m, int64_t nrows_sub, size_t tsz, bool reg) {
+    const double nnz = (double)m.nnz * (double)nrows_sub / (double)m.nrows;
+    double b = nnz * (4.0 + tsz) + (double)(m.npix + 1) * 8.0 + (double)nrows_sub * tsz + (double)m.npix * tsz;
+    if (reg) b += 2.0 * m.npix * tsz;
+    return b;
+}
+
+// ---------------------------------------------------------------------------
+
+template <typename T>
+JobState<T>::JobState(const Matrix &m) {
+    nx = m.g.nx;
+    ny = m.g.ny;
+    npix = (int64_t)nx * ny;
+    nrows = m.nrows;
+    g.alloc(nrows);
+    r.alloc(nrows);
+    q.alloc(nrows);
+    af.alloc(nrows);
+    u.alloc(npix);
+    v.alloc(npix);
+    d.alloc(npix);
+    p.alloc(npix);
+    t.alloc(npix);
+    w.alloc(npix);
+    sc.alloc(1);
+    red_partial.alloc(2 * kMaxGrid);
+    red_ticket.alloc(1);
+    MI_HIP(hipMemset(red_ticket.p, 0, sizeof(unsigned)));
+    MI_HIP(hipMemset(sc.p, 0, sizeof(Scalars)));
+}
+
+template <typename T>
+RedBuf JobState<T>::rb() const {
+    return RedBuf{red_partial.p, red_ticket.p};
+}
+
+template <typename T>
+Scalars JobState<T>::read_scalars(hipStream_t st) const {
+    Scalars h;
+    MI_HIP(hipMemcpyAsync(&h, sc.p, sizeof(Scalars), hipMemcpyDeviceToHost, st));
+    MI_HIP(hipStreamSynchronize(st));
+    return h;
+}
+
+// One ordered-subset pass (everything on the stream, no host round trip).
+template <typename T>
+void os_pass(mi_context *ctx, const Matrix &m, JobState<T> &js, const mi_at_params &p, int pass_index) {
+    const hipStream_t st = ctx->stream;
+    const bool f64 = sizeof(T) == 8;
+    const RedBuf rb = js.rb();
+    Scalars *sc = js.sc.p;
+    const CsrView A = csr_view(m, f64);
+    const int lanes = bp_lanes(m);
+    const double S = (double)m.S;
+    launch_pass_begin(st, sc);
+    for (int s = 0; s < m.S; ++s) {
+        const int64_t row0 = m.sub_row0[s], nr = m.sub_row0[s + 1] - row0;
+        {
+            LaunchTimer lt(ctx, MI_K_FP, fp_bytes(m, nr, sizeof(T), FP_RESID));
+            launch_fp<T>(st, rb, FP_RESID, A, row0, nr, js.u.p, js.g.p, nullptr, js.r.p, Slot{&sc->df_pass, 1});
+        }
+        {
+            LaunchTimer lt(ctx, MI_K_BP, bp_bytes(m, nr, sizeof(T), true));
+            launch_bp_grad<T>(st, rb, csc_view(m, s, f64), js.nx, js.ny, js.r.p, S, js.u.p, js.v.p, p.alpha,
+                              p.epsilon, js.d.p, Slot{&sc->dd, 0}, lanes);
+        }
+        {
+            LaunchTimer lt(ctx, MI_K_FPSQ, fp_bytes(m, nr, sizeof(T), FP_SQ));
+            launch_fp<T>(st, rb, FP_SQ, A, row0, nr, js.d.p, nullptr, nullptr, nullptr, Slot{&sc->ad2, 0});
+        }
+        {
+            LaunchTimer lt(ctx, MI_K_STEP, (double)js.npix * sizeof(T) * 5);
+            launch_nv<T>(st, rb, js.nx, js.ny, js.d.p, js.v.p, p.epsilon, Slot{&sc->nv, 0}, sc, S, p.alpha);
+            launch_axpy<T>(st, rb, js.npix, js.u.p, js.d.p, &sc->c, 0.0, p.lower, p.upper, nullptr, Slot{});
+        }
+    }
+    if (p.median_every > 0 && pass_index % p.median_every == 0) {
+        launch_median<T>(st, js.nx, js.ny, js.u.p, js.t.p);
+        std::swap(js.u, js.t);
+    }
+    {
+        LaunchTimer lt(ctx, MI_K_EDGE, (double)js.npix * sizeof(T) * 7 * (p.iter_edge > 0 ? p.iter_edge : 0));
+        for (int k = 0; k < p.iter_edge; ++k)
+            launch_edge_iter<T>(st, rb, js.nx, js.ny, js.u.p, js.v.p, js.d.p, p.alpha, p.beta, p.epsilon, sc);
+    }
+    launch_pass_end(st, sc, p.rel_tol);
+}
+
+template <typename T>
+void os_solve(mi_context *ctx, const mi_geometry &g, const mi_at_params &p, const T *sino, T *u, T *v,
+              mi_report *rep) {
+    MI_REQUIRE(p.n_subsets >= 1 && p.max_iters >= 0 && p.iter_edge >= 0, MI_E_ARG, "bad AT parameters");
+    const bool f64 = sizeof(T) == 8;
+    Matrix &m = get_matrix(ctx, g, p.n_subsets, f64);
+    const hipStream_t st = ctx->stream;
+    JobState<T> js(m);
+    load_sino<T>(st, m, sino, js.g.p);
+    MI_HIP(hipMemcpyAsync(js.u.p, u, js.npix * sizeof(T), hipMemcpyHostToDevice, st));
+    MI_HIP(hipMemcpyAsync(js.v.p, v, js.npix * sizeof(T), hipMemcpyHostToDevice, st));
+    MI_HIP(hipStreamSynchronize(st));
+    hipEvent_t e0, e1;
+    MI_HIP(hipEventCreate(&e0));
+    MI_HIP(hipEventCreate(&e1));
+    MI_HIP(hipEventRecord(e0, st));
+    Scalars h{};
+    for (int it = 1; it <= p.max_iters; ++it) {
+        os_pass<T>(ctx, m, js, p, it);
+        h = js.read_scalars(st);
+        if (h.converged) break;
+    }
+    MI_HIP(hipEventRecord(e1, st));
+    MI_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    MI_HIP(hipEventElapsedTime(&ms, e0, e1));
+    MI_HIP(hipEventDestroy(e0));
+    MI_HIP(hipEventDestroy(e1));
+    stats_collect(ctx);
+    MI_HIP(hipMemcpyAsync(u, js.u.p, js.npix * sizeof(T), hipMemcpyDeviceToHost, st));
+    MI_HIP(hipMemcpyAsync(v, js.v.p, js.npix * sizeof(T), hipMemcpyDeviceToHost, st));
+    MI_HIP(hipStreamSynchronize(st));
+    if (rep) {
+        rep->iterations = h.iters;
+        rep->converged = h.converged;
+        rep->data_fidelity = h.df_prev;
+        rep->rel_change = h.rel;
+        rep->elapsed_ms = ms;
+        rep->nnz = m.nnz;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// GoldenC: minimize_in_IMAGE_variable (main.c:81-257), host-driven CG with
+// device reductions.  gen[] slots: 0 data term, 1 interior energy, 2 ||q||^2,
+// 3 nV(p), 4 <p,d>, 5 max |c1 p|, 6 <A d, q>.
+template <typename T>
+int gc_image_step(mi_context *ctx, const Matrix &m, JobState<T> &js, double alpha, double eps, int iter_image,
+                  int *why) {
+    const hipStream_t st = ctx->stream;
+    const bool f64 = sizeof(T) == 8;
+    const RedBuf rb = js.rb();
+    Scalars *sc = js.sc.p;
+    const CsrView A = csr_view(m, f64);
+    const CscView At = csc_view(m, 0, f64);
+    const int lanes = bp_lanes(m);
+    const double inf = std::numeric_limits<double>::infinity();
+    const int cadence = (int)(iter_image / 2 + 0.0001);
+    const size_t ib = js.npix * sizeof(T);
+    int reason = 0, ran = 0;
+
+    launch_fp<T>(st, rb, FP_RESID, A, 0, m.nrows, js.u.p, js.g.p, nullptr, js.r.p, Slot{&sc->gen[0], 0});
+    launch_bp_grad<T>(st, rb, At, js.nx, js.ny, js.r.p, 1.0, js.u.p, js.v.p, alpha, eps, js.d.p, Slot{}, lanes);
+    launch_energy_reg<T>(st, rb, js.nx, js.ny, js.u.p, js.v.p, alpha, eps, Slot{&sc->gen[1], 0});
+    MI_HIP(hipMemcpyAsync(js.p.p, js.d.p, ib, hipMemcpyDeviceToDevice, st));
+    Scalars h = js.read_scalars(st);
+    double at_old = h.gen[0] + h.gen[1];
+    for (int k = 1; k <= iter_image; ++k) {
+        ran = k;
+        launch_fp<T>(st, rb, FP_WRITE, A, 0, m.nrows, js.p.p, nullptr, nullptr, js.q.p, Slot{&sc->gen[2], 0});
+        launch_nv<T>(st, rb, js.nx, js.ny, js.p.p, js.v.p, eps, Slot{&sc->gen[3], 0}, nullptr, 0, 0);
+        launch_dot<T>(st, rb, js.npix, js.p.p, js.d.p, Slot{&sc->gen[4], 0});
+        h = js.read_scalars(st);
+        const double nap = h.gen[2];
+        const double c1 = h.gen[4] / (nap + alpha * h.gen[3]);
+        launch_axpy<T>(st, rb, js.npix, js.u.p, js.p.p, nullptr, c1, -inf, inf, js.t.p, Slot{&sc->gen[5], 0});
+        // The reference takes k % 0 when iter_image < 2 (undefined); no
+        // median is applied then.
+        if (cadence > 0 && k % cadence == 0) {
+            launch_median<T>(st, js.nx, js.ny, js.u.p, js.w.p);
+            MI_HIP(hipMemcpyAsync(js.u.p, js.w.p, ib, hipMemcpyDeviceToDevice, st));
+        }
+        h = js.read_scalars(st);
+        if (k == iter_image) { reason = 0; break; }
+        if (h.gen[5] < 1e-5) { reason = 1; break; }
+        launch_fp<T>(st, rb, FP_ERR, A, 0, m.nrows, js.u.p, js.g.p, nullptr, js.af.p, Slot{&sc->gen[0], 0});
+        launch_energy_reg<T>(st, rb, js.nx, js.ny, js.u.p, js.v.p, alpha, eps, Slot{&sc->gen[1], 0});
+        h = js.read_scalars(st);
+        const double at_new = h.gen[0] + h.gen[1];
+        if (1.1 * at_old < at_new && 1 < k) {
+            MI_HIP(hipMemcpyAsync(js.u.p, js.t.p, ib, hipMemcpyDeviceToDevice, st));
+            reason = 2;
+            break;
+        }
+        at_old = at_new;
+        launch_axpy<T>(st, rb, m.nrows, js.r.p, js.q.p, nullptr, -c1, -inf, inf, nullptr, Slot{});
+        launch_bp_grad<T>(st, rb, At, js.nx, js.ny, js.r.p, 1.0, js.u.p, js.v.p, alpha, eps, js.d.p, Slot{},
+                          lanes);
+        launch_fp<T>(st, rb, FP_DOT, A, 0, m.nrows, js.d.p, nullptr, js.q.p, nullptr, Slot{&sc->gen[6], 0});
+        h = js.read_scalars(st);
+        const double c2 = -h.gen[6] / nap;
+        launch_xpby<T>(st, js.npix, js.p.p, js.d.p, c2);
+    }
+    MI_HIP(hipStreamSynchronize(st));
+    if (why) *why = reason;
+    return ran;
+}
+
+// minimize_in_EDGE_variable (main.c:259-360) on js.v given js.u.
+template <typename T>
+void gc_edge_step(mi_context *ctx, JobState<T> &js, double alpha, double beta, double eps, int iter_edge) {
+    const hipStream_t st = ctx->stream;
+    launch_pass_begin(st, js.sc.p);
+    for (int k = 0; k < iter_edge; ++k)
+        launch_edge_iter<T>(st, js.rb(), js.nx, js.ny, js.u.p, js.v.p, js.d.p, alpha, beta, eps, js.sc.p);
+    MI_HIP(hipStreamSynchronize(st));
+}
+
+template <typename T>
+void load_sino(hipStream_t st, const Matrix &m, const T *host, T *rows) {
+    DevBuf<T> user(m.nrows);
+    MI_HIP(hipMemcpyAsync(user.p, host, m.nrows * sizeof(T), hipMemcpyHostToDevice, st));
+    launch_gather_rows<T>(st, m.nrows, m.perm.p, user.p, rows);
+    MI_HIP(hipStreamSynchronize(st));
+}
+
+template int gc_image_step<float>(mi_context *, const Matrix &, JobState<float> &, double, double, int, int *);
+template int gc_image_step<double>(mi_context *, const Matrix &, JobState<double> &, double, double, int, int *);
+template void gc_edge_step<float>(mi_context *, JobState<float> &, double, double, double, int);
+template void gc_edge_step<double>(mi_context *, JobState<double> &, double, double, double, int);
+template void load_sino<float>(hipStream_t, const Matrix &, const float *, float *);
+template void load_sino<double>(hipStream_t, const Matrix &, const double *, double *);
+
+template struct JobState<float>;
+template struct JobState<double>;
+template void os_solve<float>(mi_context *, const mi_geometry &, const mi_at_params &, const float *, float *,
+                              float *, mi_report *);
+template void os_solve<double>(mi_context *, const mi_geometry &, const mi_at_params &, const double *, double *,
+                               double *, mi_report *);
+
+}  // namespace mi

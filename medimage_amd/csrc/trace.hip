@@ -1,0 +1,417 @@
+// System-matrix construction: device ray tracing into a ray-major CSR, then
+// a stable device radix sort into per-subset pixel-major CSC lists for the
+// gather back projector.  Runs once per geometry; the matrices stay resident
+// in HBM and every projection afterwards only streams them.
+#include "runtime.hpp"
+#include "tracers.hpp"
+
+#include <hipcub/hipcub.hpp>
+
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+namespace mi {
+
+namespace {
+
+struct TraceArgs {
+    int beam;
+    int nviews, nbins, n;        // n: parallel-beam image size
+    int flavour;
+    FanGeom fan;
+    const double *fcos, *fsin;   // fan: per-view double tables
+    const float *pcos, *psin;    // parallel: per-view float tables
+    const int *vlist;            // subset-ordered views
+};
+
+template <typename Emit>
+__device__ __forceinline__ void trace_row(const TraceArgs &a, int64_t r, Emit &&emit) {
+    const int view = a.vlist[r / a.nbins];
+    const int bin = (int)(r % a.nbins);
+    if (a.beam == MI_BEAM_FAN_FLAT) {
+        trace_fan(a.fan, a.fcos[view], a.fsin[view], bin, emit);
+    } else if (a.flavour == 1) {
+        trace_parallel<float>(a.psin[view], a.pcos[view], bin, a.nbins, a.n, emit);
+    } else {
+        trace_parallel<double>(a.psin[view], a.pcos[view], bin, a.nbins, a.n, emit);
+    }
+}
+
+__global__ void k_trace_count(TraceArgs a, int64_t nrows, int64_t *counts) {
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    int64_t n = 0;
+    trace_row(a, r, [&](int, double) { ++n; });
+    counts[r + 1] = n;
+}
+
+__global__ void k_trace_fill(TraceArgs a, int64_t nrows, const int64_t *rowptr, uint32_t *idx,
+                             float *len32, double *len64) {
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    int64_t k = rowptr[r];
+    const int64_t end = rowptr[r + 1];
+    trace_row(a, r, [&](int pix, double w) {
+        if (k < end) {
+            idx[k] = (uint32_t)pix;
+            len32[k] = (float)w;
+            if (len64) len64[k] = w;
+        }
+        ++k;
+    });
+}
+
+// One wave per row: sort key (subset * npix + pixel), the row id and the
+// entry's position for every entry.
+__global__ void k_csc_keys(const int64_t *rowptr, const uint32_t *idx, int64_t nrows, const int64_t *sub_row0,
+                           int S, uint32_t npix, uint32_t *keys, uint32_t *pos, uint32_t *rowid) {
+    const int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    if (r >= nrows) return;
+    int s = 0;
+    while (s + 1 < S && sub_row0[s + 1] <= r) ++s;
+    for (int64_t k = rowptr[r] + lane; k < rowptr[r + 1]; k += kWave) {
+        keys[k] = (uint32_t)s * npix + idx[k];
+        pos[k] = (uint32_t)k;
+        rowid[k] = (uint32_t)r;
+    }
+}
+
+__global__ void k_csc_gather(const uint32_t *pos, int64_t nnz, const uint32_t *rowid, const float *len32,
+                             const double *len64, uint32_t *crow, float *clen32, double *clen64) {
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nnz) return;
+    uint32_t k = pos[j];
+    crow[j] = rowid[k];
+    clen32[j] = len32[k];
+    if (clen64) clen64[j] = len64[k];
+}
+
+// colptr from the sorted keys: entry j opens every column between the
+// previous entry's key (exclusive) and its own key (inclusive).
+__global__ void k_csc_colptr(const uint32_t *keys, int64_t nnz, int64_t ncols, int64_t *colptr) {
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nnz) return;
+    int64_t prev = (j == 0) ? -1 : (int64_t)keys[j - 1];
+    for (int64_t c = prev + 1; c <= (int64_t)keys[j]; ++c) colptr[c] = j;
+    if (j == nnz - 1)
+        for (int64_t c = (int64_t)keys[j] + 1; c <= ncols; ++c) colptr[c] = nnz;
+}
+
+__global__ void k_fill_i64(int64_t *p, int64_t n, int64_t v) {
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) p[j] = v;
+}
+
+__global__ void k_perm(const int *vlist, int nbins, int64_t nrows, int64_t *perm) {
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nrows) perm[r] = (int64_t)vlist[r / nbins] * nbins + r % nbins;
+}
+
+inline unsigned grid_of(int64_t n, int b = kBlock) { return (unsigned)((n + b - 1) / b); }
+
+// ---------------------------------------------------------------------------
+// GoldenC symmetric tiled DDA (origin/GoldenC/src/proj.c:14-397), walked on
+// the host: one traversal per (angle 1..45, tile, ray 0..383) feeds up to
+// eight (view, bin) rays through the grid's rotations and mirror.  Entries
+// are kept in the order the reference accumulates them.  The view-90 column
+// differs between A (W/2 + k, proj.c:32) and AStar (W/2 - 1 + k, proj.c:225),
+// so the adjoint list is walked separately.
+struct GcRows {
+    std::vector<std::vector<std::pair<uint32_t, double>>> rows;  // [view*767 + bin]
+};
+
+void goldenc_walk(GcRows &out, bool adjoint) {
+    const int N = 512, NB = 767, NV = 180, MID = 383, TILE = 128, TX = 4, NT = 16;
+    out.rows.assign((size_t)NV * NB, {});
+    auto put = [&](int view, int bin, int r, int c, double w) {
+        out.rows[(size_t)view * NB + bin].emplace_back((uint32_t)(r * N + c), w);
+    };
+    for (int ray = 0; ray < NB; ++ray) {
+        int k = ray - NB / 2;
+        if (k > N / 2 - 1 || k < 1 - N / 2) continue;
+        for (int c = 0; c < N; ++c) put(0, ray, N / 2 - 1 + k, c, 1.0);
+    }
+    for (int ray = 0; ray < NB; ++ray) {
+        int k = ray - NB / 2;
+        if (k > N / 2 - 1 || k < 1 - N / 2) continue;
+        int col = adjoint ? N / 2 - 1 + k : N / 2 + k;
+        for (int r = 0; r < N; ++r) put(90, ray, r, col, 1.0);
+    }
+    auto eight = [&](int al, int ray, int ix, int iy, int dx, int dy, double w) {
+        const int W = N;
+        put(al, ray, iy + dy, ix + dx, w);
+        put(90 + al, ray, W - 1 - ix - dx, iy + dy, w);
+        if (ray != MID) {
+            put(al, NB - 1 - ray, W - 1 - iy - dy, W - 1 - ix - dx, w);
+            put(90 + al, NB - 1 - ray, ix + dx, W - 1 - iy - dy, w);
+        }
+        if (al != 45) {
+            put(90 - al, ray, ix + dx, iy + dy, w);
+            put(180 - al, ray, W - 1 - iy - dy, ix + dx, w);
+            if (ray != MID) {
+                put(90 - al, NB - 1 - ray, W - 1 - ix - dx, W - 1 - iy - dy, w);
+                put(180 - al, NB - 1 - ray, iy + dy, W - 1 - ix - dx, w);
+            }
+        }
+    };
+    for (int al = 1; al < 46; ++al) {
+        const double sf = std::sin(3.141593 / 180 * al), cf = std::cos(3.141593 / 180 * al);
+        const double hx = 1.0 / sf, hy = 1.0 / cf, tn = sf / cf, rt = cf / sf;
+        for (int tile = 0; tile < NT; ++tile)
+            for (int ray = 0; ray < MID + 1; ++ray) {
+                const int k = ray - NB / 2;
+                const int dx = TILE * (TX - 1 - (tile % TX)), dy = TILE * (tile / TX);
+                double c = cf * k + sf * k * tn;
+                c = (N / 2 - dy) - tn * (N / 2 - TILE * (tile % TX)) - c;
+                double fp, w, yc, xc;
+                int ix, iy, step;
+                const int incx = -1;
+                if (c < 0) {
+                    c = -c * rt;
+                    ix = (int)c; iy = 0;
+                    fp = ((double)(ix + 1)) - c;
+                    ix = TILE - ix - 1;
+                    w = hy * fp; yc = fp * tn; xc = -fp;
+                    step = 2;
+                } else {
+                    ix = TILE - 1;
+                    iy = (int)c;
+                    fp = ((double)(iy + 1)) - c;
+                    yc = -fp; w = hx * fp; xc = fp * rt;
+                    step = 1;
+                }
+                while (ix >= 0 && ix < TILE && iy >= 0 && iy < TILE) {
+                    if (step == 1) {
+                        xc -= 1.0; yc += tn;
+                        if (xc > 0) { eight(al, ray, ix, iy, dx, dy, hy); w -= hy; ix += incx; }
+                        else { eight(al, ray, ix, iy, dx, dy, w); w = hy - w; iy++; step = 2; }
+                    } else {
+                        yc -= 1.0; xc += rt;
+                        if (yc > 0) { w -= hx; eight(al, ray, ix, iy, dx, dy, hx); iy++; }
+                        else { eight(al, ray, ix, iy, dx, dy, w); w = hx - w; ix += incx; step = 1; }
+                    }
+                }
+            }
+    }
+}
+
+// Host CSR in subset row order from per-(view, bin) lists.
+void gc_to_csr(const GcRows &gr, const std::vector<int> &vlist, int nbins, std::vector<int64_t> &rowptr,
+               std::vector<uint32_t> &idx, std::vector<double> &len) {
+    const int64_t nrows = (int64_t)vlist.size() * nbins;
+    rowptr.assign(nrows + 1, 0);
+    for (int64_t r = 0; r < nrows; ++r)
+        rowptr[r + 1] = rowptr[r] + (int64_t)gr.rows[(size_t)vlist[r / nbins] * nbins + r % nbins].size();
+    idx.resize(rowptr[nrows]);
+    len.resize(rowptr[nrows]);
+    for (int64_t r = 0; r < nrows; ++r) {
+        int64_t k = rowptr[r];
+        for (auto &e : gr.rows[(size_t)vlist[r / nbins] * nbins + r % nbins]) {
+            idx[k] = e.first;
+            len[k] = e.second;
+            ++k;
+        }
+    }
+}
+
+template <typename T>
+void upload(DevBuf<T> &b, const std::vector<T> &h, hipStream_t st) {
+    b.alloc(h.size());
+    if (!h.empty()) MI_HIP(hipMemcpyAsync(b.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
+}
+
+void inclusive_scan_i64(int64_t *d, int64_t n, hipStream_t st) {
+    size_t tmp = 0;
+    MI_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, tmp, d, d, (int)n, st));
+    DevBuf<unsigned char> t(tmp ? tmp : 1);
+    MI_HIP(hipcub::DeviceScan::InclusiveSum(t.p, tmp, d, d, (int)n, st));
+    MI_HIP(hipStreamSynchronize(st));
+}
+
+// Per-subset CSC from a subset-ordered CSR (device arrays).
+void build_csc(Matrix &m, const int64_t *rowptr, const uint32_t *idx, const float *len32, const double *len64,
+               hipStream_t st) {
+    const int64_t nnz = m.nnz, ncols = (int64_t)m.S * m.npix;
+    MI_REQUIRE(nnz < (int64_t)UINT32_MAX && ncols < (int64_t)UINT32_MAX, MI_E_LIMIT,
+               "system matrix too large for 32-bit entry positions");
+    DevBuf<int64_t> d_sub(m.S + 1);
+    MI_HIP(hipMemcpyAsync(d_sub.p, m.sub_row0.data(), (m.S + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    m.colptr.alloc(ncols + 1);
+    m.crow.alloc(nnz ? nnz : 1);
+    m.clen32.alloc(nnz ? nnz : 1);
+    if (len64) m.clen64.alloc(nnz ? nnz : 1);
+    if (nnz == 0) {
+        k_fill_i64<<<grid_of(ncols + 1), kBlock, 0, st>>>(m.colptr.p, ncols + 1, 0);
+        MI_HIP(hipGetLastError());
+        return;
+    }
+    DevBuf<uint32_t> keys(nnz), keys2(nnz), pos(nnz), pos2(nnz), rowid(nnz);
+    k_csc_keys<<<grid_of(m.nrows * kWave), kBlock, 0, st>>>(rowptr, idx, m.nrows, d_sub.p, m.S, (uint32_t)m.npix,
+                                                           keys.p, pos.p, rowid.p);
+    MI_HIP(hipGetLastError());
+    int bits = 1;
+    while (bits < 32 && ((int64_t)1 << bits) < ncols) ++bits;
+    size_t tmp = 0;
+    MI_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, keys.p, keys2.p, pos.p, pos2.p, (int)nnz, 0, bits, st));
+    {
+        DevBuf<unsigned char> t(tmp ? tmp : 1);
+        MI_HIP(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, keys.p, keys2.p, pos.p, pos2.p, (int)nnz, 0, bits, st));
+        k_csc_gather<<<grid_of(nnz), kBlock, 0, st>>>(pos2.p, nnz, rowid.p, len32, len64, m.crow.p, m.clen32.p,
+                                                      len64 ? m.clen64.p : nullptr);
+        MI_HIP(hipGetLastError());
+        k_csc_colptr<<<grid_of(nnz), kBlock, 0, st>>>(keys2.p, nnz, ncols, m.colptr.p);
+        MI_HIP(hipGetLastError());
+        MI_HIP(hipStreamSynchronize(st));
+    }
+}
+
+}  // namespace
+
+std::vector<int> subset_order(int S) {
+    int bits = 0;
+    while ((1 << bits) < S) ++bits;
+    std::vector<int> o;
+    for (int i = 0; i < (1 << bits); ++i) {
+        int r = 0;
+        for (int b = 0; b < bits; ++b)
+            if (i & (1 << b)) r |= 1 << (bits - 1 - b);
+        if (r < S) o.push_back(r);
+    }
+    return o;
+}
+
+size_t Matrix::bytes() const {
+    return d_vlist.bytes() + rowptr.bytes() + idx.bytes() + len32.bytes() + len64.bytes() + colptr.bytes() +
+           crow.bytes() + clen32.bytes() + clen64.bytes() + perm.bytes();
+}
+
+std::unique_ptr<Matrix> build_matrix(const mi_geometry &g, int S, bool want64, hipStream_t st) {
+    validate_geometry(g);
+    MI_REQUIRE(S >= 1 && S <= g.n_views, MI_E_ARG, "n_subsets must be in [1, n_views]");
+    auto m = std::make_unique<Matrix>();
+    m->g = g;
+    m->S = S;
+    m->npix = g.nx * g.ny;
+    m->nrows = (int64_t)g.n_views * g.n_bins;
+    m->has64 = want64;
+    const auto order = subset_order(S);
+    m->sub_row0.assign(S + 1, 0);
+    for (int s = 0; s < S; ++s) {
+        int cnt = 0;
+        for (int v = 0; v < g.n_views; ++v)
+            if (v % S == order[s]) { m->vlist.push_back(v); ++cnt; }
+        m->sub_row0[s + 1] = m->sub_row0[s] + (int64_t)cnt * g.n_bins;
+    }
+    upload(m->d_vlist, m->vlist, st);
+    m->perm.alloc(m->nrows);
+    k_perm<<<grid_of(m->nrows), kBlock, 0, st>>>(m->d_vlist.p, g.n_bins, m->nrows, m->perm.p);
+    MI_HIP(hipGetLastError());
+
+    if (g.beam == MI_BEAM_GOLDENC) {
+        GcRows fwd, adj;
+        goldenc_walk(fwd, false);
+        goldenc_walk(adj, true);
+        std::vector<int64_t> rp, rpa;
+        std::vector<uint32_t> ix, ixa;
+        std::vector<double> ln, lna;
+        gc_to_csr(fwd, m->vlist, g.n_bins, rp, ix, ln);
+        gc_to_csr(adj, m->vlist, g.n_bins, rpa, ixa, lna);
+        MI_REQUIRE(rp.back() == rpa.back(), MI_E_STATE, "GoldenC adjoint walk size mismatch");
+        m->nnz = rp.back();
+        std::vector<float> ln32(ln.begin(), ln.end()), lna32(lna.begin(), lna.end());
+        upload(m->rowptr, rp, st);
+        upload(m->idx, ix, st);
+        upload(m->len32, ln32, st);
+        if (want64) upload(m->len64, ln, st);
+        DevBuf<int64_t> arp;
+        DevBuf<uint32_t> aix;
+        DevBuf<float> aln32;
+        DevBuf<double> aln64;
+        upload(arp, rpa, st);
+        upload(aix, ixa, st);
+        upload(aln32, lna32, st);
+        if (want64) upload(aln64, lna, st);
+        build_csc(*m, arp.p, aix.p, aln32.p, want64 ? aln64.p : nullptr, st);
+        return m;
+    }
+
+    TraceArgs a{};
+    a.beam = g.beam;
+    a.nviews = g.n_views;
+    a.nbins = g.n_bins;
+    a.n = g.nx;
+    a.flavour = g.flavour;
+    a.vlist = m->d_vlist.p;
+    DevBuf<double> dcos, dsin;
+    DevBuf<float> pcos, psin;
+    if (g.beam == MI_BEAM_FAN_FLAT) {
+        std::vector<double> c(g.n_views), s(g.n_views);
+        const double span = g.span_deg * (M_PI / 180.0);
+        for (int v = 0; v < g.n_views; ++v) {
+            double th = span * (double)v / (double)g.n_views;
+            c[v] = std::cos(th);
+            s[v] = std::sin(th);
+        }
+        upload(dcos, c, st);
+        upload(dsin, s, st);
+        a.fan = FanGeom{g.nx, g.ny, g.n_bins, g.pixel_mm, g.pixel_mm, g.bin_mm, g.sod_mm, g.sdd_mm, g.det_offset};
+        a.fcos = dcos.p;
+        a.fsin = dsin.p;
+    } else {
+        // posit() of msbeam/CPU_OpenMP/src/proj.c:55-56 (double pi, double
+        // libm) or blockrecon/src/wray.cpp:11-12 (float pi, float overloads).
+        std::vector<float> c(g.n_views), s(g.n_views);
+        for (int v = 0; v < g.n_views; ++v) {
+            if (g.flavour == 1) {
+                const float pi = 3.14159265359f;
+                s[v] = sinf(pi / g.n_views * v);
+                c[v] = cosf(pi / g.n_views * v);
+            } else {
+                s[v] = (float)std::sin(3.14159265359 / g.n_views * v);
+                c[v] = (float)std::cos(3.14159265359 / g.n_views * v);
+            }
+        }
+        upload(pcos, c, st);
+        upload(psin, s, st);
+        a.pcos = pcos.p;
+        a.psin = psin.p;
+    }
+
+    m->rowptr.alloc(m->nrows + 1);
+    MI_HIP(hipMemsetAsync(m->rowptr.p, 0, sizeof(int64_t), st));
+    k_trace_count<<<grid_of(m->nrows, 64), 64, 0, st>>>(a, m->nrows, m->rowptr.p);
+    MI_HIP(hipGetLastError());
+    inclusive_scan_i64(m->rowptr.p, m->nrows + 1, st);
+    MI_HIP(hipMemcpy(&m->nnz, m->rowptr.p + m->nrows, sizeof(int64_t), hipMemcpyDeviceToHost));
+    m->idx.alloc(m->nnz ? m->nnz : 1);
+    m->len32.alloc(m->nnz ? m->nnz : 1);
+    if (want64) m->len64.alloc(m->nnz ? m->nnz : 1);
+    k_trace_fill<<<grid_of(m->nrows, 64), 64, 0, st>>>(a, m->nrows, m->rowptr.p, m->idx.p, m->len32.p,
+                                                       want64 ? m->len64.p : nullptr);
+    MI_HIP(hipGetLastError());
+    build_csc(*m, m->rowptr.p, m->idx.p, m->len32.p, want64 ? m->len64.p : nullptr, st);
+    return m;
+}
+
+int trace_one(const mi_geometry &g, int view, int bin, int32_t *ind, double *len, int cap, hipStream_t st) {
+    validate_geometry(g);
+    MI_REQUIRE(view >= 0 && view < g.n_views && bin >= 0 && bin < g.n_bins, MI_E_ARG, "ray out of range");
+    MI_REQUIRE(g.beam != MI_BEAM_GOLDENC, MI_E_ARG, "GoldenC rays are assembled from tiles; use the matrix");
+    auto m = build_matrix(g, 1, true, st);
+    std::vector<int64_t> rp(2);
+    const int64_t r = (int64_t)view * g.n_bins + bin;
+    MI_HIP(hipMemcpy(rp.data(), m->rowptr.p + r, 2 * sizeof(int64_t), hipMemcpyDeviceToHost));
+    const int n = (int)(rp[1] - rp[0]);
+    const int take = n < cap ? n : cap;
+    std::vector<uint32_t> ix(take);
+    if (take > 0) {
+        MI_HIP(hipMemcpy(ix.data(), m->idx.p + rp[0], take * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        MI_HIP(hipMemcpy(len, m->len64.p + rp[0], take * sizeof(double), hipMemcpyDeviceToHost));
+        for (int i = 0; i < take; ++i) ind[i] = (int32_t)ix[i];
+    }
+    return n;
+}
+
+}  // namespace mi

@@ -1,0 +1,198 @@
+// Device ray tracers.  Each walks one ray and hands every (pixel, length)
+// pair to an emit functor; the count pass and the fill pass of the system
+// matrix use the same walk, so the two passes cannot disagree.
+//
+// The arithmetic is written in the reference's own types and operation
+// order and the whole library is compiled with -ffp-contract=off, which
+// keeps the index/length sequences bit-identical to a strict-IEEE build of
+// the reference tracers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace mi {
+
+// ---------------------------------------------------------------------------
+// Parallel beam, incremental tracer of msbeam/CPU_OpenMP/src/proj.c:67-211
+// (= blockrecon/src/wray.cpp:23-167), posit() of proj.c:54-65 with its sin/cos
+// supplied per view by the host (libm, in the reference's type).  CT is the
+// type of the tolerance constants: double for msbeam (proto.h:16-17), float
+// for blockrecon (config.h:35-36); mixed expressions promote as they do
+// there.
+template <typename CT, typename Emit>
+__device__ __forceinline__ void trace_parallel(float s_view, float c_view, int nr, int nray, int n, Emit &&emit) {
+    const CT F_TOL = (CT)1e-8;
+    const CT GRID_TOL = (CT)1e-2;
+    float px, py, cf, sf, rcos, tanp, rtan, c, fp, yc;
+    int ix, iy, incx, incy, idx, flip, swapxy;
+    bool up;
+
+    {
+        float off = (float)(nr - nray / 2) * 1;
+        px = -s_view * off;
+        py = c_view * off;
+        cf = c_view;
+        sf = s_view;
+    }
+    if (sf < -F_TOL) { sf = -sf; cf = -cf; }
+    flip = 0;
+    if (sf * cf < -F_TOL * F_TOL) { flip = 1; cf = -cf; }
+    if (flip > 0) px = -px;
+    if ((CT)fabsf(sf) <= F_TOL) { if (sf < 0) sf = -F_TOL * F_TOL; else sf = F_TOL * F_TOL; }
+    if ((CT)fabsf(cf) <= F_TOL) { if (cf < 0) cf = -F_TOL * F_TOL; else cf = F_TOL * F_TOL; }
+    if ((CT)fabsf(cf) > F_TOL) tanp = sf / cf; else tanp = 1 / (F_TOL * F_TOL);
+    if ((CT)fabsf(sf) > F_TOL) rtan = cf / sf; else rtan = 1 / (F_TOL * F_TOL);
+
+    swapxy = 0;
+    if (tanp > 1 + F_TOL) {
+        swapxy = 1;
+        tanp = 1 / tanp;
+        rtan = 1 / rtan;
+        float t = py; py = px; px = t;
+    }
+    // Correctly rounded float sqrt == double sqrt rounded to float.
+    rcos = __fsqrt_rn(1 + tanp * tanp);
+    py = (float)((double)py + n / 2.0);
+    px = (float)((double)px + n / 2.0);
+    c = py - px * tanp;
+
+    if ((c - n) >= GRID_TOL) return;
+    if (c < -GRID_TOL) {
+        c = -c * rtan;
+        if ((c - n) >= GRID_TOL) return;
+        ix = (int)c;
+        iy = 0;
+        fp = ((float)(ix + 1)) - c;
+        yc = fp * tanp;
+        yc = (float)((double)yc + -1.0);
+    } else {
+        ix = 0;
+        iy = (int)c;
+        fp = ((float)(iy + 1)) - c;
+        yc = -fp;
+        yc += tanp;
+    }
+    up = (yc >= 0);
+
+    if (swapxy > 0) {
+        idx = iy + ix * n; incx = n; incy = 1;
+        if (flip > 0) { idx = n - 1 - iy + ix * n; incx = n; incy = -1; }
+    } else {
+        idx = iy * n + ix; incx = 1; incy = n;
+        if (flip > 0) { idx = n - 1 - ix + iy * n; incx = -1; incy = n; }
+    }
+
+    for (;;) {
+        if (up) {
+            if (iy + 1 >= n || ix < 0) return;
+            float w = rcos * (1 - yc * rtan);
+            emit(idx, (double)w);
+            iy++; idx += incy;
+            if (ix + 1 >= n) return;
+            w = rcos * (yc * rtan);
+            emit(idx, (double)w);
+            ix++; idx += incx;
+            yc = (float)((double)yc + ((double)tanp - 1.0));
+        } else {
+            if (iy >= n || ix + 1 >= n || ix < 0) return;
+            emit(idx, (double)rcos);
+            ix++; idx += incx;
+            yc += tanp;
+        }
+        up = (yc >= 0);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Parametric Siddon in double, helical/rebin/src/tracing.cpp:22-147.  Image
+// spans [0, nx*vx] x [0, ny*vy]; pixel (x, y) -> x*ny + y.  The walk stops
+// 5e-2 before the exit (tracing.cpp:118).  The grid test inside the loop only
+// triggers for rays through a grid corner, where the reference would index
+// outside its arrays.
+__device__ __forceinline__ double sd_cross(double p, double s, double d, double L) { return L * (p - s) / (d - s); }
+
+template <typename Emit>
+__device__ __forceinline__ void trace_siddon2d(double sx, double sy, double dx, double dy,
+                                               int nx, int ny, double vx, double vy, Emit &&emit) {
+    double rx = dx - sx, ry = dy - sy;
+    double L = sqrt(rx * rx + ry * ry);
+    double lo = 0.0, hi = L, a0, a1, t, ax, ay, lam;
+    int sgx = (rx > 0) ? 1 : -1, sgy = (ry > 0) ? 1 : -1;
+    double arx = fabs(rx), ary = fabs(ry);
+    double stx = (arx > 1.e-4) ? (L / arx) * vx : 1.e6;
+    double sty = (ary > 1.e-4) ? (L / ary) * vy : 1.e6;
+    int entry = 0, cx, cy;
+
+    a0 = sd_cross(0, sx, dx, L);
+    a1 = sd_cross(nx * vx, sx, dx, L);
+    t = (a0 < a1) ? a0 : a1;
+    lo = (lo > t) ? lo : t;
+    if (lo == t) entry = 1;
+    t = (a0 > a1) ? a0 : a1;
+    hi = (hi < t) ? hi : t;
+
+    a0 = sd_cross(0, sy, dy, L);
+    a1 = sd_cross(ny * vy, sy, dy, L);
+    t = (a0 < a1) ? a0 : a1;
+    lo = (lo > t) ? lo : t;
+    if (lo == t) entry = 2;
+    t = (a0 > a1) ? a0 : a1;
+    hi = (hi < t) ? hi : t;
+
+    if (lo >= hi) return;
+    lam = lo;
+    if (entry == 1) {
+        cx = (sgx == 1) ? 0 : nx - 1;
+        ax = lam + stx;
+        cy = (sy + lam * ry / L) / vy;
+        double py = cy * vy;
+        ay = (ary < 1.e-4) ? 1.e6 : sd_cross(py + (sgy > 0) * vy, sy, dy, L);
+    } else {
+        cy = (sgy == 1) ? 0 : ny - 1;
+        ay = lam + sty;
+        cx = (sx + lam * rx / L) / vx;
+        double px = cx * vx;
+        ax = (arx < 1.e-4) ? 1.e6 : sd_cross(px + (sgx > 0) * vx, sx, dx, L);
+    }
+
+    while (lam < hi - 5.e-2) {
+        if (cx < 0 || cx >= nx || cy < 0 || cy >= ny) return;
+        if (ax <= ay) {
+            emit(cx * ny + cy, ax - lam);
+            lam = ax; ax += stx; cx += sgx;
+        } else {
+            emit(cx * ny + cy, ay - lam);
+            lam = ay; ay += sty; cy += sgy;
+        }
+    }
+}
+
+// Flat-detector fan geometry: industry3d/src/tracing.cpp:51-81 (cone(), on
+// the mid-plane) with rotate_2d of industry3d/src/utility.cpp:82-98, in
+// double and millimetres, image corner at the origin (as
+// helical/rebin/src/ms2d.cpp:172-183 places its arc-detector rays).
+struct FanGeom {
+    int nx, ny, nbins;
+    double vx, vy, bin_mm, sod, sdd, det_offset;
+};
+
+__device__ __forceinline__ void fan_rotate(double &x, double &y, double c, double s) {
+    double r0 = 0.0, r1 = 0.0;
+    r0 += c * x; r1 += s * x;
+    r0 += -s * y; r1 += c * y;
+    x = r0; y = r1;
+}
+
+template <typename Emit>
+__device__ __forceinline__ void trace_fan(const FanGeom &g, double c, double s, int bin, Emit &&emit) {
+    double x0 = 0.0, y0 = -g.sod;
+    double x1 = ((double)(bin - g.nbins / 2) + 0.5 + g.det_offset) * g.bin_mm;
+    double y1 = g.sdd - g.sod;
+    double cx = g.nx * g.vx / 2.0, cy = g.ny * g.vy / 2.0;
+    fan_rotate(x0, y0, c, s);
+    fan_rotate(x1, y1, c, s);
+    trace_siddon2d(x0 + cx, y0 + cy, x1 + cx, y1 + cy, g.nx, g.ny, g.vx, g.vy, emit);
+}
+
+}  // namespace mi

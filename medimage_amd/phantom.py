@@ -1,0 +1,81 @@
+"""Synthetic inputs: seeded random-ellipse phantoms and the Poisson/log
+sinogram pipeline.
+
+All randomness comes from numpy's counter-based Philox generator keyed by
+(seed, stream), so an input is a pure function of its key.  The noise-free
+sinogram is an FP64 forward projection through the device projector; the
+noise is drawn on the host.
+
+Conventions (shared with the tracers): image row index x, column index y,
+pixel (x, y) centred at ((x + 0.5) * pixel_mm, (y + 0.5) * pixel_mm) with the
+image corner at the origin; ellipse centres are given relative to the image
+centre.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+
+def rng(seed: int, stream: int = 0) -> np.random.Generator:
+    return np.random.Generator(np.random.Philox(key=[int(seed) & (2**64 - 1), int(stream) & (2**64 - 1)]))
+
+
+def random_ellipses(n_ellipses, fov_mm, seed, stream=0, axes_mm=(5.0, 100.0), mu_max=0.02):
+    """n_ellipses rows of (cx, cy, a, b, theta, mu): centres uniform in the
+    inner 80% of the field of view, semi-axes uniform in axes_mm, rotation
+    uniform in [0, pi), attenuation uniform in [0, mu_max] per mm."""
+    g = rng(seed, stream)
+    half = 0.4 * fov_mm
+    e = np.empty((n_ellipses, 6))
+    e[:, 0] = g.uniform(-half, half, n_ellipses)
+    e[:, 1] = g.uniform(-half, half, n_ellipses)
+    e[:, 2] = g.uniform(axes_mm[0], axes_mm[1], n_ellipses)
+    e[:, 3] = g.uniform(axes_mm[0], axes_mm[1], n_ellipses)
+    e[:, 4] = g.uniform(0.0, math.pi, n_ellipses)
+    e[:, 5] = g.uniform(0.0, mu_max, n_ellipses)
+    return e
+
+
+def rasterize(ellipses, n, pixel_mm):
+    """Sum of ellipse indicators times attenuation, sampled at pixel centres."""
+    c = (np.arange(n) + 0.5) * pixel_mm - n * pixel_mm / 2.0
+    X, Y = np.meshgrid(c, c, indexing="ij")
+    img = np.zeros((n, n), np.float64)
+    for cx, cy, a, b, th, mu in ellipses:
+        dx, dy = X - cx, Y - cy
+        ct, st = math.cos(th), math.sin(th)
+        xr = ct * dx + st * dy
+        yr = -st * dx + ct * dy
+        img += mu * ((xr / a) ** 2 + (yr / b) ** 2 <= 1.0)
+    return img
+
+
+def ellipse_line_integral(ellipse, p0, p1):
+    """Closed-form length of the chord of the line p0 -> p1 through the
+    ellipse times its attenuation (coordinates relative to the image centre)."""
+    cx, cy, a, b, th, mu = ellipse
+    ct, st = math.cos(th), math.sin(th)
+    d = np.asarray(p1, float) - np.asarray(p0, float)
+    L = float(np.hypot(*d))
+    u = d / L
+    q = np.asarray(p0, float) - (cx, cy)
+    qx, qy = ct * q[0] + st * q[1], -st * q[0] + ct * q[1]
+    ux, uy = ct * u[0] + st * u[1], -st * u[0] + ct * u[1]
+    A = (ux / a) ** 2 + (uy / b) ** 2
+    B = 2 * (qx * ux / a**2 + qy * uy / b**2)
+    Cc = (qx / a) ** 2 + (qy / b) ** 2 - 1
+    disc = B * B - 4 * A * Cc
+    if disc <= 0:
+        return 0.0
+    return mu * math.sqrt(disc) / A
+
+
+def noisy_sinogram(p, i0, seed, stream=0):
+    """counts ~ Poisson(i0 * exp(-p)); zero counts are clamped to one; returns
+    -ln(counts / i0)."""
+    g = rng(seed, 1_000_003 + stream)
+    lam = i0 * np.exp(-np.asarray(p, np.float64))
+    counts = g.poisson(lam).astype(np.float64)
+    return -np.log(np.maximum(counts, 1.0) / i0)

@@ -1,0 +1,269 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle and
+the reference's fixtures.
+
+Bars: bit-exact for tracer indices and lengths; FP64 runs within 1e-11 of
+the oracle (summation order differs); FP32 runs within 1e-5 relative L2 and
+max (the stated tolerance) of the FP64 oracle.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import medimage_amd as mi
+from medimage_amd import configs, phantom
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def rel_max(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def small_fan(n=64, views=60, bins=96):
+    return mi.fan_geometry(n, 358.4 / n * 0.25 * 4 / 4, views, bins, 4.0 * 64 / bins)
+
+
+# --------------------------------------------------------------------------
+# Tracers: bit-exact.
+
+@pytest.mark.parametrize("case", ["small", "c1", "c2"])
+def test_fan_tracer_bitexact(ctx, case):
+    g = {"small": mi.fan_geometry(48, 1.1, 40, 72, 1.3),
+         "c1": configs.c1_geometry(),
+         "c2": configs.c2_geometry()}[case]
+    print(f"[fan tracer {case}] {g}", flush=True)
+    rp, ind, ln = ctx.matrix(g)
+    orp, oind, oln = O.fan_csr(O.fan_from(g))
+    assert np.array_equal(rp, orp)
+    assert np.array_equal(ind, oind)
+    assert np.array_equal(ln.view(np.uint64), oln.view(np.uint64))
+
+
+@pytest.mark.parametrize("flavour", [0, 1])
+def test_parallel_tracer_bitexact(ctx, flavour):
+    g = mi.parallel_geometry(512, 180, 768, flavour)
+    rp, ind, ln = ctx.matrix(g)
+    orp, oind, oln = O.wray_csr(180, 768, 512, flavour)
+    assert np.array_equal(rp, orp)
+    assert np.array_equal(ind, oind)
+    assert np.array_equal(ln.astype(np.float32).view(np.uint32), oln.view(np.uint32))
+    assert np.array_equal(ln, oln.astype(np.float64))
+
+
+def test_trace_ray_matches_matrix(ctx):
+    g = configs.c1_geometry()
+    rp, ind, ln = ctx.matrix(g)
+    for view, b in [(0, 0), (17, 255), (359, 511), (180, 300)]:
+        i2, l2 = ctx.trace_ray(g, view, b)
+        r = view * g.n_bins + b
+        assert np.array_equal(i2, ind[rp[r]:rp[r + 1]])
+        assert np.array_equal(l2, ln[rp[r]:rp[r + 1]])
+
+
+# --------------------------------------------------------------------------
+# Projectors.
+
+def _img(g, seed=1):
+    ell = phantom.random_ellipses(10, g.nx * g.pixel_mm, seed)
+    return phantom.rasterize(ell, g.nx, g.pixel_mm)
+
+
+@pytest.mark.parametrize("case", ["small", "c1"])
+def test_forward_backward_fp64(ctx, case):
+    g = mi.fan_geometry(48, 1.1, 40, 72, 1.3) if case == "small" else configs.c1_geometry()
+    fg = O.fan_from(g)
+    x = _img(g)
+    rs = np.random.RandomState(2)
+    y = rs.rand(g.n_views, g.n_bins)
+    assert rel_max(ctx.forward(g, x, np.float64), O.fan_fp(fg, x)) < 1e-12
+    assert rel_max(ctx.backproject(g, y, np.float64), O.fan_bp(fg, y)) < 1e-12
+
+
+def test_forward_backward_fp32_kahan(ctx):
+    g = configs.c1_geometry()
+    fg = O.fan_from(g)
+    x = _img(g).astype(np.float32)
+    rs = np.random.RandomState(3)
+    y = rs.rand(g.n_views, g.n_bins).astype(np.float32)
+    fp = ctx.forward(g, x)
+    bp = ctx.backproject(g, y)
+    ofp = O.fan_fp(fg, x.astype(np.float64))
+    obp = O.fan_bp(fg, y.astype(np.float64))
+    # inputs are exact in both; only the float lengths and the Kahan sums differ
+    assert rel_l2(fp, ofp) < 1e-6 and rel_max(fp, ofp) < 1e-6
+    assert rel_l2(bp, obp) < 1e-6 and rel_max(bp, obp) < 1e-6
+
+
+def test_adjoint_identity_fp64(ctx):
+    g = configs.c1_geometry()
+    rs = np.random.RandomState(4)
+    x = rs.rand(g.nx, g.ny)
+    y = rs.rand(g.n_views, g.n_bins)
+    lhs = np.sum(ctx.forward(g, x, np.float64) * y)
+    rhs = np.sum(x * ctx.backproject(g, y, np.float64))
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+def test_kat_std_g_on_gpu(ctx, golden):
+    """The GPU parallel projector on blockrecon's phantom meets the same
+    known-answer bar as the oracle, and agrees with the oracle's sequential
+    float sums to 1e-6."""
+    z = golden("blockrecon_std.npz")
+    g = mi.parallel_geometry(512, 180, 768, flavour=1)
+    gp = ctx.forward(g, z["f"]).astype(np.float64)
+    orc = O.par_fp_f32(z["f"], 180, 768, 1).astype(np.float64)
+    assert rel_max(gp, orc) < 1e-6
+    gref = z["g"]
+    unit = 10.0 ** (np.floor(np.log10(np.where(gref > 0, gref, 1.0))) - 5)
+    assert (np.abs(gp - gref) <= 1.0000001 * unit).mean() >= 0.997
+
+
+def test_goldenc_projector_fp64(ctx, golden):
+    g = mi.goldenc_geometry()
+    f = golden("blockrecon_std.npz")["f"].astype(np.float64)
+    rs = np.random.RandomState(5)
+    s = rs.rand(180, 767)
+    assert rel_max(ctx.forward(g, f, np.float64), O.gc_fp(f)) < 1e-13
+    assert rel_max(ctx.backproject(g, s, np.float64), O.gc_bp(s)) < 1e-13
+
+
+def test_goldenc_projector_matches_reference_fixture(ctx, golden):
+    g = mi.goldenc_geometry()
+    z = golden("goldenc_ms.npz")
+    f = golden("blockrecon_std.npz")["f"].astype(np.float64)
+    assert rel_max(ctx.forward(g, f, np.float64), z["g"]) < 1e-13
+
+
+# --------------------------------------------------------------------------
+# GoldenC AT building blocks.
+
+def test_medfilt2_exact(ctx):
+    rs = np.random.RandomState(6)
+    for shape in [(512, 512), (37, 53)]:
+        a = rs.randint(-4, 5, size=shape).astype(np.float64) * 0.5
+        assert np.array_equal(ctx.medfilt2(a), O.medfilt2(a))
+    a = rs.rand(300, 200)
+    assert np.array_equal(ctx.medfilt2(a), O.medfilt2(a))
+
+
+def test_goldenc_image_and_edge_steps(ctx, golden):
+    g = mi.goldenc_geometry()
+    z = golden("goldenc_ms.npz")
+    f0 = np.zeros((512, 512))
+    v0 = np.ones((512, 512))
+    gpu = ctx.image_step(g, z["g"], f0, v0, 1000.0, 1e-4, 3)
+    orc, _, _ = O.image_step(O.op_goldenc(), z["g"], f0, v0, 1000.0, 1e-4, 3)
+    assert rel_l2(gpu, orc) < 1e-10
+    ve = ctx.edge_step(orc, v0, 1000.0, 0.002, 1e-4, 4)
+    vo, _ = O.edge_step(orc, v0, 1000.0, 0.002, 1e-4, 4)
+    assert rel_max(ve, vo) < 1e-10
+
+
+def test_goldenc_mumford_shah_end_to_end(ctx, golden):
+    """MumfordShah(1000, 0.002, 1e-4), 10 x (10 CG + 10 SD), on the device in
+    FP64 against the reference program's own output."""
+    g = mi.goldenc_geometry()
+    z = golden("goldenc_ms.npz")
+    p = mi.at_params(1000.0, 0.002, 1e-4, iter_alt=10, iter_image=10, iter_edge=10, precision=mi.MI_FP64)
+    f, v, rep = ctx.mumford_shah(g, z["g"], p)
+    print(f"[mumford_shah] rel_l2 f {rel_l2(f, z['f']):.3e} v {rel_l2(v, z['v']):.3e} "
+          f"{rep.elapsed_ms:.1f} ms", flush=True)
+    assert rel_l2(f, z["f"]) < 1e-7 and rel_l2(v, z["v"]) < 1e-7
+
+
+# --------------------------------------------------------------------------
+# Ordered-subset AT.
+
+def _case(ctx, g, seed=7, i0=5e4):
+    img, sino = configs.make_case(ctx, g, i0, seed)
+    return img, sino
+
+
+def test_os_fp64_matches_oracle(ctx):
+    g = mi.fan_geometry(64, 4.0, 60, 96, 4.0)
+    _, sino = _case(ctx, g)
+    p = mi.at_params(1e-3, 1e-4, 1e-2, n_subsets=4, max_iters=5, iter_edge=1, precision=mi.MI_FP64)
+    u, v, rep = ctx.at_reconstruct(g, sino, p)
+    ou, ov, n, hist = O.os_at(O.op_fan(O.fan_from(g)), sino, 1e-3, 1e-4, 1e-2, 4, 5, iter_edge=1)
+    assert rep.iterations == 5 == n
+    assert rel_l2(u, ou) < 1e-10 and rel_l2(v, ov) < 1e-10
+    assert abs(rep.data_fidelity - hist[-1]) <= 1e-10 * hist[-1]
+
+
+def test_os_fp32_within_tolerance(ctx):
+    g = configs.c1_geometry()
+    _, sino = _case(ctx, g, seed=11, i0=1e5)
+    p = configs.c1_params(n_subsets=4, max_iters=10)
+    u, v, rep = ctx.at_reconstruct(g, sino.astype(np.float32), p)
+    ou, ov, n, hist = O.os_at(O.op_fan(O.fan_from(g)), sino.astype(np.float32).astype(np.float64),
+                              1e-3, 1e-4, 1e-2, 4, 10, iter_edge=1)
+    print(f"[os fp32 c1] rel_l2 u {rel_l2(u, ou):.3e} max {rel_max(u, ou):.3e} v {rel_l2(v, ov):.3e}",
+          flush=True)
+    assert rel_l2(u, ou) < 1e-5 and rel_max(u, ou) < 1e-5
+    assert rel_l2(v, ov) < 1e-5
+
+
+def test_os_deterministic(ctx):
+    g = mi.fan_geometry(128, 2.0, 90, 128, 2.0)
+    _, sino = _case(ctx, g, seed=12)
+    p = mi.at_params(n_subsets=6, max_iters=4)
+    a = ctx.at_reconstruct(g, sino, p)
+    b = ctx.at_reconstruct(g, sino, p)
+    assert np.array_equal(a[0].view(np.uint32), b[0].view(np.uint32))
+    assert np.array_equal(a[1].view(np.uint32), b[1].view(np.uint32))
+
+
+def test_os_convergence_stop(ctx):
+    g = mi.fan_geometry(64, 4.0, 60, 96, 4.0)
+    _, sino = _case(ctx, g, seed=13)
+    p = mi.at_params(n_subsets=4, max_iters=200, rel_tol=1e-3, precision=mi.MI_FP64)
+    u, v, rep = ctx.at_reconstruct(g, sino, p)
+    ou, ov, n, hist = O.os_at(O.op_fan(O.fan_from(g)), sino, 1e-3, 1e-4, 1e-2, 4, 200, iter_edge=1, rel_tol=1e-3)
+    assert rep.converged == 1 and rep.iterations == n < 200
+
+
+def test_os_clamp_and_median(ctx):
+    g = mi.fan_geometry(64, 4.0, 60, 96, 4.0)
+    _, sino = _case(ctx, g, seed=14)
+    p = mi.at_params(n_subsets=3, max_iters=4, median_every=2, lower=0.0, upper=0.015, precision=mi.MI_FP64)
+    u, v, rep = ctx.at_reconstruct(g, sino, p)
+    ou, ov, n, hist = O.os_at(O.op_fan(O.fan_from(g)), sino, 1e-3, 1e-4, 1e-2, 3, 4, iter_edge=1,
+                              median_every=2, lower=0.0, upper=0.015)
+    assert u.min() >= 0 and u.max() <= 0.015
+    assert rel_l2(u, ou) < 1e-10 and rel_l2(v, ov) < 1e-10
+
+
+def test_c2_fp32_vs_fp64(ctx):
+    """The precision workload (512^2, 720 views, 736 bins, 12 subsets, 100
+    AT iterations): FP32 bulk with Kahan sums against the full-FP64 run of
+    the same device algorithm on the same seed."""
+    g = configs.c2_geometry()
+    _, sino = _case(ctx, g, seed=2, i0=2e4)
+    p32 = configs.c2_params()
+    p64 = configs.c2_params(precision=mi.MI_FP64)
+    u32, v32, r32 = ctx.at_reconstruct(g, sino.astype(np.float32), p32)
+    u64, v64, r64 = ctx.at_reconstruct(g, sino.astype(np.float32).astype(np.float64), p64)
+    print(f"[c2] fp32 {r32.elapsed_ms:.1f} ms fp64 {r64.elapsed_ms:.1f} ms rel_l2 u {rel_l2(u32, u64):.3e} "
+          f"max {rel_max(u32, u64):.3e} v {rel_l2(v32, v64):.3e}", flush=True)
+    assert r32.iterations == r64.iterations == 100
+    assert rel_l2(u32, u64) < 1e-5 and rel_max(u32, u64) < 1e-5
+    assert rel_l2(v32, v64) < 1e-5
+
+
+def test_errors_are_reported(ctx):
+    bad = mi.fan_geometry(64, 1.0, 10, 16, 1.0, sod=500.0, sdd=400.0)
+    with pytest.raises(mi.MIError) as e:
+        ctx.forward(bad, np.zeros((64, 64), np.float32))
+    assert e.value.code == mi.MI_E_ARG
+    with pytest.raises(mi.MIError):
+        ctx.at_reconstruct(configs.c1_geometry(), np.zeros((360, 512), np.float32), mi.at_params(n_subsets=0))
