@@ -77,9 +77,6 @@ std::unique_ptr<Matrix> build_matrix(const mi_geometry &g, int S, bool want64, h
 // Subset order: bit-reversed subset indices (values >= S dropped).
 std::vector<int> subset_order(int S);
 
-// Device trace of one ray (for mi_trace_ray).
-int trace_one(const mi_geometry &g, int view, int bin, int32_t *ind, double *len, int cap, hipStream_t st);
-
 // Timing / byte counters of the instrumented kernels.
 struct Stats {
     bool on = false;

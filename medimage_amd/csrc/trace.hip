@@ -99,6 +99,13 @@ __global__ void k_csc_colptr(const uint32_t *keys, int64_t nnz, int64_t ncols, i
         for (int64_t c = (int64_t)keys[j] + 1; c <= ncols; ++c) colptr[c] = nnz;
 }
 
+// Counts entries whose pixel index is outside the image (must be zero before
+// any kernel indexes an image with them).
+__global__ void k_check_idx(const uint32_t *idx, int64_t nnz, uint32_t npix, unsigned *bad) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nnz && idx[k] >= npix) atomicAdd(bad, 1u);
+}
+
 __global__ void k_fill_i64(int64_t *p, int64_t n, int64_t v) {
     int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j < n) p[j] = v;
@@ -230,9 +237,22 @@ void inclusive_scan_i64(int64_t *d, int64_t n, hipStream_t st) {
     MI_HIP(hipStreamSynchronize(st));
 }
 
+void check_indices(const uint32_t *idx, int64_t nnz, int npix, hipStream_t st) {
+    if (nnz == 0) return;
+    DevBuf<unsigned> bad(1);
+    MI_HIP(hipMemsetAsync(bad.p, 0, sizeof(unsigned), st));
+    k_check_idx<<<grid_of(nnz), kBlock, 0, st>>>(idx, nnz, (uint32_t)npix, bad.p);
+    MI_HIP(hipGetLastError());
+    unsigned h = 0;
+    MI_HIP(hipMemcpyAsync(&h, bad.p, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    MI_HIP(hipStreamSynchronize(st));
+    MI_REQUIRE(h == 0, MI_E_STATE, "tracer produced " + std::to_string(h) + " pixel indices outside the image");
+}
+
 // Per-subset CSC from a subset-ordered CSR (device arrays).
 void build_csc(Matrix &m, const int64_t *rowptr, const uint32_t *idx, const float *len32, const double *len64,
                hipStream_t st) {
+    check_indices(idx, m.nnz, m.npix, st);
     const int64_t nnz = m.nnz, ncols = (int64_t)m.S * m.npix;
     MI_REQUIRE(nnz < (int64_t)UINT32_MAX && ncols < (int64_t)UINT32_MAX, MI_E_LIMIT,
                "system matrix too large for 32-bit entry positions");
@@ -393,25 +413,6 @@ std::unique_ptr<Matrix> build_matrix(const mi_geometry &g, int S, bool want64, h
     MI_HIP(hipGetLastError());
     build_csc(*m, m->rowptr.p, m->idx.p, m->len32.p, want64 ? m->len64.p : nullptr, st);
     return m;
-}
-
-int trace_one(const mi_geometry &g, int view, int bin, int32_t *ind, double *len, int cap, hipStream_t st) {
-    validate_geometry(g);
-    MI_REQUIRE(view >= 0 && view < g.n_views && bin >= 0 && bin < g.n_bins, MI_E_ARG, "ray out of range");
-    MI_REQUIRE(g.beam != MI_BEAM_GOLDENC, MI_E_ARG, "GoldenC rays are assembled from tiles; use the matrix");
-    auto m = build_matrix(g, 1, true, st);
-    std::vector<int64_t> rp(2);
-    const int64_t r = (int64_t)view * g.n_bins + bin;
-    MI_HIP(hipMemcpy(rp.data(), m->rowptr.p + r, 2 * sizeof(int64_t), hipMemcpyDeviceToHost));
-    const int n = (int)(rp[1] - rp[0]);
-    const int take = n < cap ? n : cap;
-    std::vector<uint32_t> ix(take);
-    if (take > 0) {
-        MI_HIP(hipMemcpy(ix.data(), m->idx.p + rp[0], take * sizeof(uint32_t), hipMemcpyDeviceToHost));
-        MI_HIP(hipMemcpy(len, m->len64.p + rp[0], take * sizeof(double), hipMemcpyDeviceToHost));
-        for (int i = 0; i < take; ++i) ind[i] = (int32_t)ix[i];
-    }
-    return n;
 }
 
 }  // namespace mi
