@@ -80,10 +80,11 @@ typedef struct or_fan_geom {
     double bin_mm;        /* detector pitch at the detector, mm             */
     double sod, sdd;      /* source-isocentre, source-detector, mm          */
     double det_offset;    /* detector offset in bins                        */
-    double span;          /* angular span, radians (2*pi for a full scan)   */
+    double span_deg;      /* angular span in degrees (360 for a full scan)  */
 } or_fan_geom;
 
-/* Per-view trig tables: theta_a = span * a / nviews, glibc cos/sin. */
+/* Per-view trig tables: theta_a = (span_deg * (pi / 180)) * a / nviews,
+ * glibc cos/sin (the expression the device library's host code uses). */
 void or_fan_tables(const or_fan_geom *g, double *cos_t, double *sin_t);
 void or_fan_endpoints(const or_fan_geom *g, const double *cos_t, const double *sin_t,
                       int view, int bin, double *sx, double *sy, double *dx, double *dy);

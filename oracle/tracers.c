@@ -164,7 +164,8 @@ int or_siddon2d(double sx, double sy, double dx, double dy,
 void or_fan_tables(const or_fan_geom *g, double *cos_t, double *sin_t)
 {
     for (int a = 0; a < g->nviews; ++a) {
-        double th = g->span * (double)a / (double)g->nviews;
+        const double span = g->span_deg * (3.14159265358979323846 / 180.0);
+        double th = span * (double)a / (double)g->nviews;
         cos_t[a] = cos(th);
         sin_t[a] = sin(th);
     }

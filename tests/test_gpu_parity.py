@@ -46,6 +46,13 @@ def test_fan_tracer_bitexact(ctx, case):
     orp, oind, oln = O.fan_csr(O.fan_from(g))
     assert np.array_equal(rp, orp)
     assert np.array_equal(ind, oind)
+    diff = ln != oln
+    if diff.any():
+        k = np.flatnonzero(diff)
+        rows = np.searchsorted(orp, k, side="right") - 1
+        print(f"  {diff.sum()} of {len(ln)} lengths differ in {len(np.unique(rows))} rays; max rel "
+              f"{np.max(np.abs(ln[k] - oln[k]) / np.abs(oln[k])):.3e}; first rays {np.unique(rows)[:8]}",
+              flush=True)
     assert np.array_equal(ln.view(np.uint64), oln.view(np.uint64))
 
 
