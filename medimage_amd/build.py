@@ -27,6 +27,8 @@ FLAGS = [
     "-std=c++17",
     "-fPIC",
     "-ffp-contract=off",
+    # per-view tables call libm sin and cos separately (no sincos merge)
+    "-fno-builtin-sin", "-fno-builtin-cos", "-fno-builtin-sinf", "-fno-builtin-cosf",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall",
     "-Wno-unused-result",

@@ -51,8 +51,10 @@ __device__ __forceinline__ void trace_parallel(float s_view, float c_view, int n
         rtan = 1 / rtan;
         float t = py; py = px; px = t;
     }
-    // Correctly rounded float sqrt == double sqrt rounded to float.
-    rcos = __fsqrt_rn(1 + tanp * tanp);
+    // The reference's double sqrt rounded to float (double sqrt is correctly
+    // rounded on the device; HIP's __fsqrt_rn is not: it carries a 3-ulp
+    // accuracy annotation and lowers to the bare hardware sqrt).
+    rcos = (float)sqrt((double)(1 + tanp * tanp));
     py = (float)((double)py + n / 2.0);
     px = (float)((double)px + n / 2.0);
     c = py - px * tanp;
