@@ -1,0 +1,135 @@
+"""Benchmark of the AT reconstruction loop on one MI355X.
+
+Workload: the precision form (C2): 512 x 512 image, 0.7 mm pixels, 720 views
+over 360 degrees, 736 detector bins of 1.0 mm, flat fan beam (SOD 541 mm,
+SDD 949 mm), 12 ordered subsets, FP32 bulk with Kahan sums and FP64
+reductions.  One step = one AT iteration (a pass over all 12 subsets: forward
+projection, residual, gather back projection + image gradient, line search,
+image update; then one edge step) of one job whose sinogram and state are
+resident in HBM.  Synthetic input: a seeded 10-ellipse phantom with Poisson
+noise at I0 = 2e4.
+
+Prints one JSON line: AT iterations/s (value), rays/s, the roofline of the
+dominant kernel from HIP events recorded on the library's stream during the
+timed region, and the CPU oracle timed on a bounded sample of the same
+workload on this host.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "AT iters/sec & rays/sec (512², 720 views) plain vs batched; p50/p99 job latency"
+
+
+def cpu_baseline(sino, passes):
+    """The oracle (single-threaded C restatement, FP64) on `passes` AT
+    iterations of the same job."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_py as O
+    from medimage_amd import configs
+    g = configs.c2_geometry()
+    op = O.op_fan(O.fan_from(g))
+    t0 = time.perf_counter()
+    O.os_at(op, sino.astype(np.float64), configs.ALPHA, configs.BETA, configs.EPS, 12, passes, iter_edge=1)
+    dt = time.perf_counter() - t0
+    return {"value": passes / dt, "unit": "AT iterations/s", "cores": 1, "kind": "port",
+            "sample": f"{passes} AT iterations of the same 512x512 / 720-view / 736-bin / 12-subset job, "
+                      f"FP64 oracle, one thread, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cpu-passes", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(jobs shard by GPU without any exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import medimage_amd as mi
+    from medimage_amd import configs
+
+    ctx = mi.Context(0)
+    g = configs.c2_geometry()
+    print(f"[bench] tracing C2 geometry and generating the phantom", file=sys.stderr, flush=True)
+    t = time.perf_counter()
+    nnz = ctx.prepare(g, 12)
+    _, sino = configs.make_case(ctx, g, 2e4, seed=2)
+    print(f"[bench] nnz {nnz}, setup {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
+    p = configs.c2_params()
+    job = ctx.job(g, sino.astype(np.float32), p)
+    job.enqueue(args.warmup)
+    job.synchronize()
+
+    ctx.stats_reset()
+    ctx.stats_enable(True)
+    job.synchronize()
+    t0 = time.perf_counter()
+    job.enqueue(args.steps)
+    job.synchronize()
+    t1 = time.perf_counter()
+    ctx.stats_enable(False)
+    elapsed = t1 - t0
+    rep = job.report()
+    print(f"[bench] {args.steps} AT iterations in {elapsed * 1e3:.1f} ms; data fidelity {rep.data_fidelity:.6e}",
+          file=sys.stderr, flush=True)
+
+    names = {mi.MI_K_FP: "fp_residual", mi.MI_K_BP: "bp_gather_grad", mi.MI_K_FPSQ: "fp_direction_norm"}
+    ks = {k: ctx.stats(k) for k in names}
+    dom = max(ks, key=lambda k: ks[k].total_ms)
+    s = ks[dom]
+    avg_ms = s.total_ms / max(s.launches, 1)
+    bytes_per = s.bytes / max(s.launches, 1)
+    achieved = bytes_per / (avg_ms * 1e-3) / 1e9
+    per_kernel = {names[k]: {"launches": ks[k].launches, "avg_us": 1e3 * ks[k].total_ms / max(ks[k].launches, 1),
+                             "GBps": (ks[k].bytes / max(ks[k].launches, 1)) /
+                                     (max(ks[k].total_ms, 1e-9) / max(ks[k].launches, 1) * 1e-3) / 1e9}
+                  for k in names}
+
+    rays = g.n_views * g.n_bins
+    out = {
+        "metric": METRIC,
+        "value": args.steps / elapsed,
+        "unit": "AT iterations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded 10-ellipse phantom, Poisson noise I0=2e4, log transform",
+        "config": {"workload": "C2 plain per-job form: 512x512 fan beam (0.7 mm), 720 views, 736 bins (1.0 mm), "
+                               "SOD 541 mm, SDD 949 mm, 12 ordered subsets, alpha 1e-3, beta 1e-4, eps 1e-2",
+                   "rays": rays, "nnz": nnz, "subsets": 12},
+        "rays_per_s": rays * args.steps / elapsed,
+        "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": bytes_per, "avg_launch_us": avg_ms * 1e3},
+        "kernels": per_kernel,
+    }
+    if not args.no_cpu and args.cpu_passes > 0:
+        print(f"[bench] CPU baseline: {args.cpu_passes} AT iterations of the oracle", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(sino.astype(np.float32), args.cpu_passes)
+    job.close()
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

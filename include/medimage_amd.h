@@ -164,6 +164,25 @@ int mi_at_reconstruct(mi_context *ctx, const mi_geometry *g, const mi_at_params 
 int mi_at_reconstruct_f64(mi_context *ctx, const mi_geometry *g, const mi_at_params *p,
                           const double *sino, double *u, double *v, mi_report *rep);
 
+/* ---- device-resident jobs (ordered-subset form) ----------------------------
+ * A job keeps its sinogram, image u, edge v and scalars in HBM between
+ * calls.  Element type of sino/u0/v0/u/v: float for MI_FP32, double for
+ * MI_FP64 (p->precision).  u0 / v0 may be NULL (u = 0, v = 1, as
+ * origin/GoldenC/src/main.c:36-40 and msbeam/CPU_OpenMP/src/main.c:366-369).
+ * Convergence (p->rel_tol) is tested on the device: once met, the job's
+ * remaining queued passes do no work. */
+typedef struct mi_job mi_job;
+int mi_job_create(mi_context *ctx, const mi_geometry *g, const mi_at_params *p, const void *sino,
+                  const void *u0, const void *v0, mi_job **out);
+/* Runs up to max_passes AT iterations (fewer if it converges); blocking. */
+int mi_job_run(mi_job *job, int max_passes, int *passes_run);
+/* Queues passes on the context's stream and returns without waiting. */
+int mi_job_enqueue(mi_job *job, int passes);
+int mi_job_synchronize(mi_job *job);
+int mi_job_report(mi_job *job, mi_report *rep);
+int mi_job_read(mi_job *job, void *u, void *v);
+int mi_job_destroy(mi_job *job);
+
 /* ---- instrumentation ----------------------------------------------------- */
 enum mi_kernel_id {
     MI_K_FP = 0,          /* forward projection of a subset (residual form)  */

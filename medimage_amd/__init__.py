@@ -122,6 +122,13 @@ def _load():
         "mi_mumford_shah": [P, G, C.POINTER(AtParams), P, P, P, C.POINTER(Report)],
         "mi_at_reconstruct": [P, G, C.POINTER(AtParams), P, P, P, C.POINTER(Report)],
         "mi_at_reconstruct_f64": [P, G, C.POINTER(AtParams), P, P, P, C.POINTER(Report)],
+        "mi_job_create": [P, G, C.POINTER(AtParams), P, P, P, C.POINTER(C.c_void_p)],
+        "mi_job_run": [P, C.c_int, C.POINTER(C.c_int)],
+        "mi_job_enqueue": [P, C.c_int],
+        "mi_job_synchronize": [P],
+        "mi_job_report": [P, C.POINTER(Report)],
+        "mi_job_read": [P, P, P],
+        "mi_job_destroy": [P],
         "mi_stats_enable": [P, C.c_int],
         "mi_stats_get": [P, C.c_int, C.POINTER(KernelStats)],
         "mi_stats_reset": [P],
@@ -271,6 +278,9 @@ class Context:
         _check(fn(self._h, C.byref(g), C.byref(p), _ptr(sino), _ptr(u), _ptr(v), C.byref(rep)))
         return u, v, rep
 
+    def job(self, g: Geometry, sino, p: AtParams, u0=None, v0=None):
+        return Job(self, g, sino, p, u0, v0)
+
     # -- instrumentation --------------------------------------------------
     def stats_enable(self, on=True):
         _check(self._lib.mi_stats_enable(self._h, int(bool(on))))
@@ -282,3 +292,54 @@ class Context:
 
     def stats_reset(self):
         _check(self._lib.mi_stats_reset(self._h))
+
+
+class Job:
+    """Device-resident ordered-subset job (mi_job)."""
+
+    def __init__(self, ctx: Context, g: Geometry, sino, p: AtParams, u0=None, v0=None):
+        self._lib = ctx._lib
+        self.ctx = ctx
+        self.g = g
+        self.dtype = np.float64 if p.precision == MI_FP64 else np.float32
+        sino = _arr(sino, self.dtype, (g.n_views, g.n_bins))
+        u0 = None if u0 is None else _arr(u0, self.dtype, (g.nx, g.ny))
+        v0 = None if v0 is None else _arr(v0, self.dtype, (g.nx, g.ny))
+        h = C.c_void_p()
+        _check(self._lib.mi_job_create(ctx._h, C.byref(g), C.byref(p), _ptr(sino),
+                                       None if u0 is None else _ptr(u0), None if v0 is None else _ptr(v0),
+                                       C.byref(h)))
+        self._h = h
+
+    def run(self, max_passes):
+        n = C.c_int()
+        _check(self._lib.mi_job_run(self._h, max_passes, C.byref(n)))
+        return n.value
+
+    def enqueue(self, passes):
+        _check(self._lib.mi_job_enqueue(self._h, passes))
+
+    def synchronize(self):
+        _check(self._lib.mi_job_synchronize(self._h))
+
+    def report(self) -> Report:
+        r = Report()
+        _check(self._lib.mi_job_report(self._h, C.byref(r)))
+        return r
+
+    def read(self):
+        u = np.empty((self.g.nx, self.g.ny), self.dtype)
+        v = np.empty((self.g.nx, self.g.ny), self.dtype)
+        _check(self._lib.mi_job_read(self._h, _ptr(u), _ptr(v)))
+        return u, v
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.mi_job_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

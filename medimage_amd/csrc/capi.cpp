@@ -98,15 +98,14 @@ GeomKey make_key(const mi_geometry &g, int S) {
     return k;
 }
 
-Matrix &get_matrix(mi_context *ctx, const mi_geometry &g, int S, bool want64) {
+std::shared_ptr<Matrix> get_matrix(mi_context *ctx, const mi_geometry &g, int S, bool want64) {
     const GeomKey key = make_key(g, S);
     auto it = ctx->matrices.find(key);
-    if (it != ctx->matrices.end() && (!want64 || it->second->has64)) return *it->second;
+    if (it != ctx->matrices.end() && (!want64 || it->second->has64)) return it->second;
     if (it != ctx->matrices.end()) ctx->matrices.erase(it);
-    auto m = build_matrix(g, S, want64, ctx->stream);
-    Matrix &ref = *m;
-    ctx->matrices[key] = std::move(m);
-    return ref;
+    std::shared_ptr<Matrix> m(build_matrix(g, S, want64, ctx->stream).release());
+    ctx->matrices[key] = m;
+    return m;
 }
 
 LaunchTimer::LaunchTimer(mi_context *c, int k, double by) : ctx(c), kernel(k), bytes(by) {
@@ -150,6 +149,13 @@ void stats_collect(mi_context *ctx) {
 
 using namespace mi;
 
+struct mi_job {
+    mi_context *ctx = nullptr;
+    int precision = MI_FP32;
+    std::unique_ptr<Job<float>> j32;
+    std::unique_ptr<Job<double>> j64;
+};
+
 namespace {
 
 mi_context *check_ctx(mi_context *ctx) {
@@ -162,7 +168,7 @@ template <typename T>
 void forward_impl(mi_context *ctx, const mi_geometry *g, const T *img, T *sino) {
     MI_REQUIRE(g && img && sino, MI_E_ARG, "null argument");
     const bool f64 = sizeof(T) == 8;
-    Matrix &m = get_matrix(ctx, *g, 1, f64);
+    Matrix &m = *get_matrix(ctx, *g, 1, f64);
     const hipStream_t st = ctx->stream;
     DevBuf<T> x(m.npix), rows(m.nrows), user(m.nrows);
     DevBuf<double> part(2 * kMaxGrid);
@@ -184,7 +190,7 @@ template <typename T>
 void backproject_impl(mi_context *ctx, const mi_geometry *g, const T *sino, T *img) {
     MI_REQUIRE(g && img && sino, MI_E_ARG, "null argument");
     const bool f64 = sizeof(T) == 8;
-    Matrix &m = get_matrix(ctx, *g, 1, f64);
+    Matrix &m = *get_matrix(ctx, *g, 1, f64);
     const hipStream_t st = ctx->stream;
     DevBuf<T> x(m.npix), rows(m.nrows);
     DevBuf<double> part(2 * kMaxGrid);
@@ -271,7 +277,7 @@ int mi_geometry_prepare(mi_context *ctx, const mi_geometry *g, int n_subsets, in
     return guarded([&] {
         check_ctx(ctx);
         MI_REQUIRE(g, MI_E_ARG, "null geometry");
-        Matrix &m = get_matrix(ctx, *g, n_subsets, false);
+        Matrix &m = *get_matrix(ctx, *g, n_subsets, false);
         if (nnz) *nnz = m.nnz;
     });
 }
@@ -282,7 +288,7 @@ int mi_trace_ray(mi_context *ctx, const mi_geometry *g, int view, int bin, int32
         check_ctx(ctx);
         MI_REQUIRE(g && numb && cap >= 0 && (cap == 0 || (ind && len)), MI_E_ARG, "bad arguments");
         MI_REQUIRE(view >= 0 && view < g->n_views && bin >= 0 && bin < g->n_bins, MI_E_ARG, "ray out of range");
-        Matrix &m = get_matrix(ctx, *g, 1, true);
+        Matrix &m = *get_matrix(ctx, *g, 1, true);
         const int64_t r = (int64_t)view * g->n_bins + bin;  // one subset: rows in view order
         int64_t rp[2];
         MI_HIP(hipMemcpy(rp, m.rowptr.p + r, 2 * sizeof(int64_t), hipMemcpyDeviceToHost));
@@ -303,7 +309,7 @@ int mi_matrix_export(mi_context *ctx, const mi_geometry *g, int64_t *rowptr, int
     return guarded([&] {
         check_ctx(ctx);
         MI_REQUIRE(g && rowptr && (nnz == 0 || (ind && len)), MI_E_ARG, "null argument");
-        Matrix &m = get_matrix(ctx, *g, 1, true);
+        Matrix &m = *get_matrix(ctx, *g, 1, true);
         MI_REQUIRE(nnz == m.nnz, MI_E_ARG, "nnz does not match the geometry's matrix");
         MI_HIP(hipMemcpy(rowptr, m.rowptr.p, (m.nrows + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
         if (nnz > 0) {
@@ -332,7 +338,7 @@ int mi_image_step_f64(mi_context *ctx, const mi_geometry *g, double *f_out, cons
     return guarded([&] {
         check_ctx(ctx);
         MI_REQUIRE(g && f_out && sino && f && v && iter_image >= 1, MI_E_ARG, "bad arguments");
-        Matrix &m = get_matrix(ctx, *g, 1, true);
+        Matrix &m = *get_matrix(ctx, *g, 1, true);
         JobState<double> js(m);
         const hipStream_t st = ctx->stream;
         load_sino<double>(st, m, sino, js.g.p);
@@ -385,7 +391,7 @@ int mi_mumford_shah(mi_context *ctx, const mi_geometry *g, const mi_at_params *p
         check_ctx(ctx);
         MI_REQUIRE(g && p && sino && f && v, MI_E_ARG, "null argument");
         MI_REQUIRE(p->iter_alt >= 0 && p->iter_image >= 1 && p->iter_edge >= 1, MI_E_ARG, "bad iteration counts");
-        Matrix &m = get_matrix(ctx, *g, 1, true);
+        Matrix &m = *get_matrix(ctx, *g, 1, true);
         JobState<double> js(m);
         const hipStream_t st = ctx->stream;
         load_sino<double>(st, m, sino, js.g.p);
@@ -432,6 +438,83 @@ int mi_at_reconstruct_f64(mi_context *ctx, const mi_geometry *g, const mi_at_par
         check_ctx(ctx);
         MI_REQUIRE(g && p && sino && u && v, MI_E_ARG, "null argument");
         os_solve<double>(ctx, *g, *p, sino, u, v, rep);
+    });
+}
+
+int mi_job_create(mi_context *ctx, const mi_geometry *g, const mi_at_params *p, const void *sino, const void *u0,
+                  const void *v0, mi_job **out) {
+    return guarded([&] {
+        check_ctx(ctx);
+        MI_REQUIRE(g && p && sino && out, MI_E_ARG, "null argument");
+        MI_REQUIRE(p->n_subsets >= 1 && p->iter_edge >= 0, MI_E_ARG, "bad AT parameters");
+        MI_REQUIRE(p->precision == MI_FP32 || p->precision == MI_FP64, MI_E_ARG, "bad precision");
+        *out = nullptr;
+        auto j = std::make_unique<mi_job>();
+        j->ctx = ctx;
+        j->precision = p->precision;
+        if (p->precision == MI_FP64) {
+            j->j64 = std::make_unique<Job<double>>(ctx, get_matrix(ctx, *g, p->n_subsets, true), *p);
+            j->j64->load((const double *)sino, (const double *)u0, (const double *)v0);
+        } else {
+            j->j32 = std::make_unique<Job<float>>(ctx, get_matrix(ctx, *g, p->n_subsets, false), *p);
+            j->j32->load((const float *)sino, (const float *)u0, (const float *)v0);
+        }
+        *out = j.release();
+    });
+}
+
+int mi_job_run(mi_job *job, int max_passes, int *passes_run) {
+    return guarded([&] {
+        MI_REQUIRE(job && max_passes >= 0, MI_E_ARG, "bad arguments");
+        check_ctx(job->ctx);
+        const int every = (job->j32 ? job->j32->p.rel_tol : job->j64->p.rel_tol) > 0 ? 4 : 0;
+        int n = job->j32 ? job->j32->run(max_passes, every) : job->j64->run(max_passes, every);
+        if (passes_run) *passes_run = n;
+    });
+}
+
+int mi_job_enqueue(mi_job *job, int passes) {
+    return guarded([&] {
+        MI_REQUIRE(job && passes >= 0, MI_E_ARG, "bad arguments");
+        check_ctx(job->ctx);
+        if (job->j32) job->j32->enqueue(passes);
+        else job->j64->enqueue(passes);
+    });
+}
+
+int mi_job_synchronize(mi_job *job) {
+    return guarded([&] {
+        MI_REQUIRE(job, MI_E_ARG, "null job");
+        check_ctx(job->ctx);
+        MI_HIP(hipStreamSynchronize(job->ctx->stream));
+        stats_collect(job->ctx);
+    });
+}
+
+int mi_job_report(mi_job *job, mi_report *rep) {
+    return guarded([&] {
+        MI_REQUIRE(job && rep, MI_E_ARG, "null argument");
+        check_ctx(job->ctx);
+        if (job->j32) job->j32->report(rep);
+        else job->j64->report(rep);
+    });
+}
+
+int mi_job_read(mi_job *job, void *u, void *v) {
+    return guarded([&] {
+        MI_REQUIRE(job, MI_E_ARG, "null job");
+        check_ctx(job->ctx);
+        if (job->j32) job->j32->read((float *)u, (float *)v);
+        else job->j64->read((double *)u, (double *)v);
+    });
+}
+
+int mi_job_destroy(mi_job *job) {
+    return guarded([&] {
+        if (!job) return;
+        (void)hipSetDevice(job->ctx->device);
+        (void)hipStreamSynchronize(job->ctx->stream);
+        delete job;
     });
 }
 

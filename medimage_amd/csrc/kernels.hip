@@ -73,8 +73,9 @@ __global__ __launch_bounds__(kBlock) void k_fp(RedBuf rb, const int64_t *__restr
                                                const uint32_t *__restrict__ idx, const T *__restrict__ len,
                                                int64_t row0, int64_t nrows, const T *__restrict__ x,
                                                const T *__restrict__ g, const T *__restrict__ q,
-                                               T *__restrict__ out, Slot slot) {
+                                               T *__restrict__ out, Slot slot, const int32_t *skip) {
     __shared__ double lds[kBlock / kWave];
+    if (skip && *skip) return;
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t wid = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
     const int64_t nw = (int64_t)gridDim.x * (kBlock / kWave);
@@ -138,8 +139,10 @@ __global__ __launch_bounds__(kBlock) void k_bp_grad(RedBuf rb, const int64_t *__
                                                     const uint32_t *__restrict__ crow, const T *__restrict__ clen,
                                                     int nx, int ny, const T *__restrict__ r, double scale,
                                                     const T *__restrict__ u, const T *__restrict__ v, double alpha,
-                                                    double eps, T *__restrict__ d, Slot slot) {
+                                                    double eps, T *__restrict__ d, Slot slot,
+                                                    const int32_t *skip) {
     __shared__ double lds[kBlock / kWave];
+    if (skip && *skip) return;
     constexpr int G = kBlock / LPC;
     const int gl = threadIdx.x % LPC, grp = threadIdx.x / LPC;
     const int64_t npix = (int64_t)nx * ny;
@@ -171,8 +174,9 @@ __global__ __launch_bounds__(kBlock) void k_bp_grad(RedBuf rb, const int64_t *__
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_nv(RedBuf rb, int nx, int ny, const T *__restrict__ p,
                                                const T *__restrict__ v, double eps, Slot slot, Scalars *sc, double S,
-                                               double alpha) {
+                                               double alpha, const int32_t *skip) {
     __shared__ double lds[kBlock / kWave];
+    if (skip && *skip) return;
     const int64_t npix = (int64_t)nx * ny;
     double mine = 0.0;
     const double e2 = 0.01 * (eps * eps);
@@ -225,8 +229,9 @@ __global__ __launch_bounds__(kBlock) void k_dot(RedBuf rb, int64_t n, const T *_
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_axpy(RedBuf rb, int64_t n, T *__restrict__ y, const T *__restrict__ x,
                                                  const double *cptr, double cval, double lo, double hi,
-                                                 T *__restrict__ save, Slot slot) {
+                                                 T *__restrict__ save, Slot slot, const int32_t *skip) {
     __shared__ double lds[kBlock / kWave];
+    if (skip && *skip) return;
     const double c = cptr ? *cptr : cval;
     double mx = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
@@ -286,7 +291,7 @@ __global__ __launch_bounds__(kBlock) void k_edge_grad(RedBuf rb, int nx, int ny,
                                                       const T *__restrict__ v, T *__restrict__ d, double alpha,
                                                       double beta, double eps, Scalars *sc) {
     __shared__ double lds[kBlock / kWave];
-    if (sc->edge_done) return;
+    if (sc->edge_done || sc->converged) return;
     const int64_t npix = (int64_t)nx * ny;
     double m0 = 0.0, m1 = 0.0;
     for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < npix; q += (int64_t)gridDim.x * kBlock) {
@@ -316,7 +321,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_edge_nabla(RedBuf rb, int nx, int ny, const T *__restrict__ d,
                                                        double alpha, double beta, double eps, Scalars *sc) {
     __shared__ double lds[kBlock / kWave];
-    if (sc->edge_done) return;
+    if (sc->edge_done || sc->converged) return;
     const int64_t npix = (int64_t)nx * ny;
     double mine = 0.0;
     for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < npix; q += (int64_t)gridDim.x * kBlock) {
@@ -336,7 +341,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_edge_update(RedBuf rb, int nx, int ny, T *__restrict__ v,
                                                         const T *__restrict__ d, Scalars *sc) {
     __shared__ double lds[kBlock / kWave];
-    if (sc->edge_done) return;
+    if (sc->edge_done || sc->converged) return;
     const int64_t npix = (int64_t)nx * ny;
     const double c = sc->e_c;
     double mx = 0.0;
@@ -357,11 +362,13 @@ __global__ __launch_bounds__(kBlock) void k_edge_update(RedBuf rb, int nx, int n
 }
 
 __global__ void k_pass_begin(Scalars *sc) {
+    if (sc->converged) return;
     sc->df_pass = 0.0;
     sc->edge_done = 0;
 }
 
 __global__ void k_pass_end(Scalars *sc, double rel_tol) {
+    if (sc->converged) return;
     const double df = sc->df_pass;
     if (sc->iters > 0) {
         const double diff = fabs(df - sc->df_prev);
@@ -393,11 +400,11 @@ __global__ void k_scatter_rows(int64_t n, const int64_t *__restrict__ perm, cons
 
 template <typename T>
 void launch_fp(hipStream_t st, const RedBuf &rb, int mode, const CsrView &A, int64_t row0, int64_t nrows,
-               const T *x, const T *g, const T *q, T *out, Slot slot) {
+               const T *x, const T *g, const T *q, T *out, Slot slot, const int32_t *skip) {
     const unsigned grid = cap_grid((nrows + 3) / 4);
     const T *len = (const T *)A.len;
 #define MI_FP_CASE(M) \
-    case M: k_fp<T, M><<<grid, kBlock, 0, st>>>(rb, A.rowptr, A.idx, len, row0, nrows, x, g, q, out, slot); break;
+    case M: k_fp<T, M><<<grid, kBlock, 0, st>>>(rb, A.rowptr, A.idx, len, row0, nrows, x, g, q, out, slot, skip); break;
     switch (mode) {
         MI_FP_CASE(FP_WRITE)
         MI_FP_CASE(FP_RESID)
@@ -412,14 +419,15 @@ void launch_fp(hipStream_t st, const RedBuf &rb, int mode, const CsrView &A, int
 
 template <typename T>
 void launch_bp_grad(hipStream_t st, const RedBuf &rb, const CscView &At, int nx, int ny, const T *r, double scale,
-                    const T *u, const T *v, double alpha, double eps, T *d, Slot slot, int lanes) {
+                    const T *u, const T *v, double alpha, double eps, T *d, Slot slot, int lanes,
+                    const int32_t *skip) {
     const int64_t npix = (int64_t)nx * ny;
     const T *len = (const T *)At.len;
 #define MI_BP_CASE(L)                                                                                          \
     case L: {                                                                                                  \
         const unsigned grid = cap_grid((npix + kBlock / L - 1) / (kBlock / L));                                \
         k_bp_grad<T, L><<<grid, kBlock, 0, st>>>(rb, At.colptr, At.crow, len, nx, ny, r, scale, u, v, alpha,   \
-                                                 eps, d, slot);                                                \
+                                                 eps, d, slot, skip);                                          \
         break;                                                                                                 \
     }
     switch (lanes) {
@@ -436,9 +444,10 @@ void launch_bp_grad(hipStream_t st, const RedBuf &rb, const CscView &At, int nx,
 
 template <typename T>
 void launch_nv(hipStream_t st, const RedBuf &rb, int nx, int ny, const T *p, const T *v, double eps, Slot slot,
-               Scalars *sc, double S, double alpha) {
+               Scalars *sc, double S, double alpha, const int32_t *skip) {
     const int64_t npix = (int64_t)nx * ny;
-    k_nv<T><<<cap_grid((npix + kBlock - 1) / kBlock), kBlock, 0, st>>>(rb, nx, ny, p, v, eps, slot, sc, S, alpha);
+    k_nv<T><<<cap_grid((npix + kBlock - 1) / kBlock), kBlock, 0, st>>>(rb, nx, ny, p, v, eps, slot, sc, S, alpha,
+                                                                       skip);
     MI_HIP(hipGetLastError());
 }
 
@@ -458,8 +467,9 @@ void launch_dot(hipStream_t st, const RedBuf &rb, int64_t n, const T *a, const T
 
 template <typename T>
 void launch_axpy(hipStream_t st, const RedBuf &rb, int64_t n, T *y, const T *x, const double *cptr, double cval,
-                 double lo, double hi, T *save, Slot slot) {
-    k_axpy<T><<<cap_grid((n + kBlock - 1) / kBlock), kBlock, 0, st>>>(rb, n, y, x, cptr, cval, lo, hi, save, slot);
+                 double lo, double hi, T *save, Slot slot, const int32_t *skip) {
+    k_axpy<T><<<cap_grid((n + kBlock - 1) / kBlock), kBlock, 0, st>>>(rb, n, y, x, cptr, cval, lo, hi, save, slot,
+                                                                      skip);
     MI_HIP(hipGetLastError());
 }
 
@@ -513,16 +523,16 @@ void launch_scatter_rows(hipStream_t st, int64_t n, const int64_t *perm, const T
 
 #define MI_INST(T)                                                                                              \
     template void launch_fp<T>(hipStream_t, const RedBuf &, int, const CsrView &, int64_t, int64_t, const T *,  \
-                               const T *, const T *, T *, Slot);                                                \
+                               const T *, const T *, T *, Slot, const int32_t *);                               \
     template void launch_bp_grad<T>(hipStream_t, const RedBuf &, const CscView &, int, int, const T *, double,  \
-                                    const T *, const T *, double, double, T *, Slot, int);                      \
+                                    const T *, const T *, double, double, T *, Slot, int, const int32_t *);     \
     template void launch_nv<T>(hipStream_t, const RedBuf &, int, int, const T *, const T *, double, Slot,       \
-                               Scalars *, double, double);                                                      \
+                               Scalars *, double, double, const int32_t *);                                     \
     template void launch_energy_reg<T>(hipStream_t, const RedBuf &, int, int, const T *, const T *, double,     \
                                        double, Slot);                                                           \
     template void launch_dot<T>(hipStream_t, const RedBuf &, int64_t, const T *, const T *, Slot);              \
     template void launch_axpy<T>(hipStream_t, const RedBuf &, int64_t, T *, const T *, const double *, double,  \
-                                 double, double, T *, Slot);                                                    \
+                                 double, double, T *, Slot, const int32_t *);                                   \
     template void launch_xpby<T>(hipStream_t, int64_t, T *, const T *, double);                                 \
     template void launch_median<T>(hipStream_t, int, int, const T *, T *);                                      \
     template void launch_edge_iter<T>(hipStream_t, const RedBuf &, int, int, const T *, T *, T *, double,       \

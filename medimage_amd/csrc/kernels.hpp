@@ -40,25 +40,30 @@ struct CscView {
     const void *len;
 };
 
+// Every launcher takes an optional device flag `skip`: when *skip != 0 the
+// kernel returns at once (a converged job's remaining passes cost one launch
+// each and no memory traffic).
+//
 // Forward projection of rows [row0, row0 + nrows).  x: image; g/q/out are
-// indexed by row.  Returns the grid size used (for byte accounting).
+// indexed by row.
 template <typename T>
 void launch_fp(hipStream_t st, const RedBuf &rb, int mode, const CsrView &A, int64_t row0, int64_t nrows,
-               const T *x, const T *g, const T *q, T *out, Slot slot);
+               const T *x, const T *g, const T *q, T *out, Slot slot, const int32_t *skip = nullptr);
 
 // d = scale * (A_s^T r) + alpha * R(u, v) over an nx x ny image (R is the
 // GoldenC image-gradient regulariser; skipped when u == nullptr).  Reduces
 // sum d^2 into slot.  lanes: lanes per pixel column (8, 16, 32 or 64).
 template <typename T>
 void launch_bp_grad(hipStream_t st, const RedBuf &rb, const CscView &At, int nx, int ny, const T *r, double scale,
-                    const T *u, const T *v, double alpha, double eps, T *d, Slot slot, int lanes);
+                    const T *u, const T *v, double alpha, double eps, T *d, Slot slot, int lanes,
+                    const int32_t *skip = nullptr);
 
 // sum (v^2 + 0.01 eps^2)((p - p_up)^2 + (p - p_left)^2), p zero padded, v
 // one padded (origin/GoldenC/src/main.c:152-158).  When sc != nullptr the
 // last block also sets sc->c = sc->dd / (S * sc->ad2 + alpha * nv).
 template <typename T>
 void launch_nv(hipStream_t st, const RedBuf &rb, int nx, int ny, const T *p, const T *v, double eps, Slot slot,
-               Scalars *sc, double S, double alpha);
+               Scalars *sc, double S, double alpha, const int32_t *skip = nullptr);
 
 // Interior energy sum of origin/GoldenC/src/main.c:135-139.
 template <typename T>
@@ -73,7 +78,7 @@ void launch_dot(hipStream_t st, const RedBuf &rb, int64_t n, const T *a, const T
 // save (optional) receives the old y; reduces max |c x| into slot.
 template <typename T>
 void launch_axpy(hipStream_t st, const RedBuf &rb, int64_t n, T *y, const T *x, const double *cptr, double cval,
-                 double lo, double hi, T *save, Slot slot);
+                 double lo, double hi, T *save, Slot slot, const int32_t *skip = nullptr);
 
 // y = x + c y (c on the host).
 template <typename T>

@@ -91,13 +91,14 @@ struct Stats {
 struct mi_context {
     int device = 0;
     hipStream_t stream = nullptr;
-    std::map<mi::GeomKey, std::unique_ptr<mi::Matrix>> matrices;
+    std::map<mi::GeomKey, std::shared_ptr<mi::Matrix>> matrices;
     mi::Stats stats;
 };
 
 namespace mi {
-// Matrix for (g, S) from the context cache, building it if needed.
-Matrix &get_matrix(mi_context *ctx, const mi_geometry &g, int S, bool want64);
+// Matrix for (g, S) from the context cache, building it if needed.  Jobs
+// hold the shared pointer, so trimming the cache never frees a matrix in use.
+std::shared_ptr<Matrix> get_matrix(mi_context *ctx, const mi_geometry &g, int S, bool want64);
 // Stats helpers: bracket a launch with events when enabled.
 struct LaunchTimer {
     mi_context *ctx;

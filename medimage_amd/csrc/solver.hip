@@ -11,16 +11,6 @@ namespace mi {
 
 namespace {
 
-template <typename T> struct LenSel;
-template <> struct LenSel<float> {
-    static const void *csr(const Matrix &m) { return m.len32.p; }
-    static const void *csc(const Matrix &m) { return m.clen32.p; }
-};
-template <> struct LenSel<double> {
-    static const void *csr(const Matrix &m) { return m.len64.p; }
-    static const void *csc(const Matrix &m) { return m.clen64.p; }
-};
-
 __global__ void k_fill_f32(float *p, int64_t n, float v) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -114,12 +104,15 @@ Scalars JobState<T>::read_scalars(hipStream_t st) const {
 }
 
 // One ordered-subset pass (everything on the stream, no host round trip).
+// Every kernel of the pass reads sc->converged and returns at once when it is
+// set, so passes can be queued ahead of the convergence test.
 template <typename T>
 void os_pass(mi_context *ctx, const Matrix &m, JobState<T> &js, const mi_at_params &p, int pass_index) {
     const hipStream_t st = ctx->stream;
     const bool f64 = sizeof(T) == 8;
     const RedBuf rb = js.rb();
     Scalars *sc = js.sc.p;
+    const int32_t *skip = &sc->converged;
     const CsrView A = csr_view(m, f64);
     const int lanes = bp_lanes(m);
     const double S = (double)m.S;
@@ -128,21 +121,22 @@ void os_pass(mi_context *ctx, const Matrix &m, JobState<T> &js, const mi_at_para
         const int64_t row0 = m.sub_row0[s], nr = m.sub_row0[s + 1] - row0;
         {
             LaunchTimer lt(ctx, MI_K_FP, fp_bytes(m, nr, sizeof(T), FP_RESID));
-            launch_fp<T>(st, rb, FP_RESID, A, row0, nr, js.u.p, js.g.p, nullptr, js.r.p, Slot{&sc->df_pass, 1});
+            launch_fp<T>(st, rb, FP_RESID, A, row0, nr, js.u.p, js.g.p, nullptr, js.r.p, Slot{&sc->df_pass, 1},
+                         skip);
         }
         {
             LaunchTimer lt(ctx, MI_K_BP, bp_bytes(m, nr, sizeof(T), true));
             launch_bp_grad<T>(st, rb, csc_view(m, s, f64), js.nx, js.ny, js.r.p, S, js.u.p, js.v.p, p.alpha,
-                              p.epsilon, js.d.p, Slot{&sc->dd, 0}, lanes);
+                              p.epsilon, js.d.p, Slot{&sc->dd, 0}, lanes, skip);
         }
         {
             LaunchTimer lt(ctx, MI_K_FPSQ, fp_bytes(m, nr, sizeof(T), FP_SQ));
-            launch_fp<T>(st, rb, FP_SQ, A, row0, nr, js.d.p, nullptr, nullptr, nullptr, Slot{&sc->ad2, 0});
+            launch_fp<T>(st, rb, FP_SQ, A, row0, nr, js.d.p, nullptr, nullptr, nullptr, Slot{&sc->ad2, 0}, skip);
         }
         {
             LaunchTimer lt(ctx, MI_K_STEP, (double)js.npix * sizeof(T) * 5);
-            launch_nv<T>(st, rb, js.nx, js.ny, js.d.p, js.v.p, p.epsilon, Slot{&sc->nv, 0}, sc, S, p.alpha);
-            launch_axpy<T>(st, rb, js.npix, js.u.p, js.d.p, &sc->c, 0.0, p.lower, p.upper, nullptr, Slot{});
+            launch_nv<T>(st, rb, js.nx, js.ny, js.d.p, js.v.p, p.epsilon, Slot{&sc->nv, 0}, sc, S, p.alpha, skip);
+            launch_axpy<T>(st, rb, js.npix, js.u.p, js.d.p, &sc->c, 0.0, p.lower, p.upper, nullptr, Slot{}, skip);
         }
     }
     if (p.median_every > 0 && pass_index % p.median_every == 0) {
@@ -157,45 +151,91 @@ void os_pass(mi_context *ctx, const Matrix &m, JobState<T> &js, const mi_at_para
     launch_pass_end(st, sc, p.rel_tol);
 }
 
+// ---------------------------------------------------------------------------
+// Device-resident job of the ordered-subset form.
+
+template <typename T>
+Job<T>::Job(mi_context *c, std::shared_ptr<Matrix> mat, const mi_at_params &prm)
+    : ctx(c), m(std::move(mat)), p(prm), js(*m) {}
+
+template <typename T>
+void Job<T>::load(const T *sino, const T *u0, const T *v0) {
+    const hipStream_t st = ctx->stream;
+    load_sino<T>(st, *m, sino, js.g.p);
+    if (u0) MI_HIP(hipMemcpyAsync(js.u.p, u0, js.npix * sizeof(T), hipMemcpyHostToDevice, st));
+    else fill<T>(st, js.u.p, js.npix, T(0));
+    if (v0) MI_HIP(hipMemcpyAsync(js.v.p, v0, js.npix * sizeof(T), hipMemcpyHostToDevice, st));
+    else fill<T>(st, js.v.p, js.npix, T(1));
+    MI_HIP(hipMemsetAsync(js.sc.p, 0, sizeof(Scalars), st));
+    MI_HIP(hipStreamSynchronize(st));
+    passes = 0;
+}
+
+template <typename T>
+int Job<T>::run(int max_passes, int check_every) {
+    const hipStream_t st = ctx->stream;
+    int queued = 0;
+    Scalars h{};
+    while (queued < max_passes) {
+        const int chunk = std::min(check_every > 0 ? check_every : max_passes, max_passes - queued);
+        for (int k = 0; k < chunk; ++k) os_pass<T>(ctx, *m, js, p, passes + queued + k + 1);
+        queued += chunk;
+        h = js.read_scalars(st);
+        if (h.converged) break;
+    }
+    const int ran = h.iters - passes;
+    passes = h.iters;
+    stats_collect(ctx);
+    return ran;
+}
+
+template <typename T>
+void Job<T>::enqueue(int n) {
+    for (int k = 0; k < n; ++k) os_pass<T>(ctx, *m, js, p, passes + k + 1);
+    passes += n;
+}
+
+template <typename T>
+void Job<T>::report(mi_report *rep) {
+    Scalars h = js.read_scalars(ctx->stream);
+    passes = h.iters;
+    rep->iterations = h.iters;
+    rep->converged = h.converged;
+    rep->data_fidelity = h.df_prev;
+    rep->rel_change = h.rel;
+    rep->elapsed_ms = 0;
+    rep->nnz = m->nnz;
+}
+
+template <typename T>
+void Job<T>::read(T *u, T *v) {
+    const hipStream_t st = ctx->stream;
+    if (u) MI_HIP(hipMemcpyAsync(u, js.u.p, js.npix * sizeof(T), hipMemcpyDeviceToHost, st));
+    if (v) MI_HIP(hipMemcpyAsync(v, js.v.p, js.npix * sizeof(T), hipMemcpyDeviceToHost, st));
+    MI_HIP(hipStreamSynchronize(st));
+}
+
 template <typename T>
 void os_solve(mi_context *ctx, const mi_geometry &g, const mi_at_params &p, const T *sino, T *u, T *v,
               mi_report *rep) {
     MI_REQUIRE(p.n_subsets >= 1 && p.max_iters >= 0 && p.iter_edge >= 0, MI_E_ARG, "bad AT parameters");
-    const bool f64 = sizeof(T) == 8;
-    Matrix &m = get_matrix(ctx, g, p.n_subsets, f64);
-    const hipStream_t st = ctx->stream;
-    JobState<T> js(m);
-    load_sino<T>(st, m, sino, js.g.p);
-    MI_HIP(hipMemcpyAsync(js.u.p, u, js.npix * sizeof(T), hipMemcpyHostToDevice, st));
-    MI_HIP(hipMemcpyAsync(js.v.p, v, js.npix * sizeof(T), hipMemcpyHostToDevice, st));
-    MI_HIP(hipStreamSynchronize(st));
+    Job<T> job(ctx, get_matrix(ctx, g, p.n_subsets, sizeof(T) == 8), p);
+    job.load(sino, u, v);
     hipEvent_t e0, e1;
     MI_HIP(hipEventCreate(&e0));
     MI_HIP(hipEventCreate(&e1));
-    MI_HIP(hipEventRecord(e0, st));
-    Scalars h{};
-    for (int it = 1; it <= p.max_iters; ++it) {
-        os_pass<T>(ctx, m, js, p, it);
-        h = js.read_scalars(st);
-        if (h.converged) break;
-    }
-    MI_HIP(hipEventRecord(e1, st));
+    MI_HIP(hipEventRecord(e0, ctx->stream));
+    job.run(p.max_iters, p.rel_tol > 0 ? 4 : 0);
+    MI_HIP(hipEventRecord(e1, ctx->stream));
     MI_HIP(hipEventSynchronize(e1));
     float ms = 0.f;
     MI_HIP(hipEventElapsedTime(&ms, e0, e1));
-    MI_HIP(hipEventDestroy(e0));
-    MI_HIP(hipEventDestroy(e1));
-    stats_collect(ctx);
-    MI_HIP(hipMemcpyAsync(u, js.u.p, js.npix * sizeof(T), hipMemcpyDeviceToHost, st));
-    MI_HIP(hipMemcpyAsync(v, js.v.p, js.npix * sizeof(T), hipMemcpyDeviceToHost, st));
-    MI_HIP(hipStreamSynchronize(st));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    job.read(u, v);
     if (rep) {
-        rep->iterations = h.iters;
-        rep->converged = h.converged;
-        rep->data_fidelity = h.df_prev;
-        rep->rel_change = h.rel;
+        job.report(rep);
         rep->elapsed_ms = ms;
-        rep->nnz = m.nnz;
     }
 }
 
@@ -292,6 +332,8 @@ template void load_sino<double>(hipStream_t, const Matrix &, const double *, dou
 
 template struct JobState<float>;
 template struct JobState<double>;
+template struct Job<float>;
+template struct Job<double>;
 template void os_solve<float>(mi_context *, const mi_geometry &, const mi_at_params &, const float *, float *,
                               float *, mi_report *);
 template void os_solve<double>(mi_context *, const mi_geometry &, const mi_at_params &, const double *, double *,
