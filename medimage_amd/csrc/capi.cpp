@@ -67,6 +67,7 @@ template struct DevBuf<int64_t>;
 template struct DevBuf<uint32_t>;
 template struct DevBuf<unsigned char>;
 template struct DevBuf<Scalars>;
+template struct DevBuf<unsigned long long>;
 
 void validate_geometry(const mi_geometry &g) {
     MI_REQUIRE(g.nx > 0 && g.ny > 0 && g.n_views > 0 && g.n_bins > 0, MI_E_ARG, "geometry sizes must be positive");
@@ -171,14 +172,10 @@ void forward_impl(mi_context *ctx, const mi_geometry *g, const T *img, T *sino) 
     Matrix &m = *get_matrix(ctx, *g, 1, f64);
     const hipStream_t st = ctx->stream;
     DevBuf<T> x(m.npix), rows(m.nrows), user(m.nrows);
-    DevBuf<double> part(2 * kMaxGrid);
-    DevBuf<unsigned> tick(1);
-    MI_HIP(hipMemsetAsync(tick.p, 0, sizeof(unsigned), st));
     MI_HIP(hipMemcpyAsync(x.p, img, m.npix * sizeof(T), hipMemcpyHostToDevice, st));
     {
         LaunchTimer lt(ctx, MI_K_FP, fp_bytes(m, m.nrows, sizeof(T), FP_WRITE));
-        launch_fp<T>(st, RedBuf{part.p, tick.p}, FP_WRITE, csr_view(m, f64), 0, m.nrows, x.p, nullptr, nullptr,
-                     rows.p, Slot{});
+        launch_fp<T>(st, FP_WRITE, csr_view(m, f64), 0, m.nrows, x.p, nullptr, nullptr, rows.p, nullptr);
     }
     launch_scatter_rows<T>(st, m.nrows, m.perm.p, rows.p, user.p);
     MI_HIP(hipMemcpyAsync(sino, user.p, m.nrows * sizeof(T), hipMemcpyDeviceToHost, st));
@@ -193,14 +190,11 @@ void backproject_impl(mi_context *ctx, const mi_geometry *g, const T *sino, T *i
     Matrix &m = *get_matrix(ctx, *g, 1, f64);
     const hipStream_t st = ctx->stream;
     DevBuf<T> x(m.npix), rows(m.nrows);
-    DevBuf<double> part(2 * kMaxGrid);
-    DevBuf<unsigned> tick(1);
-    MI_HIP(hipMemsetAsync(tick.p, 0, sizeof(unsigned), st));
     load_sino<T>(st, m, sino, rows.p);
     {
         LaunchTimer lt(ctx, MI_K_BP, bp_bytes(m, m.nrows, sizeof(T), false));
-        launch_bp_grad<T>(st, RedBuf{part.p, tick.p}, csc_view(m, 0, f64), g->nx, g->ny, rows.p, 1.0, nullptr,
-                          nullptr, 0.0, 0.0, x.p, Slot{}, bp_lanes(m));
+        launch_bp_grad<T>(st, csc_view(m, 0, f64), g->nx, g->ny, rows.p, 1.0, nullptr, nullptr, 0.0, 0.0, x.p,
+                          nullptr, bp_lanes(m));
     }
     MI_HIP(hipMemcpyAsync(img, x.p, m.npix * sizeof(T), hipMemcpyDeviceToHost, st));
     MI_HIP(hipStreamSynchronize(st));

@@ -89,42 +89,6 @@ __device__ __forceinline__ double block_max(double x, double *lds) {
     return r;
 }
 
-// ---------------------------------------------------------------------------
-// Grid reductions.  Every block writes its partial; the last block to arrive
-// (integer ticket) sums the partials in index order and runs `fin` on the
-// total.  The ticket is reset by that block, so a later launch (or a graph
-// replay) starts from zero.
-struct GridRed {
-    double *partial;     // >= gridDim.x entries
-    unsigned *ticket;    // one counter, zero between launches
-};
-
-template <typename Fin>
-__device__ __forceinline__ void grid_finish(const GridRed &g, double mine, bool is_max, double *lds, Fin fin) {
-    __shared__ bool last;
-    if (threadIdx.x == 0) {
-        g.partial[blockIdx.x] = mine;
-        __threadfence();
-        unsigned t = atomicAdd(g.ticket, 1u);
-        last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    double acc = 0.0;
-    // Fixed order: thread t sums partials t, t+kBlock, ...; then block_sum.
-    const volatile double *part = g.partial;
-    for (unsigned i = threadIdx.x; i < gridDim.x; i += kBlock) {
-        double p = part[i];
-        acc = is_max ? fmax(acc, p) : acc + p;
-    }
-    double tot = is_max ? block_max(acc, lds) : block_sum(acc, lds);
-    if (threadIdx.x == 0) {
-        fin(tot);
-        *g.ticket = 0u;
-    }
-}
-
 // Device-resident scalars of one job's solve.  Kernels read step sizes from
 // here and write reduced sums here, so a pass needs no host round trip.
 struct Scalars {

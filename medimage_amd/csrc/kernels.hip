@@ -17,48 +17,17 @@ inline unsigned cap_grid(int64_t blocks) {
     return (unsigned)(blocks < kMaxGrid ? blocks : kMaxGrid);
 }
 
-__device__ __forceinline__ void slot_put(Slot s, double v) {
-    if (!s.dst) return;
-    if (s.accumulate) *s.dst += v;
-    else *s.dst = v;
+// Block partial: thread 0 of the block stores the block's sum (or max) at
+// part[blockIdx.x].  No cross-block traffic inside the kernel; a one-block
+// finalize kernel sums the partials in index order after the kernel boundary.
+__device__ __forceinline__ void put_partial(double *part, double mine, double *lds) {
+    const double b = block_sum(mine, lds);
+    if (part && threadIdx.x == 0) part[blockIdx.x] = b;
 }
 
-// Block partial in thread 0 -> grid total -> slot (+ optional extra action).
-template <typename Fin>
-__device__ __forceinline__ void reduce_to(const RedBuf &rb, double mine, double *lds, Fin fin) {
-    double b = block_sum(mine, lds);
-    GridRed g{rb.partial, rb.ticket};
-    grid_finish(g, b, false, lds, fin);
-}
-
-// Two sums at once (partials interleaved).
-template <typename Fin>
-__device__ __forceinline__ void reduce2_to(const RedBuf &rb, double m0, double m1, double *lds, Fin fin) {
-    double b0 = block_sum(m0, lds);
-    double b1 = block_sum(m1, lds);
-    __shared__ bool last;
-    if (threadIdx.x == 0) {
-        rb.partial[2 * blockIdx.x] = b0;
-        rb.partial[2 * blockIdx.x + 1] = b1;
-        __threadfence();
-        unsigned t = atomicAdd(rb.ticket, 1u);
-        last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    const volatile double *part = rb.partial;
-    double a0 = 0.0, a1 = 0.0;
-    for (unsigned i = threadIdx.x; i < gridDim.x; i += kBlock) {
-        a0 += part[2 * i];
-        a1 += part[2 * i + 1];
-    }
-    double t0 = block_sum(a0, lds);
-    double t1 = block_sum(a1, lds);
-    if (threadIdx.x == 0) {
-        fin(t0, t1);
-        *rb.ticket = 0u;
-    }
+__device__ __forceinline__ void put_partial_max(double *part, double mine, double *lds) {
+    const double b = block_max(mine, lds);
+    if (part && threadIdx.x == 0) part[blockIdx.x] = b;
 }
 
 template <typename T>
@@ -69,11 +38,11 @@ __device__ __forceinline__ T ld(const T *a, int i, int j, int nx, int ny, T pad)
 // ---------------------------------------------------------------------------
 // Forward projection: one wave per ray, lanes stride the ray's entries.
 template <typename T, int MODE>
-__global__ __launch_bounds__(kBlock) void k_fp(RedBuf rb, const int64_t *__restrict__ rowptr,
+__global__ __launch_bounds__(kBlock) void k_fp(const int64_t *__restrict__ rowptr,
                                                const uint32_t *__restrict__ idx, const T *__restrict__ len,
                                                int64_t row0, int64_t nrows, const T *__restrict__ x,
                                                const T *__restrict__ g, const T *__restrict__ q,
-                                               T *__restrict__ out, Slot slot, const int32_t *skip) {
+                                               T *__restrict__ out, double *part, const int32_t *skip) {
     __shared__ double lds[kBlock / kWave];
     if (skip && *skip) return;
     const int lane = threadIdx.x & (kWave - 1);
@@ -116,7 +85,7 @@ __global__ __launch_bounds__(kBlock) void k_fp(RedBuf rb, const int64_t *__restr
             }
         }
     }
-    reduce_to(rb, mine, lds, [&](double t) { slot_put(slot, t); });
+    put_partial(part, mine, lds);
 }
 
 // GoldenC image-gradient regulariser at pixel (i, j): f zero padded, v one
@@ -135,11 +104,11 @@ __device__ __forceinline__ double gc_reg(const T *u, const T *v, int i, int j, i
 // Gather back projection: LPC lanes per pixel column of the subset's CSC,
 // then the fused gradient d = scale * b + alpha * R(u, v).
 template <typename T, int LPC>
-__global__ __launch_bounds__(kBlock) void k_bp_grad(RedBuf rb, const int64_t *__restrict__ colptr,
+__global__ __launch_bounds__(kBlock) void k_bp_grad(const int64_t *__restrict__ colptr,
                                                     const uint32_t *__restrict__ crow, const T *__restrict__ clen,
                                                     int nx, int ny, const T *__restrict__ r, double scale,
                                                     const T *__restrict__ u, const T *__restrict__ v, double alpha,
-                                                    double eps, T *__restrict__ d, Slot slot,
+                                                    double eps, T *__restrict__ d, double *part,
                                                     const int32_t *skip) {
     __shared__ double lds[kBlock / kWave];
     if (skip && *skip) return;
@@ -168,13 +137,13 @@ __global__ __launch_bounds__(kBlock) void k_bp_grad(RedBuf rb, const int64_t *__
             mine += (double)dt * (double)dt;
         }
     }
-    reduce_to(rb, mine, lds, [&](double t) { slot_put(slot, t); });
+    put_partial(part, mine, lds);
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_nv(RedBuf rb, int nx, int ny, const T *__restrict__ p,
-                                               const T *__restrict__ v, double eps, Slot slot, Scalars *sc, double S,
-                                               double alpha, const int32_t *skip) {
+__global__ __launch_bounds__(kBlock) void k_nv(int nx, int ny, const T *__restrict__ p,
+                                               const T *__restrict__ v, double eps, double *part,
+                                               const int32_t *skip) {
     __shared__ double lds[kBlock / kWave];
     if (skip && *skip) return;
     const int64_t npix = (int64_t)nx * ny;
@@ -188,19 +157,13 @@ __global__ __launch_bounds__(kBlock) void k_nv(RedBuf rb, int nx, int ny, const 
         const double w = (double)v[q];
         mine += (w * w + e2) * (a * a + b * b);
     }
-    reduce_to(rb, mine, lds, [&](double t) {
-        slot_put(slot, t);
-        if (sc) {
-            const double den = S * sc->ad2 + alpha * t;
-            sc->nv = t;
-            sc->c = (den > 0) ? sc->dd / den : 0.0;
-        }
-    });
+    put_partial(part, mine, lds);
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_energy_reg(RedBuf rb, int nx, int ny, const T *__restrict__ f,
-                                                       const T *__restrict__ v, double alpha, double eps, Slot slot) {
+__global__ __launch_bounds__(kBlock) void k_energy_reg(int nx, int ny, const T *__restrict__ f,
+                                                       const T *__restrict__ v, double alpha, double eps,
+                                                       double *part) {
     __shared__ double lds[kBlock / kWave];
     const int64_t npix = (int64_t)nx * ny;
     double mine = 0.0;
@@ -213,23 +176,23 @@ __global__ __launch_bounds__(kBlock) void k_energy_reg(RedBuf rb, int nx, int ny
         const double w = (double)v[q];
         mine += alpha * (w * w + e2) * (a * a + b * b);
     }
-    reduce_to(rb, mine, lds, [&](double t) { slot_put(slot, t); });
+    put_partial(part, mine, lds);
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_dot(RedBuf rb, int64_t n, const T *__restrict__ a,
-                                                const T *__restrict__ b, Slot slot) {
+__global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const T *__restrict__ a, const T *__restrict__ b,
+                                                double *part) {
     __shared__ double lds[kBlock / kWave];
     double mine = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
         mine += (double)a[i] * (double)b[i];
-    reduce_to(rb, mine, lds, [&](double t) { slot_put(slot, t); });
+    put_partial(part, mine, lds);
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_axpy(RedBuf rb, int64_t n, T *__restrict__ y, const T *__restrict__ x,
+__global__ __launch_bounds__(kBlock) void k_axpy(int64_t n, T *__restrict__ y, const T *__restrict__ x,
                                                  const double *cptr, double cval, double lo, double hi,
-                                                 T *__restrict__ save, Slot slot, const int32_t *skip) {
+                                                 T *__restrict__ save, double *part, const int32_t *skip) {
     __shared__ double lds[kBlock / kWave];
     if (skip && *skip) return;
     const double c = cptr ? *cptr : cval;
@@ -244,10 +207,7 @@ __global__ __launch_bounds__(kBlock) void k_axpy(RedBuf rb, int64_t n, T *__rest
         y[i] = (T)yn;
         mx = fmax(mx, fabs(cx));
     }
-    if (!slot.dst) return;
-    double bm = block_max(mx, lds);
-    GridRed g{rb.partial, rb.ticket};
-    grid_finish(g, bm, true, lds, [&](double t) { slot_put(slot, t); });
+    if (part) put_partial_max(part, mx, lds);
 }
 
 template <typename T>
@@ -287,9 +247,10 @@ __global__ void k_median(int nx, int ny, const T *__restrict__ in, T *__restrict
 
 // Edge step, origin/GoldenC/src/main.c:294-305: gradient and the two sums.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_edge_grad(RedBuf rb, int nx, int ny, const T *__restrict__ f,
+__global__ __launch_bounds__(kBlock) void k_edge_grad(int nx, int ny, const T *__restrict__ f,
                                                       const T *__restrict__ v, T *__restrict__ d, double alpha,
-                                                      double beta, double eps, Scalars *sc) {
+                                                      double beta, double eps, const Scalars *sc, double *part0,
+                                                      double *part1) {
     __shared__ double lds[kBlock / kWave];
     if (sc->edge_done || sc->converged) return;
     const int64_t npix = (int64_t)nx * ny;
@@ -310,16 +271,14 @@ __global__ __launch_bounds__(kBlock) void k_edge_grad(RedBuf rb, int nx, int ny,
         m0 += dd;
         m1 += n * dd;
     }
-    reduce2_to(rb, m0, m1, lds, [&](double t0, double t1) {
-        sc->e_nd = t0;
-        sc->e_nd2 = t1;
-    });
+    put_partial(part0, m0, lds);
+    put_partial(part1, m1, lds);
 }
 
 // main.c:311-329: |central gradient of d|^2 and the step size.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_edge_nabla(RedBuf rb, int nx, int ny, const T *__restrict__ d,
-                                                       double alpha, double beta, double eps, Scalars *sc) {
+__global__ __launch_bounds__(kBlock) void k_edge_nabla(int nx, int ny, const T *__restrict__ d, const Scalars *sc,
+                                                       double *part) {
     __shared__ double lds[kBlock / kWave];
     if (sc->edge_done || sc->converged) return;
     const int64_t npix = (int64_t)nx * ny;
@@ -330,16 +289,13 @@ __global__ __launch_bounds__(kBlock) void k_edge_nabla(RedBuf rb, int nx, int ny
         const double b = 0.5 * ((double)ld(d, i, j + 1, nx, ny, T(0)) - (double)ld(d, i, j - 1, nx, ny, T(0)));
         mine += a * a + b * b;
     }
-    reduce_to(rb, mine, lds, [&](double t) {
-        sc->e_nnd = t;
-        sc->e_c = sc->e_nd / (alpha * sc->e_nd2 + beta * eps * t + (beta / (4 * eps)) * sc->e_nd);
-    });
+    put_partial(part, mine, lds);
 }
 
 // main.c:332-354: update, clamp to [0, 1], early-stop flag.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_edge_update(RedBuf rb, int nx, int ny, T *__restrict__ v,
-                                                        const T *__restrict__ d, Scalars *sc) {
+__global__ __launch_bounds__(kBlock) void k_edge_update(int nx, int ny, T *__restrict__ v, const T *__restrict__ d,
+                                                        const Scalars *sc, double *part) {
     __shared__ double lds[kBlock / kWave];
     if (sc->edge_done || sc->converged) return;
     const int64_t npix = (int64_t)nx * ny;
@@ -353,12 +309,52 @@ __global__ __launch_bounds__(kBlock) void k_edge_update(RedBuf rb, int nx, int n
         if (x < 0) x = 0;
         v[q] = (T)x;
     }
-    double bm = block_max(mx, lds);
-    GridRed g{rb.partial, rb.ticket};
-    grid_finish(g, bm, true, lds, [&](double t) {
-        sc->e_maxcd = t;
-        if (t < 1e-5) sc->edge_done = 1;
-    });
+    put_partial_max(part, mx, lds);
+}
+
+// One block: reduces partial arrays in index order and applies `op`.
+__global__ __launch_bounds__(kBlock) void k_finalize(FinArgs a) {
+    __shared__ double lds[kBlock / kWave];
+    Scalars *sc = a.sc;
+    if (a.skip && *a.skip) return;
+    if ((a.op == FIN_EDGE_C || a.op == FIN_EDGE_DONE) && (sc->edge_done || sc->converged)) return;
+    double tot[4] = {0, 0, 0, 0};
+    for (int k = 0; k < a.nin; ++k) {
+        double acc = 0.0;
+        const bool mx = a.is_max[k];
+        for (int i = threadIdx.x; i < a.n[k]; i += kBlock) acc = mx ? fmax(acc, a.in[k][i]) : acc + a.in[k][i];
+        tot[k] = mx ? block_max(acc, lds) : block_sum(acc, lds);
+    }
+    if (threadIdx.x != 0) return;
+    switch (a.op) {
+        case FIN_STORE:
+            *a.dst = tot[0];
+            break;
+        case FIN_ACCUM:
+            *a.dst += tot[0];
+            break;
+        case FIN_STEP: {
+            // in: <d,d>, ||A_s d||^2, nV(d), and the subset's ||r||^2
+            sc->df_pass += tot[3];
+            sc->dd = tot[0];
+            sc->ad2 = tot[1];
+            sc->nv = tot[2];
+            const double den = a.S * tot[1] + a.alpha * tot[2];
+            sc->c = (den > 0) ? tot[0] / den : 0.0;
+            break;
+        }
+        case FIN_EDGE_C:
+            // in: ||d||^2, sum n d^2, |grad d|^2 (origin/GoldenC/src/main.c:329)
+            sc->e_nd = tot[0];
+            sc->e_nd2 = tot[1];
+            sc->e_nnd = tot[2];
+            sc->e_c = tot[0] / (a.alpha * tot[1] + a.beta * a.eps * tot[2] + (a.beta / (4 * a.eps)) * tot[0]);
+            break;
+        case FIN_EDGE_DONE:
+            sc->e_maxcd = tot[0];
+            if (tot[0] < 1e-5) sc->edge_done = 1;
+            break;
+    }
 }
 
 __global__ void k_pass_begin(Scalars *sc) {
@@ -398,13 +394,17 @@ __global__ void k_scatter_rows(int64_t n, const int64_t *__restrict__ perm, cons
 // ---------------------------------------------------------------------------
 // Launchers.
 
+unsigned grid_rows(int64_t nrows) { return cap_grid((nrows + 3) / 4); }
+unsigned grid_pixels(int64_t n) { return cap_grid((n + kBlock - 1) / kBlock); }
+unsigned grid_bp(int64_t npix, int lanes) { return cap_grid((npix + kBlock / lanes - 1) / (kBlock / lanes)); }
+
 template <typename T>
-void launch_fp(hipStream_t st, const RedBuf &rb, int mode, const CsrView &A, int64_t row0, int64_t nrows,
-               const T *x, const T *g, const T *q, T *out, Slot slot, const int32_t *skip) {
-    const unsigned grid = cap_grid((nrows + 3) / 4);
+unsigned launch_fp(hipStream_t st, int mode, const CsrView &A, int64_t row0, int64_t nrows, const T *x, const T *g,
+                   const T *q, T *out, double *part, const int32_t *skip) {
+    const unsigned grid = grid_rows(nrows);
     const T *len = (const T *)A.len;
 #define MI_FP_CASE(M) \
-    case M: k_fp<T, M><<<grid, kBlock, 0, st>>>(rb, A.rowptr, A.idx, len, row0, nrows, x, g, q, out, slot, skip); break;
+    case M: k_fp<T, M><<<grid, kBlock, 0, st>>>(A.rowptr, A.idx, len, row0, nrows, x, g, q, out, part, skip); break;
     switch (mode) {
         MI_FP_CASE(FP_WRITE)
         MI_FP_CASE(FP_RESID)
@@ -415,21 +415,20 @@ void launch_fp(hipStream_t st, const RedBuf &rb, int mode, const CsrView &A, int
     }
 #undef MI_FP_CASE
     MI_HIP(hipGetLastError());
+    return grid;
 }
 
 template <typename T>
-void launch_bp_grad(hipStream_t st, const RedBuf &rb, const CscView &At, int nx, int ny, const T *r, double scale,
-                    const T *u, const T *v, double alpha, double eps, T *d, Slot slot, int lanes,
-                    const int32_t *skip) {
+unsigned launch_bp_grad(hipStream_t st, const CscView &At, int nx, int ny, const T *r, double scale, const T *u,
+                        const T *v, double alpha, double eps, T *d, double *part, int lanes, const int32_t *skip) {
     const int64_t npix = (int64_t)nx * ny;
     const T *len = (const T *)At.len;
+    const unsigned grid = grid_bp(npix, lanes);
 #define MI_BP_CASE(L)                                                                                          \
-    case L: {                                                                                                  \
-        const unsigned grid = cap_grid((npix + kBlock / L - 1) / (kBlock / L));                                \
-        k_bp_grad<T, L><<<grid, kBlock, 0, st>>>(rb, At.colptr, At.crow, len, nx, ny, r, scale, u, v, alpha,   \
-                                                 eps, d, slot, skip);                                          \
-        break;                                                                                                 \
-    }
+    case L:                                                                                                    \
+        k_bp_grad<T, L><<<grid, kBlock, 0, st>>>(At.colptr, At.crow, len, nx, ny, r, scale, u, v, alpha, eps, d, \
+                                                 part, skip);                                                  \
+        break;
     switch (lanes) {
         MI_BP_CASE(4)
         MI_BP_CASE(8)
@@ -440,63 +439,91 @@ void launch_bp_grad(hipStream_t st, const RedBuf &rb, const CscView &At, int nx,
     }
 #undef MI_BP_CASE
     MI_HIP(hipGetLastError());
+    return grid;
 }
 
 template <typename T>
-void launch_nv(hipStream_t st, const RedBuf &rb, int nx, int ny, const T *p, const T *v, double eps, Slot slot,
-               Scalars *sc, double S, double alpha, const int32_t *skip) {
-    const int64_t npix = (int64_t)nx * ny;
-    k_nv<T><<<cap_grid((npix + kBlock - 1) / kBlock), kBlock, 0, st>>>(rb, nx, ny, p, v, eps, slot, sc, S, alpha,
-                                                                       skip);
+unsigned launch_nv(hipStream_t st, int nx, int ny, const T *p, const T *v, double eps, double *part,
+                   const int32_t *skip) {
+    const unsigned grid = grid_pixels((int64_t)nx * ny);
+    k_nv<T><<<grid, kBlock, 0, st>>>(nx, ny, p, v, eps, part, skip);
     MI_HIP(hipGetLastError());
+    return grid;
 }
 
 template <typename T>
-void launch_energy_reg(hipStream_t st, const RedBuf &rb, int nx, int ny, const T *f, const T *v, double alpha,
-                       double eps, Slot slot) {
-    const int64_t npix = (int64_t)nx * ny;
-    k_energy_reg<T><<<cap_grid((npix + kBlock - 1) / kBlock), kBlock, 0, st>>>(rb, nx, ny, f, v, alpha, eps, slot);
+unsigned launch_energy_reg(hipStream_t st, int nx, int ny, const T *f, const T *v, double alpha, double eps,
+                           double *part) {
+    const unsigned grid = grid_pixels((int64_t)nx * ny);
+    k_energy_reg<T><<<grid, kBlock, 0, st>>>(nx, ny, f, v, alpha, eps, part);
     MI_HIP(hipGetLastError());
+    return grid;
 }
 
 template <typename T>
-void launch_dot(hipStream_t st, const RedBuf &rb, int64_t n, const T *a, const T *b, Slot slot) {
-    k_dot<T><<<cap_grid((n + kBlock - 1) / kBlock), kBlock, 0, st>>>(rb, n, a, b, slot);
+unsigned launch_dot(hipStream_t st, int64_t n, const T *a, const T *b, double *part) {
+    const unsigned grid = grid_pixels(n);
+    k_dot<T><<<grid, kBlock, 0, st>>>(n, a, b, part);
     MI_HIP(hipGetLastError());
+    return grid;
 }
 
 template <typename T>
-void launch_axpy(hipStream_t st, const RedBuf &rb, int64_t n, T *y, const T *x, const double *cptr, double cval,
-                 double lo, double hi, T *save, Slot slot, const int32_t *skip) {
-    k_axpy<T><<<cap_grid((n + kBlock - 1) / kBlock), kBlock, 0, st>>>(rb, n, y, x, cptr, cval, lo, hi, save, slot,
-                                                                      skip);
+unsigned launch_axpy(hipStream_t st, int64_t n, T *y, const T *x, const double *cptr, double cval, double lo,
+                     double hi, T *save, double *part, const int32_t *skip) {
+    const unsigned grid = grid_pixels(n);
+    k_axpy<T><<<grid, kBlock, 0, st>>>(n, y, x, cptr, cval, lo, hi, save, part, skip);
     MI_HIP(hipGetLastError());
+    return grid;
 }
 
 template <typename T>
 void launch_xpby(hipStream_t st, int64_t n, T *y, const T *x, double c) {
-    k_xpby<T><<<cap_grid((n + kBlock - 1) / kBlock), kBlock, 0, st>>>(n, y, x, c);
+    k_xpby<T><<<grid_pixels(n), kBlock, 0, st>>>(n, y, x, c);
     MI_HIP(hipGetLastError());
 }
 
 template <typename T>
 void launch_median(hipStream_t st, int nx, int ny, const T *in, T *out) {
-    const int64_t npix = (int64_t)nx * ny;
-    k_median<T><<<cap_grid((npix + kBlock - 1) / kBlock), kBlock, 0, st>>>(nx, ny, in, out);
+    k_median<T><<<grid_pixels((int64_t)nx * ny), kBlock, 0, st>>>(nx, ny, in, out);
+    MI_HIP(hipGetLastError());
+}
+
+void launch_finalize(hipStream_t st, const FinArgs &a) {
+    k_finalize<<<1, kBlock, 0, st>>>(a);
     MI_HIP(hipGetLastError());
 }
 
 template <typename T>
 void launch_edge_iter(hipStream_t st, const RedBuf &rb, int nx, int ny, const T *f, T *v, T *dwork, double alpha,
                       double beta, double eps, Scalars *sc) {
-    const int64_t npix = (int64_t)nx * ny;
-    const unsigned grid = cap_grid((npix + kBlock - 1) / kBlock);
-    k_edge_grad<T><<<grid, kBlock, 0, st>>>(rb, nx, ny, f, v, dwork, alpha, beta, eps, sc);
+    const unsigned grid = grid_pixels((int64_t)nx * ny);
+    k_edge_grad<T><<<grid, kBlock, 0, st>>>(nx, ny, f, v, dwork, alpha, beta, eps, sc, rb.slot(0), rb.slot(1));
     MI_HIP(hipGetLastError());
-    k_edge_nabla<T><<<grid, kBlock, 0, st>>>(rb, nx, ny, dwork, alpha, beta, eps, sc);
+    k_edge_nabla<T><<<grid, kBlock, 0, st>>>(nx, ny, dwork, sc, rb.slot(2));
     MI_HIP(hipGetLastError());
-    k_edge_update<T><<<grid, kBlock, 0, st>>>(rb, nx, ny, v, dwork, sc);
+    FinArgs c{};
+    c.op = FIN_EDGE_C;
+    c.nin = 3;
+    for (int k = 0; k < 3; ++k) {
+        c.in[k] = rb.slot(k);
+        c.n[k] = (int)grid;
+    }
+    c.sc = sc;
+    c.alpha = alpha;
+    c.beta = beta;
+    c.eps = eps;
+    launch_finalize(st, c);
+    k_edge_update<T><<<grid, kBlock, 0, st>>>(nx, ny, v, dwork, sc, rb.slot(3));
     MI_HIP(hipGetLastError());
+    FinArgs d{};
+    d.op = FIN_EDGE_DONE;
+    d.nin = 1;
+    d.in[0] = rb.slot(3);
+    d.n[0] = (int)grid;
+    d.is_max[0] = 1;
+    d.sc = sc;
+    launch_finalize(st, d);
 }
 
 void launch_pass_begin(hipStream_t st, Scalars *sc) {
@@ -522,17 +549,17 @@ void launch_scatter_rows(hipStream_t st, int64_t n, const int64_t *perm, const T
 }
 
 #define MI_INST(T)                                                                                              \
-    template void launch_fp<T>(hipStream_t, const RedBuf &, int, const CsrView &, int64_t, int64_t, const T *,  \
-                               const T *, const T *, T *, Slot, const int32_t *);                               \
-    template void launch_bp_grad<T>(hipStream_t, const RedBuf &, const CscView &, int, int, const T *, double,  \
-                                    const T *, const T *, double, double, T *, Slot, int, const int32_t *);     \
-    template void launch_nv<T>(hipStream_t, const RedBuf &, int, int, const T *, const T *, double, Slot,       \
-                               Scalars *, double, double, const int32_t *);                                     \
-    template void launch_energy_reg<T>(hipStream_t, const RedBuf &, int, int, const T *, const T *, double,     \
-                                       double, Slot);                                                           \
-    template void launch_dot<T>(hipStream_t, const RedBuf &, int64_t, const T *, const T *, Slot);              \
-    template void launch_axpy<T>(hipStream_t, const RedBuf &, int64_t, T *, const T *, const double *, double,  \
-                                 double, double, T *, Slot, const int32_t *);                                   \
+    template unsigned launch_fp<T>(hipStream_t, int, const CsrView &, int64_t, int64_t, const T *, const T *,   \
+                                   const T *, T *, double *, const int32_t *);                                  \
+    template unsigned launch_bp_grad<T>(hipStream_t, const CscView &, int, int, const T *, double, const T *,   \
+                                        const T *, double, double, T *, double *, int, const int32_t *);        \
+    template unsigned launch_nv<T>(hipStream_t, int, int, const T *, const T *, double, double *,               \
+                                   const int32_t *);                                                            \
+    template unsigned launch_energy_reg<T>(hipStream_t, int, int, const T *, const T *, double, double,         \
+                                           double *);                                                           \
+    template unsigned launch_dot<T>(hipStream_t, int64_t, const T *, const T *, double *);                      \
+    template unsigned launch_axpy<T>(hipStream_t, int64_t, T *, const T *, const double *, double, double,      \
+                                     double, T *, double *, const int32_t *);                                   \
     template void launch_xpby<T>(hipStream_t, int64_t, T *, const T *, double);                                 \
     template void launch_median<T>(hipStream_t, int, int, const T *, T *);                                      \
     template void launch_edge_iter<T>(hipStream_t, const RedBuf &, int, int, const T *, T *, T *, double,       \

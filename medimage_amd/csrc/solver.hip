@@ -84,15 +84,13 @@ JobState<T>::JobState(const Matrix &m) {
     t.alloc(npix);
     w.alloc(npix);
     sc.alloc(1);
-    red_partial.alloc(2 * kMaxGrid);
-    red_ticket.alloc(1);
-    MI_HIP(hipMemset(red_ticket.p, 0, sizeof(unsigned)));
+    red_partial.alloc((size_t)kSlots * kMaxGrid);
     MI_HIP(hipMemset(sc.p, 0, sizeof(Scalars)));
 }
 
 template <typename T>
 RedBuf JobState<T>::rb() const {
-    return RedBuf{red_partial.p, red_ticket.p};
+    return RedBuf{red_partial.p};
 }
 
 template <typename T>
@@ -119,24 +117,32 @@ void os_pass(mi_context *ctx, const Matrix &m, JobState<T> &js, const mi_at_para
     launch_pass_begin(st, sc);
     for (int s = 0; s < m.S; ++s) {
         const int64_t row0 = m.sub_row0[s], nr = m.sub_row0[s + 1] - row0;
+        FinArgs f{};
+        f.op = FIN_STEP;
+        f.nin = 4;
+        f.sc = sc;
+        f.skip = skip;
+        f.S = S;
+        f.alpha = p.alpha;
         {
             LaunchTimer lt(ctx, MI_K_FP, fp_bytes(m, nr, sizeof(T), FP_RESID));
-            launch_fp<T>(st, rb, FP_RESID, A, row0, nr, js.u.p, js.g.p, nullptr, js.r.p, Slot{&sc->df_pass, 1},
-                         skip);
+            f.n[3] = (int)launch_fp<T>(st, FP_RESID, A, row0, nr, js.u.p, js.g.p, nullptr, js.r.p, rb.slot(3), skip);
         }
         {
             LaunchTimer lt(ctx, MI_K_BP, bp_bytes(m, nr, sizeof(T), true));
-            launch_bp_grad<T>(st, rb, csc_view(m, s, f64), js.nx, js.ny, js.r.p, S, js.u.p, js.v.p, p.alpha,
-                              p.epsilon, js.d.p, Slot{&sc->dd, 0}, lanes, skip);
+            f.n[0] = (int)launch_bp_grad<T>(st, csc_view(m, s, f64), js.nx, js.ny, js.r.p, S, js.u.p, js.v.p,
+                                            p.alpha, p.epsilon, js.d.p, rb.slot(0), lanes, skip);
         }
         {
             LaunchTimer lt(ctx, MI_K_FPSQ, fp_bytes(m, nr, sizeof(T), FP_SQ));
-            launch_fp<T>(st, rb, FP_SQ, A, row0, nr, js.d.p, nullptr, nullptr, nullptr, Slot{&sc->ad2, 0}, skip);
+            f.n[1] = (int)launch_fp<T>(st, FP_SQ, A, row0, nr, js.d.p, nullptr, nullptr, nullptr, rb.slot(1), skip);
         }
         {
             LaunchTimer lt(ctx, MI_K_STEP, (double)js.npix * sizeof(T) * 5);
-            launch_nv<T>(st, rb, js.nx, js.ny, js.d.p, js.v.p, p.epsilon, Slot{&sc->nv, 0}, sc, S, p.alpha, skip);
-            launch_axpy<T>(st, rb, js.npix, js.u.p, js.d.p, &sc->c, 0.0, p.lower, p.upper, nullptr, Slot{}, skip);
+            f.n[2] = (int)launch_nv<T>(st, js.nx, js.ny, js.d.p, js.v.p, p.epsilon, rb.slot(2), skip);
+            for (int k = 0; k < 4; ++k) f.in[k] = rb.slot(k);
+            launch_finalize(st, f);
+            launch_axpy<T>(st, js.npix, js.u.p, js.d.p, &sc->c, 0.0, p.lower, p.upper, nullptr, nullptr, skip);
         }
     }
     if (p.median_every > 0 && pass_index % p.median_every == 0) {
@@ -258,21 +264,32 @@ int gc_image_step(mi_context *ctx, const Matrix &m, JobState<T> &js, double alph
     const size_t ib = js.npix * sizeof(T);
     int reason = 0, ran = 0;
 
-    launch_fp<T>(st, rb, FP_RESID, A, 0, m.nrows, js.u.p, js.g.p, nullptr, js.r.p, Slot{&sc->gen[0], 0});
-    launch_bp_grad<T>(st, rb, At, js.nx, js.ny, js.r.p, 1.0, js.u.p, js.v.p, alpha, eps, js.d.p, Slot{}, lanes);
-    launch_energy_reg<T>(st, rb, js.nx, js.ny, js.u.p, js.v.p, alpha, eps, Slot{&sc->gen[1], 0});
+    // Producer into partial slot k, then a one-block sum into sc->gen[k].
+    auto to_gen = [&](unsigned grid, int k, bool is_max = false) {
+        FinArgs f{};
+        f.op = FIN_STORE;
+        f.nin = 1;
+        f.in[0] = rb.slot(k);
+        f.n[0] = (int)grid;
+        f.is_max[0] = is_max;
+        f.dst = &sc->gen[k];
+        launch_finalize(st, f);
+    };
+    to_gen(launch_fp<T>(st, FP_RESID, A, 0, m.nrows, js.u.p, js.g.p, nullptr, js.r.p, rb.slot(0)), 0);
+    launch_bp_grad<T>(st, At, js.nx, js.ny, js.r.p, 1.0, js.u.p, js.v.p, alpha, eps, js.d.p, nullptr, lanes);
+    to_gen(launch_energy_reg<T>(st, js.nx, js.ny, js.u.p, js.v.p, alpha, eps, rb.slot(1)), 1);
     MI_HIP(hipMemcpyAsync(js.p.p, js.d.p, ib, hipMemcpyDeviceToDevice, st));
     Scalars h = js.read_scalars(st);
     double at_old = h.gen[0] + h.gen[1];
     for (int k = 1; k <= iter_image; ++k) {
         ran = k;
-        launch_fp<T>(st, rb, FP_WRITE, A, 0, m.nrows, js.p.p, nullptr, nullptr, js.q.p, Slot{&sc->gen[2], 0});
-        launch_nv<T>(st, rb, js.nx, js.ny, js.p.p, js.v.p, eps, Slot{&sc->gen[3], 0}, nullptr, 0, 0);
-        launch_dot<T>(st, rb, js.npix, js.p.p, js.d.p, Slot{&sc->gen[4], 0});
+        to_gen(launch_fp<T>(st, FP_WRITE, A, 0, m.nrows, js.p.p, nullptr, nullptr, js.q.p, rb.slot(2)), 2);
+        to_gen(launch_nv<T>(st, js.nx, js.ny, js.p.p, js.v.p, eps, rb.slot(3)), 3);
+        to_gen(launch_dot<T>(st, js.npix, js.p.p, js.d.p, rb.slot(4)), 4);
         h = js.read_scalars(st);
         const double nap = h.gen[2];
         const double c1 = h.gen[4] / (nap + alpha * h.gen[3]);
-        launch_axpy<T>(st, rb, js.npix, js.u.p, js.p.p, nullptr, c1, -inf, inf, js.t.p, Slot{&sc->gen[5], 0});
+        to_gen(launch_axpy<T>(st, js.npix, js.u.p, js.p.p, nullptr, c1, -inf, inf, js.t.p, rb.slot(5)), 5, true);
         // The reference takes k % 0 when iter_image < 2 (undefined); no
         // median is applied then.
         if (cadence > 0 && k % cadence == 0) {
@@ -282,8 +299,8 @@ int gc_image_step(mi_context *ctx, const Matrix &m, JobState<T> &js, double alph
         h = js.read_scalars(st);
         if (k == iter_image) { reason = 0; break; }
         if (h.gen[5] < 1e-5) { reason = 1; break; }
-        launch_fp<T>(st, rb, FP_ERR, A, 0, m.nrows, js.u.p, js.g.p, nullptr, js.af.p, Slot{&sc->gen[0], 0});
-        launch_energy_reg<T>(st, rb, js.nx, js.ny, js.u.p, js.v.p, alpha, eps, Slot{&sc->gen[1], 0});
+        to_gen(launch_fp<T>(st, FP_ERR, A, 0, m.nrows, js.u.p, js.g.p, nullptr, js.af.p, rb.slot(0)), 0);
+        to_gen(launch_energy_reg<T>(st, js.nx, js.ny, js.u.p, js.v.p, alpha, eps, rb.slot(1)), 1);
         h = js.read_scalars(st);
         const double at_new = h.gen[0] + h.gen[1];
         if (1.1 * at_old < at_new && 1 < k) {
@@ -292,10 +309,9 @@ int gc_image_step(mi_context *ctx, const Matrix &m, JobState<T> &js, double alph
             break;
         }
         at_old = at_new;
-        launch_axpy<T>(st, rb, m.nrows, js.r.p, js.q.p, nullptr, -c1, -inf, inf, nullptr, Slot{});
-        launch_bp_grad<T>(st, rb, At, js.nx, js.ny, js.r.p, 1.0, js.u.p, js.v.p, alpha, eps, js.d.p, Slot{},
-                          lanes);
-        launch_fp<T>(st, rb, FP_DOT, A, 0, m.nrows, js.d.p, nullptr, js.q.p, nullptr, Slot{&sc->gen[6], 0});
+        launch_axpy<T>(st, m.nrows, js.r.p, js.q.p, nullptr, -c1, -inf, inf, nullptr, nullptr);
+        launch_bp_grad<T>(st, At, js.nx, js.ny, js.r.p, 1.0, js.u.p, js.v.p, alpha, eps, js.d.p, nullptr, lanes);
+        to_gen(launch_fp<T>(st, FP_DOT, A, 0, m.nrows, js.d.p, nullptr, js.q.p, nullptr, rb.slot(6)), 6);
         h = js.read_scalars(st);
         const double c2 = -h.gen[6] / nap;
         launch_xpby<T>(st, js.npix, js.p.p, js.d.p, c2);

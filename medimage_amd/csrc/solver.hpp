@@ -16,7 +16,6 @@ struct JobState {
     DevBuf<T> u, v, d, p, t, w;    // images: update, edge, gradient, CG direction, saved update, scratch
     DevBuf<Scalars> sc;
     DevBuf<double> red_partial;
-    DevBuf<unsigned> red_ticket;
     explicit JobState(const Matrix &m);
     RedBuf rb() const;
     Scalars read_scalars(hipStream_t st) const;
