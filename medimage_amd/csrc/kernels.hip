@@ -12,9 +12,46 @@ namespace mi {
 
 namespace {
 
+// Grids of the streaming kernels: at most kStreamGrid blocks of 4 waves (one
+// full residency of the chip at 8 waves per SIMD), rounded to a multiple of
+// the 8 XCDs so that xcd_block() is a permutation.
+constexpr int kStreamGrid = 2048;
 inline unsigned cap_grid(int64_t blocks) {
     if (blocks < 1) blocks = 1;
-    return (unsigned)(blocks < kMaxGrid ? blocks : kMaxGrid);
+    if (blocks > kStreamGrid) blocks = kStreamGrid;
+    if (blocks > 8) blocks = (blocks + 7) / 8 * 8;
+    return (unsigned)blocks;
+}
+
+// Workgroups are dispatched to the 8 XCDs round-robin.  Renumbering them so
+// that each XCD owns a contiguous range of the work keeps neighbouring rays
+// (or pixels), which gather the same image (or sinogram) lines, in one L2.
+__device__ __forceinline__ unsigned xcd_block() {
+    const unsigned g = gridDim.x, b = blockIdx.x;
+    if (g < 8 || (g & 7)) return b;
+    return (b & 7) * (g >> 3) + (b >> 3);
+}
+
+// Four consecutive entries of a padded matrix stream (16-byte aligned start).
+template <typename T> struct Vec4;
+template <> struct Vec4<float> {
+    float v[4];
+    __device__ __forceinline__ void load(const float *p) {
+        const float4 a = *reinterpret_cast<const float4 *>(p);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    }
+};
+template <> struct Vec4<double> {
+    double v[4];
+    __device__ __forceinline__ void load(const double *p) {
+        const double2 a = reinterpret_cast<const double2 *>(p)[0];
+        const double2 b = reinterpret_cast<const double2 *>(p)[1];
+        v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+    }
+};
+__device__ __forceinline__ void load_idx4(const uint32_t *p, uint32_t (&o)[4]) {
+    const uint4 a = *reinterpret_cast<const uint4 *>(p);
+    o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
 }
 
 // Block partial: thread 0 of the block stores the block's sum (or max) at
@@ -36,7 +73,11 @@ __device__ __forceinline__ T ld(const T *a, int i, int j, int nx, int ny, T pad)
 }
 
 // ---------------------------------------------------------------------------
-// Forward projection: one wave per ray, lanes stride the ray's entries.
+// Forward projection: one wave per ray.  Each lane takes four consecutive
+// entries per trip with 16-byte loads (the matrix arrays are padded and the
+// walk starts at the 4-aligned entry below the row start; entries outside
+// the row are loaded but weighted by zero), so a trip covers 256 entries and
+// issues four independent image gathers per lane.
 template <typename T, int MODE>
 __global__ __launch_bounds__(kBlock) void k_fp(const int64_t *__restrict__ rowptr,
                                                const uint32_t *__restrict__ idx, const T *__restrict__ len,
@@ -46,23 +87,25 @@ __global__ __launch_bounds__(kBlock) void k_fp(const int64_t *__restrict__ rowpt
     __shared__ double lds[kBlock / kWave];
     if (skip && *skip) return;
     const int lane = threadIdx.x & (kWave - 1);
-    const int64_t wid = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    const int64_t wid = (int64_t)xcd_block() * (kBlock / kWave) + threadIdx.x / kWave;
     const int64_t nw = (int64_t)gridDim.x * (kBlock / kWave);
     double mine = 0.0;
     for (int64_t r = wid; r < nrows; r += nw) {
         const int64_t rr = row0 + r;
         const int64_t b = rowptr[rr], e = rowptr[rr + 1];
         Acc<T> acc;
-        int64_t k = b + lane;
-        // Two independent entries per lane per trip keep more loads in flight.
-        for (; k + kWave < e; k += 2 * kWave) {
-            const uint32_t i0 = idx[k], i1 = idx[k + kWave];
-            const T l0 = len[k], l1 = len[k + kWave];
-            const T x0 = x[i0], x1 = x[i1];
-            acc.add(x0 * l0);
-            acc.add(x1 * l1);
+        for (int64_t k = (b & ~(int64_t)3) + 4 * lane; k < e; k += 4 * kWave) {
+            uint32_t i4[4];
+            Vec4<T> l4;
+            load_idx4(idx + k, i4);
+            l4.load(len + k);
+            T xv[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) xv[j] = x[i4[j]];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (k + j >= b && k + j < e) acc.add(xv[j] * l4.v[j]);
         }
-        if (k < e) acc.add(x[idx[k]] * len[k]);
         const double s = group_sum<kWave>(acc.value());
         if (lane == 0) {
             if (MODE == FP_WRITE) {
@@ -102,6 +145,7 @@ __device__ __forceinline__ double gc_reg(const T *u, const T *v, int i, int j, i
 }
 
 // Gather back projection: LPC lanes per pixel column of the subset's CSC,
+// four consecutive entries per lane per trip (16-byte loads, padded arrays),
 // then the fused gradient d = scale * b + alpha * R(u, v).
 template <typename T, int LPC>
 __global__ __launch_bounds__(kBlock) void k_bp_grad(const int64_t *__restrict__ colptr,
@@ -116,18 +160,21 @@ __global__ __launch_bounds__(kBlock) void k_bp_grad(const int64_t *__restrict__ 
     const int gl = threadIdx.x % LPC, grp = threadIdx.x / LPC;
     const int64_t npix = (int64_t)nx * ny;
     double mine = 0.0;
-    for (int64_t p = (int64_t)blockIdx.x * G + grp; p < npix; p += (int64_t)gridDim.x * G) {
+    for (int64_t p = (int64_t)xcd_block() * G + grp; p < npix; p += (int64_t)gridDim.x * G) {
         const int64_t b = colptr[p], e = colptr[p + 1];
         Acc<T> acc;
-        int64_t k = b + gl;
-        for (; k + LPC < e; k += 2 * LPC) {
-            const uint32_t r0 = crow[k], r1 = crow[k + LPC];
-            const T l0 = clen[k], l1 = clen[k + LPC];
-            const T a0 = r[r0], a1 = r[r1];
-            acc.add(a0 * l0);
-            acc.add(a1 * l1);
+        for (int64_t k = (b & ~(int64_t)3) + 4 * gl; k < e; k += 4 * LPC) {
+            uint32_t r4[4];
+            Vec4<T> l4;
+            load_idx4(crow + k, r4);
+            l4.load(clen + k);
+            T rv[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) rv[j] = r[r4[j]];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (k + j >= b && k + j < e) acc.add(rv[j] * l4.v[j]);
         }
-        if (k < e) acc.add(r[crow[k]] * clen[k]);
         const double s = group_sum<LPC>(acc.value());
         if (gl == 0) {
             double dv = scale * s;
@@ -313,17 +360,37 @@ __global__ __launch_bounds__(kBlock) void k_edge_update(int nx, int ny, T *__res
 }
 
 // One block: reduces partial arrays in index order and applies `op`.
-__global__ __launch_bounds__(kBlock) void k_finalize(FinArgs a) {
-    __shared__ double lds[kBlock / kWave];
+constexpr int kFinBlock = 1024;
+__global__ __launch_bounds__(kFinBlock) void k_finalize(FinArgs a) {
+    __shared__ double lds[kFinBlock / kWave];
     Scalars *sc = a.sc;
     if (a.skip && *a.skip) return;
     if ((a.op == FIN_EDGE_C || a.op == FIN_EDGE_DONE) && (sc->edge_done || sc->converged)) return;
     double tot[4] = {0, 0, 0, 0};
     for (int k = 0; k < a.nin; ++k) {
-        double acc = 0.0;
         const bool mx = a.is_max[k];
-        for (int i = threadIdx.x; i < a.n[k]; i += kBlock) acc = mx ? fmax(acc, a.in[k][i]) : acc + a.in[k][i];
-        tot[k] = mx ? block_max(acc, lds) : block_sum(acc, lds);
+        const int n = a.n[k];
+        // Eight independent loads per thread before any add.
+        double vals[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int i = threadIdx.x + u * kFinBlock;
+            vals[u] = (i < n) ? a.in[k][i] : 0.0;
+        }
+        double acc = 0.0;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc = mx ? fmax(acc, vals[u]) : acc + vals[u];
+        // Fixed-order tree: wave butterfly, then the waves in order.
+        acc = mx ? wave_max(acc) : group_sum<kWave>(acc);
+        const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+        if (lane == 0) lds[w] = acc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int i = 0; i < kFinBlock / kWave; ++i) t = mx ? fmax(t, lds[i]) : t + lds[i];
+            tot[k] = t;
+        }
+        __syncthreads();
     }
     if (threadIdx.x != 0) return;
     switch (a.op) {
@@ -490,7 +557,8 @@ void launch_median(hipStream_t st, int nx, int ny, const T *in, T *out) {
 }
 
 void launch_finalize(hipStream_t st, const FinArgs &a) {
-    k_finalize<<<1, kBlock, 0, st>>>(a);
+    for (int k = 0; k < a.nin; ++k) MI_REQUIRE(a.n[k] <= 8 * kFinBlock, MI_E_LIMIT, "too many partials");
+    k_finalize<<<1, kFinBlock, 0, st>>>(a);
     MI_HIP(hipGetLastError());
 }
 

@@ -40,12 +40,13 @@ CscView csc_view(const Matrix &m, int s, bool f64) {
     return CscView{m.colptr.p + (int64_t)s * m.npix, m.crow.p, f64 ? (const void *)m.clen64.p : (const void *)m.clen32.p};
 }
 
-// Lanes per pixel column of the gather back projector: about a quarter of the
-// subset's mean column length, within [4, 64].
+// Lanes per pixel column of the gather back projector: each lane takes four
+// entries per trip; aim at about three trips over the subset's mean column
+// length, within [4, 64].
 int bp_lanes(const Matrix &m) {
     const double mean = (double)m.nnz / ((double)m.S * (double)m.npix);
     int l = 4;
-    while (l < 64 && l * 4 < mean) l *= 2;
+    while (l < 64 && l * 12 < mean) l *= 2;
     return l;
 }
 

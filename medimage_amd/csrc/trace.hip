@@ -223,6 +223,22 @@ void gc_to_csr(const GcRows &gr, const std::vector<int> &vlist, int nbins, std::
     }
 }
 
+// Matrix streams are read four entries at a time from the 4-aligned entry at
+// or below a row's start, so every stream carries kPad zero entries at its end
+// (index 0 and length 0: a valid pixel/row with no weight).
+constexpr size_t kPad = 4;
+template <typename T>
+void alloc_padded(DevBuf<T> &b, size_t n, hipStream_t st) {
+    b.alloc(n + kPad);
+    MI_HIP(hipMemsetAsync(b.p + n, 0, kPad * sizeof(T), st));
+}
+
+template <typename T>
+void upload_padded(DevBuf<T> &b, const std::vector<T> &h, hipStream_t st) {
+    alloc_padded(b, h.size(), st);
+    if (!h.empty()) MI_HIP(hipMemcpyAsync(b.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
+}
+
 template <typename T>
 void upload(DevBuf<T> &b, const std::vector<T> &h, hipStream_t st) {
     b.alloc(h.size());
@@ -259,9 +275,9 @@ void build_csc(Matrix &m, const int64_t *rowptr, const uint32_t *idx, const floa
     DevBuf<int64_t> d_sub(m.S + 1);
     MI_HIP(hipMemcpyAsync(d_sub.p, m.sub_row0.data(), (m.S + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
     m.colptr.alloc(ncols + 1);
-    m.crow.alloc(nnz ? nnz : 1);
-    m.clen32.alloc(nnz ? nnz : 1);
-    if (len64) m.clen64.alloc(nnz ? nnz : 1);
+    alloc_padded(m.crow, nnz, st);
+    alloc_padded(m.clen32, nnz, st);
+    if (len64) alloc_padded(m.clen64, nnz, st);
     if (nnz == 0) {
         k_fill_i64<<<grid_of(ncols + 1), kBlock, 0, st>>>(m.colptr.p, ncols + 1, 0);
         MI_HIP(hipGetLastError());
@@ -342,9 +358,9 @@ std::unique_ptr<Matrix> build_matrix(const mi_geometry &g, int S, bool want64, h
         m->nnz = rp.back();
         std::vector<float> ln32(ln.begin(), ln.end()), lna32(lna.begin(), lna.end());
         upload(m->rowptr, rp, st);
-        upload(m->idx, ix, st);
-        upload(m->len32, ln32, st);
-        if (want64) upload(m->len64, ln, st);
+        upload_padded(m->idx, ix, st);
+        upload_padded(m->len32, ln32, st);
+        if (want64) upload_padded(m->len64, ln, st);
         DevBuf<int64_t> arp;
         DevBuf<uint32_t> aix;
         DevBuf<float> aln32;
@@ -405,9 +421,9 @@ std::unique_ptr<Matrix> build_matrix(const mi_geometry &g, int S, bool want64, h
     MI_HIP(hipGetLastError());
     inclusive_scan_i64(m->rowptr.p, m->nrows + 1, st);
     MI_HIP(hipMemcpy(&m->nnz, m->rowptr.p + m->nrows, sizeof(int64_t), hipMemcpyDeviceToHost));
-    m->idx.alloc(m->nnz ? m->nnz : 1);
-    m->len32.alloc(m->nnz ? m->nnz : 1);
-    if (want64) m->len64.alloc(m->nnz ? m->nnz : 1);
+    alloc_padded(m->idx, m->nnz, st);
+    alloc_padded(m->len32, m->nnz, st);
+    if (want64) alloc_padded(m->len64, m->nnz, st);
     k_trace_fill<<<grid_of(m->nrows, 64), 64, 0, st>>>(a, m->nrows, m->rowptr.p, m->idx.p, m->len32.p,
                                                        want64 ? m->len64.p : nullptr);
     MI_HIP(hipGetLastError());
