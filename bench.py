@@ -68,6 +68,7 @@ def main():
     print(f"[bench] tracing C2 geometry and generating the phantom", file=sys.stderr, flush=True)
     t = time.perf_counter()
     nnz = ctx.prepare(g, 12)
+    tune = ctx.autotune(g, 12)
     _, sino = configs.make_case(ctx, g, 2e4, seed=2)
     print(f"[bench] nnz {nnz}, setup {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
     p = configs.c2_params()
@@ -116,7 +117,8 @@ def main():
         "data": "synthetic: seeded 10-ellipse phantom, Poisson noise I0=2e4, log transform",
         "config": {"workload": "C2 plain per-job form: 512x512 fan beam (0.7 mm), 720 views, 736 bins (1.0 mm), "
                                "SOD 541 mm, SDD 949 mm, 12 ordered subsets, alpha 1e-3, beta 1e-4, eps 1e-2",
-                   "rays": rays, "nnz": nnz, "subsets": 12},
+                   "rays": rays, "nnz": nnz, "subsets": 12,
+                   "tuning": {"fp_vec": tune.fp_vec, "bp_lanes": tune.bp_lanes}},
         "rays_per_s": rays * args.steps / elapsed,
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,

@@ -130,6 +130,20 @@ int mi_trace_ray(mi_context *ctx, const mi_geometry *g, int view, int bin,
 int mi_matrix_export(mi_context *ctx, const mi_geometry *g, int64_t *rowptr, int32_t *ind, double *len,
                      int64_t nnz);
 
+/* Launch shapes of the two streaming kernels for one (geometry, subsets)
+ * matrix.  mi_autotune times every candidate on the device and keeps the
+ * fastest; mi_set_tuning installs a stored choice (medimage_amd/tuning.json
+ * is the project's offline table). */
+typedef struct mi_tuning {
+    int32_t fp_vec;         /* forward projector: 1 = four consecutive entries
+                               per lane (16-byte loads), 0 = lane-strided      */
+    int32_t bp_lanes;       /* back projector lanes per pixel: 4..64           */
+    double  fp_us, bp_us;   /* measured per-subset launch times (autotune)     */
+} mi_tuning;
+int mi_autotune(mi_context *ctx, const mi_geometry *g, int n_subsets, mi_tuning *out);
+int mi_set_tuning(mi_context *ctx, const mi_geometry *g, int n_subsets, const mi_tuning *t);
+int mi_get_tuning(mi_context *ctx, const mi_geometry *g, int n_subsets, mi_tuning *out);
+
 /* ---- projectors (replace A / AStar) --------------------------------------
  * forward:     sino[view][bin] = sum over the ray of img[x][y] * len
  *              (origin/GoldenC/src/proj.c:14-204, msbeam proj.c:14-32)

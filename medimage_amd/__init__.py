@@ -61,6 +61,10 @@ class Report(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("fp_vec", C.c_int32), ("bp_lanes", C.c_int32), ("fp_us", C.c_double), ("bp_us", C.c_double)]
+
+
 class KernelStats(C.Structure):
     _fields_ = [("launches", C.c_int64), ("total_ms", C.c_double), ("bytes", C.c_double)]
 
@@ -111,6 +115,9 @@ def _load():
         "mi_context_trim": [P],
         "mi_geometry_prepare": [P, G, C.c_int, C.POINTER(C.c_int64)],
         "mi_trace_ray": [P, G, C.c_int, C.c_int, P, P, C.c_int, C.POINTER(C.c_int)],
+        "mi_autotune": [P, G, C.c_int, C.POINTER(Tuning)],
+        "mi_set_tuning": [P, G, C.c_int, C.POINTER(Tuning)],
+        "mi_get_tuning": [P, G, C.c_int, C.POINTER(Tuning)],
         "mi_matrix_export": [P, G, P, P, P, C.c_int64],
         "mi_forward": [P, G, P, P],
         "mi_backproject": [P, G, P, P],
@@ -194,6 +201,20 @@ class Context:
         nnz = C.c_int64()
         _check(self._lib.mi_geometry_prepare(self._h, C.byref(g), n_subsets, C.byref(nnz)))
         return nnz.value
+
+    def autotune(self, g: Geometry, n_subsets: int = 1) -> Tuning:
+        t = Tuning()
+        _check(self._lib.mi_autotune(self._h, C.byref(g), n_subsets, C.byref(t)))
+        return t
+
+    def set_tuning(self, g: Geometry, n_subsets: int, fp_vec: int, bp_lanes: int):
+        t = Tuning(fp_vec, bp_lanes, 0.0, 0.0)
+        _check(self._lib.mi_set_tuning(self._h, C.byref(g), n_subsets, C.byref(t)))
+
+    def get_tuning(self, g: Geometry, n_subsets: int = 1) -> Tuning:
+        t = Tuning()
+        _check(self._lib.mi_get_tuning(self._h, C.byref(g), n_subsets, C.byref(t)))
+        return t
 
     def trace_ray(self, g: Geometry, view: int, bin: int, cap: int = 8192):
         ind = np.zeros(cap, np.int32)

@@ -276,6 +276,37 @@ int mi_geometry_prepare(mi_context *ctx, const mi_geometry *g, int n_subsets, in
     });
 }
 
+int mi_autotune(mi_context *ctx, const mi_geometry *g, int n_subsets, mi_tuning *out) {
+    return guarded([&] {
+        check_ctx(ctx);
+        MI_REQUIRE(g, MI_E_ARG, "null geometry");
+        mi_tuning t = autotune(ctx, *get_matrix(ctx, *g, n_subsets, false));
+        if (out) *out = t;
+    });
+}
+
+int mi_set_tuning(mi_context *ctx, const mi_geometry *g, int n_subsets, const mi_tuning *t) {
+    return guarded([&] {
+        check_ctx(ctx);
+        MI_REQUIRE(g && t, MI_E_ARG, "null argument");
+        MI_REQUIRE(t->fp_vec == 0 || t->fp_vec == 1, MI_E_ARG, "fp_vec must be 0 or 1");
+        MI_REQUIRE(t->bp_lanes == 4 || t->bp_lanes == 8 || t->bp_lanes == 16 || t->bp_lanes == 32 ||
+                       t->bp_lanes == 64,
+                   MI_E_ARG, "bp_lanes must be 4, 8, 16, 32 or 64");
+        get_matrix(ctx, *g, n_subsets, false)->tune = *t;
+    });
+}
+
+int mi_get_tuning(mi_context *ctx, const mi_geometry *g, int n_subsets, mi_tuning *out) {
+    return guarded([&] {
+        check_ctx(ctx);
+        MI_REQUIRE(g && out, MI_E_ARG, "null argument");
+        auto m = get_matrix(ctx, *g, n_subsets, false);
+        *out = m->tune;
+        if (!out->bp_lanes) out->bp_lanes = bp_lanes(*m);
+    });
+}
+
 int mi_trace_ray(mi_context *ctx, const mi_geometry *g, int view, int bin, int32_t *ind, double *len, int cap,
                  int *numb) {
     return guarded([&] {

@@ -78,7 +78,7 @@ __device__ __forceinline__ T ld(const T *a, int i, int j, int nx, int ny, T pad)
 // walk starts at the 4-aligned entry below the row start; entries outside
 // the row are loaded but weighted by zero), so a trip covers 256 entries and
 // issues four independent image gathers per lane.
-template <typename T, int MODE>
+template <typename T, int MODE, int VEC>
 __global__ __launch_bounds__(kBlock) void k_fp(const int64_t *__restrict__ rowptr,
                                                const uint32_t *__restrict__ idx, const T *__restrict__ len,
                                                int64_t row0, int64_t nrows, const T *__restrict__ x,
@@ -94,17 +94,40 @@ __global__ __launch_bounds__(kBlock) void k_fp(const int64_t *__restrict__ rowpt
         const int64_t rr = row0 + r;
         const int64_t b = rowptr[rr], e = rowptr[rr + 1];
         Acc<T> acc;
-        for (int64_t k = (b & ~(int64_t)3) + 4 * lane; k < e; k += 4 * kWave) {
-            uint32_t i4[4];
-            Vec4<T> l4;
-            load_idx4(idx + k, i4);
-            l4.load(len + k);
-            T xv[4];
+        if (VEC) {
+            for (int64_t k = (b & ~(int64_t)3) + 4 * lane; k < e; k += 4 * kWave) {
+                uint32_t i4[4];
+                Vec4<T> l4;
+                load_idx4(idx + k, i4);
+                l4.load(len + k);
+                T xv[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) xv[j] = x[i4[j]];
+                for (int j = 0; j < 4; ++j) xv[j] = x[i4[j]];
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-                if (k + j >= b && k + j < e) acc.add(xv[j] * l4.v[j]);
+                for (int j = 0; j < 4; ++j)
+                    if (k + j >= b && k + j < e) acc.add(xv[j] * l4.v[j]);
+            }
+        } else {
+            // Lane-strided: every load instruction is one contiguous 256-byte
+            // (or 512-byte) segment and every gather instruction covers 64
+            // consecutive pixels of the ray; four entries per lane in flight.
+            // Entries past the row end read the padded tail or the next row
+            // (valid indices) and are weighted by zero.
+            for (int64_t k = b + lane; k < e; k += 4 * kWave) {
+                uint32_t i4[4];
+                T l4[4], xv[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int64_t kk = (k + j * kWave < e) ? k + j * kWave : k;
+                    i4[j] = idx[kk];
+                    l4[j] = len[kk];
+                }
+#pragma unroll
+                for (int j = 0; j < 4; ++j) xv[j] = x[i4[j]];
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (k + j * kWave < e) acc.add(xv[j] * l4[j]);
+            }
         }
         const double s = group_sum<kWave>(acc.value());
         if (lane == 0) {
@@ -467,11 +490,14 @@ unsigned grid_bp(int64_t npix, int lanes) { return cap_grid((npix + kBlock / lan
 
 template <typename T>
 unsigned launch_fp(hipStream_t st, int mode, const CsrView &A, int64_t row0, int64_t nrows, const T *x, const T *g,
-                   const T *q, T *out, double *part, const int32_t *skip) {
+                   const T *q, T *out, double *part, const int32_t *skip, int vec) {
     const unsigned grid = grid_rows(nrows);
     const T *len = (const T *)A.len;
-#define MI_FP_CASE(M) \
-    case M: k_fp<T, M><<<grid, kBlock, 0, st>>>(A.rowptr, A.idx, len, row0, nrows, x, g, q, out, part, skip); break;
+#define MI_FP_CASE(M)                                                                                          \
+    case M:                                                                                                    \
+        if (vec) k_fp<T, M, 1><<<grid, kBlock, 0, st>>>(A.rowptr, A.idx, len, row0, nrows, x, g, q, out, part, skip); \
+        else k_fp<T, M, 0><<<grid, kBlock, 0, st>>>(A.rowptr, A.idx, len, row0, nrows, x, g, q, out, part, skip); \
+        break;
     switch (mode) {
         MI_FP_CASE(FP_WRITE)
         MI_FP_CASE(FP_RESID)
@@ -618,7 +644,7 @@ void launch_scatter_rows(hipStream_t st, int64_t n, const int64_t *perm, const T
 
 #define MI_INST(T)                                                                                              \
     template unsigned launch_fp<T>(hipStream_t, int, const CsrView &, int64_t, int64_t, const T *, const T *,   \
-                                   const T *, T *, double *, const int32_t *);                                  \
+                                   const T *, T *, double *, const int32_t *, int);                             \
     template unsigned launch_bp_grad<T>(hipStream_t, const CscView &, int, int, const T *, double, const T *,   \
                                         const T *, double, double, T *, double *, int, const int32_t *);        \
     template unsigned launch_nv<T>(hipStream_t, int, int, const T *, const T *, double, double *,               \

@@ -67,9 +67,11 @@ unsigned grid_bp(int64_t npix, int lanes);
 // Each returns its grid size.
 
 // Forward projection of rows [row0, row0 + nrows); g/q/out are indexed by row.
+// vec: 1 = each lane takes four consecutive entries (16-byte loads);
+// 0 = lane-strided 4-byte loads, four entries in flight per lane.
 template <typename T>
 unsigned launch_fp(hipStream_t st, int mode, const CsrView &A, int64_t row0, int64_t nrows, const T *x, const T *g,
-                   const T *q, T *out, double *part, const int32_t *skip = nullptr);
+                   const T *q, T *out, double *part, const int32_t *skip = nullptr, int vec = 0);
 
 // d = scale * (A_s^T r) + alpha * R(u, v) over an nx x ny image (R is the
 // GoldenC image-gradient regulariser; skipped when u == nullptr); partial

@@ -44,6 +44,7 @@ CscView csc_view(const Matrix &m, int s, bool f64) {
 // entries per trip; aim at about three trips over the subset's mean column
 // length, within [4, 64].
 int bp_lanes(const Matrix &m) {
+    if (m.tune.bp_lanes) return m.tune.bp_lanes;
     const double mean = (double)m.nnz / ((double)m.S * (double)m.npix);
     int l = 4;
     while (l < 64 && l * 12 < mean) l *= 2;
@@ -127,7 +128,8 @@ void os_pass(mi_context *ctx, const Matrix &m, JobState<T> &js, const mi_at_para
         f.alpha = p.alpha;
         {
             LaunchTimer lt(ctx, MI_K_FP, fp_bytes(m, nr, sizeof(T), FP_RESID));
-            f.n[3] = (int)launch_fp<T>(st, FP_RESID, A, row0, nr, js.u.p, js.g.p, nullptr, js.r.p, rb.slot(3), skip);
+            f.n[3] = (int)launch_fp<T>(st, FP_RESID, A, row0, nr, js.u.p, js.g.p, nullptr, js.r.p, rb.slot(3), skip,
+                                       m.tune.fp_vec);
         }
         {
             LaunchTimer lt(ctx, MI_K_BP, bp_bytes(m, nr, sizeof(T), true));
@@ -136,7 +138,8 @@ void os_pass(mi_context *ctx, const Matrix &m, JobState<T> &js, const mi_at_para
         }
         {
             LaunchTimer lt(ctx, MI_K_FPSQ, fp_bytes(m, nr, sizeof(T), FP_SQ));
-            f.n[1] = (int)launch_fp<T>(st, FP_SQ, A, row0, nr, js.d.p, nullptr, nullptr, nullptr, rb.slot(1), skip);
+            f.n[1] = (int)launch_fp<T>(st, FP_SQ, A, row0, nr, js.d.p, nullptr, nullptr, nullptr, rb.slot(1), skip,
+                                       m.tune.fp_vec);
         }
         {
             LaunchTimer lt(ctx, MI_K_STEP, (double)js.npix * sizeof(T) * 5);
@@ -346,6 +349,48 @@ template void gc_edge_step<float>(mi_context *, JobState<float> &, double, doubl
 template void gc_edge_step<double>(mi_context *, JobState<double> &, double, double, double, int);
 template void load_sino<float>(hipStream_t, const Matrix &, const float *, float *);
 template void load_sino<double>(hipStream_t, const Matrix &, const double *, double *);
+
+// Times every launch shape of the two streaming kernels on subset 0 of the
+// matrix (FP32, random data, HIP events) and keeps the fastest.
+mi_tuning autotune(mi_context *ctx, Matrix &m) {
+    const hipStream_t st = ctx->stream;
+    DevBuf<float> x(m.npix), r(m.nrows), d(m.npix);
+    fill<float>(st, x.p, m.npix, 0.5f);
+    fill<float>(st, r.p, m.nrows, 0.25f);
+    const int64_t nr = m.sub_row0[1];
+    hipEvent_t a, b;
+    MI_HIP(hipEventCreate(&a));
+    MI_HIP(hipEventCreate(&b));
+    auto time_us = [&](auto &&launch) {
+        launch();  // warm
+        MI_HIP(hipEventRecord(a, st));
+        const int reps = 5;
+        for (int i = 0; i < reps; ++i) launch();
+        MI_HIP(hipEventRecord(b, st));
+        MI_HIP(hipEventSynchronize(b));
+        float ms = 0.f;
+        MI_HIP(hipEventElapsedTime(&ms, a, b));
+        return 1e3 * ms / reps;
+    };
+    mi_tuning best{0, 4, 1e30, 1e30};
+    const CsrView A = csr_view(m, false);
+    for (int vec = 0; vec <= 1; ++vec) {
+        const double us = time_us([&] { launch_fp<float>(st, FP_SQ, A, 0, nr, x.p, nullptr, nullptr, nullptr, nullptr,
+                                                          nullptr, vec); });
+        if (us < best.fp_us) { best.fp_us = us; best.fp_vec = vec; }
+    }
+    for (int lanes = 4; lanes <= 64; lanes *= 2) {
+        const double us = time_us([&] {
+            launch_bp_grad<float>(st, csc_view(m, 0, false), m.g.nx, m.g.ny, r.p, 1.0, nullptr, nullptr, 0.0, 0.0,
+                                  d.p, nullptr, lanes);
+        });
+        if (us < best.bp_us) { best.bp_us = us; best.bp_lanes = lanes; }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    m.tune = best;
+    return best;
+}
 
 template struct JobState<float>;
 template struct JobState<double>;

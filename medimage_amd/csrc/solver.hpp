@@ -64,6 +64,10 @@ int gc_image_step(mi_context *ctx, const Matrix &m, JobState<T> &js, double alph
 template <typename T>
 void gc_edge_step(mi_context *ctx, JobState<T> &js, double alpha, double beta, double eps, int iter_edge);
 
+// Measures the launch shapes of the streaming kernels for m and stores the
+// fastest in m.tune.
+mi_tuning autotune(mi_context *ctx, Matrix &m);
+
 // Host <-> device sinogram transfer in the caller's [view][bin] order.
 template <typename T>
 void load_sino(hipStream_t st, const Matrix &m, const T *host, T *rows);
