@@ -118,7 +118,8 @@ def main():
         "config": {"workload": "C2 plain per-job form: 512x512 fan beam (0.7 mm), 720 views, 736 bins (1.0 mm), "
                                "SOD 541 mm, SDD 949 mm, 12 ordered subsets, alpha 1e-3, beta 1e-4, eps 1e-2",
                    "rays": rays, "nnz": nnz, "subsets": 12,
-                   "tuning": {"fp_vec": tune.fp_vec, "bp_lanes": tune.bp_lanes}},
+                   "tuning": {"fp_vec": tune.fp_vec, "bp_lanes": tune.bp_lanes, "fp_grid": tune.fp_grid,
+                              "bp_grid": tune.bp_grid, "fp_us": tune.fp_us, "bp_us": tune.bp_us}},
         "rays_per_s": rays * args.steps / elapsed,
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,

@@ -138,6 +138,8 @@ typedef struct mi_tuning {
     int32_t fp_vec;         /* forward projector: 1 = four consecutive entries
                                per lane (16-byte loads), 0 = lane-strided      */
     int32_t bp_lanes;       /* back projector lanes per pixel: 4..64           */
+    int32_t fp_grid;        /* forward projector grid cap (blocks), 0 default  */
+    int32_t bp_grid;        /* back projector grid cap (blocks), 0 default     */
     double  fp_us, bp_us;   /* measured per-subset launch times (autotune)     */
 } mi_tuning;
 int mi_autotune(mi_context *ctx, const mi_geometry *g, int n_subsets, mi_tuning *out);

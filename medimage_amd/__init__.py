@@ -62,7 +62,8 @@ class Report(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("fp_vec", C.c_int32), ("bp_lanes", C.c_int32), ("fp_us", C.c_double), ("bp_us", C.c_double)]
+    _fields_ = [("fp_vec", C.c_int32), ("bp_lanes", C.c_int32), ("fp_grid", C.c_int32), ("bp_grid", C.c_int32),
+                ("fp_us", C.c_double), ("bp_us", C.c_double)]
 
 
 class KernelStats(C.Structure):
@@ -207,8 +208,8 @@ class Context:
         _check(self._lib.mi_autotune(self._h, C.byref(g), n_subsets, C.byref(t)))
         return t
 
-    def set_tuning(self, g: Geometry, n_subsets: int, fp_vec: int, bp_lanes: int):
-        t = Tuning(fp_vec, bp_lanes, 0.0, 0.0)
+    def set_tuning(self, g: Geometry, n_subsets: int, fp_vec: int, bp_lanes: int, fp_grid: int = 0, bp_grid: int = 0):
+        t = Tuning(fp_vec, bp_lanes, fp_grid, bp_grid, 0.0, 0.0)
         _check(self._lib.mi_set_tuning(self._h, C.byref(g), n_subsets, C.byref(t)))
 
     def get_tuning(self, g: Geometry, n_subsets: int = 1) -> Tuning:

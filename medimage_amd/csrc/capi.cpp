@@ -293,6 +293,8 @@ int mi_set_tuning(mi_context *ctx, const mi_geometry *g, int n_subsets, const mi
         MI_REQUIRE(t->bp_lanes == 4 || t->bp_lanes == 8 || t->bp_lanes == 16 || t->bp_lanes == 32 ||
                        t->bp_lanes == 64,
                    MI_E_ARG, "bp_lanes must be 4, 8, 16, 32 or 64");
+        MI_REQUIRE(t->fp_grid >= 0 && t->fp_grid <= kMaxGrid && t->bp_grid >= 0 && t->bp_grid <= kMaxGrid, MI_E_ARG,
+                   "grid caps must be in [0, 8192]");
         get_matrix(ctx, *g, n_subsets, false)->tune = *t;
     });
 }

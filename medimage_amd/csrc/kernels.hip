@@ -16,9 +16,9 @@ namespace {
 // full residency of the chip at 8 waves per SIMD), rounded to a multiple of
 // the 8 XCDs so that xcd_block() is a permutation.
 constexpr int kStreamGrid = 2048;
-inline unsigned cap_grid(int64_t blocks) {
+inline unsigned cap_grid(int64_t blocks, int cap = kStreamGrid) {
     if (blocks < 1) blocks = 1;
-    if (blocks > kStreamGrid) blocks = kStreamGrid;
+    if (blocks > cap) blocks = cap;
     if (blocks > 8) blocks = (blocks + 7) / 8 * 8;
     return (unsigned)blocks;
 }
@@ -484,14 +484,16 @@ __global__ void k_scatter_rows(int64_t n, const int64_t *__restrict__ perm, cons
 // ---------------------------------------------------------------------------
 // Launchers.
 
-unsigned grid_rows(int64_t nrows) { return cap_grid((nrows + 3) / 4); }
+unsigned grid_rows(int64_t nrows, int cap) { return cap_grid((nrows + 3) / 4, cap ? cap : kStreamGrid); }
 unsigned grid_pixels(int64_t n) { return cap_grid((n + kBlock - 1) / kBlock); }
-unsigned grid_bp(int64_t npix, int lanes) { return cap_grid((npix + kBlock / lanes - 1) / (kBlock / lanes)); }
+unsigned grid_bp(int64_t npix, int lanes, int cap) {
+    return cap_grid((npix + kBlock / lanes - 1) / (kBlock / lanes), cap ? cap : kStreamGrid);
+}
 
 template <typename T>
 unsigned launch_fp(hipStream_t st, int mode, const CsrView &A, int64_t row0, int64_t nrows, const T *x, const T *g,
-                   const T *q, T *out, double *part, const int32_t *skip, int vec) {
-    const unsigned grid = grid_rows(nrows);
+                   const T *q, T *out, double *part, const int32_t *skip, int vec, int grid_cap) {
+    const unsigned grid = grid_rows(nrows, grid_cap);
     const T *len = (const T *)A.len;
 #define MI_FP_CASE(M)                                                                                          \
     case M:                                                                                                    \
@@ -513,10 +515,11 @@ unsigned launch_fp(hipStream_t st, int mode, const CsrView &A, int64_t row0, int
 
 template <typename T>
 unsigned launch_bp_grad(hipStream_t st, const CscView &At, int nx, int ny, const T *r, double scale, const T *u,
-                        const T *v, double alpha, double eps, T *d, double *part, int lanes, const int32_t *skip) {
+                        const T *v, double alpha, double eps, T *d, double *part, int lanes, const int32_t *skip,
+                        int grid_cap) {
     const int64_t npix = (int64_t)nx * ny;
     const T *len = (const T *)At.len;
-    const unsigned grid = grid_bp(npix, lanes);
+    const unsigned grid = grid_bp(npix, lanes, grid_cap);
 #define MI_BP_CASE(L)                                                                                          \
     case L:                                                                                                    \
         k_bp_grad<T, L><<<grid, kBlock, 0, st>>>(At.colptr, At.crow, len, nx, ny, r, scale, u, v, alpha, eps, d, \
@@ -644,9 +647,9 @@ void launch_scatter_rows(hipStream_t st, int64_t n, const int64_t *perm, const T
 
 #define MI_INST(T)                                                                                              \
     template unsigned launch_fp<T>(hipStream_t, int, const CsrView &, int64_t, int64_t, const T *, const T *,   \
-                                   const T *, T *, double *, const int32_t *, int);                             \
+                                   const T *, T *, double *, const int32_t *, int, int);                        \
     template unsigned launch_bp_grad<T>(hipStream_t, const CscView &, int, int, const T *, double, const T *,   \
-                                        const T *, double, double, T *, double *, int, const int32_t *);        \
+                                        const T *, double, double, T *, double *, int, const int32_t *, int);   \
     template unsigned launch_nv<T>(hipStream_t, int, int, const T *, const T *, double, double *,               \
                                    const int32_t *);                                                            \
     template unsigned launch_energy_reg<T>(hipStream_t, int, int, const T *, const T *, double, double,         \

@@ -57,9 +57,10 @@ struct FinArgs {
 };
 
 // Grid sizes the producers use (the number of partials they write).
-unsigned grid_rows(int64_t nrows);
+// grid_cap 0 selects the default cap (2048 blocks).
+unsigned grid_rows(int64_t nrows, int grid_cap = 0);
 unsigned grid_pixels(int64_t n);
-unsigned grid_bp(int64_t npix, int lanes);
+unsigned grid_bp(int64_t npix, int lanes, int grid_cap = 0);
 
 // Every producer takes an optional device flag `skip`: when *skip != 0 the
 // kernel returns at once (a converged job's queued passes then cost one
@@ -71,7 +72,8 @@ unsigned grid_bp(int64_t npix, int lanes);
 // 0 = lane-strided 4-byte loads, four entries in flight per lane.
 template <typename T>
 unsigned launch_fp(hipStream_t st, int mode, const CsrView &A, int64_t row0, int64_t nrows, const T *x, const T *g,
-                   const T *q, T *out, double *part, const int32_t *skip = nullptr, int vec = 0);
+                   const T *q, T *out, double *part, const int32_t *skip = nullptr, int vec = 0,
+                   int grid_cap = 0);
 
 // d = scale * (A_s^T r) + alpha * R(u, v) over an nx x ny image (R is the
 // GoldenC image-gradient regulariser; skipped when u == nullptr); partial
@@ -79,7 +81,7 @@ unsigned launch_fp(hipStream_t st, int mode, const CsrView &A, int64_t row0, int
 template <typename T>
 unsigned launch_bp_grad(hipStream_t st, const CscView &At, int nx, int ny, const T *r, double scale, const T *u,
                         const T *v, double alpha, double eps, T *d, double *part, int lanes,
-                        const int32_t *skip = nullptr);
+                        const int32_t *skip = nullptr, int grid_cap = 0);
 
 // partial sum (v^2 + 0.01 eps^2)((p - p_up)^2 + (p - p_left)^2), p zero
 // padded (origin/GoldenC/src/main.c:152-158).

@@ -53,7 +53,7 @@ struct Matrix {
     std::vector<int> vlist;          // subset-ordered view list
     std::vector<int64_t> sub_row0;   // S+1 row offsets
     bool has64 = false;
-    mi_tuning tune{0, 0, 0.0, 0.0};  // bp_lanes 0: not set (heuristic)
+    mi_tuning tune{0, 0, 0, 0, 0.0, 0.0};  // bp_lanes 0: not set (heuristic)
 
     DevBuf<int> d_vlist;
     DevBuf<int64_t> rowptr;          // nrows + 1

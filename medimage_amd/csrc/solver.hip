@@ -129,17 +129,17 @@ void os_pass(mi_context *ctx, const Matrix &m, JobState<T> &js, const mi_at_para
         {
             LaunchTimer lt(ctx, MI_K_FP, fp_bytes(m, nr, sizeof(T), FP_RESID));
             f.n[3] = (int)launch_fp<T>(st, FP_RESID, A, row0, nr, js.u.p, js.g.p, nullptr, js.r.p, rb.slot(3), skip,
-                                       m.tune.fp_vec);
+                                       m.tune.fp_vec, m.tune.fp_grid);
         }
         {
             LaunchTimer lt(ctx, MI_K_BP, bp_bytes(m, nr, sizeof(T), true));
             f.n[0] = (int)launch_bp_grad<T>(st, csc_view(m, s, f64), js.nx, js.ny, js.r.p, S, js.u.p, js.v.p,
-                                            p.alpha, p.epsilon, js.d.p, rb.slot(0), lanes, skip);
+                                            p.alpha, p.epsilon, js.d.p, rb.slot(0), lanes, skip, m.tune.bp_grid);
         }
         {
             LaunchTimer lt(ctx, MI_K_FPSQ, fp_bytes(m, nr, sizeof(T), FP_SQ));
             f.n[1] = (int)launch_fp<T>(st, FP_SQ, A, row0, nr, js.d.p, nullptr, nullptr, nullptr, rb.slot(1), skip,
-                                       m.tune.fp_vec);
+                                       m.tune.fp_vec, m.tune.fp_grid);
         }
         {
             LaunchTimer lt(ctx, MI_K_STEP, (double)js.npix * sizeof(T) * 5);
@@ -372,20 +372,24 @@ mi_tuning autotune(mi_context *ctx, Matrix &m) {
         MI_HIP(hipEventElapsedTime(&ms, a, b));
         return 1e3 * ms / reps;
     };
-    mi_tuning best{0, 4, 1e30, 1e30};
+    mi_tuning best{0, 4, 0, 0, 1e30, 1e30};
     const CsrView A = csr_view(m, false);
-    for (int vec = 0; vec <= 1; ++vec) {
-        const double us = time_us([&] { launch_fp<float>(st, FP_SQ, A, 0, nr, x.p, nullptr, nullptr, nullptr, nullptr,
-                                                          nullptr, vec); });
-        if (us < best.fp_us) { best.fp_us = us; best.fp_vec = vec; }
-    }
-    for (int lanes = 4; lanes <= 64; lanes *= 2) {
-        const double us = time_us([&] {
-            launch_bp_grad<float>(st, csc_view(m, 0, false), m.g.nx, m.g.ny, r.p, 1.0, nullptr, nullptr, 0.0, 0.0,
-                                  d.p, nullptr, lanes);
-        });
-        if (us < best.bp_us) { best.bp_us = us; best.bp_lanes = lanes; }
-    }
+    const int grids[3] = {2048, 4096, 8192};
+    for (int vec = 0; vec <= 1; ++vec)
+        for (int gc : grids) {
+            const double us = time_us([&] {
+                launch_fp<float>(st, FP_SQ, A, 0, nr, x.p, nullptr, nullptr, nullptr, nullptr, nullptr, vec, gc);
+            });
+            if (us < best.fp_us) { best.fp_us = us; best.fp_vec = vec; best.fp_grid = gc; }
+        }
+    for (int lanes = 4; lanes <= 64; lanes *= 2)
+        for (int gc : grids) {
+            const double us = time_us([&] {
+                launch_bp_grad<float>(st, csc_view(m, 0, false), m.g.nx, m.g.ny, r.p, 1.0, nullptr, nullptr, 0.0,
+                                      0.0, d.p, nullptr, lanes, nullptr, gc);
+            });
+            if (us < best.bp_us) { best.bp_us = us; best.bp_lanes = lanes; best.bp_grid = gc; }
+        }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     m.tune = best;
