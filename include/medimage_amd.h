@@ -130,14 +130,16 @@ int mi_trace_ray(mi_context *ctx, const mi_geometry *g, int view, int bin,
 int mi_matrix_export(mi_context *ctx, const mi_geometry *g, int64_t *rowptr, int32_t *ind, double *len,
                      int64_t nnz);
 
-/* Launch shapes of the two streaming kernels for one (geometry, subsets)
+/* Launch shapes of the streaming kernels for one (geometry, subsets)
  * matrix.  mi_autotune times every candidate on the device and keeps the
- * fastest; mi_set_tuning installs a stored choice (medimage_amd/tuning.json
- * is the project's offline table). */
+ * fastest; mi_set_tuning installs a stored choice.  The AT engine (jobs and
+ * batches) reads bp_lanes only; fp_vec / fp_grid / bp_grid shape the single
+ * projector calls (mi_forward, mi_backproject and the GoldenC steps). */
 typedef struct mi_tuning {
     int32_t fp_vec;         /* forward projector: 1 = four consecutive entries
                                per lane (16-byte loads), 0 = lane-strided      */
-    int32_t bp_lanes;       /* back projector lanes per pixel: 4..64           */
+    int32_t bp_lanes;       /* back projector lanes per pixel: 4..64 (the AT
+                               engine uses at least 8)                         */
     int32_t fp_grid;        /* forward projector grid cap (blocks), 0 default  */
     int32_t bp_grid;        /* back projector grid cap (blocks), 0 default     */
     double  fp_us, bp_us;   /* measured per-subset launch times (autotune)     */
@@ -198,6 +200,59 @@ int mi_job_synchronize(mi_job *job);
 int mi_job_report(mi_job *job, mi_report *rep);
 int mi_job_read(mi_job *job, void *u, void *v);
 int mi_job_destroy(mi_job *job);
+
+/* ---- batches: many jobs per launch, continuous batching ---------------------
+ * A batch runs the ordered-subset AT iteration of many jobs at once: jobs
+ * that share a geometry (and so a system matrix) are packed into units of
+ * K = 1, 2, 4 or 8 jobs whose state is interleaved, and each pass -- all
+ * subset steps of all units, the median and edge steps, the convergence
+ * test -- is one sequence of launches that every unit shares, replayed from
+ * a hipGraph.  Jobs join and leave between passes only.  All jobs of a batch
+ * use the batch's subset count and precision; each keeps its own geometry
+ * and parameters (p->max_iters caps its passes, 0 = no cap; p->rel_tol
+ * stops it early).  A job's result does not depend on which other jobs
+ * share the batch: every reduction is cut into fixed chunks combined in a
+ * fixed order.  This replaces the reference's one-job-at-a-time drivers
+ * (origin/GoldenC/src/main.c:35-49, msbeam/CPU_OpenMP/src/main.c:364-394);
+ * the reference has no batched form. */
+typedef struct mi_batch mi_batch;
+int mi_batch_create(mi_context *ctx, int n_subsets, int precision, mi_batch **out);
+int mi_batch_destroy(mi_batch *b);
+enum mi_batch_option {
+    MI_BATCH_GRAPH = 0,       /* 1 (default): replay passes from a hipGraph  */
+    MI_BATCH_MIN_K = 1,       /* smallest K of a new unit (1, 2, 4, 8)       */
+    MI_BATCH_RESERVE_MB = 2   /* device memory left free at admission (1024) */
+};
+int mi_batch_set(mi_batch *b, int option, int64_t value);
+/* Queues a job (inputs are copied; u0 / v0 may be NULL).  It is admitted to
+ * the device at the next mi_batch_step while free device memory allows
+ * (idle cached matrices are evicted first, least recently used). */
+int mi_batch_submit(mi_batch *b, const mi_geometry *g, const mi_at_params *p, const void *sino,
+                    const void *u0, const void *v0, int64_t *job);
+/* Admits what fits, runs `passes` passes of every active job, waits, and
+ * retires finished jobs (their u, v are copied to host memory held by the
+ * batch and their device slot is freed).  Blocking. */
+int mi_batch_step(mi_batch *b, int passes, int *n_active, int *n_pending);
+/* Queues passes without admission or waiting; mi_batch_poll waits and retires. */
+int mi_batch_enqueue(mi_batch *b, int passes);
+int mi_batch_poll(mi_batch *b, int *n_active);
+enum mi_job_state { MI_JOB_PENDING = 0, MI_JOB_ACTIVE = 1, MI_JOB_DONE = 2, MI_JOB_FAILED = 3 };
+typedef struct mi_job_status {
+    int32_t state;          /* enum mi_job_state                               */
+    int32_t iterations;     /* AT passes completed                             */
+    int32_t converged;      /* rel_tol met                                     */
+    int32_t unit_k;         /* K of the job's unit while active                */
+    double  data_fidelity, rel_change;
+    double  t_submit, t_admit, t_done;   /* seconds on the batch's clock      */
+    int64_t nnz;
+} mi_job_status;
+int mi_batch_status(mi_batch *b, int64_t job, mi_job_status *out);
+/* u, v of a finished (or active) job, element type of the batch precision. */
+int mi_batch_read(mi_batch *b, int64_t job, void *u, void *v);
+/* Forgets a job: cancels it if pending or active, frees its results if done. */
+int mi_batch_release(mi_batch *b, int64_t job);
+/* K of each unit currently on the device (up to cap entries); *n = count. */
+int mi_batch_units(mi_batch *b, int32_t *k, int cap, int *n);
 
 /* ---- instrumentation ----------------------------------------------------- */
 enum mi_kernel_id {

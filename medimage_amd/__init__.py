@@ -27,6 +27,8 @@ MI_OK, MI_E_ARG, MI_E_DEVICE, MI_E_NOMEM, MI_E_STATE, MI_E_LIMIT = 0, -1, -2, -3
 MI_BEAM_FAN_FLAT, MI_BEAM_PARALLEL, MI_BEAM_GOLDENC = 0, 1, 2
 MI_FP32, MI_FP64 = 0, 1
 MI_K_FP, MI_K_BP, MI_K_FPSQ, MI_K_STEP, MI_K_EDGE = 0, 1, 2, 3, 4
+MI_BATCH_GRAPH, MI_BATCH_MIN_K, MI_BATCH_RESERVE_MB = 0, 1, 2
+MI_JOB_PENDING, MI_JOB_ACTIVE, MI_JOB_DONE, MI_JOB_FAILED = 0, 1, 2, 3
 
 
 class Geometry(C.Structure):
@@ -68,6 +70,12 @@ class Tuning(C.Structure):
 
 class KernelStats(C.Structure):
     _fields_ = [("launches", C.c_int64), ("total_ms", C.c_double), ("bytes", C.c_double)]
+
+
+class JobStatus(C.Structure):
+    _fields_ = [("state", C.c_int32), ("iterations", C.c_int32), ("converged", C.c_int32), ("unit_k", C.c_int32),
+                ("data_fidelity", C.c_double), ("rel_change", C.c_double), ("t_submit", C.c_double),
+                ("t_admit", C.c_double), ("t_done", C.c_double), ("nnz", C.c_int64)]
 
 
 def fan_geometry(n, pixel_mm, views, bins, bin_mm, sod=541.0, sdd=949.0, span_deg=360.0, det_offset=0.0):
@@ -137,6 +145,17 @@ def _load():
         "mi_job_report": [P, C.POINTER(Report)],
         "mi_job_read": [P, P, P],
         "mi_job_destroy": [P],
+        "mi_batch_create": [P, C.c_int, C.c_int, C.POINTER(C.c_void_p)],
+        "mi_batch_destroy": [P],
+        "mi_batch_set": [P, C.c_int, C.c_int64],
+        "mi_batch_submit": [P, G, C.POINTER(AtParams), P, P, P, C.POINTER(C.c_int64)],
+        "mi_batch_step": [P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+        "mi_batch_enqueue": [P, C.c_int],
+        "mi_batch_poll": [P, C.POINTER(C.c_int)],
+        "mi_batch_status": [P, C.c_int64, C.POINTER(JobStatus)],
+        "mi_batch_read": [P, C.c_int64, P, P],
+        "mi_batch_release": [P, C.c_int64],
+        "mi_batch_units": [P, P, C.c_int, C.POINTER(C.c_int)],
         "mi_stats_enable": [P, C.c_int],
         "mi_stats_get": [P, C.c_int, C.POINTER(KernelStats)],
         "mi_stats_reset": [P],
@@ -303,6 +322,9 @@ class Context:
     def job(self, g: Geometry, sino, p: AtParams, u0=None, v0=None):
         return Job(self, g, sino, p, u0, v0)
 
+    def batch(self, n_subsets: int, precision: int = MI_FP32, graph: bool = True, min_k: int = 1):
+        return Batch(self, n_subsets, precision, graph, min_k)
+
     # -- instrumentation --------------------------------------------------
     def stats_enable(self, on=True):
         _check(self._lib.mi_stats_enable(self._h, int(bool(on))))
@@ -358,6 +380,97 @@ class Job:
     def close(self):
         if getattr(self, "_h", None):
             self._lib.mi_job_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many ordered-subset jobs advanced together (mi_batch): units of K jobs
+    sharing a system matrix, one launch sequence per pass for all of them,
+    jobs admitted and retired between passes."""
+
+    def __init__(self, ctx: Context, n_subsets: int, precision: int = MI_FP32, graph: bool = True, min_k: int = 1):
+        self._lib = ctx._lib
+        self.ctx = ctx
+        self.S = n_subsets
+        self.dtype = np.float64 if precision == MI_FP64 else np.float32
+        self.precision = precision
+        h = C.c_void_p()
+        _check(self._lib.mi_batch_create(ctx._h, n_subsets, precision, C.byref(h)))
+        self._h = h
+        self._geom = {}
+        self.set(MI_BATCH_GRAPH, int(graph))
+        if min_k != 1:
+            self.set(MI_BATCH_MIN_K, min_k)
+
+    def set(self, option: int, value: int):
+        _check(self._lib.mi_batch_set(self._h, option, value))
+
+    def submit(self, g: Geometry, sino, p: AtParams, u0=None, v0=None) -> int:
+        sino = _arr(sino, self.dtype, (g.n_views, g.n_bins))
+        u0 = None if u0 is None else _arr(u0, self.dtype, (g.nx, g.ny))
+        v0 = None if v0 is None else _arr(v0, self.dtype, (g.nx, g.ny))
+        jid = C.c_int64()
+        _check(self._lib.mi_batch_submit(self._h, C.byref(g), C.byref(p), _ptr(sino),
+                                         None if u0 is None else _ptr(u0), None if v0 is None else _ptr(v0),
+                                         C.byref(jid)))
+        self._geom[jid.value] = (g.nx, g.ny)
+        return jid.value
+
+    def step(self, passes: int = 1):
+        a, q = C.c_int(), C.c_int()
+        _check(self._lib.mi_batch_step(self._h, passes, C.byref(a), C.byref(q)))
+        return a.value, q.value
+
+    def enqueue(self, passes: int):
+        _check(self._lib.mi_batch_enqueue(self._h, passes))
+
+    def poll(self) -> int:
+        a = C.c_int()
+        _check(self._lib.mi_batch_poll(self._h, C.byref(a)))
+        return a.value
+
+    def status(self, jid: int) -> JobStatus:
+        st = JobStatus()
+        _check(self._lib.mi_batch_status(self._h, jid, C.byref(st)))
+        return st
+
+    def read(self, jid: int):
+        nx, ny = self._geom[jid]
+        u = np.empty((nx, ny), self.dtype)
+        v = np.empty((nx, ny), self.dtype)
+        _check(self._lib.mi_batch_read(self._h, jid, _ptr(u), _ptr(v)))
+        return u, v
+
+    def release(self, jid: int):
+        _check(self._lib.mi_batch_release(self._h, jid))
+        self._geom.pop(jid, None)
+
+    def units(self):
+        k = np.zeros(64, np.int32)
+        n = C.c_int()
+        _check(self._lib.mi_batch_units(self._h, _ptr(k), 64, C.byref(n)))
+        return [int(x) for x in k[:n.value]]
+
+    def run(self, max_passes: int = 1 << 30, chunk: int = 1):
+        """Steps until every submitted job finished (each job stops at its own
+        max_iters / rel_tol); returns the number of passes run."""
+        n = 0
+        while n < max_passes:
+            a, q = self.step(chunk)
+            n += chunk
+            if a == 0 and q == 0:
+                break
+        return n
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.mi_batch_destroy(self._h)
             self._h = None
 
     def __del__(self):

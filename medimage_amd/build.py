@@ -18,7 +18,7 @@ CSRC = PKG / "csrc"
 OBJ = PKG / "_obj"
 LIB = PKG / "libmedimage_amd.so"
 
-SOURCES = ["trace.hip", "kernels.hip", "solver.hip", "capi.cpp"]
+SOURCES = ["trace.hip", "kernels.hip", "engine.hip", "solver.hip", "batch.cpp", "capi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -49,7 +49,7 @@ def _compile(src: str) -> Path:
 
 def build(verbose: bool = False) -> Path:
     OBJ.mkdir(exist_ok=True)
-    with ThreadPoolExecutor(max_workers=min(4, os.cpu_count() or 1)) as ex:
+    with ThreadPoolExecutor(max_workers=min(6, os.cpu_count() or 1)) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]

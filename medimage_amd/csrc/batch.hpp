@@ -1,0 +1,134 @@
+// Batches of device-resident ordered-subset AT jobs (batch.cpp).
+//
+// A batch owns units (K <= 8 jobs sharing one system matrix, state
+// interleaved) and runs one pass -- every subset step of every unit, then
+// the median and edge steps, then the convergence test -- as one sequence of
+// engine launches, captured once per batch composition into a hipGraph and
+// replayed.  Jobs join and leave only between passes (continuous batching):
+// submit() queues a job with host copies of its inputs, admit() moves
+// pending jobs onto the device while device memory allows, and jobs whose
+// device flag says done are retired (results copied to the host, slot
+// freed) by poll().
+#pragma once
+
+#include "engine.hpp"
+#include "runtime.hpp"
+
+#include <chrono>
+#include <deque>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace mi {
+
+enum BatchJobState { BJ_PENDING = 0, BJ_ACTIVE = 1, BJ_DONE = 2, BJ_FAILED = 3 };
+
+template <typename T>
+struct Unit {
+    std::shared_ptr<Matrix> m;
+    int uid = -1;                  // stable: job slots uid * kMaxK + j
+    int K = 1;
+    int lanes = 8;
+    int64_t jobs[kMaxK];           // job id per slot, -1 when free
+    DevBuf<T> g, r, u, v, d, w;
+    int used() const {
+        int n = 0;
+        for (int j = 0; j < K; ++j) n += jobs[j] >= 0;
+        return n;
+    }
+};
+
+template <typename T>
+struct BatchJob {
+    int64_t id = 0;
+    mi_geometry geom{};
+    mi_at_params p{};
+    int state = BJ_PENDING;
+    std::shared_ptr<Matrix> m;
+    int uid = -1, slot = -1;
+    std::vector<T> sino, u0, v0;   // inputs until admission
+    std::vector<T> u, v;           // results after retirement
+    Scalars last{};                // device scalars at retirement
+    double t_submit = 0, t_admit = 0, t_done = 0;   // seconds on the batch clock
+    std::string error;
+};
+
+template <typename T>
+struct Batch {
+    mi_context *ctx;
+    int S;
+    int min_k = 1;                 // smallest K of a new unit
+    bool use_graph = true;
+    bool auto_retire = true;
+    int64_t reserve_bytes = (int64_t)1 << 30;   // device memory kept free at admission
+    int64_t next_id = 1;
+    std::map<int64_t, BatchJob<T>> jobs;
+    std::deque<int64_t> pending;
+    std::vector<std::unique_ptr<Unit<T>>> units;   // by uid (kMaxUnits)
+
+    DevBuf<UnitDesc<T>> d_units;
+    DevBuf<JobParams> d_jp;
+    DevBuf<Scalars> d_sc;
+    DevBuf<double> d_part;
+    DevBuf<T> stage;
+    std::vector<JobParams> h_jp;
+    std::vector<Scalars> h_sc;
+    std::vector<int> live;         // uids in table order
+    BlockMap rows{}, bp{}, pix{};
+    EngArgs<T> args{};
+    int kmax = 1;
+    int edge_iters = 0;            // max iter_edge over live jobs
+    bool any_median = false;
+    bool dirty = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::chrono::steady_clock::time_point t0;
+    int64_t passes_run = 0;
+
+    Batch(mi_context *c, int n_subsets);
+    ~Batch();
+    Batch(const Batch &) = delete;
+    Batch &operator=(const Batch &) = delete;
+
+    double now() const;
+    int64_t submit(const mi_geometry &g, const mi_at_params &p, const T *sino, const T *u0, const T *v0);
+    // Admits pending jobs in arrival order while memory allows; returns the
+    // number admitted.  `require` makes a job that cannot be placed an error
+    // (used by the single-job API) instead of leaving it pending.
+    int admit(bool require = false);
+    // Queues `passes` passes of every live job (no wait).
+    void enqueue(int passes);
+    // Waits for the stream, reads every job's device scalars and retires
+    // the finished ones (when auto_retire).  Returns the number still active.
+    int poll();
+    // admit + enqueue + poll.
+    int step(int passes);
+    void cancel(int64_t id);
+    void release(int64_t id);
+    BatchJob<T> &job(int64_t id);
+    Scalars scalars(int64_t id);
+    void read(int64_t id, T *u, T *v);
+    int n_active() const;
+    int n_pending() const { return (int)pending.size(); }
+    int n_units() const { return (int)live.size(); }
+
+    // Internals (also used by the launch-shape tuner).
+    Unit<T> *place(BatchJob<T> &j, int group_size);
+    void load_job(BatchJob<T> &j);
+    void retire(BatchJob<T> &j);
+    void free_slot(BatchJob<T> &j);
+    void compact();
+    void rebuild();
+    void record_pass(bool timed);
+    void drop_graph();
+    void ensure_stage(int64_t n);
+};
+
+// Engine back-projector lanes per pixel of a matrix (8..64).
+int engine_lanes(const Matrix &m);
+// Times the engine's back projector per lanes candidate; stores the fastest.
+void autotune_lanes(mi_context *ctx, const std::shared_ptr<Matrix> &m);
+
+}  // namespace mi

@@ -1,5 +1,6 @@
 // C ABI of libmedimage_amd.so (include/medimage_amd.h): argument checks,
 // error translation, context and geometry cache, instrumentation.
+#include "batch.hpp"
 #include "solver.hpp"
 
 #include <cmath>
@@ -38,37 +39,6 @@ void hip_check(hipError_t e, const char *what, const char *file, int line) {
     throw Error(e == hipErrorOutOfMemory ? MI_E_NOMEM : MI_E_DEVICE, m);
 }
 
-template <typename T>
-void DevBuf<T>::alloc(size_t count) {
-    reset();
-    if (count == 0) return;
-    void *q = nullptr;
-    hipError_t e = hipMalloc(&q, count * sizeof(T));
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        fail(e == hipErrorOutOfMemory ? MI_E_NOMEM : MI_E_DEVICE,
-             "device allocation of " + std::to_string(count * sizeof(T)) + " bytes failed");
-    }
-    p = (T *)q;
-    n = count;
-}
-
-template <typename T>
-void DevBuf<T>::reset() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-}
-
-template struct DevBuf<float>;
-template struct DevBuf<double>;
-template struct DevBuf<int>;
-template struct DevBuf<int64_t>;
-template struct DevBuf<uint32_t>;
-template struct DevBuf<unsigned char>;
-template struct DevBuf<Scalars>;
-template struct DevBuf<unsigned long long>;
-
 void validate_geometry(const mi_geometry &g) {
     MI_REQUIRE(g.nx > 0 && g.ny > 0 && g.n_views > 0 && g.n_bins > 0, MI_E_ARG, "geometry sizes must be positive");
     MI_REQUIRE((int64_t)g.nx * g.ny < (int64_t)1 << 31, MI_E_LIMIT, "image too large");
@@ -102,11 +72,31 @@ GeomKey make_key(const mi_geometry &g, int S) {
 std::shared_ptr<Matrix> get_matrix(mi_context *ctx, const mi_geometry &g, int S, bool want64) {
     const GeomKey key = make_key(g, S);
     auto it = ctx->matrices.find(key);
-    if (it != ctx->matrices.end() && (!want64 || it->second->has64)) return it->second;
+    if (it != ctx->matrices.end() && (!want64 || it->second->has64)) {
+        it->second->last_use = ++ctx->tick;
+        return it->second;
+    }
     if (it != ctx->matrices.end()) ctx->matrices.erase(it);
     std::shared_ptr<Matrix> m(build_matrix(g, S, want64, ctx->stream).release());
+    m->last_use = ++ctx->tick;
     ctx->matrices[key] = m;
     return m;
+}
+
+int64_t evict_idle(mi_context *ctx, int64_t bytes) {
+    int64_t freed = 0;
+    while (bytes < 0 || freed < bytes) {
+        auto victim = ctx->matrices.end();
+        for (auto it = ctx->matrices.begin(); it != ctx->matrices.end(); ++it)
+            if (it->second.use_count() == 1 &&
+                (victim == ctx->matrices.end() || it->second->last_use < victim->second->last_use))
+                victim = it;
+        if (victim == ctx->matrices.end()) break;
+        MI_HIP(hipStreamSynchronize(ctx->stream));
+        freed += (int64_t)victim->second->bytes();
+        ctx->matrices.erase(victim);
+    }
+    return freed;
 }
 
 LaunchTimer::LaunchTimer(mi_context *c, int k, double by) : ctx(c), kernel(k), bytes(by) {
@@ -150,11 +140,22 @@ void stats_collect(mi_context *ctx) {
 
 using namespace mi;
 
+// The single-job API is a batch holding one job that is never retired, so
+// it runs the same engine code as the batched form.
 struct mi_job {
     mi_context *ctx = nullptr;
     int precision = MI_FP32;
-    std::unique_ptr<Job<float>> j32;
-    std::unique_ptr<Job<double>> j64;
+    std::unique_ptr<Batch<float>> b32;
+    std::unique_ptr<Batch<double>> b64;
+    int64_t id = 0;
+    double rel_tol = 0;
+};
+
+struct mi_batch {
+    mi_context *ctx = nullptr;
+    int precision = MI_FP32;
+    std::unique_ptr<Batch<float>> b32;
+    std::unique_ptr<Batch<double>> b64;
 };
 
 namespace {
@@ -163,6 +164,68 @@ mi_context *check_ctx(mi_context *ctx) {
     MI_REQUIRE(ctx != nullptr, MI_E_ARG, "null context");
     MI_HIP(hipSetDevice(ctx->device));
     return ctx;
+}
+
+template <typename T>
+int64_t single_job(Batch<T> &b, const mi_geometry &g, mi_at_params p, const void *sino, const void *u0,
+                   const void *v0) {
+    b.auto_retire = false;
+    const int64_t id = b.submit(g, p, (const T *)sino, (const T *)u0, (const T *)v0);
+    b.admit(true);
+    return id;
+}
+
+// Queues passes in chunks of `every` (all at once when 0) until the job's
+// device flag says done or max_passes were queued; returns passes run.
+template <typename T>
+int run_job(mi_context *ctx, Batch<T> &b, int64_t id, int max_passes, int every) {
+    const Scalars h0 = b.scalars(id);
+    Scalars h = h0;
+    int queued = 0;
+    while (queued < max_passes && !h.done) {
+        const int chunk = std::min(every > 0 ? every : max_passes, max_passes - queued);
+        b.enqueue(chunk);
+        queued += chunk;
+        h = b.scalars(id);
+    }
+    stats_collect(ctx);
+    return h.iters - h0.iters;
+}
+
+template <typename T>
+void fill_report(Batch<T> &b, int64_t id, mi_report *rep) {
+    const Scalars h = b.scalars(id);
+    std::memset(rep, 0, sizeof(*rep));
+    rep->iterations = h.iters;
+    rep->converged = h.converged;
+    rep->data_fidelity = h.df_prev;
+    rep->rel_change = h.rel;
+    rep->nnz = b.job(id).m ? b.job(id).m->nnz : 0;
+}
+
+template <typename T>
+void at_solve(mi_context *ctx, const mi_geometry &g, const mi_at_params &p, const T *sino, T *u, T *v,
+              mi_report *rep) {
+    MI_REQUIRE(p.n_subsets >= 1 && p.max_iters >= 0 && p.iter_edge >= 0, MI_E_ARG, "bad AT parameters");
+    MI_REQUIRE((p.precision == MI_FP64) == (sizeof(T) == 8), MI_E_ARG, "precision does not match the buffers");
+    Batch<T> b(ctx, p.n_subsets);
+    const int64_t id = single_job<T>(b, g, p, sino, u, v);
+    hipEvent_t e0, e1;
+    MI_HIP(hipEventCreate(&e0));
+    MI_HIP(hipEventCreate(&e1));
+    MI_HIP(hipEventRecord(e0, ctx->stream));
+    run_job<T>(ctx, b, id, p.max_iters, p.rel_tol > 0 ? 4 : 0);
+    MI_HIP(hipEventRecord(e1, ctx->stream));
+    MI_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    MI_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b.read(id, u, v);
+    if (rep) {
+        fill_report<T>(b, id, rep);
+        rep->elapsed_ms = ms;
+    }
 }
 
 template <typename T>
@@ -280,8 +343,10 @@ int mi_autotune(mi_context *ctx, const mi_geometry *g, int n_subsets, mi_tuning 
     return guarded([&] {
         check_ctx(ctx);
         MI_REQUIRE(g, MI_E_ARG, "null geometry");
-        mi_tuning t = autotune(ctx, *get_matrix(ctx, *g, n_subsets, false));
-        if (out) *out = t;
+        auto m = get_matrix(ctx, *g, n_subsets, false);
+        autotune(ctx, *m);
+        autotune_lanes(ctx, m);
+        if (out) *out = m->tune;
     });
 }
 
@@ -455,7 +520,7 @@ int mi_at_reconstruct(mi_context *ctx, const mi_geometry *g, const mi_at_params 
     return guarded([&] {
         check_ctx(ctx);
         MI_REQUIRE(g && p && sino && u && v, MI_E_ARG, "null argument");
-        os_solve<float>(ctx, *g, *p, sino, u, v, rep);
+        at_solve<float>(ctx, *g, *p, sino, u, v, rep);
     });
 }
 
@@ -464,7 +529,7 @@ int mi_at_reconstruct_f64(mi_context *ctx, const mi_geometry *g, const mi_at_par
     return guarded([&] {
         check_ctx(ctx);
         MI_REQUIRE(g && p && sino && u && v, MI_E_ARG, "null argument");
-        os_solve<double>(ctx, *g, *p, sino, u, v, rep);
+        at_solve<double>(ctx, *g, *p, sino, u, v, rep);
     });
 }
 
@@ -479,12 +544,15 @@ int mi_job_create(mi_context *ctx, const mi_geometry *g, const mi_at_params *p, 
         auto j = std::make_unique<mi_job>();
         j->ctx = ctx;
         j->precision = p->precision;
+        j->rel_tol = p->rel_tol;
+        mi_at_params q = *p;
+        q.max_iters = 0;  // passes are counted by the caller
         if (p->precision == MI_FP64) {
-            j->j64 = std::make_unique<Job<double>>(ctx, get_matrix(ctx, *g, p->n_subsets, true), *p);
-            j->j64->load((const double *)sino, (const double *)u0, (const double *)v0);
+            j->b64 = std::make_unique<Batch<double>>(ctx, p->n_subsets);
+            j->id = single_job<double>(*j->b64, *g, q, sino, u0, v0);
         } else {
-            j->j32 = std::make_unique<Job<float>>(ctx, get_matrix(ctx, *g, p->n_subsets, false), *p);
-            j->j32->load((const float *)sino, (const float *)u0, (const float *)v0);
+            j->b32 = std::make_unique<Batch<float>>(ctx, p->n_subsets);
+            j->id = single_job<float>(*j->b32, *g, q, sino, u0, v0);
         }
         *out = j.release();
     });
@@ -494,8 +562,9 @@ int mi_job_run(mi_job *job, int max_passes, int *passes_run) {
     return guarded([&] {
         MI_REQUIRE(job && max_passes >= 0, MI_E_ARG, "bad arguments");
         check_ctx(job->ctx);
-        const int every = (job->j32 ? job->j32->p.rel_tol : job->j64->p.rel_tol) > 0 ? 4 : 0;
-        int n = job->j32 ? job->j32->run(max_passes, every) : job->j64->run(max_passes, every);
+        const int every = job->rel_tol > 0 ? 4 : 0;
+        const int n = job->b32 ? run_job<float>(job->ctx, *job->b32, job->id, max_passes, every)
+                               : run_job<double>(job->ctx, *job->b64, job->id, max_passes, every);
         if (passes_run) *passes_run = n;
     });
 }
@@ -504,8 +573,8 @@ int mi_job_enqueue(mi_job *job, int passes) {
     return guarded([&] {
         MI_REQUIRE(job && passes >= 0, MI_E_ARG, "bad arguments");
         check_ctx(job->ctx);
-        if (job->j32) job->j32->enqueue(passes);
-        else job->j64->enqueue(passes);
+        if (job->b32) job->b32->enqueue(passes);
+        else job->b64->enqueue(passes);
     });
 }
 
@@ -522,8 +591,8 @@ int mi_job_report(mi_job *job, mi_report *rep) {
     return guarded([&] {
         MI_REQUIRE(job && rep, MI_E_ARG, "null argument");
         check_ctx(job->ctx);
-        if (job->j32) job->j32->report(rep);
-        else job->j64->report(rep);
+        if (job->b32) fill_report<float>(*job->b32, job->id, rep);
+        else fill_report<double>(*job->b64, job->id, rep);
     });
 }
 
@@ -531,8 +600,8 @@ int mi_job_read(mi_job *job, void *u, void *v) {
     return guarded([&] {
         MI_REQUIRE(job, MI_E_ARG, "null job");
         check_ctx(job->ctx);
-        if (job->j32) job->j32->read((float *)u, (float *)v);
-        else job->j64->read((double *)u, (double *)v);
+        if (job->b32) job->b32->read(job->id, (float *)u, (float *)v);
+        else job->b64->read(job->id, (double *)u, (double *)v);
     });
 }
 
@@ -542,6 +611,149 @@ int mi_job_destroy(mi_job *job) {
         (void)hipSetDevice(job->ctx->device);
         (void)hipStreamSynchronize(job->ctx->stream);
         delete job;
+    });
+}
+
+// ---- batches -------------------------------------------------------------
+
+int mi_batch_create(mi_context *ctx, int n_subsets, int precision, mi_batch **out) {
+    return guarded([&] {
+        check_ctx(ctx);
+        MI_REQUIRE(out && n_subsets >= 1, MI_E_ARG, "bad arguments");
+        MI_REQUIRE(precision == MI_FP32 || precision == MI_FP64, MI_E_ARG, "bad precision");
+        *out = nullptr;
+        auto b = std::make_unique<mi_batch>();
+        b->ctx = ctx;
+        b->precision = precision;
+        if (precision == MI_FP64) b->b64 = std::make_unique<Batch<double>>(ctx, n_subsets);
+        else b->b32 = std::make_unique<Batch<float>>(ctx, n_subsets);
+        *out = b.release();
+    });
+}
+
+int mi_batch_destroy(mi_batch *b) {
+    return guarded([&] {
+        if (!b) return;
+        (void)hipSetDevice(b->ctx->device);
+        delete b;
+    });
+}
+
+int mi_batch_set(mi_batch *b, int option, int64_t value) {
+    return guarded([&] {
+        MI_REQUIRE(b, MI_E_ARG, "null batch");
+        auto apply = [&](auto &B) {
+            switch (option) {
+                case MI_BATCH_GRAPH:
+                    B.use_graph = value != 0;
+                    B.drop_graph();
+                    break;
+                case MI_BATCH_MIN_K:
+                    MI_REQUIRE(value == 1 || value == 2 || value == 4 || value == 8, MI_E_ARG, "min K must be 1, 2, 4 or 8");
+                    B.min_k = (int)value;
+                    break;
+                case MI_BATCH_RESERVE_MB:
+                    MI_REQUIRE(value >= 0, MI_E_ARG, "negative reserve");
+                    B.reserve_bytes = value << 20;
+                    break;
+                default:
+                    fail(MI_E_ARG, "unknown batch option");
+            }
+        };
+        if (b->b32) apply(*b->b32);
+        else apply(*b->b64);
+    });
+}
+
+int mi_batch_submit(mi_batch *b, const mi_geometry *g, const mi_at_params *p, const void *sino, const void *u0,
+                    const void *v0, int64_t *job) {
+    return guarded([&] {
+        MI_REQUIRE(b && g && p && sino && job, MI_E_ARG, "null argument");
+        check_ctx(b->ctx);
+        *job = b->b32 ? b->b32->submit(*g, *p, (const float *)sino, (const float *)u0, (const float *)v0)
+                      : b->b64->submit(*g, *p, (const double *)sino, (const double *)u0, (const double *)v0);
+    });
+}
+
+int mi_batch_step(mi_batch *b, int passes, int *n_active, int *n_pending) {
+    return guarded([&] {
+        MI_REQUIRE(b && passes >= 0, MI_E_ARG, "bad arguments");
+        check_ctx(b->ctx);
+        const int a = b->b32 ? b->b32->step(passes) : b->b64->step(passes);
+        if (n_active) *n_active = a;
+        if (n_pending) *n_pending = b->b32 ? b->b32->n_pending() : b->b64->n_pending();
+    });
+}
+
+int mi_batch_enqueue(mi_batch *b, int passes) {
+    return guarded([&] {
+        MI_REQUIRE(b && passes >= 0, MI_E_ARG, "bad arguments");
+        check_ctx(b->ctx);
+        if (b->b32) b->b32->enqueue(passes);
+        else b->b64->enqueue(passes);
+    });
+}
+
+int mi_batch_poll(mi_batch *b, int *n_active) {
+    return guarded([&] {
+        MI_REQUIRE(b, MI_E_ARG, "null batch");
+        check_ctx(b->ctx);
+        const int a = b->b32 ? b->b32->poll() : b->b64->poll();
+        if (n_active) *n_active = a;
+    });
+}
+
+int mi_batch_status(mi_batch *b, int64_t job, mi_job_status *out) {
+    return guarded([&] {
+        MI_REQUIRE(b && out, MI_E_ARG, "null argument");
+        check_ctx(b->ctx);
+        auto fillst = [&](auto &B) {
+            auto &j = B.job(job);
+            const Scalars h = B.scalars(job);
+            std::memset(out, 0, sizeof(*out));
+            out->state = j.state;
+            out->iterations = h.iters;
+            out->converged = h.converged;
+            out->data_fidelity = h.df_prev;
+            out->rel_change = h.rel;
+            out->t_submit = j.t_submit;
+            out->t_admit = j.t_admit;
+            out->t_done = j.t_done;
+            out->nnz = j.m ? j.m->nnz : 0;
+            out->unit_k = j.state == BJ_ACTIVE ? B.units[j.uid]->K : 0;
+        };
+        if (b->b32) fillst(*b->b32);
+        else fillst(*b->b64);
+    });
+}
+
+int mi_batch_read(mi_batch *b, int64_t job, void *u, void *v) {
+    return guarded([&] {
+        MI_REQUIRE(b, MI_E_ARG, "null batch");
+        check_ctx(b->ctx);
+        if (b->b32) b->b32->read(job, (float *)u, (float *)v);
+        else b->b64->read(job, (double *)u, (double *)v);
+    });
+}
+
+int mi_batch_release(mi_batch *b, int64_t job) {
+    return guarded([&] {
+        MI_REQUIRE(b, MI_E_ARG, "null batch");
+        check_ctx(b->ctx);
+        if (b->b32) b->b32->release(job);
+        else b->b64->release(job);
+    });
+}
+
+int mi_batch_units(mi_batch *b, int32_t *k, int cap, int *n) {
+    return guarded([&] {
+        MI_REQUIRE(b && n && cap >= 0 && (cap == 0 || k), MI_E_ARG, "bad arguments");
+        auto list = [&](auto &B) {
+            *n = (int)B.live.size();
+            for (int i = 0; i < *n && i < cap; ++i) k[i] = B.units[B.live[i]]->K;
+        };
+        if (b->b32) list(*b->b32);
+        else list(*b->b64);
     });
 }
 

@@ -89,6 +89,37 @@ __device__ __forceinline__ double block_max(double x, double *lds) {
     return r;
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin.  Renumbering them so
+// that each XCD owns a contiguous range of the work keeps neighbouring rays
+// (or pixels), which gather the same image (or sinogram) lines, in one L2.
+__device__ __forceinline__ unsigned xcd_block() {
+    const unsigned g = gridDim.x, b = blockIdx.x;
+    if (g < 8 || (g & 7)) return b;
+    return (b & 7) * (g >> 3) + (b >> 3);
+}
+
+// Four consecutive entries of a padded matrix stream (16-byte aligned start).
+template <typename T> struct Vec4;
+template <> struct Vec4<float> {
+    float v[4];
+    __device__ __forceinline__ void load(const float *p) {
+        const float4 a = *reinterpret_cast<const float4 *>(p);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    }
+};
+template <> struct Vec4<double> {
+    double v[4];
+    __device__ __forceinline__ void load(const double *p) {
+        const double2 a = reinterpret_cast<const double2 *>(p)[0];
+        const double2 b = reinterpret_cast<const double2 *>(p)[1];
+        v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+    }
+};
+__device__ __forceinline__ void load_idx4(const uint32_t *p, uint32_t (&o)[4]) {
+    const uint4 a = *reinterpret_cast<const uint4 *>(p);
+    o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+}
+
 // Device-resident scalars of one job's solve.  Kernels read step sizes from
 // here and write reduced sums here, so a pass needs no host round trip.
 struct Scalars {
@@ -108,7 +139,7 @@ struct Scalars {
     int32_t edge_done;  // edge inner loop stopped early
     int32_t converged;  // rel_tol met
     int32_t iters;      // passes completed
-    int32_t pad;
+    int32_t done;       // converged or out of passes: kernels leave the job alone
 };
 
 }  // namespace mi

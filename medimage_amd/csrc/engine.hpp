@@ -1,0 +1,102 @@
+// Batched ordered-subset AT engine (engine.hip): one launch per phase covers
+// every batch unit, where a unit is K <= 8 jobs that share one system matrix
+// and whose state is stored job-interleaved (element e of job j at e*K + j),
+// so one matrix stream and one vector gather serve all K jobs.
+//
+// Every reduction that sets a step size is cut into fixed chunks (16 rays,
+// 64 pixels of the back projector, 256 pixels of the stencils) whose partial
+// sums are combined in index order by a one-block-per-job finalize.  The
+// chunking depends on neither the grid, nor K, nor the other units of the
+// launch, so a job's result is bit-for-bit the same alone and in any batch.
+#pragma once
+
+#include "common.hpp"
+#include "../../include/medimage_amd.h"
+
+namespace mi {
+
+constexpr int kMaxK = 8;             // jobs per unit
+constexpr int kMaxUnits = 64;        // units per batch
+constexpr int kRaysPerWave = 4;
+constexpr int kRaysPerBlock = kRaysPerWave * (kBlock / kWave);   // 16
+constexpr int kBpPix = 64;           // pixels per back-projector block
+constexpr int kPixBlock = kBlock;    // pixels per stencil block
+
+// Partial-sum slots of a unit (each K * cap doubles, see UnitDesc).
+enum EngSlot { SL_DF = 0, SL_DD, SL_AD, SL_NV, SL_E0, SL_E1, SL_E2, SL_E3, SL_COUNT };
+
+// Per-job parameters on the device.
+struct JobParams {
+    double alpha, beta, eps, lower, upper, rel_tol;
+    int32_t iter_edge, median_every, max_iters, active;
+};
+
+// One unit.  Pointers to the matrix (shared) and to the unit's interleaved
+// state.  Lengths are T (float or double).
+template <typename T>
+struct UnitDesc {
+    const int64_t *rowptr;
+    const uint32_t *idx;
+    const T *len;
+    const int64_t *colptr;      // all subsets: colptr[s * npix + p]
+    const uint32_t *crow;
+    const T *clen;
+    const int64_t *sub_row0;    // S + 1 row offsets
+    T *g, *r;                   // rows * K
+    T *u, *v, *d, *w;           // npix * K
+    int64_t npix, nrows;
+    int64_t part_off, cap;      // partials: part + part_off + (slot * K + j) * cap
+    int nx, ny, K, lanes;       // lanes: back-projector lanes per pixel, 8..64, >= K
+    int job0;                   // job slots job0 .. job0 + K - 1 (stable)
+    int pad;
+};
+
+// Block ranges of one launch: unit u owns blocks [start[u], start[u+1]).
+struct BlockMap {
+    int nunits;
+    int start[kMaxUnits + 1];
+};
+
+// Arguments common to the engine's launches.
+template <typename T>
+struct EngArgs {
+    const UnitDesc<T> *units;
+    const JobParams *jp;
+    Scalars *sc;                // one per job slot
+    double *part;               // partial sums, laid out per unit
+    int nunits;                 // units in the table (finalize grid: nunits * kMaxK)
+    int S;
+};
+
+// Launchers (all on `st`).  `s` is the subset step.  kmax: 1 when every unit
+// has K = 1 (selects code compiled for K = 1 only), else 8.
+template <typename T>
+void eng_fp_resid(hipStream_t st, const EngArgs<T> &a, const BlockMap &m, int s, int kmax);
+template <typename T>
+void eng_bp_grad(hipStream_t st, const EngArgs<T> &a, const BlockMap &m, int s, int kmax);
+// ||A_s d||^2 (rows map) and nV(d) (pixel map) in one launch.
+template <typename T>
+void eng_fp_sq_nv(hipStream_t st, const EngArgs<T> &a, const BlockMap &rows, const BlockMap &pix, int s, int kmax);
+template <typename T>
+void eng_axpy(hipStream_t st, const EngArgs<T> &a, const BlockMap &pix, int kmax);
+template <typename T>
+void eng_median(hipStream_t st, const EngArgs<T> &a, const BlockMap &pix, int kmax);
+template <typename T>
+void eng_edge_iter(hipStream_t st, const EngArgs<T> &a, const BlockMap &pix, int it, int kmax);
+
+enum EngFin { EF_BEGIN = 0, EF_STEP, EF_EDGE_C, EF_EDGE_DONE, EF_END };
+template <typename T>
+void eng_finalize(hipStream_t st, const EngArgs<T> &a, int op, int s);
+
+// Job I/O on one slot of a unit (n elements of the interleaved array).
+template <typename T>
+void eng_put_rows(hipStream_t st, T *dst, int K, int slot, int64_t n, const int64_t *perm, const T *src);
+template <typename T>
+void eng_put(hipStream_t st, T *dst, int K, int slot, int64_t n, const T *src, T fillv);
+template <typename T>
+void eng_get(hipStream_t st, T *dst, const T *src, int K, int slot, int64_t n);
+// Copies slot `from` of a unit array to slot `to` of another (repacking).
+template <typename T>
+void eng_move(hipStream_t st, T *dst, int Kd, int sd, const T *src, int Ks, int ss, int64_t n);
+
+}  // namespace mi

@@ -10,6 +10,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <vector>
 
 namespace mi {
@@ -26,8 +27,24 @@ struct DevBuf {
     DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
     DevBuf &operator=(DevBuf &&o) noexcept { reset(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; return *this; }
     ~DevBuf() { reset(); }
-    void alloc(size_t count);
-    void reset();
+    void alloc(size_t count) {
+        reset();
+        if (count == 0) return;
+        void *q = nullptr;
+        const hipError_t e = hipMalloc(&q, count * sizeof(T));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            fail(e == hipErrorOutOfMemory ? MI_E_NOMEM : MI_E_DEVICE,
+                 "device allocation of " + std::to_string(count * sizeof(T)) + " bytes failed");
+        }
+        p = (T *)q;
+        n = count;
+    }
+    void reset() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
     size_t bytes() const { return n * sizeof(T); }
 };
 
@@ -52,10 +69,13 @@ struct Matrix {
     int64_t nrows = 0, nnz = 0;
     std::vector<int> vlist;          // subset-ordered view list
     std::vector<int64_t> sub_row0;   // S+1 row offsets
+    std::vector<int64_t> sub_nnz;    // S entry counts (CSR rows of each subset)
+    uint64_t last_use = 0;           // context tick of the last use (LRU eviction)
     bool has64 = false;
     mi_tuning tune{0, 0, 0, 0, 0.0, 0.0};  // bp_lanes 0: not set (heuristic)
 
     DevBuf<int> d_vlist;
+    DevBuf<int64_t> d_sub_row0;      // S + 1
     DevBuf<int64_t> rowptr;          // nrows + 1
     DevBuf<uint32_t> idx;            // nnz, pixel index
     DevBuf<float> len32;             // nnz
@@ -94,12 +114,16 @@ struct mi_context {
     hipStream_t stream = nullptr;
     std::map<mi::GeomKey, std::shared_ptr<mi::Matrix>> matrices;
     mi::Stats stats;
+    uint64_t tick = 0;
 };
 
 namespace mi {
 // Matrix for (g, S) from the context cache, building it if needed.  Jobs
 // hold the shared pointer, so trimming the cache never frees a matrix in use.
 std::shared_ptr<Matrix> get_matrix(mi_context *ctx, const mi_geometry &g, int S, bool want64);
+// Frees cached matrices that no job holds, least recently used first, until
+// at least `bytes` were freed (all idle ones when bytes < 0).  Returns bytes freed.
+int64_t evict_idle(mi_context *ctx, int64_t bytes);
 // Stats helpers: bracket a launch with events when enabled.
 struct LaunchTimer {
     mi_context *ctx;

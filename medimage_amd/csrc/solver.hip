@@ -103,151 +103,6 @@ Scalars JobState<T>::read_scalars(hipStream_t st) const {
     return h;
 }
 
-// One ordered-subset pass (everything on the stream, no host round trip).
-// Every kernel of the pass reads sc->converged and returns at once when it is
-// set, so passes can be queued ahead of the convergence test.
-template <typename T>
-void os_pass(mi_context *ctx, const Matrix &m, JobState<T> &js, const mi_at_params &p, int pass_index) {
-    const hipStream_t st = ctx->stream;
-    const bool f64 = sizeof(T) == 8;
-    const RedBuf rb = js.rb();
-    Scalars *sc = js.sc.p;
-    const int32_t *skip = &sc->converged;
-    const CsrView A = csr_view(m, f64);
-    const int lanes = bp_lanes(m);
-    const double S = (double)m.S;
-    launch_pass_begin(st, sc);
-    for (int s = 0; s < m.S; ++s) {
-        const int64_t row0 = m.sub_row0[s], nr = m.sub_row0[s + 1] - row0;
-        FinArgs f{};
-        f.op = FIN_STEP;
-        f.nin = 4;
-        f.sc = sc;
-        f.skip = skip;
-        f.S = S;
-        f.alpha = p.alpha;
-        {
-            LaunchTimer lt(ctx, MI_K_FP, fp_bytes(m, nr, sizeof(T), FP_RESID));
-            f.n[3] = (int)launch_fp<T>(st, FP_RESID, A, row0, nr, js.u.p, js.g.p, nullptr, js.r.p, rb.slot(3), skip,
-                                       m.tune.fp_vec, m.tune.fp_grid);
-        }
-        {
-            LaunchTimer lt(ctx, MI_K_BP, bp_bytes(m, nr, sizeof(T), true));
-            f.n[0] = (int)launch_bp_grad<T>(st, csc_view(m, s, f64), js.nx, js.ny, js.r.p, S, js.u.p, js.v.p,
-                                            p.alpha, p.epsilon, js.d.p, rb.slot(0), lanes, skip, m.tune.bp_grid);
-        }
-        {
-            LaunchTimer lt(ctx, MI_K_FPSQ, fp_bytes(m, nr, sizeof(T), FP_SQ));
-            f.n[1] = (int)launch_fp<T>(st, FP_SQ, A, row0, nr, js.d.p, nullptr, nullptr, nullptr, rb.slot(1), skip,
-                                       m.tune.fp_vec, m.tune.fp_grid);
-        }
-        {
-            LaunchTimer lt(ctx, MI_K_STEP, (double)js.npix * sizeof(T) * 5);
-            f.n[2] = (int)launch_nv<T>(st, js.nx, js.ny, js.d.p, js.v.p, p.epsilon, rb.slot(2), skip);
-            for (int k = 0; k < 4; ++k) f.in[k] = rb.slot(k);
-            launch_finalize(st, f);
-            launch_axpy<T>(st, js.npix, js.u.p, js.d.p, &sc->c, 0.0, p.lower, p.upper, nullptr, nullptr, skip);
-        }
-    }
-    if (p.median_every > 0 && pass_index % p.median_every == 0) {
-        launch_median<T>(st, js.nx, js.ny, js.u.p, js.t.p);
-        std::swap(js.u, js.t);
-    }
-    {
-        LaunchTimer lt(ctx, MI_K_EDGE, (double)js.npix * sizeof(T) * 7 * (p.iter_edge > 0 ? p.iter_edge : 0));
-        for (int k = 0; k < p.iter_edge; ++k)
-            launch_edge_iter<T>(st, rb, js.nx, js.ny, js.u.p, js.v.p, js.d.p, p.alpha, p.beta, p.epsilon, sc);
-    }
-    launch_pass_end(st, sc, p.rel_tol);
-}
-
-// ---------------------------------------------------------------------------
-// Device-resident job of the ordered-subset form.
-
-template <typename T>
-Job<T>::Job(mi_context *c, std::shared_ptr<Matrix> mat, const mi_at_params &prm)
-    : ctx(c), m(std::move(mat)), p(prm), js(*m) {}
-
-template <typename T>
-void Job<T>::load(const T *sino, const T *u0, const T *v0) {
-    const hipStream_t st = ctx->stream;
-    load_sino<T>(st, *m, sino, js.g.p);
-    if (u0) MI_HIP(hipMemcpyAsync(js.u.p, u0, js.npix * sizeof(T), hipMemcpyHostToDevice, st));
-    else fill<T>(st, js.u.p, js.npix, T(0));
-    if (v0) MI_HIP(hipMemcpyAsync(js.v.p, v0, js.npix * sizeof(T), hipMemcpyHostToDevice, st));
-    else fill<T>(st, js.v.p, js.npix, T(1));
-    MI_HIP(hipMemsetAsync(js.sc.p, 0, sizeof(Scalars), st));
-    MI_HIP(hipStreamSynchronize(st));
-    passes = 0;
-}
-
-template <typename T>
-int Job<T>::run(int max_passes, int check_every) {
-    const hipStream_t st = ctx->stream;
-    int queued = 0;
-    Scalars h{};
-    while (queued < max_passes) {
-        const int chunk = std::min(check_every > 0 ? check_every : max_passes, max_passes - queued);
-        for (int k = 0; k < chunk; ++k) os_pass<T>(ctx, *m, js, p, passes + queued + k + 1);
-        queued += chunk;
-        h = js.read_scalars(st);
-        if (h.converged) break;
-    }
-    const int ran = h.iters - passes;
-    passes = h.iters;
-    stats_collect(ctx);
-    return ran;
-}
-
-template <typename T>
-void Job<T>::enqueue(int n) {
-    for (int k = 0; k < n; ++k) os_pass<T>(ctx, *m, js, p, passes + k + 1);
-    passes += n;
-}
-
-template <typename T>
-void Job<T>::report(mi_report *rep) {
-    Scalars h = js.read_scalars(ctx->stream);
-    passes = h.iters;
-    rep->iterations = h.iters;
-    rep->converged = h.converged;
-    rep->data_fidelity = h.df_prev;
-    rep->rel_change = h.rel;
-    rep->elapsed_ms = 0;
-    rep->nnz = m->nnz;
-}
-
-template <typename T>
-void Job<T>::read(T *u, T *v) {
-    const hipStream_t st = ctx->stream;
-    if (u) MI_HIP(hipMemcpyAsync(u, js.u.p, js.npix * sizeof(T), hipMemcpyDeviceToHost, st));
-    if (v) MI_HIP(hipMemcpyAsync(v, js.v.p, js.npix * sizeof(T), hipMemcpyDeviceToHost, st));
-    MI_HIP(hipStreamSynchronize(st));
-}
-
-template <typename T>
-void os_solve(mi_context *ctx, const mi_geometry &g, const mi_at_params &p, const T *sino, T *u, T *v,
-              mi_report *rep) {
-    MI_REQUIRE(p.n_subsets >= 1 && p.max_iters >= 0 && p.iter_edge >= 0, MI_E_ARG, "bad AT parameters");
-    Job<T> job(ctx, get_matrix(ctx, g, p.n_subsets, sizeof(T) == 8), p);
-    job.load(sino, u, v);
-    hipEvent_t e0, e1;
-    MI_HIP(hipEventCreate(&e0));
-    MI_HIP(hipEventCreate(&e1));
-    MI_HIP(hipEventRecord(e0, ctx->stream));
-    job.run(p.max_iters, p.rel_tol > 0 ? 4 : 0);
-    MI_HIP(hipEventRecord(e1, ctx->stream));
-    MI_HIP(hipEventSynchronize(e1));
-    float ms = 0.f;
-    MI_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    job.read(u, v);
-    if (rep) {
-        job.report(rep);
-        rep->elapsed_ms = ms;
-    }
-}
 
 // ---------------------------------------------------------------------------
 // GoldenC: minimize_in_IMAGE_variable (main.c:81-257), host-driven CG with
@@ -398,11 +253,5 @@ mi_tuning autotune(mi_context *ctx, Matrix &m) {
 
 template struct JobState<float>;
 template struct JobState<double>;
-template struct Job<float>;
-template struct Job<double>;
-template void os_solve<float>(mi_context *, const mi_geometry &, const mi_at_params &, const float *, float *,
-                              float *, mi_report *);
-template void os_solve<double>(mi_context *, const mi_geometry &, const mi_at_params &, const double *, double *,
-                               double *, mi_report *);
 
 }  // namespace mi

@@ -30,33 +30,6 @@ int bp_lanes(const Matrix &m);
 double fp_bytes(const Matrix &m, int64_t nrows, size_t tsz, int mode);
 double bp_bytes(const Matrix &m, int64_t nrows_sub, size_t tsz, bool reg);
 
-// A device-resident ordered-subset job: its matrix (shared with the context
-// cache), its state and its parameters.
-template <typename T>
-struct Job {
-    mi_context *ctx;
-    std::shared_ptr<Matrix> m;
-    mi_at_params p;
-    JobState<T> js;
-    int passes = 0;
-    Job(mi_context *c, std::shared_ptr<Matrix> mat, const mi_at_params &prm);
-    void load(const T *sino, const T *u0, const T *v0);
-    // Runs up to max_passes passes, testing convergence on the host every
-    // check_every passes (0: only at the end).  Returns passes run.
-    int run(int max_passes, int check_every);
-    // Queues n passes without waiting (a converged job skips on device).
-    void enqueue(int n);
-    void report(mi_report *rep);
-    void read(T *u, T *v);
-};
-
-// Ordered-subset AT: one pass over all subsets, then the edge step.
-template <typename T>
-void os_pass(mi_context *ctx, const Matrix &m, JobState<T> &js, const mi_at_params &p, int pass_index);
-template <typename T>
-void os_solve(mi_context *ctx, const mi_geometry &g, const mi_at_params &p, const T *sino, T *u, T *v,
-              mi_report *rep);
-
 // GoldenC steps on a job's device state (u: image, v: edge).
 template <typename T>
 int gc_image_step(mi_context *ctx, const Matrix &m, JobState<T> &js, double alpha, double eps, int iter_image,

@@ -303,6 +303,16 @@ void build_csc(Matrix &m, const int64_t *rowptr, const uint32_t *idx, const floa
     }
 }
 
+// Entries of each subset's rows, from the device row pointer.
+void subset_counts(Matrix &m, hipStream_t st) {
+    std::vector<int64_t> at(m.S + 1);
+    for (int s = 0; s <= m.S; ++s)
+        MI_HIP(hipMemcpyAsync(&at[s], m.rowptr.p + m.sub_row0[s], sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    MI_HIP(hipStreamSynchronize(st));
+    m.sub_nnz.assign(m.S, 0);
+    for (int s = 0; s < m.S; ++s) m.sub_nnz[s] = at[s + 1] - at[s];
+}
+
 }  // namespace
 
 std::vector<int> subset_order(int S) {
@@ -319,7 +329,7 @@ std::vector<int> subset_order(int S) {
 }
 
 size_t Matrix::bytes() const {
-    return d_vlist.bytes() + rowptr.bytes() + idx.bytes() + len32.bytes() + len64.bytes() + colptr.bytes() +
+    return d_vlist.bytes() + d_sub_row0.bytes() + rowptr.bytes() + idx.bytes() + len32.bytes() + len64.bytes() + colptr.bytes() +
            crow.bytes() + clen32.bytes() + clen64.bytes() + perm.bytes();
 }
 
@@ -341,6 +351,7 @@ std::unique_ptr<Matrix> build_matrix(const mi_geometry &g, int S, bool want64, h
         m->sub_row0[s + 1] = m->sub_row0[s] + (int64_t)cnt * g.n_bins;
     }
     upload(m->d_vlist, m->vlist, st);
+    upload(m->d_sub_row0, m->sub_row0, st);
     m->perm.alloc(m->nrows);
     k_perm<<<grid_of(m->nrows), kBlock, 0, st>>>(m->d_vlist.p, g.n_bins, m->nrows, m->perm.p);
     MI_HIP(hipGetLastError());
@@ -370,6 +381,7 @@ std::unique_ptr<Matrix> build_matrix(const mi_geometry &g, int S, bool want64, h
         upload(aln32, lna32, st);
         if (want64) upload(aln64, lna, st);
         build_csc(*m, arp.p, aix.p, aln32.p, want64 ? aln64.p : nullptr, st);
+        subset_counts(*m, st);
         return m;
     }
 
@@ -428,6 +440,7 @@ std::unique_ptr<Matrix> build_matrix(const mi_geometry &g, int S, bool want64, h
                                                        want64 ? m->len64.p : nullptr);
     MI_HIP(hipGetLastError());
     build_csc(*m, m->rowptr.p, m->idx.p, m->len32.p, want64 ? m->len64.p : nullptr, st);
+    subset_counts(*m, st);
     return m;
 }
 

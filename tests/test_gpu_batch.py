@@ -1,0 +1,189 @@
+"""GPU tests of the batched engine (mi_batch): many jobs of mixed geometry
+per launch, continuous batching, and the property the engine is built for:
+a job's result is bit-for-bit the same alone and in any mix (every
+reduction is cut into fixed chunks combined in a fixed order, and the
+per-job arithmetic does not depend on K).  FP64 batches are also checked
+against the CPU oracle (ordered-subset AT, oracle/solvers.c)."""
+import numpy as np
+import pytest
+
+import medimage_amd as mi
+from medimage_amd import configs
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+GEOMS = {
+    "a": mi.fan_geometry(64, 4.0, 60, 96, 4.0),
+    "b": mi.fan_geometry(96, 2.5, 72, 128, 3.0),
+    "c": mi.fan_geometry(128, 2.0, 90, 128, 2.0),
+}
+
+
+def _jobs(ctx, precision=mi.MI_FP32):
+    """Nine jobs over three geometries with differing parameters: a x3, b x1,
+    c x5; pass caps 3..6, one with a median every 2 passes, one with two
+    edge steps, one stopped by rel_tol."""
+    dt = np.float64 if precision == mi.MI_FP64 else np.float32
+    spec = [("a", 3, {}), ("a", 5, {"median_every": 2}), ("a", 4, {}),
+            ("b", 6, {"iter_edge": 2}),
+            ("c", 3, {}), ("c", 4, {"alpha": 2e-3}), ("c", 6, {"rel_tol": 5e-2}), ("c", 5, {}), ("c", 4, {})]
+    out = []
+    for i, (gk, iters, kw) in enumerate(spec):
+        g = GEOMS[gk]
+        _, sino = configs.make_case(ctx, g, 3e4, seed=100 + i)
+        p = mi.at_params(kw.get("alpha", 1e-3), 1e-4, 1e-2, n_subsets=4, max_iters=iters,
+                         iter_edge=kw.get("iter_edge", 1), rel_tol=kw.get("rel_tol", 0.0),
+                         median_every=kw.get("median_every", 0), precision=precision)
+        out.append((g, sino.astype(dt), p))
+    return out
+
+
+def _run_batch(ctx, jobs, precision=mi.MI_FP32, graph=True, waves=(0,)):
+    """Submits jobs in waves (wave w enters after waves[w] passes) and runs
+    to completion.  Returns {index: (u, v, status)}."""
+    b = ctx.batch(4, precision, graph=graph)
+    ids = {}
+    order = np.array_split(np.arange(len(jobs)), len(waves))
+    passes = 0
+    for w, start in enumerate(waves):
+        while passes < start:
+            b.step(1)
+            passes += 1
+        for i in order[w]:
+            g, sino, p = jobs[i]
+            ids[int(i)] = b.submit(g, sino, p)
+    b.run()
+    out = {}
+    for i, jid in ids.items():
+        st = b.status(jid)
+        assert st.state == mi.MI_JOB_DONE, f"job {i} state {st.state}"
+        u, v = b.read(jid)
+        out[i] = (u, v, st)
+    b.close()
+    return out
+
+
+def _alone(ctx, jobs, precision=mi.MI_FP32):
+    out = {}
+    for i, (g, sino, p) in enumerate(jobs):
+        out[i] = _run_batch(ctx, [(g, sino, p)], precision)[0]
+    return out
+
+
+def _same(x, y):
+    return x.dtype == y.dtype and np.array_equal(x.view(np.uint8), y.view(np.uint8))
+
+
+def test_batch_alone_equals_mixed_bitwise(ctx):
+    jobs = _jobs(ctx)
+    print(f"[batch mixed] {len(jobs)} jobs, 3 geometries", flush=True)
+    mixed = _run_batch(ctx, jobs)
+    alone = _alone(ctx, jobs)
+    for i in range(len(jobs)):
+        um, vm, sm = mixed[i]
+        ua, va, sa = alone[i]
+        assert sm.iterations == sa.iterations, i
+        assert _same(um, ua) and _same(vm, va), f"job {i} differs alone vs mixed"
+        assert sm.data_fidelity == sa.data_fidelity
+    # the rel_tol job may stop early; the capped ones run exactly their cap
+    assert mixed[0][2].iterations == 3 and mixed[3][2].iterations == 6
+
+
+def test_batch_continuous_admission_bitwise(ctx):
+    """Jobs joining at pass 0, 2 and 3 (continuous batching) end with the
+    same bits as alone."""
+    jobs = _jobs(ctx)
+    staged = _run_batch(ctx, jobs, waves=(0, 2, 3))
+    alone = _alone(ctx, jobs)
+    for i in range(len(jobs)):
+        assert _same(staged[i][0], alone[i][0]) and _same(staged[i][1], alone[i][1]), f"job {i}"
+
+
+def test_batch_graph_equals_eager(ctx):
+    jobs = _jobs(ctx)
+    a = _run_batch(ctx, jobs, graph=True)
+    b = _run_batch(ctx, jobs, graph=False)
+    for i in range(len(jobs)):
+        assert _same(a[i][0], b[i][0]) and _same(a[i][1], b[i][1]), f"job {i}"
+
+
+def test_batch_graph_replays_repeat(ctx):
+    """The captured pass replayed many times, with host synchronisation and
+    eager work in between, gives the same result as eager launches."""
+    g = GEOMS["c"]
+    jobs = [(g, configs.make_case(ctx, g, 3e4, seed=300 + i)[1].astype(np.float32),
+             mi.at_params(n_subsets=4, max_iters=8)) for i in range(3)]
+    res = []
+    for graph in (True, False):
+        b = ctx.batch(4, graph=graph)
+        ids = [b.submit(*j) for j in jobs]
+        for _ in range(8):
+            b.step(1)
+            ctx.forward(g, np.ones((g.nx, g.ny), np.float32))   # eager work between replays
+        res.append([b.read(i) for i in ids])
+        b.close()
+    for (ua, va), (ub, vb) in zip(*res):
+        assert _same(ua, ub) and _same(va, vb)
+
+
+def test_batch_packing(ctx):
+    jobs = _jobs(ctx)
+    b = ctx.batch(4)
+    for g, sino, p in jobs:
+        b.submit(g, sino, p)
+    b.step(0)
+    ks = sorted(b.units())
+    print(f"[packing] units K = {ks}", flush=True)
+    assert ks == [1, 4, 8]          # b x1, a x3 (K=4), c x5 (K=8)
+    b.run()
+    assert b.units() == []
+    b.close()
+
+
+def test_batch_fp64_matches_oracle(ctx):
+    jobs = _jobs(ctx, mi.MI_FP64)
+    res = _run_batch(ctx, jobs, mi.MI_FP64)
+    for i in (0, 1, 3, 5):
+        g, sino, p = jobs[i]
+        ou, ov, n, hist = O.os_at(O.op_fan(O.fan_from(g)), sino, p.alpha, p.beta, p.epsilon, p.n_subsets,
+                                  p.max_iters, iter_edge=p.iter_edge, rel_tol=p.rel_tol,
+                                  median_every=p.median_every)
+        u, v, st = res[i]
+        err_u = float(np.linalg.norm(u - ou) / np.linalg.norm(ou))
+        err_v = float(np.linalg.norm(v - ov) / np.linalg.norm(ov))
+        print(f"[batch fp64 job {i}] iters {st.iterations}/{n} rel u {err_u:.2e} v {err_v:.2e}", flush=True)
+        assert st.iterations == n
+        assert err_u < 1e-10 and err_v < 1e-10
+
+
+def test_batch_fp32_within_tolerance_of_oracle(ctx):
+    jobs = _jobs(ctx)
+    res = _run_batch(ctx, jobs)
+    for i in (2, 7):
+        g, sino, p = jobs[i]
+        ou, ov, n, _ = O.os_at(O.op_fan(O.fan_from(g)), sino.astype(np.float64), p.alpha, p.beta, p.epsilon,
+                               p.n_subsets, p.max_iters, iter_edge=p.iter_edge)
+        u, v, _ = res[i]
+        assert np.linalg.norm(u - ou) / np.linalg.norm(ou) < 1e-5
+        assert np.abs(u - ou).max() / np.abs(ou).max() < 1e-5
+        assert np.linalg.norm(v - ov) / np.linalg.norm(ov) < 1e-5
+
+
+def test_batch_release_and_errors(ctx):
+    g = GEOMS["a"]
+    _, sino = configs.make_case(ctx, g, 3e4, seed=9)
+    b = ctx.batch(4)
+    j1 = b.submit(g, sino.astype(np.float32), mi.at_params(n_subsets=4, max_iters=50))
+    j2 = b.submit(g, sino.astype(np.float32), mi.at_params(n_subsets=4, max_iters=2))
+    b.step(1)
+    b.release(j1)                     # cancel an active job
+    a, q = b.step(1)
+    assert b.status(j2).state == mi.MI_JOB_DONE and a == 0 and q == 0
+    with pytest.raises(mi.MIError) as e:
+        b.status(j1)
+    assert e.value.code == mi.MI_E_ARG
+    with pytest.raises(mi.MIError) as e:
+        b.submit(g, sino, mi.at_params(n_subsets=3))
+    assert e.value.code == mi.MI_E_ARG
+    b.close()
