@@ -47,6 +47,71 @@ def cpu_baseline(sino, passes):
                       f"FP64 oracle, one thread, {dt:.1f} s"}
 
 
+def bench_c3(ctx, n_jobs, iters, log):
+    """Batched form: n_jobs mixed fan-beam slices (sizes, views, bins and I0
+    drawn from the seed), 8 subsets, `iters` AT iterations each, run (a) one
+    at a time through the plain per-job form and (b) all together in one
+    batch.  Returns throughput of both and the per-job agreement."""
+    import medimage_amd as mi
+    from medimage_amd import configs
+    spec = configs.c3_jobs(n_jobs, seed=3)
+    t = time.perf_counter()
+    jobs = []
+    geoms = {}
+    for j in spec:
+        g = configs.mixed_geometry(j["n"], j["views"], j["bins"])
+        geoms[g.key()] = g
+        _, sino = configs.make_case(ctx, g, j["i0"], j["seed"])
+        jobs.append((g, sino.astype(np.float32), j))
+    ctx.trim()
+    nnz = {}
+    for k, g in geoms.items():
+        nnz[k] = ctx.prepare(g, 8)
+        ctx.autotune(g, 8)
+    log(f"[c3] {n_jobs} jobs over {len(geoms)} geometries prepared in {time.perf_counter() - t:.1f} s")
+    p = configs.at_params(configs.ALPHA, configs.BETA, configs.EPS, n_subsets=8, max_iters=iters)
+    rays = sum(g.n_views * g.n_bins for g, _, _ in jobs)
+
+    # (a) plain: one job at a time (each a batch of one; graph replay).
+    ctx.at_reconstruct(jobs[0][0], jobs[0][1], configs.at_params(n_subsets=8, max_iters=2))  # warm
+    t0 = time.perf_counter()
+    plain = [ctx.at_reconstruct(g, s, p) for g, s, _ in jobs]
+    t_plain = time.perf_counter() - t0
+    log(f"[c3] plain: {n_jobs} x {iters} iterations in {t_plain:.2f} s")
+
+    # (b) batched: every job in one batch.
+    b = ctx.batch(8)
+    ids = [b.submit(g, s, p) for g, s, _ in jobs]
+    b.step(0)                       # admission (device copies, unit packing)
+    units = b.units()
+    t0 = time.perf_counter()
+    b.run()
+    t_batch = time.perf_counter() - t0
+    log(f"[c3] batched: {len(units)} units K={sorted(units)}; {t_batch:.2f} s")
+    same = 0
+    worst = 0.0
+    for (u0, v0, r0), jid in zip(plain, ids):
+        u1, v1 = b.read(jid)
+        if np.array_equal(u0.view(np.uint32), u1.view(np.uint32)) and np.array_equal(v0.view(np.uint32), v1.view(np.uint32)):
+            same += 1
+        worst = max(worst, float(np.abs(u1 - u0).max() / max(np.abs(u0).max(), 1e-30)))
+    b.close()
+    work = n_jobs * iters
+    return {
+        "workload": f"{n_jobs} fan-beam slices, N in {{256,384,512}}, views in {{360,720,1152}}, bins in "
+                    f"{{512,736}}, I0 U[1e4,1e5], 8 subsets, {iters} AT iterations each",
+        "geometries": len(geoms),
+        "units_K": sorted(units),
+        "plain_iters_per_s": work / t_plain,
+        "batched_iters_per_s": work / t_batch,
+        "speedup": t_plain / t_batch,
+        "plain_rays_per_s": rays * iters / t_plain,
+        "batched_rays_per_s": rays * iters / t_batch,
+        "jobs_bitwise_identical": same,
+        "max_rel_diff": worst,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -54,6 +119,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-passes", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--c3-jobs", type=int, default=64)
+    ap.add_argument("--c3-iters", type=int, default=50)
+    ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--copy-gib", type=float, default=0.0,
+                    help="also run a plain device copy of this many GiB (reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -126,6 +196,15 @@ def main():
                      "bytes_per_launch": bytes_per, "avg_launch_us": avg_ms * 1e3},
         "kernels": per_kernel,
     }
+    if args.copy_gib > 0:
+        nb = int(args.copy_gib * (1 << 30)) // 16 * 16
+        ms = ctx.device_copy(nb, 3)
+        out["copy_reference"] = {"kernel": "k_copy16", "bytes_read": nb, "bytes_written": nb, "launches": 3,
+                                 "ms": ms, "GBps": 2 * nb / (ms * 1e-3) / 1e9}
+    if not args.no_c3:
+        job.close()
+        out["batched"] = bench_c3(ctx, args.c3_jobs, args.c3_iters,
+                                  lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_cpu and args.cpu_passes > 0:
         print(f"[bench] CPU baseline: {args.cpu_passes} AT iterations of the oracle", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(sino.astype(np.float32), args.cpu_passes)

@@ -273,6 +273,11 @@ typedef struct mi_kernel_stats {
 int mi_stats_enable(mi_context *ctx, int on);
 int mi_stats_get(mi_context *ctx, int kernel, mi_kernel_stats *out);
 int mi_stats_reset(mi_context *ctx);
+/* A plain streaming copy of `bytes` (multiple of 16) between two fresh
+ * device buffers, `reps` times; *ms = mean time per copy.  It has no
+ * counterpart in the reference: it is the known-size kernel against which
+ * hardware memory counters of the other kernels are read. */
+int mi_device_copy(mi_context *ctx, int64_t bytes, int reps, double *ms);
 
 #ifdef __cplusplus
 }

@@ -16,12 +16,13 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import os
 from pathlib import Path
 
 import numpy as np
 
 _PKG = Path(__file__).resolve().parent
-LIB_PATH = _PKG / "libmedimage_amd.so"
+LIB_PATH = Path(os.environ.get("MEDIMAGE_AMD_LIB") or str(_PKG / "libmedimage_amd.so"))
 
 MI_OK, MI_E_ARG, MI_E_DEVICE, MI_E_NOMEM, MI_E_STATE, MI_E_LIMIT = 0, -1, -2, -3, -4, -5
 MI_BEAM_FAN_FLAT, MI_BEAM_PARALLEL, MI_BEAM_GOLDENC = 0, 1, 2
@@ -159,6 +160,7 @@ def _load():
         "mi_stats_enable": [P, C.c_int],
         "mi_stats_get": [P, C.c_int, C.POINTER(KernelStats)],
         "mi_stats_reset": [P],
+        "mi_device_copy": [P, C.c_int64, C.c_int, C.POINTER(C.c_double)],
     }
     for name, args in sigs.items():
         fn = getattr(lib, name)
@@ -336,6 +338,12 @@ class Context:
 
     def stats_reset(self):
         _check(self._lib.mi_stats_reset(self._h))
+
+    def device_copy(self, nbytes: int, reps: int = 1) -> float:
+        """Plain device copy of nbytes, reps times; returns ms per copy."""
+        ms = C.c_double()
+        _check(self._lib.mi_device_copy(self._h, nbytes, reps, C.byref(ms)))
+        return ms.value
 
 
 class Job:

@@ -36,9 +36,9 @@ FLAGS = [
 ]
 
 
-def _compile(src: str) -> Path:
-    out = OBJ / (Path(src).stem + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", str(CSRC / src), "-o", str(out)]
+def _compile(src: str, obj: Path = OBJ, defines=()) -> Path:
+    out = obj / (Path(src).stem + ".o")
+    cmd = [HIPCC, *FLAGS, *[f"-D{d}" for d in defines], "-c", str(CSRC / src), "-o", str(out)]
     if src.endswith(".cpp"):
         cmd.insert(1, "-xhip")
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -47,19 +47,21 @@ def _compile(src: str) -> Path:
     return out
 
 
-def build(verbose: bool = False) -> Path:
-    OBJ.mkdir(exist_ok=True)
+def build(verbose: bool = False, out: Path = LIB, obj: Path = OBJ, defines=()) -> Path:
+    """Compiles every source and links `out`; `defines` (NAME=VALUE) select
+    compile-time kernel variants."""
+    obj.mkdir(parents=True, exist_ok=True)
     with ThreadPoolExecutor(max_workers=min(6, os.cpu_count() or 1)) as ex:
-        objs = list(ex.map(_compile, SOURCES))
-    tmp = LIB.with_suffix(".so.tmp")
+        objs = list(ex.map(lambda s: _compile(s, obj, defines), SOURCES))
+    tmp = out.with_suffix(".so.tmp")
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, out)
     if verbose:
-        print(f"built {LIB}")
-    return LIB
+        print(f"built {out}")
+    return out
 
 
 if __name__ == "__main__":

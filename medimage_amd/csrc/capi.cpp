@@ -757,6 +757,28 @@ int mi_batch_units(mi_batch *b, int32_t *k, int cap, int *n) {
     });
 }
 
+int mi_device_copy(mi_context *ctx, int64_t bytes, int reps, double *ms) {
+    return guarded([&] {
+        check_ctx(ctx);
+        MI_REQUIRE(bytes > 0 && bytes % 16 == 0 && reps >= 1, MI_E_ARG, "bad copy size");
+        DevBuf<unsigned char> a((size_t)bytes), b((size_t)bytes);
+        const hipStream_t st = ctx->stream;
+        MI_HIP(hipMemsetAsync(a.p, 1, bytes, st));
+        hipEvent_t e0, e1;
+        MI_HIP(hipEventCreate(&e0));
+        MI_HIP(hipEventCreate(&e1));
+        MI_HIP(hipEventRecord(e0, st));
+        for (int i = 0; i < reps; ++i) eng_copy(st, b.p, a.p, bytes);
+        MI_HIP(hipEventRecord(e1, st));
+        MI_HIP(hipEventSynchronize(e1));
+        float t = 0.f;
+        MI_HIP(hipEventElapsedTime(&t, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (ms) *ms = t / reps;
+    });
+}
+
 int mi_stats_enable(mi_context *ctx, int on) {
     return guarded([&] {
         check_ctx(ctx);

@@ -21,26 +21,87 @@ namespace mi {
 
 namespace {
 
+// Global address space.  The unit table is read from device memory, so the
+// compiler cannot infer that its pointers are global and would emit flat
+// loads (64-bit addresses, counted against lgkmcnt as well as vmcnt, which
+// serialises them with every scalar and LDS wait).  Every access of the
+// engine goes through these pointers instead.
+#define GL __attribute__((address_space(1)))
+
 template <typename T>
-__device__ __forceinline__ void load4(const T *p, T (&o)[4]) {
-    Vec4<T> a;
-    a.load(p);
-#pragma unroll
-    for (int m = 0; m < 4; ++m) o[m] = a.v[m];
+struct UG {
+    const GL int64_t *rowptr;
+    const GL uint32_t *idx;
+    const GL T *len;
+    const GL int64_t *colptr;
+    const GL uint32_t *crow;
+    const GL T *clen;
+    const GL int64_t *sub_row0;
+    GL T *g, *r, *u, *v, *d, *w;
+    int64_t npix, nrows, part_off, cap;
+    int nx, ny, K, lanes, job0;
+    __device__ __forceinline__ explicit UG(const UnitDesc<T> &D)
+        : rowptr((const GL int64_t *)D.rowptr), idx((const GL uint32_t *)D.idx), len((const GL T *)D.len),
+          colptr((const GL int64_t *)D.colptr), crow((const GL uint32_t *)D.crow), clen((const GL T *)D.clen),
+          sub_row0((const GL int64_t *)D.sub_row0), g((GL T *)D.g), r((GL T *)D.r), u((GL T *)D.u),
+          v((GL T *)D.v), d((GL T *)D.d), w((GL T *)D.w), npix(D.npix), nrows(D.nrows), part_off(D.part_off),
+          cap(D.cap), nx(D.nx), ny(D.ny), K(D.K), lanes(D.lanes), job0(D.job0) {}
+};
+
+template <typename T>
+__device__ __forceinline__ const GL JobParams &jpar(const EngArgs<T> &a, int js) {
+    return ((const GL JobParams *)a.jp)[js];
+}
+template <typename T>
+__device__ __forceinline__ GL Scalars &jsc(const EngArgs<T> &a, int js) {
+    return ((GL Scalars *)a.sc)[js];
+}
+
+#ifndef MI_FP_G1
+#define MI_FP_G1 2      // vec4 groups per lane per trip of the K = 1 forward projector
+#endif
+#ifndef MI_NT_FP
+#define MI_NT_FP 1      // non-temporal matrix streams in the forward projector
+#endif
+#ifndef MI_NT_BP
+#define MI_NT_BP 0      // ... and in the back projector (measured slower with it)
+#endif
+
+// Matrix streams are read once per launch: non-temporal 16-byte loads, so
+// that they do not evict the image / residual lines the gathers reuse from L2.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+template <bool NT>
+__device__ __forceinline__ void stream_idx4(const GL uint32_t *p, uint32_t (&o)[4]) {
+    const u32x4 a = NT ? __builtin_nontemporal_load((const GL u32x4 *)p) : *(const GL u32x4 *)p;
+    o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+}
+template <bool NT>
+__device__ __forceinline__ void stream4(const GL float *p, float (&o)[4]) {
+    const f32x4 a = NT ? __builtin_nontemporal_load((const GL f32x4 *)p) : *(const GL f32x4 *)p;
+    o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+}
+template <bool NT>
+__device__ __forceinline__ void stream4(const GL double *p, double (&o)[4]) {
+    const f64x2 a = NT ? __builtin_nontemporal_load((const GL f64x2 *)p) : *(const GL f64x2 *)p;
+    const f64x2 b = NT ? __builtin_nontemporal_load((const GL f64x2 *)p + 1) : *((const GL f64x2 *)p + 1);
+    o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
 }
 
 // K consecutive values at p (p is aligned to K * sizeof(T)).
 template <typename T, int K>
-__device__ __forceinline__ void loadk(const T *__restrict__ p, T (&o)[K]) {
+__device__ __forceinline__ void loadk(const GL T *__restrict__ p, T (&o)[K]) {
     constexpr int B = K * (int)sizeof(T);
     if constexpr (B >= 16) {
 #pragma unroll
         for (int c = 0; c < B / 16; ++c) {
-            const uint4 a = reinterpret_cast<const uint4 *>(p)[c];
+            const u32x4 a = ((const GL u32x4 *)p)[c];
             __builtin_memcpy(&o[c * (16 / (int)sizeof(T))], &a, 16);
         }
     } else if constexpr (B == 8) {
-        const uint2 a = *reinterpret_cast<const uint2 *>(p);
+        const uint64_t a = *(const GL uint64_t *)p;
         __builtin_memcpy(&o[0], &a, 8);
     } else {
         o[0] = p[0];
@@ -48,19 +109,19 @@ __device__ __forceinline__ void loadk(const T *__restrict__ p, T (&o)[K]) {
 }
 
 template <typename T, int K>
-__device__ __forceinline__ void storek(T *__restrict__ p, const T (&o)[K]) {
+__device__ __forceinline__ void storek(GL T *__restrict__ p, const T (&o)[K]) {
     constexpr int B = K * (int)sizeof(T);
     if constexpr (B >= 16) {
 #pragma unroll
         for (int c = 0; c < B / 16; ++c) {
-            uint4 a;
+            u32x4 a;
             __builtin_memcpy(&a, &o[c * (16 / (int)sizeof(T))], 16);
-            reinterpret_cast<uint4 *>(p)[c] = a;
+            ((GL u32x4 *)p)[c] = a;
         }
     } else if constexpr (B == 8) {
-        uint2 a;
+        uint64_t a;
         __builtin_memcpy(&a, &o[0], 8);
-        *reinterpret_cast<uint2 *>(p) = a;
+        *(GL uint64_t *)p = a;
     } else {
         p[0] = o[0];
     }
@@ -109,25 +170,25 @@ __device__ __forceinline__ int find_unit(const BlockMap &m, int b) {
 
 template <typename T>
 __device__ __forceinline__ bool job_live(const EngArgs<T> &a, int js) {
-    return a.jp[js].active && !a.sc[js].done;
+    return jpar(a, js).active && !jsc(a, js).done;
 }
 
 template <typename T>
-__device__ __forceinline__ bool unit_idle(const EngArgs<T> &a, const UnitDesc<T> &U) {
+__device__ __forceinline__ bool unit_idle(const EngArgs<T> &a, const UG<T> &U) {
     for (int j = 0; j < U.K; ++j)
         if (job_live(a, U.job0 + j)) return false;
     return true;
 }
 
 template <typename T>
-__device__ __forceinline__ double *part_at(const EngArgs<T> &a, const UnitDesc<T> &U, int slot, int j) {
-    return a.part + U.part_off + ((int64_t)slot * U.K + j) * U.cap;
+__device__ __forceinline__ GL double *part_at(const EngArgs<T> &a, const UG<T> &U, int slot, int j) {
+    return (GL double *)a.part + U.part_off + ((int64_t)slot * U.K + j) * U.cap;
 }
 
 // Sum (or max) of one value per thread for each of K jobs; thread j < K
 // stores job j's block total at part_at(slot, j)[lb].  Fixed order.
 template <typename T, int K, bool MAX>
-__device__ __forceinline__ void block_store(const EngArgs<T> &a, const UnitDesc<T> &U, int slot, int lb,
+__device__ __forceinline__ void block_store(const EngArgs<T> &a, const UG<T> &U, int slot, int lb,
                                             const double (&m)[K], double *lds) {
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
 #pragma unroll
@@ -148,14 +209,14 @@ __device__ __forceinline__ void block_store(const EngArgs<T> &a, const UnitDesc<
 // rays per wave.  RESID: r = g - A u (stored), partial sum r^2; else partial
 // sum (A d)^2.
 template <typename T, int K, bool RESID>
-__device__ __forceinline__ void fp_unit(const EngArgs<T> &a, const UnitDesc<T> &U, int s, int lb, double *lds) {
-    constexpr int G = (K <= 2) ? 2 : 1;
+__device__ __forceinline__ void fp_unit(const EngArgs<T> &a, const UG<T> &U, int s, int lb, double *lds) {
+    constexpr int G = (K == 1) ? MI_FP_G1 : (K == 2 ? 2 : 1);
     constexpr int64_t kTrip = 4 * kWave;
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
     const int64_t row0 = U.sub_row0[s], nr = U.sub_row0[s + 1] - row0;
-    const T *__restrict__ x = RESID ? U.u : U.d;
-    const uint32_t *__restrict__ idx = U.idx;
-    const T *__restrict__ len = U.len;
+    const GL T *__restrict__ x = RESID ? U.u : U.d;
+    const GL uint32_t *__restrict__ idx = U.idx;
+    const GL T *__restrict__ len = U.len;
     double wpart = 0.0;
     const int64_t r0 = (int64_t)lb * kRaysPerBlock + wv * kRaysPerWave;
     for (int i = 0; i < kRaysPerWave; ++i) {
@@ -171,8 +232,8 @@ __device__ __forceinline__ void fp_unit(const EngArgs<T> &a, const UnitDesc<T> &
 #pragma unroll
             for (int q = 0; q < G; ++q) {
                 if (k0 + kTrip * q < e) {
-                    load_idx4(idx + k0 + kTrip * q, id[q]);
-                    load4(len + k0 + kTrip * q, ln[q]);
+                    stream_idx4<MI_NT_FP>(idx + k0 + kTrip * q, id[q]);
+                    stream4<MI_NT_FP>(len + k0 + kTrip * q, ln[q]);
                 } else {
 #pragma unroll
                     for (int m = 0; m < 4; ++m) { id[q][m] = 0; ln[q][m] = T(0); }
@@ -221,9 +282,9 @@ __device__ __forceinline__ void fp_unit(const EngArgs<T> &a, const UnitDesc<T> &
 // GoldenC image-gradient regulariser of job o of an interleaved pair (u, v)
 // at pixel (i, j): u zero padded, v one padded (origin/GoldenC/src/main.c:112-125).
 template <typename T>
-__device__ __forceinline__ double gc_reg_k(const T *u, const T *v, int i, int j, int nx, int ny, double eps, int K,
+__device__ __forceinline__ double gc_reg_k(const GL T *u, const GL T *v, int i, int j, int nx, int ny, double eps, int K,
                                            int o) {
-    auto at = [&](const T *arr, int ii, int jj, double pad) -> double {
+    auto at = [&](const GL T *arr, int ii, int jj, double pad) -> double {
         return (ii < 0 || jj < 0 || ii >= nx || jj >= ny) ? pad : (double)arr[((int64_t)ii * ny + jj) * K + o];
     };
     const double z = at(u, i, j, 0.0);
@@ -238,16 +299,16 @@ __device__ __forceinline__ double gc_reg_k(const T *u, const T *v, int i, int j,
 // Gather back projection of subset s over kBpPix pixels, U.lanes lanes per
 // pixel column, fused with d = S * A_s^T r + alpha * R(u, v); partial sum d^2.
 template <typename T, int K>
-__device__ __forceinline__ void bp_unit(const EngArgs<T> &a, const UnitDesc<T> &U, int s, int lb, double *lds) {
+__device__ __forceinline__ void bp_unit(const EngArgs<T> &a, const UG<T> &U, int s, int lb, double *lds) {
     constexpr int G = (K <= 2) ? 2 : 1;
     const int L = U.lanes;
     const int gl = threadIdx.x & (L - 1), grp = threadIdx.x / L;
     const int per = kBlock / L;
     const int64_t trip = 4 * (int64_t)L;
-    const int64_t *__restrict__ colptr = U.colptr + (int64_t)s * U.npix;
-    const uint32_t *__restrict__ crow = U.crow;
-    const T *__restrict__ clen = U.clen;
-    const T *__restrict__ r = U.r;
+    const GL int64_t *__restrict__ colptr = U.colptr + (int64_t)s * U.npix;
+    const GL uint32_t *__restrict__ crow = U.crow;
+    const GL T *__restrict__ clen = U.clen;
+    const GL T *__restrict__ r = U.r;
     const double Ssc = (double)a.S;
     for (int p0 = 0; p0 < kBpPix; p0 += per) {
         const int pl = p0 + grp;
@@ -262,8 +323,8 @@ __device__ __forceinline__ void bp_unit(const EngArgs<T> &a, const UnitDesc<T> &
 #pragma unroll
                 for (int q = 0; q < G; ++q) {
                     if (k0 + trip * q < e) {
-                        load_idx4(crow + k0 + trip * q, id[q]);
-                        load4(clen + k0 + trip * q, ln[q]);
+                        stream_idx4<MI_NT_BP>(crow + k0 + trip * q, id[q]);
+                        stream4<MI_NT_BP>(clen + k0 + trip * q, ln[q]);
                     } else {
 #pragma unroll
                         for (int m = 0; m < 4; ++m) { id[q][m] = 0; ln[q][m] = T(0); }
@@ -293,7 +354,7 @@ __device__ __forceinline__ void bp_unit(const EngArgs<T> &a, const UnitDesc<T> &
             const int pi = (int)(p / U.ny), pj = (int)(p % U.ny);
             if (gl < K) {
                 const int j = gl;
-                const JobParams &P = a.jp[U.job0 + j];
+                const GL JobParams &P = jpar(a, U.job0 + j);
                 double dv = Ssc * sv;
                 dv += P.alpha * gc_reg_k<T>(U.u, U.v, pi, pj, U.nx, U.ny, P.eps, K, j);
                 const T dt = (T)dv;
@@ -317,7 +378,7 @@ __device__ __forceinline__ void bp_unit(const EngArgs<T> &a, const UnitDesc<T> &
 // nV(d) = sum (v^2 + 0.01 eps^2)((d - d_up)^2 + (d - d_left)^2), d zero
 // padded (origin/GoldenC/src/main.c:152-158).
 template <typename T, int K>
-__device__ __forceinline__ void nv_unit(const EngArgs<T> &a, const UnitDesc<T> &U, int lb, double *lds) {
+__device__ __forceinline__ void nv_unit(const EngArgs<T> &a, const UG<T> &U, int lb, double *lds) {
     const int64_t q = (int64_t)lb * kPixBlock + threadIdx.x;
     double m[K];
 #pragma unroll
@@ -333,7 +394,7 @@ __device__ __forceinline__ void nv_unit(const EngArgs<T> &a, const UnitDesc<T> &
         else zerok<T, K>(dl);
 #pragma unroll
         for (int j = 0; j < K; ++j) {
-            const double eps = a.jp[U.job0 + j].eps;
+            const double eps = jpar(a, U.job0 + j).eps;
             const double e2 = 0.01 * (eps * eps);
             const double z = (double)dq[j];
             const double A = z - (double)du[j], B = z - (double)dl[j];
@@ -346,7 +407,7 @@ __device__ __forceinline__ void nv_unit(const EngArgs<T> &a, const UnitDesc<T> &
 
 // u = clamp(u + c d, lower, upper) for live jobs.
 template <typename T, int K>
-__device__ __forceinline__ void axpy_unit(const EngArgs<T> &a, const UnitDesc<T> &U, int lb) {
+__device__ __forceinline__ void axpy_unit(const EngArgs<T> &a, const UG<T> &U, int lb) {
     const int64_t q = (int64_t)lb * kPixBlock + threadIdx.x;
     if (q >= U.npix) return;
     T uu[K], dd[K];
@@ -356,8 +417,8 @@ __device__ __forceinline__ void axpy_unit(const EngArgs<T> &a, const UnitDesc<T>
     for (int j = 0; j < K; ++j) {
         const int js = U.job0 + j;
         if (!job_live(a, js)) continue;
-        const JobParams &P = a.jp[js];
-        double yn = (double)uu[j] + a.sc[js].c * (double)dd[j];
+        const GL JobParams &P = jpar(a, js);
+        double yn = (double)uu[j] + jsc(a, js).c * (double)dd[j];
         if (yn < P.lower) yn = P.lower;
         if (yn > P.upper) yn = P.upper;
         uu[j] = (T)yn;
@@ -374,14 +435,14 @@ __device__ __forceinline__ void cswap(T &x, T &y) {
 
 template <typename T>
 __device__ __forceinline__ bool median_due(const EngArgs<T> &a, int js) {
-    const JobParams &P = a.jp[js];
-    return job_live(a, js) && P.median_every > 0 && (a.sc[js].iters + 1) % P.median_every == 0;
+    const GL JobParams &P = jpar(a, js);
+    return job_live(a, js) && P.median_every > 0 && (jsc(a, js).iters + 1) % P.median_every == 0;
 }
 
 // w = 3x3 median of u (zero padding, origin/GoldenC/src/main.c:366-390) for
 // jobs whose median is due this pass, else w = u.
 template <typename T, int K>
-__device__ __forceinline__ void median_unit(const EngArgs<T> &a, const UnitDesc<T> &U, int lb) {
+__device__ __forceinline__ void median_unit(const EngArgs<T> &a, const UG<T> &U, int lb) {
     const int64_t q = (int64_t)lb * kPixBlock + threadIdx.x;
     if (q >= U.npix) return;
     const int i = (int)(q / U.ny), jj = (int)(q % U.ny);
@@ -419,7 +480,7 @@ __device__ __forceinline__ void median_unit(const EngArgs<T> &a, const UnitDesc<
 }
 
 template <typename T, int K>
-__device__ __forceinline__ void copy_wu_unit(const UnitDesc<T> &U, int lb) {
+__device__ __forceinline__ void copy_wu_unit(const UG<T> &U, int lb) {
     const int64_t q = (int64_t)lb * kPixBlock + threadIdx.x;
     if (q >= U.npix) return;
     T t[K];
@@ -429,13 +490,13 @@ __device__ __forceinline__ void copy_wu_unit(const UnitDesc<T> &U, int lb) {
 
 template <typename T>
 __device__ __forceinline__ bool edge_live(const EngArgs<T> &a, int js, int it) {
-    return job_live(a, js) && !a.sc[js].edge_done && it < a.jp[js].iter_edge;
+    return job_live(a, js) && !jsc(a, js).edge_done && it < jpar(a, js).iter_edge;
 }
 
 // Edge step, origin/GoldenC/src/main.c:294-305: w = -grad_v, partials
 // ||w||^2 and sum n w^2 (f = u zero padded, v one padded).
 template <typename T, int K>
-__device__ __forceinline__ void edge_grad_unit(const EngArgs<T> &a, const UnitDesc<T> &U, int lb, int it,
+__device__ __forceinline__ void edge_grad_unit(const EngArgs<T> &a, const UG<T> &U, int lb, int it,
                                                double *lds) {
     const int64_t q = (int64_t)lb * kPixBlock + threadIdx.x;
     double m0[K], m1[K];
@@ -444,7 +505,7 @@ __device__ __forceinline__ void edge_grad_unit(const EngArgs<T> &a, const UnitDe
     if (q < U.npix) {
         const int i = (int)(q / U.ny), jj = (int)(q % U.ny);
         T fd[K], fu[K], fr[K], fl[K], vc[K], vd[K], vu[K], vr[K], vl[K];
-        auto ldz = [&](const T *arr, int ii, int jc, T (&o)[K], T pad) {
+        auto ldz = [&](const GL T *arr, int ii, int jc, T (&o)[K], T pad) {
             if (ii < 0 || jc < 0 || ii >= U.nx || jc >= U.ny) {
 #pragma unroll
                 for (int j = 0; j < K; ++j) o[j] = pad;
@@ -467,7 +528,7 @@ __device__ __forceinline__ void edge_grad_unit(const EngArgs<T> &a, const UnitDe
         for (int j = 0; j < K; ++j) {
             const int js = U.job0 + j;
             if (!edge_live(a, js, it)) continue;
-            const JobParams &P = a.jp[js];
+            const GL JobParams &P = jpar(a, js);
             const double dxf = 0.5 * ((double)fd[j] - (double)fu[j]);
             const double dyf = 0.5 * ((double)fr[j] - (double)fl[j]);
             const double n = dxf * dxf + dyf * dyf;
@@ -489,7 +550,7 @@ __device__ __forceinline__ void edge_grad_unit(const EngArgs<T> &a, const UnitDe
 
 // main.c:311-329: |central gradient of w|^2.
 template <typename T, int K>
-__device__ __forceinline__ void edge_nabla_unit(const EngArgs<T> &a, const UnitDesc<T> &U, int lb, int it,
+__device__ __forceinline__ void edge_nabla_unit(const EngArgs<T> &a, const UG<T> &U, int lb, int it,
                                                 double *lds) {
     const int64_t q = (int64_t)lb * kPixBlock + threadIdx.x;
     double m[K];
@@ -519,7 +580,7 @@ __device__ __forceinline__ void edge_nabla_unit(const EngArgs<T> &a, const UnitD
 
 // main.c:332-354: v = clamp(v + c w, 0, 1); partial max |c w|.
 template <typename T, int K>
-__device__ __forceinline__ void edge_update_unit(const EngArgs<T> &a, const UnitDesc<T> &U, int lb, int it,
+__device__ __forceinline__ void edge_update_unit(const EngArgs<T> &a, const UG<T> &U, int lb, int it,
                                                  double *lds) {
     const int64_t q = (int64_t)lb * kPixBlock + threadIdx.x;
     double m[K];
@@ -533,7 +594,7 @@ __device__ __forceinline__ void edge_update_unit(const EngArgs<T> &a, const Unit
         for (int j = 0; j < K; ++j) {
             const int js = U.job0 + j;
             if (!edge_live(a, js, it)) continue;
-            const double cd = a.sc[js].e_c * (double)ww[j];
+            const double cd = jsc(a, js).e_c * (double)ww[j];
             double x = (double)vv[j] + cd;
             m[j] = fabs(cd);
             if (x > 1) x = 1;
@@ -567,7 +628,7 @@ __global__ __launch_bounds__(kBlock) void k_eng_fp(EngArgs<T> a, BlockMap m, int
     const int b = (int)xcd_block();
     if (b >= m.start[m.nunits]) return;
     const int ui = find_unit(m, b);
-    const UnitDesc<T> U = a.units[ui];
+    const UG<T> U(a.units[ui]);
     if (unit_idle(a, U)) return;
     const int lb = b - m.start[ui];
     MI_KSWITCH(KMAX, (fp_unit<T, K, RESID>(a, U, s, lb, lds)))
@@ -579,7 +640,7 @@ __global__ __launch_bounds__(kBlock) void k_eng_bp(EngArgs<T> a, BlockMap m, int
     const int b = (int)xcd_block();
     if (b >= m.start[m.nunits]) return;
     const int ui = find_unit(m, b);
-    const UnitDesc<T> U = a.units[ui];
+    const UG<T> U(a.units[ui]);
     if (unit_idle(a, U)) return;
     const int lb = b - m.start[ui];
     MI_KSWITCH(KMAX, (bp_unit<T, K>(a, U, s, lb, lds)))
@@ -598,7 +659,7 @@ __global__ __launch_bounds__(kBlock) void k_eng_fpsq_nv(EngArgs<T> a, BlockMap r
         const int b = x * r8 + t;
         if (b >= nrb) return;
         const int ui = find_unit(rows, b);
-        const UnitDesc<T> U = a.units[ui];
+        const UG<T> U(a.units[ui]);
         if (unit_idle(a, U)) return;
         const int lb = b - rows.start[ui];
         MI_KSWITCH(KMAX, (fp_unit<T, K, false>(a, U, s, lb, lds)))
@@ -606,7 +667,7 @@ __global__ __launch_bounds__(kBlock) void k_eng_fpsq_nv(EngArgs<T> a, BlockMap r
         const int b = x * p8 + (t - r8);
         if (b >= npb) return;
         const int ui = find_unit(pix, b);
-        const UnitDesc<T> U = a.units[ui];
+        const UG<T> U(a.units[ui]);
         if (unit_idle(a, U)) return;
         const int lb = b - pix.start[ui];
         MI_KSWITCH(KMAX, (nv_unit<T, K>(a, U, lb, lds)))
@@ -621,7 +682,7 @@ __global__ __launch_bounds__(kBlock) void k_eng_pix(EngArgs<T> a, BlockMap m, in
     const int b = (int)xcd_block();
     if (b >= m.start[m.nunits]) return;
     const int ui = find_unit(m, b);
-    const UnitDesc<T> U = a.units[ui];
+    const UG<T> U(a.units[ui]);
     if (unit_idle(a, U)) return;
     const int lb = b - m.start[ui];
     if constexpr (OP == PX_AXPY) { MI_KSWITCH(KMAX, (axpy_unit<T, K>(a, U, lb))) }
@@ -633,7 +694,7 @@ __global__ __launch_bounds__(kBlock) void k_eng_pix(EngArgs<T> a, BlockMap m, in
 }
 
 // Fixed-order block reduction of n partials (sum or max), 256 threads.
-__device__ __forceinline__ double fin_reduce(const double *p, int64_t n, bool mx, double *lds) {
+__device__ __forceinline__ double fin_reduce(const GL double *p, int64_t n, bool mx, double *lds) {
     double acc = 0.0;
     for (int64_t base = 0; base < n; base += (int64_t)kBlock * 8) {
         double v[8];
@@ -662,12 +723,12 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_eng_fin(EngArgs<T> a, int op, int s) {
     __shared__ double lds[kBlock / kWave];
     const int ui = blockIdx.x / kMaxK, j = blockIdx.x % kMaxK;
-    const UnitDesc<T> U = a.units[ui];
+    const UG<T> U(a.units[ui]);
     if (j >= U.K) return;
     const int js = U.job0 + j;
     if (!job_live(a, js)) return;
-    Scalars *sc = a.sc + js;
-    const JobParams &P = a.jp[js];
+    GL Scalars *sc = &jsc(a, js);
+    const GL JobParams &P = jpar(a, js);
     const int64_t npx = cdiv(U.npix, kPixBlock);
     switch (op) {
         case EF_BEGIN:
@@ -750,6 +811,14 @@ template <typename T>
 __global__ void k_move(T *dst, int Kd, int sd, const T *src, int Ks, int ss, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) dst[i * Kd + sd] = src[i * Ks + ss];
+}
+
+// Plain streaming copy, 16 bytes per thread per step (the reference kernel
+// against which memory counters are read).
+__global__ __launch_bounds__(kBlock) void k_copy16(const u32x4 *__restrict__ src, u32x4 *__restrict__ dst,
+                                                   int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        dst[i] = src[i];
 }
 
 unsigned grid8(int blocks) {
@@ -843,6 +912,14 @@ template <typename T>
 void eng_move(hipStream_t st, T *dst, int Kd, int sd, const T *src, int Ks, int ss, int64_t n) {
     if (n <= 0) return;
     k_move<T><<<grid_of(n), kBlock, 0, st>>>(dst, Kd, sd, src, Ks, ss, n);
+    MI_HIP(hipGetLastError());
+}
+
+void eng_copy(hipStream_t st, void *dst, const void *src, int64_t bytes) {
+    MI_REQUIRE(bytes % 16 == 0, MI_E_ARG, "copy size must be a multiple of 16 bytes");
+    const int64_t n = bytes / 16;
+    if (n == 0) return;
+    k_copy16<<<8192, kBlock, 0, st>>>((const u32x4 *)src, (u32x4 *)dst, n);
     MI_HIP(hipGetLastError());
 }
 

@@ -17,7 +17,10 @@ namespace mi {
 
 constexpr int kMaxK = 8;             // jobs per unit
 constexpr int kMaxUnits = 64;        // units per batch
-constexpr int kRaysPerWave = 4;
+#ifndef MI_RAYS_PER_WAVE
+#define MI_RAYS_PER_WAVE 4
+#endif
+constexpr int kRaysPerWave = MI_RAYS_PER_WAVE;   // consecutive rays per forward-projector wave
 constexpr int kRaysPerBlock = kRaysPerWave * (kBlock / kWave);   // 16
 constexpr int kBpPix = 64;           // pixels per back-projector block
 constexpr int kPixBlock = kBlock;    // pixels per stencil block
@@ -98,5 +101,8 @@ void eng_get(hipStream_t st, T *dst, const T *src, int K, int slot, int64_t n);
 // Copies slot `from` of a unit array to slot `to` of another (repacking).
 template <typename T>
 void eng_move(hipStream_t st, T *dst, int Kd, int sd, const T *src, int Ks, int ss, int64_t n);
+
+// Plain device copy (bytes a multiple of 16).
+void eng_copy(hipStream_t st, void *dst, const void *src, int64_t bytes);
 
 }  // namespace mi
