@@ -18,21 +18,24 @@ int pow2ceil(int n) {
 
 template <typename T>
 int64_t unit_state_bytes(const Matrix &m, int K) {
-    return (int64_t)K * (2 * m.nrows + 4 * (int64_t)m.npix) * (int64_t)sizeof(T);
-}
-
-int64_t max_subset_rows(const Matrix &m) {
-    int64_t r = 0;
-    for (int s = 0; s < m.S; ++s) r = std::max(r, m.sub_row0[s + 1] - m.sub_row0[s]);
-    return r;
+    return (int64_t)K * (2 * m.nrows + 4 * m.npix_t) * (int64_t)sizeof(T);
 }
 
 int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Blocks of the projector launches: four slices each.
+int64_t max_slices(const Sell &q) {
+    int64_t r = 0;
+    for (size_t s = 0; s + 1 < q.slice0.size(); ++s) r = std::max(r, q.slice0[s + 1] - q.slice0[s]);
+    return r;
+}
+int64_t fp_blocks(const Matrix &m) { return cdiv(max_slices(m.fs), kBlock / kWave); }
+int64_t bp_blocks(const Matrix &m) { return cdiv(max_slices(m.bs), kBlock / kWave); }
+int64_t px_blocks(const Matrix &m) { return cdiv(m.npix_t, kPixBlock); }
+
 // Partials per job per slot of a unit over `m`.
 int64_t part_cap(const Matrix &m) {
-    return std::max({cdiv(max_subset_rows(m), kRaysPerBlock), cdiv(m.npix, kBpPix), cdiv(m.npix, kPixBlock),
-                     (int64_t)1});
+    return std::max({fp_blocks(m), bp_blocks(m), px_blocks(m), (int64_t)1});
 }
 
 int64_t free_device_bytes() {
@@ -42,15 +45,6 @@ int64_t free_device_bytes() {
 }
 
 }  // namespace
-
-// Back-projector lanes per pixel of the engine: the matrix's tuned (or
-// heuristic) choice, within [8, 64] so that every K <= 8 fits one group.
-int engine_lanes(const Matrix &m) {
-    int l = bp_lanes(m);
-    if (l < 8) l = 8;
-    if (l > 64) l = 64;
-    return l;
-}
 
 template <typename T>
 Batch<T>::Batch(mi_context *c, int n_subsets) : ctx(c), S(n_subsets) {
@@ -143,6 +137,7 @@ Unit<T> *Batch<T>::place(BatchJob<T> &j, int group) {
         if (!units[i]) { uid = i; break; }
     MI_REQUIRE(uid >= 0, MI_E_LIMIT, "batch holds the maximum number of units");
     const int K = pow2ceil(std::min(kMaxK, std::max(std::max(group, min_k), 1)));
+    if (!j.m->has_sell) build_sell(*j.m, ctx->stream);
     const Matrix &m = *j.m;
     const int64_t need = unit_state_bytes<T>(m, K);
     if (free_device_bytes() - need < reserve_bytes) evict_idle(ctx, need + reserve_bytes - free_device_bytes());
@@ -151,15 +146,14 @@ Unit<T> *Batch<T>::place(BatchJob<T> &j, int group) {
     U->m = j.m;
     U->uid = uid;
     U->K = K;
-    U->lanes = engine_lanes(m);
     for (auto &x : U->jobs) x = -1;
     const hipStream_t st = ctx->stream;
     U->g.alloc((size_t)m.nrows * K);
     U->r.alloc((size_t)m.nrows * K);
-    U->u.alloc((size_t)m.npix * K);
-    U->v.alloc((size_t)m.npix * K);
-    U->d.alloc((size_t)m.npix * K);
-    U->w.alloc((size_t)m.npix * K);
+    U->u.alloc((size_t)m.npix_t * K);
+    U->v.alloc((size_t)m.npix_t * K);
+    U->d.alloc((size_t)m.npix_t * K);
+    U->w.alloc((size_t)m.npix_t * K);
     for (auto *b : {&U->g, &U->r, &U->u, &U->v, &U->d, &U->w}) MI_HIP(hipMemsetAsync(b->p, 0, b->bytes(), st));
     U->jobs[0] = j.id;
     j.uid = uid;
@@ -180,19 +174,20 @@ void Batch<T>::load_job(BatchJob<T> &j) {
     MI_HIP(hipMemcpyAsync(stage.p, j.sino.data(), m.nrows * sizeof(T), hipMemcpyHostToDevice, st));
     eng_put_rows<T>(st, U.g.p, U.K, j.slot, m.nrows, m.perm.p, stage.p);
     MI_HIP(hipStreamSynchronize(st));
+    const int nx = m.g.nx, ny = m.g.ny;
     if (!j.u0.empty()) {
         MI_HIP(hipMemcpyAsync(stage.p, j.u0.data(), m.npix * sizeof(T), hipMemcpyHostToDevice, st));
-        eng_put<T>(st, U.u.p, U.K, j.slot, m.npix, stage.p, T(0));
+        eng_put_img<T>(st, U.u.p, U.K, j.slot, nx, ny, m.tny, stage.p, T(0));
         MI_HIP(hipStreamSynchronize(st));
     } else {
-        eng_put<T>(st, U.u.p, U.K, j.slot, m.npix, nullptr, T(0));
+        eng_put_img<T>(st, U.u.p, U.K, j.slot, nx, ny, m.tny, nullptr, T(0));
     }
     if (!j.v0.empty()) {
         MI_HIP(hipMemcpyAsync(stage.p, j.v0.data(), m.npix * sizeof(T), hipMemcpyHostToDevice, st));
-        eng_put<T>(st, U.v.p, U.K, j.slot, m.npix, stage.p, T(1));
+        eng_put_img<T>(st, U.v.p, U.K, j.slot, nx, ny, m.tny, stage.p, T(1));
         MI_HIP(hipStreamSynchronize(st));
     } else {
-        eng_put<T>(st, U.v.p, U.K, j.slot, m.npix, nullptr, T(1));
+        eng_put_img<T>(st, U.v.p, U.K, j.slot, nx, ny, m.tny, nullptr, T(1));
     }
     JobParams &P = h_jp[js];
     P.alpha = j.p.alpha;
@@ -310,14 +305,13 @@ void Batch<T>::compact() {
             U->m = mp;
             U->uid = uid;
             U->K = K;
-            U->lanes = engine_lanes(*mp);
             for (auto &x : U->jobs) x = -1;
             U->g.alloc((size_t)mp->nrows * K);
             U->r.alloc((size_t)mp->nrows * K);
-            U->u.alloc((size_t)mp->npix * K);
-            U->v.alloc((size_t)mp->npix * K);
-            U->d.alloc((size_t)mp->npix * K);
-            U->w.alloc((size_t)mp->npix * K);
+            U->u.alloc((size_t)mp->npix_t * K);
+            U->v.alloc((size_t)mp->npix_t * K);
+            U->d.alloc((size_t)mp->npix_t * K);
+            U->w.alloc((size_t)mp->npix_t * K);
             for (auto *b : {&U->g, &U->r, &U->u, &U->v, &U->d, &U->w}) MI_HIP(hipMemsetAsync(b->p, 0, b->bytes(), st));
             for (int k = 0; k < K && next < ids.size(); ++k, ++next) {
                 BatchJob<T> &j = job(ids[next]);
@@ -326,8 +320,8 @@ void Batch<T>::compact() {
                     if (o->uid == j.uid) src = o.get();
                 const int from = j.uid * kMaxK + j.slot, to = uid * kMaxK + k;
                 eng_move<T>(st, U->g.p, K, k, src->g.p, src->K, j.slot, mp->nrows);
-                eng_move<T>(st, U->u.p, K, k, src->u.p, src->K, j.slot, mp->npix);
-                eng_move<T>(st, U->v.p, K, k, src->v.p, src->K, j.slot, mp->npix);
+                eng_move<T>(st, U->u.p, K, k, src->u.p, src->K, j.slot, mp->npix_t);
+                eng_move<T>(st, U->v.p, K, k, src->v.p, src->K, j.slot, mp->npix_t);
                 MI_HIP(hipMemcpyAsync(d_sc.p + to, sc_old.p + from, sizeof(Scalars), hipMemcpyDeviceToDevice, st));
                 h_jp[to] = jp_old[from];
                 U->jobs[k] = j.id;
@@ -374,12 +368,16 @@ void Batch<T>::rebuild() {
         const Matrix &m = *U.m;
         UnitDesc<T> &D = desc[i];
         std::memset(&D, 0, sizeof(D));
-        D.rowptr = m.rowptr.p;
-        D.idx = m.idx.p;
-        D.len = f64 ? (const T *)m.len64.p : (const T *)m.len32.p;
-        D.colptr = m.colptr.p;
-        D.crow = m.crow.p;
-        D.clen = f64 ? (const T *)m.clen64.p : (const T *)m.clen32.p;
+        D.fs_slice0 = m.fs.d_slice0.p;
+        D.fs_base = m.fs.base.p;
+        D.fs_T = m.fs.T.p;
+        D.fs_idx = m.fs.idx.p;
+        D.fs_len = f64 ? (const T *)m.fs.l64.p : (const T *)m.fs.l32.p;
+        D.bs_slice0 = m.bs.d_slice0.p;
+        D.bs_base = m.bs.base.p;
+        D.bs_T = m.bs.T.p;
+        D.bs_idx = m.bs.idx.p;
+        D.bs_len = f64 ? (const T *)m.bs.l64.p : (const T *)m.bs.l32.p;
         D.sub_row0 = m.d_sub_row0.p;
         D.g = U.g.p;
         D.r = U.r.p;
@@ -387,20 +385,22 @@ void Batch<T>::rebuild() {
         D.v = U.v.p;
         D.d = U.d.p;
         D.w = U.w.p;
-        D.npix = m.npix;
+        D.npix_t = m.npix_t;
         D.nrows = m.nrows;
         D.cap = part_cap(m);
         D.part_off = off;
         off += (int64_t)SL_COUNT * U.K * D.cap;
         D.nx = m.g.nx;
         D.ny = m.g.ny;
+        D.tny = m.tny;
         D.K = U.K;
-        D.lanes = U.lanes;
+        D.fs_P = m.fs.P;
+        D.bs_P = m.bs.P;
         D.job0 = U.uid * kMaxK;
-        MI_REQUIRE(D.lanes >= U.K, MI_E_STATE, "back-projector lanes below K");
-        rows.start[i + 1] = rows.start[i] + (int)cdiv(max_subset_rows(m), kRaysPerBlock);
-        bp.start[i + 1] = bp.start[i] + (int)cdiv(m.npix, kBpPix);
-        pix.start[i + 1] = pix.start[i] + (int)cdiv(m.npix, kPixBlock);
+        MI_REQUIRE(m.has_sell && (!f64 || m.has64), MI_E_STATE, "matrix lacks the engine's sliced layout");
+        rows.start[i + 1] = rows.start[i] + (int)fp_blocks(m);
+        bp.start[i + 1] = bp.start[i] + (int)bp_blocks(m);
+        pix.start[i + 1] = pix.start[i] + (int)px_blocks(m);
         if (U.K > 1) kmax = kMaxK;
         for (int k = 0; k < U.K; ++k) {
             if (U.jobs[k] < 0) continue;
@@ -442,19 +442,19 @@ void Batch<T>::record_pass(bool timed) {
         auto nr_s = [&](const Unit<T> &U) { return (double)(U.m->sub_row0[s + 1] - U.m->sub_row0[s]); };
         {
             LaunchTimer lt(ctx, MI_K_FP, sum([&](const Unit<T> &U) {
-                return nnz_s(U) * (4 + sz) + (nr_s(U) + 1) * 8 + U.m->npix * sz * U.K + 2 * nr_s(U) * sz * U.K;
+                return nnz_s(U) * (4 + sz) + U.m->npix * sz * U.K + 2 * nr_s(U) * sz * U.K;
             }));
             eng_fp_resid<T>(st, args, rows, s, kmax);
         }
         {
             LaunchTimer lt(ctx, MI_K_BP, sum([&](const Unit<T> &U) {
-                return nnz_s(U) * (4 + sz) + (U.m->npix + 1.0) * 8 + nr_s(U) * sz * U.K + 3.0 * U.m->npix * sz * U.K;
+                return nnz_s(U) * (4 + sz) + nr_s(U) * sz * U.K + 3.0 * U.m->npix * sz * U.K;
             }));
             eng_bp_grad<T>(st, args, bp, s, kmax);
         }
         {
             LaunchTimer lt(ctx, MI_K_FPSQ, sum([&](const Unit<T> &U) {
-                return nnz_s(U) * (4 + sz) + (nr_s(U) + 1) * 8 + 3.0 * U.m->npix * sz * U.K;
+                return nnz_s(U) * (4 + sz) + 3.0 * U.m->npix * sz * U.K;
             }));
             eng_fp_sq_nv<T>(st, args, rows, pix, s, kmax);
         }
@@ -510,10 +510,10 @@ void Batch<T>::retire(BatchJob<T> &j) {
     j.u.resize(m.npix);
     j.v.resize(m.npix);
     ensure_stage(m.npix);
-    eng_get<T>(st, stage.p, U.u.p, U.K, j.slot, m.npix);
+    eng_get_img<T>(st, stage.p, U.u.p, U.K, j.slot, m.g.nx, m.g.ny, m.tny);
     MI_HIP(hipMemcpyAsync(j.u.data(), stage.p, m.npix * sizeof(T), hipMemcpyDeviceToHost, st));
     MI_HIP(hipStreamSynchronize(st));
-    eng_get<T>(st, stage.p, U.v.p, U.K, j.slot, m.npix);
+    eng_get_img<T>(st, stage.p, U.v.p, U.K, j.slot, m.g.nx, m.g.ny, m.tny);
     MI_HIP(hipMemcpyAsync(j.v.data(), stage.p, m.npix * sizeof(T), hipMemcpyDeviceToHost, st));
     MI_HIP(hipStreamSynchronize(st));
     j.state = BJ_DONE;
@@ -586,16 +586,17 @@ void Batch<T>::read(int64_t id, T *u, T *v) {
     }
     MI_REQUIRE(j.state == BJ_ACTIVE, MI_E_STATE, "job has no state on the device");
     Unit<T> &U = *units[j.uid];
-    const int64_t npix = U.m->npix;
+    const Matrix &m = *U.m;
+    const int64_t npix = m.npix;
     const hipStream_t st = ctx->stream;
     ensure_stage(npix);
     if (u) {
-        eng_get<T>(st, stage.p, U.u.p, U.K, j.slot, npix);
+        eng_get_img<T>(st, stage.p, U.u.p, U.K, j.slot, m.g.nx, m.g.ny, m.tny);
         MI_HIP(hipMemcpyAsync(u, stage.p, npix * sizeof(T), hipMemcpyDeviceToHost, st));
         MI_HIP(hipStreamSynchronize(st));
     }
     if (v) {
-        eng_get<T>(st, stage.p, U.v.p, U.K, j.slot, npix);
+        eng_get_img<T>(st, stage.p, U.v.p, U.K, j.slot, m.g.nx, m.g.ny, m.tny);
         MI_HIP(hipMemcpyAsync(v, stage.p, npix * sizeof(T), hipMemcpyDeviceToHost, st));
         MI_HIP(hipStreamSynchronize(st));
     }
@@ -621,53 +622,5 @@ void Batch<T>::release(int64_t id) {
 
 template struct Batch<float>;
 template struct Batch<double>;
-
-// Times the engine's back projector for each lanes-per-pixel candidate on a
-// zero job of the matrix and keeps the fastest in m.tune.bp_lanes.
-void autotune_lanes(mi_context *ctx, const std::shared_ptr<Matrix> &mp) {
-    Matrix &m = *mp;
-    const hipStream_t st = ctx->stream;
-    Batch<float> b(ctx, m.S);
-    mi_at_params p{};
-    p.alpha = 1e-3;
-    p.beta = 1e-4;
-    p.epsilon = 1e-2;
-    p.n_subsets = m.S;
-    p.lower = -1e30;
-    p.upper = 1e30;
-    std::vector<float> zero((size_t)m.nrows, 0.f);
-    b.submit(m.g, p, zero.data(), nullptr, nullptr);
-    b.admit(true);
-    b.enqueue(0);
-    b.dirty = true;
-    hipEvent_t e0, e1;
-    MI_HIP(hipEventCreate(&e0));
-    MI_HIP(hipEventCreate(&e1));
-    double best = 1e30;
-    int best_l = 8;
-    const bool was_on = ctx->stats.on;
-    ctx->stats.on = false;
-    for (int L : {8, 16, 32, 64}) {
-        for (int uid : b.live) b.units[uid]->lanes = L;
-        b.rebuild();
-        for (int rep = 0; rep < 2; ++rep) {
-            MI_HIP(hipEventRecord(e0, st));
-            for (int s = 0; s < m.S; ++s) eng_bp_grad<float>(st, b.args, b.bp, s, b.kmax);
-            MI_HIP(hipEventRecord(e1, st));
-            MI_HIP(hipEventSynchronize(e1));
-        }
-        float ms = 0.f;
-        MI_HIP(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best) {
-            best = ms;
-            best_l = L;
-        }
-    }
-    ctx->stats.on = was_on;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    m.tune.bp_lanes = best_l;
-    m.tune.bp_us = 1e3 * best / m.S;
-}
 
 }  // namespace mi

@@ -30,7 +30,6 @@ struct Unit {
     std::shared_ptr<Matrix> m;
     int uid = -1;                  // stable: job slots uid * kMaxK + j
     int K = 1;
-    int lanes = 8;
     int64_t jobs[kMaxK];           // job id per slot, -1 when free
     DevBuf<T> g, r, u, v, d, w;
     int used() const {
@@ -125,10 +124,5 @@ struct Batch {
     void drop_graph();
     void ensure_stage(int64_t n);
 };
-
-// Engine back-projector lanes per pixel of a matrix (8..64).
-int engine_lanes(const Matrix &m);
-// Times the engine's back projector per lanes candidate; stores the fastest.
-void autotune_lanes(mi_context *ctx, const std::shared_ptr<Matrix> &m);
 
 }  // namespace mi

@@ -345,7 +345,6 @@ int mi_autotune(mi_context *ctx, const mi_geometry *g, int n_subsets, mi_tuning 
         MI_REQUIRE(g, MI_E_ARG, "null geometry");
         auto m = get_matrix(ctx, *g, n_subsets, false);
         autotune(ctx, *m);
-        autotune_lanes(ctx, m);
         if (out) *out = m->tune;
     });
 }

@@ -7,12 +7,13 @@
 // for that K; a launch whose units all have K = 1 uses a kernel compiled for
 // K = 1 alone (plain form), so it keeps the register budget of that case.
 //
-// Per-entry work per lane: entries are taken in groups of four consecutive
-// entries (16-byte loads of the padded idx/len streams) from the 4-aligned
-// entry at or below the row start; group g of a row goes to lane g % 64 (or
-// g % lanes-per-pixel), and a lane visits its groups in increasing order.
-// How many groups a lane loads per trip (G) only changes how many loads are
-// in flight, never the order of the Kahan sums.
+// Projectors read the sliced matrices (Sell, runtime.hpp): a lane owns one
+// segment of one ray (forward) or of one pixel column (back) and walks it in
+// entry order with a Kahan sum, each step of a wave reading one contiguous
+// 256-byte row of the index and length streams; neighbouring lanes own
+// neighbouring rays / pixels, so one gather instruction touches few cache
+// lines.  Images are stored in 4 x 8 tiles (tile_index).  The P segment sums
+// of an item are combined by a fixed butterfly.
 #include "engine.hpp"
 
 #include <cmath>
@@ -30,22 +31,31 @@ namespace {
 
 template <typename T>
 struct UG {
-    const GL int64_t *rowptr;
-    const GL uint32_t *idx;
-    const GL T *len;
-    const GL int64_t *colptr;
-    const GL uint32_t *crow;
-    const GL T *clen;
+    const GL int64_t *fs_slice0, *fs_base;
+    const GL int32_t *fs_T;
+    const GL uint32_t *fs_idx;
+    const GL T *fs_len;
+    const GL int64_t *bs_slice0, *bs_base;
+    const GL int32_t *bs_T;
+    const GL uint32_t *bs_idx;
+    const GL T *bs_len;
     const GL int64_t *sub_row0;
     GL T *g, *r, *u, *v, *d, *w;
-    int64_t npix, nrows, part_off, cap;
-    int nx, ny, K, lanes, job0;
+    int64_t npix_t, nrows, part_off, cap;
+    int nx, ny, tny, K, fs_P, bs_P, job0;
     __device__ __forceinline__ explicit UG(const UnitDesc<T> &D)
-        : rowptr((const GL int64_t *)D.rowptr), idx((const GL uint32_t *)D.idx), len((const GL T *)D.len),
-          colptr((const GL int64_t *)D.colptr), crow((const GL uint32_t *)D.crow), clen((const GL T *)D.clen),
+        : fs_slice0((const GL int64_t *)D.fs_slice0), fs_base((const GL int64_t *)D.fs_base),
+          fs_T((const GL int32_t *)D.fs_T), fs_idx((const GL uint32_t *)D.fs_idx), fs_len((const GL T *)D.fs_len),
+          bs_slice0((const GL int64_t *)D.bs_slice0), bs_base((const GL int64_t *)D.bs_base),
+          bs_T((const GL int32_t *)D.bs_T), bs_idx((const GL uint32_t *)D.bs_idx), bs_len((const GL T *)D.bs_len),
           sub_row0((const GL int64_t *)D.sub_row0), g((GL T *)D.g), r((GL T *)D.r), u((GL T *)D.u),
-          v((GL T *)D.v), d((GL T *)D.d), w((GL T *)D.w), npix(D.npix), nrows(D.nrows), part_off(D.part_off),
-          cap(D.cap), nx(D.nx), ny(D.ny), K(D.K), lanes(D.lanes), job0(D.job0) {}
+          v((GL T *)D.v), d((GL T *)D.d), w((GL T *)D.w), npix_t(D.npix_t), nrows(D.nrows),
+          part_off(D.part_off), cap(D.cap), nx(D.nx), ny(D.ny), tny(D.tny), K(D.K), fs_P(D.fs_P), bs_P(D.bs_P),
+          job0(D.job0) {}
+    // Tiled index of pixel (i, j), or -1 outside the image.
+    __device__ __forceinline__ int64_t at(int i, int j) const {
+        return (i < 0 || j < 0 || i >= nx || j >= ny) ? -1 : tile_index(i, j, tny);
+    }
 };
 
 template <typename T>
@@ -57,37 +67,24 @@ __device__ __forceinline__ GL Scalars &jsc(const EngArgs<T> &a, int js) {
     return ((GL Scalars *)a.sc)[js];
 }
 
-#ifndef MI_FP_G1
-#define MI_FP_G1 2      // vec4 groups per lane per trip of the K = 1 forward projector
-#endif
 #ifndef MI_NT_FP
 #define MI_NT_FP 1      // non-temporal matrix streams in the forward projector
 #endif
 #ifndef MI_NT_BP
 #define MI_NT_BP 0      // ... and in the back projector (measured slower with it)
 #endif
+#ifndef MI_STEPS
+#define MI_STEPS 8      // steps of a projector lane in flight (K <= 2; 4 above)
+#endif
 
-// Matrix streams are read once per launch: non-temporal 16-byte loads, so
-// that they do not evict the image / residual lines the gathers reuse from L2.
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef double f64x2 __attribute__((ext_vector_type(2)));
 
-template <bool NT>
-__device__ __forceinline__ void stream_idx4(const GL uint32_t *p, uint32_t (&o)[4]) {
-    const u32x4 a = NT ? __builtin_nontemporal_load((const GL u32x4 *)p) : *(const GL u32x4 *)p;
-    o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
-}
-template <bool NT>
-__device__ __forceinline__ void stream4(const GL float *p, float (&o)[4]) {
-    const f32x4 a = NT ? __builtin_nontemporal_load((const GL f32x4 *)p) : *(const GL f32x4 *)p;
-    o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
-}
-template <bool NT>
-__device__ __forceinline__ void stream4(const GL double *p, double (&o)[4]) {
-    const f64x2 a = NT ? __builtin_nontemporal_load((const GL f64x2 *)p) : *(const GL f64x2 *)p;
-    const f64x2 b = NT ? __builtin_nontemporal_load((const GL f64x2 *)p + 1) : *((const GL f64x2 *)p + 1);
-    o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
+// Matrix streams are read once per launch; non-temporal loads keep them from
+// evicting the image / residual lines that the gathers reuse from L2.
+template <bool NT, typename V>
+__device__ __forceinline__ V stream_ld(const GL V *p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
 }
 
 // K consecutive values at p (p is aligned to K * sizeof(T)).
@@ -133,29 +130,12 @@ __device__ __forceinline__ void zerok(T (&o)[K]) {
     for (int j = 0; j < K; ++j) o[j] = T(0);
 }
 
-// Sums v[j] over each aligned group of L lanes (L >= K, powers of two) and
-// leaves job (lane % K)'s total in v[0].  The levels pair lane with lane ^ o
-// for o = L/2 .. 1, as a plain butterfly does, so every total is bit-for-bit
-// the butterfly's; below o = K each lane keeps the half of its K values that
-// its lane bit selects instead of all of them, which needs no dynamic
-// register indexing.
+// Sums v[j] over aligned groups of P lanes (fixed butterfly).
 template <int K>
-__device__ __forceinline__ double reduce_scatter(double (&v)[K], int L) {
-    for (int o = L >> 1; o >= K; o >>= 1)
+__device__ __forceinline__ void group_sums(double (&v)[K], int P) {
+    for (int o = P >> 1; o > 0; o >>= 1)
 #pragma unroll
         for (int j = 0; j < K; ++j) v[j] += __shfl_xor(v[j], o, kWave);
-    const int lane = threadIdx.x & (kWave - 1);
-#pragma unroll
-    for (int h = K / 2; h >= 1; h >>= 1) {
-        const bool up = (lane & h) != 0;
-#pragma unroll
-        for (int j = 0; j < h; ++j) {
-            const double send = up ? v[j] : v[j + h];
-            const double keep = up ? v[j + h] : v[j];
-            v[j] = keep + __shfl_xor(send, h, kWave);
-        }
-    }
-    return v[0];
 }
 
 __device__ __forceinline__ int find_unit(const BlockMap &m, int b) {
@@ -205,173 +185,165 @@ __device__ __forceinline__ void block_store(const EngArgs<T> &a, const UG<T> &U,
 }
 
 // ---------------------------------------------------------------------------
-// Forward projection of subset s: one wave per ray, kRaysPerWave consecutive
-// rays per wave.  RESID: r = g - A u (stored), partial sum r^2; else partial
-// sum (A d)^2.
+// One projector lane: its segment's entries t = 0 .. n-1 at p0 + 64 t,
+// Kahan-summed in entry order (zero-length padding skipped), gathering K
+// interleaved values of `x` per entry.
+template <typename T, int K, bool NT>
+__device__ __forceinline__ void walk(const GL uint32_t *__restrict__ ip, const GL T *__restrict__ lp, int n,
+                                     const GL T *__restrict__ x, Acc<T> (&acc)[K]) {
+    constexpr int UN = (K <= 2) ? MI_STEPS : 4;
+    int t = 0;
+    for (; t + UN <= n; t += UN) {
+        uint32_t id[UN];
+        T ln[UN], xv[UN][K];
+#pragma unroll
+        for (int q = 0; q < UN; ++q) {
+            id[q] = stream_ld<NT>(ip + (int64_t)(t + q) * kWave);
+            ln[q] = stream_ld<NT>(lp + (int64_t)(t + q) * kWave);
+        }
+#pragma unroll
+        for (int q = 0; q < UN; ++q) loadk<T, K>(x + (uint32_t)(id[q] * K), xv[q]);
+#pragma unroll
+        for (int q = 0; q < UN; ++q)
+            if (ln[q] != T(0))
+#pragma unroll
+                for (int j = 0; j < K; ++j) acc[j].add(xv[q][j] * ln[q]);
+    }
+    for (; t < n; ++t) {
+        const uint32_t id = stream_ld<NT>(ip + (int64_t)t * kWave);
+        const T ln = stream_ld<NT>(lp + (int64_t)t * kWave);
+        T xv[K];
+        loadk<T, K>(x + (uint32_t)(id * K), xv);
+        if (ln != T(0))
+#pragma unroll
+            for (int j = 0; j < K; ++j) acc[j].add(xv[j] * ln);
+    }
+}
+
+// Forward projection of subset s: a block is 4 slices (one per wave).
+// RESID: r = g - A u (stored), partial sum r^2; else partial sum (A d)^2.
 template <typename T, int K, bool RESID>
 __device__ __forceinline__ void fp_unit(const EngArgs<T> &a, const UG<T> &U, int s, int lb, double *lds) {
-    constexpr int G = (K == 1) ? MI_FP_G1 : (K == 2 ? 2 : 1);
-    constexpr int64_t kTrip = 4 * kWave;
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
-    const int64_t row0 = U.sub_row0[s], nr = U.sub_row0[s + 1] - row0;
-    const GL T *__restrict__ x = RESID ? U.u : U.d;
-    const GL uint32_t *__restrict__ idx = U.idx;
-    const GL T *__restrict__ len = U.len;
-    double wpart = 0.0;
-    const int64_t r0 = (int64_t)lb * kRaysPerBlock + wv * kRaysPerWave;
-    for (int i = 0; i < kRaysPerWave; ++i) {
-        const int64_t r = r0 + i;
-        if (r >= nr) break;
-        const int64_t rr = row0 + r;
-        const int64_t b = U.rowptr[rr], e = U.rowptr[rr + 1];
+    const int P = U.fs_P, C = kWave / P;
+    const int64_t sl0 = U.fs_slice0[s], li = (int64_t)lb * (kBlock / kWave) + wv;
+    double m[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) m[j] = 0.0;
+    if (li < U.fs_slice0[s + 1] - sl0) {
+        const int64_t sl = sl0 + li, base = U.fs_base[sl] + lane;
         Acc<T> acc[K];
-        for (int64_t k0 = (b & ~(int64_t)3) + 4 * lane; k0 < e; k0 += kTrip * G) {
-            uint32_t id[G][4];
-            T ln[G][4];
-            T xv[G][4][K];
-#pragma unroll
-            for (int q = 0; q < G; ++q) {
-                if (k0 + kTrip * q < e) {
-                    stream_idx4<MI_NT_FP>(idx + k0 + kTrip * q, id[q]);
-                    stream4<MI_NT_FP>(len + k0 + kTrip * q, ln[q]);
-                } else {
-#pragma unroll
-                    for (int m = 0; m < 4; ++m) { id[q][m] = 0; ln[q][m] = T(0); }
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < G; ++q)
-#pragma unroll
-                for (int m = 0; m < 4; ++m) {
-                    if (k0 + kTrip * q < e) loadk<T, K>(x + (uint32_t)(id[q][m] * K), xv[q][m]);
-                    else zerok<T, K>(xv[q][m]);
-                }
-#pragma unroll
-            for (int q = 0; q < G; ++q)
-#pragma unroll
-                for (int m = 0; m < 4; ++m) {
-                    const int64_t kk = k0 + kTrip * q + m;
-                    if (kk >= b && kk < e)
-#pragma unroll
-                        for (int j = 0; j < K; ++j) acc[j].add(xv[q][m][j] * ln[q][m]);
-                }
-        }
+        walk<T, K, MI_NT_FP>(U.fs_idx + base, U.fs_len + base, U.fs_T[sl], RESID ? U.u : U.d, acc);
         double tot[K];
 #pragma unroll
         for (int j = 0; j < K; ++j) tot[j] = acc[j].value();
-        const double sv = reduce_scatter<K>(tot, kWave);
-        if (lane < K) {
+        group_sums<K>(tot, P);
+        const int64_t row = U.sub_row0[s] + li * C + lane / P;
+        if (lane % P == 0 && row < U.sub_row0[s + 1]) {
             if (RESID) {
-                const T o = (T)((double)U.g[rr * K + lane] - sv);
-                U.r[rr * K + lane] = o;
-                wpart += (double)o * (double)o;
+                T gg[K], o[K];
+                loadk<T, K>(U.g + row * K, gg);
+#pragma unroll
+                for (int j = 0; j < K; ++j) {
+                    o[j] = (T)((double)gg[j] - tot[j]);
+                    m[j] = (double)o[j] * (double)o[j];
+                }
+                storek<T, K>(U.r + row * K, o);
             } else {
-                wpart += sv * sv;
+#pragma unroll
+                for (int j = 0; j < K; ++j) m[j] = tot[j] * tot[j];
             }
         }
     }
-    if (lane < K) lds[wv * K + lane] = wpart;
-    __syncthreads();
-    if (threadIdx.x < K) {
-        double t = 0.0;
-        for (int w = 0; w < kBlock / kWave; ++w) t += lds[w * K + threadIdx.x];
-        part_at(a, U, RESID ? SL_DF : SL_AD, threadIdx.x)[lb] = t;
+    block_store<T, K, false>(a, U, RESID ? SL_DF : SL_AD, lb, m, lds);
+}
+
+// GoldenC image-gradient regulariser of K interleaved jobs at pixel (i, j):
+// u zero padded, v one padded (origin/GoldenC/src/main.c:112-125).
+template <typename T, int K>
+__device__ __forceinline__ void gc_reg(const UG<T> &U, int i, int j, const double (&eps)[K], double (&out)[K]) {
+    T z[K], zd[K], zr[K], zu[K], zl[K], p[K], pu[K], pl[K];
+    auto ld = [&](const GL T *arr, int ii, int jj, T (&o)[K], T pad) {
+        const int64_t q = U.at(ii, jj);
+        if (q < 0) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) o[k] = pad;
+        } else {
+            loadk<T, K>(arr + q * K, o);
+        }
+    };
+    ld(U.u, i, j, z, T(0));
+    ld(U.u, i + 1, j, zd, T(0));
+    ld(U.u, i, j + 1, zr, T(0));
+    ld(U.u, i - 1, j, zu, T(0));
+    ld(U.u, i, j - 1, zl, T(0));
+    ld(U.v, i, j, p, T(1));
+    ld(U.v, i - 1, j, pu, T(1));
+    ld(U.v, i, j - 1, pl, T(1));
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double Z = z[k], Zd = zd[k], Zr = zr[k], Zu = zu[k], Zl = zl[k];
+        const double Pp = p[k], Pu = pu[k], Pl = pl[k];
+        out[k] = +(Pp * Pp) * (Zd - Z) + (Pp * Pp) * (Zr - Z) - (Pu * Pu) * (Z - Zu) - (Pl * Pl) * (Z - Zl) +
+                 0.01 * (eps[k] * eps[k]) * (Zd + Zr - 4 * Z + Zu + Zl);
     }
 }
 
-// GoldenC image-gradient regulariser of job o of an interleaved pair (u, v)
-// at pixel (i, j): u zero padded, v one padded (origin/GoldenC/src/main.c:112-125).
-template <typename T>
-__device__ __forceinline__ double gc_reg_k(const GL T *u, const GL T *v, int i, int j, int nx, int ny, double eps, int K,
-                                           int o) {
-    auto at = [&](const GL T *arr, int ii, int jj, double pad) -> double {
-        return (ii < 0 || jj < 0 || ii >= nx || jj >= ny) ? pad : (double)arr[((int64_t)ii * ny + jj) * K + o];
-    };
-    const double z = at(u, i, j, 0.0);
-    const double zd = at(u, i + 1, j, 0.0), zr = at(u, i, j + 1, 0.0);
-    const double zu = at(u, i - 1, j, 0.0), zl = at(u, i, j - 1, 0.0);
-    const double p = at(v, i, j, 1.0);
-    const double pu = at(v, i - 1, j, 1.0), pl = at(v, i, j - 1, 1.0);
-    return +(p * p) * (zd - z) + (p * p) * (zr - z) - (pu * pu) * (z - zu) - (pl * pl) * (z - zl) +
-           0.01 * (eps * eps) * (zd + zr - 4 * z + zu + zl);
-}
-
-// Gather back projection of subset s over kBpPix pixels, U.lanes lanes per
-// pixel column, fused with d = S * A_s^T r + alpha * R(u, v); partial sum d^2.
+// Gather back projection of subset s over 4 slices of tiled pixels, fused
+// with d = S * A_s^T r + alpha * R(u, v); partial sum d^2.
 template <typename T, int K>
 __device__ __forceinline__ void bp_unit(const EngArgs<T> &a, const UG<T> &U, int s, int lb, double *lds) {
-    constexpr int G = (K <= 2) ? 2 : 1;
-    const int L = U.lanes;
-    const int gl = threadIdx.x & (L - 1), grp = threadIdx.x / L;
-    const int per = kBlock / L;
-    const int64_t trip = 4 * (int64_t)L;
-    const GL int64_t *__restrict__ colptr = U.colptr + (int64_t)s * U.npix;
-    const GL uint32_t *__restrict__ crow = U.crow;
-    const GL T *__restrict__ clen = U.clen;
-    const GL T *__restrict__ r = U.r;
-    const double Ssc = (double)a.S;
-    for (int p0 = 0; p0 < kBpPix; p0 += per) {
-        const int pl = p0 + grp;
-        const int64_t p = (int64_t)lb * kBpPix + pl;
-        if (p < U.npix) {
-            const int64_t b = colptr[p], e = colptr[p + 1];
-            Acc<T> acc[K];
-            for (int64_t k0 = (b & ~(int64_t)3) + 4 * gl; k0 < e; k0 += trip * G) {
-                uint32_t id[G][4];
-                T ln[G][4];
-                T rv[G][4][K];
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    const int P = U.bs_P, C = kWave / P;
+    const int64_t sl0 = U.bs_slice0[s], li = (int64_t)lb * (kBlock / kWave) + wv;
+    double m[K];
 #pragma unroll
-                for (int q = 0; q < G; ++q) {
-                    if (k0 + trip * q < e) {
-                        stream_idx4<MI_NT_BP>(crow + k0 + trip * q, id[q]);
-                        stream4<MI_NT_BP>(clen + k0 + trip * q, ln[q]);
-                    } else {
+    for (int j = 0; j < K; ++j) m[j] = 0.0;
+    if (li < U.bs_slice0[s + 1] - sl0) {
+        const int64_t sl = sl0 + li, base = U.bs_base[sl] + lane;
+        Acc<T> acc[K];
+        walk<T, K, MI_NT_BP>(U.bs_idx + base, U.bs_len + base, U.bs_T[sl], U.r, acc);
+        double tot[K];
 #pragma unroll
-                        for (int m = 0; m < 4; ++m) { id[q][m] = 0; ln[q][m] = T(0); }
-                    }
+        for (int j = 0; j < K; ++j) tot[j] = acc[j].value();
+        group_sums<K>(tot, P);
+        const int64_t q = li * C + lane / P;
+        if (lane % P == 0 && q < U.npix_t) {
+            int i, j;
+            tile_coords(q, U.tny, i, j);
+            T dv[K];
+            if (i < U.nx && j < U.ny) {
+                double eps[K], reg[K];
+#pragma unroll
+                for (int k = 0; k < K; ++k) eps[k] = jpar(a, U.job0 + k).eps;
+                gc_reg<T, K>(U, i, j, eps, reg);
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const double d = (double)a.S * tot[k] + jpar(a, U.job0 + k).alpha * reg[k];
+                    dv[k] = (T)d;
+                    m[k] = (double)dv[k] * (double)dv[k];
                 }
-#pragma unroll
-                for (int q = 0; q < G; ++q)
-#pragma unroll
-                    for (int m = 0; m < 4; ++m) {
-                        if (k0 + trip * q < e) loadk<T, K>(r + (uint32_t)(id[q][m] * K), rv[q][m]);
-                        else zerok<T, K>(rv[q][m]);
-                    }
-#pragma unroll
-                for (int q = 0; q < G; ++q)
-#pragma unroll
-                    for (int m = 0; m < 4; ++m) {
-                        const int64_t kk = k0 + trip * q + m;
-                        if (kk >= b && kk < e)
-#pragma unroll
-                            for (int j = 0; j < K; ++j) acc[j].add(rv[q][m][j] * ln[q][m]);
-                    }
+            } else {
+                zerok<T, K>(dv);
             }
-            double tot[K];
-#pragma unroll
-            for (int j = 0; j < K; ++j) tot[j] = acc[j].value();
-            const double sv = reduce_scatter<K>(tot, L);
-            const int pi = (int)(p / U.ny), pj = (int)(p % U.ny);
-            if (gl < K) {
-                const int j = gl;
-                const GL JobParams &P = jpar(a, U.job0 + j);
-                double dv = Ssc * sv;
-                dv += P.alpha * gc_reg_k<T>(U.u, U.v, pi, pj, U.nx, U.ny, P.eps, K, j);
-                const T dt = (T)dv;
-                U.d[p * K + j] = dt;
-                lds[pl * K + j] = (double)dt * (double)dt;
-            }
-        } else if (gl < K) {
-            lds[pl * K + gl] = 0.0;
+            storek<T, K>(U.d + q * K, dv);
         }
     }
-    __syncthreads();
-    if (threadIdx.x < kWave) {
+    block_store<T, K, false>(a, U, SL_DD, lb, m, lds);
+}
+
+// Stencil kernels: one thread per tiled position q (256 per block); the
+// padding positions of the tiles do no work.
+
+// K interleaved values at pixel (i, j), or `pad` outside the image.
+template <typename T, int K>
+__device__ __forceinline__ void ldn(const UG<T> &U, const GL T *arr, int i, int j, T (&o)[K], T pad) {
+    const int64_t q = U.at(i, j);
+    if (q < 0) {
 #pragma unroll
-        for (int j = 0; j < K; ++j) {
-            const double t = group_sum<kWave>(lds[threadIdx.x * K + j]);
-            if (threadIdx.x == j) part_at(a, U, SL_DD, j)[lb] = t;
-        }
+        for (int k = 0; k < K; ++k) o[k] = pad;
+    } else {
+        loadk<T, K>(arr + q * K, o);
     }
 }
 
@@ -383,15 +355,14 @@ __device__ __forceinline__ void nv_unit(const EngArgs<T> &a, const UG<T> &U, int
     double m[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) m[j] = 0.0;
-    if (q < U.npix) {
-        const int i = (int)(q / U.ny), jj = (int)(q % U.ny);
+    int i = 0, jj = 0;
+    if (q < U.npix_t) tile_coords(q, U.tny, i, jj);
+    if (q < U.npix_t && i < U.nx && jj < U.ny) {
         T dq[K], du[K], dl[K], vv[K];
         loadk<T, K>(U.d + q * K, dq);
         loadk<T, K>(U.v + q * K, vv);
-        if (i > 0) loadk<T, K>(U.d + (q - U.ny) * K, du);
-        else zerok<T, K>(du);
-        if (jj > 0) loadk<T, K>(U.d + (q - 1) * K, dl);
-        else zerok<T, K>(dl);
+        ldn<T, K>(U, U.d, i - 1, jj, du, T(0));
+        ldn<T, K>(U, U.d, i, jj - 1, dl, T(0));
 #pragma unroll
         for (int j = 0; j < K; ++j) {
             const double eps = jpar(a, U.job0 + j).eps;
@@ -405,11 +376,18 @@ __device__ __forceinline__ void nv_unit(const EngArgs<T> &a, const UG<T> &U, int
     block_store<T, K, false>(a, U, SL_NV, lb, m, lds);
 }
 
+__device__ __forceinline__ bool real_pixel(int64_t q, int64_t npix_t, int tny, int nx, int ny) {
+    if (q >= npix_t) return false;
+    int i, j;
+    tile_coords(q, tny, i, j);
+    return i < nx && j < ny;
+}
+
 // u = clamp(u + c d, lower, upper) for live jobs.
 template <typename T, int K>
 __device__ __forceinline__ void axpy_unit(const EngArgs<T> &a, const UG<T> &U, int lb) {
     const int64_t q = (int64_t)lb * kPixBlock + threadIdx.x;
-    if (q >= U.npix) return;
+    if (!real_pixel(q, U.npix_t, U.tny, U.nx, U.ny)) return;
     T uu[K], dd[K];
     loadk<T, K>(U.u + q * K, uu);
     loadk<T, K>(U.d + q * K, dd);
@@ -444,18 +422,15 @@ __device__ __forceinline__ bool median_due(const EngArgs<T> &a, int js) {
 template <typename T, int K>
 __device__ __forceinline__ void median_unit(const EngArgs<T> &a, const UG<T> &U, int lb) {
     const int64_t q = (int64_t)lb * kPixBlock + threadIdx.x;
-    if (q >= U.npix) return;
-    const int i = (int)(q / U.ny), jj = (int)(q % U.ny);
+    if (!real_pixel(q, U.npix_t, U.tny, U.nx, U.ny)) return;
+    int i, jj;
+    tile_coords(q, U.tny, i, jj);
     T nb[9][K];
     int n = 0;
 #pragma unroll
     for (int di = -1; di <= 1; ++di)
 #pragma unroll
-        for (int dj = -1; dj <= 1; ++dj, ++n) {
-            const int ii = i + di, jc = jj + dj;
-            if (ii < 0 || jc < 0 || ii >= U.nx || jc >= U.ny) zerok<T, K>(nb[n]);
-            else loadk<T, K>(U.u + ((int64_t)ii * U.ny + jc) * K, nb[n]);
-        }
+        for (int dj = -1; dj <= 1; ++dj, ++n) ldn<T, K>(U, U.u, i + di, jj + dj, nb[n], T(0));
     T out[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) {
@@ -482,7 +457,7 @@ __device__ __forceinline__ void median_unit(const EngArgs<T> &a, const UG<T> &U,
 template <typename T, int K>
 __device__ __forceinline__ void copy_wu_unit(const UG<T> &U, int lb) {
     const int64_t q = (int64_t)lb * kPixBlock + threadIdx.x;
-    if (q >= U.npix) return;
+    if (!real_pixel(q, U.npix_t, U.tny, U.nx, U.ny)) return;
     T t[K];
     loadk<T, K>(U.w + q * K, t);
     storek<T, K>(U.u + q * K, t);
@@ -502,26 +477,19 @@ __device__ __forceinline__ void edge_grad_unit(const EngArgs<T> &a, const UG<T> 
     double m0[K], m1[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) m0[j] = m1[j] = 0.0;
-    if (q < U.npix) {
-        const int i = (int)(q / U.ny), jj = (int)(q % U.ny);
+    if (real_pixel(q, U.npix_t, U.tny, U.nx, U.ny)) {
+        int i, jj;
+        tile_coords(q, U.tny, i, jj);
         T fd[K], fu[K], fr[K], fl[K], vc[K], vd[K], vu[K], vr[K], vl[K];
-        auto ldz = [&](const GL T *arr, int ii, int jc, T (&o)[K], T pad) {
-            if (ii < 0 || jc < 0 || ii >= U.nx || jc >= U.ny) {
-#pragma unroll
-                for (int j = 0; j < K; ++j) o[j] = pad;
-            } else {
-                loadk<T, K>(arr + ((int64_t)ii * U.ny + jc) * K, o);
-            }
-        };
-        ldz(U.u, i + 1, jj, fd, T(0));
-        ldz(U.u, i - 1, jj, fu, T(0));
-        ldz(U.u, i, jj + 1, fr, T(0));
-        ldz(U.u, i, jj - 1, fl, T(0));
+        ldn<T, K>(U, U.u, i + 1, jj, fd, T(0));
+        ldn<T, K>(U, U.u, i - 1, jj, fu, T(0));
+        ldn<T, K>(U, U.u, i, jj + 1, fr, T(0));
+        ldn<T, K>(U, U.u, i, jj - 1, fl, T(0));
         loadk<T, K>(U.v + q * K, vc);
-        ldz(U.v, i + 1, jj, vd, T(1));
-        ldz(U.v, i - 1, jj, vu, T(1));
-        ldz(U.v, i, jj + 1, vr, T(1));
-        ldz(U.v, i, jj - 1, vl, T(1));
+        ldn<T, K>(U, U.v, i + 1, jj, vd, T(1));
+        ldn<T, K>(U, U.v, i - 1, jj, vu, T(1));
+        ldn<T, K>(U, U.v, i, jj + 1, vr, T(1));
+        ldn<T, K>(U, U.v, i, jj - 1, vl, T(1));
         T out[K];
         zerok<T, K>(out);
 #pragma unroll
@@ -556,17 +524,14 @@ __device__ __forceinline__ void edge_nabla_unit(const EngArgs<T> &a, const UG<T>
     double m[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) m[j] = 0.0;
-    if (q < U.npix) {
-        const int i = (int)(q / U.ny), jj = (int)(q % U.ny);
+    if (real_pixel(q, U.npix_t, U.tny, U.nx, U.ny)) {
+        int i, jj;
+        tile_coords(q, U.tny, i, jj);
         T d1[K], d2[K], d3[K], d4[K];
-        auto ldz = [&](int ii, int jc, T (&o)[K]) {
-            if (ii < 0 || jc < 0 || ii >= U.nx || jc >= U.ny) zerok<T, K>(o);
-            else loadk<T, K>(U.w + ((int64_t)ii * U.ny + jc) * K, o);
-        };
-        ldz(i + 1, jj, d1);
-        ldz(i - 1, jj, d2);
-        ldz(i, jj + 1, d3);
-        ldz(i, jj - 1, d4);
+        ldn<T, K>(U, U.w, i + 1, jj, d1, T(0));
+        ldn<T, K>(U, U.w, i - 1, jj, d2, T(0));
+        ldn<T, K>(U, U.w, i, jj + 1, d3, T(0));
+        ldn<T, K>(U, U.w, i, jj - 1, d4, T(0));
 #pragma unroll
         for (int j = 0; j < K; ++j) {
             if (!edge_live(a, U.job0 + j, it)) continue;
@@ -586,7 +551,7 @@ __device__ __forceinline__ void edge_update_unit(const EngArgs<T> &a, const UG<T
     double m[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) m[j] = 0.0;
-    if (q < U.npix) {
+    if (real_pixel(q, U.npix_t, U.tny, U.nx, U.ny)) {
         T vv[K], ww[K];
         loadk<T, K>(U.v + q * K, vv);
         loadk<T, K>(U.w + q * K, ww);
@@ -636,7 +601,7 @@ __global__ __launch_bounds__(kBlock) void k_eng_fp(EngArgs<T> a, BlockMap m, int
 
 template <typename T, int KMAX>
 __global__ __launch_bounds__(kBlock) void k_eng_bp(EngArgs<T> a, BlockMap m, int s) {
-    __shared__ double lds[kBpPix * kMaxK];
+    __shared__ double lds[kBlock / kWave * kMaxK];
     const int b = (int)xcd_block();
     if (b >= m.start[m.nunits]) return;
     const int ui = find_unit(m, b);
@@ -729,7 +694,7 @@ __global__ __launch_bounds__(kBlock) void k_eng_fin(EngArgs<T> a, int op, int s)
     if (!job_live(a, js)) return;
     GL Scalars *sc = &jsc(a, js);
     const GL JobParams &P = jpar(a, js);
-    const int64_t npx = cdiv(U.npix, kPixBlock);
+    const int64_t npx = cdiv(U.npix_t, kPixBlock);
     switch (op) {
         case EF_BEGIN:
             if (threadIdx.x == 0) {
@@ -738,8 +703,9 @@ __global__ __launch_bounds__(kBlock) void k_eng_fin(EngArgs<T> a, int op, int s)
             }
             break;
         case EF_STEP: {
-            const int64_t nr = cdiv(U.sub_row0[s + 1] - U.sub_row0[s], kRaysPerBlock);
-            const double dd = fin_reduce(part_at(a, U, SL_DD, j), cdiv(U.npix, kBpPix), false, lds);
+            const int64_t nr = cdiv(U.fs_slice0[s + 1] - U.fs_slice0[s], kBlock / kWave);
+            const int64_t nb = cdiv(U.bs_slice0[s + 1] - U.bs_slice0[s], kBlock / kWave);
+            const double dd = fin_reduce(part_at(a, U, SL_DD, j), nb, false, lds);
             const double ad2 = fin_reduce(part_at(a, U, SL_AD, j), nr, false, lds);
             const double nv = fin_reduce(part_at(a, U, SL_NV, j), npx, false, lds);
             const double df = fin_reduce(part_at(a, U, SL_DF, j), nr, false, lds);
@@ -797,15 +763,19 @@ __global__ void k_put_rows(T *dst, int K, int slot, int64_t n, const int64_t *pe
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) dst[i * K + slot] = src[perm[i]];
 }
+// Row-major image <-> slot of a tiled interleaved image.
 template <typename T>
-__global__ void k_put(T *dst, int K, int slot, int64_t n, const T *src, T fillv) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i * K + slot] = src ? src[i] : fillv;
+__global__ void k_put_img(T *dst, int K, int slot, int nx, int ny, int tny, const T *src, T fillv) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (int64_t)nx * ny) return;
+    const int64_t q = tile_index((int)(p / ny), (int)(p % ny), tny);
+    dst[q * K + slot] = src ? src[p] : fillv;
 }
 template <typename T>
-__global__ void k_get(T *dst, const T *src, int K, int slot, int64_t n) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = src[i * K + slot];
+__global__ void k_get_img(T *dst, const T *src, int K, int slot, int nx, int ny, int tny) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (int64_t)nx * ny) return;
+    dst[p] = src[tile_index((int)(p / ny), (int)(p % ny), tny) * K + slot];
 }
 template <typename T>
 __global__ void k_move(T *dst, int Kd, int sd, const T *src, int Ks, int ss, int64_t n) {
@@ -897,15 +867,17 @@ void eng_put_rows(hipStream_t st, T *dst, int K, int slot, int64_t n, const int6
     MI_HIP(hipGetLastError());
 }
 template <typename T>
-void eng_put(hipStream_t st, T *dst, int K, int slot, int64_t n, const T *src, T fillv) {
+void eng_put_img(hipStream_t st, T *dst, int K, int slot, int nx, int ny, int tny, const T *src, T fillv) {
+    const int64_t n = (int64_t)nx * ny;
     if (n <= 0) return;
-    k_put<T><<<grid_of(n), kBlock, 0, st>>>(dst, K, slot, n, src, fillv);
+    k_put_img<T><<<grid_of(n), kBlock, 0, st>>>(dst, K, slot, nx, ny, tny, src, fillv);
     MI_HIP(hipGetLastError());
 }
 template <typename T>
-void eng_get(hipStream_t st, T *dst, const T *src, int K, int slot, int64_t n) {
+void eng_get_img(hipStream_t st, T *dst, const T *src, int K, int slot, int nx, int ny, int tny) {
+    const int64_t n = (int64_t)nx * ny;
     if (n <= 0) return;
-    k_get<T><<<grid_of(n), kBlock, 0, st>>>(dst, src, K, slot, n);
+    k_get_img<T><<<grid_of(n), kBlock, 0, st>>>(dst, src, K, slot, nx, ny, tny);
     MI_HIP(hipGetLastError());
 }
 template <typename T>
@@ -933,8 +905,8 @@ void eng_copy(hipStream_t st, void *dst, const void *src, int64_t bytes) {
     template void eng_edge_iter<T>(hipStream_t, const EngArgs<T> &, const BlockMap &, int, int);            \
     template void eng_finalize<T>(hipStream_t, const EngArgs<T> &, int, int);                               \
     template void eng_put_rows<T>(hipStream_t, T *, int, int, int64_t, const int64_t *, const T *);         \
-    template void eng_put<T>(hipStream_t, T *, int, int, int64_t, const T *, T);                            \
-    template void eng_get<T>(hipStream_t, T *, const T *, int, int, int64_t);                               \
+    template void eng_put_img<T>(hipStream_t, T *, int, int, int, int, int, const T *, T);                  \
+    template void eng_get_img<T>(hipStream_t, T *, const T *, int, int, int, int, int);                     \
     template void eng_move<T>(hipStream_t, T *, int, int, const T *, int, int, int64_t);
 
 MI_ENG_INST(float)

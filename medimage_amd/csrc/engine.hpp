@@ -3,27 +3,21 @@
 // and whose state is stored job-interleaved (element e of job j at e*K + j),
 // so one matrix stream and one vector gather serve all K jobs.
 //
-// Every reduction that sets a step size is cut into fixed chunks (16 rays,
-// 64 pixels of the back projector, 256 pixels of the stencils) whose partial
-// sums are combined in index order by a one-block-per-job finalize.  The
-// chunking depends on neither the grid, nor K, nor the other units of the
-// launch, so a job's result is bit-for-bit the same alone and in any batch.
+// Every reduction that sets a step size is cut into fixed chunks (a block:
+// four slices of the sliced forward / back projector, or 256 tiled pixel
+// positions of the stencils) whose partial sums are combined in index order
+// by a one-block-per-job finalize.  The chunking depends only on the
+// geometry -- not on the grid, nor K, nor the other units of the launch -- so
+// a job's result is bit-for-bit the same alone and in any batch.
 #pragma once
 
-#include "common.hpp"
-#include "../../include/medimage_amd.h"
+#include "runtime.hpp"
 
 namespace mi {
 
 constexpr int kMaxK = 8;             // jobs per unit
 constexpr int kMaxUnits = 64;        // units per batch
-#ifndef MI_RAYS_PER_WAVE
-#define MI_RAYS_PER_WAVE 4
-#endif
-constexpr int kRaysPerWave = MI_RAYS_PER_WAVE;   // consecutive rays per forward-projector wave
-constexpr int kRaysPerBlock = kRaysPerWave * (kBlock / kWave);   // 16
-constexpr int kBpPix = 64;           // pixels per back-projector block
-constexpr int kPixBlock = kBlock;    // pixels per stencil block
+constexpr int kPixBlock = kBlock;    // tiled pixel positions per stencil block
 
 // Partial-sum slots of a unit (each K * cap doubles, see UnitDesc).
 enum EngSlot { SL_DF = 0, SL_DD, SL_AD, SL_NV, SL_E0, SL_E1, SL_E2, SL_E3, SL_COUNT };
@@ -34,23 +28,25 @@ struct JobParams {
     int32_t iter_edge, median_every, max_iters, active;
 };
 
-// One unit.  Pointers to the matrix (shared) and to the unit's interleaved
-// state.  Lengths are T (float or double).
+// One unit.  Pointers to the sliced matrix (shared) and to the unit's
+// interleaved state; images are tiled (npix_t positions).  Lengths are T.
 template <typename T>
 struct UnitDesc {
-    const int64_t *rowptr;
-    const uint32_t *idx;
-    const T *len;
-    const int64_t *colptr;      // all subsets: colptr[s * npix + p]
-    const uint32_t *crow;
-    const T *clen;
-    const int64_t *sub_row0;    // S + 1 row offsets
-    T *g, *r;                   // rows * K
-    T *u, *v, *d, *w;           // npix * K
-    int64_t npix, nrows;
-    int64_t part_off, cap;      // partials: part + part_off + (slot * K + j) * cap
-    int nx, ny, K, lanes;       // lanes: back-projector lanes per pixel, 8..64, >= K
-    int job0;                   // job slots job0 .. job0 + K - 1 (stable)
+    const int64_t *fs_slice0, *fs_base;   // forward: S + 1 slice offsets, per-slice entry base
+    const int32_t *fs_T;                   // per-slice entries per lane
+    const uint32_t *fs_idx;                // tiled pixel indices
+    const T *fs_len;
+    const int64_t *bs_slice0, *bs_base;   // back: same, items are tiled pixels
+    const int32_t *bs_T;
+    const uint32_t *bs_idx;                // rows
+    const T *bs_len;
+    const int64_t *sub_row0;               // S + 1 row offsets
+    T *g, *r;                              // rows * K
+    T *u, *v, *d, *w;                      // npix_t * K
+    int64_t npix_t, nrows;
+    int64_t part_off, cap;                 // partials: part + part_off + (slot * K + j) * cap
+    int nx, ny, tny, K, fs_P, bs_P;
+    int job0;                              // job slots job0 .. job0 + K - 1 (stable)
     int pad;
 };
 
@@ -95,9 +91,9 @@ void eng_finalize(hipStream_t st, const EngArgs<T> &a, int op, int s);
 template <typename T>
 void eng_put_rows(hipStream_t st, T *dst, int K, int slot, int64_t n, const int64_t *perm, const T *src);
 template <typename T>
-void eng_put(hipStream_t st, T *dst, int K, int slot, int64_t n, const T *src, T fillv);
+void eng_put_img(hipStream_t st, T *dst, int K, int slot, int nx, int ny, int tny, const T *src, T fillv);
 template <typename T>
-void eng_get(hipStream_t st, T *dst, const T *src, int K, int slot, int64_t n);
+void eng_get_img(hipStream_t st, T *dst, const T *src, int K, int slot, int nx, int ny, int tny);
 // Copies slot `from` of a unit array to slot `to` of another (repacking).
 template <typename T>
 void eng_move(hipStream_t st, T *dst, int Kd, int sd, const T *src, int Ks, int ss, int64_t n);

@@ -57,6 +57,41 @@ struct GeomKey {
 GeomKey make_key(const mi_geometry &g, int n_subsets);
 void validate_geometry(const mi_geometry &g);
 
+// Sliced storage of one projection direction for the AT engine.  Items
+// (rays for the forward projector, pixels for the back projector) are cut
+// into P segments of ceil(len / P) entries; one lane owns one (item,
+// segment) and a wave's 64 lanes are C = 64 / P consecutive items of one
+// subset (a slice).  Entry t of lane l of slice sl is at base[sl] + 64 t + l,
+// so every step of a wave reads one contiguous 256-byte row of each stream;
+// lanes past their segment read padding (the lane's last pixel/row, length 0).
+struct Sell {
+    int P = 1, C = 64;
+    std::vector<int64_t> slice0;     // S + 1: first slice of each subset
+    DevBuf<int64_t> d_slice0;
+    DevBuf<int64_t> base;            // per slice
+    DevBuf<int32_t> T;               // per slice: entries per lane
+    DevBuf<uint32_t> idx;            // forward: tiled pixel; back: row
+    DevBuf<float> l32;
+    DevBuf<double> l64;
+    int64_t entries = 0;             // stored, padding included
+    size_t bytes() const {
+        return d_slice0.bytes() + base.bytes() + T.bytes() + idx.bytes() + l32.bytes() + l64.bytes();
+    }
+};
+
+// Images of the AT engine are stored in 4 x 8 pixel tiles (128 bytes of
+// FP32): tile (ti, tj) at ti * tny + tj, pixel (i & 3, j & 7) inside.
+constexpr int kTileH = 4, kTileW = 8;
+__host__ __device__ __forceinline__ int64_t tile_index(int i, int j, int tny) {
+    return ((int64_t)(i >> 2) * tny + (j >> 3)) * (kTileH * kTileW) + ((i & 3) << 3) + (j & 7);
+}
+__host__ __device__ __forceinline__ void tile_coords(int64_t q, int tny, int &i, int &j) {
+    const int64_t t = q >> 5;
+    const int w = (int)(q & 31);
+    i = (int)(t / tny) * kTileH + (w >> 3);
+    j = (int)(t % tny) * kTileW + (w & 7);
+}
+
 // System matrix of one geometry, rows (rays) grouped by view subset.
 //   Row order: subsets in order; inside a subset its views ascending; inside
 //   a view the bins ascending.  Row r is ray (vlist[r / nbins], r % nbins).
@@ -87,9 +122,17 @@ struct Matrix {
     // Row permutation between the caller's [view][bin] sinogram and the
     // subset-ordered rows: perm[r] = view * nbins + bin of row r.
     DevBuf<int64_t> perm;
+    // AT engine: sliced forward (rays, tiled pixel indices) and back
+    // (pixels in tile order, per subset) projectors; tiled image size.
+    Sell fs, bs;
+    int tny = 0;
+    int64_t npix_t = 0;
+    bool has_sell = false;
 
     size_t bytes() const;
 };
+// Builds m.fs / m.bs from the CSR / CSC (sell.hip).
+void build_sell(Matrix &m, hipStream_t st);
 
 // Builds the matrix for (g, S) on the stream; want64 also keeps double
 // lengths.  Host-side GoldenC walk for MI_BEAM_GOLDENC, device tracers

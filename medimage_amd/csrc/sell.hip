@@ -1,0 +1,190 @@
+// Sliced (lane-per-segment) copies of a system matrix for the AT engine,
+// built on the device from the ray-major CSR and the per-subset CSC.
+//
+// Forward direction: items are the rows of a subset in row order, so a
+// slice holds C consecutive rays (neighbouring bins of one view); pixel
+// indices are rewritten into the engine's tiled image layout.  Back
+// direction: items are the pixels of the tiled layout (tile order), per
+// subset; the entries keep their row order (the CSC order).  The entry
+// sequence of every item is unchanged, only cut into P segments.
+#include "runtime.hpp"
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+
+namespace mi {
+
+namespace {
+
+inline unsigned grid_of(int64_t n, int b = kBlock) { return (unsigned)((n + b - 1) / b); }
+
+__device__ __forceinline__ int subset_of(const int64_t *slice0, int S, int64_t sl) {
+    int lo = 0, hi = S;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (slice0[mid] <= sl) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Entry range [b, e) of forward item `lane` of slice sl (b == e: none).
+struct FItems {
+    const int64_t *rowptr, *sub_row0, *slice0;
+    int S, P, C;
+    __device__ __forceinline__ void range(int64_t sl, int item, int64_t &b, int64_t &e) const {
+        const int s = subset_of(slice0, S, sl);
+        const int64_t r = sub_row0[s] + (sl - slice0[s]) * C + item;
+        if (r >= sub_row0[s + 1]) { b = e = 0; return; }
+        b = rowptr[r];
+        e = rowptr[r + 1];
+    }
+};
+
+// Back item: tiled pixel q of subset s -> its CSC column.
+struct BItems {
+    const int64_t *colptr, *slice0;
+    int S, P, C, nx, ny, tny;
+    int64_t npix, npix_t;
+    __device__ __forceinline__ void range(int64_t sl, int item, int64_t &b, int64_t &e) const {
+        const int s = subset_of(slice0, S, sl);
+        const int64_t q = (sl - slice0[s]) * C + item;
+        int i, j;
+        tile_coords(q, tny, i, j);
+        if (q >= npix_t || i >= nx || j >= ny) { b = e = 0; return; }
+        const int64_t c = (int64_t)s * npix + (int64_t)i * ny + j;
+        b = colptr[c];
+        e = colptr[c + 1];
+    }
+};
+
+template <typename It>
+__global__ void k_sell_T(It it, int64_t nslices, int32_t *T) {
+    const int64_t sl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (sl >= nslices) return;
+    int64_t t = 0;
+    for (int item = 0; item < it.C; ++item) {
+        int64_t b, e;
+        it.range(sl, item, b, e);
+        t = max(t, (e - b + it.P - 1) / it.P);
+    }
+    T[sl] = (int32_t)t;
+}
+
+__global__ void k_times64(const int32_t *T, int64_t n, int64_t *out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (int64_t)T[i] * kWave;
+}
+
+// One thread per (slice, lane).  FWD: rewrite pixel indices into tiles.
+template <typename It, bool FWD>
+__global__ void k_sell_fill(It it, int64_t nslices, const int64_t *base, const int32_t *T, const uint32_t *src_idx,
+                            const float *src32, const double *src64, int ny, int tny, uint32_t *idx, float *l32,
+                            double *l64) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t sl = g / kWave;
+    const int lane = (int)(g % kWave);
+    if (sl >= nslices) return;
+    int64_t b, e;
+    it.range(sl, lane / it.P, b, e);
+    const int p = lane % it.P;
+    const int64_t Ls = (e - b + it.P - 1) / it.P;
+    const int64_t k0 = b + p * Ls;
+    const int64_t cnt = max((int64_t)0, min(e, k0 + Ls) - k0);
+    const int n = T[sl];
+    int64_t pos = base[sl] + lane;
+    uint32_t last = 0;
+    for (int t = 0; t < n; ++t, pos += kWave) {
+        if (t < cnt) {
+            uint32_t v = src_idx[k0 + t];
+            if (FWD) v = (uint32_t)tile_index((int)(v / (uint32_t)ny), (int)(v % (uint32_t)ny), tny);
+            idx[pos] = v;
+            l32[pos] = src32[k0 + t];
+            if (l64) l64[pos] = src64[k0 + t];
+            last = v;
+        } else {
+            idx[pos] = last;
+            l32[pos] = 0.f;
+            if (l64) l64[pos] = 0.0;
+        }
+    }
+}
+
+// Segments per item: enough (item, segment) lanes for about `waves`
+// wavefronts, at least 8 entries per lane on average, P in {1, ..., 16}.
+int choose_p(int64_t items, double mean_len, int64_t waves) {
+    int P = 1;
+    while (P < 16 && items * P < waves * kWave && mean_len / (2 * P) >= 8.0) P *= 2;
+    return P;
+}
+
+template <typename It>
+void fill_sell(Sell &sl, It it, const uint32_t *src_idx, const float *s32, const double *s64, bool fwd, int ny,
+               int tny, hipStream_t st) {
+    const int64_t ns = sl.slice0.back();
+    sl.d_slice0.alloc(sl.slice0.size());
+    MI_HIP(hipMemcpyAsync(sl.d_slice0.p, sl.slice0.data(), sl.slice0.size() * sizeof(int64_t), hipMemcpyHostToDevice,
+                          st));
+    it.slice0 = sl.d_slice0.p;
+    sl.T.alloc(ns);
+    sl.base.alloc(ns + 1);
+    k_sell_T<<<grid_of(ns), kBlock, 0, st>>>(it, ns, sl.T.p);
+    MI_HIP(hipGetLastError());
+    DevBuf<int64_t> n64(ns + 1);
+    MI_HIP(hipMemsetAsync(n64.p + ns, 0, sizeof(int64_t), st));
+    k_times64<<<grid_of(ns), kBlock, 0, st>>>(sl.T.p, ns, n64.p);
+    MI_HIP(hipGetLastError());
+    size_t tmp = 0;
+    MI_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, n64.p, sl.base.p, (int)(ns + 1), st));
+    {
+        DevBuf<unsigned char> t(tmp ? tmp : 1);
+        MI_HIP(hipcub::DeviceScan::ExclusiveSum(t.p, tmp, n64.p, sl.base.p, (int)(ns + 1), st));
+        MI_HIP(hipStreamSynchronize(st));
+    }
+    MI_HIP(hipMemcpy(&sl.entries, sl.base.p + ns, sizeof(int64_t), hipMemcpyDeviceToHost));
+    sl.idx.alloc((size_t)std::max<int64_t>(sl.entries, 1));
+    sl.l32.alloc((size_t)std::max<int64_t>(sl.entries, 1));
+    if (s64) sl.l64.alloc((size_t)std::max<int64_t>(sl.entries, 1));
+    if (fwd)
+        k_sell_fill<It, true><<<grid_of(ns * kWave), kBlock, 0, st>>>(it, ns, sl.base.p, sl.T.p, src_idx, s32, s64, ny,
+                                                                      tny, sl.idx.p, sl.l32.p,
+                                                                      s64 ? sl.l64.p : nullptr);
+    else
+        k_sell_fill<It, false><<<grid_of(ns * kWave), kBlock, 0, st>>>(it, ns, sl.base.p, sl.T.p, src_idx, s32, s64,
+                                                                       ny, tny, sl.idx.p, sl.l32.p,
+                                                                       s64 ? sl.l64.p : nullptr);
+    MI_HIP(hipGetLastError());
+    MI_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace
+
+void build_sell(Matrix &m, hipStream_t st) {
+    if (m.has_sell) return;
+    const mi_geometry &g = m.g;
+    m.tny = (g.ny + kTileW - 1) / kTileW;
+    m.npix_t = (int64_t)((g.nx + kTileH - 1) / kTileH) * kTileH * m.tny * kTileW;
+    const bool f64 = m.has64;
+    // forward
+    int64_t rmax = 0;
+    for (int s = 0; s < m.S; ++s) rmax = std::max(rmax, m.sub_row0[s + 1] - m.sub_row0[s]);
+    m.fs.P = choose_p(rmax, (double)m.nnz / std::max<int64_t>(m.nrows, 1), 4096);
+    m.fs.C = kWave / m.fs.P;
+    m.fs.slice0.assign(m.S + 1, 0);
+    for (int s = 0; s < m.S; ++s)
+        m.fs.slice0[s + 1] = m.fs.slice0[s] + (m.sub_row0[s + 1] - m.sub_row0[s] + m.fs.C - 1) / m.fs.C;
+    FItems fi{m.rowptr.p, m.d_sub_row0.p, nullptr, m.S, m.fs.P, m.fs.C};
+    fill_sell(m.fs, fi, m.idx.p, m.len32.p, f64 ? m.len64.p : nullptr, true, g.ny, m.tny, st);
+    // back
+    m.bs.P = choose_p(m.npix_t, (double)m.nnz / ((double)m.S * std::max<int64_t>(m.npix, 1)), 4096);
+    m.bs.C = kWave / m.bs.P;
+    const int64_t per = (m.npix_t + m.bs.C - 1) / m.bs.C;
+    m.bs.slice0.assign(m.S + 1, 0);
+    for (int s = 0; s < m.S; ++s) m.bs.slice0[s + 1] = m.bs.slice0[s] + per;
+    BItems bi{m.colptr.p, nullptr, m.S, m.bs.P, m.bs.C, g.nx, g.ny, m.tny, m.npix, m.npix_t};
+    fill_sell(m.bs, bi, m.crow.p, m.clen32.p, f64 ? m.clen64.p : nullptr, false, g.ny, m.tny, st);
+    m.has_sell = true;
+}
+
+}  // namespace mi

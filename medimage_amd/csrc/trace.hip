@@ -330,7 +330,7 @@ std::vector<int> subset_order(int S) {
 
 size_t Matrix::bytes() const {
     return d_vlist.bytes() + d_sub_row0.bytes() + rowptr.bytes() + idx.bytes() + len32.bytes() + len64.bytes() + colptr.bytes() +
-           crow.bytes() + clen32.bytes() + clen64.bytes() + perm.bytes();
+           crow.bytes() + clen32.bytes() + clen64.bytes() + perm.bytes() + fs.bytes() + bs.bytes();
 }
 
 std::unique_ptr<Matrix> build_matrix(const mi_geometry &g, int S, bool want64, hipStream_t st) {
