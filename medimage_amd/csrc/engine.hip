@@ -71,7 +71,7 @@ __device__ __forceinline__ GL Scalars &jsc(const EngArgs<T> &a, int js) {
 #define MI_NT_FP 1      // non-temporal matrix streams in the forward projector
 #endif
 #ifndef MI_NT_BP
-#define MI_NT_BP 0      // ... and in the back projector (measured slower with it)
+#define MI_NT_BP 1      // ... and in the back projector
 #endif
 #ifndef MI_STEPS
 #define MI_STEPS 8      // steps of a projector lane in flight (K <= 2; 4 above)

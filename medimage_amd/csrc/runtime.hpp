@@ -58,12 +58,12 @@ GeomKey make_key(const mi_geometry &g, int n_subsets);
 void validate_geometry(const mi_geometry &g);
 
 // Sliced storage of one projection direction for the AT engine.  Items
-// (rays for the forward projector, pixels for the back projector) are cut
-// into P segments of ceil(len / P) entries; one lane owns one (item,
-// segment) and a wave's 64 lanes are C = 64 / P consecutive items of one
-// subset (a slice).  Entry t of lane l of slice sl is at base[sl] + 64 t + l,
-// so every step of a wave reads one contiguous 256-byte row of each stream;
-// lanes past their segment read padding (the lane's last pixel/row, length 0).
+// (rays for the forward projector, pixels for the back projector) get P
+// lanes each, lane p taking entries p, p + P, ... of its item; a wave's 64
+// lanes are C = 64 / P consecutive items of one subset (a slice).  Entry t of
+// lane l of slice sl is at base[sl] + 64 t + l, so every step of a wave reads
+// one contiguous 256-byte row of each stream; lanes past their last entry
+// read padding (the lane's last pixel/row, length 0).
 struct Sell {
     int P = 1, C = 64;
     std::vector<int64_t> slice0;     // S + 1: first slice of each subset

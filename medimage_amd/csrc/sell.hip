@@ -5,8 +5,8 @@
 // slice holds C consecutive rays (neighbouring bins of one view); pixel
 // indices are rewritten into the engine's tiled image layout.  Back
 // direction: items are the pixels of the tiled layout (tile order), per
-// subset; the entries keep their row order (the CSC order).  The entry
-// sequence of every item is unchanged, only cut into P segments.
+// subset; the entries keep their row order (the CSC order).  The entries of
+// an item are dealt round-robin to its P lanes.
 #include "runtime.hpp"
 
 #include <hipcub/hipcub.hpp>
@@ -89,19 +89,20 @@ __global__ void k_sell_fill(It it, int64_t nslices, const int64_t *base, const i
     int64_t b, e;
     it.range(sl, lane / it.P, b, e);
     const int p = lane % it.P;
-    const int64_t Ls = (e - b + it.P - 1) / it.P;
-    const int64_t k0 = b + p * Ls;
-    const int64_t cnt = max((int64_t)0, min(e, k0 + Ls) - k0);
+    // Lane p of an item takes its entries p, p + P, p + 2P, ... (step t holds
+    // entries tP .. tP + P - 1 of the item across its P lanes).
+    const int64_t cnt = (e - b > p) ? (e - b - p + it.P - 1) / it.P : 0;
     const int n = T[sl];
     int64_t pos = base[sl] + lane;
     uint32_t last = 0;
     for (int t = 0; t < n; ++t, pos += kWave) {
         if (t < cnt) {
-            uint32_t v = src_idx[k0 + t];
+            const int64_t k = b + p + (int64_t)t * it.P;
+            uint32_t v = src_idx[k];
             if (FWD) v = (uint32_t)tile_index((int)(v / (uint32_t)ny), (int)(v % (uint32_t)ny), tny);
             idx[pos] = v;
-            l32[pos] = src32[k0 + t];
-            if (l64) l64[pos] = src64[k0 + t];
+            l32[pos] = src32[k];
+            if (l64) l64[pos] = src64[k];
             last = v;
         } else {
             idx[pos] = last;
@@ -111,11 +112,20 @@ __global__ void k_sell_fill(It it, int64_t nslices, const int64_t *base, const i
     }
 }
 
-// Segments per item: enough (item, segment) lanes for about `waves`
-// wavefronts, at least 8 entries per lane on average, P in {1, ..., 16}.
-int choose_p(int64_t items, double mean_len, int64_t waves) {
-    int P = 1;
-    while (P < 16 && items * P < waves * kWave && mean_len / (2 * P) >= 8.0) P *= 2;
+// Lanes per item (measured on the 512^2 / 720-view workload).  Forward: 8,
+// so a wave step covers 8 neighbouring rays x 8 consecutive entries, an
+// 8 x 8 patch of pixels (60 us per subset against 76 us at 4 and 73 us for
+// contiguous segments).  Back: 1, a lane walks a whole pixel column while the
+// wave covers 64 neighbouring pixels (60 us; 82 us at 8).  Fewer when items
+// are short.
+#ifndef MI_SELL_PF
+#define MI_SELL_PF 8
+#endif
+#ifndef MI_SELL_PB
+#define MI_SELL_PB 1
+#endif
+int choose_p(int P, double mean_len) {
+    while (P > 1 && mean_len / P < 4.0) P /= 2;
     return P;
 }
 
@@ -167,9 +177,7 @@ void build_sell(Matrix &m, hipStream_t st) {
     m.npix_t = (int64_t)((g.nx + kTileH - 1) / kTileH) * kTileH * m.tny * kTileW;
     const bool f64 = m.has64;
     // forward
-    int64_t rmax = 0;
-    for (int s = 0; s < m.S; ++s) rmax = std::max(rmax, m.sub_row0[s + 1] - m.sub_row0[s]);
-    m.fs.P = choose_p(rmax, (double)m.nnz / std::max<int64_t>(m.nrows, 1), 4096);
+    m.fs.P = choose_p(MI_SELL_PF, (double)m.nnz / std::max<int64_t>(m.nrows, 1));
     m.fs.C = kWave / m.fs.P;
     m.fs.slice0.assign(m.S + 1, 0);
     for (int s = 0; s < m.S; ++s)
@@ -177,7 +185,7 @@ void build_sell(Matrix &m, hipStream_t st) {
     FItems fi{m.rowptr.p, m.d_sub_row0.p, nullptr, m.S, m.fs.P, m.fs.C};
     fill_sell(m.fs, fi, m.idx.p, m.len32.p, f64 ? m.len64.p : nullptr, true, g.ny, m.tny, st);
     // back
-    m.bs.P = choose_p(m.npix_t, (double)m.nnz / ((double)m.S * std::max<int64_t>(m.npix, 1)), 4096);
+    m.bs.P = choose_p(MI_SELL_PB, (double)m.nnz / ((double)m.S * std::max<int64_t>(m.npix, 1)));
     m.bs.C = kWave / m.bs.P;
     const int64_t per = (m.npix_t + m.bs.C - 1) / m.bs.C;
     m.bs.slice0.assign(m.S + 1, 0);
