@@ -67,7 +67,6 @@ def bench_c3(ctx, n_jobs, iters, log):
     nnz = {}
     for k, g in geoms.items():
         nnz[k] = ctx.prepare(g, 8)
-        ctx.autotune(g, 8)
     log(f"[c3] {n_jobs} jobs over {len(geoms)} geometries prepared in {time.perf_counter() - t:.1f} s")
     p = configs.at_params(configs.ALPHA, configs.BETA, configs.EPS, n_subsets=8, max_iters=iters)
     rays = sum(g.n_views * g.n_bins for g, _, _ in jobs)
@@ -112,6 +111,47 @@ def bench_c3(ctx, n_jobs, iters, log):
     }
 
 
+def bench_serving(ctx, n_jobs, rate, log):
+    """Serving form: a Poisson stream (rate jobs/s) of fan-beam jobs with N in
+    {256..1024}, views in {360..2304}, bins 1.5 N, I0 log-uniform, each run
+    until its relative data-fidelity change is below 1e-4 or 200 AT
+    iterations (8 subsets), served (a) one job at a time and (b) by the
+    continuous-batching request loop.  Sinograms: exact ellipse line
+    integrals with Poisson noise.  The system matrices of the stream's
+    geometries are built before the clock starts (a warm matrix cache)."""
+    from medimage_amd import configs, serve
+    spec = configs.c5_requests(n_jobs, seed=5, rate=rate)
+    t = time.perf_counter()
+    reqs = []
+    geoms = {}
+    for r in spec:
+        g = configs.mixed_geometry(r["n"], r["views"], r["bins"])
+        geoms.setdefault(g.key(), g)
+        sino = configs.analytic_case(g, r["i0"], r["seed"]).astype(np.float32)
+        p = configs.at_params(configs.ALPHA, configs.BETA, configs.EPS, n_subsets=8, max_iters=200, rel_tol=1e-4)
+        reqs.append(dict(t=r["t"], geom=g, sino=sino, params=p))
+    nnz = sum(ctx.prepare(g, 8) for g in geoms.values())
+    log(f"[serve] {n_jobs} requests over {len(geoms)} geometries ({nnz / 1e9:.2f}e9 matrix entries) "
+        f"prepared in {time.perf_counter() - t:.1f} s")
+    out = {"workload": f"{n_jobs} Poisson arrivals at {rate:g} jobs/s, N in {{256,384,512,768,1024}}, views in "
+                       f"{{360,720,1152,2304}}, bins 1.5N, I0 log-uniform [1e4,1e5], 8 subsets, "
+                       f"rel. data-fidelity change < 1e-4 or 200 AT iterations",
+           "geometries": len(geoms)}
+    res = {}
+    for mode in ("plain", "batched"):
+        r = serve.run_stream(ctx, reqs, n_subsets=8, mode=mode, log=log)
+        res[mode] = r
+        log(f"[serve] {mode}: {r['makespan_s']:.1f} s, {r['iters_per_s']:.0f} AT iter/s, "
+            f"p50 {r['latency_p50_s']:.2f} s, p99 {r['latency_p99_s']:.2f} s")
+        out[mode] = {k: r[k] for k in ("makespan_s", "jobs_per_s", "iters_per_s", "latency_p50_s",
+                                       "latency_p99_s", "latency_mean_s", "mean_iterations")}
+    same = sum(np.array_equal(res["plain"]["results"][i][0].view(np.uint32),
+                              res["batched"]["results"][i][0].view(np.uint32)) for i in range(n_jobs))
+    out["jobs_bitwise_identical"] = int(same)
+    out["throughput_ratio"] = res["batched"]["iters_per_s"] / res["plain"]["iters_per_s"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +162,9 @@ def main():
     ap.add_argument("--c3-jobs", type=int, default=64)
     ap.add_argument("--c3-iters", type=int, default=50)
     ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=48)
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="also run a plain device copy of this many GiB (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -137,9 +180,9 @@ def main():
     g = configs.c2_geometry()
     print(f"[bench] tracing C2 geometry and generating the phantom", file=sys.stderr, flush=True)
     t = time.perf_counter()
-    nnz = ctx.prepare(g, 12)
-    tune = ctx.autotune(g, 12)
     _, sino = configs.make_case(ctx, g, 2e4, seed=2)
+    ctx.trim()
+    nnz = ctx.prepare(g, 12)
     print(f"[bench] nnz {nnz}, setup {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
     p = configs.c2_params()
     job = ctx.job(g, sino.astype(np.float32), p)
@@ -187,9 +230,7 @@ def main():
         "data": "synthetic: seeded 10-ellipse phantom, Poisson noise I0=2e4, log transform",
         "config": {"workload": "C2 plain per-job form: 512x512 fan beam (0.7 mm), 720 views, 736 bins (1.0 mm), "
                                "SOD 541 mm, SDD 949 mm, 12 ordered subsets, alpha 1e-3, beta 1e-4, eps 1e-2",
-                   "rays": rays, "nnz": nnz, "subsets": 12,
-                   "tuning": {"fp_vec": tune.fp_vec, "bp_lanes": tune.bp_lanes, "fp_grid": tune.fp_grid,
-                              "bp_grid": tune.bp_grid, "fp_us": tune.fp_us, "bp_us": tune.bp_us}},
+                   "rays": rays, "nnz": nnz, "subsets": 12},
         "rays_per_s": rays * args.steps / elapsed,
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
@@ -205,6 +246,10 @@ def main():
         job.close()
         out["batched"] = bench_c3(ctx, args.c3_jobs, args.c3_iters,
                                   lambda m: print(m, file=sys.stderr, flush=True))
+    if not args.no_serve:
+        job.close()
+        out["serving"] = bench_serving(ctx, args.serve_jobs, args.serve_rate,
+                                       lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_cpu and args.cpu_passes > 0:
         print(f"[bench] CPU baseline: {args.cpu_passes} AT iterations of the oracle", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(sino.astype(np.float32), args.cpu_passes)

@@ -114,8 +114,10 @@ int mi_context_mem(mi_context *ctx, int64_t *bytes);
 int mi_context_trim(mi_context *ctx);
 
 /* ---- geometry / system matrix ------------------------------------------ */
-/* Traces the geometry (if not cached) with n_subsets view subsets and
- * returns its entry count.  Later calls with the same geometry reuse it. */
+/* Traces the geometry (if not cached) for the AT engine with n_subsets view
+ * subsets and returns its entry count.  Jobs and batches of the same
+ * geometry and subset count reuse it (it is kept in the engine's sliced
+ * layout only; the projector calls below build their own copy on first use). */
 int mi_geometry_prepare(mi_context *ctx, const mi_geometry *g, int n_subsets, int64_t *nnz);
 /* Copies the (index, length) sequence of ray (view, bin) as traced on the
  * device: up to cap entries; *numb receives the full count.  Lengths are the
@@ -251,6 +253,8 @@ int mi_batch_status(mi_batch *b, int64_t job, mi_job_status *out);
 int mi_batch_read(mi_batch *b, int64_t job, void *u, void *v);
 /* Forgets a job: cancels it if pending or active, frees its results if done. */
 int mi_batch_release(mi_batch *b, int64_t job);
+/* Ids of jobs retired since the last call (up to cap; *n = how many). */
+int mi_batch_retired(mi_batch *b, int64_t *ids, int cap, int *n);
 /* K of each unit currently on the device (up to cap entries); *n = count. */
 int mi_batch_units(mi_batch *b, int32_t *k, int cap, int *n);
 

@@ -157,6 +157,7 @@ def _load():
         "mi_batch_read": [P, C.c_int64, P, P],
         "mi_batch_release": [P, C.c_int64],
         "mi_batch_units": [P, P, C.c_int, C.POINTER(C.c_int)],
+        "mi_batch_retired": [P, P, C.c_int, C.POINTER(C.c_int)],
         "mi_stats_enable": [P, C.c_int],
         "mi_stats_get": [P, C.c_int, C.POINTER(KernelStats)],
         "mi_stats_reset": [P],
@@ -458,6 +459,17 @@ class Batch:
     def release(self, jid: int):
         _check(self._lib.mi_batch_release(self._h, jid))
         self._geom.pop(jid, None)
+
+    def retired(self):
+        """Ids of the jobs retired since the last call."""
+        out = []
+        buf = np.zeros(256, np.int64)
+        n = C.c_int()
+        while True:
+            _check(self._lib.mi_batch_retired(self._h, _ptr(buf), len(buf), C.byref(n)))
+            out += [int(x) for x in buf[:n.value]]
+            if n.value < len(buf):
+                return out
 
     def units(self):
         k = np.zeros(64, np.int32)

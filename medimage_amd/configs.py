@@ -65,6 +65,32 @@ def c3_jobs(n_jobs=64, seed=3):
     return out
 
 
+def c5_requests(n_jobs=2000, seed=5, rate=20.0):
+    """Serving stream: Poisson arrivals at `rate` jobs/s; N from {256, 384,
+    512, 768, 1024}, views from {360, 720, 1152, 2304}, bins = 1.5 N, I0
+    log-uniform on [1e4, 1e5]."""
+    g = phantom.rng(seed, 55)
+    t = 0.0
+    out = []
+    for j in range(n_jobs):
+        t += float(g.exponential(1.0 / rate))
+        n = int(g.choice([256, 384, 512, 768, 1024]))
+        v = int(g.choice([360, 720, 1152, 2304]))
+        i0 = float(math.exp(g.uniform(math.log(1e4), math.log(1e5))))
+        out.append(dict(job=j, t=t, n=n, views=v, bins=3 * n // 2, i0=i0, seed=seed * 100000 + j))
+    return out
+
+
+def analytic_case(geom, i0, seed, n_ellipses=10, mu_max=0.02):
+    """Noisy sinogram (float64) from the exact line integrals of the seeded
+    ellipse phantom (no device projection; used where an FP64 system matrix
+    of the geometry would not be worth building)."""
+    fov = geom.nx * geom.pixel_mm
+    ell = phantom.random_ellipses(n_ellipses, fov, seed, mu_max=mu_max)
+    p = phantom.analytic_fan_sinogram(ell, geom)
+    return phantom.noisy_sinogram(p, i0, seed)
+
+
 def make_case(ctx, geom, i0, seed, n_ellipses=10, mu_max=0.02):
     """Phantom and noisy sinogram (float64) for a geometry."""
     fov = geom.nx * geom.pixel_mm
@@ -84,5 +110,5 @@ def subsets_for(views, want):
     return max(1, s)
 
 
-__all__ = ["c1_geometry", "c2_geometry", "c1_params", "c2_params", "mixed_geometry", "c3_jobs", "make_case",
-           "rays", "math"]
+__all__ = ["c1_geometry", "c2_geometry", "c1_params", "c2_params", "mixed_geometry", "c3_jobs", "c5_requests",
+           "make_case", "analytic_case", "rays", "math"]

@@ -137,7 +137,6 @@ Unit<T> *Batch<T>::place(BatchJob<T> &j, int group) {
         if (!units[i]) { uid = i; break; }
     MI_REQUIRE(uid >= 0, MI_E_LIMIT, "batch holds the maximum number of units");
     const int K = pow2ceil(std::min(kMaxK, std::max(std::max(group, min_k), 1)));
-    if (!j.m->has_sell) build_sell(*j.m, ctx->stream);
     const Matrix &m = *j.m;
     const int64_t need = unit_state_bytes<T>(m, K);
     if (free_device_bytes() - need < reserve_bytes) evict_idle(ctx, need + reserve_bytes - free_device_bytes());
@@ -219,12 +218,12 @@ int Batch<T>::admit(bool require) {
         try {
             if (!j.m) {
                 try {
-                    j.m = get_matrix(ctx, j.geom, S, sizeof(T) == 8);
+                    j.m = get_engine_matrix(ctx, j.geom, S, sizeof(T) == 8);
                 } catch (const Error &e) {
                     if (e.code != MI_E_NOMEM) throw;
                     (void)hipGetLastError();
                     if (evict_idle(ctx, -1) <= 0) throw;
-                    j.m = get_matrix(ctx, j.geom, S, sizeof(T) == 8);
+                    j.m = get_engine_matrix(ctx, j.geom, S, sizeof(T) == 8);
                 }
             }
             j.m->last_use = ++ctx->tick;
@@ -518,6 +517,7 @@ void Batch<T>::retire(BatchJob<T> &j) {
     MI_HIP(hipStreamSynchronize(st));
     j.state = BJ_DONE;
     j.t_done = now();
+    retired_log.push_back(j.id);
     free_slot(j);
 }
 

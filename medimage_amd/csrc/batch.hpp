@@ -85,6 +85,7 @@ struct Batch {
     hipGraphExec_t gexec = nullptr;
     std::chrono::steady_clock::time_point t0;
     int64_t passes_run = 0;
+    std::vector<int64_t> retired_log;   // ids retired since the caller last drained it
 
     Batch(mi_context *c, int n_subsets);
     ~Batch();

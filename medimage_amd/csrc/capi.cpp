@@ -61,9 +61,9 @@ void validate_geometry(const mi_geometry &g) {
     }
 }
 
-GeomKey make_key(const mi_geometry &g, int S) {
+GeomKey make_key(const mi_geometry &g, int S, int layout) {
     GeomKey k;
-    k.i = {g.beam, g.nx, g.ny, g.n_views, g.n_bins, g.beam == MI_BEAM_PARALLEL ? g.flavour : 0, S};
+    k.i = {g.beam, g.nx, g.ny, g.n_views, g.n_bins, g.beam == MI_BEAM_PARALLEL ? g.flavour : 0, S, layout};
     if (g.beam == MI_BEAM_FAN_FLAT) k.d = {g.pixel_mm, g.span_deg, g.bin_mm, g.sod_mm, g.sdd_mm, g.det_offset};
     else k.d = {0, 0, 0, 0, 0, 0};
     return k;
@@ -78,6 +78,33 @@ std::shared_ptr<Matrix> get_matrix(mi_context *ctx, const mi_geometry &g, int S,
     }
     if (it != ctx->matrices.end()) ctx->matrices.erase(it);
     std::shared_ptr<Matrix> m(build_matrix(g, S, want64, ctx->stream).release());
+    m->last_use = ++ctx->tick;
+    ctx->matrices[key] = m;
+    return m;
+}
+
+std::shared_ptr<Matrix> get_engine_matrix(mi_context *ctx, const mi_geometry &g, int S, bool want64) {
+    const GeomKey key = make_key(g, S, 1);
+    auto it = ctx->matrices.find(key);
+    if (it != ctx->matrices.end() && (!want64 || it->second->has64)) {
+        it->second->last_use = ++ctx->tick;
+        return it->second;
+    }
+    if (it != ctx->matrices.end()) ctx->matrices.erase(it);
+    std::shared_ptr<Matrix> m;
+    for (int attempt = 0;; ++attempt) {
+        try {
+            m.reset(build_matrix(g, S, want64, ctx->stream).release());
+            build_sell(*m, ctx->stream, true);
+            break;
+        } catch (const Error &e) {
+            // Out of device memory: drop idle cached matrices (least recently
+            // used first) and build once more.
+            (void)hipGetLastError();
+            m.reset();
+            if (e.code != MI_E_NOMEM || attempt > 0 || evict_idle(ctx, -1) <= 0) throw;
+        }
+    }
     m->last_use = ++ctx->tick;
     ctx->matrices[key] = m;
     return m;
@@ -333,8 +360,8 @@ int mi_context_trim(mi_context *ctx) {
 int mi_geometry_prepare(mi_context *ctx, const mi_geometry *g, int n_subsets, int64_t *nnz) {
     return guarded([&] {
         check_ctx(ctx);
-        MI_REQUIRE(g, MI_E_ARG, "null geometry");
-        Matrix &m = *get_matrix(ctx, *g, n_subsets, false);
+        MI_REQUIRE(g && n_subsets >= 1, MI_E_ARG, "bad arguments");
+        Matrix &m = *get_engine_matrix(ctx, *g, n_subsets, false);
         if (nnz) *nnz = m.nnz;
     });
 }
@@ -741,6 +768,21 @@ int mi_batch_release(mi_batch *b, int64_t job) {
         check_ctx(b->ctx);
         if (b->b32) b->b32->release(job);
         else b->b64->release(job);
+    });
+}
+
+int mi_batch_retired(mi_batch *b, int64_t *ids, int cap, int *n) {
+    return guarded([&] {
+        MI_REQUIRE(b && n && cap >= 0 && (cap == 0 || ids), MI_E_ARG, "bad arguments");
+        auto drain = [&](auto &B) {
+            auto &log = B.retired_log;
+            const int take = std::min<int>(cap, (int)log.size());
+            for (int i = 0; i < take; ++i) ids[i] = log[i];
+            log.erase(log.begin(), log.begin() + take);
+            *n = take;
+        };
+        if (b->b32) drain(*b->b32);
+        else drain(*b->b64);
     });
 }
 

@@ -50,11 +50,12 @@ struct DevBuf {
 
 // Geometry + subset count identify one system matrix.
 struct GeomKey {
-    std::array<int32_t, 7> i;
+    std::array<int32_t, 8> i;
     std::array<double, 6> d;
     bool operator<(const GeomKey &o) const { return i != o.i ? i < o.i : d < o.d; }
 };
-GeomKey make_key(const mi_geometry &g, int n_subsets);
+// layout: 0 = CSR / CSC (projector calls), 1 = sliced (AT engine).
+GeomKey make_key(const mi_geometry &g, int n_subsets, int layout = 0);
 void validate_geometry(const mi_geometry &g);
 
 // Sliced storage of one projection direction for the AT engine.  Items
@@ -128,11 +129,13 @@ struct Matrix {
     int tny = 0;
     int64_t npix_t = 0;
     bool has_sell = false;
+    bool has_csr = true;             // false once the engine layout replaced it
 
     size_t bytes() const;
 };
-// Builds m.fs / m.bs from the CSR / CSC (sell.hip).
-void build_sell(Matrix &m, hipStream_t st);
+// Builds m.fs / m.bs from the CSR / CSC (sell.hip); drop_csr frees each of
+// the CSR and CSC as soon as its sliced copy exists.
+void build_sell(Matrix &m, hipStream_t st, bool drop_csr = false);
 
 // Builds the matrix for (g, S) on the stream; want64 also keeps double
 // lengths.  Host-side GoldenC walk for MI_BEAM_GOLDENC, device tracers
@@ -164,6 +167,8 @@ namespace mi {
 // Matrix for (g, S) from the context cache, building it if needed.  Jobs
 // hold the shared pointer, so trimming the cache never frees a matrix in use.
 std::shared_ptr<Matrix> get_matrix(mi_context *ctx, const mi_geometry &g, int S, bool want64);
+// The same geometry in the AT engine's sliced layout only (CSR / CSC freed).
+std::shared_ptr<Matrix> get_engine_matrix(mi_context *ctx, const mi_geometry &g, int S, bool want64);
 // Frees cached matrices that no job holds, least recently used first, until
 // at least `bytes` were freed (all idle ones when bytes < 0).  Returns bytes freed.
 int64_t evict_idle(mi_context *ctx, int64_t bytes);

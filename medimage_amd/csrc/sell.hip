@@ -170,7 +170,7 @@ void fill_sell(Sell &sl, It it, const uint32_t *src_idx, const float *s32, const
 
 }  // namespace
 
-void build_sell(Matrix &m, hipStream_t st) {
+void build_sell(Matrix &m, hipStream_t st, bool drop_csr) {
     if (m.has_sell) return;
     const mi_geometry &g = m.g;
     m.tny = (g.ny + kTileW - 1) / kTileW;
@@ -184,6 +184,12 @@ void build_sell(Matrix &m, hipStream_t st) {
         m.fs.slice0[s + 1] = m.fs.slice0[s] + (m.sub_row0[s + 1] - m.sub_row0[s] + m.fs.C - 1) / m.fs.C;
     FItems fi{m.rowptr.p, m.d_sub_row0.p, nullptr, m.S, m.fs.P, m.fs.C};
     fill_sell(m.fs, fi, m.idx.p, m.len32.p, f64 ? m.len64.p : nullptr, true, g.ny, m.tny, st);
+    if (drop_csr) {
+        m.rowptr.reset();
+        m.idx.reset();
+        m.len32.reset();
+        m.len64.reset();
+    }
     // back
     m.bs.P = choose_p(MI_SELL_PB, (double)m.nnz / ((double)m.S * std::max<int64_t>(m.npix, 1)));
     m.bs.C = kWave / m.bs.P;
@@ -192,6 +198,13 @@ void build_sell(Matrix &m, hipStream_t st) {
     for (int s = 0; s < m.S; ++s) m.bs.slice0[s + 1] = m.bs.slice0[s] + per;
     BItems bi{m.colptr.p, nullptr, m.S, m.bs.P, m.bs.C, g.nx, g.ny, m.tny, m.npix, m.npix_t};
     fill_sell(m.bs, bi, m.crow.p, m.clen32.p, f64 ? m.clen64.p : nullptr, false, g.ny, m.tny, st);
+    if (drop_csr) {
+        m.colptr.reset();
+        m.crow.reset();
+        m.clen32.reset();
+        m.clen64.reset();
+        m.has_csr = false;
+    }
     m.has_sell = true;
 }
 

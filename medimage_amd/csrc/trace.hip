@@ -62,41 +62,39 @@ __global__ void k_trace_fill(TraceArgs a, int64_t nrows, const int64_t *rowptr, 
     });
 }
 
-// One wave per row: sort key (subset * npix + pixel), the row id and the
-// entry's position for every entry.
-__global__ void k_csc_keys(const int64_t *rowptr, const uint32_t *idx, int64_t nrows, const int64_t *sub_row0,
-                           int S, uint32_t npix, uint32_t *keys, uint32_t *pos, uint32_t *rowid) {
-    const int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+// One wave per row of one subset: sort key (pixel), the entry's position in
+// the subset's entry range and its row.
+__global__ void k_csc_keys(const int64_t *rowptr, const uint32_t *idx, int64_t row0, int64_t nrows, int64_t e0,
+                           uint32_t *keys, uint32_t *pos, uint32_t *rowid) {
+    const int64_t r = row0 + ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const int lane = threadIdx.x & (kWave - 1);
-    if (r >= nrows) return;
-    int s = 0;
-    while (s + 1 < S && sub_row0[s + 1] <= r) ++s;
+    if (r >= row0 + nrows) return;
     for (int64_t k = rowptr[r] + lane; k < rowptr[r + 1]; k += kWave) {
-        keys[k] = (uint32_t)s * npix + idx[k];
-        pos[k] = (uint32_t)k;
-        rowid[k] = (uint32_t)r;
+        keys[k - e0] = idx[k];
+        pos[k - e0] = (uint32_t)(k - e0);
+        rowid[k - e0] = (uint32_t)r;
     }
 }
 
-__global__ void k_csc_gather(const uint32_t *pos, int64_t nnz, const uint32_t *rowid, const float *len32,
+__global__ void k_csc_gather(const uint32_t *pos, int64_t n, int64_t e0, const uint32_t *rowid, const float *len32,
                              const double *len64, uint32_t *crow, float *clen32, double *clen64) {
-    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= nnz) return;
-    uint32_t k = pos[j];
-    crow[j] = rowid[k];
-    clen32[j] = len32[k];
-    if (clen64) clen64[j] = len64[k];
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t k = pos[j];
+    crow[e0 + j] = rowid[k];
+    clen32[e0 + j] = len32[e0 + k];
+    if (clen64) clen64[e0 + j] = len64[e0 + k];
 }
 
-// colptr from the sorted keys: entry j opens every column between the
-// previous entry's key (exclusive) and its own key (inclusive).
-__global__ void k_csc_colptr(const uint32_t *keys, int64_t nnz, int64_t ncols, int64_t *colptr) {
-    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= nnz) return;
-    int64_t prev = (j == 0) ? -1 : (int64_t)keys[j - 1];
-    for (int64_t c = prev + 1; c <= (int64_t)keys[j]; ++c) colptr[c] = j;
-    if (j == nnz - 1)
-        for (int64_t c = (int64_t)keys[j] + 1; c <= ncols; ++c) colptr[c] = nnz;
+// colptr of one subset from its sorted keys: entry j opens every column
+// between the previous entry's key (exclusive) and its own (inclusive).
+__global__ void k_csc_colptr(const uint32_t *keys, int64_t n, int64_t e0, int64_t ncols, int64_t *colptr) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int64_t prev = (j == 0) ? -1 : (int64_t)keys[j - 1];
+    for (int64_t c = prev + 1; c <= (int64_t)keys[j]; ++c) colptr[c] = e0 + j;
+    if (j == n - 1)
+        for (int64_t c = (int64_t)keys[j] + 1; c < ncols; ++c) colptr[c] = e0 + n;
 }
 
 // Counts entries whose pixel index is outside the image (must be zero before
@@ -265,42 +263,54 @@ void check_indices(const uint32_t *idx, int64_t nnz, int npix, hipStream_t st) {
     MI_REQUIRE(h == 0, MI_E_STATE, "tracer produced " + std::to_string(h) + " pixel indices outside the image");
 }
 
-// Per-subset CSC from a subset-ordered CSR (device arrays).
+// Per-subset CSC from a subset-ordered CSR (device arrays).  The rows of a
+// subset hold a contiguous entry range; each range is sorted on its own by
+// pixel (stable radix sort), so positions stay 32-bit however large the whole
+// matrix is, and subset s's columns occupy the same entry range in the CSC.
 void build_csc(Matrix &m, const int64_t *rowptr, const uint32_t *idx, const float *len32, const double *len64,
                hipStream_t st) {
     check_indices(idx, m.nnz, m.npix, st);
     const int64_t nnz = m.nnz, ncols = (int64_t)m.S * m.npix;
-    MI_REQUIRE(nnz < (int64_t)UINT32_MAX && ncols < (int64_t)UINT32_MAX, MI_E_LIMIT,
-               "system matrix too large for 32-bit entry positions");
-    DevBuf<int64_t> d_sub(m.S + 1);
-    MI_HIP(hipMemcpyAsync(d_sub.p, m.sub_row0.data(), (m.S + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
     m.colptr.alloc(ncols + 1);
     alloc_padded(m.crow, nnz, st);
     alloc_padded(m.clen32, nnz, st);
     if (len64) alloc_padded(m.clen64, nnz, st);
-    if (nnz == 0) {
-        k_fill_i64<<<grid_of(ncols + 1), kBlock, 0, st>>>(m.colptr.p, ncols + 1, 0);
-        MI_HIP(hipGetLastError());
-        return;
-    }
-    DevBuf<uint32_t> keys(nnz), keys2(nnz), pos(nnz), pos2(nnz), rowid(nnz);
-    k_csc_keys<<<grid_of(m.nrows * kWave), kBlock, 0, st>>>(rowptr, idx, m.nrows, d_sub.p, m.S, (uint32_t)m.npix,
-                                                           keys.p, pos.p, rowid.p);
+    k_fill_i64<<<grid_of(ncols + 1), kBlock, 0, st>>>(m.colptr.p, ncols + 1, nnz);
     MI_HIP(hipGetLastError());
+    if (nnz == 0) return;
+    std::vector<int64_t> e(m.S + 1);
+    for (int s = 0; s <= m.S; ++s)
+        MI_HIP(hipMemcpyAsync(&e[s], rowptr + m.sub_row0[s], sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    MI_HIP(hipStreamSynchronize(st));
+    int64_t most = 0;
+    for (int s = 0; s < m.S; ++s) most = std::max(most, e[s + 1] - e[s]);
+    MI_REQUIRE(most < (int64_t)INT32_MAX && m.nrows < (int64_t)UINT32_MAX, MI_E_LIMIT,
+               "one subset of the system matrix exceeds 2^31 entries");
     int bits = 1;
-    while (bits < 32 && ((int64_t)1 << bits) < ncols) ++bits;
+    while (bits < 32 && ((int64_t)1 << bits) < m.npix) ++bits;
+    DevBuf<uint32_t> keys(most), keys2(most), pos(most), pos2(most), rowid(most);
     size_t tmp = 0;
-    MI_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, keys.p, keys2.p, pos.p, pos2.p, (int)nnz, 0, bits, st));
-    {
-        DevBuf<unsigned char> t(tmp ? tmp : 1);
-        MI_HIP(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, keys.p, keys2.p, pos.p, pos2.p, (int)nnz, 0, bits, st));
-        k_csc_gather<<<grid_of(nnz), kBlock, 0, st>>>(pos2.p, nnz, rowid.p, len32, len64, m.crow.p, m.clen32.p,
-                                                      len64 ? m.clen64.p : nullptr);
+    MI_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, keys.p, keys2.p, pos.p, pos2.p, (int)most, 0, bits, st));
+    DevBuf<unsigned char> t(tmp ? tmp : 1);
+    for (int s = 0; s < m.S; ++s) {
+        const int64_t n = e[s + 1] - e[s], nr = m.sub_row0[s + 1] - m.sub_row0[s];
+        if (n == 0) {
+            k_fill_i64<<<grid_of(m.npix), kBlock, 0, st>>>(m.colptr.p + (int64_t)s * m.npix, m.npix, e[s]);
+            MI_HIP(hipGetLastError());
+            continue;
+        }
+        k_csc_keys<<<grid_of(nr * kWave), kBlock, 0, st>>>(rowptr, idx, m.sub_row0[s], nr, e[s], keys.p, pos.p,
+                                                           rowid.p);
         MI_HIP(hipGetLastError());
-        k_csc_colptr<<<grid_of(nnz), kBlock, 0, st>>>(keys2.p, nnz, ncols, m.colptr.p);
+        size_t tn = tmp;
+        MI_HIP(hipcub::DeviceRadixSort::SortPairs(t.p, tn, keys.p, keys2.p, pos.p, pos2.p, (int)n, 0, bits, st));
+        k_csc_gather<<<grid_of(n), kBlock, 0, st>>>(pos2.p, n, e[s], rowid.p, len32, len64, m.crow.p, m.clen32.p,
+                                                    len64 ? m.clen64.p : nullptr);
         MI_HIP(hipGetLastError());
-        MI_HIP(hipStreamSynchronize(st));
+        k_csc_colptr<<<grid_of(n), kBlock, 0, st>>>(keys2.p, n, e[s], m.npix, m.colptr.p + (int64_t)s * m.npix);
+        MI_HIP(hipGetLastError());
     }
+    MI_HIP(hipStreamSynchronize(st));
 }
 
 // Entries of each subset's rows, from the device row pointer.

@@ -187,3 +187,43 @@ def test_batch_release_and_errors(ctx):
         b.submit(g, sino, mi.at_params(n_subsets=3))
     assert e.value.code == mi.MI_E_ARG
     b.close()
+
+
+def test_serving_stream_results_equal_alone(ctx):
+    """A Poisson stream served by the request loop (continuous batching,
+    jobs joining and leaving between passes): every job's result equals the
+    same job run alone, bit for bit; the plain one-at-a-time loop agrees too."""
+    from medimage_amd import serve, phantom
+    g = phantom.rng(77, 1)
+    reqs = []
+    t = 0.0
+    for i in range(10):
+        t += float(g.exponential(1 / 400.0))
+        geo = GEOMS["abc"[i % 3]]
+        _, sino = configs.make_case(ctx, geo, 3e4, seed=500 + i)
+        p = mi.at_params(n_subsets=4, max_iters=40, rel_tol=1e-3)
+        reqs.append(dict(t=t, geom=geo, sino=sino.astype(np.float32), params=p))
+    bat = serve.run_stream(ctx, reqs, n_subsets=4, mode="batched")
+    pla = serve.run_stream(ctx, reqs, n_subsets=4, mode="plain")
+    print(f"[serve] batched p50 {bat['latency_p50_s']*1e3:.1f} ms p99 {bat['latency_p99_s']*1e3:.1f} ms; "
+          f"plain p50 {pla['latency_p50_s']*1e3:.1f} ms p99 {pla['latency_p99_s']*1e3:.1f} ms", flush=True)
+    for i, r in enumerate(reqs):
+        alone = _run_batch(ctx, [(r["geom"], r["sino"], r["params"])])[0]
+        for res in (bat, pla):
+            u, v = res["results"][i]
+            assert _same(u, alone[0]) and _same(v, alone[1]), f"job {i} ({res['mode']})"
+        assert bat["iterations"][i] == alone[2].iterations
+
+
+def test_batch_admission_refuses_without_memory(ctx):
+    """With a reserve larger than the device, a job cannot be admitted and,
+    with nothing else on the device to wait for, it fails with a message."""
+    g = GEOMS["a"]
+    _, sino = configs.make_case(ctx, g, 3e4, seed=9)
+    b = ctx.batch(4)
+    b.set(mi.MI_BATCH_RESERVE_MB, 1 << 30)
+    jid = b.submit(g, sino.astype(np.float32), mi.at_params(n_subsets=4, max_iters=2))
+    a, q = b.step(1)
+    assert a == 0 and q == 0
+    assert b.status(jid).state == mi.MI_JOB_FAILED
+    b.close()
