@@ -203,6 +203,8 @@ def main():
           file=sys.stderr, flush=True)
 
     names = {mi.MI_K_FP: "fp_residual", mi.MI_K_BP: "bp_gather_grad", mi.MI_K_FPSQ: "fp_direction_norm"}
+    kernel_symbol = {mi.MI_K_FP: "k_eng_fp<float, 1, true>", mi.MI_K_BP: "k_eng_bp<float, 1>",
+                     mi.MI_K_FPSQ: "k_eng_fpsq_nv<float, 1>"}
     ks = {k: ctx.stats(k) for k in names}
     dom = max(ks, key=lambda k: ks[k].total_ms)
     s = ks[dom]
@@ -214,6 +216,15 @@ def main():
                                      (max(ks[k].total_ms, 1e-9) / max(ks[k].launches, 1) * 1e-3) / 1e9}
                   for k in names}
 
+    # HBM bytes per launch of the same kernel on the same workload, from the
+    # rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated against a plain copy
+    # kernel) recorded next to the kernel statistics.
+    traffic = None
+    tfile = ROOT / "profiles" / "c2_plain_traffic.json"
+    if tfile.exists():
+        tk = json.loads(tfile.read_text())["kernels"].get(kernel_symbol[dom])
+        if tk:
+            traffic = tk["traffic_bytes"]
     rays = g.n_views * g.n_bins
     out = {
         "metric": METRIC,
@@ -232,8 +243,9 @@ def main():
                                "SOD 541 mm, SDD 949 mm, 12 ordered subsets, alpha 1e-3, beta 1e-4, eps 1e-2",
                    "rays": rays, "nnz": nnz, "subsets": 12},
         "rays_per_s": rays * args.steps / elapsed,
-        "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+        "roofline": {"bound": "hbm", "kernel": kernel_symbol[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": "profiles/c2_plain_traffic.json" if traffic else None,
                      "bytes_per_launch": bytes_per, "avg_launch_us": avg_ms * 1e3},
         "kernels": per_kernel,
     }
