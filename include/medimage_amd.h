@@ -275,6 +275,9 @@ typedef struct mi_kernel_stats {
 /* Enable per-launch HIP-event timing of the kernels above (adds one event
  * pair per launch on the context's stream). */
 int mi_stats_enable(mi_context *ctx, int on);
+/* Times only the kernels whose bit is set in mask (bit k = kernel id k);
+ * each timed launch adds one event pair to the stream.  0 turns timing off. */
+int mi_stats_select(mi_context *ctx, uint32_t mask);
 int mi_stats_get(mi_context *ctx, int kernel, mi_kernel_stats *out);
 int mi_stats_reset(mi_context *ctx);
 /* A plain streaming copy of `bytes` (multiple of 16) between two fresh

@@ -159,6 +159,7 @@ def _load():
         "mi_batch_units": [P, P, C.c_int, C.POINTER(C.c_int)],
         "mi_batch_retired": [P, P, C.c_int, C.POINTER(C.c_int)],
         "mi_stats_enable": [P, C.c_int],
+        "mi_stats_select": [P, C.c_uint32],
         "mi_stats_get": [P, C.c_int, C.POINTER(KernelStats)],
         "mi_stats_reset": [P],
         "mi_device_copy": [P, C.c_int64, C.c_int, C.POINTER(C.c_double)],
@@ -331,6 +332,13 @@ class Context:
     # -- instrumentation --------------------------------------------------
     def stats_enable(self, on=True):
         _check(self._lib.mi_stats_enable(self._h, int(bool(on))))
+
+    def stats_select(self, kernels):
+        """Time only the given kernel ids (an empty list turns timing off)."""
+        mask = 0
+        for k in kernels:
+            mask |= 1 << k
+        _check(self._lib.mi_stats_select(self._h, mask))
 
     def stats(self, kernel: int) -> KernelStats:
         s = KernelStats()

@@ -127,7 +127,7 @@ int64_t evict_idle(mi_context *ctx, int64_t bytes) {
 }
 
 LaunchTimer::LaunchTimer(mi_context *c, int k, double by) : ctx(c), kernel(k), bytes(by) {
-    if (!ctx->stats.on) return;
+    if (!ctx->stats.on || !((ctx->stats.mask >> k) & 1u)) return;
     auto &pool = ctx->stats.pool;
     auto take = [&]() {
         hipEvent_t e;
@@ -824,6 +824,15 @@ int mi_stats_enable(mi_context *ctx, int on) {
     return guarded([&] {
         check_ctx(ctx);
         ctx->stats.on = on != 0;
+        ctx->stats.mask = ~0u;
+    });
+}
+
+int mi_stats_select(mi_context *ctx, uint32_t mask) {
+    return guarded([&] {
+        check_ctx(ctx);
+        ctx->stats.on = mask != 0;
+        ctx->stats.mask = mask;
     });
 }
 

@@ -147,6 +147,7 @@ std::vector<int> subset_order(int S);
 // Timing / byte counters of the instrumented kernels.
 struct Stats {
     bool on = false;
+    uint32_t mask = ~0u;             // instrumented kernel ids (bit k = kernel k)
     std::array<mi_kernel_stats, MI_K_COUNT> k{};
     struct Pending { int kernel; hipEvent_t a, b; double bytes; };
     std::vector<Pending> pending;
