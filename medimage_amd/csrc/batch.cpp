@@ -371,11 +371,15 @@ void Batch<T>::rebuild() {
         D.fs_base = m.fs.base.p;
         D.fs_T = m.fs.T.p;
         D.fs_idx = m.fs.idx.p;
+        D.fs_first = m.fs.first.p;
+        D.fs_d16 = m.fs.d16.p;
         D.fs_len = f64 ? (const T *)m.fs.l64.p : (const T *)m.fs.l32.p;
         D.bs_slice0 = m.bs.d_slice0.p;
         D.bs_base = m.bs.base.p;
         D.bs_T = m.bs.T.p;
         D.bs_idx = m.bs.idx.p;
+        D.bs_first = m.bs.first.p;
+        D.bs_d16 = m.bs.d16.p;
         D.bs_len = f64 ? (const T *)m.bs.l64.p : (const T *)m.bs.l32.p;
         D.sub_row0 = m.d_sub_row0.p;
         D.g = U.g.p;

@@ -33,11 +33,13 @@ template <typename T>
 struct UG {
     const GL int64_t *fs_slice0, *fs_base;
     const GL int32_t *fs_T;
-    const GL uint32_t *fs_idx;
+    const GL uint32_t *fs_idx, *fs_first;
+    const GL int16_t *fs_d16;
     const GL T *fs_len;
     const GL int64_t *bs_slice0, *bs_base;
     const GL int32_t *bs_T;
-    const GL uint32_t *bs_idx;
+    const GL uint32_t *bs_idx, *bs_first;
+    const GL int16_t *bs_d16;
     const GL T *bs_len;
     const GL int64_t *sub_row0;
     GL T *g, *r, *u, *v, *d, *w;
@@ -45,9 +47,12 @@ struct UG {
     int nx, ny, tny, K, fs_P, bs_P, job0;
     __device__ __forceinline__ explicit UG(const UnitDesc<T> &D)
         : fs_slice0((const GL int64_t *)D.fs_slice0), fs_base((const GL int64_t *)D.fs_base),
-          fs_T((const GL int32_t *)D.fs_T), fs_idx((const GL uint32_t *)D.fs_idx), fs_len((const GL T *)D.fs_len),
-          bs_slice0((const GL int64_t *)D.bs_slice0), bs_base((const GL int64_t *)D.bs_base),
-          bs_T((const GL int32_t *)D.bs_T), bs_idx((const GL uint32_t *)D.bs_idx), bs_len((const GL T *)D.bs_len),
+          fs_T((const GL int32_t *)D.fs_T), fs_idx((const GL uint32_t *)D.fs_idx),
+          fs_first((const GL uint32_t *)D.fs_first), fs_d16((const GL int16_t *)D.fs_d16),
+          fs_len((const GL T *)D.fs_len), bs_slice0((const GL int64_t *)D.bs_slice0),
+          bs_base((const GL int64_t *)D.bs_base), bs_T((const GL int32_t *)D.bs_T),
+          bs_idx((const GL uint32_t *)D.bs_idx), bs_first((const GL uint32_t *)D.bs_first),
+          bs_d16((const GL int16_t *)D.bs_d16), bs_len((const GL T *)D.bs_len),
           sub_row0((const GL int64_t *)D.sub_row0), g((GL T *)D.g), r((GL T *)D.r), u((GL T *)D.u),
           v((GL T *)D.v), d((GL T *)D.d), w((GL T *)D.w), npix_t(D.npix_t), nrows(D.nrows),
           part_off(D.part_off), cap(D.cap), nx(D.nx), ny(D.ny), tny(D.tny), K(D.K), fs_P(D.fs_P), bs_P(D.bs_P),
@@ -185,22 +190,34 @@ __device__ __forceinline__ void block_store(const EngArgs<T> &a, const UG<T> &U,
 }
 
 // ---------------------------------------------------------------------------
-// One projector lane: its segment's entries t = 0 .. n-1 at p0 + 64 t,
-// Kahan-summed in entry order (zero-length padding skipped), gathering K
-// interleaved values of `x` per entry.
-template <typename T, int K, bool NT>
-__device__ __forceinline__ void walk(const GL uint32_t *__restrict__ ip, const GL T *__restrict__ lp, int n,
-                                     const GL T *__restrict__ x, Acc<T> (&acc)[K]) {
+// One projector lane: its entries t = 0 .. n-1 at p0 + 64 t, Kahan-summed in
+// entry order (zero-length padding skipped), gathering K interleaved values
+// of `x` per entry.  Index stream: absolute (ip) or, when D16, 16-bit
+// differences (dp) accumulated from the lane's first index.
+template <typename T, int K, bool NT, bool D16>
+__device__ __forceinline__ void walk(const GL uint32_t *__restrict__ ip, const GL int16_t *__restrict__ dp,
+                                     uint32_t cur, const GL T *__restrict__ lp, int n, const GL T *__restrict__ x,
+                                     Acc<T> (&acc)[K]) {
     constexpr int UN = (K <= 2) ? MI_STEPS : 4;
     int t = 0;
     for (; t + UN <= n; t += UN) {
         uint32_t id[UN];
         T ln[UN], xv[UN][K];
+        if constexpr (D16) {
+            int16_t dd[UN];
 #pragma unroll
-        for (int q = 0; q < UN; ++q) {
-            id[q] = stream_ld<NT>(ip + (int64_t)(t + q) * kWave);
-            ln[q] = stream_ld<NT>(lp + (int64_t)(t + q) * kWave);
+            for (int q = 0; q < UN; ++q) dd[q] = stream_ld<NT>(dp + (int64_t)(t + q) * kWave);
+#pragma unroll
+            for (int q = 0; q < UN; ++q) {
+                cur += (uint32_t)(int32_t)dd[q];
+                id[q] = cur;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < UN; ++q) id[q] = stream_ld<NT>(ip + (int64_t)(t + q) * kWave);
         }
+#pragma unroll
+        for (int q = 0; q < UN; ++q) ln[q] = stream_ld<NT>(lp + (int64_t)(t + q) * kWave);
 #pragma unroll
         for (int q = 0; q < UN; ++q) loadk<T, K>(x + (uint32_t)(id[q] * K), xv[q]);
 #pragma unroll
@@ -210,7 +227,13 @@ __device__ __forceinline__ void walk(const GL uint32_t *__restrict__ ip, const G
                 for (int j = 0; j < K; ++j) acc[j].add(xv[q][j] * ln[q]);
     }
     for (; t < n; ++t) {
-        const uint32_t id = stream_ld<NT>(ip + (int64_t)t * kWave);
+        uint32_t id;
+        if constexpr (D16) {
+            cur += (uint32_t)(int32_t)stream_ld<NT>(dp + (int64_t)t * kWave);
+            id = cur;
+        } else {
+            id = stream_ld<NT>(ip + (int64_t)t * kWave);
+        }
         const T ln = stream_ld<NT>(lp + (int64_t)t * kWave);
         T xv[K];
         loadk<T, K>(x + (uint32_t)(id * K), xv);
@@ -218,6 +241,18 @@ __device__ __forceinline__ void walk(const GL uint32_t *__restrict__ ip, const G
 #pragma unroll
             for (int j = 0; j < K; ++j) acc[j].add(xv[j] * ln);
     }
+}
+
+// Walks lane `lane` of slice sl of one sliced matrix in whichever index form
+// it is stored.
+template <typename T, int K, bool NT>
+__device__ __forceinline__ void walk_slice(const GL int64_t *base, const GL int32_t *Tn, const GL uint32_t *idx,
+                                           const GL uint32_t *first, const GL int16_t *d16, const GL T *len,
+                                           int64_t sl, int lane, const GL T *x, Acc<T> (&acc)[K]) {
+    const int64_t p0 = base[sl] + lane;
+    const int n = Tn[sl];
+    if (d16) walk<T, K, NT, true>(nullptr, d16 + p0, first[sl * kWave + lane], len + p0, n, x, acc);
+    else walk<T, K, NT, false>(idx + p0, nullptr, 0u, len + p0, n, x, acc);
 }
 
 // Forward projection of subset s: a block is 4 slices (one per wave).
@@ -231,9 +266,10 @@ __device__ __forceinline__ void fp_unit(const EngArgs<T> &a, const UG<T> &U, int
 #pragma unroll
     for (int j = 0; j < K; ++j) m[j] = 0.0;
     if (li < U.fs_slice0[s + 1] - sl0) {
-        const int64_t sl = sl0 + li, base = U.fs_base[sl] + lane;
+        const int64_t sl = sl0 + li;
         Acc<T> acc[K];
-        walk<T, K, MI_NT_FP>(U.fs_idx + base, U.fs_len + base, U.fs_T[sl], RESID ? U.u : U.d, acc);
+        walk_slice<T, K, MI_NT_FP>(U.fs_base, U.fs_T, U.fs_idx, U.fs_first, U.fs_d16, U.fs_len, sl, lane,
+                                   RESID ? U.u : U.d, acc);
         double tot[K];
 #pragma unroll
         for (int j = 0; j < K; ++j) tot[j] = acc[j].value();
@@ -300,9 +336,10 @@ __device__ __forceinline__ void bp_unit(const EngArgs<T> &a, const UG<T> &U, int
 #pragma unroll
     for (int j = 0; j < K; ++j) m[j] = 0.0;
     if (li < U.bs_slice0[s + 1] - sl0) {
-        const int64_t sl = sl0 + li, base = U.bs_base[sl] + lane;
+        const int64_t sl = sl0 + li;
         Acc<T> acc[K];
-        walk<T, K, MI_NT_BP>(U.bs_idx + base, U.bs_len + base, U.bs_T[sl], U.r, acc);
+        walk_slice<T, K, MI_NT_BP>(U.bs_base, U.bs_T, U.bs_idx, U.bs_first, U.bs_d16, U.bs_len, sl, lane, U.r,
+                                   acc);
         double tot[K];
 #pragma unroll
         for (int j = 0; j < K; ++j) tot[j] = acc[j].value();

@@ -34,11 +34,15 @@ template <typename T>
 struct UnitDesc {
     const int64_t *fs_slice0, *fs_base;   // forward: S + 1 slice offsets, per-slice entry base
     const int32_t *fs_T;                   // per-slice entries per lane
-    const uint32_t *fs_idx;                // tiled pixel indices
+    const uint32_t *fs_idx;                // tiled pixel indices (null when compact)
+    const uint32_t *fs_first;              // compact form: first index per lane
+    const int16_t *fs_d16;                 // compact form: index differences
     const T *fs_len;
     const int64_t *bs_slice0, *bs_base;   // back: same, items are tiled pixels
     const int32_t *bs_T;
-    const uint32_t *bs_idx;                // rows
+    const uint32_t *bs_idx;                // rows (null when compact)
+    const uint32_t *bs_first;
+    const int16_t *bs_d16;
     const T *bs_len;
     const int64_t *sub_row0;               // S + 1 row offsets
     T *g, *r;                              // rows * K

@@ -72,11 +72,17 @@ struct Sell {
     DevBuf<int64_t> base;            // per slice
     DevBuf<int32_t> T;               // per slice: entries per lane
     DevBuf<uint32_t> idx;            // forward: tiled pixel; back: row
+    // Compact index stream (when every step fits 16 bits): the lane's first
+    // index, then per entry the difference to the lane's previous entry
+    // (0 at t = 0 and in the padding).  idx is freed then.
+    DevBuf<uint32_t> first;          // per (slice, lane)
+    DevBuf<int16_t> d16;
     DevBuf<float> l32;
     DevBuf<double> l64;
     int64_t entries = 0;             // stored, padding included
     size_t bytes() const {
-        return d_slice0.bytes() + base.bytes() + T.bytes() + idx.bytes() + l32.bytes() + l64.bytes();
+        return d_slice0.bytes() + base.bytes() + T.bytes() + idx.bytes() + first.bytes() + d16.bytes() +
+               l32.bytes() + l64.bytes();
     }
 };
 

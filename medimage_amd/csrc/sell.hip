@@ -112,6 +112,51 @@ __global__ void k_sell_fill(It it, int64_t nslices, const int64_t *base, const i
     }
 }
 
+// Index deltas of one lane: first[] = its first index, d16 = difference to
+// the previous entry; *overflow set when a difference does not fit 16 bits.
+__global__ void k_sell_delta(int64_t nslices, const int64_t *base, const int32_t *T, const uint32_t *idx,
+                             uint32_t *first, int16_t *d16, unsigned *overflow) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t sl = g / kWave;
+    if (sl >= nslices) return;
+    const int lane = (int)(g % kWave);
+    const int n = T[sl];
+    int64_t pos = base[sl] + lane;
+    int64_t prev = n > 0 ? (int64_t)idx[pos] : 0;
+    first[g] = (uint32_t)prev;
+    bool bad = false;
+    for (int t = 0; t < n; ++t, pos += kWave) {
+        const int64_t cur = idx[pos];
+        const int64_t d = cur - prev;
+        if (d > 32767 || d < -32768) bad = true;
+        d16[pos] = (int16_t)d;
+        prev = cur;
+    }
+    if (bad) atomicOr(overflow, 1u);
+}
+
+// Replaces sl.idx by the delta form when every difference fits.
+void compact_sell(Sell &sl, hipStream_t st) {
+    const int64_t ns = sl.slice0.back();
+    if (ns == 0 || sl.entries == 0) return;
+    DevBuf<unsigned> flag(1);
+    MI_HIP(hipMemsetAsync(flag.p, 0, sizeof(unsigned), st));
+    sl.first.alloc((size_t)ns * kWave);
+    sl.d16.alloc((size_t)sl.entries);
+    k_sell_delta<<<grid_of(ns * kWave), kBlock, 0, st>>>(ns, sl.base.p, sl.T.p, sl.idx.p, sl.first.p, sl.d16.p,
+                                                         flag.p);
+    MI_HIP(hipGetLastError());
+    unsigned h = 0;
+    MI_HIP(hipMemcpyAsync(&h, flag.p, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    MI_HIP(hipStreamSynchronize(st));
+    if (h) {
+        sl.first.reset();
+        sl.d16.reset();
+    } else {
+        sl.idx.reset();
+    }
+}
+
 // Lanes per item (measured on the 512^2 / 720-view workload).  Forward: 8,
 // so a wave step covers 8 neighbouring rays x 8 consecutive entries, an
 // 8 x 8 patch of pixels (60 us per subset against 76 us at 4 and 73 us for
@@ -184,6 +229,7 @@ void build_sell(Matrix &m, hipStream_t st, bool drop_csr) {
         m.fs.slice0[s + 1] = m.fs.slice0[s] + (m.sub_row0[s + 1] - m.sub_row0[s] + m.fs.C - 1) / m.fs.C;
     FItems fi{m.rowptr.p, m.d_sub_row0.p, nullptr, m.S, m.fs.P, m.fs.C};
     fill_sell(m.fs, fi, m.idx.p, m.len32.p, f64 ? m.len64.p : nullptr, true, g.ny, m.tny, st);
+    compact_sell(m.fs, st);
     if (drop_csr) {
         m.rowptr.reset();
         m.idx.reset();
@@ -198,6 +244,7 @@ void build_sell(Matrix &m, hipStream_t st, bool drop_csr) {
     for (int s = 0; s < m.S; ++s) m.bs.slice0[s + 1] = m.bs.slice0[s] + per;
     BItems bi{m.colptr.p, nullptr, m.S, m.bs.P, m.bs.C, g.nx, g.ny, m.tny, m.npix, m.npix_t};
     fill_sell(m.bs, bi, m.crow.p, m.clen32.p, f64 ? m.clen64.p : nullptr, false, g.ny, m.tny, st);
+    compact_sell(m.bs, st);
     if (drop_csr) {
         m.colptr.reset();
         m.crow.reset();
