@@ -29,7 +29,14 @@ int64_t max_slices(const Sell &q) {
     for (size_t s = 0; s + 1 < q.slice0.size(); ++s) r = std::max(r, q.slice0[s + 1] - q.slice0[s]);
     return r;
 }
-int64_t fp_blocks(const Matrix &m) { return cdiv(max_slices(m.fs), kBlock / kWave); }
+int64_t fp_blocks(const Matrix &m) {
+    if (m.has_otf) {
+        int64_t r = 0;
+        for (int s = 0; s < m.S; ++s) r = std::max(r, m.otf.slice0[s + 1] - m.otf.slice0[s]);
+        return cdiv(r, kBlock / kWave);
+    }
+    return cdiv(max_slices(m.fs), kBlock / kWave);
+}
 int64_t bp_blocks(const Matrix &m) { return cdiv(max_slices(m.bs), kBlock / kWave); }
 int64_t px_blocks(const Matrix &m) { return cdiv(m.npix_t, kPixBlock); }
 
@@ -367,7 +374,18 @@ void Batch<T>::rebuild() {
         const Matrix &m = *U.m;
         UnitDesc<T> &D = desc[i];
         std::memset(&D, 0, sizeof(D));
-        D.fs_slice0 = m.fs.d_slice0.p;
+        D.fs_slice0 = m.has_otf ? m.otf.d_slice0.p : m.fs.d_slice0.p;
+        if (m.has_otf) {
+            D.ot_lam = m.otf.lam.p;
+            D.ot_ax = m.otf.ax.p;
+            D.ot_ay = m.otf.ay.p;
+            D.ot_stx = m.otf.stx.p;
+            D.ot_sty = m.otf.sty.p;
+            D.ot_cx = m.otf.cx.p;
+            D.ot_cy = m.otf.cy.p;
+            D.ot_sg = m.otf.sg.p;
+            D.ot_cnt = m.otf.cnt.p;
+        }
         D.fs_base = m.fs.base.p;
         D.fs_T = m.fs.T.p;
         D.fs_idx = m.fs.idx.p;

@@ -94,7 +94,7 @@ std::shared_ptr<Matrix> get_engine_matrix(mi_context *ctx, const mi_geometry &g,
     std::shared_ptr<Matrix> m;
     for (int attempt = 0;; ++attempt) {
         try {
-            m.reset(build_matrix(g, S, want64, ctx->stream).release());
+            m.reset(build_matrix(g, S, want64, ctx->stream, true).release());
             build_sell(*m, ctx->stream, true);
             break;
         } catch (const Error &e) {

@@ -40,12 +40,21 @@ template <> struct Acc<float> {
         c = (t - s) - y;
         s = t;
     }
+    // add(x) when `take`, else nothing (a select, no branch).
+    __device__ __forceinline__ void add_if(float x, bool take) {
+        const float y = x - c;
+        const float t = s + y;
+        const float cn = (t - s) - y;
+        s = take ? t : s;
+        c = take ? cn : c;
+    }
     __device__ __forceinline__ double value() const { return (double)s - (double)c; }
 };
 
 template <> struct Acc<double> {
     double s = 0.0;
     __device__ __forceinline__ void add(double x) { s += x; }
+    __device__ __forceinline__ void add_if(double x, bool take) { s = take ? s + x : s; }
     __device__ __forceinline__ double value() const { return s; }
 };
 

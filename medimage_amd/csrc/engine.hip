@@ -15,8 +15,10 @@
 // lines.  Images are stored in 4 x 8 tiles (tile_index).  The P segment sums
 // of an item are combined by a fixed butterfly.
 #include "engine.hpp"
+#include "tracers.hpp"
 
 #include <cmath>
+#include <type_traits>
 
 namespace mi {
 
@@ -41,6 +43,8 @@ struct UG {
     const GL uint32_t *bs_idx, *bs_first;
     const GL int16_t *bs_d16;
     const GL T *bs_len;
+    const GL double *ot_lam, *ot_ax, *ot_ay, *ot_stx, *ot_sty;
+    const GL int32_t *ot_cx, *ot_cy, *ot_sg, *ot_cnt;
     const GL int64_t *sub_row0;
     GL T *g, *r, *u, *v, *d, *w;
     int64_t npix_t, nrows, part_off, cap;
@@ -53,6 +57,10 @@ struct UG {
           bs_base((const GL int64_t *)D.bs_base), bs_T((const GL int32_t *)D.bs_T),
           bs_idx((const GL uint32_t *)D.bs_idx), bs_first((const GL uint32_t *)D.bs_first),
           bs_d16((const GL int16_t *)D.bs_d16), bs_len((const GL T *)D.bs_len),
+          ot_lam((const GL double *)D.ot_lam), ot_ax((const GL double *)D.ot_ax), ot_ay((const GL double *)D.ot_ay),
+          ot_stx((const GL double *)D.ot_stx), ot_sty((const GL double *)D.ot_sty),
+          ot_cx((const GL int32_t *)D.ot_cx), ot_cy((const GL int32_t *)D.ot_cy),
+          ot_sg((const GL int32_t *)D.ot_sg), ot_cnt((const GL int32_t *)D.ot_cnt),
           sub_row0((const GL int64_t *)D.sub_row0), g((GL T *)D.g), r((GL T *)D.r), u((GL T *)D.u),
           v((GL T *)D.v), d((GL T *)D.d), w((GL T *)D.w), npix_t(D.npix_t), nrows(D.nrows),
           part_off(D.part_off), cap(D.cap), nx(D.nx), ny(D.ny), tny(D.tny), K(D.K), fs_P(D.fs_P), bs_P(D.bs_P),
@@ -107,6 +115,34 @@ __device__ __forceinline__ void loadk(const GL T *__restrict__ p, T (&o)[K]) {
         __builtin_memcpy(&o[0], &a, 8);
     } else {
         o[0] = p[0];
+    }
+}
+
+// Gathers through a raw buffer resource: a 32-bit byte offset per lane against
+// a base held in scalar registers (the image is < 4 GiB).
+#ifndef MI_BUF_GATHER
+#define MI_BUF_GATHER 1
+#endif
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, 0x00020000);
+}
+
+template <typename T, int K>
+__device__ __forceinline__ void loadk_buf(__amdgpu_buffer_rsrc_t rs, uint32_t off, T (&o)[K]) {
+    constexpr int B = K * (int)sizeof(T);
+    if constexpr (B >= 16) {
+#pragma unroll
+        for (int c = 0; c < B / 16; ++c) {
+            const u32x4 a = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)(off + 16 * c), 0, 0);
+            __builtin_memcpy(&o[c * (16 / (int)sizeof(T))], &a, 16);
+        }
+    } else if constexpr (B == 8) {
+        typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+        const u32x2 a = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)off, 0, 0);
+        __builtin_memcpy(&o[0], &a, 8);
+    } else {
+        const uint32_t a = __builtin_amdgcn_raw_buffer_load_b32(rs, (int)off, 0, 0);
+        __builtin_memcpy(&o[0], &a, 4);
     }
 }
 
@@ -190,56 +226,118 @@ __device__ __forceinline__ void block_store(const EngArgs<T> &a, const UG<T> &U,
 }
 
 // ---------------------------------------------------------------------------
-// One projector lane: its entries t = 0 .. n-1 at p0 + 64 t, Kahan-summed in
-// entry order (zero-length padding skipped), gathering K interleaved values
-// of `x` per entry.  Index stream: absolute (ip) or, when D16, 16-bit
-// differences (dp) accumulated from the lane's first index.
+// One projector lane: its entries t = 0 .. n-1 (sell_pos), Kahan-summed in
+// entry order (zero-length padding skipped), gathering K interleaved values of
+// `x` per entry.  Index stream: absolute (ip) or, when D16, 16-bit
+// differences (dp) accumulated from the lane's first index.  ip / dp / lp
+// point at the lane's first group; a group of 4 steps is one 16-byte length
+// load and one 16-byte (absolute) or 8-byte (differences) index load.
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
 template <typename T, int K, bool NT, bool D16>
 __device__ __forceinline__ void walk(const GL uint32_t *__restrict__ ip, const GL int16_t *__restrict__ dp,
                                      uint32_t cur, const GL T *__restrict__ lp, int n, const GL T *__restrict__ x,
-                                     Acc<T> (&acc)[K]) {
-    constexpr int UN = (K <= 2) ? MI_STEPS : 4;
-    int t = 0;
-    for (; t + UN <= n; t += UN) {
-        uint32_t id[UN];
-        T ln[UN], xv[UN][K];
+                                     uint32_t xbytes, Acc<T> (&acc)[K]) {
+    constexpr int GR = (K <= 2) ? MI_STEPS / kGroup : 1;   // groups per pipeline stage
+    constexpr int64_t kStride = kGroup * kWave;             // entries between a lane's groups
+#if MI_BUF_GATHER
+    const __amdgpu_buffer_rsrc_t xr = make_rsrc((const void *)x, xbytes);
+#else
+    (void)xbytes;
+#endif
+    uint32_t sv[GR][kGroup];
+    T lv[GR][kGroup];
+    auto fetch = [&](int g0) {
+#pragma unroll
+        for (int r = 0; r < GR; ++r) {
+            const int64_t o = (int64_t)(g0 + r) * kStride;
+            if constexpr (D16) {
+                const u32x2 a = stream_ld<NT>((const GL u32x2 *)(dp + o));
+                sv[r][0] = (uint32_t)(int32_t)(int16_t)(a.x & 0xffffu);
+                sv[r][1] = (uint32_t)(int32_t)(int16_t)(a.x >> 16);
+                sv[r][2] = (uint32_t)(int32_t)(int16_t)(a.y & 0xffffu);
+                sv[r][3] = (uint32_t)(int32_t)(int16_t)(a.y >> 16);
+            } else {
+                const u32x4 a = stream_ld<NT>((const GL u32x4 *)(ip + o));
+                sv[r][0] = a.x; sv[r][1] = a.y; sv[r][2] = a.z; sv[r][3] = a.w;
+            }
+            if constexpr (sizeof(T) == 4) {
+                const u32x4 a = stream_ld<NT>((const GL u32x4 *)(lp + o));
+                __builtin_memcpy(&lv[r][0], &a, 16);
+            } else {
+                const u32x4 a = stream_ld<NT>((const GL u32x4 *)(lp + o));
+                const u32x4 b = stream_ld<NT>((const GL u32x4 *)(lp + o) + 1);
+                __builtin_memcpy(&lv[r][0], &a, 16);
+                __builtin_memcpy(&lv[r][2], &b, 16);
+            }
+        }
+    };
+    auto gather = [&](uint32_t id, T (&o)[K]) {
+#if MI_BUF_GATHER
+        loadk_buf<T, K>(xr, id * (uint32_t)(K * sizeof(T)), o);
+#else
+        loadk<T, K>(x + (uint32_t)(id * K), o);
+#endif
+    };
+    const int ng = n / kGroup, full = ng / GR;
+    // Software pipeline: the next stage's stream loads are issued after the
+    // current stage's gathers (the barriers keep that order) and before they
+    // are consumed, so a lane waits for one memory round trip per stage.
+    if (full > 0) fetch(0);
+    int g = 0;
+    for (int c = 0; c < full; ++c, g += GR) {
+        uint32_t id[GR][kGroup];
+        T ln[GR][kGroup], xv[GR][kGroup][K];
+#pragma unroll
+        for (int r = 0; r < GR; ++r)
+#pragma unroll
+            for (int q = 0; q < kGroup; ++q) {
+                if constexpr (D16) {
+                    cur += sv[r][q];
+                    id[r][q] = cur;
+                } else {
+                    id[r][q] = sv[r][q];
+                }
+                ln[r][q] = lv[r][q];
+            }
+#pragma unroll
+        for (int r = 0; r < GR; ++r)
+#pragma unroll
+            for (int q = 0; q < kGroup; ++q) gather(id[r][q], xv[r][q]);
+        __builtin_amdgcn_sched_barrier(0);
+        fetch(c + 1 < full ? g + GR : g);   // the last stage re-reads its own groups
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int r = 0; r < GR; ++r)
+#pragma unroll
+            for (int q = 0; q < kGroup; ++q)
+#pragma unroll
+                for (int j = 0; j < K; ++j) acc[j].add_if(xv[r][q][j] * ln[r][q], ln[r][q] != T(0));
+    }
+    for (; g < ng; ++g) {
+        const int64_t o = (int64_t)g * kStride;
+        uint32_t id[kGroup];
+        T ln[kGroup], xv[kGroup][K];
         if constexpr (D16) {
-            int16_t dd[UN];
+            const u32x2 a = stream_ld<NT>((const GL u32x2 *)(dp + o));
+            const uint32_t w[2] = {a.x, a.y};
 #pragma unroll
-            for (int q = 0; q < UN; ++q) dd[q] = stream_ld<NT>(dp + (int64_t)(t + q) * kWave);
-#pragma unroll
-            for (int q = 0; q < UN; ++q) {
-                cur += (uint32_t)(int32_t)dd[q];
+            for (int q = 0; q < kGroup; ++q) {
+                cur += (uint32_t)(int32_t)(int16_t)((w[q >> 1] >> (16 * (q & 1))) & 0xffffu);
                 id[q] = cur;
             }
         } else {
-#pragma unroll
-            for (int q = 0; q < UN; ++q) id[q] = stream_ld<NT>(ip + (int64_t)(t + q) * kWave);
+            const u32x4 a = stream_ld<NT>((const GL u32x4 *)(ip + o));
+            id[0] = a.x; id[1] = a.y; id[2] = a.z; id[3] = a.w;
         }
 #pragma unroll
-        for (int q = 0; q < UN; ++q) ln[q] = stream_ld<NT>(lp + (int64_t)(t + q) * kWave);
+        for (int q = 0; q < kGroup; ++q) ln[q] = lp[o + q];
 #pragma unroll
-        for (int q = 0; q < UN; ++q) loadk<T, K>(x + (uint32_t)(id[q] * K), xv[q]);
+        for (int q = 0; q < kGroup; ++q) gather(id[q], xv[q]);
 #pragma unroll
-        for (int q = 0; q < UN; ++q)
-            if (ln[q] != T(0))
+        for (int q = 0; q < kGroup; ++q)
 #pragma unroll
-                for (int j = 0; j < K; ++j) acc[j].add(xv[q][j] * ln[q]);
-    }
-    for (; t < n; ++t) {
-        uint32_t id;
-        if constexpr (D16) {
-            cur += (uint32_t)(int32_t)stream_ld<NT>(dp + (int64_t)t * kWave);
-            id = cur;
-        } else {
-            id = stream_ld<NT>(ip + (int64_t)t * kWave);
-        }
-        const T ln = stream_ld<NT>(lp + (int64_t)t * kWave);
-        T xv[K];
-        loadk<T, K>(x + (uint32_t)(id * K), xv);
-        if (ln != T(0))
-#pragma unroll
-            for (int j = 0; j < K; ++j) acc[j].add(xv[j] * ln);
+            for (int j = 0; j < K; ++j) acc[j].add_if(xv[q][j] * ln[q], ln[q] != T(0));
     }
 }
 
@@ -248,11 +346,75 @@ __device__ __forceinline__ void walk(const GL uint32_t *__restrict__ ip, const G
 template <typename T, int K, bool NT>
 __device__ __forceinline__ void walk_slice(const GL int64_t *base, const GL int32_t *Tn, const GL uint32_t *idx,
                                            const GL uint32_t *first, const GL int16_t *d16, const GL T *len,
-                                           int64_t sl, int lane, const GL T *x, Acc<T> (&acc)[K]) {
-    const int64_t p0 = base[sl] + lane;
+                                           int64_t sl, int lane, const GL T *x, int64_t xn, Acc<T> (&acc)[K]) {
+    const int64_t p0 = sell_pos(base[sl], 0, lane);
     const int n = Tn[sl];
-    if (d16) walk<T, K, NT, true>(nullptr, d16 + p0, first[sl * kWave + lane], len + p0, n, x, acc);
-    else walk<T, K, NT, false>(idx + p0, nullptr, 0u, len + p0, n, x, acc);
+    const uint32_t xb = (uint32_t)(xn * K * (int64_t)sizeof(T));
+    if (d16) walk<T, K, NT, true>(nullptr, d16 + p0, first[sl * kWave + lane], len + p0, n, x, xb, acc);
+    else walk<T, K, NT, false>(idx + p0, nullptr, 0u, len + p0, n, x, xb, acc);
+}
+
+// One lane of the checkpointed forward walk: n Siddon steps from the stored
+// tracer state (the reference tracer's operations, tracers.hpp), each step's
+// pixel gathered from the tiled image x and its length (rounded to T as the
+// stored matrix rounds it) Kahan-summed.  Steps are taken OS at a time: the
+// tracer arithmetic of the next group runs while this group's gathers are in
+// flight.
+template <typename T, int K>
+__device__ __forceinline__ void walk_otf(const UG<T> &U, int64_t q, const GL T *__restrict__ x, Acc<T> (&acc)[K]) {
+    constexpr int OS = (K <= 2) ? 8 : 4;
+    SiddonState st;
+    st.lam = U.ot_lam[q];
+    st.ax = U.ot_ax[q];
+    st.ay = U.ot_ay[q];
+    st.stx = U.ot_stx[q];
+    st.sty = U.ot_sty[q];
+    st.hi = 0.0;
+    st.cx = U.ot_cx[q];
+    st.cy = U.ot_cy[q];
+    const int sgb = U.ot_sg[q];
+    st.sgx = (sgb & 1) ? 1 : -1;
+    st.sgy = (sgb & 2) ? 1 : -1;
+    const int n = U.ot_cnt[q];
+#if MI_BUF_GATHER
+    const __amdgpu_buffer_rsrc_t xr = make_rsrc((const void *)x, (uint32_t)(U.npix_t * K * (int64_t)sizeof(T)));
+#endif
+    auto gather = [&](uint32_t id, T (&o)[K]) {
+#if MI_BUF_GATHER
+        loadk_buf<T, K>(xr, id * (uint32_t)(K * sizeof(T)), o);
+#else
+        loadk<T, K>(x + (uint32_t)(id * K), o);
+#endif
+    };
+    int t = 0;
+    for (; t + OS <= n; t += OS) {
+        uint32_t id[OS];
+        T ln[OS], xv[OS][K];
+#pragma unroll
+        for (int k = 0; k < OS; ++k) {
+            int px, py;
+            double len;
+            siddon_step(st, px, py, len);
+            id[k] = (uint32_t)tile_index(px, py, U.tny);
+            ln[k] = (T)len;
+        }
+#pragma unroll
+        for (int k = 0; k < OS; ++k) gather(id[k], xv[k]);
+#pragma unroll
+        for (int k = 0; k < OS; ++k)
+#pragma unroll
+            for (int j = 0; j < K; ++j) acc[j].add_if(xv[k][j] * ln[k], ln[k] != T(0));
+    }
+    for (; t < n; ++t) {
+        int px, py;
+        double len;
+        siddon_step(st, px, py, len);
+        T xv[K];
+        gather((uint32_t)tile_index(px, py, U.tny), xv);
+        const T ln = (T)len;
+#pragma unroll
+        for (int j = 0; j < K; ++j) acc[j].add_if(xv[j] * ln, ln != T(0));
+    }
 }
 
 // Forward projection of subset s: a block is 4 slices (one per wave).
@@ -268,14 +430,30 @@ __device__ __forceinline__ void fp_unit(const EngArgs<T> &a, const UG<T> &U, int
     if (li < U.fs_slice0[s + 1] - sl0) {
         const int64_t sl = sl0 + li;
         Acc<T> acc[K];
-        walk_slice<T, K, MI_NT_FP>(U.fs_base, U.fs_T, U.fs_idx, U.fs_first, U.fs_d16, U.fs_len, sl, lane,
-                                   RESID ? U.u : U.d, acc);
         double tot[K];
+        int64_t row;
+        bool owner;
+        if (U.ot_cnt) {
+            // rays fastest: lane 8 p + a; the 8 segments of ray a combine
+            // across lanes a, a + 8, ..., a + 56
+            walk_otf<T, K>(U, sl * kWave + lane, RESID ? U.u : U.d, acc);
 #pragma unroll
-        for (int j = 0; j < K; ++j) tot[j] = acc[j].value();
-        group_sums<K>(tot, P);
-        const int64_t row = U.sub_row0[s] + li * C + lane / P;
-        if (lane % P == 0 && row < U.sub_row0[s + 1]) {
+            for (int j = 0; j < K; ++j) tot[j] = acc[j].value();
+            for (int o = kOtfRays; o < kWave; o <<= 1)
+#pragma unroll
+                for (int j = 0; j < K; ++j) tot[j] += __shfl_xor(tot[j], o, kWave);
+            row = U.sub_row0[s] + li * kOtfRays + lane;
+            owner = lane < kOtfRays;
+        } else {
+            walk_slice<T, K, MI_NT_FP>(U.fs_base, U.fs_T, U.fs_idx, U.fs_first, U.fs_d16, U.fs_len, sl, lane,
+                                       RESID ? U.u : U.d, U.npix_t, acc);
+#pragma unroll
+            for (int j = 0; j < K; ++j) tot[j] = acc[j].value();
+            group_sums<K>(tot, P);
+            row = U.sub_row0[s] + li * C + lane / P;
+            owner = lane % P == 0;
+        }
+        if (owner && row < U.sub_row0[s + 1]) {
             if (RESID) {
                 T gg[K], o[K];
                 loadk<T, K>(U.g + row * K, gg);
@@ -339,7 +517,7 @@ __device__ __forceinline__ void bp_unit(const EngArgs<T> &a, const UG<T> &U, int
         const int64_t sl = sl0 + li;
         Acc<T> acc[K];
         walk_slice<T, K, MI_NT_BP>(U.bs_base, U.bs_T, U.bs_idx, U.bs_first, U.bs_d16, U.bs_len, sl, lane, U.r,
-                                   acc);
+                                   U.nrows, acc);
         double tot[K];
 #pragma unroll
         for (int j = 0; j < K; ++j) tot[j] = acc[j].value();

@@ -44,6 +44,10 @@ struct UnitDesc {
     const uint32_t *bs_first;
     const int16_t *bs_d16;
     const T *bs_len;
+    // checkpointed forward walk (fan beam; null otherwise): per lane of a
+    // forward slice of 8 rays x 8 segments
+    const double *ot_lam, *ot_ax, *ot_ay, *ot_stx, *ot_sty;
+    const int32_t *ot_cx, *ot_cy, *ot_sg, *ot_cnt;
     const int64_t *sub_row0;               // S + 1 row offsets
     T *g, *r;                              // rows * K
     T *u, *v, *d, *w;                      // npix_t * K

@@ -61,10 +61,12 @@ void validate_geometry(const mi_geometry &g);
 // Sliced storage of one projection direction for the AT engine.  Items
 // (rays for the forward projector, pixels for the back projector) get P
 // lanes each, lane p taking entries p, p + P, ... of its item; a wave's 64
-// lanes are C = 64 / P consecutive items of one subset (a slice).  Entry t of
-// lane l of slice sl is at base[sl] + 64 t + l, so every step of a wave reads
-// one contiguous 256-byte row of each stream; lanes past their last entry
-// read padding (the lane's last pixel/row, length 0).
+// lanes are C = 64 / P consecutive items of one subset (a slice).  Steps go
+// in groups of four: entry t of lane l of slice sl is at
+// base[sl] + 256 (t / 4) + 4 l + t % 4 (sell_pos), so one 16-byte load per
+// lane (one contiguous kilobyte per wave) brings four steps of lengths.
+// Lanes past their last entry read padding (their last index, length 0); a
+// slice's step count T is a multiple of four.
 struct Sell {
     int P = 1, C = 64;
     std::vector<int64_t> slice0;     // S + 1: first slice of each subset
@@ -86,17 +88,42 @@ struct Sell {
     }
 };
 
+// Checkpointed forward walk (fan beam).  A forward slice is 8 rays x 8
+// lanes, rays fastest (lane = 8 p + a); lane p of ray a walks the ray's
+// entries [p Ls, min(L, (p + 1) Ls)), Ls = ceil(L / 8), starting from the
+// tracer's own state at entry p Ls, so it reproduces the stored CSR entries
+// bit for bit while reading ~56 bytes per lane instead of 6 per entry.
+constexpr int kOtfRays = 8, kOtfSeg = 8;
+struct Otf {
+    std::vector<int64_t> slice0;     // S + 1
+    DevBuf<int64_t> d_slice0;
+    DevBuf<double> lam, ax, ay, stx, sty;
+    DevBuf<int32_t> cx, cy, sg, cnt;  // sg: bit 0 x step > 0, bit 1 y step > 0
+    size_t bytes() const {
+        return d_slice0.bytes() + lam.bytes() + ax.bytes() + ay.bytes() + stx.bytes() + sty.bytes() + cx.bytes() +
+               cy.bytes() + sg.bytes() + cnt.bytes();
+    }
+};
+
+constexpr int kGroup = 4;
+__host__ __device__ __forceinline__ int64_t sell_pos(int64_t base, int t, int lane) {
+    return base + (int64_t)(t >> 2) * (kGroup * 64) + lane * kGroup + (t & 3);
+}
+
 // Images of the AT engine are stored in 4 x 8 pixel tiles (128 bytes of
-// FP32): tile (ti, tj) at ti * tny + tj, pixel (i & 3, j & 7) inside.
+// FP32): tile (ti, tj) at ti * tny + tj; inside a tile the two 4 x 4 halves
+// follow each other (pixel (i & 3, j & 7) at (j >> 2) * 16 + (i & 3) * 4 +
+// (j & 3)), so each 64-byte half line is a square.
 constexpr int kTileH = 4, kTileW = 8;
 __host__ __device__ __forceinline__ int64_t tile_index(int i, int j, int tny) {
-    return ((int64_t)(i >> 2) * tny + (j >> 3)) * (kTileH * kTileW) + ((i & 3) << 3) + (j & 7);
+    return ((int64_t)(i >> 2) * tny + (j >> 3)) * (kTileH * kTileW) + (((j >> 2) & 1) << 4) + ((i & 3) << 2) +
+           (j & 3);
 }
 __host__ __device__ __forceinline__ void tile_coords(int64_t q, int tny, int &i, int &j) {
     const int64_t t = q >> 5;
     const int w = (int)(q & 31);
-    i = (int)(t / tny) * kTileH + (w >> 3);
-    j = (int)(t % tny) * kTileW + (w & 7);
+    i = (int)(t / tny) * kTileH + ((w >> 2) & 3);
+    j = (int)(t % tny) * kTileW + ((w >> 4) << 2) + (w & 3);
 }
 
 // System matrix of one geometry, rows (rays) grouped by view subset.
@@ -132,6 +159,8 @@ struct Matrix {
     // AT engine: sliced forward (rays, tiled pixel indices) and back
     // (pixels in tile order, per subset) projectors; tiled image size.
     Sell fs, bs;
+    Otf otf;
+    bool has_otf = false;
     int tny = 0;
     int64_t npix_t = 0;
     bool has_sell = false;
@@ -146,7 +175,8 @@ void build_sell(Matrix &m, hipStream_t st, bool drop_csr = false);
 // Builds the matrix for (g, S) on the stream; want64 also keeps double
 // lengths.  Host-side GoldenC walk for MI_BEAM_GOLDENC, device tracers
 // otherwise.
-std::unique_ptr<Matrix> build_matrix(const mi_geometry &g, int S, bool want64, hipStream_t st);
+// otf: also record the checkpointed forward walk (fan beam only).
+std::unique_ptr<Matrix> build_matrix(const mi_geometry &g, int S, bool want64, hipStream_t st, bool otf = false);
 // Subset order: bit-reversed subset indices (values >= S dropped).
 std::vector<int> subset_order(int S);
 

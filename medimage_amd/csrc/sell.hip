@@ -69,7 +69,7 @@ __global__ void k_sell_T(It it, int64_t nslices, int32_t *T) {
         it.range(sl, item, b, e);
         t = max(t, (e - b + it.P - 1) / it.P);
     }
-    T[sl] = (int32_t)t;
+    T[sl] = (int32_t)((t + kGroup - 1) / kGroup * kGroup);
 }
 
 __global__ void k_times64(const int32_t *T, int64_t n, int64_t *out) {
@@ -93,9 +93,9 @@ __global__ void k_sell_fill(It it, int64_t nslices, const int64_t *base, const i
     // entries tP .. tP + P - 1 of the item across its P lanes).
     const int64_t cnt = (e - b > p) ? (e - b - p + it.P - 1) / it.P : 0;
     const int n = T[sl];
-    int64_t pos = base[sl] + lane;
     uint32_t last = 0;
-    for (int t = 0; t < n; ++t, pos += kWave) {
+    for (int t = 0; t < n; ++t) {
+        const int64_t pos = sell_pos(base[sl], t, lane);
         if (t < cnt) {
             const int64_t k = b + p + (int64_t)t * it.P;
             uint32_t v = src_idx[k];
@@ -121,11 +121,11 @@ __global__ void k_sell_delta(int64_t nslices, const int64_t *base, const int32_t
     if (sl >= nslices) return;
     const int lane = (int)(g % kWave);
     const int n = T[sl];
-    int64_t pos = base[sl] + lane;
-    int64_t prev = n > 0 ? (int64_t)idx[pos] : 0;
+    int64_t prev = n > 0 ? (int64_t)idx[sell_pos(base[sl], 0, lane)] : 0;
     first[g] = (uint32_t)prev;
     bool bad = false;
-    for (int t = 0; t < n; ++t, pos += kWave) {
+    for (int t = 0; t < n; ++t) {
+        const int64_t pos = sell_pos(base[sl], t, lane);
         const int64_t cur = idx[pos];
         const int64_t d = cur - prev;
         if (d > 32767 || d < -32768) bad = true;
@@ -222,14 +222,16 @@ void build_sell(Matrix &m, hipStream_t st, bool drop_csr) {
     m.npix_t = (int64_t)((g.nx + kTileH - 1) / kTileH) * kTileH * m.tny * kTileW;
     const bool f64 = m.has64;
     // forward
-    m.fs.P = choose_p(MI_SELL_PF, (double)m.nnz / std::max<int64_t>(m.nrows, 1));
-    m.fs.C = kWave / m.fs.P;
-    m.fs.slice0.assign(m.S + 1, 0);
-    for (int s = 0; s < m.S; ++s)
-        m.fs.slice0[s + 1] = m.fs.slice0[s] + (m.sub_row0[s + 1] - m.sub_row0[s] + m.fs.C - 1) / m.fs.C;
-    FItems fi{m.rowptr.p, m.d_sub_row0.p, nullptr, m.S, m.fs.P, m.fs.C};
-    fill_sell(m.fs, fi, m.idx.p, m.len32.p, f64 ? m.len64.p : nullptr, true, g.ny, m.tny, st);
-    compact_sell(m.fs, st);
+    if (!m.has_otf) {   // the checkpointed walk replaces the stored forward slices
+        m.fs.P = choose_p(MI_SELL_PF, (double)m.nnz / std::max<int64_t>(m.nrows, 1));
+        m.fs.C = kWave / m.fs.P;
+        m.fs.slice0.assign(m.S + 1, 0);
+        for (int s = 0; s < m.S; ++s)
+            m.fs.slice0[s + 1] = m.fs.slice0[s] + (m.sub_row0[s + 1] - m.sub_row0[s] + m.fs.C - 1) / m.fs.C;
+        FItems fi{m.rowptr.p, m.d_sub_row0.p, nullptr, m.S, m.fs.P, m.fs.C};
+        fill_sell(m.fs, fi, m.idx.p, m.len32.p, f64 ? m.len64.p : nullptr, true, g.ny, m.tny, st);
+        compact_sell(m.fs, st);
+    }
     if (drop_csr) {
         m.rowptr.reset();
         m.idx.reset();

@@ -313,6 +313,69 @@ void build_csc(Matrix &m, const int64_t *rowptr, const uint32_t *idx, const floa
     MI_HIP(hipStreamSynchronize(st));
 }
 
+// Checkpoints of the forward walk: one thread per row (subset order) walks
+// its ray once and records the tracer state at the first entry of each of
+// its kOtfSeg segments (lane 8 p + a of the row's slice, a = row % 8).
+__global__ void k_otf_fill(TraceArgs a, int64_t nrows, const int64_t *rowptr, const int64_t *sub_row0,
+                           const int64_t *slice0, int S, double *lam, double *ax, double *ay, double *stx,
+                           double *sty, int32_t *cx, int32_t *cy, int32_t *sg, int32_t *cnt) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    int s = 0;
+    while (s + 1 < S && sub_row0[s + 1] <= r) ++s;
+    const int64_t li = r - sub_row0[s];
+    const int64_t lane0 = (slice0[s] + li / kOtfRays) * kWave + li % kOtfRays;
+    const int64_t L = rowptr[r + 1] - rowptr[r];
+    const int64_t Ls = (L + kOtfSeg - 1) / kOtfSeg;
+    const int view = a.vlist[r / a.nbins];
+    const int bin = (int)(r % a.nbins);
+    double sx, sy, dx, dy;
+    fan_endpoints(a.fan, a.fcos[view], a.fsin[view], bin, sx, sy, dx, dy);
+    SiddonState st{};
+    if (L > 0) siddon_init(sx, sy, dx, dy, a.fan.nx, a.fan.ny, a.fan.vx, a.fan.vy, st);
+    int64_t k = 0;
+    for (int p = 0; p < kOtfSeg; ++p) {
+        const int64_t q = lane0 + (int64_t)p * kOtfRays;
+        const int64_t b = min(L, p * Ls), e = min(L, (p + 1) * Ls);
+        for (; k < b; ++k) {
+            int x, y;
+            double w;
+            siddon_step(st, x, y, w);
+        }
+        lam[q] = st.lam;
+        ax[q] = st.ax;
+        ay[q] = st.ay;
+        stx[q] = st.stx;
+        sty[q] = st.sty;
+        cx[q] = st.cx;
+        cy[q] = st.cy;
+        sg[q] = (st.sgx > 0 ? 1 : 0) | (st.sgy > 0 ? 2 : 0);
+        cnt[q] = (int32_t)(e - b);
+    }
+}
+
+void build_otf(Matrix &m, const TraceArgs &a, hipStream_t st) {
+    Otf &o = m.otf;
+    o.slice0.assign(m.S + 1, 0);
+    for (int s = 0; s < m.S; ++s)
+        o.slice0[s + 1] = o.slice0[s] + (m.sub_row0[s + 1] - m.sub_row0[s] + kOtfRays - 1) / kOtfRays;
+    const int64_t lanes = o.slice0.back() * kWave;
+    upload(o.d_slice0, o.slice0, st);
+    for (auto *b : {&o.lam, &o.ax, &o.ay, &o.stx, &o.sty}) b->alloc((size_t)lanes);
+    for (auto *b : {&o.cx, &o.cy, &o.sg, &o.cnt}) b->alloc((size_t)lanes);
+    // Lanes of missing rows (the last slice of a subset) walk nothing.
+    MI_HIP(hipMemsetAsync(o.cnt.p, 0, o.cnt.bytes(), st));
+    MI_HIP(hipMemsetAsync(o.sg.p, 0, o.sg.bytes(), st));
+    for (auto *b : {&o.lam, &o.ax, &o.ay, &o.stx, &o.sty}) MI_HIP(hipMemsetAsync(b->p, 0, b->bytes(), st));
+    for (auto *b : {&o.cx, &o.cy}) MI_HIP(hipMemsetAsync(b->p, 0, b->bytes(), st));
+    k_otf_fill<<<grid_of(m.nrows, 64), 64, 0, st>>>(a, m.nrows, m.rowptr.p, m.d_sub_row0.p, o.d_slice0.p, m.S,
+                                                   o.lam.p, o.ax.p, o.ay.p, o.stx.p, o.sty.p, o.cx.p, o.cy.p,
+                                                   o.sg.p, o.cnt.p);
+    MI_HIP(hipGetLastError());
+    MI_HIP(hipStreamSynchronize(st));
+    m.has_otf = true;
+}
+
 // Entries of each subset's rows, from the device row pointer.
 void subset_counts(Matrix &m, hipStream_t st) {
     std::vector<int64_t> at(m.S + 1);
@@ -339,11 +402,11 @@ std::vector<int> subset_order(int S) {
 }
 
 size_t Matrix::bytes() const {
-    return d_vlist.bytes() + d_sub_row0.bytes() + rowptr.bytes() + idx.bytes() + len32.bytes() + len64.bytes() + colptr.bytes() +
+    return otf.bytes() + d_vlist.bytes() + d_sub_row0.bytes() + rowptr.bytes() + idx.bytes() + len32.bytes() + len64.bytes() + colptr.bytes() +
            crow.bytes() + clen32.bytes() + clen64.bytes() + perm.bytes() + fs.bytes() + bs.bytes();
 }
 
-std::unique_ptr<Matrix> build_matrix(const mi_geometry &g, int S, bool want64, hipStream_t st) {
+std::unique_ptr<Matrix> build_matrix(const mi_geometry &g, int S, bool want64, hipStream_t st, bool otf) {
     validate_geometry(g);
     MI_REQUIRE(S >= 1 && S <= g.n_views, MI_E_ARG, "n_subsets must be in [1, n_views]");
     auto m = std::make_unique<Matrix>();
@@ -451,6 +514,7 @@ std::unique_ptr<Matrix> build_matrix(const mi_geometry &g, int S, bool want64, h
     MI_HIP(hipGetLastError());
     build_csc(*m, m->rowptr.p, m->idx.p, m->len32.p, want64 ? m->len64.p : nullptr, st);
     subset_counts(*m, st);
+    if (otf && g.beam == MI_BEAM_FAN_FLAT) build_otf(*m, a, st);
     return m;
 }
 

@@ -114,9 +114,16 @@ __device__ __forceinline__ void trace_parallel(float s_view, float c_view, int n
 // outside its arrays.
 __device__ __forceinline__ double sd_cross(double p, double s, double d, double L) { return L * (p - s) / (d - s); }
 
-template <typename Emit>
-__device__ __forceinline__ void trace_siddon2d(double sx, double sy, double dx, double dy,
-                                               int nx, int ny, double vx, double vy, Emit &&emit) {
+// Walk state of the parametric Siddon tracer: lam is the current crossing
+// parameter, ax / ay the next x / y crossings, (cx, cy) the current pixel.
+struct SiddonState {
+    double lam, ax, ay, stx, sty, hi;
+    int cx, cy, sgx, sgy;
+};
+
+// Entry computation of the walk; false when the ray misses the image.
+__device__ __forceinline__ bool siddon_init(double sx, double sy, double dx, double dy, int nx, int ny, double vx,
+                                            double vy, SiddonState &st) {
     double rx = dx - sx, ry = dy - sy;
     double L = sqrt(rx * rx + ry * ry);
     double lo = 0.0, hi = L, a0, a1, t, ax, ay, lam;
@@ -142,7 +149,7 @@ __device__ __forceinline__ void trace_siddon2d(double sx, double sy, double dx, 
     t = (a0 > a1) ? a0 : a1;
     hi = (hi < t) ? hi : t;
 
-    if (lo >= hi) return;
+    if (lo >= hi) return false;
     lam = lo;
     if (entry == 1) {
         cx = (sgx == 1) ? 0 : nx - 1;
@@ -157,16 +164,39 @@ __device__ __forceinline__ void trace_siddon2d(double sx, double sy, double dx, 
         double px = cx * vx;
         ax = (arx < 1.e-4) ? 1.e6 : sd_cross(px + (sgx > 0) * vx, sx, dx, L);
     }
+    st = SiddonState{lam, ax, ay, stx, sty, hi, cx, cy, sgx, sgy};
+    return true;
+}
 
-    while (lam < hi - 5.e-2) {
-        if (cx < 0 || cx >= nx || cy < 0 || cy >= ny) return;
-        if (ax <= ay) {
-            emit(cx * ny + cy, ax - lam);
-            lam = ax; ax += stx; cx += sgx;
-        } else {
-            emit(cx * ny + cy, ay - lam);
-            lam = ay; ay += sty; cy += sgy;
-        }
+// One crossing of the walk: the current pixel and its intersection length.
+// Same operations, in the same order, as the loop of the reference tracer.
+__device__ __forceinline__ void siddon_step(SiddonState &st, int &cx, int &cy, double &len) {
+    cx = st.cx;
+    cy = st.cy;
+    if (st.ax <= st.ay) {
+        len = st.ax - st.lam;
+        st.lam = st.ax;
+        st.ax += st.stx;
+        st.cx += st.sgx;
+    } else {
+        len = st.ay - st.lam;
+        st.lam = st.ay;
+        st.ay += st.sty;
+        st.cy += st.sgy;
+    }
+}
+
+template <typename Emit>
+__device__ __forceinline__ void trace_siddon2d(double sx, double sy, double dx, double dy,
+                                               int nx, int ny, double vx, double vy, Emit &&emit) {
+    SiddonState st;
+    if (!siddon_init(sx, sy, dx, dy, nx, ny, vx, vy, st)) return;
+    while (st.lam < st.hi - 5.e-2) {
+        if (st.cx < 0 || st.cx >= nx || st.cy < 0 || st.cy >= ny) return;
+        int cx, cy;
+        double len;
+        siddon_step(st, cx, cy, len);
+        emit(cx * ny + cy, len);
     }
 }
 
@@ -186,15 +216,27 @@ __device__ __forceinline__ void fan_rotate(double &x, double &y, double c, doubl
     x = r0; y = r1;
 }
 
-template <typename Emit>
-__device__ __forceinline__ void trace_fan(const FanGeom &g, double c, double s, int bin, Emit &&emit) {
+// Source and detector points of fan ray (view with cos c / sin s, bin),
+// shifted to the image-corner origin.
+__device__ __forceinline__ void fan_endpoints(const FanGeom &g, double c, double s, int bin, double &sx, double &sy,
+                                              double &dx, double &dy) {
     double x0 = 0.0, y0 = -g.sod;
     double x1 = ((double)(bin - g.nbins / 2) + 0.5 + g.det_offset) * g.bin_mm;
     double y1 = g.sdd - g.sod;
     double cx = g.nx * g.vx / 2.0, cy = g.ny * g.vy / 2.0;
     fan_rotate(x0, y0, c, s);
     fan_rotate(x1, y1, c, s);
-    trace_siddon2d(x0 + cx, y0 + cy, x1 + cx, y1 + cy, g.nx, g.ny, g.vx, g.vy, emit);
+    sx = x0 + cx;
+    sy = y0 + cy;
+    dx = x1 + cx;
+    dy = y1 + cy;
+}
+
+template <typename Emit>
+__device__ __forceinline__ void trace_fan(const FanGeom &g, double c, double s, int bin, Emit &&emit) {
+    double sx, sy, dx, dy;
+    fan_endpoints(g, c, s, bin, sx, sy, dx, dy);
+    trace_siddon2d(sx, sy, dx, dy, g.nx, g.ny, g.vx, g.vy, emit);
 }
 
 }  // namespace mi
