@@ -355,72 +355,105 @@ __device__ __forceinline__ void walk_slice(const GL int64_t *base, const GL int3
 }
 
 // One lane of the checkpointed forward walk: n Siddon steps from the stored
-// tracer state (the reference tracer's operations, tracers.hpp), each step's
-// pixel gathered from the tiled image x and its length (rounded to T as the
-// stored matrix rounds it) Kahan-summed.  Steps are taken OS at a time: the
-// tracer arithmetic of the next group runs while this group's gathers are in
-// flight.
+// tracer state, each step's pixel gathered from the tiled image x and its
+// length (rounded to T as the stored matrix rounds it) Kahan-summed in step
+// order.  The step is siddon_step (tracers.hpp) written with selects instead
+// of a branch: the same comparisons and the same additions on the taken side,
+// so lam / ax / ay and the lengths are bit-identical.  The pixel position is
+// carried as byte offsets into the tiled layout, X = 4 E cx and Y = E cy
+// (E bytes per pixel), so the tiled offset is
+//   E tile_index(cx, cy) = (cx >> 2) (32 tny - 16) E + X + (cy >> 2) 12 E + Y.
+// Every step of a lane is summed, zero-length crossings included (as the
+// oracle sums every matrix entry).  Steps go in groups of OS, software
+// pipelined: the next group is traced and its gathers issued before this
+// group's gathers are consumed.
 template <typename T, int K>
 __device__ __forceinline__ void walk_otf(const UG<T> &U, int64_t q, const GL T *__restrict__ x, Acc<T> (&acc)[K]) {
     constexpr int OS = (K <= 2) ? 8 : 4;
-    SiddonState st;
-    st.lam = U.ot_lam[q];
-    st.ax = U.ot_ax[q];
-    st.ay = U.ot_ay[q];
-    st.stx = U.ot_stx[q];
-    st.sty = U.ot_sty[q];
-    st.hi = 0.0;
-    st.cx = U.ot_cx[q];
-    st.cy = U.ot_cy[q];
+    constexpr uint32_t E = K * sizeof(T), LE = __builtin_ctz(E);
+    double lam = U.ot_lam[q], ax = U.ot_ax[q], ay = U.ot_ay[q];
+    const double stx = U.ot_stx[q], sty = U.ot_sty[q];
     const int sgb = U.ot_sg[q];
-    st.sgx = (sgb & 1) ? 1 : -1;
-    st.sgy = (sgb & 2) ? 1 : -1;
+    uint32_t X = (uint32_t)U.ot_cx[q] * (4 * E), Y = (uint32_t)U.ot_cy[q] * E;
+    const uint32_t dX = (sgb & 1) ? 4 * E : 0u - 4 * E, dY = (sgb & 2) ? E : 0u - E;
+    const uint32_t rowE = (uint32_t)(U.tny * (kTileH * kTileW) - 16) * E;
     const int n = U.ot_cnt[q];
 #if MI_BUF_GATHER
-    const __amdgpu_buffer_rsrc_t xr = make_rsrc((const void *)x, (uint32_t)(U.npix_t * K * (int64_t)sizeof(T)));
+    const __amdgpu_buffer_rsrc_t xr = make_rsrc((const void *)x, (uint32_t)(U.npix_t * (int64_t)E));
 #endif
-    auto gather = [&](uint32_t id, T (&o)[K]) {
+    auto step = [&](uint32_t &off, T &ln) {
+        const bool bx = ax <= ay;
+        const double nxt = bx ? ax : ay;
+        ln = (T)(nxt - lam);
+        lam = nxt;
+        off = (uint32_t)__umul24(X >> (4 + LE), rowE) + X + (uint32_t)__umul24(Y >> (2 + LE), 12 * E) + Y;
+        double nax = ax + stx, nay = ay + sty;
+        // both sums exist on every lane: keeps the selects below from being
+        // turned back into a divergent branch
+        asm volatile("" : "+v"(nax), "+v"(nay));
+        ax = bx ? nax : ax;
+        ay = bx ? ay : nay;
+        X += bx ? dX : 0u;
+        Y += bx ? 0u : dY;
+    };
+    auto gather = [&](uint32_t off, T (&o)[K]) {
 #if MI_BUF_GATHER
-        loadk_buf<T, K>(xr, id * (uint32_t)(K * sizeof(T)), o);
+        loadk_buf<T, K>(xr, off, o);
 #else
-        loadk<T, K>(x + (uint32_t)(id * K), o);
+        loadk<T, K>((const GL T *)((const GL char *)x + off), o);
 #endif
     };
-    int t = 0;
-    for (; t + OS <= n; t += OS) {
-        uint32_t id[OS];
-        T ln[OS], xv[OS][K];
+    auto stage = [&](T (&ln)[OS], T (&xv)[OS][K]) {
+        uint32_t off[OS];
 #pragma unroll
-        for (int k = 0; k < OS; ++k) {
-            int px, py;
-            double len;
-            siddon_step(st, px, py, len);
-            id[k] = (uint32_t)tile_index(px, py, U.tny);
-            ln[k] = (T)len;
-        }
+        for (int k = 0; k < OS; ++k) step(off[k], ln[k]);
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int k = 0; k < OS; ++k) gather(id[k], xv[k]);
+        for (int k = 0; k < OS; ++k) gather(off[k], xv[k]);
+        __builtin_amdgcn_sched_barrier(0);
+    };
+    auto consume = [&](const T (&ln)[OS], const T (&xv)[OS][K]) {
 #pragma unroll
         for (int k = 0; k < OS; ++k)
 #pragma unroll
-            for (int j = 0; j < K; ++j) acc[j].add_if(xv[k][j] * ln[k], ln[k] != T(0));
+            for (int j = 0; j < K; ++j) acc[j].add(xv[k][j] * ln[k]);
+    };
+    const int ng = n / OS;
+    T la[OS], lb[OS], xa[OS][K], xb[OS][K];
+    int g = 0;
+    if (ng > 0) {
+        stage(la, xa);
+        for (g = 1; g + 1 < ng; g += 2) {
+            stage(lb, xb);
+            consume(la, xa);
+            stage(la, xa);
+            consume(lb, xb);
+        }
+        if (g < ng) {
+            stage(lb, xb);
+            consume(la, xa);
+            consume(lb, xb);
+        } else {
+            consume(la, xa);
+        }
     }
-    for (; t < n; ++t) {
-        int px, py;
-        double len;
-        siddon_step(st, px, py, len);
-        T xv[K];
-        gather((uint32_t)tile_index(px, py, U.tny), xv);
-        const T ln = (T)len;
+    for (int t = ng * OS; t < n; ++t) {
+        uint32_t off;
+        T ln, xv[K];
+        step(off, ln);
+        gather(off, xv);
 #pragma unroll
-        for (int j = 0; j < K; ++j) acc[j].add_if(xv[j] * ln, ln != T(0));
+        for (int j = 0; j < K; ++j) acc[j].add(xv[j] * ln);
     }
 }
 
 // Forward projection of subset s: a block is 4 slices (one per wave).
 // RESID: r = g - A u (stored), partial sum r^2; else partial sum (A d)^2.
+static_assert(kOtfWaves == 4, "the forward walk's cross-wave combination is written for 4 waves");
+
 template <typename T, int K, bool RESID>
-__device__ __forceinline__ void fp_unit(const EngArgs<T> &a, const UG<T> &U, int s, int lb, double *lds) {
+__device__ __forceinline__ void fp_unit(const EngArgs<T> &a, const UG<T> &U, int s, int lb, double *lds,
+                                        double *ldr /* kBlock / kOtfPerWave * K */) {
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
     const int P = U.fs_P, C = kWave / P;
     const int64_t sl0 = U.fs_slice0[s], li = (int64_t)lb * (kBlock / kWave) + wv;
@@ -434,16 +467,27 @@ __device__ __forceinline__ void fp_unit(const EngArgs<T> &a, const UG<T> &U, int
         int64_t row;
         bool owner;
         if (U.ot_cnt) {
-            // rays fastest: lane 8 p + a; the 8 segments of ray a combine
-            // across lanes a, a + 8, ..., a + 56
+            // the block is kOtfRays rays x kOtfSeg segments (runtime.hpp);
+            // a ray's segment sums combine in a fixed tree: inside the wave
+            // (lanes a, a + kOtfRays, ...), then over the block's waves
             walk_otf<T, K>(U, sl * kWave + lane, RESID ? U.u : U.d, acc);
 #pragma unroll
             for (int j = 0; j < K; ++j) tot[j] = acc[j].value();
             for (int o = kOtfRays; o < kWave; o <<= 1)
 #pragma unroll
                 for (int j = 0; j < K; ++j) tot[j] += __shfl_xor(tot[j], o, kWave);
-            row = U.sub_row0[s] + li * kOtfRays + lane;
-            owner = lane < kOtfRays;
+            if (lane < kOtfRays)
+#pragma unroll
+                for (int j = 0; j < K; ++j) ldr[(wv * kOtfRays + lane) * K + j] = tot[j];
+            __syncthreads();
+            if (wv == 0 && lane < kOtfRays)
+#pragma unroll
+                for (int j = 0; j < K; ++j) {
+                    const double *q = ldr + lane * K + j;
+                    tot[j] = (q[0] + q[kOtfRays * K]) + (q[2 * kOtfRays * K] + q[3 * kOtfRays * K]);
+                }
+            row = U.sub_row0[s] + (int64_t)lb * kOtfRays + lane;
+            owner = wv == 0 && lane < kOtfRays;
         } else {
             walk_slice<T, K, MI_NT_FP>(U.fs_base, U.fs_T, U.fs_idx, U.fs_first, U.fs_d16, U.fs_len, sl, lane,
                                        RESID ? U.u : U.d, U.npix_t, acc);
@@ -805,13 +849,14 @@ __device__ __forceinline__ void edge_update_unit(const EngArgs<T> &a, const UG<T
 template <typename T, int KMAX, bool RESID>
 __global__ __launch_bounds__(kBlock) void k_eng_fp(EngArgs<T> a, BlockMap m, int s) {
     __shared__ double lds[kBlock / kWave * kMaxK];
+    __shared__ double ldr[kBlock / kOtfPerWave * KMAX];
     const int b = (int)xcd_block();
     if (b >= m.start[m.nunits]) return;
     const int ui = find_unit(m, b);
     const UG<T> U(a.units[ui]);
     if (unit_idle(a, U)) return;
     const int lb = b - m.start[ui];
-    MI_KSWITCH(KMAX, (fp_unit<T, K, RESID>(a, U, s, lb, lds)))
+    MI_KSWITCH(KMAX, (fp_unit<T, K, RESID>(a, U, s, lb, lds, ldr)))
 }
 
 template <typename T, int KMAX>
@@ -832,6 +877,7 @@ __global__ __launch_bounds__(kBlock) void k_eng_bp(EngArgs<T> a, BlockMap m, int
 template <typename T, int KMAX>
 __global__ __launch_bounds__(kBlock) void k_eng_fpsq_nv(EngArgs<T> a, BlockMap rows, BlockMap pix, int s) {
     __shared__ double lds[kBlock / kWave * kMaxK];
+    __shared__ double ldr[kBlock / kOtfPerWave * KMAX];
     const int nrb = rows.start[rows.nunits], npb = pix.start[pix.nunits];
     const int r8 = (nrb + 7) / 8, p8 = (npb + 7) / 8;
     const int x = blockIdx.x & 7, t = blockIdx.x >> 3;
@@ -842,7 +888,7 @@ __global__ __launch_bounds__(kBlock) void k_eng_fpsq_nv(EngArgs<T> a, BlockMap r
         const UG<T> U(a.units[ui]);
         if (unit_idle(a, U)) return;
         const int lb = b - rows.start[ui];
-        MI_KSWITCH(KMAX, (fp_unit<T, K, false>(a, U, s, lb, lds)))
+        MI_KSWITCH(KMAX, (fp_unit<T, K, false>(a, U, s, lb, lds, ldr)))
     } else {
         const int b = x * p8 + (t - r8);
         if (b >= npb) return;

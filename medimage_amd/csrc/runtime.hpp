@@ -88,12 +88,20 @@ struct Sell {
     }
 };
 
-// Checkpointed forward walk (fan beam).  A forward slice is 8 rays x 8
-// lanes, rays fastest (lane = 8 p + a); lane p of ray a walks the ray's
-// entries [p Ls, min(L, (p + 1) Ls)), Ls = ceil(L / 8), starting from the
-// tracer's own state at entry p Ls, so it reproduces the stored CSR entries
-// bit for bit while reading ~56 bytes per lane instead of 6 per entry.
-constexpr int kOtfRays = 8, kOtfSeg = 8;
+// Checkpointed forward walk (fan beam).  A forward block (kBlock lanes)
+// covers kOtfRays consecutive rays of a subset, each cut into kOtfSeg
+// segments of equal entry count: segment p of ray a is lane kOtfRays h + a
+// of the block's wave w, p = kOtfPerWave w + h, so a wave's lanes at one step
+// sit on kOtfRays neighbouring rays (neighbouring pixels).  The lane walks the
+// ray's entries [p Ls, min(L, (p + 1) Ls)), Ls = ceil(L / kOtfSeg), from the
+// tracer's own state at entry p Ls, reproducing the stored CSR entries bit for
+// bit while reading ~56 bytes per lane instead of 6 per entry.
+#ifndef MI_OTF_RAYS
+#define MI_OTF_RAYS 32
+#endif
+constexpr int kOtfRays = MI_OTF_RAYS, kOtfSeg = kBlock / kOtfRays, kOtfPerWave = kWave / kOtfRays,
+              kOtfWaves = kBlock / kWave;
+static_assert(kOtfRays >= 16 && kOtfRays <= kWave && kOtfSeg * kOtfRays == kBlock, "forward block shape");
 struct Otf {
     std::vector<int64_t> slice0;     // S + 1
     DevBuf<int64_t> d_slice0;

@@ -315,7 +315,8 @@ void build_csc(Matrix &m, const int64_t *rowptr, const uint32_t *idx, const floa
 
 // Checkpoints of the forward walk: one thread per row (subset order) walks
 // its ray once and records the tracer state at the first entry of each of
-// its kOtfSeg segments (lane 8 p + a of the row's slice, a = row % 8).
+// its kOtfSeg segments (segment p of the row's block: wave p / kOtfPerWave,
+// lane kOtfRays (p % kOtfPerWave) + row % kOtfRays).
 __global__ void k_otf_fill(TraceArgs a, int64_t nrows, const int64_t *rowptr, const int64_t *sub_row0,
                            const int64_t *slice0, int S, double *lam, double *ax, double *ay, double *stx,
                            double *sty, int32_t *cx, int32_t *cy, int32_t *sg, int32_t *cnt) {
@@ -324,7 +325,7 @@ __global__ void k_otf_fill(TraceArgs a, int64_t nrows, const int64_t *rowptr, co
     int s = 0;
     while (s + 1 < S && sub_row0[s + 1] <= r) ++s;
     const int64_t li = r - sub_row0[s];
-    const int64_t lane0 = (slice0[s] + li / kOtfRays) * kWave + li % kOtfRays;
+    const int64_t lane0 = (slice0[s] + li / kOtfRays * kOtfWaves) * kWave + li % kOtfRays;
     const int64_t L = rowptr[r + 1] - rowptr[r];
     const int64_t Ls = (L + kOtfSeg - 1) / kOtfSeg;
     const int view = a.vlist[r / a.nbins];
@@ -335,7 +336,7 @@ __global__ void k_otf_fill(TraceArgs a, int64_t nrows, const int64_t *rowptr, co
     if (L > 0) siddon_init(sx, sy, dx, dy, a.fan.nx, a.fan.ny, a.fan.vx, a.fan.vy, st);
     int64_t k = 0;
     for (int p = 0; p < kOtfSeg; ++p) {
-        const int64_t q = lane0 + (int64_t)p * kOtfRays;
+        const int64_t q = lane0 + (int64_t)(p / kOtfPerWave) * kWave + (p % kOtfPerWave) * kOtfRays;
         const int64_t b = min(L, p * Ls), e = min(L, (p + 1) * Ls);
         for (; k < b; ++k) {
             int x, y;
@@ -358,7 +359,7 @@ void build_otf(Matrix &m, const TraceArgs &a, hipStream_t st) {
     Otf &o = m.otf;
     o.slice0.assign(m.S + 1, 0);
     for (int s = 0; s < m.S; ++s)
-        o.slice0[s + 1] = o.slice0[s] + (m.sub_row0[s + 1] - m.sub_row0[s] + kOtfRays - 1) / kOtfRays;
+        o.slice0[s + 1] = o.slice0[s] + (m.sub_row0[s + 1] - m.sub_row0[s] + kOtfRays - 1) / kOtfRays * kOtfWaves;
     const int64_t lanes = o.slice0.back() * kWave;
     upload(o.d_slice0, o.slice0, st);
     for (auto *b : {&o.lam, &o.ax, &o.ay, &o.stx, &o.sty}) b->alloc((size_t)lanes);
