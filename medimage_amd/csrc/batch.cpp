@@ -364,6 +364,7 @@ void Batch<T>::rebuild() {
     std::vector<UnitDesc<T>> desc(live.size());
     int64_t off = 0;
     kmax = 1;
+    all_walk = true;
     edge_iters = 0;
     any_median = false;
     rows.nunits = bp.nunits = pix.nunits = (int)live.size();
@@ -423,6 +424,7 @@ void Batch<T>::rebuild() {
         bp.start[i + 1] = bp.start[i] + (int)bp_blocks(m);
         pix.start[i + 1] = pix.start[i] + (int)px_blocks(m);
         if (U.K > 1) kmax = kMaxK;
+        all_walk = all_walk && m.has_otf;
         for (int k = 0; k < U.K; ++k) {
             if (U.jobs[k] < 0) continue;
             const JobParams &P = h_jp[D.job0 + k];
@@ -465,7 +467,7 @@ void Batch<T>::record_pass(bool timed) {
             LaunchTimer lt(ctx, MI_K_FP, sum([&](const Unit<T> &U) {
                 return nnz_s(U) * (4 + sz) + U.m->npix * sz * U.K + 2 * nr_s(U) * sz * U.K;
             }));
-            eng_fp_resid<T>(st, args, rows, s, kmax);
+            eng_fp_resid<T>(st, args, rows, s, kmax, all_walk);
         }
         {
             LaunchTimer lt(ctx, MI_K_BP, sum([&](const Unit<T> &U) {
@@ -475,12 +477,13 @@ void Batch<T>::record_pass(bool timed) {
         }
         {
             LaunchTimer lt(ctx, MI_K_FPSQ, sum([&](const Unit<T> &U) {
-                return nnz_s(U) * (4 + sz) + 3.0 * U.m->npix * sz * U.K;
+                return nnz_s(U) * (4 + sz) + U.m->npix * sz * U.K;
             }));
-            eng_fp_sq_nv<T>(st, args, rows, pix, s, kmax);
+            eng_fp_sq<T>(st, args, rows, s, kmax, all_walk);
         }
         {
-            LaunchTimer lt(ctx, MI_K_STEP, sum([&](const Unit<T> &U) { return 3.0 * U.m->npix * sz * U.K; }));
+            LaunchTimer lt(ctx, MI_K_STEP, sum([&](const Unit<T> &U) { return 5.0 * U.m->npix * sz * U.K; }));
+            eng_nv<T>(st, args, pix, kmax);
             eng_finalize<T>(st, args, EF_STEP, s);
             eng_axpy<T>(st, args, pix, kmax);
         }

@@ -78,6 +78,7 @@ struct Batch {
     BlockMap rows{}, bp{}, pix{};
     EngArgs<T> args{};
     int kmax = 1;
+    bool all_walk = true;   // every unit's forward projector is the checkpointed walk
     int edge_iters = 0;            // max iter_edge over live jobs
     bool any_median = false;
     bool dirty = true;

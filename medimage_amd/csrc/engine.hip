@@ -437,13 +437,22 @@ __device__ __forceinline__ void walk_otf(const UG<T> &U, int64_t q, const GL T *
             consume(la, xa);
         }
     }
-    for (int t = ng * OS; t < n; ++t) {
-        uint32_t off;
-        T ln, xv[K];
-        step(off, ln);
-        gather(off, xv);
+    // the last n % OS steps: traced and gathered together, one round trip
+    const int rem = n - ng * OS;
+    if (rem > 0) {
+        uint32_t off[OS - 1];
+        T ln[OS - 1], xv[OS - 1][K];
 #pragma unroll
-        for (int j = 0; j < K; ++j) acc[j].add(xv[j] * ln);
+        for (int k = 0; k < OS - 1; ++k)
+            if (k < rem) step(off[k], ln[k]);
+#pragma unroll
+        for (int k = 0; k < OS - 1; ++k)
+            if (k < rem) gather(off[k], xv[k]);
+#pragma unroll
+        for (int k = 0; k < OS - 1; ++k)
+            if (k < rem)
+#pragma unroll
+                for (int j = 0; j < K; ++j) acc[j].add(xv[k][j] * ln[k]);
     }
 }
 
@@ -451,7 +460,9 @@ __device__ __forceinline__ void walk_otf(const UG<T> &U, int64_t q, const GL T *
 // RESID: r = g - A u (stored), partial sum r^2; else partial sum (A d)^2.
 static_assert(kOtfWaves == 4, "the forward walk's cross-wave combination is written for 4 waves");
 
-template <typename T, int K, bool RESID>
+// FM: 1 when every unit of the launch has the checkpointed walk (the sliced
+// branch is not compiled), 3 when either form may occur.
+template <typename T, int K, bool RESID, int FM>
 __device__ __forceinline__ void fp_unit(const EngArgs<T> &a, const UG<T> &U, int s, int lb, double *lds,
                                         double *ldr /* kBlock / kOtfPerWave * K */) {
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
@@ -466,7 +477,7 @@ __device__ __forceinline__ void fp_unit(const EngArgs<T> &a, const UG<T> &U, int
         double tot[K];
         int64_t row;
         bool owner;
-        if (U.ot_cnt) {
+        if (FM == 1 || U.ot_cnt) {
             // the block is kOtfRays rays x kOtfSeg segments (runtime.hpp);
             // a ray's segment sums combine in a fixed tree: inside the wave
             // (lanes a, a + kOtfRays, ...), then over the block's waves
@@ -488,7 +499,7 @@ __device__ __forceinline__ void fp_unit(const EngArgs<T> &a, const UG<T> &U, int
                 }
             row = U.sub_row0[s] + (int64_t)lb * kOtfRays + lane;
             owner = wv == 0 && lane < kOtfRays;
-        } else {
+        } else if constexpr (FM != 1) {
             walk_slice<T, K, MI_NT_FP>(U.fs_base, U.fs_T, U.fs_idx, U.fs_first, U.fs_d16, U.fs_len, sl, lane,
                                        RESID ? U.u : U.d, U.npix_t, acc);
 #pragma unroll
@@ -846,7 +857,7 @@ __device__ __forceinline__ void edge_update_unit(const EngArgs<T> &a, const UG<T
         }                                       \
     }
 
-template <typename T, int KMAX, bool RESID>
+template <typename T, int KMAX, bool RESID, int FM>
 __global__ __launch_bounds__(kBlock) void k_eng_fp(EngArgs<T> a, BlockMap m, int s) {
     __shared__ double lds[kBlock / kWave * kMaxK];
     __shared__ double ldr[kBlock / kOtfPerWave * KMAX];
@@ -856,7 +867,7 @@ __global__ __launch_bounds__(kBlock) void k_eng_fp(EngArgs<T> a, BlockMap m, int
     const UG<T> U(a.units[ui]);
     if (unit_idle(a, U)) return;
     const int lb = b - m.start[ui];
-    MI_KSWITCH(KMAX, (fp_unit<T, K, RESID>(a, U, s, lb, lds, ldr)))
+    MI_KSWITCH(KMAX, (fp_unit<T, K, RESID, FM>(a, U, s, lb, lds, ldr)))
 }
 
 template <typename T, int KMAX>
@@ -871,36 +882,7 @@ __global__ __launch_bounds__(kBlock) void k_eng_bp(EngArgs<T> a, BlockMap m, int
     MI_KSWITCH(KMAX, (bp_unit<T, K>(a, U, s, lb, lds)))
 }
 
-// Two block maps in one launch: each XCD runs a contiguous 1/8 of the
-// rows part and a contiguous 1/8 of the pixel part (both parts padded to a
-// multiple of 8 blocks), so neither part is left to a few XCDs.
-template <typename T, int KMAX>
-__global__ __launch_bounds__(kBlock) void k_eng_fpsq_nv(EngArgs<T> a, BlockMap rows, BlockMap pix, int s) {
-    __shared__ double lds[kBlock / kWave * kMaxK];
-    __shared__ double ldr[kBlock / kOtfPerWave * KMAX];
-    const int nrb = rows.start[rows.nunits], npb = pix.start[pix.nunits];
-    const int r8 = (nrb + 7) / 8, p8 = (npb + 7) / 8;
-    const int x = blockIdx.x & 7, t = blockIdx.x >> 3;
-    if (t < r8) {
-        const int b = x * r8 + t;
-        if (b >= nrb) return;
-        const int ui = find_unit(rows, b);
-        const UG<T> U(a.units[ui]);
-        if (unit_idle(a, U)) return;
-        const int lb = b - rows.start[ui];
-        MI_KSWITCH(KMAX, (fp_unit<T, K, false>(a, U, s, lb, lds, ldr)))
-    } else {
-        const int b = x * p8 + (t - r8);
-        if (b >= npb) return;
-        const int ui = find_unit(pix, b);
-        const UG<T> U(a.units[ui]);
-        if (unit_idle(a, U)) return;
-        const int lb = b - pix.start[ui];
-        MI_KSWITCH(KMAX, (nv_unit<T, K>(a, U, lb, lds)))
-    }
-}
-
-enum PixOp { PX_AXPY = 0, PX_MEDIAN, PX_COPY_WU, PX_EDGE_GRAD, PX_EDGE_NABLA, PX_EDGE_UPDATE };
+enum PixOp { PX_AXPY = 0, PX_MEDIAN, PX_COPY_WU, PX_EDGE_GRAD, PX_EDGE_NABLA, PX_EDGE_UPDATE, PX_NV };
 
 template <typename T, int KMAX, int OP>
 __global__ __launch_bounds__(kBlock) void k_eng_pix(EngArgs<T> a, BlockMap m, int it) {
@@ -914,6 +896,7 @@ __global__ __launch_bounds__(kBlock) void k_eng_pix(EngArgs<T> a, BlockMap m, in
     if constexpr (OP == PX_AXPY) { MI_KSWITCH(KMAX, (axpy_unit<T, K>(a, U, lb))) }
     else if constexpr (OP == PX_MEDIAN) { MI_KSWITCH(KMAX, (median_unit<T, K>(a, U, lb))) }
     else if constexpr (OP == PX_COPY_WU) { MI_KSWITCH(KMAX, (copy_wu_unit<T, K>(U, lb))) }
+    else if constexpr (OP == PX_NV) { MI_KSWITCH(KMAX, (nv_unit<T, K>(a, U, lb, lds))) }
     else if constexpr (OP == PX_EDGE_GRAD) { MI_KSWITCH(KMAX, (edge_grad_unit<T, K>(a, U, lb, it, lds))) }
     else if constexpr (OP == PX_EDGE_NABLA) { MI_KSWITCH(KMAX, (edge_nabla_unit<T, K>(a, U, lb, it, lds))) }
     else { MI_KSWITCH(KMAX, (edge_update_unit<T, K>(a, U, lb, it, lds))) }
@@ -1063,10 +1046,15 @@ unsigned grid_of(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 // ---------------------------------------------------------------------------
 
 template <typename T>
-void eng_fp_resid(hipStream_t st, const EngArgs<T> &a, const BlockMap &m, int s, int kmax) {
+void eng_fp_resid(hipStream_t st, const EngArgs<T> &a, const BlockMap &m, int s, int kmax, bool all_walk) {
     const unsigned grid = grid8(m.start[m.nunits]);
-    if (kmax == 1) k_eng_fp<T, 1, true><<<grid, kBlock, 0, st>>>(a, m, s);
-    else k_eng_fp<T, kMaxK, true><<<grid, kBlock, 0, st>>>(a, m, s);
+    if (all_walk) {
+        if (kmax == 1) k_eng_fp<T, 1, true, 1><<<grid, kBlock, 0, st>>>(a, m, s);
+        else k_eng_fp<T, kMaxK, true, 1><<<grid, kBlock, 0, st>>>(a, m, s);
+    } else {
+        if (kmax == 1) k_eng_fp<T, 1, true, 3><<<grid, kBlock, 0, st>>>(a, m, s);
+        else k_eng_fp<T, kMaxK, true, 3><<<grid, kBlock, 0, st>>>(a, m, s);
+    }
     MI_HIP(hipGetLastError());
 }
 
@@ -1079,10 +1067,15 @@ void eng_bp_grad(hipStream_t st, const EngArgs<T> &a, const BlockMap &m, int s, 
 }
 
 template <typename T>
-void eng_fp_sq_nv(hipStream_t st, const EngArgs<T> &a, const BlockMap &rows, const BlockMap &pix, int s, int kmax) {
-    const unsigned grid = 8 * (unsigned)((rows.start[rows.nunits] + 7) / 8 + (pix.start[pix.nunits] + 7) / 8);
-    if (kmax == 1) k_eng_fpsq_nv<T, 1><<<grid, kBlock, 0, st>>>(a, rows, pix, s);
-    else k_eng_fpsq_nv<T, kMaxK><<<grid, kBlock, 0, st>>>(a, rows, pix, s);
+void eng_fp_sq(hipStream_t st, const EngArgs<T> &a, const BlockMap &rows, int s, int kmax, bool all_walk) {
+    const unsigned grid = grid8(rows.start[rows.nunits]);
+    if (all_walk) {
+        if (kmax == 1) k_eng_fp<T, 1, false, 1><<<grid, kBlock, 0, st>>>(a, rows, s);
+        else k_eng_fp<T, kMaxK, false, 1><<<grid, kBlock, 0, st>>>(a, rows, s);
+    } else {
+        if (kmax == 1) k_eng_fp<T, 1, false, 3><<<grid, kBlock, 0, st>>>(a, rows, s);
+        else k_eng_fp<T, kMaxK, false, 3><<<grid, kBlock, 0, st>>>(a, rows, s);
+    }
     MI_HIP(hipGetLastError());
 }
 
@@ -1097,6 +1090,10 @@ static void launch_pix(hipStream_t st, const EngArgs<T> &a, const BlockMap &m, i
 template <typename T>
 void eng_axpy(hipStream_t st, const EngArgs<T> &a, const BlockMap &pix, int kmax) {
     launch_pix<T, PX_AXPY>(st, a, pix, 0, kmax);
+}
+template <typename T>
+void eng_nv(hipStream_t st, const EngArgs<T> &a, const BlockMap &pix, int kmax) {
+    launch_pix<T, PX_NV>(st, a, pix, 0, kmax);
 }
 
 template <typename T>
@@ -1157,10 +1154,10 @@ void eng_copy(hipStream_t st, void *dst, const void *src, int64_t bytes) {
 }
 
 #define MI_ENG_INST(T)                                                                                     \
-    template void eng_fp_resid<T>(hipStream_t, const EngArgs<T> &, const BlockMap &, int, int);             \
+    template void eng_fp_resid<T>(hipStream_t, const EngArgs<T> &, const BlockMap &, int, int, bool);       \
     template void eng_bp_grad<T>(hipStream_t, const EngArgs<T> &, const BlockMap &, int, int);              \
-    template void eng_fp_sq_nv<T>(hipStream_t, const EngArgs<T> &, const BlockMap &, const BlockMap &, int,  \
-                                  int);                                                                     \
+    template void eng_fp_sq<T>(hipStream_t, const EngArgs<T> &, const BlockMap &, int, int, bool);         \
+    template void eng_nv<T>(hipStream_t, const EngArgs<T> &, const BlockMap &, int);                        \
     template void eng_axpy<T>(hipStream_t, const EngArgs<T> &, const BlockMap &, int);                      \
     template void eng_median<T>(hipStream_t, const EngArgs<T> &, const BlockMap &, int);                    \
     template void eng_edge_iter<T>(hipStream_t, const EngArgs<T> &, const BlockMap &, int, int);            \

@@ -76,14 +76,19 @@ struct EngArgs {
 };
 
 // Launchers (all on `st`).  `s` is the subset step.  kmax: 1 when every unit
-// has K = 1 (selects code compiled for K = 1 only), else 8.
+// has K = 1 (selects code compiled for K = 1 only), else 8.  all_walk: every
+// unit's forward projector is the checkpointed walk (selects code without
+// the sliced forward branch).
 template <typename T>
-void eng_fp_resid(hipStream_t st, const EngArgs<T> &a, const BlockMap &m, int s, int kmax);
+void eng_fp_resid(hipStream_t st, const EngArgs<T> &a, const BlockMap &m, int s, int kmax, bool all_walk);
 template <typename T>
 void eng_bp_grad(hipStream_t st, const EngArgs<T> &a, const BlockMap &m, int s, int kmax);
-// ||A_s d||^2 (rows map) and nV(d) (pixel map) in one launch.
+// ||A_s d||^2 (rows map; the walk of the residual launch without the
+// residual) and nV(d) (pixel map), as two launches.
 template <typename T>
-void eng_fp_sq_nv(hipStream_t st, const EngArgs<T> &a, const BlockMap &rows, const BlockMap &pix, int s, int kmax);
+void eng_fp_sq(hipStream_t st, const EngArgs<T> &a, const BlockMap &rows, int s, int kmax, bool all_walk);
+template <typename T>
+void eng_nv(hipStream_t st, const EngArgs<T> &a, const BlockMap &pix, int kmax);
 template <typename T>
 void eng_axpy(hipStream_t st, const EngArgs<T> &a, const BlockMap &pix, int kmax);
 template <typename T>

@@ -170,6 +170,20 @@ def test_batch_fp32_within_tolerance_of_oracle(ctx):
         assert np.linalg.norm(v - ov) / np.linalg.norm(ov) < 1e-5
 
 
+def test_batch_walk_and_sliced_units_mixed_bitwise(ctx):
+    """Fan-beam jobs (checkpointed forward walk) and a parallel-beam job
+    (stored sliced forward matrix) in one batch run the kernels that hold
+    both forward forms; every job still ends with the bits it has alone."""
+    jobs = _jobs(ctx)[:4]
+    gp = mi.parallel_geometry(64, 60, 96, flavour=0)
+    _, sp = configs.make_case(ctx, gp, 3e4, seed=123)
+    jobs.append((gp, sp.astype(np.float32), mi.at_params(n_subsets=4, max_iters=4)))
+    mixed = _run_batch(ctx, jobs)
+    alone = _alone(ctx, jobs)
+    for i in range(len(jobs)):
+        assert _same(mixed[i][0], alone[i][0]) and _same(mixed[i][1], alone[i][1]), f"job {i}"
+
+
 def test_batch_release_and_errors(ctx):
     g = GEOMS["a"]
     _, sino = configs.make_case(ctx, g, 3e4, seed=9)
