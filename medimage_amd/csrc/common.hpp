@@ -48,6 +48,21 @@ template <> struct Acc<float> {
         s = take ? t : s;
         c = take ? cn : c;
     }
+    // add(x * l) with the product folded into the compensation step
+    // (y = x l - c rounded once).
+    __device__ __forceinline__ void add_prod(float x, float l) {
+        const float y = __builtin_fmaf(x, l, -c);
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    __device__ __forceinline__ void add_prod_if(float x, float l, bool take) {
+        const float y = __builtin_fmaf(x, l, -c);
+        const float t = s + y;
+        const float cn = (t - s) - y;
+        s = take ? t : s;
+        c = take ? cn : c;
+    }
     __device__ __forceinline__ double value() const { return (double)s - (double)c; }
 };
 
@@ -55,6 +70,8 @@ template <> struct Acc<double> {
     double s = 0.0;
     __device__ __forceinline__ void add(double x) { s += x; }
     __device__ __forceinline__ void add_if(double x, bool take) { s = take ? s + x : s; }
+    __device__ __forceinline__ void add_prod(double x, double l) { s += x * l; }
+    __device__ __forceinline__ void add_prod_if(double x, double l, bool take) { s = take ? s + x * l : s; }
     __device__ __forceinline__ double value() const { return s; }
 };
 

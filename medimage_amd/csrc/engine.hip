@@ -312,7 +312,7 @@ __device__ __forceinline__ void walk(const GL uint32_t *__restrict__ ip, const G
 #pragma unroll
             for (int q = 0; q < kGroup; ++q)
 #pragma unroll
-                for (int j = 0; j < K; ++j) acc[j].add_if(xv[r][q][j] * ln[r][q], ln[r][q] != T(0));
+                for (int j = 0; j < K; ++j) acc[j].add_prod_if(xv[r][q][j], ln[r][q], ln[r][q] != T(0));
     }
     for (; g < ng; ++g) {
         const int64_t o = (int64_t)g * kStride;
@@ -337,7 +337,7 @@ __device__ __forceinline__ void walk(const GL uint32_t *__restrict__ ip, const G
 #pragma unroll
         for (int q = 0; q < kGroup; ++q)
 #pragma unroll
-            for (int j = 0; j < K; ++j) acc[j].add_if(xv[q][j] * ln[q], ln[q] != T(0));
+            for (int j = 0; j < K; ++j) acc[j].add_prod_if(xv[q][j], ln[q], ln[q] != T(0));
     }
 }
 
@@ -369,7 +369,10 @@ __device__ __forceinline__ void walk_slice(const GL int64_t *base, const GL int3
 // group's gathers are consumed.
 template <typename T, int K>
 __device__ __forceinline__ void walk_otf(const UG<T> &U, int64_t q, const GL T *__restrict__ x, Acc<T> (&acc)[K]) {
-    constexpr int OS = (K <= 2) ? 8 : 4;
+#ifndef MI_OTF_OS8
+#define MI_OTF_OS8 2
+#endif
+    constexpr int OS = (K <= 2) ? 8 : (K == 4 ? 4 : MI_OTF_OS8);
     constexpr uint32_t E = K * sizeof(T), LE = __builtin_ctz(E);
     double lam = U.ot_lam[q], ax = U.ot_ax[q], ay = U.ot_ay[q];
     const double stx = U.ot_stx[q], sty = U.ot_sty[q];
@@ -416,7 +419,7 @@ __device__ __forceinline__ void walk_otf(const UG<T> &U, int64_t q, const GL T *
 #pragma unroll
         for (int k = 0; k < OS; ++k)
 #pragma unroll
-            for (int j = 0; j < K; ++j) acc[j].add(xv[k][j] * ln[k]);
+            for (int j = 0; j < K; ++j) acc[j].add_prod(xv[k][j], ln[k]);
     };
     const int ng = n / OS;
     T la[OS], lb[OS], xa[OS][K], xb[OS][K];
@@ -452,7 +455,7 @@ __device__ __forceinline__ void walk_otf(const UG<T> &U, int64_t q, const GL T *
         for (int k = 0; k < OS - 1; ++k)
             if (k < rem)
 #pragma unroll
-                for (int j = 0; j < K; ++j) acc[j].add(xv[k][j] * ln[k]);
+                for (int j = 0; j < K; ++j) acc[j].add_prod(xv[k][j], ln[k]);
     }
 }
 
