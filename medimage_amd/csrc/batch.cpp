@@ -143,7 +143,7 @@ Unit<T> *Batch<T>::place(BatchJob<T> &j, int group) {
     for (int i = 0; i < kMaxUnits; ++i)
         if (!units[i]) { uid = i; break; }
     MI_REQUIRE(uid >= 0, MI_E_LIMIT, "batch holds the maximum number of units");
-    const int K = pow2ceil(std::min(kMaxK, std::max(std::max(group, min_k), 1)));
+    const int K = std::min(k_cap(*j.m), pow2ceil(std::min(kMaxK, std::max(std::max(group, min_k), 1))));
     const Matrix &m = *j.m;
     const int64_t need = unit_state_bytes<T>(m, K);
     if (free_device_bytes() - need < reserve_bytes) evict_idle(ctx, need + reserve_bytes - free_device_bytes());
@@ -282,9 +282,10 @@ void Batch<T>::compact() {
         }
         if (n == 0) continue;
         std::vector<int> target;
-        for (int left = n; left > 0; left -= kMaxK) target.push_back(pow2ceil(std::min(left, kMaxK)));
+        const int kc = k_cap(*kv.first);
+        for (int left = n; left > 0; left -= kc) target.push_back(pow2ceil(std::min(left, kc)));
         int tcap = 0;
-        for (int k : target) tcap += std::max(k, min_k);
+        for (int k : target) tcap += std::min(kc, std::max(k, min_k));
         if (tcap >= cap && target.size() >= uids.size()) continue;
         // Collect the jobs in slot order, then rebuild the units.
         std::vector<int64_t> ids;
@@ -303,7 +304,7 @@ void Batch<T>::compact() {
         DevBuf<Scalars> sc_old(d_sc.n);
         MI_HIP(hipMemcpyAsync(sc_old.p, d_sc.p, d_sc.bytes(), hipMemcpyDeviceToDevice, st));
         for (int K : target) {
-            K = pow2ceil(std::max(K, min_k));
+            K = std::min(kc, pow2ceil(std::max(K, min_k)));
             int uid = -1;
             for (int i = 0; i < kMaxUnits; ++i)
                 if (!units[i]) { uid = i; break; }
@@ -423,7 +424,7 @@ void Batch<T>::rebuild() {
         rows.start[i + 1] = rows.start[i] + (int)fp_blocks(m);
         bp.start[i + 1] = bp.start[i] + (int)bp_blocks(m);
         pix.start[i + 1] = pix.start[i] + (int)px_blocks(m);
-        if (U.K > 1) kmax = kMaxK;
+        kmax = std::max(kmax, U.K);
         all_walk = all_walk && m.has_otf;
         for (int k = 0; k < U.K; ++k) {
             if (U.jobs[k] < 0) continue;

@@ -59,6 +59,15 @@ struct Batch {
     mi_context *ctx;
     int S;
     int min_k = 1;                 // smallest K of a new unit
+    int max_k = kMaxK;             // largest K of a unit
+    // Largest K of a unit over m: the unit's K interleaved images, which the
+    // projectors gather from, stay within one XCD's 4 MiB L2 (K = 4 at 512^2
+    // FP32, 8 at 256^2); at most max_k.
+    int k_cap(const Matrix &m) const {
+        int k = max_k;
+        while (k > 1 && (int64_t)k * m.npix_t * (int64_t)sizeof(T) > (int64_t)(4 << 20)) k >>= 1;
+        return k;
+    }
     bool use_graph = true;
     bool auto_retire = true;
     int64_t reserve_bytes = (int64_t)1 << 30;   // device memory kept free at admission

@@ -851,6 +851,12 @@ __device__ __forceinline__ void edge_update_unit(const EngArgs<T> &a, const UG<T
     if constexpr (KMAX == 1) {                  \
         constexpr int K = 1;                    \
         CALL;                                   \
+    } else if constexpr (KMAX == 4) {           \
+        switch (U.K) {                          \
+            case 1: { constexpr int K = 1; CALL; break; } \
+            case 2: { constexpr int K = 2; CALL; break; } \
+            default: { constexpr int K = 4; CALL; break; } \
+        }                                       \
     } else {                                    \
         switch (U.K) {                          \
             case 1: { constexpr int K = 1; CALL; break; } \
@@ -1053,9 +1059,11 @@ void eng_fp_resid(hipStream_t st, const EngArgs<T> &a, const BlockMap &m, int s,
     const unsigned grid = grid8(m.start[m.nunits]);
     if (all_walk) {
         if (kmax == 1) k_eng_fp<T, 1, true, 1><<<grid, kBlock, 0, st>>>(a, m, s);
+        else if (kmax <= 4) k_eng_fp<T, 4, true, 1><<<grid, kBlock, 0, st>>>(a, m, s);
         else k_eng_fp<T, kMaxK, true, 1><<<grid, kBlock, 0, st>>>(a, m, s);
     } else {
         if (kmax == 1) k_eng_fp<T, 1, true, 3><<<grid, kBlock, 0, st>>>(a, m, s);
+        else if (kmax <= 4) k_eng_fp<T, 4, true, 3><<<grid, kBlock, 0, st>>>(a, m, s);
         else k_eng_fp<T, kMaxK, true, 3><<<grid, kBlock, 0, st>>>(a, m, s);
     }
     MI_HIP(hipGetLastError());
@@ -1065,6 +1073,7 @@ template <typename T>
 void eng_bp_grad(hipStream_t st, const EngArgs<T> &a, const BlockMap &m, int s, int kmax) {
     const unsigned grid = grid8(m.start[m.nunits]);
     if (kmax == 1) k_eng_bp<T, 1><<<grid, kBlock, 0, st>>>(a, m, s);
+    else if (kmax <= 4) k_eng_bp<T, 4><<<grid, kBlock, 0, st>>>(a, m, s);
     else k_eng_bp<T, kMaxK><<<grid, kBlock, 0, st>>>(a, m, s);
     MI_HIP(hipGetLastError());
 }
@@ -1074,9 +1083,11 @@ void eng_fp_sq(hipStream_t st, const EngArgs<T> &a, const BlockMap &rows, int s,
     const unsigned grid = grid8(rows.start[rows.nunits]);
     if (all_walk) {
         if (kmax == 1) k_eng_fp<T, 1, false, 1><<<grid, kBlock, 0, st>>>(a, rows, s);
+        else if (kmax <= 4) k_eng_fp<T, 4, false, 1><<<grid, kBlock, 0, st>>>(a, rows, s);
         else k_eng_fp<T, kMaxK, false, 1><<<grid, kBlock, 0, st>>>(a, rows, s);
     } else {
         if (kmax == 1) k_eng_fp<T, 1, false, 3><<<grid, kBlock, 0, st>>>(a, rows, s);
+        else if (kmax <= 4) k_eng_fp<T, 4, false, 3><<<grid, kBlock, 0, st>>>(a, rows, s);
         else k_eng_fp<T, kMaxK, false, 3><<<grid, kBlock, 0, st>>>(a, rows, s);
     }
     MI_HIP(hipGetLastError());
@@ -1086,6 +1097,7 @@ template <typename T, int OP>
 static void launch_pix(hipStream_t st, const EngArgs<T> &a, const BlockMap &m, int it, int kmax) {
     const unsigned grid = grid8(m.start[m.nunits]);
     if (kmax == 1) k_eng_pix<T, 1, OP><<<grid, kBlock, 0, st>>>(a, m, it);
+    else if (kmax <= 4) k_eng_pix<T, 4, OP><<<grid, kBlock, 0, st>>>(a, m, it);
     else k_eng_pix<T, kMaxK, OP><<<grid, kBlock, 0, st>>>(a, m, it);
     MI_HIP(hipGetLastError());
 }

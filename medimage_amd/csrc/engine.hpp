@@ -75,8 +75,8 @@ struct EngArgs {
     int S;
 };
 
-// Launchers (all on `st`).  `s` is the subset step.  kmax: 1 when every unit
-// has K = 1 (selects code compiled for K = 1 only), else 8.  all_walk: every
+// Launchers (all on `st`).  `s` is the subset step.  kmax: the largest K of
+// the launch's units (selects code compiled for K <= 1, 4 or 8).  all_walk: every
 // unit's forward projector is the checkpointed walk (selects code without
 // the sliced forward branch).
 template <typename T>
