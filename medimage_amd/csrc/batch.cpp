@@ -120,6 +120,14 @@ int Batch<T>::n_active() const {
 }
 
 template <typename T>
+int64_t Batch<T>::active_entries() const {
+    int64_t e = 0;
+    for (const auto &kv : jobs)
+        if (kv.second.state == BJ_ACTIVE && kv.second.m) e += kv.second.m->nnz;
+    return e;
+}
+
+template <typename T>
 void Batch<T>::ensure_stage(int64_t n) {
     if ((int64_t)stage.n >= n) return;
     MI_HIP(hipStreamSynchronize(ctx->stream));
@@ -234,6 +242,9 @@ int Batch<T>::admit(bool require) {
                 }
             }
             j.m->last_use = ++ctx->tick;
+            // the budget keeps the device as full as it needs to be and lets
+            // later jobs wait in arrival order instead of sharing it
+            if (max_entries > 0 && n_active() > 0 && active_entries() + j.m->nnz > max_entries) break;
             const GeomKey key = make_key(j.geom, S);
             int group = 0;
             for (int64_t q : pending) {

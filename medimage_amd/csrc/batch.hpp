@@ -60,6 +60,7 @@ struct Batch {
     int S;
     int min_k = 1;                 // smallest K of a new unit
     int max_k = kMaxK;             // largest K of a unit
+    int64_t max_entries = 0;       // admission budget: matrix entries of the active jobs (0: none)
     // Largest K of a unit over m: the unit's K interleaved images, which the
     // projectors gather from, stay within one XCD's 4 MiB L2 (K = 4 at 512^2
     // FP32, 8 at 256^2); at most max_k.
@@ -121,6 +122,7 @@ struct Batch {
     Scalars scalars(int64_t id);
     void read(int64_t id, T *u, T *v);
     int n_active() const;
+    int64_t active_entries() const;   // sum over active jobs of their matrix entries
     int n_pending() const { return (int)pending.size(); }
     int n_units() const { return (int)live.size(); }
 

