@@ -184,6 +184,38 @@ def test_batch_walk_and_sliced_units_mixed_bitwise(ctx):
         assert _same(mixed[i][0], alone[i][0]) and _same(mixed[i][1], alone[i][1]), f"job {i}"
 
 
+def test_batch_entry_budget_and_max_k(ctx):
+    """MI_BATCH_MAX_ENTRIES admits one job at a time when every matrix is
+    over the budget (the rest wait in arrival order); MI_BATCH_MAX_K caps the
+    unit size.  Results stay bitwise those of the job alone."""
+    jobs = _jobs(ctx)[4:8]                   # four jobs of geometry c
+    alone = _alone(ctx, jobs)
+    b = ctx.batch(4)
+    b.set(mi.MI_BATCH_MAX_ENTRIES, 1)
+    ids = [b.submit(*j) for j in jobs]
+    a, q = b.step(1)
+    assert a == 1 and q == 3
+    b.run()
+    for i, jid in enumerate(ids):
+        u, v = b.read(jid)
+        assert _same(u, alone[i][0]) and _same(v, alone[i][1]), f"budget job {i}"
+    b.close()
+    b = ctx.batch(4)
+    b.set(mi.MI_BATCH_MAX_K, 2)
+    ids = [b.submit(*j) for j in jobs]
+    b.step(0)
+    assert max(b.units()) <= 2
+    b.run()
+    for i, jid in enumerate(ids):
+        u, v = b.read(jid)
+        assert _same(u, alone[i][0]) and _same(v, alone[i][1]), f"max-K job {i}"
+    b.close()
+    b = ctx.batch(4)
+    with pytest.raises(mi.MIError):
+        b.set(mi.MI_BATCH_MAX_K, 3)
+    b.close()
+
+
 def test_batch_release_and_errors(ctx):
     g = GEOMS["a"]
     _, sino = configs.make_case(ctx, g, 3e4, seed=9)
