@@ -377,6 +377,7 @@ void Batch<T>::rebuild() {
     int64_t off = 0;
     kmax = 1;
     all_walk = true;
+    size_t lds_need = 0;
     edge_iters = 0;
     any_median = false;
     rows.nunits = bp.nunits = pix.nunits = (int)live.size();
@@ -412,6 +413,12 @@ void Batch<T>::rebuild() {
         D.bs_first = m.bs.first.p;
         D.bs_d16 = m.bs.d16.p;
         D.bs_len = f64 ? (const T *)m.bs.l64.p : (const T *)m.bs.l32.p;
+        D.bs_slot16 = m.bs.slot16.p;
+        D.bs_vlo = m.bs.vlo.p;
+        D.bs_voff = m.bs.voff.p;
+        D.bs_vtab0 = m.bs.d_vtab0.p;
+        D.nbins = m.g.n_bins;
+        lds_need = std::max(lds_need, m.bs.slot16.p ? (size_t)m.bs.max_slots * U.K * sizeof(T) : kBpLdsMax + 1);
         D.sub_row0 = m.d_sub_row0.p;
         D.g = U.g.p;
         D.r = U.r.p;
@@ -444,6 +451,7 @@ void Batch<T>::rebuild() {
             any_median = any_median || P.median_every > 0;
         }
     }
+    bp_lds = lds_need <= kBpLdsMax ? lds_need : 0;
     MI_HIP(hipStreamSynchronize(st));
     if ((int64_t)d_part.n < off) d_part.alloc((size_t)std::max<int64_t>(off, 1));
     if (!desc.empty())
@@ -485,7 +493,7 @@ void Batch<T>::record_pass(bool timed) {
             LaunchTimer lt(ctx, MI_K_BP, sum([&](const Unit<T> &U) {
                 return nnz_s(U) * (4 + sz) + nr_s(U) * sz * U.K + 3.0 * U.m->npix * sz * U.K;
             }));
-            eng_bp_grad<T>(st, args, bp, s, kmax);
+            eng_bp_grad<T>(st, args, bp, s, kmax, bp_lds);
         }
         {
             LaunchTimer lt(ctx, MI_K_FPSQ, sum([&](const Unit<T> &U) {

@@ -43,12 +43,15 @@ struct UG {
     const GL uint32_t *bs_idx, *bs_first;
     const GL int16_t *bs_d16;
     const GL T *bs_len;
+    const GL uint16_t *bs_slot16;
+    const GL int32_t *bs_vlo, *bs_voff;
+    const GL int64_t *bs_vtab0;
     const GL double *ot_lam, *ot_ax, *ot_ay, *ot_stx, *ot_sty;
     const GL int32_t *ot_cx, *ot_cy, *ot_sg, *ot_cnt;
     const GL int64_t *sub_row0;
     GL T *g, *r, *u, *v, *d, *w;
     int64_t npix_t, nrows, part_off, cap;
-    int nx, ny, tny, K, fs_P, bs_P, job0;
+    int nx, ny, tny, K, fs_P, bs_P, job0, nbins;
     __device__ __forceinline__ explicit UG(const UnitDesc<T> &D)
         : fs_slice0((const GL int64_t *)D.fs_slice0), fs_base((const GL int64_t *)D.fs_base),
           fs_T((const GL int32_t *)D.fs_T), fs_idx((const GL uint32_t *)D.fs_idx),
@@ -57,6 +60,8 @@ struct UG {
           bs_base((const GL int64_t *)D.bs_base), bs_T((const GL int32_t *)D.bs_T),
           bs_idx((const GL uint32_t *)D.bs_idx), bs_first((const GL uint32_t *)D.bs_first),
           bs_d16((const GL int16_t *)D.bs_d16), bs_len((const GL T *)D.bs_len),
+          bs_slot16((const GL uint16_t *)D.bs_slot16), bs_vlo((const GL int32_t *)D.bs_vlo),
+          bs_voff((const GL int32_t *)D.bs_voff), bs_vtab0((const GL int64_t *)D.bs_vtab0),
           ot_lam((const GL double *)D.ot_lam), ot_ax((const GL double *)D.ot_ax), ot_ay((const GL double *)D.ot_ay),
           ot_stx((const GL double *)D.ot_stx), ot_sty((const GL double *)D.ot_sty),
           ot_cx((const GL int32_t *)D.ot_cx), ot_cy((const GL int32_t *)D.ot_cy),
@@ -64,7 +69,7 @@ struct UG {
           sub_row0((const GL int64_t *)D.sub_row0), g((GL T *)D.g), r((GL T *)D.r), u((GL T *)D.u),
           v((GL T *)D.v), d((GL T *)D.d), w((GL T *)D.w), npix_t(D.npix_t), nrows(D.nrows),
           part_off(D.part_off), cap(D.cap), nx(D.nx), ny(D.ny), tny(D.tny), K(D.K), fs_P(D.fs_P), bs_P(D.bs_P),
-          job0(D.job0) {}
+          job0(D.job0), nbins(D.nbins) {}
     // Tiled index of pixel (i, j), or -1 outside the image.
     __device__ __forceinline__ int64_t at(int i, int j) const {
         return (i < 0 || j < 0 || i >= nx || j >= ny) ? -1 : tile_index(i, j, tny);
@@ -234,10 +239,17 @@ __device__ __forceinline__ void block_store(const EngArgs<T> &a, const UG<T> &U,
 // load and one 16-byte (absolute) or 8-byte (differences) index load.
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
-template <typename T, int K, bool NT, bool D16>
+#define LDSP __attribute__((address_space(3)))
+enum IdxMode { IM_ABS32 = 0, IM_DELTA16 = 1, IM_SLOT16 = 2 };
+
+// IM: absolute 32-bit indices (ip), 16-bit differences (dp, from `cur`), or
+// 16-bit LDS slots (dp read unsigned).  LDSG: gather from `lx` (LDS) instead
+// of the global array x.
+template <typename T, int K, bool NT, int IM, bool LDSG = false>
 __device__ __forceinline__ void walk(const GL uint32_t *__restrict__ ip, const GL int16_t *__restrict__ dp,
                                      uint32_t cur, const GL T *__restrict__ lp, int n, const GL T *__restrict__ x,
-                                     uint32_t xbytes, Acc<T> (&acc)[K]) {
+                                     uint32_t xbytes, Acc<T> (&acc)[K], const LDSP T *lx = nullptr) {
+    constexpr bool D16 = IM != IM_ABS32;
     constexpr int GR = (K <= 2) ? MI_STEPS / kGroup : 1;   // groups per pipeline stage
     constexpr int64_t kStride = kGroup * kWave;             // entries between a lane's groups
 #if MI_BUF_GATHER
@@ -251,7 +263,13 @@ __device__ __forceinline__ void walk(const GL uint32_t *__restrict__ ip, const G
 #pragma unroll
         for (int r = 0; r < GR; ++r) {
             const int64_t o = (int64_t)(g0 + r) * kStride;
-            if constexpr (D16) {
+            if constexpr (IM == IM_SLOT16) {
+                const u32x2 a = stream_ld<NT>((const GL u32x2 *)(dp + o));
+                sv[r][0] = a.x & 0xffffu;
+                sv[r][1] = a.x >> 16;
+                sv[r][2] = a.y & 0xffffu;
+                sv[r][3] = a.y >> 16;
+            } else if constexpr (D16) {
                 const u32x2 a = stream_ld<NT>((const GL u32x2 *)(dp + o));
                 sv[r][0] = (uint32_t)(int32_t)(int16_t)(a.x & 0xffffu);
                 sv[r][1] = (uint32_t)(int32_t)(int16_t)(a.x >> 16);
@@ -273,11 +291,16 @@ __device__ __forceinline__ void walk(const GL uint32_t *__restrict__ ip, const G
         }
     };
     auto gather = [&](uint32_t id, T (&o)[K]) {
+        if constexpr (LDSG) {
+#pragma unroll
+            for (int j = 0; j < K; ++j) o[j] = lx[id * K + j];
+        } else {
 #if MI_BUF_GATHER
-        loadk_buf<T, K>(xr, id * (uint32_t)(K * sizeof(T)), o);
+            loadk_buf<T, K>(xr, id * (uint32_t)(K * sizeof(T)), o);
 #else
-        loadk<T, K>(x + (uint32_t)(id * K), o);
+            loadk<T, K>(x + (uint32_t)(id * K), o);
 #endif
+        }
     };
     const int ng = n / kGroup, full = ng / GR;
     // Software pipeline: the next stage's stream loads are issued after the
@@ -292,7 +315,7 @@ __device__ __forceinline__ void walk(const GL uint32_t *__restrict__ ip, const G
         for (int r = 0; r < GR; ++r)
 #pragma unroll
             for (int q = 0; q < kGroup; ++q) {
-                if constexpr (D16) {
+                if constexpr (IM == IM_DELTA16) {
                     cur += sv[r][q];
                     id[r][q] = cur;
                 } else {
@@ -318,7 +341,12 @@ __device__ __forceinline__ void walk(const GL uint32_t *__restrict__ ip, const G
         const int64_t o = (int64_t)g * kStride;
         uint32_t id[kGroup];
         T ln[kGroup], xv[kGroup][K];
-        if constexpr (D16) {
+        if constexpr (IM == IM_SLOT16) {
+            const u32x2 a = stream_ld<NT>((const GL u32x2 *)(dp + o));
+            const uint32_t w[2] = {a.x, a.y};
+#pragma unroll
+            for (int q = 0; q < kGroup; ++q) id[q] = (w[q >> 1] >> (16 * (q & 1))) & 0xffffu;
+        } else if constexpr (D16) {
             const u32x2 a = stream_ld<NT>((const GL u32x2 *)(dp + o));
             const uint32_t w[2] = {a.x, a.y};
 #pragma unroll
@@ -350,8 +378,8 @@ __device__ __forceinline__ void walk_slice(const GL int64_t *base, const GL int3
     const int64_t p0 = sell_pos(base[sl], 0, lane);
     const int n = Tn[sl];
     const uint32_t xb = (uint32_t)(xn * K * (int64_t)sizeof(T));
-    if (d16) walk<T, K, NT, true>(nullptr, d16 + p0, first[sl * kWave + lane], len + p0, n, x, xb, acc);
-    else walk<T, K, NT, false>(idx + p0, nullptr, 0u, len + p0, n, x, xb, acc);
+    if (d16) walk<T, K, NT, IM_DELTA16>(nullptr, d16 + p0, first[sl * kWave + lane], len + p0, n, x, xb, acc);
+    else walk<T, K, NT, IM_ABS32>(idx + p0, nullptr, 0u, len + p0, n, x, xb, acc);
 }
 
 // One lane of the checkpointed forward walk: n Siddon steps from the stored
@@ -563,25 +591,69 @@ __device__ __forceinline__ void gc_reg(const UG<T> &U, int i, int j, const doubl
 
 // Gather back projection of subset s over 4 slices of tiled pixels, fused
 // with d = S * A_s^T r + alpha * R(u, v); partial sum d^2.
-template <typename T, int K>
-__device__ __forceinline__ void bp_unit(const EngArgs<T> &a, const UG<T> &U, int s, int lb, double *lds) {
+// Back projection of subset s over one 16 x 16 block of pixels (bp_item;
+// wave w holds tile row w of the block).  LDS: the block first copies the
+// residual values of every (view, bin) its entries use into `ldr` (slots of
+// the block's table, `vlo` / `voff`), then gathers from there.
+template <typename T, int K, bool LDS>
+__device__ __forceinline__ void bp_unit(const EngArgs<T> &a, const UG<T> &U, int s, int lb, double *lds,
+                                        LDSP T *ldr) {
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
-    const int P = U.bs_P, C = kWave / P;
     const int64_t sl0 = U.bs_slice0[s], li = (int64_t)lb * (kBlock / kWave) + wv;
+    const int tnx = (U.nx + kTileH - 1) / kTileH;
     double m[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) m[j] = 0.0;
+    if (LDS) {
+        const int nv = (int)((U.sub_row0[s + 1] - U.sub_row0[s]) / U.nbins);
+        const int64_t nblk = bp_blocks_of(tnx, U.tny);
+        const GL int32_t *vlo = U.bs_vlo + U.bs_vtab0[s] + (int64_t)lb * nv;
+        // the same offsets as voff_at (runtime.hpp)
+        const GL int32_t *voff = U.bs_voff + (U.bs_vtab0[s] + (int64_t)s * nblk + (int64_t)lb * (nv + 1));
+        // wave w copies views w, w + 4, ...; eight views' loads in flight
+        constexpr int VU = 8, NW = kBlock / kWave;
+        for (int v0 = wv; v0 < nv; v0 += NW * VU)
+            for (int k0 = 0;; k0 += kWave) {
+                T t[VU][K];
+                int o[VU], c[VU];
+                bool more = false;
+#pragma unroll
+                for (int u = 0; u < VU; ++u) {
+                    const int v = v0 + NW * u;
+                    o[u] = c[u] = 0;
+                    if (v < nv) {
+                        o[u] = voff[v];
+                        c[u] = voff[v + 1] - o[u];
+                        more = more || c[u] > k0 + kWave;
+                        if (k0 + lane < c[u])
+                            loadk<T, K>(U.r + (U.sub_row0[s] + (int64_t)v * U.nbins + vlo[v] + k0 + lane) * K, t[u]);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < VU; ++u)
+                    if (k0 + lane < c[u])
+#pragma unroll
+                        for (int j = 0; j < K; ++j) ldr[(o[u] + k0 + lane) * K + j] = t[u][j];
+                if (!__any(more)) break;   // wave-uniform exit
+            }
+        __syncthreads();
+    }
     if (li < U.bs_slice0[s + 1] - sl0) {
         const int64_t sl = sl0 + li;
         Acc<T> acc[K];
-        walk_slice<T, K, MI_NT_BP>(U.bs_base, U.bs_T, U.bs_idx, U.bs_first, U.bs_d16, U.bs_len, sl, lane, U.r,
-                                   U.nrows, acc);
+        if (LDS) {
+            const int64_t p0 = sell_pos(U.bs_base[sl], 0, lane);
+            walk<T, K, MI_NT_BP, IM_SLOT16, true>(nullptr, (const GL int16_t *)(U.bs_slot16 + p0), 0u,
+                                                  U.bs_len + p0, U.bs_T[sl], U.r, 0u, acc, ldr);
+        } else {
+            walk_slice<T, K, MI_NT_BP>(U.bs_base, U.bs_T, U.bs_idx, U.bs_first, U.bs_d16, U.bs_len, sl, lane, U.r,
+                                       U.nrows, acc);
+        }
         double tot[K];
 #pragma unroll
         for (int j = 0; j < K; ++j) tot[j] = acc[j].value();
-        group_sums<K>(tot, P);
-        const int64_t q = li * C + lane / P;
-        if (lane % P == 0 && q < U.npix_t) {
+        const int64_t q = bp_item(lb, wv, lane, tnx, U.tny);
+        if (q >= 0) {
             int i, j;
             tile_coords(q, U.tny, i, j);
             T dv[K];
@@ -879,16 +951,17 @@ __global__ __launch_bounds__(kBlock) void k_eng_fp(EngArgs<T> a, BlockMap m, int
     MI_KSWITCH(KMAX, (fp_unit<T, K, RESID, FM>(a, U, s, lb, lds, ldr)))
 }
 
-template <typename T, int KMAX>
+template <typename T, int KMAX, bool LDS>
 __global__ __launch_bounds__(kBlock) void k_eng_bp(EngArgs<T> a, BlockMap m, int s) {
     __shared__ double lds[kBlock / kWave * kMaxK];
+    extern __shared__ __align__(16) unsigned char ldr_raw[];   // LDS: the block's residual slots
     const int b = (int)xcd_block();
     if (b >= m.start[m.nunits]) return;
     const int ui = find_unit(m, b);
     const UG<T> U(a.units[ui]);
     if (unit_idle(a, U)) return;
     const int lb = b - m.start[ui];
-    MI_KSWITCH(KMAX, (bp_unit<T, K>(a, U, s, lb, lds)))
+    MI_KSWITCH(KMAX, (bp_unit<T, K, LDS>(a, U, s, lb, lds, (LDSP T *)ldr_raw)))
 }
 
 enum PixOp { PX_AXPY = 0, PX_MEDIAN, PX_COPY_WU, PX_EDGE_GRAD, PX_EDGE_NABLA, PX_EDGE_UPDATE, PX_NV };
@@ -1070,11 +1143,17 @@ void eng_fp_resid(hipStream_t st, const EngArgs<T> &a, const BlockMap &m, int s,
 }
 
 template <typename T>
-void eng_bp_grad(hipStream_t st, const EngArgs<T> &a, const BlockMap &m, int s, int kmax) {
+void eng_bp_grad(hipStream_t st, const EngArgs<T> &a, const BlockMap &m, int s, int kmax, size_t lds_bytes) {
     const unsigned grid = grid8(m.start[m.nunits]);
-    if (kmax == 1) k_eng_bp<T, 1><<<grid, kBlock, 0, st>>>(a, m, s);
-    else if (kmax <= 4) k_eng_bp<T, 4><<<grid, kBlock, 0, st>>>(a, m, s);
-    else k_eng_bp<T, kMaxK><<<grid, kBlock, 0, st>>>(a, m, s);
+    if (lds_bytes) {
+        if (kmax == 1) k_eng_bp<T, 1, true><<<grid, kBlock, lds_bytes, st>>>(a, m, s);
+        else if (kmax <= 4) k_eng_bp<T, 4, true><<<grid, kBlock, lds_bytes, st>>>(a, m, s);
+        else k_eng_bp<T, kMaxK, true><<<grid, kBlock, lds_bytes, st>>>(a, m, s);
+    } else {
+        if (kmax == 1) k_eng_bp<T, 1, false><<<grid, kBlock, 0, st>>>(a, m, s);
+        else if (kmax <= 4) k_eng_bp<T, 4, false><<<grid, kBlock, 0, st>>>(a, m, s);
+        else k_eng_bp<T, kMaxK, false><<<grid, kBlock, 0, st>>>(a, m, s);
+    }
     MI_HIP(hipGetLastError());
 }
 
@@ -1170,7 +1249,7 @@ void eng_copy(hipStream_t st, void *dst, const void *src, int64_t bytes) {
 
 #define MI_ENG_INST(T)                                                                                     \
     template void eng_fp_resid<T>(hipStream_t, const EngArgs<T> &, const BlockMap &, int, int, bool);       \
-    template void eng_bp_grad<T>(hipStream_t, const EngArgs<T> &, const BlockMap &, int, int);              \
+    template void eng_bp_grad<T>(hipStream_t, const EngArgs<T> &, const BlockMap &, int, int, size_t);      \
     template void eng_fp_sq<T>(hipStream_t, const EngArgs<T> &, const BlockMap &, int, int, bool);         \
     template void eng_nv<T>(hipStream_t, const EngArgs<T> &, const BlockMap &, int);                        \
     template void eng_axpy<T>(hipStream_t, const EngArgs<T> &, const BlockMap &, int);                      \

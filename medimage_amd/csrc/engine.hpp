@@ -18,6 +18,10 @@ namespace mi {
 constexpr int kMaxK = 8;             // jobs per unit
 constexpr int kMaxUnits = 64;        // units per batch
 constexpr int kPixBlock = kBlock;    // tiled pixel positions per stencil block
+// Largest dynamic LDS of a back-projection block (the staged residual
+// slots); above it the launch gathers from global memory.  64 KiB keeps two
+// blocks per CU at the largest size.
+constexpr size_t kBpLdsMax = 64 << 10;
 
 // Partial-sum slots of a unit (each K * cap doubles, see UnitDesc).
 enum EngSlot { SL_DF = 0, SL_DD, SL_AD, SL_NV, SL_E0, SL_E1, SL_E2, SL_E3, SL_COUNT };
@@ -44,6 +48,9 @@ struct UnitDesc {
     const uint32_t *bs_first;
     const int16_t *bs_d16;
     const T *bs_len;
+    const uint16_t *bs_slot16;             // LDS slot per entry (null: global gathers only)
+    const int32_t *bs_vlo, *bs_voff;       // per (subset, block, view): first bin, slot offset
+    const int64_t *bs_vtab0;               // S + 1 table offsets
     // checkpointed forward walk (fan beam; null otherwise): per lane of a
     // forward slice of 8 rays x 8 segments
     const double *ot_lam, *ot_ax, *ot_ay, *ot_stx, *ot_sty;
@@ -55,7 +62,7 @@ struct UnitDesc {
     int64_t part_off, cap;                 // partials: part + part_off + (slot * K + j) * cap
     int nx, ny, tny, K, fs_P, bs_P;
     int job0;                              // job slots job0 .. job0 + K - 1 (stable)
-    int pad;
+    int nbins;
 };
 
 // Block ranges of one launch: unit u owns blocks [start[u], start[u+1]).
@@ -82,7 +89,9 @@ struct EngArgs {
 template <typename T>
 void eng_fp_resid(hipStream_t st, const EngArgs<T> &a, const BlockMap &m, int s, int kmax, bool all_walk);
 template <typename T>
-void eng_bp_grad(hipStream_t st, const EngArgs<T> &a, const BlockMap &m, int s, int kmax);
+// lds_bytes: 0, or the dynamic LDS of the staged-residual form (every unit
+// has slot codes and max_slots * K * sizeof(T) fits).
+void eng_bp_grad(hipStream_t st, const EngArgs<T> &a, const BlockMap &m, int s, int kmax, size_t lds_bytes);
 // ||A_s d||^2 (rows map; the walk of the residual launch without the
 // residual) and nV(d) (pixel map), as two launches.
 template <typename T>

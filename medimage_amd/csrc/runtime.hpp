@@ -82,11 +82,42 @@ struct Sell {
     DevBuf<float> l32;
     DevBuf<double> l64;
     int64_t entries = 0;             // stored, padding included
+    // Back slices only: residual values staged in LDS.  For each (subset,
+    // block of four slices, view of the subset) the block's first bin `vlo`
+    // and slot offset `voff` (nv + 1 per block); every entry also carries the
+    // LDS slot of its (view, bin) (slot16, same positions as l32).  Empty
+    // when a block needs 65536 slots or more.
+    DevBuf<uint16_t> slot16;
+    DevBuf<int32_t> vlo, voff;
+    std::vector<int64_t> vtab0;      // S + 1: first table entry of each subset
+    DevBuf<int64_t> d_vtab0;
+    int64_t max_slots = 0;           // slots of the largest block
     size_t bytes() const {
         return d_slice0.bytes() + base.bytes() + T.bytes() + idx.bytes() + first.bytes() + d16.bytes() +
-               l32.bytes() + l64.bytes();
+               l32.bytes() + l64.bytes() + slot16.bytes() + vlo.bytes() + voff.bytes() + d_vtab0.bytes();
     }
 };
+
+// Back-projection items: a block of four slices covers a 16 x 16 pixel
+// square (4 tile rows x 2 tile columns of the tiled layout); slice w of the
+// block is tile row w of the square, lane l position l % 32 of tile column
+// l / 32.  Returns the tiled position, or -1 outside the tile grid.
+constexpr int kBpBlockTR = 4, kBpBlockTC = 2;
+__host__ __device__ __forceinline__ int64_t bp_blocks_of(int tnx, int tny) {
+    return (int64_t)((tnx + kBpBlockTR - 1) / kBpBlockTR) * ((tny + kBpBlockTC - 1) / kBpBlockTC);
+}
+// First voff entry of block blk of subset s (nv + 1 entries per block; vlo
+// has nv per block from vtab0[s] + blk nv).
+__host__ __device__ __forceinline__ int64_t voff_at(const int64_t *vtab0, int s, int64_t nblocks, int64_t blk, int nv) {
+    return vtab0[s] + (int64_t)s * nblocks + blk * (nv + 1);
+}
+__host__ __device__ __forceinline__ int64_t bp_item(int64_t blk, int w, int lane, int tnx, int tny) {
+    const int nbc = (tny + kBpBlockTC - 1) / kBpBlockTC;
+    const int64_t tr = (blk / nbc) * kBpBlockTR + w;
+    const int tc = (int)(blk % nbc) * kBpBlockTC + (lane >> 5);
+    if (tr >= tnx || tc >= tny) return -1;
+    return (tr * tny + tc) * 32 + (lane & 31);
+}
 
 // Checkpointed forward walk (fan beam).  A forward block (kBlock lanes)
 // covers kOtfRays consecutive rays of a subset, each cut into kOtfSeg

@@ -42,17 +42,23 @@ struct FItems {
     }
 };
 
-// Back item: tiled pixel q of subset s -> its CSC column.
+// Back item: lane `item` of slice sl (P = 1) -> its tiled pixel (bp_item:
+// four slices per 16 x 16 block) -> its CSC column in subset s.
 struct BItems {
     const int64_t *colptr, *slice0;
-    int S, P, C, nx, ny, tny;
+    int S, P, C, nx, ny, tnx, tny;
     int64_t npix, npix_t;
-    __device__ __forceinline__ void range(int64_t sl, int item, int64_t &b, int64_t &e) const {
-        const int s = subset_of(slice0, S, sl);
-        const int64_t q = (sl - slice0[s]) * C + item;
-        int i, j;
+    __device__ __forceinline__ int64_t pixel(int64_t sl, int item, int &s, int &i, int &j) const {
+        s = subset_of(slice0, S, sl);
+        const int64_t li = sl - slice0[s];
+        const int64_t q = bp_item(li / 4, (int)(li % 4), item, tnx, tny);
+        if (q < 0) return -1;
         tile_coords(q, tny, i, j);
-        if (q >= npix_t || i >= nx || j >= ny) { b = e = 0; return; }
+        return (i < nx && j < ny) ? q : -1;
+    }
+    __device__ __forceinline__ void range(int64_t sl, int item, int64_t &b, int64_t &e) const {
+        int s, i, j;
+        if (pixel(sl, item, s, i, j) < 0) { b = e = 0; return; }
         const int64_t c = (int64_t)s * npix + (int64_t)i * ny + j;
         b = colptr[c];
         e = colptr[c + 1];
@@ -135,6 +141,85 @@ __global__ void k_sell_delta(int64_t nslices, const int64_t *base, const int32_t
     if (bad) atomicOr(overflow, 1u);
 }
 
+// LDS staging tables of the back slices.  lo / hi: per (subset, block,
+// view) the smallest and largest bin of the block's entries (integer
+// atomics: the result does not depend on order).
+__global__ void k_vtab_init(int32_t *lo, int32_t *hi, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        lo[i] = INT32_MAX;
+        hi[i] = -1;
+    }
+}
+
+__global__ void k_vtab_minmax(BItems it, int64_t nslices, const uint32_t *crow, const int64_t *sub_row0,
+                              const int64_t *vtab0, int nbins, int32_t *lo, int32_t *hi) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nslices * kWave) return;
+    const int64_t sl = g / kWave;
+    int64_t b, e;
+    it.range(sl, (int)(g % kWave), b, e);
+    if (b == e) return;
+    const int s = subset_of(it.slice0, it.S, sl);
+    const int64_t blk = (sl - it.slice0[s]) / 4;
+    const int nv = (int)((sub_row0[s + 1] - sub_row0[s]) / nbins);
+    const int64_t tab = vtab0[s] + blk * nv;
+    for (int64_t k = b; k < e; ++k) {
+        const int64_t rl = (int64_t)crow[k] - sub_row0[s];
+        const int v = (int)(rl / nbins), bin = (int)(rl % nbins);
+        atomicMin(lo + tab + v, bin);
+        atomicMax(hi + tab + v, bin);
+    }
+}
+
+// One thread per (subset, block): slot offsets; *mx = largest block.
+__global__ void k_vtab_off(const int64_t *vtab0, const int64_t *sub_row0, int S, int64_t nblocks, int nbins,
+                           const int32_t *lo, const int32_t *hi, int32_t *vlo, int32_t *voff,
+                           unsigned long long *mx) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (int64_t)S * nblocks) return;
+    const int s = (int)(g / nblocks);
+    const int64_t blk = g % nblocks;
+    const int nv = (int)((sub_row0[s + 1] - sub_row0[s]) / nbins);
+    const int64_t t = vtab0[s] + blk * nv, to = voff_at(vtab0, s, nblocks, blk, nv);
+    int64_t off = 0;
+    for (int v = 0; v < nv; ++v) {
+        const int c = hi[t + v] >= lo[t + v] ? hi[t + v] - lo[t + v] + 1 : 0;
+        vlo[t + v] = c ? lo[t + v] : 0;
+        voff[to + v] = (int32_t)off;
+        off += c;
+    }
+    voff[to + nv] = (int32_t)off;
+    atomicMax(mx, (unsigned long long)off);
+}
+
+// Slot of every stored back entry (same positions as the lengths); padding
+// takes the lane's last slot.
+__global__ void k_slot_fill(BItems it, int64_t nslices, const int64_t *base, const int32_t *T, const uint32_t *crow,
+                            const int64_t *sub_row0, const int64_t *vtab0, int64_t nblocks, int nbins,
+                            const int32_t *vlo, const int32_t *voff, uint16_t *slot) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nslices * kWave) return;
+    const int64_t sl = g / kWave;
+    const int lane = (int)(g % kWave);
+    int64_t b, e;
+    it.range(sl, lane, b, e);
+    const int s = subset_of(it.slice0, it.S, sl);
+    const int64_t blk = (sl - it.slice0[s]) / 4;
+    const int nv = (int)((sub_row0[s + 1] - sub_row0[s]) / nbins);
+    const int64_t t = vtab0[s] + blk * nv, to = voff_at(vtab0, s, nblocks, blk, nv);
+    const int n = T[sl];
+    uint16_t last = 0;
+    for (int k = 0; k < n; ++k) {
+        if (b + k < e) {
+            const int64_t rl = (int64_t)crow[b + k] - sub_row0[s];
+            const int v = (int)(rl / nbins), bin = (int)(rl % nbins);
+            last = (uint16_t)(voff[to + v] + bin - vlo[t + v]);
+        }
+        slot[sell_pos(base[sl], k, lane)] = last;
+    }
+}
+
 // Replaces sl.idx by the delta form when every difference fits.
 void compact_sell(Sell &sl, hipStream_t st) {
     const int64_t ns = sl.slice0.back();
@@ -213,6 +298,49 @@ void fill_sell(Sell &sl, It it, const uint32_t *src_idx, const float *s32, const
     MI_HIP(hipStreamSynchronize(st));
 }
 
+// LDS staging tables and slot codes of the back slices (before the CSC is
+// dropped).  Left empty when a block would need 65536 slots or more.
+void build_slots(Matrix &m, const BItems &bi, int64_t nblk, hipStream_t st) {
+    Sell &q = m.bs;
+    const int nbins = m.g.n_bins;
+    q.vtab0.assign(m.S + 1, 0);
+    for (int s = 0; s < m.S; ++s) q.vtab0[s + 1] = q.vtab0[s] + nblk * ((m.sub_row0[s + 1] - m.sub_row0[s]) / nbins);
+    const int64_t ntab = q.vtab0.back();
+    const int64_t ns = q.slice0.back();
+    if (ntab == 0 || ns == 0) return;
+    q.d_vtab0.alloc(q.vtab0.size());
+    MI_HIP(hipMemcpyAsync(q.d_vtab0.p, q.vtab0.data(), q.vtab0.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    DevBuf<int32_t> lo(ntab), hi(ntab);
+    k_vtab_init<<<grid_of(ntab), kBlock, 0, st>>>(lo.p, hi.p, ntab);
+    MI_HIP(hipGetLastError());
+    k_vtab_minmax<<<grid_of(ns * kWave), kBlock, 0, st>>>(bi, ns, m.crow.p, m.d_sub_row0.p, q.d_vtab0.p, nbins, lo.p,
+                                                          hi.p);
+    MI_HIP(hipGetLastError());
+    q.vlo.alloc((size_t)ntab);
+    q.voff.alloc((size_t)(ntab + (int64_t)m.S * nblk));
+    DevBuf<unsigned long long> mx(1);
+    MI_HIP(hipMemsetAsync(mx.p, 0, sizeof(unsigned long long), st));
+    k_vtab_off<<<grid_of((int64_t)m.S * nblk), kBlock, 0, st>>>(q.d_vtab0.p, m.d_sub_row0.p, m.S, nblk, nbins, lo.p,
+                                                               hi.p, q.vlo.p, q.voff.p, mx.p);
+    MI_HIP(hipGetLastError());
+    unsigned long long h = 0;
+    MI_HIP(hipMemcpyAsync(&h, mx.p, sizeof(h), hipMemcpyDeviceToHost, st));
+    MI_HIP(hipStreamSynchronize(st));
+    if (h >= 65536) {
+        q.vlo.reset();
+        q.voff.reset();
+        q.d_vtab0.reset();
+        q.max_slots = 0;
+        return;
+    }
+    q.max_slots = (int64_t)h;
+    q.slot16.alloc((size_t)std::max<int64_t>(q.entries, 1));
+    k_slot_fill<<<grid_of(ns * kWave), kBlock, 0, st>>>(bi, ns, q.base.p, q.T.p, m.crow.p, m.d_sub_row0.p,
+                                                        q.d_vtab0.p, nblk, nbins, q.vlo.p, q.voff.p, q.slot16.p);
+    MI_HIP(hipGetLastError());
+    MI_HIP(hipStreamSynchronize(st));
+}
+
 }  // namespace
 
 void build_sell(Matrix &m, hipStream_t st, bool drop_csr) {
@@ -238,14 +366,17 @@ void build_sell(Matrix &m, hipStream_t st, bool drop_csr) {
         m.len32.reset();
         m.len64.reset();
     }
-    // back
-    m.bs.P = choose_p(MI_SELL_PB, (double)m.nnz / ((double)m.S * std::max<int64_t>(m.npix, 1)));
-    m.bs.C = kWave / m.bs.P;
-    const int64_t per = (m.npix_t + m.bs.C - 1) / m.bs.C;
+    // back: one lane per pixel, four slices per 16 x 16 block (bp_item)
+    m.bs.P = 1;
+    m.bs.C = kWave;
+    const int tnx = (g.nx + kTileH - 1) / kTileH;
+    const int64_t nblk = bp_blocks_of(tnx, m.tny), per = nblk * 4;
     m.bs.slice0.assign(m.S + 1, 0);
     for (int s = 0; s < m.S; ++s) m.bs.slice0[s + 1] = m.bs.slice0[s] + per;
-    BItems bi{m.colptr.p, nullptr, m.S, m.bs.P, m.bs.C, g.nx, g.ny, m.tny, m.npix, m.npix_t};
+    BItems bi{m.colptr.p, nullptr, m.S, m.bs.P, m.bs.C, g.nx, g.ny, tnx, m.tny, m.npix, m.npix_t};
     fill_sell(m.bs, bi, m.crow.p, m.clen32.p, f64 ? m.clen64.p : nullptr, false, g.ny, m.tny, st);
+    bi.slice0 = m.bs.d_slice0.p;
+    build_slots(m, bi, nblk, st);
     compact_sell(m.bs, st);
     if (drop_csr) {
         m.colptr.reset();
