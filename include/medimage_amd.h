@@ -226,9 +226,12 @@ enum mi_batch_option {
     MI_BATCH_RESERVE_MB = 2,  /* device memory left free at admission (1024) */
     MI_BATCH_MAX_K = 3,       /* largest K of a unit (1, 2, 4, 8; default 8);
                                  units are also kept to K images <= 4 MiB  */
-    MI_BATCH_MAX_ENTRIES = 4  /* admission stops while the active jobs'
+    MI_BATCH_MAX_ENTRIES = 4, /* admission stops while the active jobs'
                                  matrices hold this many entries (0: no
                                  limit, the default); one job always runs  */
+    MI_BATCH_BP_LDS = 5       /* 1 (default): back projection gathers the
+                                 residual from LDS when it fits; 0: from
+                                 global memory (same results, bit for bit) */
 };
 int mi_batch_set(mi_batch *b, int option, int64_t value);
 /* Queues a job (inputs are copied; u0 / v0 may be NULL).  It is admitted to

@@ -28,7 +28,8 @@ MI_OK, MI_E_ARG, MI_E_DEVICE, MI_E_NOMEM, MI_E_STATE, MI_E_LIMIT = 0, -1, -2, -3
 MI_BEAM_FAN_FLAT, MI_BEAM_PARALLEL, MI_BEAM_GOLDENC = 0, 1, 2
 MI_FP32, MI_FP64 = 0, 1
 MI_K_FP, MI_K_BP, MI_K_FPSQ, MI_K_STEP, MI_K_EDGE = 0, 1, 2, 3, 4
-MI_BATCH_GRAPH, MI_BATCH_MIN_K, MI_BATCH_RESERVE_MB, MI_BATCH_MAX_K, MI_BATCH_MAX_ENTRIES = 0, 1, 2, 3, 4
+MI_BATCH_GRAPH, MI_BATCH_MIN_K, MI_BATCH_RESERVE_MB, MI_BATCH_MAX_K, MI_BATCH_MAX_ENTRIES, MI_BATCH_BP_LDS = \
+    0, 1, 2, 3, 4, 5
 MI_JOB_PENDING, MI_JOB_ACTIVE, MI_JOB_DONE, MI_JOB_FAILED = 0, 1, 2, 3
 
 

@@ -451,7 +451,7 @@ void Batch<T>::rebuild() {
             any_median = any_median || P.median_every > 0;
         }
     }
-    bp_lds = lds_need <= kBpLdsMax ? lds_need : 0;
+    bp_lds = (use_bp_lds && lds_need <= kBpLdsMax) ? lds_need : 0;
     MI_HIP(hipStreamSynchronize(st));
     if ((int64_t)d_part.n < off) d_part.alloc((size_t)std::max<int64_t>(off, 1));
     if (!desc.empty())

@@ -90,6 +90,7 @@ struct Batch {
     int kmax = 1;
     bool all_walk = true;   // every unit's forward projector is the checkpointed walk
     size_t bp_lds = 0;      // dynamic LDS of the back projection (0: global gathers)
+    bool use_bp_lds = true; // MI_BATCH_BP_LDS
     int edge_iters = 0;            // max iter_edge over live jobs
     bool any_median = false;
     bool dirty = true;

@@ -682,6 +682,10 @@ int mi_batch_set(mi_batch *b, int option, int64_t value) {
                     MI_REQUIRE(value >= 0, MI_E_ARG, "negative reserve");
                     B.reserve_bytes = value << 20;
                     break;
+                case MI_BATCH_BP_LDS:
+                    B.use_bp_lds = value != 0;
+                    B.dirty = true;
+                    break;
                 case MI_BATCH_MAX_ENTRIES:
                     MI_REQUIRE(value >= 0, MI_E_ARG, "negative entry budget");
                     B.max_entries = value;

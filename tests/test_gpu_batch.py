@@ -39,10 +39,12 @@ def _jobs(ctx, precision=mi.MI_FP32):
     return out
 
 
-def _run_batch(ctx, jobs, precision=mi.MI_FP32, graph=True, waves=(0,)):
+def _run_batch(ctx, jobs, precision=mi.MI_FP32, graph=True, waves=(0,), options=()):
     """Submits jobs in waves (wave w enters after waves[w] passes) and runs
     to completion.  Returns {index: (u, v, status)}."""
     b = ctx.batch(4, precision, graph=graph)
+    for opt, val in options:
+        b.set(opt, val)
     ids = {}
     order = np.array_split(np.arange(len(jobs)), len(waves))
     passes = 0
@@ -214,6 +216,18 @@ def test_batch_entry_budget_and_max_k(ctx):
     with pytest.raises(mi.MIError):
         b.set(mi.MI_BATCH_MAX_K, 3)
     b.close()
+
+
+@pytest.mark.parametrize("precision", [mi.MI_FP32, mi.MI_FP64])
+def test_batch_bp_lds_equals_global_gathers(ctx, precision):
+    """The back projection that stages each block's residual values in LDS
+    and the one that gathers them from global memory sum the same values in
+    the same order: bitwise equal results."""
+    jobs = _jobs(ctx, precision)
+    a = _run_batch(ctx, jobs, precision)
+    b = _run_batch(ctx, jobs, precision, options=[(mi.MI_BATCH_BP_LDS, 0)])
+    for i in range(len(jobs)):
+        assert _same(a[i][0], b[i][0]) and _same(a[i][1], b[i][1]), f"job {i}"
 
 
 def test_batch_release_and_errors(ctx):
