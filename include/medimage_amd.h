@@ -53,7 +53,15 @@ enum mi_beam {
     MI_BEAM_PARALLEL = 1,
     /* GoldenC symmetric tiled DDA, origin/GoldenC/src/proj.c:14-397
      * (512 x 512 image, 180 views, 767 bins only). */
-    MI_BEAM_GOLDENC = 2
+    MI_BEAM_GOLDENC = 2,
+    /* 2D fan beam, equiangular (arc) detector of the clinical scanner: ray
+     * construction of helical/rebin/src/ms2d.cpp:50-56,172-183 (source at
+     * angle a_v = span_deg v / n_views degrees, detector element j at
+     * (j - n_bins/2 + det_offset) * bin_mm degrees + pi seen from the
+     * source), same Siddon tracer as MI_BEAM_FAN_FLAT.  Here bin_mm holds
+     * LENGTH_PER_DET (degrees per detector element), det_offset OFF_CENTER,
+     * sod_mm SOURCE_TO_ISO, sdd_mm SOURCE_TO_DET; square images, even n_bins. */
+    MI_BEAM_FAN_ARC = 3
 };
 
 enum mi_precision { MI_FP32 = 0, MI_FP64 = 1 };

@@ -25,7 +25,7 @@ _PKG = Path(__file__).resolve().parent
 LIB_PATH = Path(os.environ.get("MEDIMAGE_AMD_LIB") or str(_PKG / "libmedimage_amd.so"))
 
 MI_OK, MI_E_ARG, MI_E_DEVICE, MI_E_NOMEM, MI_E_STATE, MI_E_LIMIT = 0, -1, -2, -3, -4, -5
-MI_BEAM_FAN_FLAT, MI_BEAM_PARALLEL, MI_BEAM_GOLDENC = 0, 1, 2
+MI_BEAM_FAN_FLAT, MI_BEAM_PARALLEL, MI_BEAM_GOLDENC, MI_BEAM_FAN_ARC = 0, 1, 2, 3
 MI_FP32, MI_FP64 = 0, 1
 MI_K_FP, MI_K_BP, MI_K_FPSQ, MI_K_STEP, MI_K_EDGE = 0, 1, 2, 3, 4
 MI_BATCH_GRAPH, MI_BATCH_MIN_K, MI_BATCH_RESERVE_MB, MI_BATCH_MAX_K, MI_BATCH_MAX_ENTRIES, MI_BATCH_BP_LDS = \
@@ -83,6 +83,13 @@ class JobStatus(C.Structure):
 def fan_geometry(n, pixel_mm, views, bins, bin_mm, sod=541.0, sdd=949.0, span_deg=360.0, det_offset=0.0):
     """2D fan beam, flat equidistant detector (MI_BEAM_FAN_FLAT)."""
     return Geometry(MI_BEAM_FAN_FLAT, n, n, pixel_mm, views, span_deg, bins, bin_mm, sod, sdd, det_offset, 0, 0)
+
+
+def arc_geometry(n, pixel_mm, views, bins, deg_per_bin, sod=595.0, sdd=1085.6, off_center=0.0, span_deg=360.0):
+    """2D fan beam on an equiangular (arc) detector (MI_BEAM_FAN_ARC): the
+    helical/rebin per-slice geometry (SOURCE_TO_ISO, SOURCE_TO_DET,
+    LENGTH_PER_DET degrees per element, OFF_CENTER elements)."""
+    return Geometry(MI_BEAM_FAN_ARC, n, n, pixel_mm, views, span_deg, bins, deg_per_bin, sod, sdd, off_center, 0, 0)
 
 
 def parallel_geometry(n=512, views=180, bins=768, flavour=0):

@@ -24,7 +24,7 @@ import math
 
 import numpy as np
 
-from . import at_params, fan_geometry, phantom
+from . import arc_geometry, at_params, fan_geometry, phantom
 
 SOD, SDD = 541.0, 949.0
 FOV_MM = 358.4
@@ -112,3 +112,12 @@ def subsets_for(views, want):
 
 __all__ = ["c1_geometry", "c2_geometry", "c1_params", "c2_params", "mixed_geometry", "c3_jobs", "c5_requests",
            "make_case", "analytic_case", "rays", "math"]
+
+
+def arc_clinical_geometry(views=1152, n=874, pixel_mm=0.5859):
+    """The clinical arc-detector slice geometry of the reference's helical
+    rebinning configuration (helical/rebin/json/UCLA_PHANTOM/SCAN1_1.5mm.json:
+    874 x 874 at 0.5859 mm, 736 elements of 0.06786112 degrees, OFF_CENTER
+    1.75, SOURCE_TO_ISO 595 mm, SOURCE_TO_DET 1085.6 mm, 1152 views per turn)."""
+    return arc_geometry(n, pixel_mm, views, 736, 0.06786112, 595.0, 1085.6, 1.75)
+

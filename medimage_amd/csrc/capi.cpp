@@ -48,6 +48,14 @@ void validate_geometry(const mi_geometry &g) {
                        "fan geometry needs pixel_mm, bin_mm > 0 and sdd > sod > 0");
             MI_REQUIRE(std::isfinite(g.span_deg) && g.span_deg > 0, MI_E_ARG, "span_deg must be positive");
             break;
+        case MI_BEAM_FAN_ARC:
+            MI_REQUIRE(g.pixel_mm > 0 && g.bin_mm > 0 && g.sod_mm > 0 && g.sdd_mm > g.sod_mm, MI_E_ARG,
+                       "arc geometry needs pixel_mm, bin_mm (degrees) > 0 and sdd > sod > 0");
+            MI_REQUIRE(g.nx == g.ny && g.n_bins % 2 == 0, MI_E_ARG, "arc geometry needs a square image and even n_bins");
+            MI_REQUIRE(std::isfinite(g.span_deg) && g.span_deg > 0 && std::isfinite(g.det_offset), MI_E_ARG,
+                       "span_deg must be positive, det_offset finite");
+            MI_REQUIRE(g.bin_mm * g.n_bins < 180.0, MI_E_ARG, "the arc detector must span less than 180 degrees");
+            break;
         case MI_BEAM_PARALLEL:
             MI_REQUIRE(g.nx == g.ny, MI_E_ARG, "parallel beam needs a square image");
             MI_REQUIRE(g.flavour == 0 || g.flavour == 1, MI_E_ARG, "flavour must be 0 or 1");
@@ -64,7 +72,8 @@ void validate_geometry(const mi_geometry &g) {
 GeomKey make_key(const mi_geometry &g, int S, int layout) {
     GeomKey k;
     k.i = {g.beam, g.nx, g.ny, g.n_views, g.n_bins, g.beam == MI_BEAM_PARALLEL ? g.flavour : 0, S, layout};
-    if (g.beam == MI_BEAM_FAN_FLAT) k.d = {g.pixel_mm, g.span_deg, g.bin_mm, g.sod_mm, g.sdd_mm, g.det_offset};
+    if (g.beam == MI_BEAM_FAN_FLAT || g.beam == MI_BEAM_FAN_ARC)
+        k.d = {g.pixel_mm, g.span_deg, g.bin_mm, g.sod_mm, g.sdd_mm, g.det_offset};
     else k.d = {0, 0, 0, 0, 0, 0};
     return k;
 }

@@ -21,6 +21,7 @@ struct TraceArgs {
     int flavour;
     FanGeom fan;
     const double *fcos, *fsin;   // fan: per-view double tables
+    const double *dcos, *dsin;   // arc detector: per-bin double tables
     const float *pcos, *psin;    // parallel: per-view float tables
     const int *vlist;            // subset-ordered views
 };
@@ -29,8 +30,8 @@ template <typename Emit>
 __device__ __forceinline__ void trace_row(const TraceArgs &a, int64_t r, Emit &&emit) {
     const int view = a.vlist[r / a.nbins];
     const int bin = (int)(r % a.nbins);
-    if (a.beam == MI_BEAM_FAN_FLAT) {
-        trace_fan(a.fan, a.fcos[view], a.fsin[view], bin, emit);
+    if (a.beam == MI_BEAM_FAN_FLAT || a.beam == MI_BEAM_FAN_ARC) {
+        trace_fan(a.fan, a.fcos[view], a.fsin[view], a.dcos, a.dsin, bin, emit);
     } else if (a.flavour == 1) {
         trace_parallel<float>(a.psin[view], a.pcos[view], bin, a.nbins, a.n, emit);
     } else {
@@ -331,7 +332,7 @@ __global__ void k_otf_fill(TraceArgs a, int64_t nrows, const int64_t *rowptr, co
     const int view = a.vlist[r / a.nbins];
     const int bin = (int)(r % a.nbins);
     double sx, sy, dx, dy;
-    fan_endpoints(a.fan, a.fcos[view], a.fsin[view], bin, sx, sy, dx, dy);
+    ray_endpoints(a.fan, a.fcos[view], a.fsin[view], a.dcos, a.dsin, bin, sx, sy, dx, dy);
     SiddonState st{};
     if (L > 0) siddon_init(sx, sy, dx, dy, a.fan.nx, a.fan.ny, a.fan.vx, a.fan.vy, st);
     int64_t k = 0;
@@ -466,7 +467,7 @@ std::unique_ptr<Matrix> build_matrix(const mi_geometry &g, int S, bool want64, h
     a.n = g.nx;
     a.flavour = g.flavour;
     a.vlist = m->d_vlist.p;
-    DevBuf<double> dcos, dsin;
+    DevBuf<double> dcos, dsin, bcos, bsin;
     DevBuf<float> pcos, psin;
     if (g.beam == MI_BEAM_FAN_FLAT) {
         std::vector<double> c(g.n_views), s(g.n_views);
@@ -478,9 +479,34 @@ std::unique_ptr<Matrix> build_matrix(const mi_geometry &g, int S, bool want64, h
         }
         upload(dcos, c, st);
         upload(dsin, s, st);
-        a.fan = FanGeom{g.nx, g.ny, g.n_bins, g.pixel_mm, g.pixel_mm, g.bin_mm, g.sod_mm, g.sdd_mm, g.det_offset};
+        a.fan = FanGeom{g.nx, g.ny, g.n_bins, g.pixel_mm, g.pixel_mm, g.bin_mm, g.sod_mm, g.sdd_mm, g.det_offset, 0};
         a.fcos = dcos.p;
         a.fsin = dsin.p;
+    } else if (g.beam == MI_BEAM_FAN_ARC) {
+        // load_table of helical/rebin/src/ms2d.cpp:30-62: source angle in
+        // degrees -> cos(angle / 180 * pi); detector element j (from
+        // -n_bins/2) -> cos((j + OFF_CENTER) * LENGTH_PER_DET * pi / 180 + pi)
+        std::vector<double> c(g.n_views), s(g.n_views), dc(g.n_bins), ds(g.n_bins);
+        for (int v = 0; v < g.n_views; ++v) {
+            const double angle = g.span_deg * (double)v / (double)g.n_views;
+            c[v] = std::cos(angle / 180.0 * M_PI);
+            s[v] = std::sin(angle / 180.0 * M_PI);
+        }
+        const double lpd = g.bin_mm * M_PI / 180.;
+        const int half = g.n_bins / 2;
+        for (int j = -half; j < half; ++j) {
+            dc[j + half] = std::cos((j + g.det_offset) * lpd + M_PI);
+            ds[j + half] = std::sin((j + g.det_offset) * lpd + M_PI);
+        }
+        upload(dcos, c, st);
+        upload(dsin, s, st);
+        upload(bcos, dc, st);
+        upload(bsin, ds, st);
+        a.fan = FanGeom{g.nx, g.ny, g.n_bins, g.pixel_mm, g.pixel_mm, g.bin_mm, g.sod_mm, g.sdd_mm, g.det_offset, 1};
+        a.fcos = dcos.p;
+        a.fsin = dsin.p;
+        a.dcos = bcos.p;
+        a.dsin = bsin.p;
     } else {
         // posit() of msbeam/CPU_OpenMP/src/proj.c:55-56 (double pi, double
         // libm) or blockrecon/src/wray.cpp:11-12 (float pi, float overloads).
@@ -515,7 +541,7 @@ std::unique_ptr<Matrix> build_matrix(const mi_geometry &g, int S, bool want64, h
     MI_HIP(hipGetLastError());
     build_csc(*m, m->rowptr.p, m->idx.p, m->len32.p, want64 ? m->len64.p : nullptr, st);
     subset_counts(*m, st);
-    if (otf && g.beam == MI_BEAM_FAN_FLAT) build_otf(*m, a, st);
+    if (otf && (g.beam == MI_BEAM_FAN_FLAT || g.beam == MI_BEAM_FAN_ARC)) build_otf(*m, a, st);
     return m;
 }
 

@@ -207,6 +207,7 @@ __device__ __forceinline__ void trace_siddon2d(double sx, double sy, double dx, 
 struct FanGeom {
     int nx, ny, nbins;
     double vx, vy, bin_mm, sod, sdd, det_offset;
+    int arc;                  // 1: equiangular detector (arc_endpoints)
 };
 
 __device__ __forceinline__ void fan_rotate(double &x, double &y, double c, double s) {
@@ -232,10 +233,33 @@ __device__ __forceinline__ void fan_endpoints(const FanGeom &g, double c, double
     dy = y1 + cy;
 }
 
+// Arc-detector fan geometry, helical/rebin/src/ms2d.cpp:172-183: source at
+// angle (cos c, sin s) on the circle of radius sod around the image centre,
+// detector element at angle (dc, ds) seen from the source, distance sdd
+// (the image-centre term uses nx for both coordinates, as the reference
+// does; arc images are square).
+__device__ __forceinline__ void arc_endpoints(const FanGeom &g, double c, double s, double dc, double ds,
+                                              double &sx, double &sy, double &dx, double &dy) {
+    sx = g.sod * c + g.nx * g.vx / 2.0;
+    sy = g.sod * s + g.nx * g.vy / 2.0;
+    dx = g.sdd * (dc * c - ds * s) + sx;
+    dy = g.sdd * (ds * c + dc * s) + sy;
+}
+
+// Endpoints of fan ray (view with cos c / sin s, bin) of either detector;
+// dcos / dsin: per-bin tables of the arc detector (unused when flat).
+__device__ __forceinline__ void ray_endpoints(const FanGeom &g, double c, double s, const double *dcos,
+                                              const double *dsin, int bin, double &sx, double &sy, double &dx,
+                                              double &dy) {
+    if (g.arc) arc_endpoints(g, c, s, dcos[bin], dsin[bin], sx, sy, dx, dy);
+    else fan_endpoints(g, c, s, bin, sx, sy, dx, dy);
+}
+
 template <typename Emit>
-__device__ __forceinline__ void trace_fan(const FanGeom &g, double c, double s, int bin, Emit &&emit) {
+__device__ __forceinline__ void trace_fan(const FanGeom &g, double c, double s, const double *dcos,
+                                          const double *dsin, int bin, Emit &&emit) {
     double sx, sy, dx, dy;
-    fan_endpoints(g, c, s, bin, sx, sy, dx, dy);
+    ray_endpoints(g, c, s, dcos, dsin, bin, sx, sy, dx, dy);
     trace_siddon2d(sx, sy, dx, dy, g.nx, g.ny, g.vx, g.vy, emit);
 }
 

@@ -164,11 +164,38 @@ int or_siddon2d(double sx, double sy, double dx, double dy,
 void or_fan_tables(const or_fan_geom *g, double *cos_t, double *sin_t)
 {
     for (int a = 0; a < g->nviews; ++a) {
+        if (g->arc) {
+            /* helical/rebin/src/ms2d.cpp:44-46 (angle in degrees) */
+            double angle = g->span_deg * (double)a / (double)g->nviews;
+            cos_t[a] = cos(angle / 180.0 * 3.14159265358979323846);
+            sin_t[a] = sin(angle / 180.0 * 3.14159265358979323846);
+            continue;
+        }
         const double span = g->span_deg * (3.14159265358979323846 / 180.0);
         double th = span * (double)a / (double)g->nviews;
         cos_t[a] = cos(th);
         sin_t[a] = sin(th);
     }
+}
+
+/* Arc detector element `bin` (j = bin - nbins/2) seen from the source,
+ * helical/rebin/src/ms2d.cpp:53-57, and the ray endpoints of
+ * ms2d.cpp:172-183 (source on the circle of radius SOURCE_TO_ISO around the
+ * image centre NX*v/2, the detector SOURCE_TO_DET away from the source). */
+static void arc_endpoints(const or_fan_geom *g, double c, double s, int bin,
+                          double *sx, double *sy, double *dx, double *dy)
+{
+    const double pi = 3.14159265358979323846;
+    double lpd = g->bin_mm * pi / 180.;
+    int j = bin - g->nbins / 2;
+    double dc = cos((j + g->det_offset) * lpd + pi);
+    double ds = sin((j + g->det_offset) * lpd + pi);
+    double x0 = g->sod * c + g->nx * g->vx / 2.0;
+    double y0 = g->sod * s + g->nx * g->vy / 2.0;
+    *sx = x0;
+    *sy = y0;
+    *dx = g->sdd * (dc * c - ds * s) + x0;
+    *dy = g->sdd * (ds * c + dc * s) + y0;
 }
 
 /* rotate_2d of industry3d/src/utility.cpp:82-98 in double: the 2x2 product
@@ -185,6 +212,10 @@ void or_fan_endpoints(const or_fan_geom *g, const double *cos_t, const double *s
                       int view, int bin, double *sx, double *sy, double *dx, double *dy)
 {
     double c = cos_t[view], s = sin_t[view];
+    if (g->arc) {
+        arc_endpoints(g, c, s, bin, sx, sy, dx, dy);
+        return;
+    }
     double x0 = 0.0, y0 = -g->sod;
     double x1 = ((double)(bin - g->nbins / 2) + 0.5 + g->det_offset) * g->bin_mm;
     double y1 = g->sdd - g->sod;

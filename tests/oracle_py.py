@@ -17,7 +17,7 @@ _lib = None
 class FanGeom(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("vx", C.c_double), ("vy", C.c_double),
                 ("nviews", C.c_int), ("nbins", C.c_int), ("bin_mm", C.c_double), ("sod", C.c_double),
-                ("sdd", C.c_double), ("det_offset", C.c_double), ("span_deg", C.c_double)]
+                ("sdd", C.c_double), ("det_offset", C.c_double), ("span_deg", C.c_double), ("arc", C.c_int)]
 
 
 class Op(C.Structure):
@@ -93,10 +93,15 @@ def fan_geom(n, pixel_mm, views, bins, bin_mm, sod=541.0, sdd=949.0, span_deg=36
     return FanGeom(n, n, pixel_mm, pixel_mm, views, bins, bin_mm, sod, sdd, det_offset, span_deg)
 
 
+def arc_geom(n, pixel_mm, views, bins, deg_per_bin, sod=595.0, sdd=1085.6, off_center=0.0, span_deg=360.0):
+    """Arc (equiangular) detector geometry of helical/rebin/src/ms2d.cpp."""
+    return FanGeom(n, n, pixel_mm, pixel_mm, views, bins, deg_per_bin, sod, sdd, off_center, span_deg, 1)
+
+
 def fan_from(g):
-    """FanGeom matching a medimage_amd.Geometry (fan kind)."""
+    """FanGeom matching a medimage_amd.Geometry (flat or arc fan kind)."""
     return FanGeom(g.nx, g.ny, g.pixel_mm, g.pixel_mm, g.n_views, g.n_bins, g.bin_mm, g.sod_mm, g.sdd_mm,
-                   g.det_offset, g.span_deg)
+                   g.det_offset, g.span_deg, 1 if g.beam == 3 else 0)
 
 
 def siddon2d(sx, sy, dx, dy, nx, ny, vx, vy, cap=8192):
