@@ -302,6 +302,21 @@ int mi_stats_reset(mi_context *ctx);
  * hardware memory counters of the other kernels are read. */
 int mi_device_copy(mi_context *ctx, int64_t bytes, int reps, double *ms);
 
+/* ---- reference configuration and data files (host only, no context) ---- */
+/* Fills *g and *p from the text of one of the reference's JSON configuration
+ * files: the helical slice driver's (keys NDETECTOR, SOURCE_TO_ISO, ...;
+ * helical/rebin/src/main.cpp:62-110; gives MI_BEAM_FAN_ARC, MS_ITERATIONS as
+ * iterations) or industry3d's Parameter (industry3d/src/ct3d.cpp:20-93;
+ * BEAM "Cone" gives the mid-plane flat fan, "Parallel" the parallel beam,
+ * ITERATIONS, LOWER_X / UPPER_X).  A missing required key is MI_E_ARG, as a
+ * missing key stops the reference's parser. */
+int mi_config_parse(const char *json_text, mi_geometry *g, mi_at_params *p);
+/* Reads an industry3d projection file (industry3d/src/ct3d.cpp:199-214,
+ * industry3d/readme.txt): a 1024-byte header, nproj * ndx * ndy unsigned
+ * shorts in [proj][x][y] order into proj, then (flat != NULL) the ndx * ndy
+ * float blank scan into flat. */
+int mi_read_dr(const char *path, int64_t nproj, int64_t ndx, int64_t ndy, uint16_t *proj, float *flat);
+
 #ifdef __cplusplus
 }
 #endif

@@ -171,6 +171,8 @@ def _load():
         "mi_stats_get": [P, C.c_int, C.POINTER(KernelStats)],
         "mi_stats_reset": [P],
         "mi_device_copy": [P, C.c_int64, C.c_int, C.POINTER(C.c_double)],
+        "mi_config_parse": [C.c_char_p, G, C.POINTER(AtParams)],
+        "mi_read_dr": [C.c_char_p, C.c_int64, C.c_int64, C.c_int64, P, P],
     }
     for name, args in sigs.items():
         fn = getattr(lib, name)
@@ -184,6 +186,23 @@ def _load():
 
 def lib():
     return _load()
+
+
+def config_parse(text: str):
+    """(Geometry, AtParams) from the text of a reference JSON configuration
+    (mi_config_parse: the helical slice driver's or industry3d's)."""
+    g, p = Geometry(), AtParams()
+    _check(_load().mi_config_parse(text.encode(), C.byref(g), C.byref(p)))
+    return g, p
+
+
+def read_dr(path, nproj: int, ndx: int, ndy: int, flat: bool = True):
+    """Projections [nproj, ndx, ndy] uint16 (and the [ndx, ndy] float32 blank
+    scan when flat) of an industry3d .dr file (mi_read_dr)."""
+    proj = np.empty((nproj, ndx, ndy), np.uint16)
+    fl = np.empty((ndx, ndy), np.float32) if flat else None
+    _check(_load().mi_read_dr(str(path).encode(), nproj, ndx, ndy, _ptr(proj), _ptr(fl) if flat else None))
+    return (proj, fl) if flat else proj
 
 
 def _check(rc):

@@ -4,6 +4,9 @@
 #include "solver.hpp"
 
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
 #include <cstring>
 #include <limits>
 #include <string>
@@ -875,3 +878,209 @@ int mi_stats_reset(mi_context *ctx) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Reference configuration and data formats on the host side of the C ABI:
+// the flat JSON configuration objects of the reference's drivers and the
+// industry3d projection file (.dr).  No device work.
+
+namespace mi {
+namespace {
+
+// Top-level members of a JSON object: numbers (as double) and strings;
+// nested objects / arrays and literals are skipped.
+struct JsonObject {
+    std::map<std::string, double> num;
+    std::map<std::string, std::string> str;
+};
+
+struct JsonReader {
+    const char *p;
+    void ws() {
+        while (*p == ' ' || *p == '\t' || *p == '\n' || *p == '\r') ++p;
+    }
+    void expect(char c) {
+        ws();
+        MI_REQUIRE(*p == c, MI_E_ARG, std::string("configuration JSON: expected '") + c + "'");
+        ++p;
+    }
+    std::string string() {
+        expect('"');
+        std::string s;
+        while (*p && *p != '"') {
+            if (*p == '\\') {
+                ++p;
+                MI_REQUIRE(*p, MI_E_ARG, "configuration JSON: unterminated string");
+                const char e = *p;
+                s += e == 'n' ? '\n' : e == 't' ? '\t' : e;   // \uXXXX is not used by these files
+            } else {
+                s += *p;
+            }
+            ++p;
+        }
+        MI_REQUIRE(*p == '"', MI_E_ARG, "configuration JSON: unterminated string");
+        ++p;
+        return s;
+    }
+    void skip_value() {
+        ws();
+        if (*p == '"') {
+            string();
+        } else if (*p == '{' || *p == '[') {
+            const char open = *p, close = open == '{' ? '}' : ']';
+            int depth = 0;
+            do {
+                if (*p == '"') {
+                    string();
+                    continue;
+                }
+                if (*p == open) ++depth;
+                else if (*p == close) --depth;
+                MI_REQUIRE(*p, MI_E_ARG, "configuration JSON: unterminated object or array");
+                ++p;
+            } while (depth > 0);
+        } else {
+            while (*p && *p != ',' && *p != '}' && *p != ']' && *p != ' ' && *p != '\n' && *p != '\r' && *p != '\t')
+                ++p;
+        }
+    }
+    JsonObject object() {
+        JsonObject o;
+        expect('{');
+        ws();
+        if (*p == '}') {
+            ++p;
+            return o;
+        }
+        for (;;) {
+            const std::string key = string();
+            expect(':');
+            ws();
+            if (*p == '"') {
+                o.str[key] = string();
+            } else if (*p == '-' || (*p >= '0' && *p <= '9')) {
+                char *end = nullptr;
+                const double v = std::strtod(p, &end);
+                MI_REQUIRE(end != p, MI_E_ARG, "configuration JSON: bad number for " + key);
+                o.num[key] = v;
+                p = end;
+            } else {
+                skip_value();
+            }
+            ws();
+            if (*p == ',') {
+                ++p;
+                continue;
+            }
+            expect('}');
+            return o;
+        }
+    }
+};
+
+double need(const JsonObject &o, const char *k) {
+    auto it = o.num.find(k);
+    MI_REQUIRE(it != o.num.end(), MI_E_ARG, std::string("configuration JSON lacks the number ") + k);
+    return it->second;
+}
+double opt(const JsonObject &o, const char *k, double d) {
+    auto it = o.num.find(k);
+    return it == o.num.end() ? d : it->second;
+}
+
+}  // namespace
+}  // namespace mi
+
+
+int mi_config_parse(const char *json, mi_geometry *g, mi_at_params *p) {
+    return guarded([&] {
+        MI_REQUIRE(json && g && p, MI_E_ARG, "null argument");
+        JsonReader r{json};
+        const JsonObject o = r.object();
+        *g = mi_geometry{};
+        *p = mi_at_params{};
+        p->iter_edge = 1;
+        p->n_subsets = 1;
+        p->lower = -INFINITY;
+        p->upper = INFINITY;
+        p->precision = MI_FP32;
+        if (o.num.count("NDETECTOR")) {
+            // helical/rebin/src/main.cpp:85-106: a slice of the arc-detector
+            // scanner; vx is read from "vy" and vy from "vx" there
+            const double vx = need(o, "vy"), vy = need(o, "vx");
+            MI_REQUIRE(vx == vy, MI_E_ARG, "the arc geometry needs square pixels (vx == vy)");
+            g->beam = MI_BEAM_FAN_ARC;
+            g->nx = (int)need(o, "NX");
+            g->ny = (int)need(o, "NY");
+            g->pixel_mm = vx;
+            g->n_views = (int)need(o, "NPROJ");
+            g->span_deg = 360.0;
+            g->n_bins = (int)need(o, "NDETECTOR");
+            g->bin_mm = need(o, "LENGTH_PER_DET");
+            g->sod_mm = need(o, "SOURCE_TO_ISO");
+            g->sdd_mm = need(o, "SOURCE_TO_DET");
+            g->det_offset = need(o, "OFF_CENTER");
+            p->alpha = need(o, "ALPHA");
+            p->beta = need(o, "BETA");
+            p->epsilon = need(o, "EPSILON");
+            p->max_iters = p->iter_alt = (int)need(o, "MS_ITERATIONS");
+        } else {
+            // industry3d Parameter::parse_config, industry3d/src/ct3d.cpp:20-93;
+            // the 2D path takes the mid-plane of the cone ("Cone") or the
+            // parallel beam ("Parallel")
+            auto b = o.str.find("BEAM");
+            MI_REQUIRE(b != o.str.end(), MI_E_ARG, "configuration JSON lacks BEAM");
+            const std::string beam = b->second;
+            g->nx = (int)need(o, "NX");
+            g->ny = (int)need(o, "NY");
+            g->n_views = (int)need(o, "NPROJ");
+            g->n_bins = (int)need(o, "NDX");
+            if (beam == "Cone" || beam == "cone") {
+                g->beam = MI_BEAM_FAN_FLAT;
+                g->pixel_mm = need(o, "SAMPLESIZE");
+                g->bin_mm = need(o, "PIXELSIZE");
+                g->sod_mm = need(o, "SOD");
+                g->sdd_mm = need(o, "SDD");
+                g->span_deg = 360.0;
+                g->det_offset = opt(o, "OFF_CENTER", 0.0);
+            } else if (beam == "Parallel" || beam == "parallel") {
+                g->beam = MI_BEAM_PARALLEL;
+                g->pixel_mm = 1.0;
+                g->bin_mm = 1.0;
+                g->span_deg = 180.0;
+            } else {
+                fail(MI_E_ARG, "unknown BEAM " + beam);
+            }
+            p->alpha = need(o, "ALPHA");
+            p->beta = need(o, "BETA");
+            p->epsilon = need(o, "EPSILON");
+            p->max_iters = p->iter_alt = (int)need(o, "ITERATIONS");
+            p->lower = opt(o, "LOWER_X", -INFINITY);
+            p->upper = opt(o, "UPPER_X", INFINITY);
+        }
+        validate_geometry(*g);
+    });
+}
+
+int mi_read_dr(const char *path, int64_t nproj, int64_t ndx, int64_t ndy, uint16_t *proj, float *flat) {
+    return guarded([&] {
+        MI_REQUIRE(path && proj && nproj > 0 && ndx > 0 && ndy > 0, MI_E_ARG, "bad .dr arguments");
+        FILE *f = std::fopen(path, "rb");
+        MI_REQUIRE(f, MI_E_ARG, std::string("cannot open ") + path);
+        struct Closer {
+            FILE *f;
+            ~Closer() { std::fclose(f); }
+        } closer{f};
+        const int64_t n = nproj * ndx * ndy, nf = ndx * ndy;
+        MI_REQUIRE(std::fseek(f, 0, SEEK_END) == 0, MI_E_ARG, "cannot size the .dr file");
+        const int64_t size = (int64_t)std::ftell(f);
+        const int64_t need_bytes = 1024 + n * 2 + (flat ? nf * 4 : 0);
+        MI_REQUIRE(size >= need_bytes, MI_E_ARG,
+                   ".dr file shorter than its 1024-byte header, the projections and the flat field asked for");
+        // industry3d/src/ct3d.cpp:199-214: projections after a 1024-byte header
+        MI_REQUIRE(std::fseek(f, 1024, SEEK_SET) == 0, MI_E_ARG, "cannot seek past the .dr header");
+        MI_REQUIRE((int64_t)std::fread(proj, 2, (size_t)n, f) == n, MI_E_ARG, "short read of .dr projections");
+        if (flat)
+            MI_REQUIRE((int64_t)std::fread(flat, 4, (size_t)nf, f) == nf, MI_E_ARG, "short read of .dr flat field");
+    });
+}

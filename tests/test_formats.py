@@ -1,0 +1,66 @@
+"""CPU tests of the reference configuration and data formats on the C ABI
+(mi_config_parse, mi_read_dr): the reference's own JSON files (kept under
+tests/golden/) map to the geometry and parameters field by field, and a
+synthetic .dr file reads back exactly."""
+import math
+
+import numpy as np
+import pytest
+
+import medimage_amd as mi
+
+GOLD = __import__("pathlib").Path(__file__).parent / "golden"
+
+
+def test_helical_slice_config():
+    g, p = mi.config_parse((GOLD / "helical_scan1_1.5mm.json").read_text())
+    assert g.beam == mi.MI_BEAM_FAN_ARC
+    assert (g.nx, g.ny, g.n_views, g.n_bins) == (874, 874, 1152, 736)
+    assert (g.pixel_mm, g.bin_mm, g.sod_mm, g.sdd_mm, g.det_offset) == (0.5859, 0.06786112, 595.0, 1085.6, 1.75)
+    assert g.span_deg == 360.0
+    assert (p.alpha, p.beta, p.epsilon, p.max_iters) == (0.1, 0.01, 0.001, 3)
+    # the configuration describes the clinical slice geometry of the bench
+    from medimage_amd import configs
+    c = configs.arc_clinical_geometry()
+    assert bytes(c) == bytes(g)
+
+
+def test_industry3d_configs():
+    g, p = mi.config_parse((GOLD / "industry3d_cone_taiji.json").read_text())
+    assert g.beam == mi.MI_BEAM_FAN_FLAT
+    assert (g.nx, g.ny, g.n_views, g.n_bins) == (128, 128, 180, 256)
+    assert (g.pixel_mm, g.bin_mm, g.sod_mm, g.sdd_mm) == (1.0, 1.0, 450.0, 836.70001221)
+    assert (p.alpha, p.beta, p.epsilon, p.max_iters, p.lower, p.upper) == (2.0, 0.001, 0.125, 10, 0.0, 10.0)
+    g, p = mi.config_parse((GOLD / "industry3d_parallel_head.json").read_text())
+    assert g.beam == mi.MI_BEAM_PARALLEL and (g.nx, g.n_views, g.n_bins) == (256, 360, 256)
+    assert p.alpha == 10.0 and p.max_iters == 10
+
+
+def test_config_errors():
+    text = (GOLD / "industry3d_cone_taiji.json").read_text()
+    with pytest.raises(mi.MIError) as e:
+        mi.config_parse(text.replace('"SOD"', '"SOD_X"'))
+    assert e.value.code == mi.MI_E_ARG and "SOD" in str(e.value)
+    with pytest.raises(mi.MIError):
+        mi.config_parse(text.replace('"Cone"', '"Helix"'))
+    with pytest.raises(mi.MIError):
+        mi.config_parse("{ \"NX\" : 12,")
+    # nested members and literals are skipped
+    g, _ = mi.config_parse(text.replace('"MODEL"', '"X"').replace('{', '{ "nested" : {"a": [1, {"b": "}"}]}, '
+                                                                   '"flag": true,', 1))
+    assert g.nx == 128
+
+
+def test_read_dr_roundtrip(tmp_path):
+    rs = np.random.RandomState(5)
+    proj = rs.randint(0, 65535, size=(3, 5, 7)).astype(np.uint16)
+    flat = rs.rand(5, 7).astype(np.float32)
+    f = tmp_path / "x.dr"
+    f.write_bytes(bytes(rs.randint(0, 255, 1024).astype(np.uint8)) + proj.tobytes() + flat.tobytes())
+    p2, f2 = mi.read_dr(f, 3, 5, 7)
+    assert np.array_equal(p2, proj) and np.array_equal(f2, flat)
+    assert np.array_equal(mi.read_dr(f, 3, 5, 7, flat=False), proj)
+    with pytest.raises(mi.MIError):
+        mi.read_dr(f, 4, 5, 7)          # file too short
+    with pytest.raises(mi.MIError):
+        mi.read_dr(tmp_path / "missing.dr", 1, 1, 1)
