@@ -380,8 +380,9 @@ void Batch<T>::rebuild() {
     size_t lds_need = 0;
     edge_iters = 0;
     any_median = false;
-    rows.nunits = bp.nunits = pix.nunits = (int)live.size();
-    rows.start[0] = bp.start[0] = pix.start[0] = 0;
+    rows.nunits = bpl.nunits = bpg.nunits = pix.nunits = (int)live.size();
+    rows.start[0] = bpl.start[0] = bpg.start[0] = pix.start[0] = 0;
+    kmax_l = kmax_g = 1;
     const bool f64 = sizeof(T) == 8;
     for (size_t i = 0; i < live.size(); ++i) {
         Unit<T> &U = *units[live[i]];
@@ -418,7 +419,14 @@ void Batch<T>::rebuild() {
         D.bs_voff = m.bs.voff.p;
         D.bs_vtab0 = m.bs.d_vtab0.p;
         D.nbins = m.g.n_bins;
-        lds_need = std::max(lds_need, m.bs.slot16.p ? (size_t)m.bs.max_slots * U.K * sizeof(T) : kBpLdsMax + 1);
+        const size_t need = m.bs.slot16.p ? (size_t)m.bs.max_slots * U.K * sizeof(T) : kBpLdsMax + 1;
+        const bool in_lds = use_bp_lds && need <= kBpLdsMax;
+        if (in_lds) {
+            lds_need = std::max(lds_need, need);
+            kmax_l = std::max(kmax_l, U.K);
+        } else {
+            kmax_g = std::max(kmax_g, U.K);
+        }
         D.sub_row0 = m.d_sub_row0.p;
         D.g = U.g.p;
         D.r = U.r.p;
@@ -440,7 +448,8 @@ void Batch<T>::rebuild() {
         D.job0 = U.uid * kMaxK;
         MI_REQUIRE(m.has_sell && (!f64 || m.has64), MI_E_STATE, "matrix lacks the engine's sliced layout");
         rows.start[i + 1] = rows.start[i] + (int)fp_blocks(m);
-        bp.start[i + 1] = bp.start[i] + (int)bp_blocks(m);
+        bpl.start[i + 1] = bpl.start[i] + (in_lds ? (int)bp_blocks(m) : 0);
+        bpg.start[i + 1] = bpg.start[i] + (in_lds ? 0 : (int)bp_blocks(m));
         pix.start[i + 1] = pix.start[i] + (int)px_blocks(m);
         kmax = std::max(kmax, U.K);
         all_walk = all_walk && m.has_otf;
@@ -451,7 +460,7 @@ void Batch<T>::rebuild() {
             any_median = any_median || P.median_every > 0;
         }
     }
-    bp_lds = (use_bp_lds && lds_need <= kBpLdsMax) ? lds_need : 0;
+    bp_lds = lds_need;
     MI_HIP(hipStreamSynchronize(st));
     if ((int64_t)d_part.n < off) d_part.alloc((size_t)std::max<int64_t>(off, 1));
     if (!desc.empty())
@@ -493,7 +502,8 @@ void Batch<T>::record_pass(bool timed) {
             LaunchTimer lt(ctx, MI_K_BP, sum([&](const Unit<T> &U) {
                 return nnz_s(U) * (4 + sz) + nr_s(U) * sz * U.K + 3.0 * U.m->npix * sz * U.K;
             }));
-            eng_bp_grad<T>(st, args, bp, s, kmax, bp_lds);
+            if (bpl.start[bpl.nunits] > 0) eng_bp_grad<T>(st, args, bpl, s, kmax_l, bp_lds);
+            if (bpg.start[bpg.nunits] > 0) eng_bp_grad<T>(st, args, bpg, s, kmax_g, 0);
         }
         {
             LaunchTimer lt(ctx, MI_K_FPSQ, sum([&](const Unit<T> &U) {

@@ -85,7 +85,12 @@ struct Batch {
     std::vector<JobParams> h_jp;
     std::vector<Scalars> h_sc;
     std::vector<int> live;         // uids in table order
-    BlockMap rows{}, bp{}, pix{};
+    BlockMap rows{}, pix{};
+    // back projection: units whose residual slots fit kBpLdsMax in the LDS
+    // launch (bpl), the others in the global-gather launch (bpg); a unit
+    // has blocks in exactly one of the two maps
+    BlockMap bpl{}, bpg{};
+    int kmax_l = 1, kmax_g = 1;
     EngArgs<T> args{};
     int kmax = 1;
     bool all_walk = true;   // every unit's forward projector is the checkpointed walk

@@ -1146,6 +1146,18 @@ template <typename T>
 void eng_bp_grad(hipStream_t st, const EngArgs<T> &a, const BlockMap &m, int s, int kmax, size_t lds_bytes) {
     const unsigned grid = grid8(m.start[m.nunits]);
     if (lds_bytes) {
+        if (kBpLdsMax > (64 << 10)) {   // dynamic LDS above 64 KiB is opted into once per kernel
+            static bool once = [] {
+                MI_HIP(hipFuncSetAttribute((const void *)k_eng_bp<T, 1, true>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kBpLdsMax));
+                MI_HIP(hipFuncSetAttribute((const void *)k_eng_bp<T, 4, true>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kBpLdsMax));
+                MI_HIP(hipFuncSetAttribute((const void *)k_eng_bp<T, kMaxK, true>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kBpLdsMax));
+                return true;
+            }();
+            (void)once;
+        }
         if (kmax == 1) k_eng_bp<T, 1, true><<<grid, kBlock, lds_bytes, st>>>(a, m, s);
         else if (kmax <= 4) k_eng_bp<T, 4, true><<<grid, kBlock, lds_bytes, st>>>(a, m, s);
         else k_eng_bp<T, kMaxK, true><<<grid, kBlock, lds_bytes, st>>>(a, m, s);

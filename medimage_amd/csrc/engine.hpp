@@ -19,9 +19,16 @@ constexpr int kMaxK = 8;             // jobs per unit
 constexpr int kMaxUnits = 64;        // units per batch
 constexpr int kPixBlock = kBlock;    // tiled pixel positions per stencil block
 // Largest dynamic LDS of a back-projection block (the staged residual
-// slots); above it the launch gathers from global memory.  64 KiB keeps two
-// blocks per CU at the largest size.
-constexpr size_t kBpLdsMax = 64 << 10;
+// slots); units above it run in the global-gather launch.  Measured: the
+// staged form wins where the slots are few (512^2 x 720 views, K = 1:
+// ~8 KiB, 48.5 -> 42.5 us per launch); for K = 4 units of 1152-view
+// geometries (tens of KiB) the copy and the lower occupancy cost more than
+// the gathers save (64-job batch: 2632 it/s all global, 2256 with 64 KiB,
+// 1851 with 96 KiB, 2627 with 16 KiB).
+#ifndef MI_BP_LDS_KB
+#define MI_BP_LDS_KB 24
+#endif
+constexpr size_t kBpLdsMax = (size_t)MI_BP_LDS_KB << 10;
 
 // Partial-sum slots of a unit (each K * cap doubles, see UnitDesc).
 enum EngSlot { SL_DF = 0, SL_DD, SL_AD, SL_NV, SL_E0, SL_E1, SL_E2, SL_E3, SL_COUNT };
