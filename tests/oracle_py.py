@@ -269,6 +269,24 @@ def os_at(op, g, alpha, beta, eps, S, iters, iter_edge=1, rel_tol=0.0, median_ev
     return u, v, n, hist[:n].copy()
 
 
+_cpu = None
+
+
+def cpu_os_at_f32(op, g, alpha, beta, eps, S, iters, iter_edge=1, threads=1):
+    """The multi-threaded FP32 CPU baseline (oracle/cpu_at.c) of the same
+    iteration as os_at; returns (u, v, passes)."""
+    global _cpu
+    if _cpu is None:
+        _cpu = C.CDLL(str(LIB.parent / "libcpu_at.so"))
+        _cpu.cpu_os_at_f32.restype = C.c_int
+    g = np.ascontiguousarray(g, np.float32)
+    u = np.zeros((op.h, op.w), np.float32)
+    v = np.ones((op.h, op.w), np.float32)
+    p = AtParams(alpha, beta, eps, S, iters, iter_edge, 0, 0.0, -math.inf, math.inf)
+    n = _cpu.cpu_os_at_f32(C.byref(op), P(g), P(u), P(v), C.byref(p), int(threads))
+    return u, v, n
+
+
 def msbeam_solve(g, n, alpha, beta, eps, iters, flavour=0):
     g = np.ascontiguousarray(g, np.float32)
     f = np.zeros(n * n, np.float32)

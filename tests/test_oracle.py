@@ -482,3 +482,23 @@ def test_msbeam_matches_reference_fixture(golden):
     f, v = O.msbeam_solve(z["g"], 512, 10.0, 0.35, 0.001, 10, 0)
     assert np.array_equal(f.view(np.uint32), z["f"].view(np.uint32))
     assert np.array_equal(v.view(np.uint32), z["v"].view(np.uint32))
+
+
+def test_cpu_baseline_matches_oracle():
+    """The multi-threaded FP32 CPU baseline (oracle/cpu_at.c, what the
+    benchmark's CPU leg times) runs the same iteration as the FP64 oracle:
+    within FP32 rounding of it, and independent of the thread count."""
+    fg = O.fan_geom(64, 4.0, 60, 96, 4.0)
+    rs = np.random.RandomState(8)
+    img = np.zeros((64, 64))
+    img[20:44, 16:40] = 0.02
+    g = O.fan_fp(fg, img) + rs.normal(0, 1e-3, (60, 96))
+    op = O.op_fan(fg)
+    ou, ov, n, _ = O.os_at(op, g, 1e-3, 1e-4, 1e-2, 4, 3)
+    u1, v1, n1 = O.cpu_os_at_f32(op, g, 1e-3, 1e-4, 1e-2, 4, 3, threads=1)
+    u4, v4, n4 = O.cpu_os_at_f32(op, g, 1e-3, 1e-4, 1e-2, 4, 3, threads=4)
+    assert n1 == n4 == n == 3
+    assert np.linalg.norm(u1 - ou) / np.linalg.norm(ou) < 1e-5
+    assert np.linalg.norm(v1 - ov) / np.linalg.norm(ov) < 1e-5
+    assert np.linalg.norm(u4 - u1) / np.linalg.norm(u1) < 1e-5
+
