@@ -4,16 +4,21 @@
 // is first renumbered so that each of the 8 XCDs owns a contiguous range
 // (xcd_block), then mapped to (unit, local block) through the BlockMap passed
 // by value.  K, the number of jobs of a unit, selects a code path compiled
-// for that K; a launch whose units all have K = 1 uses a kernel compiled for
-// K = 1 alone (plain form), so it keeps the register budget of that case.
+// for that K; a launch uses the kernel compiled for K <= 1, <= 4 or <= 8
+// that covers its units, so it keeps the register budget of that case.
 //
-// Projectors read the sliced matrices (Sell, runtime.hpp): a lane owns one
-// segment of one ray (forward) or of one pixel column (back) and walks it in
-// entry order with a Kahan sum, each step of a wave reading one contiguous
-// 256-byte row of the index and length streams; neighbouring lanes own
+// Forward projection of fan beams is the checkpointed walk (walk_otf): the
+// tracer re-run from stored per-segment states, a block holding 32
+// neighbouring rays x 8 segments.  Back projection (and the forward
+// projection of other geometries) reads the sliced matrices (Sell,
+// runtime.hpp): a lane owns one pixel column (back) or one segment of one
+// ray (forward) and walks it in entry order with a Kahan sum, four entries
+// per 16-byte load, a wave's loads contiguous; neighbouring lanes own
 // neighbouring rays / pixels, so one gather instruction touches few cache
-// lines.  Images are stored in 4 x 8 tiles (tile_index).  The P segment sums
-// of an item are combined by a fixed butterfly.
+// lines.  The back projection stages each 16 x 16 pixel block's residual
+// values in LDS first where they fit (bp_unit).  Images are stored in 4 x 8
+// tiles (tile_index).  Partial sums of an item are combined in a fixed
+// order.
 #include "engine.hpp"
 #include "tracers.hpp"
 

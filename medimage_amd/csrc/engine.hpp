@@ -58,8 +58,8 @@ struct UnitDesc {
     const uint16_t *bs_slot16;             // LDS slot per entry (null: global gathers only)
     const int32_t *bs_vlo, *bs_voff;       // per (subset, block, view): first bin, slot offset
     const int64_t *bs_vtab0;               // S + 1 table offsets
-    // checkpointed forward walk (fan beam; null otherwise): per lane of a
-    // forward slice of 8 rays x 8 segments
+    // checkpointed forward walk (fan beams; null otherwise): per lane of a
+    // forward block of kOtfRays rays x kOtfSeg segments (runtime.hpp)
     const double *ot_lam, *ot_ax, *ot_ay, *ot_stx, *ot_sty;
     const int32_t *ot_cx, *ot_cy, *ot_sg, *ot_cnt;
     const int64_t *sub_row0;               // S + 1 row offsets

@@ -242,17 +242,15 @@ void compact_sell(Sell &sl, hipStream_t st) {
     }
 }
 
-// Lanes per item (measured on the 512^2 / 720-view workload).  Forward: 8,
-// so a wave step covers 8 neighbouring rays x 8 consecutive entries, an
-// 8 x 8 patch of pixels (60 us per subset against 76 us at 4 and 73 us for
-// contiguous segments).  Back: 1, a lane walks a whole pixel column while the
-// wave covers 64 neighbouring pixels (60 us; 82 us at 8).  Fewer when items
-// are short.
+// Lanes per forward item (stored forward slices: geometries without the
+// checkpointed walk).  8, so a wave step covers 8 neighbouring rays x 8
+// consecutive entries, an 8 x 8 patch of pixels (measured on the 512^2 /
+// 720-view fan workload before the walk replaced it there: 60 us per subset
+// against 76 us at 4 and 73 us for contiguous segments).  Fewer when rays
+// are short.  Back slices always have one lane per pixel (a lane walks its
+// pixel's column; 60 us against 82 us at 8 lanes in the same measurement).
 #ifndef MI_SELL_PF
 #define MI_SELL_PF 8
-#endif
-#ifndef MI_SELL_PB
-#define MI_SELL_PB 1
 #endif
 int choose_p(int P, double mean_len) {
     while (P > 1 && mean_len / P < 4.0) P /= 2;
