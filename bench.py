@@ -51,7 +51,7 @@ def cpu_baseline(sino, passes):
                       f"FP32 CPU port with OpenMP over rays, {threads} threads, {dt:.1f} s"}
 
 
-def bench_c3(ctx, n_jobs, iters, log):
+def bench_c3(ctx, n_jobs, iters, log, plain_pass=True):
     """Batched form: n_jobs mixed fan-beam slices (sizes, views, bins and I0
     drawn from the seed), 8 subsets, `iters` AT iterations each, run (a) one
     at a time through the plain per-job form and (b) all together in one
@@ -76,11 +76,13 @@ def bench_c3(ctx, n_jobs, iters, log):
     rays = sum(g.n_views * g.n_bins for g, _, _ in jobs)
 
     # (a) plain: one job at a time (each a batch of one; graph replay).
-    ctx.at_reconstruct(jobs[0][0], jobs[0][1], configs.at_params(n_subsets=8, max_iters=2))  # warm
-    t0 = time.perf_counter()
-    plain = [ctx.at_reconstruct(g, s, p) for g, s, _ in jobs]
-    t_plain = time.perf_counter() - t0
-    log(f"[c3] plain: {n_jobs} x {iters} iterations in {t_plain:.2f} s")
+    plain, t_plain = None, None
+    if plain_pass:
+        ctx.at_reconstruct(jobs[0][0], jobs[0][1], configs.at_params(n_subsets=8, max_iters=2))  # warm
+        t0 = time.perf_counter()
+        plain = [ctx.at_reconstruct(g, s, p) for g, s, _ in jobs]
+        t_plain = time.perf_counter() - t0
+        log(f"[c3] plain: {n_jobs} x {iters} iterations in {t_plain:.2f} s")
 
     # (b) batched: every job in one batch.
     b = ctx.batch(8)
@@ -93,7 +95,7 @@ def bench_c3(ctx, n_jobs, iters, log):
     log(f"[c3] batched: {len(units)} units K={sorted(units)}; {t_batch:.2f} s")
     same = 0
     worst = 0.0
-    for (u0, v0, r0), jid in zip(plain, ids):
+    for (u0, v0, r0), jid in zip(plain or [], ids):
         u1, v1 = b.read(jid)
         if np.array_equal(u0.view(np.uint32), u1.view(np.uint32)) and np.array_equal(v0.view(np.uint32), v1.view(np.uint32)):
             same += 1
@@ -105,12 +107,12 @@ def bench_c3(ctx, n_jobs, iters, log):
                     f"{{512,736}}, I0 U[1e4,1e5], 8 subsets, {iters} AT iterations each",
         "geometries": len(geoms),
         "units_K": sorted(units),
-        "plain_iters_per_s": work / t_plain,
+        "plain_iters_per_s": work / t_plain if plain_pass else None,
         "batched_iters_per_s": work / t_batch,
-        "speedup": t_plain / t_batch,
-        "plain_rays_per_s": rays * iters / t_plain,
+        "speedup": t_plain / t_batch if plain_pass else None,
+        "plain_rays_per_s": rays * iters / t_plain if plain_pass else None,
         "batched_rays_per_s": rays * iters / t_batch,
-        "jobs_bitwise_identical": same,
+        "jobs_bitwise_identical": same if plain_pass else None,
         "max_rel_diff": worst,
     }
 
@@ -166,6 +168,8 @@ def main():
     ap.add_argument("--c3-jobs", type=int, default=64)
     ap.add_argument("--c3-iters", type=int, default=50)
     ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--c3-batched-only", action="store_true",
+                    help="run the batched form of the 64-job workload without the one-at-a-time pass (profiling)")
     ap.add_argument("--serve-jobs", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--no-serve", action="store_true")
@@ -276,7 +280,7 @@ def main():
     if not args.no_c3:
         job.close()
         out["batched"] = bench_c3(ctx, args.c3_jobs, args.c3_iters,
-                                  lambda m: print(m, file=sys.stderr, flush=True))
+                                  lambda m: print(m, file=sys.stderr, flush=True), not args.c3_batched_only)
     if not args.no_serve:
         job.close()
         out["serving"] = bench_serving(ctx, args.serve_jobs, args.serve_rate,
