@@ -281,6 +281,8 @@ enum mi_kernel_id {
     MI_K_FPSQ = 2,        /* forward projection of the direction, ||A d||^2  */
     MI_K_STEP = 3,        /* line-search step + update                       */
     MI_K_EDGE = 4,        /* edge (v) step kernels                           */
+    MI_K_CONE_FP = 5,     /* 3D cone-beam forward projection (mi_cone_*)     */
+    MI_K_CONE_BP = 6,     /* 3D cone-beam gather back projection             */
     MI_K_COUNT = 8
 };
 typedef struct mi_kernel_stats {
@@ -316,6 +318,44 @@ int mi_config_parse(const char *json_text, mi_geometry *g, mi_at_params *p);
  * shorts in [proj][x][y] order into proj, then (flat != NULL) the ndx * ndy
  * float blank scan into flat. */
 int mi_read_dr(const char *path, int64_t nproj, int64_t ndx, int64_t ndy, uint16_t *proj, float *flat);
+
+/* ---- 3D cone beam (industry3d) ------------------------------------------ */
+/* Geometry of industry3d's Parameter (industry3d/include/ct3d.h:17-63) for
+ * BEAM "Cone": volume [nz][nx][ny] (ny fastest, get_img_addr of
+ * industry3d/src/tracing.cpp:119-121) of sample_mm voxels; n_views views
+ * over 360 degrees; a flat panel of n_rows (NDX, along z) by n_cols (NDY)
+ * pixels of pixel_mm.  Both panel axes are centred on HALFDET = NDX / 2, as
+ * industry3d/src/tracing.cpp:51-81 and ct3d.cpp:88-93 place them.  The
+ * sinogram is [view][row][col] (CTInput's sino order, ct3d.h:65-105). */
+typedef struct mi_cone_geometry {
+    int32_t nx, ny, nz;        /* NX, NY, NZ                                  */
+    int32_t n_views;           /* NPROJ                                       */
+    int32_t n_rows, n_cols;    /* NDX, NDY                                    */
+    float   sample_mm;         /* SAMPLESIZE                                  */
+    float   pixel_mm;          /* PIXELSIZE                                   */
+    float   sod_mm, sdd_mm;    /* SOD, SDD                                    */
+} mi_cone_geometry;
+
+/* Replaces get_tracing + forward_proj (industry3d/src/main.cpp:58-73,
+ * industry3d/src/tracing.cpp:51-81,123-307): the (voxel index, length)
+ * sequence of ray (view, row, col) as the device traces it, in float with
+ * the reference's operation order; up to cap entries, *numb = full count. */
+int mi_cone_trace_ray(mi_context *ctx, const mi_cone_geometry *g, int view, int row, int col, int64_t *ind,
+                      float *len, int cap, int *numb);
+/* A: sino[view][row][col] = sum over the ray's traced entries of
+ * vol[idx] * len (the accum_Af of industry3d/src/main.cpp:254-266), summed
+ * in double in walk order and rounded once. */
+int mi_cone_forward(mi_context *ctx, const mi_cone_geometry *g, const float *vol, float *sino);
+int mi_cone_forward_dev(mi_context *ctx, const mi_cone_geometry *g, const float *vol, float *sino);
+/* Back projection in gather form: vol[z][x][y] = sum over the rays that
+ * cross voxel (x, y, z) of sino * (exact length of the ray inside the
+ * voxel), each voxel summed by one lane in a fixed order (no atomics).  The
+ * reference applies the transpose by scattering along the traced rays
+ * (industry3d/src/main.cpp:385-406); the lengths here are the exact ones
+ * of the same rays, which the float tracer approximates (it also drops a
+ * final piece shorter than 0.05 voxel, tracing.cpp:268). */
+int mi_cone_backproject(mi_context *ctx, const mi_cone_geometry *g, const float *sino, float *vol);
+int mi_cone_backproject_dev(mi_context *ctx, const mi_cone_geometry *g, const float *sino, float *vol);
 
 #ifdef __cplusplus
 }

@@ -27,7 +27,7 @@ LIB_PATH = Path(os.environ.get("MEDIMAGE_AMD_LIB") or str(_PKG / "libmedimage_am
 MI_OK, MI_E_ARG, MI_E_DEVICE, MI_E_NOMEM, MI_E_STATE, MI_E_LIMIT = 0, -1, -2, -3, -4, -5
 MI_BEAM_FAN_FLAT, MI_BEAM_PARALLEL, MI_BEAM_GOLDENC, MI_BEAM_FAN_ARC = 0, 1, 2, 3
 MI_FP32, MI_FP64 = 0, 1
-MI_K_FP, MI_K_BP, MI_K_FPSQ, MI_K_STEP, MI_K_EDGE = 0, 1, 2, 3, 4
+MI_K_FP, MI_K_BP, MI_K_FPSQ, MI_K_STEP, MI_K_EDGE, MI_K_CONE_FP, MI_K_CONE_BP = 0, 1, 2, 3, 4, 5, 6
 MI_BATCH_GRAPH, MI_BATCH_MIN_K, MI_BATCH_RESERVE_MB, MI_BATCH_MAX_K, MI_BATCH_MAX_ENTRIES, MI_BATCH_BP_LDS = \
     0, 1, 2, 3, 4, 5
 MI_JOB_PENDING, MI_JOB_ACTIVE, MI_JOB_DONE, MI_JOB_FAILED = 0, 1, 2, 3
@@ -46,6 +46,24 @@ class Geometry(C.Structure):
 
     def __repr__(self):
         return "Geometry(" + ", ".join(f"{f}={getattr(self, f)}" for f, _ in self._fields_[:-1]) + ")"
+
+
+class ConeGeometry(C.Structure):
+    """industry3d Parameter for BEAM "Cone" (industry3d/include/ct3d.h:17-63):
+    volume [nz][nx][ny], n_views over 360 degrees, panel n_rows (NDX) x
+    n_cols (NDY); sinogram [view][row][col]."""
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("n_views", C.c_int32),
+        ("n_rows", C.c_int32), ("n_cols", C.c_int32), ("sample_mm", C.c_float), ("pixel_mm", C.c_float),
+        ("sod_mm", C.c_float), ("sdd_mm", C.c_float),
+    ]
+
+    def __repr__(self):
+        return "ConeGeometry(" + ", ".join(f"{f}={getattr(self, f)}" for f, _ in self._fields_) + ")"
+
+
+def cone_geometry(nx, ny, nz, views, rows, cols, sample_mm, pixel_mm, sod, sdd):
+    return ConeGeometry(nx, ny, nz, views, rows, cols, sample_mm, pixel_mm, sod, sdd)
 
 
 class AtParams(C.Structure):
@@ -126,6 +144,7 @@ def _load():
     lib = C.CDLL(str(LIB_PATH))
     P = C.c_void_p
     G = C.POINTER(Geometry)
+    CG = C.POINTER(ConeGeometry)
     sigs = {
         "mi_context_create": [C.c_int, C.POINTER(C.c_void_p)],
         "mi_context_destroy": [P],
@@ -173,6 +192,11 @@ def _load():
         "mi_device_copy": [P, C.c_int64, C.c_int, C.POINTER(C.c_double)],
         "mi_config_parse": [C.c_char_p, G, C.POINTER(AtParams)],
         "mi_read_dr": [C.c_char_p, C.c_int64, C.c_int64, C.c_int64, P, P],
+        "mi_cone_trace_ray": [P, CG, C.c_int, C.c_int, C.c_int, P, P, C.c_int, C.POINTER(C.c_int)],
+        "mi_cone_forward": [P, CG, P, P],
+        "mi_cone_forward_dev": [P, CG, P, P],
+        "mi_cone_backproject": [P, CG, P, P],
+        "mi_cone_backproject_dev": [P, CG, P, P],
     }
     for name, args in sigs.items():
         fn = getattr(lib, name)
@@ -306,6 +330,38 @@ class Context:
         fn = self._lib.mi_backproject if dtype == np.float32 else self._lib.mi_backproject_f64
         _check(fn(self._h, C.byref(g), _ptr(sino), _ptr(out)))
         return out
+
+    # -- 3D cone beam (industry3d) -----------------------------------------
+    def cone_trace_ray(self, g: ConeGeometry, view: int, row: int, col: int, cap: int = 0):
+        """(voxel index, length) sequence of ray (view, row, col) as traced on
+        the device (forward_proj, industry3d/src/tracing.cpp:123-307)."""
+        cap = cap or 4 * (g.nx + g.ny + g.nz) + 8
+        ind = np.zeros(cap, np.int64)
+        ln = np.zeros(cap, np.float32)
+        n = C.c_int()
+        _check(self._lib.mi_cone_trace_ray(self._h, C.byref(g), view, row, col, _ptr(ind), _ptr(ln), cap,
+                                           C.byref(n)))
+        k = min(n.value, cap)
+        return ind[:k].copy(), ln[:k].copy()
+
+    def cone_forward(self, g: ConeGeometry, vol):
+        vol = _arr(vol, np.float32, (g.nz, g.nx, g.ny))
+        out = np.empty((g.n_views, g.n_rows, g.n_cols), np.float32)
+        _check(self._lib.mi_cone_forward(self._h, C.byref(g), _ptr(vol), _ptr(out)))
+        return out
+
+    def cone_backproject(self, g: ConeGeometry, sino):
+        sino = _arr(sino, np.float32, (g.n_views, g.n_rows, g.n_cols))
+        out = np.empty((g.nz, g.nx, g.ny), np.float32)
+        _check(self._lib.mi_cone_backproject(self._h, C.byref(g), _ptr(sino), _ptr(out)))
+        return out
+
+    def cone_forward_dev(self, g: ConeGeometry, vol_ptr: int, sino_ptr: int):
+        """Device pointers (e.g. torch tensors' data_ptr()); no copies."""
+        _check(self._lib.mi_cone_forward_dev(self._h, C.byref(g), C.c_void_p(vol_ptr), C.c_void_p(sino_ptr)))
+
+    def cone_backproject_dev(self, g: ConeGeometry, sino_ptr: int, vol_ptr: int):
+        _check(self._lib.mi_cone_backproject_dev(self._h, C.byref(g), C.c_void_p(sino_ptr), C.c_void_p(vol_ptr)))
 
     # -- GoldenC building blocks -----------------------------------------
     def image_step(self, g: Geometry, sino, f, v, alpha, epsilon, iter_image):

@@ -10,6 +10,11 @@ fixed here.
   1152}, bins from {512, 736}; field of view 358.4 mm (pixel = 358.4 / N),
   detector width 736 mm (bin = 736 / bins); I0 uniform in [1e4, 1e5];
   8 subsets, 50 AT iterations.
+* C4 cone beam: 512^3 volume of 0.5 mm voxels, 720 views over 360 degrees
+  of a flat panel of 768 rows (NDX) by 1024 columns (NDY) of 1.0 mm, SOD
+  541 mm, SDD 949 mm.  industry3d centres both panel axes on NDX / 2, so the
+  columns span -383.5 .. 639.5 mm; the 128 mm radius cylinder of the volume
+  projects within +-231 mm and every one of its rays reaches the panel.
 * C5 serving: sizes {256, 384, 512, 768, 1024}, views {360, 720, 1152, 2304},
   bins = 1.5 N, same field of view and detector width; I0 log-uniform in
   [1e4, 1e5]; stop at relative data-fidelity change < 1e-4 or 200 iterations.
@@ -24,7 +29,7 @@ import math
 
 import numpy as np
 
-from . import arc_geometry, at_params, fan_geometry, phantom
+from . import arc_geometry, at_params, cone_geometry, fan_geometry, phantom
 
 SOD, SDD = 541.0, 949.0
 FOV_MM = 358.4
@@ -121,3 +126,8 @@ def arc_clinical_geometry(views=1152, n=874, pixel_mm=0.5859):
     1.75, SOURCE_TO_ISO 595 mm, SOURCE_TO_DET 1085.6 mm, 1152 views per turn)."""
     return arc_geometry(n, pixel_mm, views, 736, 0.06786112, 595.0, 1085.6, 1.75)
 
+
+
+def c4_geometry(n=512, views=720, rows=768, cols=1024, sample_mm=0.5, pixel_mm=1.0):
+    """C4 cone-beam geometry (industry3d Parameter, BEAM "Cone")."""
+    return cone_geometry(n, n, n, views, rows, cols, sample_mm, pixel_mm, SOD, SDD)

@@ -1,6 +1,7 @@
 // C ABI of libmedimage_amd.so (include/medimage_amd.h): argument checks,
 // error translation, context and geometry cache, instrumentation.
 #include "batch.hpp"
+#include "cone.hpp"
 #include "solver.hpp"
 
 #include <cmath>
@@ -302,6 +303,39 @@ void backproject_impl(mi_context *ctx, const mi_geometry *g, const T *sino, T *i
     MI_HIP(hipStreamSynchronize(st));
     stats_collect(ctx);
 }
+
+// Algorithmic bytes of a cone projection when its kernel is timed (the
+// entry count is traced once per geometry, only then).
+double cone_timed_bytes(mi_context *ctx, ConeTables &t, int kernel) {
+    if (!ctx->stats.on || !((ctx->stats.mask >> kernel) & 1u)) return 0.0;
+    cone_nnz(ctx->stream, t);
+    return kernel == MI_K_CONE_FP ? cone_fp_bytes(t) : cone_bp_bytes(t);
+}
+
+void cone_forward_dev(mi_context *ctx, const mi_cone_geometry *g, const float *vol, float *sino) {
+    MI_REQUIRE(g && vol && sino, MI_E_ARG, "null argument");
+    ConeTables &t = cone_tables(ctx, *g);
+    {
+        LaunchTimer lt(ctx, MI_K_CONE_FP, cone_timed_bytes(ctx, t, MI_K_CONE_FP));
+        cone_forward(ctx->stream, t, vol, sino);
+    }
+    MI_HIP(hipStreamSynchronize(ctx->stream));
+    stats_collect(ctx);
+}
+
+void cone_backproject_dev(mi_context *ctx, const mi_cone_geometry *g, const float *sino, float *vol) {
+    MI_REQUIRE(g && vol && sino, MI_E_ARG, "null argument");
+    ConeTables &t = cone_tables(ctx, *g);
+    {
+        LaunchTimer lt(ctx, MI_K_CONE_BP, cone_timed_bytes(ctx, t, MI_K_CONE_BP));
+        cone_backproject(ctx->stream, t, sino, vol);
+    }
+    MI_HIP(hipStreamSynchronize(ctx->stream));
+    stats_collect(ctx);
+}
+
+size_t cone_voxels(const mi_cone_geometry &g) { return (size_t)g.nx * g.ny * g.nz; }
+size_t cone_rays(const mi_cone_geometry &g) { return (size_t)g.n_views * g.n_rows * g.n_cols; }
 
 }  // namespace
 
@@ -1084,3 +1118,63 @@ int mi_read_dr(const char *path, int64_t nproj, int64_t ndx, int64_t ndy, uint16
             MI_REQUIRE((int64_t)std::fread(flat, 4, (size_t)nf, f) == nf, MI_E_ARG, "short read of .dr flat field");
     });
 }
+
+int mi_cone_trace_ray(mi_context *ctx, const mi_cone_geometry *g, int view, int row, int col, int64_t *ind,
+                      float *len, int cap, int *numb) {
+    return guarded([&] {
+        check_ctx(ctx);
+        MI_REQUIRE(g && numb && cap >= 0 && (cap == 0 || (ind && len)), MI_E_ARG, "null argument");
+        ConeTables &t = cone_tables(ctx, *g);
+        MI_REQUIRE(view >= 0 && view < g->n_views && row >= 0 && row < g->n_rows && col >= 0 && col < g->n_cols,
+                   MI_E_ARG, "ray outside the geometry");
+        DevBuf<int64_t> di(cap > 0 ? cap : 1);
+        DevBuf<float> dl(cap > 0 ? cap : 1);
+        DevBuf<int> dn(1);
+        cone_trace(ctx->stream, t, view, row, col, di.p, dl.p, cap, dn.p);
+        int n = 0;
+        MI_HIP(hipMemcpyAsync(&n, dn.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        MI_HIP(hipStreamSynchronize(ctx->stream));
+        const int k = n < cap ? n : cap;
+        if (k > 0) {
+            MI_HIP(hipMemcpyAsync(ind, di.p, k * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+            MI_HIP(hipMemcpyAsync(len, dl.p, k * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+            MI_HIP(hipStreamSynchronize(ctx->stream));
+        }
+        *numb = n;
+    });
+}
+
+int mi_cone_forward(mi_context *ctx, const mi_cone_geometry *g, const float *vol, float *sino) {
+    return guarded([&] {
+        check_ctx(ctx);
+        MI_REQUIRE(g && vol && sino, MI_E_ARG, "null argument");
+        validate_cone(*g);
+        DevBuf<float> dv(cone_voxels(*g)), ds(cone_rays(*g));
+        MI_HIP(hipMemcpyAsync(dv.p, vol, dv.bytes(), hipMemcpyHostToDevice, ctx->stream));
+        cone_forward_dev(ctx, g, dv.p, ds.p);
+        MI_HIP(hipMemcpyAsync(sino, ds.p, ds.bytes(), hipMemcpyDeviceToHost, ctx->stream));
+        MI_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int mi_cone_forward_dev(mi_context *ctx, const mi_cone_geometry *g, const float *vol, float *sino) {
+    return guarded([&] { cone_forward_dev(check_ctx(ctx), g, vol, sino); });
+}
+
+int mi_cone_backproject(mi_context *ctx, const mi_cone_geometry *g, const float *sino, float *vol) {
+    return guarded([&] {
+        check_ctx(ctx);
+        MI_REQUIRE(g && vol && sino, MI_E_ARG, "null argument");
+        validate_cone(*g);
+        DevBuf<float> dv(cone_voxels(*g)), ds(cone_rays(*g));
+        MI_HIP(hipMemcpyAsync(ds.p, sino, ds.bytes(), hipMemcpyHostToDevice, ctx->stream));
+        cone_backproject_dev(ctx, g, ds.p, dv.p);
+        MI_HIP(hipMemcpyAsync(vol, dv.p, dv.bytes(), hipMemcpyDeviceToHost, ctx->stream));
+        MI_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int mi_cone_backproject_dev(mi_context *ctx, const mi_cone_geometry *g, const float *sino, float *vol) {
+    return guarded([&] { cone_backproject_dev(check_ctx(ctx), g, sino, vol); });
+}
+

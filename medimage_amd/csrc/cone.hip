@@ -1,0 +1,475 @@
+// 3D cone-beam projectors, industry3d geometry (flat panel, circular orbit).
+//
+// Rays.  cone() of industry3d/src/tracing.cpp:51-81 with rotate_2d of
+// industry3d/src/utility.cpp:82-98, in float: the source (0, -SOD, 0) and
+// the panel pixel ((col - HALFDET + 0.5) PIXELSIZE, SDD - SOD,
+// (HALFDET - row - 0.5) PIXELSIZE) are rotated by the view angle, divided
+// by SAMPLESIZE and shifted by N / 2 (in double, as the reference's
+// `+= NX / 2.0` does).  The view's cos / sin come from host tables made
+// with the C library's cosf / sinf, which the reference calls.
+//
+// Forward projection.  One lane per ray walks forward_proj
+// (industry3d/src/tracing.cpp:123-307) in float, operation for operation,
+// so the (voxel, length) sequence is the reference's bit for bit; the walk
+// ends where the reference's does (lambda0 >= lambda_max - 0.05, tested in
+// double) or when its voxel leaves the volume (where the reference would
+// index outside its arrays; only rays through an edge or corner of the
+// volume).  Products vol * len are summed in double in walk order and
+// rounded once.  Each wave takes an 8 x 8 tile of one view's panel, so the
+// 64 rays of a wave cross neighbouring voxels at each step; workgroups (16
+// x 16 pixels) are spread over the XCDs in contiguous ranges of tiles.
+// Steps are taken in groups of 8: the group's voxel offsets and lengths are
+// computed first, its 8 loads issued together, then accumulated in order.
+//
+// Back projection (gather form, no atomics).  One lane per voxel column
+// (x, y) and a chunk of kZc slices along z.  For each view the four corners
+// of the column are projected onto the panel: the columns whose rays can
+// cross it, and per slice the rows.  For each candidate ray the exact length
+// inside the voxel is the slab clip of the ray's parameter interval in
+// double (x / y slabs once per column, z slab per row through a per-row
+// 1 / w table), times the ray length in float; sino * length is summed in
+// double per voxel in (view, column, row) order.  The x / y interval of a
+// candidate column is shared by all kZc slices of the lane.
+#include "cone.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace mi {
+
+namespace {
+
+constexpr int kConeTile = 16;   // forward: 16 x 16 panel pixels per workgroup
+constexpr int kConeGroup = 8;   // forward: walk steps per load group
+constexpr int kZc = 8;          // back projection: slices per lane
+
+__device__ __forceinline__ void cone_rot(float &x, float &y, float c, float s) {
+    float d0 = 0.0f, d1 = 0.0f;
+    d0 += c * x;
+    d1 += s * x;
+    d0 += -s * y;
+    d1 += c * y;
+    x = d0;
+    y = d1;
+}
+
+// Source point of a view (voxel units, volume corner at the origin).
+__device__ __forceinline__ void cone_source(const ConeDev &g, float c, float s, float &sx, float &sy, float &sz) {
+    float X = 0.0f, Y = -g.sod, Z = 0.0f;
+    cone_rot(X, Y, c, s);
+    X /= g.sample;
+    Z /= g.sample;
+    Y /= g.sample;
+    sx = (float)((double)X + g.nx / 2.0);
+    sy = (float)((double)Y + g.ny / 2.0);
+    sz = (float)((double)Z + g.nz / 2.0);
+}
+
+// In-plane detector point of column col (x, y).
+__device__ __forceinline__ void cone_det_xy(const ConeDev &g, float c, float s, int col, float &dx, float &dy) {
+    float X = (float)((double)(col - g.halfdet) + 0.5);
+    X *= g.pixel;
+    float Y = g.sdd - g.sod;
+    cone_rot(X, Y, c, s);
+    X /= g.sample;
+    Y /= g.sample;
+    dx = (float)((double)X + g.nx / 2.0);
+    dy = (float)((double)Y + g.ny / 2.0);
+}
+
+// Height of row `row` of the panel.
+__device__ __host__ __forceinline__ float cone_det_z(const ConeDev &g, int row) {
+    float Z = (float)((double)(g.halfdet - row) - 0.5);
+    Z *= g.pixel;
+    Z /= g.sample;
+    return (float)((double)Z + g.nz / 2.0);
+}
+
+// LAMBDA_X of industry3d/src/tracing.cpp:83-85.
+__device__ __forceinline__ float cone_lam(float i, float s, float d, float L) { return L * (i - s) / (d - s); }
+
+// Walk state of forward_proj: lam0 the current parameter, l* the next
+// crossings, st* their increments, (vx, vy, vz) the voxel, idx its offset.
+struct ConeWalk {
+    float lam0, lx, ly, lz, stx, sty, stz, stop;
+    int vx, vy, vz, sgx, sgy, sgz, idx;
+};
+
+// Entry part of forward_proj (tracing.cpp:142-249).  stop is the smallest
+// float T with (double)T >= (double)lambda_max - 5.e-2, so that the float
+// test lam0 < T equals the reference's double test.
+__device__ __forceinline__ bool cone_walk_init(const ConeDev &g, float sx, float sy, float sz, float dx, float dy,
+                                               float dz, ConeWalk &w) {
+    const float rx = dx - sx, ry = dy - sy, rz = dz - sz;
+    const float L = sqrtf(rx * rx + ry * ry + rz * rz);
+    float lmin = 0.0f, lmax = L, l0, lN, t;
+    const int sgx = (rx > 0) ? 1 : -1, sgy = (ry > 0) ? 1 : -1, sgz = (rz > 0) ? 1 : -1;
+    const float ax = fabsf(rx), ay = fabsf(ry), az = fabsf(rz);
+    const float stx = ((double)ax > 1.e-4) ? (L / ax) : 1.e6f;
+    const float sty = ((double)ay > 1.e-4) ? (L / ay) : 1.e6f;
+    const float stz = ((double)az > 1.e-4) ? (L / az) : 1.e6f;
+    int index = 0;
+    l0 = cone_lam(0.0f, sx, dx, L);
+    lN = cone_lam((float)g.nx, sx, dx, L);
+    t = (l0 < lN) ? l0 : lN;
+    lmin = (lmin > t) ? lmin : t;
+    if (lmin == t) index = 1;
+    t = (l0 > lN) ? l0 : lN;
+    lmax = (lmax < t) ? lmax : t;
+    l0 = cone_lam(0.0f, sy, dy, L);
+    lN = cone_lam((float)g.ny, sy, dy, L);
+    t = (l0 < lN) ? l0 : lN;
+    lmin = (lmin > t) ? lmin : t;
+    if (lmin == t) index = 2;
+    t = (l0 > lN) ? l0 : lN;
+    lmax = (lmax < t) ? lmax : t;
+    l0 = cone_lam(0.0f, sz, dz, L);
+    lN = cone_lam((float)g.nz, sz, dz, L);
+    t = (l0 < lN) ? l0 : lN;
+    lmin = (lmin > t) ? lmin : t;
+    if (lmin == t) index = 3;
+    t = (l0 > lN) ? l0 : lN;
+    lmax = (lmax < t) ? lmax : t;
+    if (lmin >= lmax) return false;
+
+    const float lam0 = lmin;
+    float lx, ly, lz;
+    int vx, vy, vz;
+    if (index == 1) {
+        vx = (sgx == 1) ? 0 : g.nx - 1;
+        lx = lam0 + stx;
+        vy = (int)(sy + lam0 * ry / L);
+        ly = ((double)ay < 1.e-4) ? 1.e6f : cone_lam((float)vy + (float)(sgy > 0), sy, dy, L);
+        vz = (int)(sz + lam0 * rz / L);
+        lz = ((double)az < 1.e-4) ? 1.e6f : cone_lam((float)vz + (float)(sgz > 0), sz, dz, L);
+    } else if (index == 2) {
+        vy = (sgy == 1) ? 0 : g.ny - 1;
+        ly = lam0 + sty;
+        vx = (int)(sx + lam0 * rx / L);
+        lx = ((double)ax < 1.e-4) ? 1.e6f : cone_lam((float)vx + (float)(sgx > 0), sx, dx, L);
+        vz = (int)(sz + lam0 * rz / L);
+        lz = ((double)az < 1.e-4) ? 1.e6f : cone_lam((float)vz + (float)(sgz > 0), sz, dz, L);
+    } else {
+        vz = (sgz == 1) ? 0 : g.nz - 1;
+        lz = lam0 + stz;
+        vx = (int)(sx + lam0 * rx / L);
+        lx = ((double)ax < 1.e-4) ? 1.e6f : cone_lam((float)vx + (float)(sgx > 0), sx, dx, L);
+        vy = (int)(sy + lam0 * ry / L);
+        ly = ((double)ay < 1.e-4) ? 1.e6f : cone_lam((float)vy + (float)(sgy > 0), sy, dy, L);
+    }
+    const double hi = (double)lmax - 5.e-2;
+    float stop = (float)hi;
+    if ((double)stop < hi) stop = nextafterf(stop, INFINITY);
+    w = ConeWalk{lam0, lx, ly, lz, stx, sty, stz, stop, vx, vy, vz, sgx, sgy, sgz,
+                 (vz * g.nx + vx) * g.ny + vy};
+    return true;
+}
+
+// One step when `live` (the walk has not ended): the current voxel's offset
+// and length; state unchanged and length 0 otherwise.  The branches of the
+// reference loop (tracing.cpp:268-305) become selects.
+__device__ __forceinline__ void cone_step(const ConeDev &g, ConeWalk &w, bool &live, int &idx, float &len) {
+    live = live && (w.lam0 < w.stop) && ((unsigned)w.vx < (unsigned)g.nx) && ((unsigned)w.vy < (unsigned)g.ny) &&
+           ((unsigned)w.vz < (unsigned)g.nz);
+    const bool bx = (w.lx <= w.ly) && (w.lx <= w.lz);
+    const bool by = !bx && (w.ly <= w.lz);
+    const bool bz = !bx && !by;
+    const float nxt = bx ? w.lx : (by ? w.ly : w.lz);
+    idx = live ? w.idx : 0;
+    len = live ? nxt - w.lam0 : 0.0f;
+    const bool tx = live && bx, ty = live && by, tz = live && bz;
+    w.lam0 = live ? nxt : w.lam0;
+    w.lx = tx ? w.lx + w.stx : w.lx;
+    w.ly = ty ? w.ly + w.sty : w.ly;
+    w.lz = tz ? w.lz + w.stz : w.lz;
+    w.vx += tx ? w.sgx : 0;
+    w.vy += ty ? w.sgy : 0;
+    w.vz += tz ? w.sgz : 0;
+    w.idx += tx ? w.sgx * g.ny : (ty ? w.sgy : (tz ? w.sgz * g.nx * g.ny : 0));
+}
+
+__device__ __forceinline__ bool cone_ray_init(const ConeDev &g, const float *cs, int view, int row, int col,
+                                              ConeWalk &w) {
+    const float c = cs[2 * view], s = cs[2 * view + 1];
+    float sx, sy, sz, dx, dy;
+    cone_source(g, c, s, sx, sy, sz);
+    cone_det_xy(g, c, s, col, dx, dy);
+    return cone_walk_init(g, sx, sy, sz, dx, dy, cone_det_z(g, row), w);
+}
+
+__global__ void k_cone_trace(ConeDev g, const float *cs, int view, int row, int col, int64_t *ind, float *len,
+                             int cap, int *numb) {
+    if (threadIdx.x != 0) return;
+    ConeWalk w;
+    int n = 0;
+    if (cone_ray_init(g, cs, view, row, col, w)) {
+        bool live = true;
+        for (;;) {
+            int idx;
+            float l;
+            cone_step(g, w, live, idx, l);
+            if (!live) break;
+            if (n < cap) {
+                ind[n] = idx;
+                len[n] = l;
+            }
+            ++n;
+        }
+    }
+    *numb = n;
+}
+
+// Panel pixel of this lane: workgroup = 16 x 16 pixels of one view, wave =
+// 8 x 8 of them.
+__device__ __forceinline__ bool cone_fp_pixel(const ConeDev &g, int &view, int &row, int &col) {
+    const int tc = (g.ncols + kConeTile - 1) / kConeTile, tr = (g.nrows + kConeTile - 1) / kConeTile;
+    const unsigned b = xcd_block();
+    const unsigned per_view = (unsigned)(tc * tr);
+    view = (int)(b / per_view);
+    const int tile = (int)(b % per_view);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    row = (tile / tc) * kConeTile + (wv >> 1) * 8 + (lane >> 3);
+    col = (tile % tc) * kConeTile + (wv & 1) * 8 + (lane & 7);
+    return view < g.nviews && row < g.nrows && col < g.ncols;
+}
+
+__global__ __launch_bounds__(kBlock) void k_cone_fp(ConeDev g, const float *cs, const float *__restrict__ vol,
+                                                    float *__restrict__ sino) {
+    int view, row, col;
+    if (!cone_fp_pixel(g, view, row, col)) return;
+    ConeWalk w;
+    double acc = 0.0;
+    if (cone_ray_init(g, cs, view, row, col, w)) {
+        bool live = true;
+        while (live) {
+            int id[kConeGroup];
+            float ln[kConeGroup];
+            bool lv[kConeGroup];
+#pragma unroll
+            for (int k = 0; k < kConeGroup; ++k) cone_step(g, w, live, id[k], ln[k]), lv[k] = live;
+            float v[kConeGroup];
+#pragma unroll
+            for (int k = 0; k < kConeGroup; ++k) v[k] = vol[id[k]];
+#pragma unroll
+            for (int k = 0; k < kConeGroup; ++k) acc = lv[k] ? acc + (double)v[k] * (double)ln[k] : acc;
+        }
+    }
+    sino[((size_t)view * g.nrows + row) * g.ncols + col] = (float)acc;
+}
+
+// Entry count of every ray (for the algorithmic-byte figure).
+__global__ __launch_bounds__(kBlock) void k_cone_count(ConeDev g, const float *cs,
+                                                       unsigned long long *total) {
+    int view, row, col;
+    unsigned long long n = 0;
+    if (cone_fp_pixel(g, view, row, col)) {
+        ConeWalk w;
+        if (cone_ray_init(g, cs, view, row, col, w)) {
+            bool live = true;
+            for (;;) {
+                int idx;
+                float l;
+                cone_step(g, w, live, idx, l);
+                if (!live) break;
+                ++n;
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) n += __shfl_xor(n, o);
+    if ((threadIdx.x & 63) == 0 && n) atomicAdd(total, n);
+}
+
+// Slab interval [lo, hi] of parameter t (point s + t r) for the slab
+// [a, a + 1] of one axis; r == 0: everything or nothing.
+__device__ __forceinline__ void cone_slab(double s, double r, double a, double &lo, double &hi) {
+    if (r == 0.0) {
+        const bool in = (s >= a) && (s < a + 1.0);
+        lo = in ? lo : 1.0;
+        hi = in ? hi : 0.0;
+        return;
+    }
+    const double ir = 1.0 / r;
+    const double t0 = (a - s) * ir, t1 = (a + 1.0 - s) * ir;
+    lo = fmax(lo, fmin(t0, t1));
+    hi = fmin(hi, fmax(t0, t1));
+}
+
+__global__ __launch_bounds__(kBlock) void k_cone_bp(ConeDev g, const float *cs, const double *rowtab,
+                                                    const float *__restrict__ sino, float *__restrict__ vol) {
+    // Workgroup: 16 x 16 voxel columns, wave 8 x 8 of them; kZc slices.
+    const int tx = (g.nx + 15) / 16, ty = (g.ny + 15) / 16;
+    const unsigned b = xcd_block();
+    const int per_z = tx * ty;
+    const int zc = (int)(b / per_z), tile = (int)(b % per_z);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int x = (tile / ty) * 16 + (wv >> 1) * 8 + (lane >> 3);
+    const int y = (tile % ty) * 16 + (wv & 1) * 8 + (lane & 7);
+    const int z0 = zc * kZc;
+    if (x >= g.nx || y >= g.ny || z0 >= g.nz) return;
+
+    double acc[kZc];
+#pragma unroll
+    for (int k = 0; k < kZc; ++k) acc[k] = 0.0;
+
+    const float cxv = g.nx / 2.0f, cyv = g.ny / 2.0f, czv = g.nz / 2.0f;
+    const float kcol = g.sdd / g.pixel;          // panel pixels per (voxel offset / voxel depth)
+    const float sodv = g.sod / g.sample;         // source distance in voxels
+    const float h5 = (float)g.halfdet - 0.5f;
+    const float margin = 0.05f;                  // in pixels, covers the float rounding of the bounds
+
+    for (int v = 0; v < g.nviews; ++v) {
+        const float c = cs[2 * v], s = cs[2 * v + 1];
+        float sx, sy, sz;
+        cone_source(g, c, s, sx, sy, sz);
+        // Corners of the column in the view's frame (u across the panel,
+        // depth from the source): the panel column of a point is
+        // u * kcol / depth + h5, the row of height zr is h5 - zr * kcol / depth.
+        float cmin = 1e30f, cmax = -1e30f, imin = 1e30f, imax = -1e30f;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float px = (float)(x + (q & 1)) - cxv, py = (float)(y + (q >> 1)) - cyv;
+            const float u = c * px + s * py;
+            const float id = 1.0f / (-s * px + c * py + sodv);
+            const float cf = u * kcol * id;
+            cmin = fminf(cmin, cf);
+            cmax = fmaxf(cmax, cf);
+            imin = fminf(imin, id);
+            imax = fmaxf(imax, id);
+        }
+        const int c0 = max(0, (int)ceilf(cmin + h5 - margin));
+        const int c1 = min(g.ncols - 1, (int)floorf(cmax + h5 + margin));
+        for (int col = c0; col <= c1; ++col) {
+            float dx, dy;
+            cone_det_xy(g, c, s, col, dx, dy);
+            const double rx = (double)dx - (double)sx, ry = (double)dy - (double)sy;
+            double lo = 0.0, hi = 1.0;
+            cone_slab(sx, rx, (double)x, lo, hi);
+            cone_slab(sy, ry, (double)y, lo, hi);
+            if (!(hi > lo)) continue;
+            const double lxy2 = rx * rx + ry * ry;
+            const float *scol = sino + ((size_t)v * g.nrows) * g.ncols + col;
+#pragma unroll
+            for (int k = 0; k < kZc; ++k) {
+                const int z = z0 + k;
+                if (z >= g.nz) break;
+                // Rows whose rays can meet slice z inside the column.
+                const float za = (float)z - czv, zb = za + 1.0f;
+                const float wlo = fminf(za * imin, za * imax) * kcol;
+                const float whi = fmaxf(zb * imin, zb * imax) * kcol;
+                const int r0 = max(0, (int)ceilf(h5 - whi - margin));
+                const int r1 = min(g.nrows - 1, (int)floorf(h5 - wlo + margin));
+                for (int r = r0; r <= r1; ++r) {
+                    const double w = rowtab[2 * r], iw = rowtab[2 * r + 1];
+                    const double t0 = ((double)z - (double)sz) * iw, t1 = ((double)z + 1.0 - (double)sz) * iw;
+                    const double a = fmax(lo, fmin(t0, t1)), e = fmin(hi, fmax(t0, t1));
+                    if (!(e > a)) continue;
+                    const float L = sqrtf((float)(lxy2 + w * w));
+                    const float len = (float)(e - a) * L;
+                    acc[k] += (double)scol[(size_t)r * g.ncols] * (double)len;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kZc; ++k) {
+        const int z = z0 + k;
+        if (z < g.nz) vol[((size_t)z * g.nx + x) * g.ny + y] = (float)acc[k];
+    }
+}
+
+unsigned grid8(size_t blocks) { return (unsigned)((blocks + 7) / 8 * 8); }
+
+}  // namespace
+
+void validate_cone(const mi_cone_geometry &g) {
+    MI_REQUIRE(g.nx > 0 && g.ny > 0 && g.nz > 0 && g.n_views > 0 && g.n_rows > 0 && g.n_cols > 0, MI_E_ARG,
+               "cone geometry sizes must be positive");
+    MI_REQUIRE((int64_t)g.nx * g.ny * g.nz < ((int64_t)1 << 31), MI_E_LIMIT, "volume beyond 2^31 voxels");
+    MI_REQUIRE((int64_t)g.n_views * g.n_rows * g.n_cols < ((int64_t)1 << 40), MI_E_LIMIT, "sinogram too large");
+    MI_REQUIRE(std::isfinite(g.sample_mm) && std::isfinite(g.pixel_mm) && g.sample_mm > 0 && g.pixel_mm > 0,
+               MI_E_ARG, "sample_mm and pixel_mm must be positive");
+    MI_REQUIRE(std::isfinite(g.sdd_mm) && g.sod_mm > 0 && g.sdd_mm > g.sod_mm, MI_E_ARG, "need sdd > sod > 0");
+    // The source circle must lie outside the volume (the back projector's
+    // corner projection assumes every voxel is in front of the source).
+    const double half = 0.5 * g.sample_mm * std::sqrt((double)g.nx * g.nx + (double)g.ny * g.ny);
+    MI_REQUIRE(g.sod_mm > half, MI_E_ARG, "the source orbit must lie outside the volume");
+}
+
+ConeTables &cone_tables(mi_context *ctx, const mi_cone_geometry &g) {
+    validate_cone(g);
+    if (ctx->cone && std::memcmp(&ctx->cone->g, &g, sizeof(g)) == 0) return *ctx->cone;
+    auto t = std::make_shared<ConeTables>();
+    t->g = g;
+    t->d = ConeDev{g.nx, g.ny, g.nz, g.n_views, g.n_rows, g.n_cols, g.n_rows / 2,
+                   g.sample_mm, g.pixel_mm, g.sod_mm, g.sdd_mm};
+    std::vector<float> cs(2 * (size_t)g.n_views);
+    for (int a = 0; a < g.n_views; ++a) {
+        // theta = (float)alpha / NPROJ * 2 * PI, industry3d/src/tracing.cpp:58.
+        float th = (float)a / (float)g.n_views;
+        th = th * 2.0f;
+        th = (float)((double)th * 3.1415926535897932385);
+        cs[2 * a] = cosf(th);
+        cs[2 * a + 1] = sinf(th);
+    }
+    const float sz = (float)((double)(0.0f / g.sample_mm) + g.nz / 2.0);
+    std::vector<double> rows(2 * (size_t)g.n_rows);
+    for (int r = 0; r < g.n_rows; ++r) {
+        const double w = (double)cone_det_z(t->d, r) - (double)sz;
+        MI_REQUIRE(w != 0.0, MI_E_ARG, "a detector row lies in the plane of the source orbit");
+        rows[2 * r] = w;
+        rows[2 * r + 1] = 1.0 / w;
+    }
+    t->cs.alloc(cs.size());
+    t->rows.alloc(rows.size());
+    MI_HIP(hipMemcpyAsync(t->cs.p, cs.data(), cs.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    MI_HIP(hipMemcpyAsync(t->rows.p, rows.data(), rows.size() * sizeof(double), hipMemcpyHostToDevice,
+                          ctx->stream));
+    MI_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->cone = t;
+    return *t;
+}
+
+void cone_trace(hipStream_t st, const ConeTables &t, int view, int row, int col, int64_t *ind, float *len, int cap,
+                int *numb) {
+    hipLaunchKernelGGL(k_cone_trace, dim3(1), dim3(64), 0, st, t.d, t.cs.p, view, row, col, ind, len, cap, numb);
+    MI_HIP(hipGetLastError());
+}
+
+static size_t cone_fp_blocks(const ConeDev &d) {
+    return (size_t)((d.ncols + kConeTile - 1) / kConeTile) * ((d.nrows + kConeTile - 1) / kConeTile) * d.nviews;
+}
+
+void cone_forward(hipStream_t st, const ConeTables &t, const float *vol, float *sino) {
+    hipLaunchKernelGGL(k_cone_fp, dim3(grid8(cone_fp_blocks(t.d))), dim3(kBlock), 0, st, t.d, t.cs.p, vol, sino);
+    MI_HIP(hipGetLastError());
+}
+
+void cone_backproject(hipStream_t st, const ConeTables &t, const float *sino, float *vol) {
+    const size_t blocks = (size_t)((t.d.nx + 15) / 16) * ((t.d.ny + 15) / 16) * ((t.d.nz + kZc - 1) / kZc);
+    hipLaunchKernelGGL(k_cone_bp, dim3(grid8(blocks)), dim3(kBlock), 0, st, t.d, t.cs.p, t.rows.p, sino, vol);
+    MI_HIP(hipGetLastError());
+}
+
+double cone_fp_bytes(const ConeTables &t) {
+    const double rays = (double)t.d.nviews * t.d.nrows * t.d.ncols;
+    const double vox = (double)t.d.nx * t.d.ny * t.d.nz;
+    return (double)t.nnz * 8.0 + rays * 4.0 + vox * 4.0;
+}
+
+double cone_bp_bytes(const ConeTables &t) { return cone_fp_bytes(t); }
+
+int64_t cone_nnz(hipStream_t st, ConeTables &t) {
+    if (t.nnz >= 0) return t.nnz;
+    DevBuf<unsigned long long> n(1);
+    MI_HIP(hipMemsetAsync(n.p, 0, sizeof(unsigned long long), st));
+    hipLaunchKernelGGL(k_cone_count, dim3(grid8(cone_fp_blocks(t.d))), dim3(kBlock), 0, st, t.d, t.cs.p, n.p);
+    MI_HIP(hipGetLastError());
+    unsigned long long h = 0;
+    MI_HIP(hipMemcpyAsync(&h, n.p, sizeof(h), hipMemcpyDeviceToHost, st));
+    MI_HIP(hipStreamSynchronize(st));
+    t.nnz = (int64_t)h;
+    return t.nnz;
+}
+
+}  // namespace mi

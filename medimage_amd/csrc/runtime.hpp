@@ -229,6 +229,8 @@ struct Stats {
     std::vector<hipEvent_t> pool;
 };
 
+struct ConeTables;  // per-geometry tables of the 3D cone-beam projectors
+
 }  // namespace mi
 
 struct mi_context {
@@ -237,6 +239,7 @@ struct mi_context {
     std::map<mi::GeomKey, std::shared_ptr<mi::Matrix>> matrices;
     mi::Stats stats;
     uint64_t tick = 0;
+    std::shared_ptr<mi::ConeTables> cone;  // tables of the last cone geometry used
 };
 
 namespace mi {

@@ -28,6 +28,12 @@
  *   or_os_at                      reduces to or_gc_image_step's first step at
  *                                 one subset; exact line search checked against
  *                                 a brute-force minimisation along the direction.
+ *   or_cone_trace / or_cone_walk  closed forms (axis-aligned rays, chord length,
+ *                                 crossing count) and exact-rational enumeration
+ *                                 of the voxel sequence; the reference's
+ *                                 tracing.cpp needs boost through utility.cpp.
+ *   or_cone_exact / or_cone_bp    per-voxel slab clip of every voxel
+ *                                 (or_cone_clip) and the chord length.
  */
 #ifndef MEDIMAGE_ORACLE_H
 #define MEDIMAGE_ORACLE_H
@@ -191,6 +197,33 @@ int or_os_at(const or_op *op, const double *g, double *u, double *v,
  * ------------------------------------------------------------------------- */
 void or_msbeam_solve(const float *g, float *f, float *v, int nproj, int nray, int n,
                      double alpha, double beta, double eps, int iters, int flavour);
+
+/* ---------------------------------------------------------------------------
+ * 3D cone beam, flat panel (industry3d).  Volume [nz][nx][ny], ny fastest
+ * (get_img_addr, industry3d/src/tracing.cpp:119-121); sinogram
+ * [nproj][ndx][ndy]; coordinates in voxels, volume corner at the origin.
+ * ------------------------------------------------------------------------- */
+typedef struct or_cone_geom {
+    int nx, ny, nz, nproj, ndx, ndy;
+    float sample, pixel, sod, sdd;      /* SAMPLESIZE, PIXELSIZE, SOD, SDD */
+} or_cone_geom;
+/* cosf / sinf of each view's theta, industry3d/src/tracing.cpp:58. */
+void or_cone_tables(const or_cone_geom *g, float *cos_t, float *sin_t);
+/* cone(), industry3d/src/tracing.cpp:51-81: float source and detector point. */
+void or_cone_ray(const or_cone_geom *g, float c, float s, int row, int col, float *src, float *dst);
+/* forward_proj, industry3d/src/tracing.cpp:123-307, float; returns numb
+ * (entries beyond cap are counted, not stored). */
+int or_cone_walk(const or_cone_geom *g, const float *src, const float *dst, int64_t *ind, float *wgt, int cap);
+int or_cone_trace(const or_cone_geom *g, int view, int row, int col, int64_t *ind, float *wgt, int cap);
+/* A over the traced entries, double accumulation in walk order
+ * (accum_Af, industry3d/src/main.cpp:254-266). */
+void or_cone_fp(const or_cone_geom *g, const float *vol, double *sino);
+/* Exact (double) voxel intersection lengths of the ray src -> dst. */
+int or_cone_exact(const or_cone_geom *g, const float *src, const float *dst, int64_t *ind, double *len, int cap);
+/* Transpose of the exact-length system matrix, scatter form, double. */
+void or_cone_bp(const or_cone_geom *g, const double *sino, double *vol);
+/* Length of the ray src -> dst inside voxel (x, y, z), slab clip in double. */
+double or_cone_clip(const float *src, const float *dst, int x, int y, int z);
 
 #ifdef __cplusplus
 }

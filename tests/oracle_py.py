@@ -293,3 +293,95 @@ def msbeam_solve(g, n, alpha, beta, eps, iters, flavour=0):
     v = np.zeros(n * n, np.float32)
     lib().or_msbeam_solve(P(g), P(f), P(v), g.shape[0], g.shape[1], n, alpha, beta, eps, iters, flavour)
     return f.reshape(n, n), v.reshape(n, n)
+
+
+# ---------------------------------------------------------------------------
+# 3D cone beam (industry3d): oracle/cone3d.c
+class ConeGeom(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("nproj", C.c_int), ("ndx", C.c_int),
+                ("ndy", C.c_int), ("sample", C.c_float), ("pixel", C.c_float), ("sod", C.c_float),
+                ("sdd", C.c_float)]
+
+
+def cone_from(g):
+    """Oracle geometry of a medimage_amd.ConeGeometry."""
+    return ConeGeom(g.nx, g.ny, g.nz, g.n_views, g.n_rows, g.n_cols, g.sample_mm, g.pixel_mm, g.sod_mm, g.sdd_mm)
+
+
+def _cone_lib():
+    L = lib()
+    L.or_cone_walk.restype = C.c_int
+    L.or_cone_trace.restype = C.c_int
+    L.or_cone_exact.restype = C.c_int
+    L.or_cone_clip.restype = C.c_double
+    L.or_cone_clip.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int]
+    L.or_cone_ray.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+    return L
+
+
+def cone_cap(cg):
+    return 4 * (cg.nx + cg.ny + cg.nz) + 8
+
+
+def cone_tables(cg):
+    c = np.zeros(cg.nproj, np.float32)
+    s = np.zeros(cg.nproj, np.float32)
+    _cone_lib().or_cone_tables(C.byref(cg), P(c), P(s))
+    return c, s
+
+
+def cone_ray(cg, view, row, col):
+    """Float source and detector points of cone(), voxel units."""
+    c, s = cone_tables(cg)
+    S = np.zeros(3, np.float32)
+    D = np.zeros(3, np.float32)
+    _cone_lib().or_cone_ray(C.byref(cg), float(c[view]), float(s[view]), row, col, P(S), P(D))
+    return S, D
+
+
+def cone_walk(cg, S, D):
+    cap = cone_cap(cg)
+    ind = np.zeros(cap, np.int64)
+    w = np.zeros(cap, np.float32)
+    S = np.ascontiguousarray(S, np.float32)
+    D = np.ascontiguousarray(D, np.float32)
+    n = _cone_lib().or_cone_walk(C.byref(cg), P(S), P(D), P(ind), P(w), cap)
+    return ind[:n].copy(), w[:n].copy()
+
+
+def cone_trace(cg, view, row, col):
+    cap = cone_cap(cg)
+    ind = np.zeros(cap, np.int64)
+    w = np.zeros(cap, np.float32)
+    n = _cone_lib().or_cone_trace(C.byref(cg), view, row, col, P(ind), P(w), cap)
+    return ind[:n].copy(), w[:n].copy()
+
+
+def cone_exact(cg, S, D):
+    cap = cone_cap(cg)
+    ind = np.zeros(cap, np.int64)
+    ln = np.zeros(cap, np.float64)
+    S = np.ascontiguousarray(S, np.float32)
+    D = np.ascontiguousarray(D, np.float32)
+    n = _cone_lib().or_cone_exact(C.byref(cg), P(S), P(D), P(ind), P(ln), cap)
+    return ind[:n].copy(), ln[:n].copy()
+
+
+def cone_clip(S, D, x, y, z):
+    S = np.ascontiguousarray(S, np.float32)
+    D = np.ascontiguousarray(D, np.float32)
+    return _cone_lib().or_cone_clip(P(S), P(D), x, y, z)
+
+
+def cone_fp(cg, vol):
+    vol = np.ascontiguousarray(vol, np.float32)
+    out = np.zeros((cg.nproj, cg.ndx, cg.ndy), np.float64)
+    _cone_lib().or_cone_fp(C.byref(cg), P(vol), P(out))
+    return out
+
+
+def cone_bp(cg, sino):
+    sino = np.ascontiguousarray(sino, np.float64)
+    out = np.zeros((cg.nz, cg.nx, cg.ny), np.float64)
+    _cone_lib().or_cone_bp(C.byref(cg), P(sino), P(out))
+    return out

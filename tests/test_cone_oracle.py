@@ -1,0 +1,179 @@
+"""Pins of the 3D cone-beam oracle (oracle/cone3d.c), no GPU.
+
+The reference's tracer (industry3d/src/tracing.cpp) needs boost through its
+utility.cpp and is not compiled here, so the float restatement is pinned by
+closed forms and exact rational arithmetic:
+
+* cone() end points: source on the circle of radius SOD / SAMPLESIZE around
+  the volume centre at the view angle, the panel plane perpendicular to the
+  central ray at SDD, pixel centres on the (HALFDET - 0.5)-shifted grid;
+* forward_proj: axis-aligned rays give unit lengths and one voxel per plane;
+  for general rays the voxel sequence equals an exact enumeration with
+  Fractions, lengths agree to float resolution, the count is the number of
+  plane crossings inside the volume plus one (less a dropped final piece
+  shorter than 0.05), and the lengths add up to the chord;
+* the exact-length matrix (or_cone_exact, the back projector's) equals a
+  per-voxel slab clip of every voxel of a small grid.
+"""
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+
+
+def geom(nx=12, ny=10, nz=8, nproj=24, ndx=10, ndy=14, sample=1.0, pixel=2.0, sod=40.0, sdd=75.0):
+    return O.ConeGeom(nx, ny, nz, nproj, ndx, ndy, sample, pixel, sod, sdd)
+
+
+def test_cone_ray_closed_form():
+    g = geom(sample=0.5, pixel=1.5)
+    c, s = O.cone_tables(g)
+    ctr = np.array([g.nx / 2, g.ny / 2, g.nz / 2])
+    for v in (0, 5, 13):
+        th = 2 * math.pi * v / g.nproj
+        assert abs(c[v] - math.cos(th)) < 1e-6 and abs(s[v] - math.sin(th)) < 1e-6
+        for r, k in ((0, 0), (3, 7), (g.ndx - 1, g.ndy - 1)):
+            S, D = O.cone_ray(g, v, r, k)
+            S = S.astype(np.float64)
+            D = D.astype(np.float64)
+            # source: (0, -SOD) rotated by theta, in voxels, z at the centre
+            want_s = ctr + np.array([math.sin(th) * g.sod, -math.cos(th) * g.sod, 0.0]) / g.sample
+            assert np.allclose(S, want_s, atol=1e-4)
+            # panel point: ((k - H + .5) p, SDD - SOD, (H - r - .5) p) rotated
+            H = g.ndx // 2
+            px, py, pz = (k - H + 0.5) * g.pixel, g.sdd - g.sod, (H - r - 0.5) * g.pixel
+            want_d = ctr + np.array([math.cos(th) * px - math.sin(th) * py,
+                                     math.sin(th) * px + math.cos(th) * py, pz]) / g.sample
+            assert np.allclose(D, want_d, atol=1e-4)
+            # the central ray is perpendicular to the panel at distance SDD
+            u = (ctr - S) / np.linalg.norm(ctr - S)
+            assert abs(np.dot(D - S, u) - g.sdd / g.sample) < 1e-3
+
+
+def test_walk_axis_aligned():
+    g = geom()
+    # along +y through voxel column (x=3, z=5): ny unit lengths
+    ind, w = O.cone_walk(g, [3.5, -20.0, 5.5], [3.5, 30.0, 5.5])
+    assert len(ind) == g.ny
+    assert np.all(w == 1.0)
+    assert list(ind) == [5 * g.nx * g.ny + 3 * g.ny + y for y in range(g.ny)]
+    # along -z through (x=7, y=2)
+    ind, w = O.cone_walk(g, [7.5, 2.5, 40.0], [7.5, 2.5, -10.0])
+    assert len(ind) == g.nz and np.all(w == 1.0)
+    assert list(ind) == [z * g.nx * g.ny + 7 * g.ny + 2 for z in reversed(range(g.nz))]
+    # along +x
+    ind, w = O.cone_walk(g, [-3.0, 4.5, 1.5], [20.0, 4.5, 1.5])
+    assert len(ind) == g.nx and np.all(w == 1.0)
+    assert list(ind) == [1 * g.nx * g.ny + x * g.ny + 4 for x in range(g.nx)]
+
+
+def _exact_walk(g, S, D):
+    """Voxel sequence and lengths of segment S -> D through the grid,
+    exactly (Fractions): merged plane crossings inside the volume."""
+    S = [Fraction(float(a)) for a in S]
+    D = [Fraction(float(a)) for a in D]
+    N = (g.nx, g.ny, g.nz)
+    r = [D[a] - S[a] for a in range(3)]
+    lo, hi = Fraction(0), Fraction(1)
+    for a in range(3):
+        if r[a] == 0:
+            assert 0 <= S[a] < N[a]
+            continue
+        t0, t1 = (0 - S[a]) / r[a], (N[a] - S[a]) / r[a]
+        lo, hi = max(lo, min(t0, t1)), min(hi, max(t0, t1))
+    if lo >= hi:
+        return [], [], Fraction(0), 0
+    ts = {lo, hi}
+    crossings = 0
+    for a in range(3):
+        if r[a] == 0:
+            continue
+        for k in range(N[a] + 1):
+            t = (k - S[a]) / r[a]
+            if lo < t < hi:
+                ts.add(t)
+                crossings += 1
+    ts = sorted(ts)
+    L2 = sum(x * x for x in r)
+    seq, lens = [], []
+    for t1, t2 in zip(ts, ts[1:]):
+        tm = (t1 + t2) / 2
+        v = [math.floor(S[a] + tm * r[a]) for a in range(3)]
+        seq.append(v[2] * g.nx * g.ny + v[0] * g.ny + v[1])
+        lens.append((t2 - t1) * math.sqrt(L2))
+    return seq, lens, (hi - lo) * math.sqrt(L2), crossings
+
+
+def test_walk_matches_exact_enumeration():
+    """Random rays of the cone geometry: the float walk visits exactly the
+    voxels of the exact enumeration (the last one may be dropped when its
+    piece is shorter than 0.05), lengths to float resolution."""
+    g = geom()
+    rng = np.random.default_rng(7)
+    checked = 0
+    for _ in range(400):
+        v, r, k = int(rng.integers(g.nproj)), int(rng.integers(g.ndx)), int(rng.integers(g.ndy))
+        S, D = O.cone_ray(g, v, r, k)
+        ind, w = O.cone_walk(g, S, D)
+        ind2, w2 = O.cone_trace(g, v, r, k)
+        assert np.array_equal(ind, ind2) and np.array_equal(w, w2)
+        seq, lens, chord, crossings = _exact_walk(g, S, D)
+        if not seq:
+            assert len(ind) == 0
+            continue
+        # rays through an edge or corner (two crossings within float
+        # resolution) are left to the exact-length tests below
+        if min(float(x) for x in lens) < 1e-3:
+            continue
+        n = len(ind)
+        assert n in (len(seq), len(seq) - 1)
+        if n == len(seq) - 1:
+            assert float(lens[-1]) < 0.05 + 1e-3
+        assert list(ind) == seq[:n]
+        assert n >= crossings  # count = crossings + 1, less a dropped tail
+        assert np.allclose(w, [float(x) for x in lens[:n]], atol=2e-4)
+        assert abs(float(w.astype(np.float64).sum()) - float(chord)) < 0.05 + 1e-3
+        checked += 1
+    assert checked > 250
+
+
+def test_exact_lengths_equal_slab_clip():
+    g = geom(nx=6, ny=5, nz=4, nproj=6, ndx=6, ndy=8, pixel=3.0, sod=25.0, sdd=45.0)
+    for v in range(g.nproj):
+        for r in range(g.ndx):
+            for k in range(0, g.ndy, 3):
+                S, D = O.cone_ray(g, v, r, k)
+                ind, ln = O.cone_exact(g, S, D)
+                dense = np.zeros(g.nx * g.ny * g.nz)
+                np.add.at(dense, ind, ln)
+                brute = np.zeros_like(dense)
+                for z in range(g.nz):
+                    for x in range(g.nx):
+                        for y in range(g.ny):
+                            brute[z * g.nx * g.ny + x * g.ny + y] = O.cone_clip(S, D, x, y, z)
+                assert np.allclose(dense, brute, rtol=0, atol=1e-12)
+
+
+def test_fp_bp_pair():
+    """or_cone_fp sums the traced entries in double; or_cone_bp is the
+    transpose of the exact lengths.  On a small geometry the two agree as
+    adjoints to the walk's float resolution and dropped tails."""
+    g = geom()
+    rng = np.random.default_rng(3)
+    x = rng.random((g.nz, g.nx, g.ny)).astype(np.float32)
+    y = rng.random((g.nproj, g.ndx, g.ndy))
+    Ax = O.cone_fp(g, x)
+    # or_cone_fp is the walk's entries summed in order
+    v, r, k = 5, 4, 6
+    ind, w = O.cone_trace(g, v, r, k)
+    acc = 0.0
+    for i, l in zip(ind, w):
+        acc += float(x.reshape(-1)[i]) * float(l)
+    assert Ax[v, r, k] == acc
+    Bty = O.cone_bp(g, y)
+    lhs = float(np.sum(Ax * y))
+    rhs = float(np.sum(x.astype(np.float64) * Bty))
+    assert abs(lhs - rhs) / abs(rhs) < 2e-3
