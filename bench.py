@@ -158,6 +158,44 @@ def bench_serving(ctx, n_jobs, rate, log):
     return out
 
 
+def bench_c4(ctx, reps, log):
+    """C4 cone-beam projectors: forward and gather back projection of the
+    512^3 volume over 720 views of the 768 x 1024 panel, kernel time from HIP
+    events (host copies are outside the timed launches)."""
+    import medimage_amd as mi
+    from medimage_amd import configs
+    g = configs.c4_geometry()
+    rng = np.random.default_rng(4)
+    vol = rng.random((g.nz, g.nx, g.ny), dtype=np.float32)
+    ks = [mi.MI_K_CONE_FP, mi.MI_K_CONE_BP]
+    ctx.stats_select(ks)
+    log("[bench] C4: first forward projection (counts the traced entries once)")
+    sino = ctx.cone_forward(g, vol)
+    ctx.cone_backproject(g, sino)
+    ctx.stats_reset()
+    for i in range(reps):
+        sino = ctx.cone_forward(g, vol)
+        ctx.cone_backproject(g, sino)
+        log(f"[bench] C4 rep {i + 1}/{reps}")
+    st = {k: ctx.stats(k) for k in ks}
+    ctx.stats_select([])
+    rays = g.n_views * g.n_rows * g.n_cols
+    vox = g.nx * g.ny * g.nz
+    out = {"workload": "C4: 512^3 volume (0.5 mm), 720 views over 360 degrees, flat panel 768 x 1024 "
+                       "(1.0 mm), SOD 541 mm, SDD 949 mm; random volume (the projectors' cost does not "
+                       "depend on the values)", "rays": rays, "voxels": vox}
+    for k, name in ((mi.MI_K_CONE_FP, "forward"), (mi.MI_K_CONE_BP, "backproject")):
+        s = st[k]
+        ms = s.total_ms / max(s.launches, 1)
+        b = s.bytes / max(s.launches, 1)
+        out[name] = {"launches": s.launches, "avg_ms": ms, "rays_per_s": rays / (ms * 1e-3),
+                     "voxel_views_per_s": vox * g.n_views / (ms * 1e-3),
+                     "algorithmic_bytes": b, "GBps": b / (ms * 1e-3) / 1e9,
+                     "frac_of_hbm_peak": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    out["nnz"] = int(round((out["forward"]["algorithmic_bytes"] - 4.0 * (rays + vox)) / 8.0))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,6 +211,8 @@ def main():
     ap.add_argument("--serve-jobs", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--c4-reps", type=int, default=2)
+    ap.add_argument("--no-c4", action="store_true")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="also run a plain device copy of this many GiB (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -277,6 +317,8 @@ def main():
         ms = ctx.device_copy(nb, 3)
         out["copy_reference"] = {"kernel": "k_copy16", "bytes_read": nb, "bytes_written": nb, "launches": 3,
                                  "ms": ms, "GBps": 2 * nb / (ms * 1e-3) / 1e9}
+    if not args.no_c4:
+        out["cone_c4"] = bench_c4(ctx, args.c4_reps, lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_c3:
         job.close()
         out["batched"] = bench_c3(ctx, args.c3_jobs, args.c3_iters,
