@@ -158,32 +158,33 @@ def bench_serving(ctx, n_jobs, rate, log):
     return out
 
 
-def bench_c4(ctx, reps, log):
-    """C4 cone-beam projectors: forward and gather back projection of the
-    512^3 volume over 720 views of the 768 x 1024 panel, kernel time from HIP
-    events (host copies are outside the timed launches)."""
+def bench_c4(ctx, reps, at_iters, log):
+    """C4 cone beam (512^3 volume, 720 views of the 1024 x 768 panel): the
+    forward and gather back projection kernels (HIP-event time; host copies
+    are outside the timed launches), then ordered-subset AT passes (20
+    subsets) on a noisy sinogram of the 20-ellipsoid phantom."""
     import medimage_amd as mi
-    from medimage_amd import configs
+    from medimage_amd import configs, phantom
     g = configs.c4_geometry()
-    rng = np.random.default_rng(4)
-    vol = rng.random((g.nz, g.nx, g.ny), dtype=np.float32)
+    t0 = time.perf_counter()
+    ell = phantom.random_ellipsoids(20, g.nx * g.sample_mm, seed=4, mu_max=0.05)
+    vol = phantom.rasterize3d(ell, g.nz, g.nx, g.ny, g.sample_mm)
     ks = [mi.MI_K_CONE_FP, mi.MI_K_CONE_BP]
     ctx.stats_select(ks)
-    log("[bench] C4: first forward projection (counts the traced entries once)")
+    log(f"[bench] C4: phantom {time.perf_counter() - t0:.1f} s; first projection (counts the traced entries once)")
     sino = ctx.cone_forward(g, vol)
     ctx.cone_backproject(g, sino)
     ctx.stats_reset()
     for i in range(reps):
         sino = ctx.cone_forward(g, vol)
         ctx.cone_backproject(g, sino)
-        log(f"[bench] C4 rep {i + 1}/{reps}")
+        log(f"[bench] C4 projector rep {i + 1}/{reps}")
     st = {k: ctx.stats(k) for k in ks}
-    ctx.stats_select([])
     rays = g.n_views * g.n_rows * g.n_cols
     vox = g.nx * g.ny * g.nz
-    out = {"workload": "C4: 512^3 volume (0.5 mm), 720 views over 360 degrees, flat panel 768 x 1024 "
-                       "(1.0 mm), SOD 541 mm, SDD 949 mm; random volume (the projectors' cost does not "
-                       "depend on the values)", "rays": rays, "voxels": vox}
+    out = {"workload": "C4: 512^3 volume (0.5 mm), 720 views over 360 degrees, flat panel 1024 x 768 "
+                       "(0.4 mm), SOD 600 mm, SDD 1000 mm; 20-ellipsoid phantom (0-0.05 / mm)",
+           "rays": rays, "voxels": vox}
     for k, name in ((mi.MI_K_CONE_FP, "forward"), (mi.MI_K_CONE_BP, "backproject")):
         s = st[k]
         ms = s.total_ms / max(s.launches, 1)
@@ -193,6 +194,24 @@ def bench_c4(ctx, reps, log):
                      "algorithmic_bytes": b, "GBps": b / (ms * 1e-3) / 1e9,
                      "frac_of_hbm_peak": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
     out["nnz"] = int(round((out["forward"]["algorithmic_bytes"] - 4.0 * (rays + vox)) / 8.0))
+    if at_iters > 0:
+        t0 = time.perf_counter()
+        noisy = np.empty_like(sino)
+        for v in range(g.n_views):   # per view, so the float64 temporaries stay small
+            noisy[v] = phantom.noisy_sinogram(sino[v], 5e4, seed=4, stream=v)
+        del sino
+        log(f"[bench] C4: noisy sinogram (I0 5e4) {time.perf_counter() - t0:.1f} s; {at_iters} AT passes")
+        p = mi.at_params(1e-3, 1e-4, 1e-2, n_subsets=20, max_iters=at_iters)
+        ctx.stats_reset()
+        u, v, rep = ctx.cone_at_reconstruct(g, noisy, p)
+        st = {k: ctx.stats(k) for k in ks}
+        sec = rep.elapsed_ms * 1e-3
+        out["at"] = {"subsets": 20, "passes": rep.iterations, "elapsed_ms": rep.elapsed_ms,
+                     "at_iterations_per_s": rep.iterations / sec, "rays_per_s": rays * rep.iterations / sec,
+                     "data_fidelity": rep.data_fidelity,
+                     "kernel_ms": {"forward": st[mi.MI_K_CONE_FP].total_ms, "backproject": st[mi.MI_K_CONE_BP].total_ms},
+                     "rel_l2_to_phantom": float(np.linalg.norm(u - vol) / np.linalg.norm(vol))}
+    ctx.stats_select([])
     return out
 
 
@@ -212,6 +231,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--c4-reps", type=int, default=2)
+    ap.add_argument("--c4-iters", type=int, default=2, help="AT passes of the C4 cone-beam solve (0: projectors only)")
     ap.add_argument("--no-c4", action="store_true")
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="also run a plain device copy of this many GiB (reference kernel for HBM counters)")
@@ -318,7 +338,7 @@ def main():
         out["copy_reference"] = {"kernel": "k_copy16", "bytes_read": nb, "bytes_written": nb, "launches": 3,
                                  "ms": ms, "GBps": 2 * nb / (ms * 1e-3) / 1e9}
     if not args.no_c4:
-        out["cone_c4"] = bench_c4(ctx, args.c4_reps, lambda m: print(m, file=sys.stderr, flush=True))
+        out["cone_c4"] = bench_c4(ctx, args.c4_reps, args.c4_iters, lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_c3:
         job.close()
         out["batched"] = bench_c3(ctx, args.c3_jobs, args.c3_iters,

@@ -356,6 +356,16 @@ int mi_cone_forward_dev(mi_context *ctx, const mi_cone_geometry *g, const float 
  * final piece shorter than 0.05 voxel, tracing.cpp:268). */
 int mi_cone_backproject(mi_context *ctx, const mi_cone_geometry *g, const float *sino, float *vol);
 int mi_cone_backproject_dev(mi_context *ctx, const mi_cone_geometry *g, const float *sino, float *vol);
+/* Ordered-subset AT reconstruction of the volume: the OS form of
+ * mi_at_reconstruct on [nz][nx][ny] with the projectors above and the
+ * 26-neighbourhood widening of GoldenC's stencils (13 forward offsets k,
+ * weight 1/|k|^2, u zero- and v one-padded); takes the place of industry3d's
+ * per-row gdIMAGE / gdEDGE sweep (industry3d/src/main.cpp:233-532).  p uses
+ * alpha, beta, epsilon, n_subsets, max_iters, iter_edge, rel_tol, lower,
+ * upper; precision must be MI_FP32 and median_every 0.  u and v hold the
+ * initial values and receive the result. */
+int mi_cone_at_reconstruct(mi_context *ctx, const mi_cone_geometry *g, const mi_at_params *p, const float *sino,
+                           float *u, float *v, mi_report *rep);
 
 #ifdef __cplusplus
 }

@@ -197,6 +197,7 @@ def _load():
         "mi_cone_forward_dev": [P, CG, P, P],
         "mi_cone_backproject": [P, CG, P, P],
         "mi_cone_backproject_dev": [P, CG, P, P],
+        "mi_cone_at_reconstruct": [P, CG, C.POINTER(AtParams), P, P, P, C.POINTER(Report)],
     }
     for name, args in sigs.items():
         fn = getattr(lib, name)
@@ -362,6 +363,17 @@ class Context:
 
     def cone_backproject_dev(self, g: ConeGeometry, sino_ptr: int, vol_ptr: int):
         _check(self._lib.mi_cone_backproject_dev(self._h, C.byref(g), C.c_void_p(sino_ptr), C.c_void_p(vol_ptr)))
+
+    def cone_at_reconstruct(self, g: ConeGeometry, sino, p: AtParams, u0=None, v0=None):
+        """Ordered-subset AT on the cone-beam volume; returns (u, v, report)."""
+        sino = _arr(sino, np.float32, (g.n_views, g.n_rows, g.n_cols))
+        shape = (g.nz, g.nx, g.ny)
+        u = np.zeros(shape, np.float32) if u0 is None else np.array(u0, np.float32).reshape(shape).copy()
+        v = np.ones(shape, np.float32) if v0 is None else np.array(v0, np.float32).reshape(shape).copy()
+        rep = Report()
+        _check(self._lib.mi_cone_at_reconstruct(self._h, C.byref(g), C.byref(p), _ptr(sino), _ptr(u), _ptr(v),
+                                                C.byref(rep)))
+        return u, v, rep
 
     # -- GoldenC building blocks -----------------------------------------
     def image_step(self, g: Geometry, sino, f, v, alpha, epsilon, iter_image):

@@ -18,7 +18,7 @@ CSRC = PKG / "csrc"
 OBJ = PKG / "_obj"
 LIB = PKG / "libmedimage_amd.so"
 
-SOURCES = ["trace.hip", "cone.hip", "sell.hip", "kernels.hip", "engine.hip", "solver.hip", "batch.cpp", "capi.cpp"]
+SOURCES = ["trace.hip", "cone.hip", "cone_at.hip", "sell.hip", "kernels.hip", "engine.hip", "solver.hip", "batch.cpp", "capi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

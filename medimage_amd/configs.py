@@ -11,10 +11,12 @@ fixed here.
   detector width 736 mm (bin = 736 / bins); I0 uniform in [1e4, 1e5];
   8 subsets, 50 AT iterations.
 * C4 cone beam: 512^3 volume of 0.5 mm voxels, 720 views over 360 degrees
-  of a flat panel of 768 rows (NDX) by 1024 columns (NDY) of 1.0 mm, SOD
-  541 mm, SDD 949 mm.  industry3d centres both panel axes on NDX / 2, so the
-  columns span -383.5 .. 639.5 mm; the 128 mm radius cylinder of the volume
-  projects within +-231 mm and every one of its rays reaches the panel.
+  onto a 1024 x 768 flat panel (NDX = 1024 rows along z, NDY = 768 columns)
+  of 0.4 mm pitch, SOD 600 mm, SDD 1000 mm; 20 ellipsoids with attenuation
+  in [0, 0.05] / mm; I0 = 5e4; 20 subsets, 30 AT iterations.  industry3d
+  centres both panel axes on NDX / 2 (HALFDET), so the columns span
+  -204.6 .. +102.2 mm on the panel; the panel is narrower than the volume's
+  projection, and rays outside it are not measured.
 * C5 serving: sizes {256, 384, 512, 768, 1024}, views {360, 720, 1152, 2304},
   bins = 1.5 N, same field of view and detector width; I0 log-uniform in
   [1e4, 1e5]; stop at relative data-fidelity change < 1e-4 or 200 iterations.
@@ -128,6 +130,6 @@ def arc_clinical_geometry(views=1152, n=874, pixel_mm=0.5859):
 
 
 
-def c4_geometry(n=512, views=720, rows=768, cols=1024, sample_mm=0.5, pixel_mm=1.0):
+def c4_geometry(n=512, views=720, rows=1024, cols=768, sample_mm=0.5, pixel_mm=0.4, sod=600.0, sdd=1000.0):
     """C4 cone-beam geometry (industry3d Parameter, BEAM "Cone")."""
-    return cone_geometry(n, n, n, views, rows, cols, sample_mm, pixel_mm, SOD, SDD)
+    return cone_geometry(n, n, n, views, rows, cols, sample_mm, pixel_mm, sod, sdd)

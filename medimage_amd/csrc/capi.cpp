@@ -1178,3 +1178,12 @@ int mi_cone_backproject_dev(mi_context *ctx, const mi_cone_geometry *g, const fl
     return guarded([&] { cone_backproject_dev(check_ctx(ctx), g, sino, vol); });
 }
 
+
+int mi_cone_at_reconstruct(mi_context *ctx, const mi_cone_geometry *g, const mi_at_params *p, const float *sino,
+                           float *u, float *v, mi_report *rep) {
+    return guarded([&] {
+        check_ctx(ctx);
+        MI_REQUIRE(g && p, MI_E_ARG, "null argument");
+        cone_at_solve(ctx, *g, *p, sino, u, v, rep);
+    });
+}

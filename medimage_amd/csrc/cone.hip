@@ -220,27 +220,49 @@ __global__ void k_cone_trace(ConeDev g, const float *cs, int view, int row, int 
     *numb = n;
 }
 
-// Panel pixel of this lane: workgroup = 16 x 16 pixels of one view, wave =
-// 8 x 8 of them.
-__device__ __forceinline__ bool cone_fp_pixel(const ConeDev &g, int &view, int &row, int &col) {
+// Panel pixel of this lane: workgroup = 16 x 16 pixels of one view of the
+// list (q-th entry), wave = 8 x 8 of them.  False for padding lanes.
+__device__ __forceinline__ bool cone_fp_pixel(const ConeDev &g, int nv, unsigned &tile_id, int &q, int &row,
+                                              int &col) {
     const int tc = (g.ncols + kConeTile - 1) / kConeTile, tr = (g.nrows + kConeTile - 1) / kConeTile;
     const unsigned b = xcd_block();
     const unsigned per_view = (unsigned)(tc * tr);
-    view = (int)(b / per_view);
+    tile_id = b;
+    q = (int)(b / per_view);
     const int tile = (int)(b % per_view);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     row = (tile / tc) * kConeTile + (wv >> 1) * 8 + (lane >> 3);
     col = (tile % tc) * kConeTile + (wv & 1) * 8 + (lane & 7);
-    return view < g.nviews && row < g.nrows && col < g.ncols;
+    return q < nv && row < g.nrows && col < g.ncols;
 }
 
-__global__ __launch_bounds__(kBlock) void k_cone_fp(ConeDev g, const float *cs, const float *__restrict__ vol,
-                                                    float *__restrict__ sino) {
-    int view, row, col;
-    if (!cone_fp_pixel(g, view, row, col)) return;
-    ConeWalk w;
+// Double sum over a workgroup of 256 lanes in a fixed pairing; lane 0 of
+// wave 0 gets the total.
+__device__ __forceinline__ double block_sum256(double x) {
+    __shared__ double red[kBlock / kWave];
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
+    __syncthreads();
+    return (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// Forward projection of the views views[0 .. nv) (sinogram [view][row][col]).
+//   CFP_WRITE: out = A vol;
+//   CFP_RESID: out = meas - A vol (float), part[tile] = sum of out^2;
+//   CFP_SQ:    part[tile] = sum of (A vol)^2, nothing written.
+
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_cone_fp(ConeDev g, const float *cs, const int *views, int nv,
+                                                    const float *__restrict__ vol, const float *__restrict__ meas,
+                                                    float *__restrict__ out, double *__restrict__ part) {
+    unsigned tile_id;
+    int q, row, col;
+    const bool ok = cone_fp_pixel(g, nv, tile_id, q, row, col);
+    if (q >= nv) return;   // whole padding workgroup
     double acc = 0.0;
-    if (cone_ray_init(g, cs, view, row, col, w)) {
+    int view = 0;
+    ConeWalk w;
+    if (ok && (view = views[q], cone_ray_init(g, cs, view, row, col, w))) {
         bool live = true;
         while (live) {
             int id[kConeGroup];
@@ -255,15 +277,29 @@ __global__ __launch_bounds__(kBlock) void k_cone_fp(ConeDev g, const float *cs, 
             for (int k = 0; k < kConeGroup; ++k) acc = lv[k] ? acc + (double)v[k] * (double)ln[k] : acc;
         }
     }
-    sino[((size_t)view * g.nrows + row) * g.ncols + col] = (float)acc;
+    const size_t at = ((size_t)view * g.nrows + row) * g.ncols + col;
+    if (MODE == CFP_WRITE) {
+        if (ok) out[at] = (float)acc;
+        return;
+    }
+    double sq = 0.0;
+    if (MODE == CFP_RESID && ok) {
+        const float r = (float)((double)meas[at] - acc);
+        out[at] = r;
+        sq = (double)r * (double)r;
+    }
+    if (MODE == CFP_SQ && ok) sq = acc * acc;
+    const double tot = block_sum256(sq);
+    if (threadIdx.x == 0) part[tile_id] = tot;
 }
 
 // Entry count of every ray (for the algorithmic-byte figure).
 __global__ __launch_bounds__(kBlock) void k_cone_count(ConeDev g, const float *cs,
                                                        unsigned long long *total) {
+    unsigned tile_id;
     int view, row, col;
     unsigned long long n = 0;
-    if (cone_fp_pixel(g, view, row, col)) {
+    if (cone_fp_pixel(g, g.nviews, tile_id, view, row, col)) {
         ConeWalk w;
         if (cone_ray_init(g, cs, view, row, col, w)) {
             bool live = true;
@@ -296,7 +332,8 @@ __device__ __forceinline__ void cone_slab(double s, double r, double a, double &
 }
 
 __global__ __launch_bounds__(kBlock) void k_cone_bp(ConeDev g, const float *cs, const double *rowtab,
-                                                    const float *__restrict__ sino, float *__restrict__ vol) {
+                                                    const int *views, int nv, const float *__restrict__ sino,
+                                                    float *__restrict__ vol) {
     // Workgroup: 16 x 16 voxel columns, wave 8 x 8 of them; kZc slices.
     const int tx = (g.nx + 15) / 16, ty = (g.ny + 15) / 16;
     const unsigned b = xcd_block();
@@ -318,7 +355,8 @@ __global__ __launch_bounds__(kBlock) void k_cone_bp(ConeDev g, const float *cs, 
     const float h5 = (float)g.halfdet - 0.5f;
     const float margin = 0.05f;                  // in pixels, covers the float rounding of the bounds
 
-    for (int v = 0; v < g.nviews; ++v) {
+    for (int iv = 0; iv < nv; ++iv) {
+        const int v = views[iv];
         const float c = cs[2 * v], s = cs[2 * v + 1];
         float sx, sy, sz;
         cone_source(g, c, s, sx, sy, sz);
@@ -420,6 +458,10 @@ ConeTables &cone_tables(mi_context *ctx, const mi_cone_geometry &g) {
         rows[2 * r] = w;
         rows[2 * r + 1] = 1.0 / w;
     }
+    std::vector<int> all(g.n_views);
+    for (int a = 0; a < g.n_views; ++a) all[a] = a;
+    t->all.alloc(all.size());
+    MI_HIP(hipMemcpyAsync(t->all.p, all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     t->cs.alloc(cs.size());
     t->rows.alloc(rows.size());
     MI_HIP(hipMemcpyAsync(t->cs.p, cs.data(), cs.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
@@ -436,19 +478,41 @@ void cone_trace(hipStream_t st, const ConeTables &t, int view, int row, int col,
     MI_HIP(hipGetLastError());
 }
 
-static size_t cone_fp_blocks(const ConeDev &d) {
-    return (size_t)((d.ncols + kConeTile - 1) / kConeTile) * ((d.nrows + kConeTile - 1) / kConeTile) * d.nviews;
+static size_t cone_fp_blocks(const ConeDev &d, int nv) {
+    return (size_t)((d.ncols + kConeTile - 1) / kConeTile) * ((d.nrows + kConeTile - 1) / kConeTile) * nv;
+}
+
+size_t cone_fp_views(hipStream_t st, const ConeTables &t, const int *views, int nv, int mode, const float *vol,
+                     const float *meas, float *out, double *part) {
+    const size_t nb = cone_fp_blocks(t.d, nv);
+    const dim3 grid(grid8(nb)), block(kBlock);
+    switch (mode) {
+        case CFP_WRITE:
+            hipLaunchKernelGGL(k_cone_fp<CFP_WRITE>, grid, block, 0, st, t.d, t.cs.p, views, nv, vol, meas, out, part);
+            break;
+        case CFP_RESID:
+            hipLaunchKernelGGL(k_cone_fp<CFP_RESID>, grid, block, 0, st, t.d, t.cs.p, views, nv, vol, meas, out, part);
+            break;
+        default:
+            hipLaunchKernelGGL(k_cone_fp<CFP_SQ>, grid, block, 0, st, t.d, t.cs.p, views, nv, vol, meas, out, part);
+    }
+    MI_HIP(hipGetLastError());
+    return nb;
+}
+
+void cone_bp_views(hipStream_t st, const ConeTables &t, const int *views, int nv, const float *sino, float *vol) {
+    const size_t blocks = (size_t)((t.d.nx + 15) / 16) * ((t.d.ny + 15) / 16) * ((t.d.nz + kZc - 1) / kZc);
+    hipLaunchKernelGGL(k_cone_bp, dim3(grid8(blocks)), dim3(kBlock), 0, st, t.d, t.cs.p, t.rows.p, views, nv, sino,
+                       vol);
+    MI_HIP(hipGetLastError());
 }
 
 void cone_forward(hipStream_t st, const ConeTables &t, const float *vol, float *sino) {
-    hipLaunchKernelGGL(k_cone_fp, dim3(grid8(cone_fp_blocks(t.d))), dim3(kBlock), 0, st, t.d, t.cs.p, vol, sino);
-    MI_HIP(hipGetLastError());
+    cone_fp_views(st, t, t.all.p, t.d.nviews, CFP_WRITE, vol, nullptr, sino, nullptr);
 }
 
 void cone_backproject(hipStream_t st, const ConeTables &t, const float *sino, float *vol) {
-    const size_t blocks = (size_t)((t.d.nx + 15) / 16) * ((t.d.ny + 15) / 16) * ((t.d.nz + kZc - 1) / kZc);
-    hipLaunchKernelGGL(k_cone_bp, dim3(grid8(blocks)), dim3(kBlock), 0, st, t.d, t.cs.p, t.rows.p, sino, vol);
-    MI_HIP(hipGetLastError());
+    cone_bp_views(st, t, t.all.p, t.d.nviews, sino, vol);
 }
 
 double cone_fp_bytes(const ConeTables &t) {
@@ -463,7 +527,8 @@ int64_t cone_nnz(hipStream_t st, ConeTables &t) {
     if (t.nnz >= 0) return t.nnz;
     DevBuf<unsigned long long> n(1);
     MI_HIP(hipMemsetAsync(n.p, 0, sizeof(unsigned long long), st));
-    hipLaunchKernelGGL(k_cone_count, dim3(grid8(cone_fp_blocks(t.d))), dim3(kBlock), 0, st, t.d, t.cs.p, n.p);
+    hipLaunchKernelGGL(k_cone_count, dim3(grid8(cone_fp_blocks(t.d, t.d.nviews))), dim3(kBlock), 0, st, t.d, t.cs.p,
+                       n.p);
     MI_HIP(hipGetLastError());
     unsigned long long h = 0;
     MI_HIP(hipMemcpyAsync(&h, n.p, sizeof(h), hipMemcpyDeviceToHost, st));

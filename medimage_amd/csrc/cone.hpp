@@ -20,6 +20,7 @@ struct ConeTables {
     ConeDev d{};
     DevBuf<float> cs;      // [2 * nviews]: cos, sin interleaved
     DevBuf<double> rows;   // [2 * nrows]: w, 1 / w
+    DevBuf<int> all;       // 0 .. nviews - 1 (the view list of a full projection)
     int64_t nnz = -1;      // traced entries of all rays (-1: not counted yet)
 };
 
@@ -31,6 +32,23 @@ void cone_trace(hipStream_t st, const ConeTables &t, int view, int row, int col,
                 int cap, int *numb);
 void cone_forward(hipStream_t st, const ConeTables &t, const float *vol, float *sino);
 void cone_backproject(hipStream_t st, const ConeTables &t, const float *sino, float *vol);
+
+// Forward projection over the device view list views[0 .. nv):
+//   CFP_WRITE: out = A vol;
+//   CFP_RESID: out = meas - A vol (float), part[i] = sum of out^2 over the
+//              rays of workgroup i;
+//   CFP_SQ:    part[i] = sum of (A vol)^2, nothing written.
+// Returns the number of partial sums written (part needs that many).
+enum { CFP_WRITE = 0, CFP_RESID = 1, CFP_SQ = 2 };
+size_t cone_fp_views(hipStream_t st, const ConeTables &t, const int *views, int nv, int mode, const float *vol,
+                     const float *meas, float *out, double *part);
+// Gather back projection over the view list (overwrites vol).
+void cone_bp_views(hipStream_t st, const ConeTables &t, const int *views, int nv, const float *sino, float *vol);
+
+// Ordered-subset AT reconstruction of the volume (cone_at.hip): host sino,
+// u and v (initial values in, results out).
+void cone_at_solve(mi_context *ctx, const mi_cone_geometry &g, const mi_at_params &p, const float *sino,
+                   float *u, float *v, mi_report *rep);
 
 // Traced entries of all rays (counted on the device once per geometry).
 int64_t cone_nnz(hipStream_t st, ConeTables &t);

@@ -108,3 +108,41 @@ def analytic_fan_sinogram(ellipses, g):
         disc = Bq * Bq - 4 * A * Cq
         out += mu * np.sqrt(np.maximum(disc, 0.0)) / A
     return out
+
+
+def random_ellipsoids(n, fov_mm, seed, stream=0, axes_mm=(5.0, 100.0), mu_max=0.05):
+    """n ellipsoids (cx, cy, cz, a, b, c, theta, mu): centres uniform in the
+    inner 80% of the field of view, semi-axes uniform in axes_mm, rotation
+    theta about z uniform in [0, pi), attenuation uniform in [0, mu_max] / mm."""
+    g = rng(seed, 300 + stream)
+    out = []
+    for _ in range(n):
+        cx, cy, cz = (float(t) for t in g.uniform(-0.4 * fov_mm, 0.4 * fov_mm, 3))
+        a, b, c = (float(t) for t in g.uniform(axes_mm[0], axes_mm[1], 3))
+        out.append((cx, cy, cz, a, b, c, float(g.uniform(0.0, math.pi)), float(g.uniform(0.0, mu_max))))
+    return out
+
+
+def rasterize3d(ellipsoids, nz, nx, ny, voxel_mm):
+    """Volume [nz][nx][ny] (float32) of the ellipsoids sampled at voxel
+    centres ((i + 0.5 - n / 2) * voxel_mm along each axis); overlaps add."""
+    vol = np.zeros((nz, nx, ny), np.float32)
+    cz_ = ((np.arange(nz) + 0.5) - nz / 2) * voxel_mm
+    cx_ = ((np.arange(nx) + 0.5) - nx / 2) * voxel_mm
+    cy_ = ((np.arange(ny) + 0.5) - ny / 2) * voxel_mm
+    for cx, cy, cz, a, b, c, t, mu in ellipsoids:
+        r = max(a, b)
+        xs = np.nonzero(np.abs(cx_ - cx) <= r)[0]
+        ys = np.nonzero(np.abs(cy_ - cy) <= r)[0]
+        zs = np.nonzero(np.abs(cz_ - cz) <= c)[0]
+        if not (len(xs) and len(ys) and len(zs)):
+            continue
+        X = (cx_[xs] - cx)[:, None]
+        Y = (cy_[ys] - cy)[None, :]
+        xr = X * math.cos(t) + Y * math.sin(t)
+        yr = -X * math.sin(t) + Y * math.cos(t)
+        q2 = (xr / a) ** 2 + (yr / b) ** 2
+        zq = ((cz_[zs] - cz) / c) ** 2
+        inside = (q2[None, :, :] + zq[:, None, None]) <= 1.0
+        vol[zs[0]:zs[-1] + 1, xs[0]:xs[-1] + 1, ys[0]:ys[-1] + 1] += np.float32(mu) * inside
+    return vol

@@ -270,3 +270,217 @@ double or_cone_clip(const float *Sf, const float *Df, int x, int y, int z) {
     if (!(hi > lo)) return 0.0;
     return (hi - lo) * sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
 }
+
+/* ---------------------------------------------------------------------------
+ * Ordered-subset AT iteration on the cone-beam volume (the 3D form the
+ * device runs).  Regulariser over the 26-neighbourhood: the 13 offsets k
+ * with a positive first non-zero component, weight 1 / |k|^2, every pair
+ * (x, x + k) with at least one voxel inside the volume, f zero-padded and v
+ * one-padded (the padding of origin/GoldenC/src/main.c:112-125):
+ *   R(f; v) = sum w_k (v(x)^2 + eta) (f(x + k) - f(x))^2,  eta = 0.01 eps^2.
+ * u step: d = S A_s^T (g_s - A_s u) - alpha/2 dR/df, exact line search
+ * c = |d|^2 / (S |A_s d|^2 + alpha R(d; v)), u = clamp(u + c d).
+ * v step: AT energy alpha sum v^2 n + beta eps Q(v) + beta/(4 eps) |1 - v|^2
+ * with n(x) = sum_k w_k (f(x + k) - f(x))^2 and Q the same pair sum of v
+ * with unit weights; iter_edge steepest-descent steps with exact line
+ * search, v clamped to [0, 1].  This is the 2D ordered-subset form
+ * (or_os_at) with GoldenC's 4-neighbour stencils widened to 26 neighbours.
+ * ------------------------------------------------------------------------- */
+static const int K13[13][3] = {{0, 0, 1},  {0, 1, -1}, {0, 1, 0},  {0, 1, 1},  {1, -1, -1},
+                               {1, -1, 0}, {1, -1, 1}, {1, 0, -1}, {1, 0, 0},  {1, 0, 1},
+                               {1, 1, -1}, {1, 1, 0},  {1, 1, 1}};   /* (dz, dx, dy) */
+
+static double kw(int k) {
+    return 1.0 / (double)(K13[k][0] * K13[k][0] + K13[k][1] * K13[k][1] + K13[k][2] * K13[k][2]);
+}
+
+static int inside(int z, int x, int y, int nz, int nx, int ny) {
+    return z >= 0 && z < nz && x >= 0 && x < nx && y >= 0 && y < ny;
+}
+
+static double zp3(const double *f, int z, int x, int y, int nz, int nx, int ny) {
+    return inside(z, x, y, nz, nx, ny) ? f[((size_t)z * nx + x) * ny + y] : 0.0;
+}
+
+static double op3(const double *v, int z, int x, int y, int nz, int nx, int ny) {
+    return inside(z, x, y, nz, nx, ny) ? v[((size_t)z * nx + x) * ny + y] : 1.0;
+}
+
+void or_cone_ugrad(double *d, const double *f, const double *v, double alpha, double eps, int nz, int nx, int ny) {
+    const double eta = 0.01 * eps * eps;
+    for (int z = 0; z < nz; ++z)
+        for (int x = 0; x < nx; ++x)
+            for (int y = 0; y < ny; ++y) {
+                const double f0 = f[((size_t)z * nx + x) * ny + y], v0 = v[((size_t)z * nx + x) * ny + y];
+                double acc = 0.0;
+                for (int k = 0; k < 13; ++k) {
+                    const int a = K13[k][0], b = K13[k][1], c = K13[k][2];
+                    const double fp = zp3(f, z + a, x + b, y + c, nz, nx, ny);
+                    const double fm = zp3(f, z - a, x - b, y - c, nz, nx, ny);
+                    const double vm = op3(v, z - a, x - b, y - c, nz, nx, ny);
+                    acc += kw(k) * ((v0 * v0 + eta) * (fp - f0) - (vm * vm + eta) * (f0 - fm));
+                }
+                d[((size_t)z * nx + x) * ny + y] += alpha * acc;
+            }
+}
+
+/* Pair sum of p (zero outside) with weights (v(x)^2 + eta) when wv, else 1. */
+static double pair_sum(const double *p, const double *v, double eta, int wv, int nz, int nx, int ny) {
+    double acc = 0.0;
+    for (int z = 0; z < nz; ++z)
+        for (int x = 0; x < nx; ++x)
+            for (int y = 0; y < ny; ++y) {
+                const double p0 = p[((size_t)z * nx + x) * ny + y];
+                const double v0 = v ? v[((size_t)z * nx + x) * ny + y] : 1.0;
+                const double s0 = wv ? v0 * v0 + eta : 1.0, so = wv ? 1.0 + eta : 1.0;
+                for (int k = 0; k < 13; ++k) {
+                    const int a = K13[k][0], b = K13[k][1], c = K13[k][2];
+                    const double q = zp3(p, z + a, x + b, y + c, nz, nx, ny) - p0;
+                    acc += kw(k) * s0 * q * q;
+                    if (!inside(z - a, x - b, y - c, nz, nx, ny)) acc += kw(k) * so * p0 * p0;
+                }
+            }
+    return acc;
+}
+
+double or_cone_nV(const double *p, const double *v, double eps, int nz, int nx, int ny) {
+    return pair_sum(p, v, 0.01 * eps * eps, 1, nz, nx, ny);
+}
+
+void or_cone_edge_step(double *v, const double *f, double alpha, double beta, double eps, int iters, int nz,
+                       int nx, int ny) {
+    const size_t nv = (size_t)nz * nx * ny;
+    double *dv = (double *)malloc(sizeof(double) * nv), *nn = (double *)malloc(sizeof(double) * nv);
+    for (int z = 0; z < nz; ++z)
+        for (int x = 0; x < nx; ++x)
+            for (int y = 0; y < ny; ++y) {
+                const double f0 = f[((size_t)z * nx + x) * ny + y];
+                double acc = 0.0;
+                for (int k = 0; k < 13; ++k) {
+                    const double q = zp3(f, z + K13[k][0], x + K13[k][1], y + K13[k][2], nz, nx, ny) - f0;
+                    acc += kw(k) * q * q;
+                }
+                nn[((size_t)z * nx + x) * ny + y] = acc;
+            }
+    for (int it = 0; it < iters; ++it) {
+        double num = 0.0, dn = 0.0;
+        for (int z = 0; z < nz; ++z)
+            for (int x = 0; x < nx; ++x)
+                for (int y = 0; y < ny; ++y) {
+                    const size_t i = ((size_t)z * nx + x) * ny + y;
+                    const double v0 = v[i];
+                    double lap = 0.0;
+                    for (int k = 0; k < 13; ++k) {
+                        const int a = K13[k][0], b = K13[k][1], c = K13[k][2];
+                        lap += kw(k) * ((op3(v, z + a, x + b, y + c, nz, nx, ny) - v0) -
+                                        (v0 - op3(v, z - a, x - b, y - c, nz, nx, ny)));
+                    }
+                    dv[i] = -alpha * v0 * nn[i] + beta * eps * lap - beta / (4.0 * eps) * (v0 - 1.0);
+                    num += dv[i] * dv[i];
+                    dn += nn[i] * dv[i] * dv[i];
+                }
+        const double den = alpha * dn + beta * eps * pair_sum(dv, NULL, 0.0, 0, nz, nx, ny) +
+                           beta / (4.0 * eps) * num;
+        const double c = den > 0 ? num / den : 0.0;
+        for (size_t i = 0; i < nv; ++i) {
+            double t = v[i] + c * dv[i];
+            if (t < 0.0) t = 0.0;
+            if (t > 1.0) t = 1.0;
+            v[i] = t;
+        }
+    }
+    free(dv);
+    free(nn);
+}
+
+void or_cone_fp_views(const or_cone_geom *g, const int *views, int nv, const double *vol, double *sino) {
+    const int cap = 4 * (g->nx + g->ny + g->nz) + 8;
+    int64_t *ind = (int64_t *)malloc(sizeof(int64_t) * cap);
+    float *wgt = (float *)malloc(sizeof(float) * cap);
+    float *ct = (float *)malloc(sizeof(float) * g->nproj), *st = (float *)malloc(sizeof(float) * g->nproj);
+    or_cone_tables(g, ct, st);
+    for (int q = 0; q < nv; ++q) {
+        const int a = views[q];
+        for (int r = 0; r < g->ndx; ++r)
+            for (int c = 0; c < g->ndy; ++c) {
+                float S[3], D[3];
+                or_cone_ray(g, ct[a], st[a], r, c, S, D);
+                const int n = or_cone_walk(g, S, D, ind, wgt, cap);
+                double acc = 0.0;
+                for (int i = 0; i < n && i < cap; ++i) acc += vol[ind[i]] * (double)wgt[i];
+                sino[((int64_t)a * g->ndx + r) * g->ndy + c] = acc;
+            }
+    }
+    free(ind); free(wgt); free(ct); free(st);
+}
+
+void or_cone_bp_views(const or_cone_geom *g, const int *views, int nv, const double *sino, double *vol) {
+    const int cap = 4 * (g->nx + g->ny + g->nz) + 8;
+    int64_t *ind = (int64_t *)malloc(sizeof(int64_t) * cap);
+    double *len = (double *)malloc(sizeof(double) * cap);
+    float *ct = (float *)malloc(sizeof(float) * g->nproj), *st = (float *)malloc(sizeof(float) * g->nproj);
+    or_cone_tables(g, ct, st);
+    memset(vol, 0, sizeof(double) * (size_t)g->nx * g->ny * g->nz);
+    for (int q = 0; q < nv; ++q) {
+        const int a = views[q];
+        for (int r = 0; r < g->ndx; ++r)
+            for (int c = 0; c < g->ndy; ++c) {
+                float S[3], D[3];
+                or_cone_ray(g, ct[a], st[a], r, c, S, D);
+                const int n = or_cone_exact(g, S, D, ind, len, cap);
+                const double y = sino[((int64_t)a * g->ndx + r) * g->ndy + c];
+                for (int i = 0; i < n && i < cap; ++i) vol[ind[i]] += y * len[i];
+            }
+    }
+    free(ind); free(len); free(ct); free(st);
+}
+
+int or_cone_os_at(const or_cone_geom *g, const double *sino, double *u, double *v, const or_at_params *p,
+                  double *df_hist) {
+    const int nz = g->nz, nx = g->nx, ny = g->ny, S = p->n_subsets;
+    const size_t ni = (size_t)nz * nx * ny, ns = (size_t)g->nproj * g->ndx * g->ndy, pv = (size_t)g->ndx * g->ndy;
+    int *views = (int *)malloc(sizeof(int) * g->nproj);
+    double *q = (double *)calloc(ns, sizeof(double));
+    double *b = (double *)malloc(sizeof(double) * ni);
+    double *d = (double *)malloc(sizeof(double) * ni);
+    double df_prev = 0;
+    int it;
+    for (it = 1; it <= p->max_iters; ++it) {
+        double df = 0;
+        for (int s = 0; s < S; ++s) {
+            const int nv = or_subset_views(g->nproj, S, s, views);
+            double num = 0, ad2 = 0;
+            or_cone_fp_views(g, views, nv, u, q);
+            for (int k = 0; k < nv; ++k)
+                for (size_t j = 0; j < pv; ++j) {
+                    const size_t i = (size_t)views[k] * pv + j;
+                    q[i] = sino[i] - q[i];
+                    df += q[i] * q[i];
+                }
+            or_cone_bp_views(g, views, nv, q, b);
+            for (size_t i = 0; i < ni; ++i) d[i] = S * b[i];
+            or_cone_ugrad(d, u, v, p->alpha, p->eps, nz, nx, ny);
+            or_cone_fp_views(g, views, nv, d, q);
+            for (int k = 0; k < nv; ++k)
+                for (size_t j = 0; j < pv; ++j) {
+                    const double t = q[(size_t)views[k] * pv + j];
+                    ad2 += t * t;
+                }
+            for (size_t i = 0; i < ni; ++i) num += d[i] * d[i];
+            const double den = S * ad2 + p->alpha * or_cone_nV(d, v, p->eps, nz, nx, ny);
+            const double c = (den > 0) ? num / den : 0.0;
+            for (size_t i = 0; i < ni; ++i) {
+                double x = u[i] + c * d[i];
+                if (x < p->lower) x = p->lower;
+                if (x > p->upper) x = p->upper;
+                u[i] = x;
+            }
+        }
+        if (p->iter_edge > 0) or_cone_edge_step(v, u, p->alpha, p->beta, p->eps, p->iter_edge, nz, nx, ny);
+        if (df_hist) df_hist[it - 1] = df;
+        if (it > 1 && fabs(df - df_prev) < p->rel_tol * df_prev) break;
+        df_prev = df;
+    }
+    free(views); free(q); free(b); free(d);
+    return (it > p->max_iters) ? p->max_iters : it;
+}

@@ -34,6 +34,11 @@
  *                                 tracing.cpp needs boost through utility.cpp.
  *   or_cone_exact / or_cone_bp    per-voxel slab clip of every voxel
  *                                 (or_cone_clip) and the chord length.
+ *   or_cone_ugrad / or_cone_nV /  finite differences of the regulariser and
+ *   or_cone_edge_step             edge energy (gradient and exact quadratic
+ *                                 term), invariants (constant f, v = 1).
+ *   or_cone_os_at                 line search against a brute-force
+ *                                 minimisation along the direction.
  */
 #ifndef MEDIMAGE_ORACLE_H
 #define MEDIMAGE_ORACLE_H
@@ -224,6 +229,19 @@ int or_cone_exact(const or_cone_geom *g, const float *src, const float *dst, int
 void or_cone_bp(const or_cone_geom *g, const double *sino, double *vol);
 /* Length of the ray src -> dst inside voxel (x, y, z), slab clip in double. */
 double or_cone_clip(const float *src, const float *dst, int x, int y, int z);
+/* A and the exact-length transpose over a list of views, double vectors. */
+void or_cone_fp_views(const or_cone_geom *g, const int *views, int nv, const double *vol, double *sino);
+void or_cone_bp_views(const or_cone_geom *g, const int *views, int nv, const double *sino, double *vol);
+/* 26-neighbourhood AT stencils (cone3d.c): d += alpha * (-1/2 dR/df);
+ * R(p; v) of the line search; iters edge steps on v in place. */
+void or_cone_ugrad(double *d, const double *f, const double *v, double alpha, double eps, int nz, int nx, int ny);
+double or_cone_nV(const double *p, const double *v, double eps, int nz, int nx, int ny);
+void or_cone_edge_step(double *v, const double *f, double alpha, double beta, double eps, int iters, int nz,
+                       int nx, int ny);
+/* Ordered-subset AT on the cone-beam volume (subsets as or_subset_views;
+ * median_every is not used).  Returns passes run. */
+int or_cone_os_at(const or_cone_geom *g, const double *sino, double *u, double *v, const or_at_params *p,
+                  double *df_hist);
 
 #ifdef __cplusplus
 }

@@ -385,3 +385,17 @@ def cone_bp(cg, sino):
     out = np.zeros((cg.nz, cg.nx, cg.ny), np.float64)
     _cone_lib().or_cone_bp(C.byref(cg), P(sino), P(out))
     return out
+
+
+def cone_os_at(cg, sino, u, v, alpha, beta, eps, S, iters, iter_edge=1, rel_tol=0.0, lower=-1e30, upper=1e30):
+    """3D ordered-subset AT in FP64 (or_cone_os_at); returns (u, v, passes, df_hist)."""
+    L = _cone_lib()
+    L.or_cone_os_at.restype = C.c_int
+    L.or_cone_os_at.argtypes = [C.c_void_p] * 6
+    u = np.array(u, np.float64).copy()
+    v = np.array(v, np.float64).copy()
+    sino = np.ascontiguousarray(sino, np.float64)
+    p = AtParams(alpha, beta, eps, S, iters, iter_edge, 0, rel_tol, lower, upper)
+    hist = np.zeros(max(iters, 1))
+    n = L.or_cone_os_at(C.byref(cg), P(sino), P(u), P(v), C.byref(p), P(hist))
+    return u, v, n, hist[:n]

@@ -177,3 +177,168 @@ def test_fp_bp_pair():
     lhs = float(np.sum(Ax * y))
     rhs = float(np.sum(x.astype(np.float64) * Bty))
     assert abs(lhs - rhs) / abs(rhs) < 2e-3
+
+
+# --------------------------------------------------------------------------
+# 26-neighbourhood AT stencils and the 3D ordered-subset step, checked
+# against an independent numpy statement of the energies (pair sums over the
+# padded volume) and finite differences.
+import ctypes as C
+
+OFFS = [(a, b, c) for a in (-1, 0, 1) for b in (-1, 0, 1) for c in (-1, 0, 1)
+        if (a, b, c) > (0, 0, 0)]
+
+
+def _pairs(p, wfun):
+    """sum over offsets k of the 13 forward ones and all x of the one-voxel
+    padded grid where x or x + k is inside: w_k * wfun(x) * (p(x+k) - p(x))^2,
+    p zero outside."""
+    nz, nx, ny = p.shape
+    P = np.zeros((nz + 4, nx + 4, ny + 4))
+    P[2:-2, 2:-2, 2:-2] = p
+    inside = np.zeros(P.shape, bool)
+    inside[2:-2, 2:-2, 2:-2] = True
+    W = wfun(P.shape)
+    tot = 0.0
+    for (a, b, c) in OFFS:
+        w = 1.0 / (a * a + b * b + c * c)
+        sl0 = (slice(1, -2), slice(1, -2), slice(1, -2))
+        x0 = P[1:-1, 1:-1, 1:-1]
+        xk = P[1 + a:P.shape[0] - 1 + a, 1 + b:P.shape[1] - 1 + b, 1 + c:P.shape[2] - 1 + c]
+        m = inside[1:-1, 1:-1, 1:-1] | inside[1 + a:P.shape[0] - 1 + a, 1 + b:P.shape[1] - 1 + b,
+                                              1 + c:P.shape[2] - 1 + c]
+        tot += w * float(np.sum((W[1:-1, 1:-1, 1:-1] * (xk - x0) ** 2)[m]))
+    return tot
+
+
+def _R(f, v, eps):
+    eta = 0.01 * eps * eps
+    def wf(shape):
+        V = np.ones(shape)
+        V[2:-2, 2:-2, 2:-2] = v
+        return V * V + eta
+    return _pairs(f, wf)
+
+
+def _Q(p):
+    return _pairs(p, lambda shape: np.ones(shape))
+
+
+def _n(f):
+    out = np.zeros_like(f)
+    nz, nx, ny = f.shape
+    F = np.zeros((nz + 2, nx + 2, ny + 2))
+    F[1:-1, 1:-1, 1:-1] = f
+    for (a, b, c) in OFFS:
+        w = 1.0 / (a * a + b * b + c * c)
+        out += w * (F[1 + a:nz + 1 + a, 1 + b:nx + 1 + b, 1 + c:ny + 1 + c] - f) ** 2
+    return out
+
+
+def _E_edge(v, f, alpha, beta, eps):
+    V1 = v - 1.0       # Q of v with one-padding equals Q of (v - 1) with zero-padding
+    return alpha * float(np.sum(v * v * _n(f))) + beta * eps * _Q(V1) + beta / (4 * eps) * float(np.sum(V1 ** 2))
+
+
+def _fd_grad(E, x, h=1e-3):
+    g = np.zeros_like(x)
+    it = np.nditer(x, flags=["multi_index"])
+    for _ in it:
+        i = it.multi_index
+        xp = x.copy(); xp[i] += h
+        xm = x.copy(); xm[i] -= h
+        g[i] = (E(xp) - E(xm)) / (2 * h)
+    return g
+
+
+def test_cone_stencils_match_energies():
+    L = O._cone_lib()
+    rng = np.random.default_rng(5)
+    nz, nx, ny = 3, 4, 5
+    f = rng.random((nz, nx, ny))
+    v = 0.2 + 0.6 * rng.random((nz, nx, ny))
+    eps, alpha = 0.3, 0.7
+    d = np.zeros_like(f)
+    L.or_cone_ugrad.argtypes = [C.c_void_p] * 3 + [C.c_double, C.c_double, C.c_int, C.c_int, C.c_int]
+    L.or_cone_ugrad(O.P(d), O.P(f), O.P(v), alpha, eps, nz, nx, ny)
+    want = -0.5 * alpha * _fd_grad(lambda x: _R(x, v, eps), f)
+    assert np.allclose(d, want, rtol=1e-8, atol=1e-10)
+    L.or_cone_nV.restype = C.c_double
+    L.or_cone_nV.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int]
+    p = rng.standard_normal((nz, nx, ny))
+    assert abs(L.or_cone_nV(O.P(p), O.P(v), eps, nz, nx, ny) - _R(p, v, eps)) < 1e-10 * _R(p, v, eps)
+    # invariants: constant interior f with v = 1 has only the boundary terms;
+    # f = 0 gives a zero gradient
+    d0 = np.zeros_like(f)
+    L.or_cone_ugrad(O.P(d0), O.P(np.zeros_like(f)), O.P(v), alpha, eps, nz, nx, ny)
+    assert not d0.any()
+
+
+def test_cone_edge_step_exact_line_search():
+    L = O._cone_lib()
+    L.or_cone_edge_step.argtypes = [C.c_void_p, C.c_void_p] + [C.c_double] * 3 + [C.c_int] * 4
+    rng = np.random.default_rng(6)
+    nz, nx, ny = 3, 4, 4
+    f = 0.3 * rng.random((nz, nx, ny))
+    v = 0.4 + 0.2 * rng.random((nz, nx, ny))
+    alpha, beta, eps = 0.5, 0.05, 0.2
+    E = lambda x: _E_edge(x, f, alpha, beta, eps)
+    dv = -0.5 * _fd_grad(E, v)
+    # exact minimiser of the quadratic E(v + c dv)
+    e0, e1, e2 = E(v), E(v + dv), E(v - dv)
+    c = (e2 - e1) / (2 * (e1 + e2 - 2 * e0))
+    vn = v.copy()
+    L.or_cone_edge_step(O.P(vn), O.P(f), alpha, beta, eps, 1, nz, nx, ny)
+    assert np.all((vn > 0) & (vn < 1))      # no clamping in this case
+    assert np.allclose(vn, v + c * dv, rtol=1e-7, atol=1e-9)
+    # v = 1 with constant f = 0 is a fixed point
+    one = np.ones_like(v)
+    L.or_cone_edge_step(O.P(one), O.P(np.zeros_like(f)), alpha, beta, eps, 2, nz, nx, ny)
+    assert np.all(one == 1.0)
+
+
+def test_cone_os_at_step_from_dense_matrices():
+    """One pass of two subsets, no clamp and no edge step, rebuilt from the
+    dense matrices of the traced entries (A) and of the exact lengths (B),
+    numpy's regulariser gradient and the exact line search."""
+    g = geom(nx=5, ny=4, nz=3, nproj=8, ndx=5, ndy=7, pixel=2.5, sod=25.0, sdd=45.0)
+    nvox = g.nx * g.ny * g.nz
+    nr = g.ndx * g.ndy
+    A = np.zeros((g.nproj * nr, nvox))
+    B = np.zeros((g.nproj * nr, nvox))
+    for a in range(g.nproj):
+        for r in range(g.ndx):
+            for k in range(g.ndy):
+                row = (a * g.ndx + r) * g.ndy + k
+                ind, w = O.cone_trace(g, a, r, k)
+                np.add.at(A[row], ind, w.astype(np.float64))
+                S, D = O.cone_ray(g, a, r, k)
+                ind, ln = O.cone_exact(g, S, D)
+                np.add.at(B[row], ind, ln)
+    assert np.count_nonzero(A.sum(1)) > 40
+    rng = np.random.default_rng(8)
+    sino = A @ rng.random(nvox) + 0.01 * rng.standard_normal(A.shape[0])
+    u0 = 0.1 * rng.random(nvox)
+    v0 = 0.5 + 0.5 * rng.random(nvox)
+    alpha, beta, eps, S = 0.05, 0.01, 0.3, 2
+    u = u0.copy()
+    shape = (g.nz, g.nx, g.ny)
+    order = np.zeros(S, np.int32)
+    O.lib().or_subset_order(S, O.P(order))
+    for s in range(S):
+        views = [a for a in range(g.nproj) if a % S == order[s]]
+        rows = np.concatenate([np.arange(a * nr, (a + 1) * nr) for a in views])
+        q = sino[rows] - A[rows] @ u
+        d = S * (B[rows].T @ q)
+        d += -0.5 * alpha * _fd_grad(lambda x: _R(x, v0.reshape(shape), eps), u.reshape(shape)).reshape(-1)
+        c = float(d @ d) / (S * float(np.sum((A[rows] @ d) ** 2)) + alpha * _R(d.reshape(shape), v0.reshape(shape), eps))
+        u = u + c * d
+    p = O.AtParams(alpha, beta, eps, S, 1, 0, 0, 0.0, -1e30, 1e30)
+    uo = u0.copy()
+    vo = v0.copy()
+    L = O._cone_lib()
+    L.or_cone_os_at.argtypes = [C.c_void_p] * 5 + [C.c_void_p]
+    n = L.or_cone_os_at(C.byref(g), O.P(sino), O.P(uo), O.P(vo), C.byref(p), None)
+    assert n == 1
+    assert np.allclose(uo, u, rtol=1e-6, atol=1e-9)
+    assert np.array_equal(vo, v0)

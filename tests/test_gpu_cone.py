@@ -5,7 +5,7 @@ Bars: the traced (voxel, length) sequences bit-exact; the forward
 projection bit-exact against the oracle's double sum over the same entries
 rounded to float; the back projection within 1e-5 relative (L2 and max) of
 the oracle's double transpose of the exact lengths.  At the full C4 size
-(512^3, 720 x 768 x 1024) sampled rays are checked bit-exactly, the back
+(512^3, 720 views of a 1024 x 768 panel) sampled rays are checked bit-exactly, the back
 projection of single rays against their exact lengths, and the pair
 through the adjoint identity (the float walk and the exact lengths differ
 by the walk's rounding and its dropped final pieces, below 1e-3).
@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 import medimage_amd as mi
-from medimage_amd import configs
+from medimage_amd import configs, phantom
 import oracle_py as O
 
 pytestmark = pytest.mark.gpu
@@ -103,8 +103,53 @@ def test_cone_rejects_bad_geometry(ctx):
         ctx.cone_backproject(g, np.zeros((10, 8, 8), np.float32))
 
 
+def _cone_case(g, seed, n_ell=6):
+    fov = g.nx * g.sample_mm
+    ell = phantom.random_ellipsoids(n_ell, fov, seed, axes_mm=(0.05 * fov, 0.35 * fov), mu_max=0.05)
+    vol = phantom.rasterize3d(ell, g.nz, g.nx, g.ny, g.sample_mm)
+    p = O.cone_fp(O.cone_from(g), vol)
+    return vol, phantom.noisy_sinogram(p, 5e4, seed).astype(np.float32)
+
+
+@pytest.mark.parametrize("prm", [(1e-3, 1e-4, 1e-2, 5, 3), (0.5, 0.02, 0.05, 3, 2)])
+def test_cone_at_parity(ctx, prm):
+    """Ordered-subset AT on the volume: FP32 device run against the FP64
+    oracle (1e-5 relative, u and v), the data fidelity of the last pass, and
+    a second run bitwise identical."""
+    alpha, beta, eps, S, iters = prm
+    g = SMALL[1]
+    _print(f"cone AT {g} alpha {alpha} beta {beta} eps {eps} subsets {S} passes {iters}")
+    vol, sino = _cone_case(g, 12)
+    p = mi.at_params(alpha, beta, eps, n_subsets=S, max_iters=iters)
+    u, v, rep = ctx.cone_at_reconstruct(g, sino, p)
+    shape = (g.nz, g.nx, g.ny)
+    ou, ov, n, hist = O.cone_os_at(O.cone_from(g), sino, np.zeros(shape), np.ones(shape), alpha, beta, eps, S,
+                                   iters)
+    assert rep.iterations == n == iters
+    _print(f"  u rel L2 {rel_l2(u, ou):.2e} max {rel_max(u, ou):.2e}; v rel L2 {rel_l2(v, ov):.2e} "
+           f"max {rel_max(v, ov):.2e}; 1 - min v {1 - ov.min():.3f}")
+    assert rel_l2(u, ou) < 1e-5 and rel_max(u, ou) < 1e-5
+    assert rel_l2(v, ov) < 1e-5 and rel_max(v, ov) < 1e-5
+    assert abs(rep.data_fidelity - hist[-1]) < 1e-5 * hist[-1]
+    assert rel_l2(u, vol) < rel_l2(np.zeros_like(u), vol)
+    u2, v2, _ = ctx.cone_at_reconstruct(g, sino, p)
+    assert np.array_equal(u2.view(np.uint32), u.view(np.uint32))
+    assert np.array_equal(v2.view(np.uint32), v.view(np.uint32))
+
+
+def test_cone_at_rejects(ctx):
+    g = SMALL[0]
+    sino = np.zeros((g.n_views, g.n_rows, g.n_cols), np.float32)
+    with pytest.raises(mi.MIError):
+        ctx.cone_at_reconstruct(g, sino, mi.at_params(n_subsets=5, max_iters=1, median_every=2))
+    with pytest.raises(mi.MIError):
+        ctx.cone_at_reconstruct(g, sino, mi.at_params(n_subsets=5, max_iters=1, precision=mi.MI_FP64))
+    with pytest.raises(mi.MIError):
+        ctx.cone_at_reconstruct(g, sino, mi.at_params(n_subsets=g.n_views + 1, max_iters=1))
+
+
 def test_cone_c4_full_size(ctx):
-    """512^3, 720 views, 768 x 1024 panel: sampled rays bit-exact, single-ray
+    """512^3, 720 views, 1024 x 768 panel: sampled rays bit-exact, single-ray
     back projections against their exact lengths, the adjoint identity."""
     g = configs.c4_geometry()
     cg = O.cone_from(g)
