@@ -316,24 +316,41 @@ __global__ __launch_bounds__(kBlock) void k_cone_count(ConeDev g, const float *c
     if ((threadIdx.x & 63) == 0 && n) atomicAdd(total, n);
 }
 
-// Slab interval [lo, hi] of parameter t (point s + t r) for the slab
-// [a, a + 1] of one axis; r == 0: everything or nothing.
-__device__ __forceinline__ void cone_slab(double s, double r, double a, double &lo, double &hi) {
-    if (r == 0.0) {
+// Per (view, column) tables of the back projector: 1 / (dx - sx), 1 / (dy - sy)
+// (double) and |(dx - sx, dy - sy)|^2 (float) of the column's rays; per view
+// the source point.
+__global__ void k_cone_vctab(ConeDev g, const float *cs, double2 *inv, float *l2, float4 *src) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)g.nviews * g.ncols) return;
+    const int v = (int)(i / g.ncols), col = (int)(i % g.ncols);
+    const float c = cs[2 * v], s = cs[2 * v + 1];
+    float sx, sy, sz, dx, dy;
+    cone_source(g, c, s, sx, sy, sz);
+    cone_det_xy(g, c, s, col, dx, dy);
+    const double rx = (double)dx - (double)sx, ry = (double)dy - (double)sy;
+    inv[i] = make_double2(1.0 / rx, 1.0 / ry);
+    l2[i] = (float)(rx * rx + ry * ry);
+    if (col == 0) src[v] = make_float4(sx, sy, sz, 0.0f);
+}
+
+// Slab interval of the unit slab [a, a + 1] along one axis for the ray
+// s + t r given ir = 1 / r (infinite when r = 0: all or nothing).
+__device__ __forceinline__ void cone_slab_inv(double s, double ir, double a, double &lo, double &hi) {
+    if (isinf(ir)) {
         const bool in = (s >= a) && (s < a + 1.0);
         lo = in ? lo : 1.0;
         hi = in ? hi : 0.0;
         return;
     }
-    const double ir = 1.0 / r;
     const double t0 = (a - s) * ir, t1 = (a + 1.0 - s) * ir;
     lo = fmax(lo, fmin(t0, t1));
     hi = fmin(hi, fmax(t0, t1));
 }
 
 __global__ __launch_bounds__(kBlock) void k_cone_bp(ConeDev g, const float *cs, const double *rowtab,
-                                                    const int *views, int nv, const float *__restrict__ sino,
-                                                    float *__restrict__ vol) {
+                                                    const double2 *__restrict__ vinv, const float *__restrict__ vl2,
+                                                    const float4 *__restrict__ vsrc, const int *views, int nv,
+                                                    const float *__restrict__ sino, float *__restrict__ vol) {
     // Workgroup: 16 x 16 voxel columns, wave 8 x 8 of them; kZc slices.
     const int tx = (g.nx + 15) / 16, ty = (g.ny + 15) / 16;
     const unsigned b = xcd_block();
@@ -345,21 +362,24 @@ __global__ __launch_bounds__(kBlock) void k_cone_bp(ConeDev g, const float *cs, 
     const int z0 = zc * kZc;
     if (x >= g.nx || y >= g.ny || z0 >= g.nz) return;
 
-    double acc[kZc];
+    // Per-lane sums of the kZc voxels in LDS (each lane owns its own slots,
+    // so no barrier is needed); a register sums one (column, slice) first.
+    __shared__ double acc[kZc][kBlock];
 #pragma unroll
-    for (int k = 0; k < kZc; ++k) acc[k] = 0.0;
+    for (int k = 0; k < kZc; ++k) acc[k][threadIdx.x] = 0.0;
 
     const float cxv = g.nx / 2.0f, cyv = g.ny / 2.0f, czv = g.nz / 2.0f;
     const float kcol = g.sdd / g.pixel;          // panel pixels per (voxel offset / voxel depth)
     const float sodv = g.sod / g.sample;         // source distance in voxels
     const float h5 = (float)g.halfdet - 0.5f;
     const float margin = 0.05f;                  // in pixels, covers the float rounding of the bounds
+    const int nk = min(kZc, g.nz - z0);
 
     for (int iv = 0; iv < nv; ++iv) {
         const int v = views[iv];
         const float c = cs[2 * v], s = cs[2 * v + 1];
-        float sx, sy, sz;
-        cone_source(g, c, s, sx, sy, sz);
+        const float4 sp = vsrc[v];
+        const double sx = sp.x, sy = sp.y, sz = sp.z;
         // Corners of the column in the view's frame (u across the panel,
         // depth from the source): the panel column of a point is
         // u * kcol / depth + h5, the row of height zr is h5 - zr * kcol / depth.
@@ -378,42 +398,44 @@ __global__ __launch_bounds__(kBlock) void k_cone_bp(ConeDev g, const float *cs, 
         const int c0 = max(0, (int)ceilf(cmin + h5 - margin));
         const int c1 = min(g.ncols - 1, (int)floorf(cmax + h5 + margin));
         for (int col = c0; col <= c1; ++col) {
-            float dx, dy;
-            cone_det_xy(g, c, s, col, dx, dy);
-            const double rx = (double)dx - (double)sx, ry = (double)dy - (double)sy;
+            const size_t vc = (size_t)v * g.ncols + col;
+            const double2 ir = vinv[vc];
             double lo = 0.0, hi = 1.0;
-            cone_slab(sx, rx, (double)x, lo, hi);
-            cone_slab(sy, ry, (double)y, lo, hi);
+            cone_slab_inv(sx, ir.x, (double)x, lo, hi);
+            cone_slab_inv(sy, ir.y, (double)y, lo, hi);
             if (!(hi > lo)) continue;
-            const double lxy2 = rx * rx + ry * ry;
+            // z-slab parameters relative to lo, so that the float
+            // differences below keep full relative precision.
+            const float span = (float)(hi - lo);
+            const float lxy2 = vl2[vc];
+            const double nlo = -lo;
             const float *scol = sino + ((size_t)v * g.nrows) * g.ncols + col;
-#pragma unroll
-            for (int k = 0; k < kZc; ++k) {
+#pragma unroll 1
+            for (int k = 0; k < nk; ++k) {
                 const int z = z0 + k;
-                if (z >= g.nz) break;
                 // Rows whose rays can meet slice z inside the column.
                 const float za = (float)z - czv, zb = za + 1.0f;
                 const float wlo = fminf(za * imin, za * imax) * kcol;
                 const float whi = fmaxf(zb * imin, zb * imax) * kcol;
                 const int r0 = max(0, (int)ceilf(h5 - whi - margin));
                 const int r1 = min(g.nrows - 1, (int)floorf(h5 - wlo + margin));
+                const double dz0 = (double)z - sz;
+                double part = 0.0;
                 for (int r = r0; r <= r1; ++r) {
                     const double w = rowtab[2 * r], iw = rowtab[2 * r + 1];
-                    const double t0 = ((double)z - (double)sz) * iw, t1 = ((double)z + 1.0 - (double)sz) * iw;
-                    const double a = fmax(lo, fmin(t0, t1)), e = fmin(hi, fmax(t0, t1));
+                    const float ta = (float)fma(dz0, iw, nlo), tb = (float)fma(dz0 + 1.0, iw, nlo);
+                    const float a = fmaxf(0.0f, fminf(ta, tb)), e = fminf(span, fmaxf(ta, tb));
                     if (!(e > a)) continue;
-                    const float L = sqrtf((float)(lxy2 + w * w));
-                    const float len = (float)(e - a) * L;
-                    acc[k] += (double)scol[(size_t)r * g.ncols] * (double)len;
+                    const float wf = (float)w;
+                    const float L = sqrtf(lxy2 + wf * wf);
+                    const float len = (e - a) * L;
+                    part += (double)scol[(size_t)r * g.ncols] * (double)len;
                 }
+                acc[k][threadIdx.x] += part;
             }
         }
     }
-#pragma unroll
-    for (int k = 0; k < kZc; ++k) {
-        const int z = z0 + k;
-        if (z < g.nz) vol[((size_t)z * g.nx + x) * g.ny + y] = (float)acc[k];
-    }
+    for (int k = 0; k < nk; ++k) vol[((size_t)(z0 + k) * g.nx + x) * g.ny + y] = (float)acc[k][threadIdx.x];
 }
 
 unsigned grid8(size_t blocks) { return (unsigned)((blocks + 7) / 8 * 8); }
@@ -467,6 +489,13 @@ ConeTables &cone_tables(mi_context *ctx, const mi_cone_geometry &g) {
     MI_HIP(hipMemcpyAsync(t->cs.p, cs.data(), cs.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
     MI_HIP(hipMemcpyAsync(t->rows.p, rows.data(), rows.size() * sizeof(double), hipMemcpyHostToDevice,
                           ctx->stream));
+    const size_t nvc = (size_t)g.n_views * g.n_cols;
+    t->vinv.alloc(nvc);
+    t->vl2.alloc(nvc);
+    t->vsrc.alloc(g.n_views);
+    hipLaunchKernelGGL(k_cone_vctab, dim3((unsigned)((nvc + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, t->d,
+                       t->cs.p, t->vinv.p, t->vl2.p, t->vsrc.p);
+    MI_HIP(hipGetLastError());
     MI_HIP(hipStreamSynchronize(ctx->stream));
     ctx->cone = t;
     return *t;
@@ -502,8 +531,8 @@ size_t cone_fp_views(hipStream_t st, const ConeTables &t, const int *views, int 
 
 void cone_bp_views(hipStream_t st, const ConeTables &t, const int *views, int nv, const float *sino, float *vol) {
     const size_t blocks = (size_t)((t.d.nx + 15) / 16) * ((t.d.ny + 15) / 16) * ((t.d.nz + kZc - 1) / kZc);
-    hipLaunchKernelGGL(k_cone_bp, dim3(grid8(blocks)), dim3(kBlock), 0, st, t.d, t.cs.p, t.rows.p, views, nv, sino,
-                       vol);
+    hipLaunchKernelGGL(k_cone_bp, dim3(grid8(blocks)), dim3(kBlock), 0, st, t.d, t.cs.p, t.rows.p, t.vinv.p, t.vl2.p,
+                       t.vsrc.p, views, nv, sino, vol);
     MI_HIP(hipGetLastError());
 }
 

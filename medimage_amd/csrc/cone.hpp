@@ -21,6 +21,9 @@ struct ConeTables {
     DevBuf<float> cs;      // [2 * nviews]: cos, sin interleaved
     DevBuf<double> rows;   // [2 * nrows]: w, 1 / w
     DevBuf<int> all;       // 0 .. nviews - 1 (the view list of a full projection)
+    DevBuf<double2> vinv;  // [nviews * ncols]: 1 / (dx - sx), 1 / (dy - sy) of the column's rays
+    DevBuf<float> vl2;     // [nviews * ncols]: (dx - sx)^2 + (dy - sy)^2
+    DevBuf<float4> vsrc;   // [nviews]: source point
     int64_t nnz = -1;      // traced entries of all rays (-1: not counted yet)
 };
 
