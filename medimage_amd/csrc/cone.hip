@@ -43,6 +43,7 @@ namespace {
 constexpr int kConeTile = 16;   // forward: 16 x 16 panel pixels per workgroup
 constexpr int kConeGroup = 8;   // forward: walk steps per load group
 constexpr int kZc = 8;          // back projection: slices per lane
+constexpr int kBpViewChunk = 36; // back projection: views per launch (their rows stay in the caches)
 
 __device__ __forceinline__ void cone_rot(float &x, float &y, float c, float s) {
     float d0 = 0.0f, d1 = 0.0f;
@@ -221,19 +222,26 @@ __global__ void k_cone_trace(ConeDev g, const float *cs, int view, int row, int 
 }
 
 // Panel pixel of this lane: workgroup = 16 x 16 pixels of one view of the
-// list (q-th entry), wave = 8 x 8 of them.  False for padding lanes.
+// list (q-th entry), wave = 8 x 8 of them.  Workgroups are ordered by band of
+// 16 panel rows first, then view, then column tile, so that the workgroups in
+// flight (and each XCD's contiguous share of them) trace rays through the same
+// few slices of the volume.  Returns false for the padding workgroups of the
+// grid (all lanes alike); `ok` is false for lanes outside the panel.
 __device__ __forceinline__ bool cone_fp_pixel(const ConeDev &g, int nv, unsigned &tile_id, int &q, int &row,
-                                              int &col) {
+                                              int &col, bool &ok) {
     const int tc = (g.ncols + kConeTile - 1) / kConeTile, tr = (g.nrows + kConeTile - 1) / kConeTile;
     const unsigned b = xcd_block();
-    const unsigned per_view = (unsigned)(tc * tr);
+    const unsigned per_band = (unsigned)(nv * tc);
     tile_id = b;
-    q = (int)(b / per_view);
-    const int tile = (int)(b % per_view);
+    const int band = (int)(b / per_band);
+    const unsigned rem = b % per_band;
+    q = (int)(rem / tc);
+    const int ct = (int)(rem % tc);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    row = (tile / tc) * kConeTile + (wv >> 1) * 8 + (lane >> 3);
-    col = (tile % tc) * kConeTile + (wv & 1) * 8 + (lane & 7);
-    return q < nv && row < g.nrows && col < g.ncols;
+    row = band * kConeTile + (wv >> 1) * 8 + (lane >> 3);
+    col = ct * kConeTile + (wv & 1) * 8 + (lane & 7);
+    ok = row < g.nrows && col < g.ncols;
+    return band < tr;
 }
 
 // Double sum over a workgroup of 256 lanes in a fixed pairing; lane 0 of
@@ -257,8 +265,8 @@ __global__ __launch_bounds__(kBlock) void k_cone_fp(ConeDev g, const float *cs, 
                                                     float *__restrict__ out, double *__restrict__ part) {
     unsigned tile_id;
     int q, row, col;
-    const bool ok = cone_fp_pixel(g, nv, tile_id, q, row, col);
-    if (q >= nv) return;   // whole padding workgroup
+    bool ok;
+    if (!cone_fp_pixel(g, nv, tile_id, q, row, col, ok)) return;   // padding workgroup
     double acc = 0.0;
     int view = 0;
     ConeWalk w;
@@ -298,8 +306,9 @@ __global__ __launch_bounds__(kBlock) void k_cone_count(ConeDev g, const float *c
                                                        unsigned long long *total) {
     unsigned tile_id;
     int view, row, col;
+    bool ok;
     unsigned long long n = 0;
-    if (cone_fp_pixel(g, g.nviews, tile_id, view, row, col)) {
+    if (cone_fp_pixel(g, g.nviews, tile_id, view, row, col, ok) && ok) {
         ConeWalk w;
         if (cone_ray_init(g, cs, view, row, col, w)) {
             bool live = true;
@@ -350,7 +359,8 @@ __device__ __forceinline__ void cone_slab_inv(double s, double ir, double a, dou
 __global__ __launch_bounds__(kBlock) void k_cone_bp(ConeDev g, const float *cs, const double *rowtab,
                                                     const double2 *__restrict__ vinv, const float *__restrict__ vl2,
                                                     const float4 *__restrict__ vsrc, const int *views, int nv,
-                                                    const float *__restrict__ sino, float *__restrict__ vol) {
+                                                    const float *__restrict__ sino, double *carry, int first,
+                                                    int last, float *__restrict__ vol) {
     // Workgroup: 16 x 16 voxel columns, wave 8 x 8 of them; kZc slices.
     const int tx = (g.nx + 15) / 16, ty = (g.ny + 15) / 16;
     const unsigned b = xcd_block();
@@ -365,15 +375,18 @@ __global__ __launch_bounds__(kBlock) void k_cone_bp(ConeDev g, const float *cs, 
     // Per-lane sums of the kZc voxels in LDS (each lane owns its own slots,
     // so no barrier is needed); a register sums one (column, slice) first.
     __shared__ double acc[kZc][kBlock];
+    const int nk = min(kZc, g.nz - z0);
+    // carry: the double sums of the earlier view chunks of a back projection
+    // split into several launches (nullptr for a single launch).
 #pragma unroll
-    for (int k = 0; k < kZc; ++k) acc[k][threadIdx.x] = 0.0;
+    for (int k = 0; k < kZc; ++k)
+        acc[k][threadIdx.x] = (carry && !first && k < nk) ? carry[((size_t)(z0 + k) * g.nx + x) * g.ny + y] : 0.0;
 
     const float cxv = g.nx / 2.0f, cyv = g.ny / 2.0f, czv = g.nz / 2.0f;
     const float kcol = g.sdd / g.pixel;          // panel pixels per (voxel offset / voxel depth)
     const float sodv = g.sod / g.sample;         // source distance in voxels
     const float h5 = (float)g.halfdet - 0.5f;
     const float margin = 0.05f;                  // in pixels, covers the float rounding of the bounds
-    const int nk = min(kZc, g.nz - z0);
 
     for (int iv = 0; iv < nv; ++iv) {
         const int v = views[iv];
@@ -435,7 +448,11 @@ __global__ __launch_bounds__(kBlock) void k_cone_bp(ConeDev g, const float *cs, 
             }
         }
     }
-    for (int k = 0; k < nk; ++k) vol[((size_t)(z0 + k) * g.nx + x) * g.ny + y] = (float)acc[k][threadIdx.x];
+    for (int k = 0; k < nk; ++k) {
+        const size_t i = ((size_t)(z0 + k) * g.nx + x) * g.ny + y;
+        if (carry && !last) carry[i] = acc[k][threadIdx.x];
+        else vol[i] = (float)acc[k][threadIdx.x];
+    }
 }
 
 unsigned grid8(size_t blocks) { return (unsigned)((blocks + 7) / 8 * 8); }
@@ -531,9 +548,17 @@ size_t cone_fp_views(hipStream_t st, const ConeTables &t, const int *views, int 
 
 void cone_bp_views(hipStream_t st, const ConeTables &t, const int *views, int nv, const float *sino, float *vol) {
     const size_t blocks = (size_t)((t.d.nx + 15) / 16) * ((t.d.ny + 15) / 16) * ((t.d.nz + kZc - 1) / kZc);
-    hipLaunchKernelGGL(k_cone_bp, dim3(grid8(blocks)), dim3(kBlock), 0, st, t.d, t.cs.p, t.rows.p, t.vinv.p, t.vl2.p,
-                       t.vsrc.p, views, nv, sino, vol);
-    MI_HIP(hipGetLastError());
+    // Views in chunks whose sinogram rows stay cached while every workgroup
+    // sweeps them; double sums carried between chunks.
+    const int chunks = (nv + kBpViewChunk - 1) / kBpViewChunk;
+    DevBuf<double> carry(chunks > 1 ? (size_t)t.d.nx * t.d.ny * t.d.nz : 0);
+    for (int c = 0; c < chunks; ++c) {
+        const int v0 = c * kBpViewChunk, n = std::min(kBpViewChunk, nv - v0);
+        hipLaunchKernelGGL(k_cone_bp, dim3(grid8(blocks)), dim3(kBlock), 0, st, t.d, t.cs.p, t.rows.p, t.vinv.p,
+                           t.vl2.p, t.vsrc.p, views + v0, n, sino, carry.p, c == 0, c == chunks - 1, vol);
+        MI_HIP(hipGetLastError());
+    }
+    if (chunks > 1) MI_HIP(hipStreamSynchronize(st));   // before the carry buffer is freed
 }
 
 void cone_forward(hipStream_t st, const ConeTables &t, const float *vol, float *sino) {
