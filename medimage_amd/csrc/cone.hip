@@ -25,11 +25,16 @@
 // (x, y) and a chunk of kZc slices along z.  For each view the four corners
 // of the column are projected onto the panel: the columns whose rays can
 // cross it, and per slice the rows.  For each candidate ray the exact length
-// inside the voxel is the slab clip of the ray's parameter interval in
-// double (x / y slabs once per column, z slab per row through a per-row
-// 1 / w table), times the ray length in float; sino * length is summed in
-// double per voxel in (view, column, row) order.  The x / y interval of a
-// candidate column is shared by all kZc slices of the lane.
+// inside the voxel is the slab clip of the ray's parameter interval: the x / y
+// slab in double once per (view, column) from a per-geometry table of
+// 1 / (d - s), shared by the lane's kZc slices; the z slab per row from a
+// per-row 1 / w table, relative to the column's entry parameter so that the
+// float differences keep full precision; times the ray length in float.
+// sino * length is summed in double, per (view, column, slice) over the rows
+// in a register and then into the voxel's sum in LDS, in a fixed order.  A
+// back projection over many views runs in chunks of kBpViewChunk views (their
+// sinogram rows stay in the caches while every workgroup sweeps them), the
+// double sums carried between chunks in a scratch volume.
 #include "cone.hpp"
 
 #include <cmath>
