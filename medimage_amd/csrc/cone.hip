@@ -47,6 +47,9 @@ namespace {
 
 constexpr int kConeTile = 16;   // forward: 16 x 16 panel pixels per workgroup
 constexpr int kConeGroup = 8;   // forward: walk steps per load group
+#ifndef MI_CONE_FP_XCD
+#define MI_CONE_FP_XCD 1        // forward: each XCD takes a contiguous range of row bands
+#endif
 constexpr int kZc = 8;          // back projection: slices per lane
 constexpr int kBpViewChunk = 36; // back projection: views per launch (their rows stay in the caches)
 
@@ -99,7 +102,8 @@ __device__ __forceinline__ float cone_lam(float i, float s, float d, float L) { 
 // crossings, st* their increments, (vx, vy, vz) the voxel, idx its offset.
 struct ConeWalk {
     float lam0, lx, ly, lz, stx, sty, stz, stop;
-    int vx, vy, vz, sgx, sgy, sgz, idx;
+    int vx, vy, vz, sgx, sgy, sgz;
+    unsigned idx;
 };
 
 // Entry part of forward_proj (tracing.cpp:142-249).  stop is the smallest
@@ -168,15 +172,16 @@ __device__ __forceinline__ bool cone_walk_init(const ConeDev &g, float sx, float
     float stop = (float)hi;
     if ((double)stop < hi) stop = nextafterf(stop, INFINITY);
     w = ConeWalk{lam0, lx, ly, lz, stx, sty, stz, stop, vx, vy, vz, sgx, sgy, sgz,
-                 (vz * g.nx + vx) * g.ny + vy};
+                 (unsigned)((vz * g.nx + vx) * g.ny + vy)};
     return true;
 }
 
 // One step when `live` (the walk has not ended): the current voxel's offset
 // and length; state unchanged and length 0 otherwise.  The branches of the
 // reference loop (tracing.cpp:268-305) become selects.
-__device__ __forceinline__ void cone_step(const ConeDev &g, ConeWalk &w, bool &live, int &idx, float &len) {
-    live = live && (w.lam0 < w.stop) && ((unsigned)w.vx < (unsigned)g.nx) && ((unsigned)w.vy < (unsigned)g.ny) &&
+__device__ __forceinline__ void cone_step(const ConeDev &g, ConeWalk &w, bool &live, unsigned &idx, float &len) {
+    // Bitwise, not short-circuit: no branch per step.
+    live = live & (w.lam0 < w.stop) & ((unsigned)w.vx < (unsigned)g.nx) & ((unsigned)w.vy < (unsigned)g.ny) &
            ((unsigned)w.vz < (unsigned)g.nz);
     const bool bx = (w.lx <= w.ly) && (w.lx <= w.lz);
     const bool by = !bx && (w.ly <= w.lz);
@@ -192,7 +197,7 @@ __device__ __forceinline__ void cone_step(const ConeDev &g, ConeWalk &w, bool &l
     w.vx += tx ? w.sgx : 0;
     w.vy += ty ? w.sgy : 0;
     w.vz += tz ? w.sgz : 0;
-    w.idx += tx ? w.sgx * g.ny : (ty ? w.sgy : (tz ? w.sgz * g.nx * g.ny : 0));
+    w.idx += (unsigned)(tx ? w.sgx * g.ny : (ty ? w.sgy : (tz ? w.sgz * g.nx * g.ny : 0)));
 }
 
 __device__ __forceinline__ bool cone_ray_init(const ConeDev &g, const float *cs, int view, int row, int col,
@@ -212,7 +217,7 @@ __global__ void k_cone_trace(ConeDev g, const float *cs, int view, int row, int 
     if (cone_ray_init(g, cs, view, row, col, w)) {
         bool live = true;
         for (;;) {
-            int idx;
+            unsigned idx;
             float l;
             cone_step(g, w, live, idx, l);
             if (!live) break;
@@ -235,7 +240,11 @@ __global__ void k_cone_trace(ConeDev g, const float *cs, int view, int row, int 
 __device__ __forceinline__ bool cone_fp_pixel(const ConeDev &g, int nv, unsigned &tile_id, int &q, int &row,
                                               int &col, bool &ok) {
     const int tc = (g.ncols + kConeTile - 1) / kConeTile, tr = (g.nrows + kConeTile - 1) / kConeTile;
+#if MI_CONE_FP_XCD
     const unsigned b = xcd_block();
+#else
+    const unsigned b = blockIdx.x;
+#endif
     const unsigned per_band = (unsigned)(nv * tc);
     tile_id = b;
     const int band = (int)(b / per_band);
@@ -278,7 +287,7 @@ __global__ __launch_bounds__(kBlock) void k_cone_fp(ConeDev g, const float *cs, 
     if (ok && (view = views[q], cone_ray_init(g, cs, view, row, col, w))) {
         bool live = true;
         while (live) {
-            int id[kConeGroup];
+            unsigned id[kConeGroup];
             float ln[kConeGroup];
             bool lv[kConeGroup];
 #pragma unroll
@@ -286,8 +295,11 @@ __global__ __launch_bounds__(kBlock) void k_cone_fp(ConeDev g, const float *cs, 
             float v[kConeGroup];
 #pragma unroll
             for (int k = 0; k < kConeGroup; ++k) v[k] = vol[id[k]];
+            // The product of two floats is exact in double, so the fused
+            // multiply-add rounds once, as acc + (double)v * (double)len does.
 #pragma unroll
-            for (int k = 0; k < kConeGroup; ++k) acc = lv[k] ? acc + (double)v[k] * (double)ln[k] : acc;
+            for (int k = 0; k < kConeGroup; ++k)
+                acc = __builtin_fma((double)(lv[k] ? v[k] : 0.0f), (double)ln[k], acc);
         }
     }
     const size_t at = ((size_t)view * g.nrows + row) * g.ncols + col;
@@ -318,7 +330,7 @@ __global__ __launch_bounds__(kBlock) void k_cone_count(ConeDev g, const float *c
         if (cone_ray_init(g, cs, view, row, col, w)) {
             bool live = true;
             for (;;) {
-                int idx;
+                unsigned idx;
                 float l;
                 cone_step(g, w, live, idx, l);
                 if (!live) break;
