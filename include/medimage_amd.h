@@ -366,6 +366,12 @@ int mi_cone_backproject_dev(mi_context *ctx, const mi_cone_geometry *g, const fl
  * initial values and receive the result. */
 int mi_cone_at_reconstruct(mi_context *ctx, const mi_cone_geometry *g, const mi_at_params *p, const float *sino,
                            float *u, float *v, mi_report *rep);
+/* Fills *g and *p from the text of an industry3d configuration file with
+ * BEAM "Cone" (Parameter::parse_config, industry3d/src/ct3d.cpp:20-93: NX,
+ * NY, NZ, NPROJ, NDX, NDY, SAMPLESIZE, PIXELSIZE, SOD, SDD, ALPHA, BETA,
+ * EPSILON, ITERATIONS; LOWER_X / UPPER_X optional).  A missing key, or
+ * another BEAM, is MI_E_ARG.  Host only (no context). */
+int mi_cone_config_parse(const char *json_text, mi_cone_geometry *g, mi_at_params *p);
 
 #ifdef __cplusplus
 }

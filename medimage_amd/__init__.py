@@ -198,6 +198,7 @@ def _load():
         "mi_cone_backproject": [P, CG, P, P],
         "mi_cone_backproject_dev": [P, CG, P, P],
         "mi_cone_at_reconstruct": [P, CG, C.POINTER(AtParams), P, P, P, C.POINTER(Report)],
+        "mi_cone_config_parse": [C.c_char_p, CG, C.POINTER(AtParams)],
     }
     for name, args in sigs.items():
         fn = getattr(lib, name)
@@ -218,6 +219,13 @@ def config_parse(text: str):
     (mi_config_parse: the helical slice driver's or industry3d's)."""
     g, p = Geometry(), AtParams()
     _check(_load().mi_config_parse(text.encode(), C.byref(g), C.byref(p)))
+    return g, p
+
+
+def cone_config_parse(text: str):
+    """(ConeGeometry, AtParams) from an industry3d configuration with BEAM "Cone"."""
+    g, p = ConeGeometry(), AtParams()
+    _check(_load().mi_cone_config_parse(text.encode(), C.byref(g), C.byref(p)))
     return g, p
 
 

@@ -1187,3 +1187,37 @@ int mi_cone_at_reconstruct(mi_context *ctx, const mi_cone_geometry *g, const mi_
         cone_at_solve(ctx, *g, *p, sino, u, v, rep);
     });
 }
+
+int mi_cone_config_parse(const char *json, mi_cone_geometry *g, mi_at_params *p) {
+    return guarded([&] {
+        MI_REQUIRE(json && g && p, MI_E_ARG, "null argument");
+        JsonReader r{json};
+        const JsonObject o = r.object();
+        auto b = o.str.find("BEAM");
+        MI_REQUIRE(b != o.str.end(), MI_E_ARG, "configuration JSON lacks BEAM");
+        MI_REQUIRE(b->second == "Cone" || b->second == "cone", MI_E_ARG,
+                   "the 3D path takes BEAM \"Cone\" (got \"" + b->second + "\")");
+        *g = mi_cone_geometry{};
+        g->nx = (int)need(o, "NX");
+        g->ny = (int)need(o, "NY");
+        g->nz = (int)need(o, "NZ");
+        g->n_views = (int)need(o, "NPROJ");
+        g->n_rows = (int)need(o, "NDX");
+        g->n_cols = (int)need(o, "NDY");
+        g->sample_mm = (float)need(o, "SAMPLESIZE");
+        g->pixel_mm = (float)need(o, "PIXELSIZE");
+        g->sod_mm = (float)need(o, "SOD");
+        g->sdd_mm = (float)need(o, "SDD");
+        *p = mi_at_params{};
+        p->alpha = need(o, "ALPHA");
+        p->beta = need(o, "BETA");
+        p->epsilon = need(o, "EPSILON");
+        p->max_iters = p->iter_alt = (int)need(o, "ITERATIONS");
+        p->iter_edge = 1;
+        p->n_subsets = 1;
+        p->lower = opt(o, "LOWER_X", -INFINITY);
+        p->upper = opt(o, "UPPER_X", INFINITY);
+        p->precision = MI_FP32;
+        validate_cone(*g);
+    });
+}

@@ -36,6 +36,22 @@ def test_industry3d_configs():
     assert p.alpha == 10.0 and p.max_iters == 10
 
 
+def test_industry3d_cone_config():
+    """The same industry3d file as a 3D cone geometry: every Parameter field
+    of the volume and panel maps one to one (NDX rows, NDY columns)."""
+    text = (GOLD / "industry3d_cone_taiji.json").read_text()
+    g, p = mi.cone_config_parse(text)
+    assert (g.nx, g.ny, g.nz, g.n_views, g.n_rows, g.n_cols) == (128, 128, 128, 180, 256, 256)
+    assert (g.sample_mm, g.pixel_mm, g.sod_mm) == (1.0, 1.0, 450.0)
+    assert abs(g.sdd_mm - 836.70001221) < 1e-4
+    assert (p.alpha, p.beta, p.epsilon, p.max_iters, p.lower, p.upper) == (2.0, 0.001, 0.125, 10, 0.0, 10.0)
+    with pytest.raises(mi.MIError):
+        mi.cone_config_parse((GOLD / "industry3d_parallel_head.json").read_text())
+    with pytest.raises(mi.MIError) as e:
+        mi.cone_config_parse(text.replace('"NZ"', '"NZ_X"'))
+    assert e.value.code == mi.MI_E_ARG and "NZ" in str(e.value)
+
+
 def test_config_errors():
     text = (GOLD / "industry3d_cone_taiji.json").read_text()
     with pytest.raises(mi.MIError) as e:
