@@ -334,7 +334,14 @@ typedef struct mi_cone_geometry {
     float   sample_mm;         /* SAMPLESIZE                                  */
     float   pixel_mm;          /* PIXELSIZE                                   */
     float   sod_mm, sdd_mm;    /* SOD, SDD                                    */
+    int32_t beam;              /* BEAM: MI_CONE_BEAM (0, "Cone") or
+                                  MI_CONE_PARALLEL (1, "Parallel": parallel()
+                                  of industry3d/src/tracing.cpp:9-49, rays
+                                  from the plane at -SOD to the panel, every
+                                  axis centred on HALFSIZE = NX / 2)          */
+    int32_t reserved;
 } mi_cone_geometry;
+enum { MI_CONE_BEAM = 0, MI_CONE_PARALLEL = 1 };
 
 /* Replaces get_tracing + forward_proj (industry3d/src/main.cpp:58-73,
  * industry3d/src/tracing.cpp:51-81,123-307): the (voxel index, length)
@@ -366,11 +373,11 @@ int mi_cone_backproject_dev(mi_context *ctx, const mi_cone_geometry *g, const fl
  * initial values and receive the result. */
 int mi_cone_at_reconstruct(mi_context *ctx, const mi_cone_geometry *g, const mi_at_params *p, const float *sino,
                            float *u, float *v, mi_report *rep);
-/* Fills *g and *p from the text of an industry3d configuration file with
- * BEAM "Cone" (Parameter::parse_config, industry3d/src/ct3d.cpp:20-93: NX,
- * NY, NZ, NPROJ, NDX, NDY, SAMPLESIZE, PIXELSIZE, SOD, SDD, ALPHA, BETA,
- * EPSILON, ITERATIONS; LOWER_X / UPPER_X optional).  A missing key, or
- * another BEAM, is MI_E_ARG.  Host only (no context). */
+/* Fills *g and *p from the text of an industry3d configuration file
+ * (Parameter::parse_config, industry3d/src/ct3d.cpp:20-93: BEAM "Cone" or
+ * "Parallel", NX, NY, NZ, NPROJ, NDX, NDY, SAMPLESIZE, PIXELSIZE, SOD, SDD,
+ * ALPHA, BETA, EPSILON, ITERATIONS; LOWER_X / UPPER_X optional).  A missing
+ * key, or another BEAM, is MI_E_ARG.  Host only (no context). */
 int mi_cone_config_parse(const char *json_text, mi_cone_geometry *g, mi_at_params *p);
 
 #ifdef __cplusplus

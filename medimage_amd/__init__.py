@@ -55,15 +55,18 @@ class ConeGeometry(C.Structure):
     _fields_ = [
         ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("n_views", C.c_int32),
         ("n_rows", C.c_int32), ("n_cols", C.c_int32), ("sample_mm", C.c_float), ("pixel_mm", C.c_float),
-        ("sod_mm", C.c_float), ("sdd_mm", C.c_float),
+        ("sod_mm", C.c_float), ("sdd_mm", C.c_float), ("beam", C.c_int32), ("reserved", C.c_int32),
     ]
 
     def __repr__(self):
-        return "ConeGeometry(" + ", ".join(f"{f}={getattr(self, f)}" for f, _ in self._fields_) + ")"
+        return "ConeGeometry(" + ", ".join(f"{f}={getattr(self, f)}" for f, _ in self._fields_[:-1]) + ")"
 
 
-def cone_geometry(nx, ny, nz, views, rows, cols, sample_mm, pixel_mm, sod, sdd):
-    return ConeGeometry(nx, ny, nz, views, rows, cols, sample_mm, pixel_mm, sod, sdd)
+MI_CONE_BEAM, MI_CONE_PARALLEL = 0, 1
+
+
+def cone_geometry(nx, ny, nz, views, rows, cols, sample_mm, pixel_mm, sod, sdd, beam=MI_CONE_BEAM):
+    return ConeGeometry(nx, ny, nz, views, rows, cols, sample_mm, pixel_mm, sod, sdd, beam, 0)
 
 
 class AtParams(C.Structure):

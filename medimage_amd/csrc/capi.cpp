@@ -1195,9 +1195,11 @@ int mi_cone_config_parse(const char *json, mi_cone_geometry *g, mi_at_params *p)
         const JsonObject o = r.object();
         auto b = o.str.find("BEAM");
         MI_REQUIRE(b != o.str.end(), MI_E_ARG, "configuration JSON lacks BEAM");
-        MI_REQUIRE(b->second == "Cone" || b->second == "cone", MI_E_ARG,
-                   "the 3D path takes BEAM \"Cone\" (got \"" + b->second + "\")");
+        const bool cone = b->second == "Cone" || b->second == "cone";
+        const bool par = b->second == "Parallel" || b->second == "parallel";
+        MI_REQUIRE(cone || par, MI_E_ARG, "unknown BEAM \"" + b->second + "\"");
         *g = mi_cone_geometry{};
+        g->beam = cone ? MI_CONE_BEAM : MI_CONE_PARALLEL;
         g->nx = (int)need(o, "NX");
         g->ny = (int)need(o, "NY");
         g->nz = (int)need(o, "NZ");

@@ -95,6 +95,39 @@ __device__ __host__ __forceinline__ float cone_det_z(const ConeDev &g, int row) 
     return (float)((double)Z + g.nz / 2.0);
 }
 
+// parallel() of industry3d/src/tracing.cpp:9-49: start and end share the
+// panel column's x and the row's z; every axis is shifted by HALFSIZE (a
+// float), in float.
+__device__ __forceinline__ void par_endpoints(const ConeDev &g, float c, float s, int row, int col, float &sx,
+                                              float &sy, float &sz, float &dx, float &dy, float &dz) {
+    float X0 = (float)((double)col + 0.5 - g.halfdet);
+    X0 *= g.pixel;
+    float Y0 = -g.sod;
+    float Z0 = (float)((double)(g.halfdet - row) - 0.5);
+    Z0 *= g.pixel;
+    float X1 = X0, Y1 = g.sdd - g.sod, Z1 = Z0;
+    cone_rot(X0, Y0, c, s);
+    cone_rot(X1, Y1, c, s);
+    sx = X0 / g.sample + g.halfsize;
+    sy = Y0 / g.sample + g.halfsize;
+    sz = Z0 / g.sample + g.halfsize;
+    dx = X1 / g.sample + g.halfsize;
+    dy = Y1 / g.sample + g.halfsize;
+    dz = Z1 / g.sample + g.halfsize;
+}
+
+// End points of ray (view with cos c / sin s, row, col) of either beam.
+__device__ __forceinline__ void ray_points(const ConeDev &g, float c, float s, int row, int col, float &sx, float &sy,
+                                           float &sz, float &dx, float &dy, float &dz) {
+    if (g.parallel) {
+        par_endpoints(g, c, s, row, col, sx, sy, sz, dx, dy, dz);
+    } else {
+        cone_source(g, c, s, sx, sy, sz);
+        cone_det_xy(g, c, s, col, dx, dy);
+        dz = cone_det_z(g, row);
+    }
+}
+
 // LAMBDA_X of industry3d/src/tracing.cpp:83-85.
 __device__ __forceinline__ float cone_lam(float i, float s, float d, float L) { return L * (i - s) / (d - s); }
 
@@ -203,10 +236,9 @@ __device__ __forceinline__ void cone_step(const ConeDev &g, ConeWalk &w, bool &l
 __device__ __forceinline__ bool cone_ray_init(const ConeDev &g, const float *cs, int view, int row, int col,
                                               ConeWalk &w) {
     const float c = cs[2 * view], s = cs[2 * view + 1];
-    float sx, sy, sz, dx, dy;
-    cone_source(g, c, s, sx, sy, sz);
-    cone_det_xy(g, c, s, col, dx, dy);
-    return cone_walk_init(g, sx, sy, sz, dx, dy, cone_det_z(g, row), w);
+    float sx, sy, sz, dx, dy, dz;
+    ray_points(g, c, s, row, col, sx, sy, sz, dx, dy, dz);
+    return cone_walk_init(g, sx, sy, sz, dx, dy, dz, w);
 }
 
 __global__ void k_cone_trace(ConeDev g, const float *cs, int view, int row, int col, int64_t *ind, float *len,
@@ -345,17 +377,17 @@ __global__ __launch_bounds__(kBlock) void k_cone_count(ConeDev g, const float *c
 // Per (view, column) tables of the back projector: 1 / (dx - sx), 1 / (dy - sy)
 // (double) and |(dx - sx, dy - sy)|^2 (float) of the column's rays; per view
 // the source point.
-__global__ void k_cone_vctab(ConeDev g, const float *cs, double2 *inv, float *l2, float4 *src) {
+__global__ void k_cone_vctab(ConeDev g, const float *cs, double2 *inv, float *l2, float4 *src, float2 *pos) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)g.nviews * g.ncols) return;
     const int v = (int)(i / g.ncols), col = (int)(i % g.ncols);
     const float c = cs[2 * v], s = cs[2 * v + 1];
-    float sx, sy, sz, dx, dy;
-    cone_source(g, c, s, sx, sy, sz);
-    cone_det_xy(g, c, s, col, dx, dy);
+    float sx, sy, sz, dx, dy, dz;
+    ray_points(g, c, s, 0, col, sx, sy, sz, dx, dy, dz);   // x and y do not depend on the row
     const double rx = (double)dx - (double)sx, ry = (double)dy - (double)sy;
     inv[i] = make_double2(1.0 / rx, 1.0 / ry);
     l2[i] = (float)(rx * rx + ry * ry);
+    pos[i] = make_float2(sx, sy);
     if (col == 0) src[v] = make_float4(sx, sy, sz, 0.0f);
 }
 
@@ -373,11 +405,14 @@ __device__ __forceinline__ void cone_slab_inv(double s, double ir, double a, dou
     hi = fmin(hi, fmax(t0, t1));
 }
 
+// PAR: the parallel beam (rays at constant z, orthographic footprint; the
+// rows' z in rowtab[2 r], the columns' start points in vpos).
+template <bool PAR>
 __global__ __launch_bounds__(kBlock) void k_cone_bp(ConeDev g, const float *cs, const double *rowtab,
                                                     const double2 *__restrict__ vinv, const float *__restrict__ vl2,
-                                                    const float4 *__restrict__ vsrc, const int *views, int nv,
-                                                    const float *__restrict__ sino, double *carry, int first,
-                                                    int last, float *__restrict__ vol) {
+                                                    const float4 *__restrict__ vsrc, const float2 *__restrict__ vpos,
+                                                    const int *views, int nv, const float *__restrict__ sino,
+                                                    double *carry, int first, int last, float *__restrict__ vol) {
     // Workgroup: 16 x 16 voxel columns, wave 8 x 8 of them; kZc slices.
     const int tx = (g.nx + 15) / 16, ty = (g.ny + 15) / 16;
     const unsigned b = xcd_block();
@@ -399,7 +434,9 @@ __global__ __launch_bounds__(kBlock) void k_cone_bp(ConeDev g, const float *cs, 
     for (int k = 0; k < kZc; ++k)
         acc[k][threadIdx.x] = (carry && !first && k < nk) ? carry[((size_t)(z0 + k) * g.nx + x) * g.ny + y] : 0.0;
 
-    const float cxv = g.nx / 2.0f, cyv = g.ny / 2.0f, czv = g.nz / 2.0f;
+    const float cxv = PAR ? g.halfsize : g.nx / 2.0f, cyv = PAR ? g.halfsize : g.ny / 2.0f;
+    const float czv = PAR ? g.halfsize : g.nz / 2.0f;
+    const float kpar = g.sample / g.pixel;       // parallel: panel pixels per voxel
     const float kcol = g.sdd / g.pixel;          // panel pixels per (voxel offset / voxel depth)
     const float sodv = g.sod / g.sample;         // source distance in voxels
     const float h5 = (float)g.halfdet - 0.5f;
@@ -409,7 +446,8 @@ __global__ __launch_bounds__(kBlock) void k_cone_bp(ConeDev g, const float *cs, 
         const int v = views[iv];
         const float c = cs[2 * v], s = cs[2 * v + 1];
         const float4 sp = vsrc[v];
-        const double sx = sp.x, sy = sp.y, sz = sp.z;
+        double sx = sp.x, sy = sp.y;
+        const double sz = sp.z;
         // Corners of the column in the view's frame (u across the panel,
         // depth from the source): the panel column of a point is
         // u * kcol / depth + h5, the row of height zr is h5 - zr * kcol / depth.
@@ -418,8 +456,8 @@ __global__ __launch_bounds__(kBlock) void k_cone_bp(ConeDev g, const float *cs, 
         for (int q = 0; q < 4; ++q) {
             const float px = (float)(x + (q & 1)) - cxv, py = (float)(y + (q >> 1)) - cyv;
             const float u = c * px + s * py;
-            const float id = 1.0f / (-s * px + c * py + sodv);
-            const float cf = u * kcol * id;
+            const float id = PAR ? 1.0f : 1.0f / (-s * px + c * py + sodv);
+            const float cf = PAR ? u * kpar : u * kcol * id;
             cmin = fminf(cmin, cf);
             cmax = fmaxf(cmax, cf);
             imin = fminf(imin, id);
@@ -430,6 +468,11 @@ __global__ __launch_bounds__(kBlock) void k_cone_bp(ConeDev g, const float *cs, 
         for (int col = c0; col <= c1; ++col) {
             const size_t vc = (size_t)v * g.ncols + col;
             const double2 ir = vinv[vc];
+            if (PAR) {
+                const float2 p0 = vpos[vc];
+                sx = p0.x;
+                sy = p0.y;
+            }
             double lo = 0.0, hi = 1.0;
             cone_slab_inv(sx, ir.x, (double)x, lo, hi);
             cone_slab_inv(sy, ir.y, (double)y, lo, hi);
@@ -440,6 +483,26 @@ __global__ __launch_bounds__(kBlock) void k_cone_bp(ConeDev g, const float *cs, 
             const float lxy2 = vl2[vc];
             const double nlo = -lo;
             const float *scol = sino + ((size_t)v * g.nrows) * g.ncols + col;
+            if (PAR) {
+                // Rays at constant height: row r crosses slice z when
+                // z <= z_r < z + 1, over the whole x / y chord.
+                const float len = span * sqrtf(lxy2);
+#pragma unroll 1
+                for (int k = 0; k < nk; ++k) {
+                    const int z = z0 + k;
+                    const float za = (float)z - czv;
+                    const int r0 = max(0, (int)ceilf(h5 - (za + 1.0f) * kpar - margin));
+                    const int r1 = min(g.nrows - 1, (int)floorf(h5 - za * kpar + margin));
+                    double part = 0.0;
+                    for (int r = r0; r <= r1; ++r) {
+                        const double zr = rowtab[2 * r];
+                        if (zr >= (double)z && zr < (double)z + 1.0)
+                            part += (double)scol[(size_t)r * g.ncols] * (double)len;
+                    }
+                    acc[k][threadIdx.x] += part;
+                }
+                continue;
+            }
 #pragma unroll 1
             for (int k = 0; k < nk; ++k) {
                 const int z = z0 + k;
@@ -484,10 +547,11 @@ void validate_cone(const mi_cone_geometry &g) {
     MI_REQUIRE(std::isfinite(g.sample_mm) && std::isfinite(g.pixel_mm) && g.sample_mm > 0 && g.pixel_mm > 0,
                MI_E_ARG, "sample_mm and pixel_mm must be positive");
     MI_REQUIRE(std::isfinite(g.sdd_mm) && g.sod_mm > 0 && g.sdd_mm > g.sod_mm, MI_E_ARG, "need sdd > sod > 0");
+    MI_REQUIRE(g.beam == MI_CONE_BEAM || g.beam == MI_CONE_PARALLEL, MI_E_ARG, "unknown cone-geometry beam");
     // The source circle must lie outside the volume (the back projector's
     // corner projection assumes every voxel is in front of the source).
     const double half = 0.5 * g.sample_mm * std::sqrt((double)g.nx * g.nx + (double)g.ny * g.ny);
-    MI_REQUIRE(g.sod_mm > half, MI_E_ARG, "the source orbit must lie outside the volume");
+    MI_REQUIRE(g.beam == MI_CONE_PARALLEL || g.sod_mm > half, MI_E_ARG, "the source orbit must lie outside the volume");
 }
 
 ConeTables &cone_tables(mi_context *ctx, const mi_cone_geometry &g) {
@@ -496,7 +560,8 @@ ConeTables &cone_tables(mi_context *ctx, const mi_cone_geometry &g) {
     auto t = std::make_shared<ConeTables>();
     t->g = g;
     t->d = ConeDev{g.nx, g.ny, g.nz, g.n_views, g.n_rows, g.n_cols, g.n_rows / 2,
-                   g.sample_mm, g.pixel_mm, g.sod_mm, g.sdd_mm};
+                   g.sample_mm, g.pixel_mm, g.sod_mm, g.sdd_mm, g.beam == MI_CONE_PARALLEL ? 1 : 0,
+                   (float)(g.nx / 2.0)};
     std::vector<float> cs(2 * (size_t)g.n_views);
     for (int a = 0; a < g.n_views; ++a) {
         // theta = (float)alpha / NPROJ * 2 * PI, industry3d/src/tracing.cpp:58.
@@ -509,6 +574,14 @@ ConeTables &cone_tables(mi_context *ctx, const mi_cone_geometry &g) {
     const float sz = (float)((double)(0.0f / g.sample_mm) + g.nz / 2.0);
     std::vector<double> rows(2 * (size_t)g.n_rows);
     for (int r = 0; r < g.n_rows; ++r) {
+        if (t->d.parallel) {
+            // the row's constant z: (HALFDET - r - 0.5) PIXELSIZE / SAMPLESIZE + HALFSIZE in float
+            float Z = (float)((double)(t->d.halfdet - r) - 0.5);
+            Z *= g.pixel_mm;
+            rows[2 * r] = (double)(Z / g.sample_mm + t->d.halfsize);
+            rows[2 * r + 1] = 0.0;
+            continue;
+        }
         const double w = (double)cone_det_z(t->d, r) - (double)sz;
         MI_REQUIRE(w != 0.0, MI_E_ARG, "a detector row lies in the plane of the source orbit");
         rows[2 * r] = w;
@@ -527,8 +600,9 @@ ConeTables &cone_tables(mi_context *ctx, const mi_cone_geometry &g) {
     t->vinv.alloc(nvc);
     t->vl2.alloc(nvc);
     t->vsrc.alloc(g.n_views);
+    t->vpos.alloc(nvc);
     hipLaunchKernelGGL(k_cone_vctab, dim3((unsigned)((nvc + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, t->d,
-                       t->cs.p, t->vinv.p, t->vl2.p, t->vsrc.p);
+                       t->cs.p, t->vinv.p, t->vl2.p, t->vsrc.p, t->vpos.p);
     MI_HIP(hipGetLastError());
     MI_HIP(hipStreamSynchronize(ctx->stream));
     ctx->cone = t;
@@ -571,8 +645,14 @@ void cone_bp_views(hipStream_t st, const ConeTables &t, const int *views, int nv
     DevBuf<double> carry(chunks > 1 ? (size_t)t.d.nx * t.d.ny * t.d.nz : 0);
     for (int c = 0; c < chunks; ++c) {
         const int v0 = c * kBpViewChunk, n = std::min(kBpViewChunk, nv - v0);
-        hipLaunchKernelGGL(k_cone_bp, dim3(grid8(blocks)), dim3(kBlock), 0, st, t.d, t.cs.p, t.rows.p, t.vinv.p,
-                           t.vl2.p, t.vsrc.p, views + v0, n, sino, carry.p, c == 0, c == chunks - 1, vol);
+        if (t.d.parallel)
+            hipLaunchKernelGGL(k_cone_bp<true>, dim3(grid8(blocks)), dim3(kBlock), 0, st, t.d, t.cs.p, t.rows.p,
+                               t.vinv.p, t.vl2.p, t.vsrc.p, t.vpos.p, views + v0, n, sino, carry.p, c == 0,
+                               c == chunks - 1, vol);
+        else
+            hipLaunchKernelGGL(k_cone_bp<false>, dim3(grid8(blocks)), dim3(kBlock), 0, st, t.d, t.cs.p, t.rows.p,
+                               t.vinv.p, t.vl2.p, t.vsrc.p, t.vpos.p, views + v0, n, sino, carry.p, c == 0,
+                               c == chunks - 1, vol);
         MI_HIP(hipGetLastError());
     }
     if (chunks > 1) MI_HIP(hipStreamSynchronize(st));   // before the carry buffer is freed

@@ -10,11 +10,14 @@ namespace mi {
 struct ConeDev {
     int nx, ny, nz, nviews, nrows, ncols, halfdet;
     float sample, pixel, sod, sdd;
+    int parallel;      // BEAM "Parallel"
+    float halfsize;    // HALFSIZE = NX / 2.0 (float), the parallel beam's centre on every axis
 };
 
 // Per-geometry device tables: cosf / sinf of each view's angle (computed on
 // the host with the C library, as the reference computes them) and, per
-// detector row, w = dstZ - srcZ of the row's rays and 1 / w (double).
+// detector row, w = dstZ - srcZ of the row's rays and 1 / w (double; for the
+// parallel beam the rows' constant z instead).
 struct ConeTables {
     mi_cone_geometry g{};
     ConeDev d{};
@@ -23,7 +26,8 @@ struct ConeTables {
     DevBuf<int> all;       // 0 .. nviews - 1 (the view list of a full projection)
     DevBuf<double2> vinv;  // [nviews * ncols]: 1 / (dx - sx), 1 / (dy - sy) of the column's rays
     DevBuf<float> vl2;     // [nviews * ncols]: (dx - sx)^2 + (dy - sy)^2
-    DevBuf<float4> vsrc;   // [nviews]: source point
+    DevBuf<float4> vsrc;   // [nviews]: source point (cone beam)
+    DevBuf<float2> vpos;   // [nviews * ncols]: (x, y) of the column's rays' start (parallel beam)
     int64_t nnz = -1;      // traced entries of all rays (-1: not counted yet)
 };
 

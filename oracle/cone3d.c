@@ -63,7 +63,35 @@ static void rot2(float *x, float *y, float c, float s) {
     *y = d1;
 }
 
+/* parallel(), industry3d/src/tracing.cpp:9-49: source and detector points
+ * share their x (panel column) and z (panel row); every coordinate is
+ * shifted by HALFSIZE = NX / 2.0 held as a float (ct3d.cpp:88-93), in float. */
+static void par_ray(const or_cone_geom *g, float c, float s, int row, int col, float *src, float *dst) {
+    const int halfdet = g->ndx / 2;
+    const float half = (float)(g->nx / 2.0);
+    float sX = (float)((double)col + 0.5 - halfdet);
+    sX *= g->pixel;
+    float sY = -g->sod;
+    float sZ = (float)((double)(halfdet - row) - 0.5);
+    sZ *= g->pixel;
+    float dX = (float)((double)col + 0.5 - halfdet);
+    dX *= g->pixel;
+    float dY = g->sdd - g->sod;
+    float dZ = (float)((double)(halfdet - row) - 0.5);
+    dZ *= g->pixel;
+    rot2(&sX, &sY, c, s);
+    rot2(&dX, &dY, c, s);
+    sX /= g->sample; sY /= g->sample; sZ /= g->sample;
+    dX /= g->sample; dY /= g->sample; dZ /= g->sample;
+    src[0] = sX + half; src[1] = sY + half; src[2] = sZ + half;
+    dst[0] = dX + half; dst[1] = dY + half; dst[2] = dZ + half;
+}
+
 void or_cone_ray(const or_cone_geom *g, float c, float s, int row, int col, float *src, float *dst) {
+    if (g->parallel) {
+        par_ray(g, c, s, row, col, src, dst);
+        return;
+    }
     const int halfdet = g->ndx / 2;
     float sX = 0.0f, sY = -g->sod, sZ = 0.0f;
     rot2(&sX, &sY, c, s);

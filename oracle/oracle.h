@@ -211,10 +211,12 @@ void or_msbeam_solve(const float *g, float *f, float *v, int nproj, int nray, in
 typedef struct or_cone_geom {
     int nx, ny, nz, nproj, ndx, ndy;
     float sample, pixel, sod, sdd;      /* SAMPLESIZE, PIXELSIZE, SOD, SDD */
+    int parallel;                       /* BEAM "Parallel" (parallel(), tracing.cpp:9-49) */
 } or_cone_geom;
 /* cosf / sinf of each view's theta, industry3d/src/tracing.cpp:58. */
 void or_cone_tables(const or_cone_geom *g, float *cos_t, float *sin_t);
-/* cone(), industry3d/src/tracing.cpp:51-81: float source and detector point. */
+/* cone(), industry3d/src/tracing.cpp:51-81 (or parallel(), :9-49, when
+ * g->parallel): float source and detector point. */
 void or_cone_ray(const or_cone_geom *g, float c, float s, int row, int col, float *src, float *dst);
 /* forward_proj, industry3d/src/tracing.cpp:123-307, float; returns numb
  * (entries beyond cap are counted, not stored). */

@@ -300,12 +300,13 @@ def msbeam_solve(g, n, alpha, beta, eps, iters, flavour=0):
 class ConeGeom(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("nproj", C.c_int), ("ndx", C.c_int),
                 ("ndy", C.c_int), ("sample", C.c_float), ("pixel", C.c_float), ("sod", C.c_float),
-                ("sdd", C.c_float)]
+                ("sdd", C.c_float), ("parallel", C.c_int)]
 
 
 def cone_from(g):
     """Oracle geometry of a medimage_amd.ConeGeometry."""
-    return ConeGeom(g.nx, g.ny, g.nz, g.n_views, g.n_rows, g.n_cols, g.sample_mm, g.pixel_mm, g.sod_mm, g.sdd_mm)
+    return ConeGeom(g.nx, g.ny, g.nz, g.n_views, g.n_rows, g.n_cols, g.sample_mm, g.pixel_mm, g.sod_mm, g.sdd_mm,
+                    int(g.beam))
 
 
 def _cone_lib():

@@ -53,6 +53,49 @@ def test_cone_ray_closed_form():
             assert abs(np.dot(D - S, u) - g.sdd / g.sample) < 1e-3
 
 
+def test_parallel_ray_closed_form():
+    """parallel(): start on the plane at -SOD, end on the panel at SDD - SOD,
+    both at the column's offset and the row's height, every axis centred on
+    NX / 2."""
+    g = geom(nx=12, ny=10, nz=8, sample=0.5, pixel=1.5)
+    g.parallel = 1
+    half = g.nx / 2
+    for v in (0, 7, 19):
+        th = 2 * math.pi * v / g.nproj
+        for r, k in ((0, 0), (4, 9)):
+            S, D = O.cone_ray(g, v, r, k)
+            H = g.ndx // 2
+            px, pz = (k + 0.5 - H) * g.pixel, (H - r - 0.5) * g.pixel
+            for y0, P in ((-g.sod, S), (g.sdd - g.sod, D)):
+                want = np.array([math.cos(th) * px - math.sin(th) * y0, math.sin(th) * px + math.cos(th) * y0,
+                                 pz]) / g.sample + half
+                assert np.allclose(P, want, atol=1e-4)
+            assert S[2] == D[2]
+
+
+def test_parallel_walk_matches_exact_enumeration():
+    g = geom(nx=10, ny=10, nz=6, ndx=8, ndy=12, pixel=1.0, sod=30.0, sdd=50.0)
+    g.parallel = 1
+    rng = np.random.default_rng(9)
+    checked = 0
+    for _ in range(300):
+        v, r, k = int(rng.integers(g.nproj)), int(rng.integers(g.ndx)), int(rng.integers(g.ndy))
+        S, D = O.cone_ray(g, v, r, k)
+        N = (g.nx, g.ny, g.nz)
+        if any(S[a] == D[a] and not (0 <= S[a] < N[a]) for a in range(3)):
+            assert len(O.cone_walk(g, S, D)[0]) == 0     # parallel to a face, outside the volume
+            continue
+        ind, w = O.cone_walk(g, S, D)
+        seq, lens, chord, crossings = _exact_walk(g, S, D)
+        if not seq or min(float(x) for x in lens) < 1e-3:
+            continue
+        n = len(ind)
+        assert n in (len(seq), len(seq) - 1) and list(ind) == seq[:n]
+        assert np.allclose(w, [float(x) for x in lens[:n]], atol=2e-4)
+        checked += 1
+    assert checked > 100
+
+
 def test_walk_axis_aligned():
     g = geom()
     # along +y through voxel column (x=3, z=5): ny unit lengths

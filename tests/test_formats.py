@@ -45,8 +45,11 @@ def test_industry3d_cone_config():
     assert (g.sample_mm, g.pixel_mm, g.sod_mm) == (1.0, 1.0, 450.0)
     assert abs(g.sdd_mm - 836.70001221) < 1e-4
     assert (p.alpha, p.beta, p.epsilon, p.max_iters, p.lower, p.upper) == (2.0, 0.001, 0.125, 10, 0.0, 10.0)
+    assert g.beam == mi.MI_CONE_BEAM
+    gp, pp = mi.cone_config_parse((GOLD / "industry3d_parallel_head.json").read_text())
+    assert gp.beam == mi.MI_CONE_PARALLEL and (gp.nx, gp.n_views, gp.n_rows) == (256, 360, 256)
     with pytest.raises(mi.MIError):
-        mi.cone_config_parse((GOLD / "industry3d_parallel_head.json").read_text())
+        mi.cone_config_parse(text.replace('"Cone"', '"Helix"'))
     with pytest.raises(mi.MIError) as e:
         mi.cone_config_parse(text.replace('"NZ"', '"NZ_X"'))
     assert e.value.code == mi.MI_E_ARG and "NZ" in str(e.value)

@@ -35,6 +35,7 @@ def rel_max(a, b):
 SMALL = [
     mi.cone_geometry(20, 24, 16, 30, 20, 28, 1.0, 2.0, 60.0, 110.0),
     mi.cone_geometry(32, 32, 24, 45, 24, 40, 0.5, 0.8, 40.0, 70.0),
+    mi.cone_geometry(24, 20, 16, 36, 24, 32, 1.0, 0.9, 30.0, 60.0, beam=mi.MI_CONE_PARALLEL),
 ]
 
 
@@ -111,13 +112,14 @@ def _cone_case(g, seed, n_ell=6):
     return vol, phantom.noisy_sinogram(p, 5e4, seed).astype(np.float32)
 
 
-@pytest.mark.parametrize("prm", [(1e-3, 1e-4, 1e-2, 5, 3), (0.5, 0.02, 0.05, 3, 2)])
+@pytest.mark.parametrize("prm", [(1e-3, 1e-4, 1e-2, 5, 3, 1), (0.5, 0.02, 0.05, 3, 2, 1),
+                                 (0.5, 0.02, 0.05, 4, 2, 2)])
 def test_cone_at_parity(ctx, prm):
     """Ordered-subset AT on the volume: FP32 device run against the FP64
     oracle (1e-5 relative, u and v), the data fidelity of the last pass, and
     a second run bitwise identical."""
-    alpha, beta, eps, S, iters = prm
-    g = SMALL[1]
+    alpha, beta, eps, S, iters, gi = prm
+    g = SMALL[gi]
     _print(f"cone AT {g} alpha {alpha} beta {beta} eps {eps} subsets {S} passes {iters}")
     vol, sino = _cone_case(g, 12)
     p = mi.at_params(alpha, beta, eps, n_subsets=S, max_iters=iters)
