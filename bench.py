@@ -238,7 +238,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(jobs shard by GPU without any exchange; see DESIGN.md)", file=sys.stderr)
+              "(independent jobs shard by GPU without any exchange)", file=sys.stderr)
         sys.exit(2)
 
     import medimage_amd as mi
