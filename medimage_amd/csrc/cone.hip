@@ -52,6 +52,7 @@ constexpr int kConeGroup = 8;   // forward: walk steps per load group
 #endif
 constexpr int kZc = 8;          // back projection: slices per lane
 constexpr int kBpViewChunk = 36; // back projection: views per launch (their rows stay in the caches)
+constexpr int kBpRowLds = 4096;  // back projection: panels up to this many rows keep 1 / w in LDS
 
 __device__ __forceinline__ void cone_rot(float &x, float &y, float c, float s) {
     float d0 = 0.0f, d1 = 0.0f;
@@ -407,7 +408,9 @@ __device__ __forceinline__ void cone_slab_inv(double s, double ir, double a, dou
 
 // PAR: the parallel beam (rays at constant z, orthographic footprint; the
 // rows' z in rowtab[2 r], the columns' start points in vpos).
-template <bool PAR>
+// RLDS: the rows' 1 / w (nrows doubles) are copied to LDS (dynamic shared
+// memory) first.
+template <bool PAR, bool RLDS>
 __global__ __launch_bounds__(kBlock) void k_cone_bp(ConeDev g, const float *cs, const double *rowtab,
                                                     const double2 *__restrict__ vinv, const float *__restrict__ vl2,
                                                     const float4 *__restrict__ vsrc, const float2 *__restrict__ vpos,
@@ -422,6 +425,11 @@ __global__ __launch_bounds__(kBlock) void k_cone_bp(ConeDev g, const float *cs, 
     const int x = (tile / ty) * 16 + (wv >> 1) * 8 + (lane >> 3);
     const int y = (tile % ty) * 16 + (wv & 1) * 8 + (lane & 7);
     const int z0 = zc * kZc;
+    extern __shared__ double riw[];
+    if (RLDS) {
+        for (int r = threadIdx.x; r < g.nrows; r += kBlock) riw[r] = rowtab[2 * r + 1];
+        __syncthreads();
+    }
     if (x >= g.nx || y >= g.ny || z0 >= g.nz) return;
 
     // Per-lane sums of the kZc voxels in LDS (each lane owns its own slots,
@@ -515,14 +523,17 @@ __global__ __launch_bounds__(kBlock) void k_cone_bp(ConeDev g, const float *cs, 
                 const double dz0 = (double)z - sz;
                 double part = 0.0;
                 for (int r = r0; r <= r1; ++r) {
-                    const double w = rowtab[2 * r], iw = rowtab[2 * r + 1];
+                    const double iw = RLDS ? riw[r] : rowtab[2 * r + 1];
                     const float ta = (float)fma(dz0, iw, nlo), tb = (float)fma(dz0 + 1.0, iw, nlo);
                     const float a = fmaxf(0.0f, fminf(ta, tb)), e = fminf(span, fmaxf(ta, tb));
                     if (!(e > a)) continue;
-                    const float wf = (float)w;
-                    const float L = sqrtf(lxy2 + wf * wf);
+                    // w = 1 / iw and the ray length to about one float ulp
+                    // (hardware reciprocal and square root).
+                    const float wf = __builtin_amdgcn_rcpf((float)iw);
+                    const float L = __builtin_amdgcn_sqrtf(lxy2 + wf * wf);
                     const float len = (e - a) * L;
-                    part += (double)scol[(size_t)r * g.ncols] * (double)len;
+                    // float x float is exact in double: the fused form rounds once
+                    part = __builtin_fma((double)scol[(size_t)r * g.ncols], (double)len, part);
                 }
                 acc[k][threadIdx.x] += part;
             }
@@ -646,12 +657,16 @@ void cone_bp_views(hipStream_t st, const ConeTables &t, const int *views, int nv
     for (int c = 0; c < chunks; ++c) {
         const int v0 = c * kBpViewChunk, n = std::min(kBpViewChunk, nv - v0);
         if (t.d.parallel)
-            hipLaunchKernelGGL(k_cone_bp<true>, dim3(grid8(blocks)), dim3(kBlock), 0, st, t.d, t.cs.p, t.rows.p,
-                               t.vinv.p, t.vl2.p, t.vsrc.p, t.vpos.p, views + v0, n, sino, carry.p, c == 0,
+            hipLaunchKernelGGL((k_cone_bp<true, false>), dim3(grid8(blocks)), dim3(kBlock), 0, st, t.d, t.cs.p,
+                               t.rows.p, t.vinv.p, t.vl2.p, t.vsrc.p, t.vpos.p, views + v0, n, sino, carry.p, c == 0,
                                c == chunks - 1, vol);
+        else if (t.d.nrows <= kBpRowLds)
+            hipLaunchKernelGGL((k_cone_bp<false, true>), dim3(grid8(blocks)), dim3(kBlock),
+                               (unsigned)(t.d.nrows * sizeof(double)), st, t.d, t.cs.p, t.rows.p, t.vinv.p, t.vl2.p,
+                               t.vsrc.p, t.vpos.p, views + v0, n, sino, carry.p, c == 0, c == chunks - 1, vol);
         else
-            hipLaunchKernelGGL(k_cone_bp<false>, dim3(grid8(blocks)), dim3(kBlock), 0, st, t.d, t.cs.p, t.rows.p,
-                               t.vinv.p, t.vl2.p, t.vsrc.p, t.vpos.p, views + v0, n, sino, carry.p, c == 0,
+            hipLaunchKernelGGL((k_cone_bp<false, false>), dim3(grid8(blocks)), dim3(kBlock), 0, st, t.d, t.cs.p,
+                               t.rows.p, t.vinv.p, t.vl2.p, t.vsrc.p, t.vpos.p, views + v0, n, sino, carry.p, c == 0,
                                c == chunks - 1, vol);
         MI_HIP(hipGetLastError());
     }
