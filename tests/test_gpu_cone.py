@@ -36,6 +36,9 @@ SMALL = [
     mi.cone_geometry(20, 24, 16, 30, 20, 28, 1.0, 2.0, 60.0, 110.0),
     mi.cone_geometry(32, 32, 24, 45, 24, 40, 0.5, 0.8, 40.0, 70.0),
     mi.cone_geometry(24, 20, 16, 36, 24, 32, 1.0, 0.9, 30.0, 60.0, beam=mi.MI_CONE_PARALLEL),
+    # odd sizes: a partial slice chunk (13 = 8 + 5), partial 16 x 16 tiles,
+    # an odd panel (HALFDET = 8 of 17 rows), two back-projection launches
+    mi.cone_geometry(18, 22, 13, 50, 17, 30, 1.0, 1.7, 45.0, 80.0),
 ]
 
 
