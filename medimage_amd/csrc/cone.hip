@@ -133,11 +133,10 @@ __device__ __forceinline__ void ray_points(const ConeDev &g, float c, float s, i
 __device__ __forceinline__ float cone_lam(float i, float s, float d, float L) { return L * (i - s) / (d - s); }
 
 // Walk state of forward_proj: lam0 the current parameter, l* the next
-// crossings, st* their increments, (vx, vy, vz) the voxel, idx its offset.
+// crossings, st* their increments, (vx, vy, vz) the voxel.
 struct ConeWalk {
     float lam0, lx, ly, lz, stx, sty, stz, stop;
     int vx, vy, vz, sgx, sgy, sgz;
-    unsigned idx;
 };
 
 // Entry part of forward_proj (tracing.cpp:142-249).  stop is the smallest
@@ -205,8 +204,7 @@ __device__ __forceinline__ bool cone_walk_init(const ConeDev &g, float sx, float
     const double hi = (double)lmax - 5.e-2;
     float stop = (float)hi;
     if ((double)stop < hi) stop = nextafterf(stop, INFINITY);
-    w = ConeWalk{lam0, lx, ly, lz, stx, sty, stz, stop, vx, vy, vz, sgx, sgy, sgz,
-                 (unsigned)((vz * g.nx + vx) * g.ny + vy)};
+    w = ConeWalk{lam0, lx, ly, lz, stx, sty, stz, stop, vx, vy, vz, sgx, sgy, sgz};
     return true;
 }
 
@@ -214,24 +212,28 @@ __device__ __forceinline__ bool cone_walk_init(const ConeDev &g, float sx, float
 // and length; state unchanged and length 0 otherwise.  The branches of the
 // reference loop (tracing.cpp:268-305) become selects.
 __device__ __forceinline__ void cone_step(const ConeDev &g, ConeWalk &w, bool &live, unsigned &idx, float &len) {
-    // Bitwise, not short-circuit: no branch per step.
+    // Bitwise, not short-circuit: no branch per step.  `live` never turns
+    // back on, so after the end the state may keep moving: its offsets and
+    // lengths are masked.
     live = live & (w.lam0 < w.stop) & ((unsigned)w.vx < (unsigned)g.nx) & ((unsigned)w.vy < (unsigned)g.ny) &
            ((unsigned)w.vz < (unsigned)g.nz);
     const bool bx = (w.lx <= w.ly) && (w.lx <= w.lz);
     const bool by = !bx && (w.ly <= w.lz);
     const bool bz = !bx && !by;
     const float nxt = bx ? w.lx : (by ? w.ly : w.lz);
-    idx = live ? w.idx : 0;
+    // get_img_addr (tracing.cpp:119-121); 24-bit products (each factor is
+    // below 2^24 and the volume below 2^31 voxels).
+    const unsigned at = __umul24(__umul24((unsigned)w.vz, (unsigned)g.nx) + (unsigned)w.vx, (unsigned)g.ny) +
+                        (unsigned)w.vy;
+    idx = live ? at : 0u;
     len = live ? nxt - w.lam0 : 0.0f;
-    const bool tx = live && bx, ty = live && by, tz = live && bz;
-    w.lam0 = live ? nxt : w.lam0;
-    w.lx = tx ? w.lx + w.stx : w.lx;
-    w.ly = ty ? w.ly + w.sty : w.ly;
-    w.lz = tz ? w.lz + w.stz : w.lz;
-    w.vx += tx ? w.sgx : 0;
-    w.vy += ty ? w.sgy : 0;
-    w.vz += tz ? w.sgz : 0;
-    w.idx += (unsigned)(tx ? w.sgx * g.ny : (ty ? w.sgy : (tz ? w.sgz * g.nx * g.ny : 0)));
+    w.lam0 = nxt;
+    w.lx = bx ? w.lx + w.stx : w.lx;
+    w.ly = by ? w.ly + w.sty : w.ly;
+    w.lz = bz ? w.lz + w.stz : w.lz;
+    w.vx += bx ? w.sgx : 0;
+    w.vy += by ? w.sgy : 0;
+    w.vz += bz ? w.sgz : 0;
 }
 
 __device__ __forceinline__ bool cone_ray_init(const ConeDev &g, const float *cs, int view, int row, int col,
