@@ -210,8 +210,18 @@ __global__ __launch_bounds__(kBlock) void k_cat_edge_update(size_t n, float *__r
 // slot = (acc ? slot : 0) + sum part[0 .. n), fixed order, one workgroup.
 __global__ __launch_bounds__(kBlock) void k_cat_reduce(const double *__restrict__ part, size_t n,
                                                        double *__restrict__ slot, int acc) {
+    // eight loads in flight per lane; the order of the additions is fixed
     double s = 0.0;
-    for (size_t i = threadIdx.x; i < n; i += kBlock) s += part[i];
+    for (size_t base = 0; base < n; base += (size_t)kBlock * 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const size_t i = base + (size_t)u * kBlock + threadIdx.x;
+            v[u] = i < n ? part[i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u];
+    }
     const double t = block_sum(s);
     if (threadIdx.x == 0) *slot = acc ? *slot + t : t;
 }
