@@ -495,20 +495,20 @@ void Batch<T>::record_pass(bool timed) {
         {
             LaunchTimer lt(ctx, MI_K_FP, sum([&](const Unit<T> &U) {
                 return nnz_s(U) * (4 + sz) + U.m->npix * sz * U.K + 2 * nr_s(U) * sz * U.K;
-            }));
+            }), true);
             eng_fp_resid<T>(st, args, rows, s, kmax, all_walk);
         }
         {
             LaunchTimer lt(ctx, MI_K_BP, sum([&](const Unit<T> &U) {
                 return nnz_s(U) * (4 + sz) + nr_s(U) * sz * U.K + 3.0 * U.m->npix * sz * U.K;
-            }));
+            }), true);
             if (bpl.start[bpl.nunits] > 0) eng_bp_grad<T>(st, args, bpl, s, kmax_l, bp_lds);
             if (bpg.start[bpg.nunits] > 0) eng_bp_grad<T>(st, args, bpg, s, kmax_g, 0);
         }
         {
             LaunchTimer lt(ctx, MI_K_FPSQ, sum([&](const Unit<T> &U) {
                 return nnz_s(U) * (4 + sz) + U.m->npix * sz * U.K;
-            }));
+            }), true);
             eng_fp_sq<T>(st, args, rows, s, kmax, all_walk);
         }
         {

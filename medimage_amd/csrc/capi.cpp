@@ -139,7 +139,9 @@ int64_t evict_idle(mi_context *ctx, int64_t bytes) {
     return freed;
 }
 
-LaunchTimer::LaunchTimer(mi_context *c, int k, double by) : ctx(c), kernel(k), bytes(by) {
+thread_local DispatchEvents *tl_dispatch = nullptr;
+
+LaunchTimer::LaunchTimer(mi_context *c, int k, double by, bool disp) : ctx(c), kernel(k), bytes(by) {
     if (!ctx->stats.on || !((ctx->stats.mask >> k) & 1u)) return;
     auto &pool = ctx->stats.pool;
     auto take = [&]() {
@@ -150,12 +152,30 @@ LaunchTimer::LaunchTimer(mi_context *c, int k, double by) : ctx(c), kernel(k), b
     };
     a = take();
     b = take();
-    MI_HIP(hipEventRecord(a, ctx->stream));
+    if (disp) {
+        dispatch = true;
+        de.a = a;
+        de.b = b;
+        prev = tl_dispatch;
+        tl_dispatch = &de;
+    } else {
+        MI_HIP(hipEventRecord(a, ctx->stream));
+    }
 }
 
 LaunchTimer::~LaunchTimer() {
     if (!a) return;
-    if (hipEventRecord(b, ctx->stream) != hipSuccess) return;
+    if (dispatch) {
+        tl_dispatch = prev;
+        if (de.launches == 0) {          // nothing was launched in the scope
+            ctx->stats.pool.push_back(a);
+            ctx->stats.pool.push_back(b);
+            return;
+        }
+        if (de.launches > 1 && hipEventRecord(b, ctx->stream) != hipSuccess) return;
+    } else if (hipEventRecord(b, ctx->stream) != hipSuccess) {
+        return;
+    }
     ctx->stats.pending.push_back({kernel, a, b, bytes});
 }
 
@@ -316,7 +336,7 @@ void cone_forward_dev(mi_context *ctx, const mi_cone_geometry *g, const float *v
     MI_REQUIRE(g && vol && sino, MI_E_ARG, "null argument");
     ConeTables &t = cone_tables(ctx, *g);
     {
-        LaunchTimer lt(ctx, MI_K_CONE_FP, cone_timed_bytes(ctx, t, MI_K_CONE_FP));
+        LaunchTimer lt(ctx, MI_K_CONE_FP, cone_timed_bytes(ctx, t, MI_K_CONE_FP), true);
         cone_forward(ctx->stream, t, vol, sino);
     }
     MI_HIP(hipStreamSynchronize(ctx->stream));
@@ -327,7 +347,7 @@ void cone_backproject_dev(mi_context *ctx, const mi_cone_geometry *g, const floa
     MI_REQUIRE(g && vol && sino, MI_E_ARG, "null argument");
     ConeTables &t = cone_tables(ctx, *g);
     {
-        LaunchTimer lt(ctx, MI_K_CONE_BP, cone_timed_bytes(ctx, t, MI_K_CONE_BP));
+        LaunchTimer lt(ctx, MI_K_CONE_BP, cone_timed_bytes(ctx, t, MI_K_CONE_BP), true);
         cone_backproject(ctx->stream, t, sino, vol);
     }
     MI_HIP(hipStreamSynchronize(ctx->stream));

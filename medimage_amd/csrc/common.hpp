@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cstdint>
 #include <stdexcept>
@@ -22,6 +23,25 @@ void hip_check(hipError_t e, const char *what, const char *file, int line);
 #define MI_HIP(call) ::mi::hip_check((call), #call, __FILE__, __LINE__)
 #define MI_REQUIRE(cond, code, msg) \
     do { if (!(cond)) ::mi::fail((code), (msg)); } while (0)
+
+// Dispatch-timestamp events for the kernel launches of a timed scope: the
+// first launch through mi::launch records `a` at its start and `b` at its end
+// (the dispatch's own timestamps, no separate event packets); later launches
+// of the scope are counted and the scope records `b` again after them.
+struct DispatchEvents {
+    hipEvent_t a = nullptr, b = nullptr;
+    int launches = 0;
+};
+extern thread_local DispatchEvents *tl_dispatch;
+
+template <typename Kernel, typename... Args>
+inline void launch(Kernel kernel, dim3 grid, dim3 block, uint32_t shmem, hipStream_t st, Args... args) {
+    DispatchEvents *d = tl_dispatch;
+    if (d && d->launches++ == 0)
+        hipExtLaunchKernelGGL(kernel, grid, block, shmem, st, d->a, d->b, 0u, args...);
+    else
+        hipLaunchKernelGGL(kernel, grid, block, shmem, st, args...);
+}
 
 constexpr int kWave = 64;        // CDNA wavefront width
 constexpr int kBlock = 256;      // 4 waves per workgroup

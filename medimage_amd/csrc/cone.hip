@@ -638,13 +638,13 @@ size_t cone_fp_views(hipStream_t st, const ConeTables &t, const int *views, int 
     const dim3 grid(grid8(nb)), block(kBlock);
     switch (mode) {
         case CFP_WRITE:
-            hipLaunchKernelGGL(k_cone_fp<CFP_WRITE>, grid, block, 0, st, t.d, t.cs.p, views, nv, vol, meas, out, part);
+            launch(k_cone_fp<CFP_WRITE>, grid, block, 0u, st, t.d, t.cs.p, views, nv, vol, meas, out, part);
             break;
         case CFP_RESID:
-            hipLaunchKernelGGL(k_cone_fp<CFP_RESID>, grid, block, 0, st, t.d, t.cs.p, views, nv, vol, meas, out, part);
+            launch(k_cone_fp<CFP_RESID>, grid, block, 0u, st, t.d, t.cs.p, views, nv, vol, meas, out, part);
             break;
         default:
-            hipLaunchKernelGGL(k_cone_fp<CFP_SQ>, grid, block, 0, st, t.d, t.cs.p, views, nv, vol, meas, out, part);
+            launch(k_cone_fp<CFP_SQ>, grid, block, 0u, st, t.d, t.cs.p, views, nv, vol, meas, out, part);
     }
     MI_HIP(hipGetLastError());
     return nb;
@@ -659,15 +659,15 @@ void cone_bp_views(hipStream_t st, const ConeTables &t, const int *views, int nv
     for (int c = 0; c < chunks; ++c) {
         const int v0 = c * kBpViewChunk, n = std::min(kBpViewChunk, nv - v0);
         if (t.d.parallel)
-            hipLaunchKernelGGL((k_cone_bp<true, false>), dim3(grid8(blocks)), dim3(kBlock), 0, st, t.d, t.cs.p,
+            launch(k_cone_bp<true, false>, dim3(grid8(blocks)), dim3(kBlock), 0u, st, t.d, t.cs.p,
                                t.rows.p, t.vinv.p, t.vl2.p, t.vsrc.p, t.vpos.p, views + v0, n, sino, carry.p, c == 0,
                                c == chunks - 1, vol);
         else if (t.d.nrows <= kBpRowLds)
-            hipLaunchKernelGGL((k_cone_bp<false, true>), dim3(grid8(blocks)), dim3(kBlock),
-                               (unsigned)(t.d.nrows * sizeof(double)), st, t.d, t.cs.p, t.rows.p, t.vinv.p, t.vl2.p,
+            launch(k_cone_bp<false, true>, dim3(grid8(blocks)), dim3(kBlock),
+                               (uint32_t)(t.d.nrows * sizeof(double)), st, t.d, t.cs.p, t.rows.p, t.vinv.p, t.vl2.p,
                                t.vsrc.p, t.vpos.p, views + v0, n, sino, carry.p, c == 0, c == chunks - 1, vol);
         else
-            hipLaunchKernelGGL((k_cone_bp<false, false>), dim3(grid8(blocks)), dim3(kBlock), 0, st, t.d, t.cs.p,
+            launch(k_cone_bp<false, false>, dim3(grid8(blocks)), dim3(kBlock), 0u, st, t.d, t.cs.p,
                                t.rows.p, t.vinv.p, t.vl2.p, t.vsrc.p, t.vpos.p, views + v0, n, sino, carry.p, c == 0,
                                c == chunks - 1, vol);
         MI_HIP(hipGetLastError());

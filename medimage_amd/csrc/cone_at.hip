@@ -291,19 +291,19 @@ void cone_at_solve(mi_context *ctx, const mi_cone_geometry &g, const mi_at_param
             const double frac = (double)nv / g.n_views;
             size_t nb;
             {
-                LaunchTimer lt(ctx, MI_K_CONE_FP, timed_fp ? frac * cone_fp_bytes(t) : 0.0);
+                LaunchTimer lt(ctx, MI_K_CONE_FP, timed_fp ? frac * cone_fp_bytes(t) : 0.0, true);
                 nb = cone_fp_views(st, t, views, nv, CFP_RESID, u.p, sino.p, q.p, part0.p);
             }
             reduce(part0.p, nb, 0, 1);
             {
-                LaunchTimer lt(ctx, MI_K_CONE_BP, timed_bp ? frac * cone_bp_bytes(t) : 0.0);
+                LaunchTimer lt(ctx, MI_K_CONE_BP, timed_bp ? frac * cone_bp_bytes(t) : 0.0, true);
                 cone_bp_views(st, t, views, nv, q.p, b.p);
             }
             hipLaunchKernelGGL(k_cat_grad, vg, blk, 0, st, dm, b.p, u.p, v.p, (float)S, (float)p.alpha, eta, d.p,
                                part1.p);
             reduce(part1.p, vblocks, 1, 0);
             {
-                LaunchTimer lt(ctx, MI_K_CONE_FP, timed_fp ? frac * cone_fp_bytes(t) : 0.0);
+                LaunchTimer lt(ctx, MI_K_CONE_FP, timed_fp ? frac * cone_fp_bytes(t) : 0.0, true);
                 nb = cone_fp_views(st, t, views, nv, CFP_SQ, d.p, nullptr, nullptr, part0.p);
             }
             reduce(part0.p, nb, 2, 0);

@@ -1136,13 +1136,13 @@ template <typename T>
 void eng_fp_resid(hipStream_t st, const EngArgs<T> &a, const BlockMap &m, int s, int kmax, bool all_walk) {
     const unsigned grid = grid8(m.start[m.nunits]);
     if (all_walk) {
-        if (kmax == 1) k_eng_fp<T, 1, true, 1><<<grid, kBlock, 0, st>>>(a, m, s);
-        else if (kmax <= 4) k_eng_fp<T, 4, true, 1><<<grid, kBlock, 0, st>>>(a, m, s);
-        else k_eng_fp<T, kMaxK, true, 1><<<grid, kBlock, 0, st>>>(a, m, s);
+        if (kmax == 1) launch(k_eng_fp<T, 1, true, 1>, dim3(grid), dim3(kBlock), (uint32_t)(0), st, a, m, s);
+        else if (kmax <= 4) launch(k_eng_fp<T, 4, true, 1>, dim3(grid), dim3(kBlock), (uint32_t)(0), st, a, m, s);
+        else launch(k_eng_fp<T, kMaxK, true, 1>, dim3(grid), dim3(kBlock), (uint32_t)(0), st, a, m, s);
     } else {
-        if (kmax == 1) k_eng_fp<T, 1, true, 3><<<grid, kBlock, 0, st>>>(a, m, s);
-        else if (kmax <= 4) k_eng_fp<T, 4, true, 3><<<grid, kBlock, 0, st>>>(a, m, s);
-        else k_eng_fp<T, kMaxK, true, 3><<<grid, kBlock, 0, st>>>(a, m, s);
+        if (kmax == 1) launch(k_eng_fp<T, 1, true, 3>, dim3(grid), dim3(kBlock), (uint32_t)(0), st, a, m, s);
+        else if (kmax <= 4) launch(k_eng_fp<T, 4, true, 3>, dim3(grid), dim3(kBlock), (uint32_t)(0), st, a, m, s);
+        else launch(k_eng_fp<T, kMaxK, true, 3>, dim3(grid), dim3(kBlock), (uint32_t)(0), st, a, m, s);
     }
     MI_HIP(hipGetLastError());
 }
@@ -1163,13 +1163,13 @@ void eng_bp_grad(hipStream_t st, const EngArgs<T> &a, const BlockMap &m, int s, 
             }();
             (void)once;
         }
-        if (kmax == 1) k_eng_bp<T, 1, true><<<grid, kBlock, lds_bytes, st>>>(a, m, s);
-        else if (kmax <= 4) k_eng_bp<T, 4, true><<<grid, kBlock, lds_bytes, st>>>(a, m, s);
-        else k_eng_bp<T, kMaxK, true><<<grid, kBlock, lds_bytes, st>>>(a, m, s);
+        if (kmax == 1) launch(k_eng_bp<T, 1, true>, dim3(grid), dim3(kBlock), (uint32_t)(lds_bytes), st, a, m, s);
+        else if (kmax <= 4) launch(k_eng_bp<T, 4, true>, dim3(grid), dim3(kBlock), (uint32_t)(lds_bytes), st, a, m, s);
+        else launch(k_eng_bp<T, kMaxK, true>, dim3(grid), dim3(kBlock), (uint32_t)(lds_bytes), st, a, m, s);
     } else {
-        if (kmax == 1) k_eng_bp<T, 1, false><<<grid, kBlock, 0, st>>>(a, m, s);
-        else if (kmax <= 4) k_eng_bp<T, 4, false><<<grid, kBlock, 0, st>>>(a, m, s);
-        else k_eng_bp<T, kMaxK, false><<<grid, kBlock, 0, st>>>(a, m, s);
+        if (kmax == 1) launch(k_eng_bp<T, 1, false>, dim3(grid), dim3(kBlock), (uint32_t)(0), st, a, m, s);
+        else if (kmax <= 4) launch(k_eng_bp<T, 4, false>, dim3(grid), dim3(kBlock), (uint32_t)(0), st, a, m, s);
+        else launch(k_eng_bp<T, kMaxK, false>, dim3(grid), dim3(kBlock), (uint32_t)(0), st, a, m, s);
     }
     MI_HIP(hipGetLastError());
 }
@@ -1178,13 +1178,13 @@ template <typename T>
 void eng_fp_sq(hipStream_t st, const EngArgs<T> &a, const BlockMap &rows, int s, int kmax, bool all_walk) {
     const unsigned grid = grid8(rows.start[rows.nunits]);
     if (all_walk) {
-        if (kmax == 1) k_eng_fp<T, 1, false, 1><<<grid, kBlock, 0, st>>>(a, rows, s);
-        else if (kmax <= 4) k_eng_fp<T, 4, false, 1><<<grid, kBlock, 0, st>>>(a, rows, s);
-        else k_eng_fp<T, kMaxK, false, 1><<<grid, kBlock, 0, st>>>(a, rows, s);
+        if (kmax == 1) launch(k_eng_fp<T, 1, false, 1>, dim3(grid), dim3(kBlock), (uint32_t)(0), st, a, rows, s);
+        else if (kmax <= 4) launch(k_eng_fp<T, 4, false, 1>, dim3(grid), dim3(kBlock), (uint32_t)(0), st, a, rows, s);
+        else launch(k_eng_fp<T, kMaxK, false, 1>, dim3(grid), dim3(kBlock), (uint32_t)(0), st, a, rows, s);
     } else {
-        if (kmax == 1) k_eng_fp<T, 1, false, 3><<<grid, kBlock, 0, st>>>(a, rows, s);
-        else if (kmax <= 4) k_eng_fp<T, 4, false, 3><<<grid, kBlock, 0, st>>>(a, rows, s);
-        else k_eng_fp<T, kMaxK, false, 3><<<grid, kBlock, 0, st>>>(a, rows, s);
+        if (kmax == 1) launch(k_eng_fp<T, 1, false, 3>, dim3(grid), dim3(kBlock), (uint32_t)(0), st, a, rows, s);
+        else if (kmax <= 4) launch(k_eng_fp<T, 4, false, 3>, dim3(grid), dim3(kBlock), (uint32_t)(0), st, a, rows, s);
+        else launch(k_eng_fp<T, kMaxK, false, 3>, dim3(grid), dim3(kBlock), (uint32_t)(0), st, a, rows, s);
     }
     MI_HIP(hipGetLastError());
 }

@@ -251,13 +251,19 @@ std::shared_ptr<Matrix> get_engine_matrix(mi_context *ctx, const mi_geometry &g,
 // Frees cached matrices that no job holds, least recently used first, until
 // at least `bytes` were freed (all idle ones when bytes < 0).  Returns bytes freed.
 int64_t evict_idle(mi_context *ctx, int64_t bytes);
-// Stats helpers: bracket a launch with events when enabled.
+// Stats helpers: bracket a launch with events when enabled.  With
+// `dispatch`, the scope's kernels launched through mi::launch carry the
+// events themselves (kernel start to kernel end, as a profiler reads them);
+// otherwise the events are recorded on the stream around the scope.
 struct LaunchTimer {
     mi_context *ctx;
     int kernel;
     double bytes;
     hipEvent_t a = nullptr, b = nullptr;
-    LaunchTimer(mi_context *c, int k, double by);
+    DispatchEvents de;
+    DispatchEvents *prev = nullptr;
+    bool dispatch = false;
+    LaunchTimer(mi_context *c, int k, double by, bool dispatch = false);
     ~LaunchTimer();
 };
 void stats_collect(mi_context *ctx);
