@@ -1012,12 +1012,53 @@ __device__ __forceinline__ double fin_reduce(const GL double *p, int64_t n, bool
     return t;
 }
 
+// Fixed-order block sums of up to four partial arrays at once (all their
+// loads in flight together), 256 threads; out[i] = sum of p[i][0 .. n[i]).
+template <int NA>
+__device__ __forceinline__ void fin_reduce_n(const GL double *const (&p)[NA], const int64_t (&n)[NA],
+                                             double (&out)[NA], double *lds) {
+    double acc[NA];
+#pragma unroll
+    for (int q = 0; q < NA; ++q) acc[q] = 0.0;
+    int64_t nmax = 0;
+#pragma unroll
+    for (int q = 0; q < NA; ++q) nmax = n[q] > nmax ? n[q] : nmax;
+    for (int64_t base = 0; base < nmax; base += (int64_t)kBlock * 4) {
+        double v[NA][4];
+#pragma unroll
+        for (int q = 0; q < NA; ++q)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int64_t i = base + u * kBlock + threadIdx.x;
+                v[q][u] = (i < n[q]) ? p[q][i] : 0.0;
+            }
+#pragma unroll
+        for (int q = 0; q < NA; ++q)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc[q] += v[q][u];
+    }
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+#pragma unroll
+    for (int q = 0; q < NA; ++q) {
+        acc[q] = group_sum<kWave>(acc[q]);
+        if (lane == 0) lds[q * (kBlock / kWave) + wv] = acc[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NA; ++q) {
+        double t = 0.0;
+        for (int w = 0; w < kBlock / kWave; ++w) t += lds[q * (kBlock / kWave) + w];
+        out[q] = t;
+    }
+    __syncthreads();
+}
+
 __host__ __device__ __forceinline__ int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // One block per job slot (units listed in m; m.nunits * kMaxK blocks).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_eng_fin(EngArgs<T> a, int op, int s) {
-    __shared__ double lds[kBlock / kWave];
+    __shared__ double lds[4 * (kBlock / kWave)];
     const int ui = blockIdx.x / kMaxK, j = blockIdx.x % kMaxK;
     const UG<T> U(a.units[ui]);
     if (j >= U.K) return;
@@ -1036,10 +1077,12 @@ __global__ __launch_bounds__(kBlock) void k_eng_fin(EngArgs<T> a, int op, int s)
         case EF_STEP: {
             const int64_t nr = cdiv(U.fs_slice0[s + 1] - U.fs_slice0[s], kBlock / kWave);
             const int64_t nb = cdiv(U.bs_slice0[s + 1] - U.bs_slice0[s], kBlock / kWave);
-            const double dd = fin_reduce(part_at(a, U, SL_DD, j), nb, false, lds);
-            const double ad2 = fin_reduce(part_at(a, U, SL_AD, j), nr, false, lds);
-            const double nv = fin_reduce(part_at(a, U, SL_NV, j), npx, false, lds);
-            const double df = fin_reduce(part_at(a, U, SL_DF, j), nr, false, lds);
+            const GL double *const p[4] = {part_at(a, U, SL_DD, j), part_at(a, U, SL_AD, j),
+                                           part_at(a, U, SL_NV, j), part_at(a, U, SL_DF, j)};
+            const int64_t n[4] = {nb, nr, npx, nr};
+            double r[4];
+            fin_reduce_n<4>(p, n, r, lds);
+            const double dd = r[0], ad2 = r[1], nv = r[2], df = r[3];
             if (threadIdx.x == 0) {
                 sc->df_pass += df;
                 sc->dd = dd;
@@ -1052,9 +1095,12 @@ __global__ __launch_bounds__(kBlock) void k_eng_fin(EngArgs<T> a, int op, int s)
         }
         case EF_EDGE_C: {
             if (sc->edge_done || s >= P.iter_edge) return;
-            const double nd = fin_reduce(part_at(a, U, SL_E0, j), npx, false, lds);
-            const double nd2 = fin_reduce(part_at(a, U, SL_E1, j), npx, false, lds);
-            const double nnd = fin_reduce(part_at(a, U, SL_E2, j), npx, false, lds);
+            const GL double *const p[3] = {part_at(a, U, SL_E0, j), part_at(a, U, SL_E1, j),
+                                           part_at(a, U, SL_E2, j)};
+            const int64_t n[3] = {npx, npx, npx};
+            double r[3];
+            fin_reduce_n<3>(p, n, r, lds);
+            const double nd = r[0], nd2 = r[1], nnd = r[2];
             if (threadIdx.x == 0) {
                 // origin/GoldenC/src/main.c:329
                 sc->e_nd = nd;
