@@ -217,10 +217,15 @@ __device__ __forceinline__ void cone_step(const ConeDev &g, ConeWalk &w, bool &l
     // lengths are masked.
     live = live & (w.lam0 < w.stop) & ((unsigned)w.vx < (unsigned)g.nx) & ((unsigned)w.vy < (unsigned)g.ny) &
            ((unsigned)w.vz < (unsigned)g.nz);
-    const bool bx = (w.lx <= w.ly) && (w.lx <= w.lz);
-    const bool by = !bx && (w.ly <= w.lz);
+    // The reference's branch choice (x if lx <= ly and lx <= lz, else y if
+    // ly <= lz, else z) is "the first axis whose crossing equals the
+    // minimum": the crossings are finite (parallel axes carry the 1e6
+    // sentinel) and positive, so one three-way minimum and two equality
+    // tests give the same choice and the same value.
+    const float nxt = fminf(fminf(w.lx, w.ly), w.lz);
+    const bool bx = w.lx == nxt;
+    const bool by = !bx && w.ly == nxt;
     const bool bz = !bx && !by;
-    const float nxt = bx ? w.lx : (by ? w.ly : w.lz);
     // get_img_addr (tracing.cpp:119-121); 24-bit products (each factor is
     // below 2^24 and the volume below 2^31 voxels).
     const unsigned at = __umul24(__umul24((unsigned)w.vz, (unsigned)g.nx) + (unsigned)w.vx, (unsigned)g.ny) +
