@@ -274,8 +274,9 @@ __global__ void k_cone_trace(ConeDev g, const float *cs, int view, int row, int 
 // flight trace rays through the same few slices of the volume.  Consecutive
 // workgroups go to different XCDs (no XCD regrouping here): all eight XCDs
 // then work on one band at a time and share its slices through the Infinity
-// Cache (368 ms at C4 against 385 ms with each XCD on its own range of bands).  Returns false for the padding workgroups of the
-// grid (all lanes alike); `ok` is false for lanes outside the panel.
+// Cache (368 ms at C4 against 385 ms with each XCD on its own range of
+// bands).  Returns false for the padding workgroups of the grid (all lanes
+// alike); `ok` is false for lanes outside the panel.
 __device__ __forceinline__ bool cone_fp_pixel(const ConeDev &g, int nv, unsigned &tile_id, int &q, int &row,
                                               int &col, bool &ok) {
     const int tc = (g.ncols + kConeTile - 1) / kConeTile, tr = (g.nrows + kConeTile - 1) / kConeTile;
