@@ -116,9 +116,10 @@ typedef struct mi_context mi_context;
 int mi_context_create(int device, mi_context **out);
 int mi_context_destroy(mi_context *ctx);
 const char *mi_last_error(void);
-/* Bytes of device memory the context holds (matrices + work buffers). */
+/* Bytes of device memory the context holds (cached matrices and the cone-beam
+ * geometry tables). */
 int mi_context_mem(mi_context *ctx, int64_t *bytes);
-/* Drop cached system matrices (all geometries). */
+/* Drop cached system matrices (all geometries) and the cone-beam tables. */
 int mi_context_trim(mi_context *ctx);
 
 /* ---- geometry / system matrix ------------------------------------------ */

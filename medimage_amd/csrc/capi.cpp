@@ -411,6 +411,11 @@ int mi_context_mem(mi_context *ctx, int64_t *bytes) {
         MI_REQUIRE(bytes, MI_E_ARG, "null output");
         int64_t b = 0;
         for (auto &kv : ctx->matrices) b += (int64_t)kv.second->bytes();
+        if (ctx->cone) {
+            const ConeTables &t = *ctx->cone;
+            b += (int64_t)(t.cs.bytes() + t.rows.bytes() + t.all.bytes() + t.vinv.bytes() + t.vl2.bytes() +
+                           t.vsrc.bytes() + t.vpos.bytes());
+        }
         *bytes = b;
     });
 }
@@ -420,6 +425,7 @@ int mi_context_trim(mi_context *ctx) {
         check_ctx(ctx);
         MI_HIP(hipStreamSynchronize(ctx->stream));
         ctx->matrices.clear();
+        ctx->cone.reset();
     });
 }
 

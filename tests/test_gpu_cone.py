@@ -98,6 +98,16 @@ def test_cone_pair_adjoint_small(ctx):
     assert abs(lhs - rhs) / abs(rhs) < 1e-3
 
 
+def test_cone_tables_in_context_memory(ctx):
+    g = SMALL[0]
+    ctx.trim()
+    m0 = ctx.mem_bytes()
+    ctx.cone_forward(g, np.zeros((g.nz, g.nx, g.ny), np.float32))
+    assert ctx.mem_bytes() > m0
+    ctx.trim()
+    assert ctx.mem_bytes() == 0
+
+
 def test_cone_rejects_bad_geometry(ctx):
     g = mi.cone_geometry(64, 64, 64, 10, 8, 8, 1.0, 1.0, 30.0, 60.0)   # source inside the volume
     with pytest.raises(mi.MIError):
