@@ -6,7 +6,9 @@
 // (HALFDET - row - 0.5) PIXELSIZE) are rotated by the view angle, divided
 // by SAMPLESIZE and shifted by N / 2 (in double, as the reference's
 // `+= NX / 2.0` does).  The view's cos / sin come from host tables made
-// with the C library's cosf / sinf, which the reference calls.
+// with the C library's cosf / sinf, which the reference calls.  BEAM
+// "Parallel" takes parallel() (tracing.cpp:9-49) instead: start and end share
+// the column's x and the row's z, every axis shifted by HALFSIZE.
 //
 // Forward projection.  One lane per ray walks forward_proj
 // (industry3d/src/tracing.cpp:123-307) in float, operation for operation,
@@ -28,8 +30,10 @@
 // inside the voxel is the slab clip of the ray's parameter interval: the x / y
 // slab in double once per (view, column) from a per-geometry table of
 // 1 / (d - s), shared by the lane's kZc slices; the z slab per row from a
-// per-row 1 / w table, relative to the column's entry parameter so that the
-// float differences keep full precision; times the ray length in float.
+// per-row 1 / w table (copied to LDS), relative to the column's entry
+// parameter so that the float differences keep full precision; times the ray
+// length in float.  For the parallel beam the footprint is orthographic and
+// a row's ray crosses a slice over its whole x / y chord or not at all.
 // sino * length is summed in double, per (view, column, slice) over the rows
 // in a register and then into the voxel's sum in LDS, in a fixed order.  A
 // back projection over many views runs in chunks of kBpViewChunk views (their
