@@ -189,11 +189,15 @@ def bench_c4(ctx, reps, at_iters, log):
         s = st[k]
         ms = s.total_ms / max(s.launches, 1)
         b = s.bytes / max(s.launches, 1)
+        # The projectors are matrix-free: the algorithmic bytes are those of
+        # the equivalent stored matrix (8 B per traced entry), so the rate
+        # can exceed what HBM could stream.
         out[name] = {"launches": s.launches, "avg_ms": ms, "rays_per_s": rays / (ms * 1e-3),
                      "voxel_views_per_s": vox * g.n_views / (ms * 1e-3),
-                     "algorithmic_bytes": b, "GBps": b / (ms * 1e-3) / 1e9,
-                     "frac_of_hbm_peak": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+                     "algorithmic_bytes": b, "algorithmic_GBps": b / (ms * 1e-3) / 1e9,
+                     "stored_matrix_time_at_peak_ms": b / (HBM_PEAK_GBS * 1e9) * 1e3}
     out["nnz"] = int(round((out["forward"]["algorithmic_bytes"] - 4.0 * (rays + vox)) / 8.0))
+    out["traffic_source"] = "profiles/c4_cone_kernel_stats.md"
     if at_iters > 0:
         t0 = time.perf_counter()
         noisy = np.empty_like(sino)
