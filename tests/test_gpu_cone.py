@@ -98,6 +98,28 @@ def test_cone_pair_adjoint_small(ctx):
     assert abs(lhs - rhs) / abs(rhs) < 1e-3
 
 
+@pytest.mark.parametrize("g", [
+    mi.cone_geometry(5, 7, 1, 1, 1, 3, 1.0, 1.0, 20.0, 40.0),                         # one slice, one view
+    mi.cone_geometry(6, 6, 3, 2, 2, 2, 1.0, 1.0, 15.0, 30.0, beam=mi.MI_CONE_PARALLEL),
+    mi.cone_geometry(9, 4, 5, 7, 9, 1, 0.7, 1.3, 12.0, 25.0),                         # one panel column
+])
+def test_cone_tiny_geometries(ctx, g):
+    """Degenerate sizes: forward bitwise and back projection within 1e-5 of
+    the oracle (or both zero)."""
+    rng = np.random.default_rng(3)
+    vol = rng.random((g.nz, g.nx, g.ny)).astype(np.float32)
+    fp = ctx.cone_forward(g, vol)
+    ofp = O.cone_fp(O.cone_from(g), vol).astype(np.float32)
+    assert np.array_equal(fp.view(np.uint32), ofp.view(np.uint32))
+    sino = rng.random((g.n_views, g.n_rows, g.n_cols)).astype(np.float32)
+    bp = ctx.cone_backproject(g, sino)
+    obp = O.cone_bp(O.cone_from(g), sino.astype(np.float64))
+    if np.abs(obp).max() == 0:
+        assert np.abs(bp).max() == 0
+    else:
+        assert rel_max(bp, obp) < 1e-5
+
+
 def test_cone_tables_in_context_memory(ctx):
     g = SMALL[0]
     ctx.trim()
