@@ -10,9 +10,12 @@ resident in HBM.  Synthetic input: a seeded 10-ellipse phantom with Poisson
 noise at I0 = 2e4.
 
 Prints one JSON line: AT iterations/s (value), rays/s, the roofline of the
-dominant kernel from HIP events recorded on the library's stream during the
-timed region, and the CPU oracle timed on a bounded sample of the same
-workload on this host.
+dominant kernel from HIP events on the library's stream during the timed
+region (the dispatch's own start/stop events), and the CPU oracle timed on a
+bounded sample of the same workload on this host.  Extra blocks of the same
+line: the 64-slice batched form (C3), the served request stream (C5) and the
+3D cone beam (C4: 512^3 volume, 720 views of a 1024 x 768 panel; projector
+kernel times and ordered-subset AT passes).
 """
 import argparse
 import json
