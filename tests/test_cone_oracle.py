@@ -13,7 +13,11 @@ closed forms and exact rational arithmetic:
   plane crossings inside the volume plus one (less a dropped final piece
   shorter than 0.05), and the lengths add up to the chord;
 * the exact-length matrix (or_cone_exact, the back projector's) equals a
-  per-voxel slab clip of every voxel of a small grid.
+  per-voxel slab clip of every voxel of a small grid;
+* the 26-neighbourhood AT stencils equal finite differences of the energies
+  written independently in numpy (pair sums over the padded grid), the edge
+  step is the exact minimiser along its direction, and one ordered-subset
+  pass equals its restatement from dense matrices.
 """
 import math
 from fractions import Fraction
