@@ -374,6 +374,9 @@ int mi_cone_backproject_dev(mi_context *ctx, const mi_cone_geometry *g, const fl
  * initial values and receive the result. */
 int mi_cone_at_reconstruct(mi_context *ctx, const mi_cone_geometry *g, const mi_at_params *p, const float *sino,
                            float *u, float *v, mi_report *rep);
+/* The same on device buffers (sinogram, u, v), updated in place. */
+int mi_cone_at_reconstruct_dev(mi_context *ctx, const mi_cone_geometry *g, const mi_at_params *p,
+                               const float *sino, float *u, float *v, mi_report *rep);
 /* Fills *g and *p from the text of an industry3d configuration file
  * (Parameter::parse_config, industry3d/src/ct3d.cpp:20-93: BEAM "Cone" or
  * "Parallel", NX, NY, NZ, NPROJ, NDX, NDY, SAMPLESIZE, PIXELSIZE, SOD, SDD,

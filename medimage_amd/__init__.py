@@ -201,6 +201,7 @@ def _load():
         "mi_cone_backproject": [P, CG, P, P],
         "mi_cone_backproject_dev": [P, CG, P, P],
         "mi_cone_at_reconstruct": [P, CG, C.POINTER(AtParams), P, P, P, C.POINTER(Report)],
+        "mi_cone_at_reconstruct_dev": [P, CG, C.POINTER(AtParams), P, P, P, C.POINTER(Report)],
         "mi_cone_config_parse": [C.c_char_p, CG, C.POINTER(AtParams)],
     }
     for name, args in sigs.items():
@@ -385,6 +386,13 @@ class Context:
         _check(self._lib.mi_cone_at_reconstruct(self._h, C.byref(g), C.byref(p), _ptr(sino), _ptr(u), _ptr(v),
                                                 C.byref(rep)))
         return u, v, rep
+
+    def cone_at_reconstruct_dev(self, g: ConeGeometry, sino_ptr: int, u_ptr: int, v_ptr: int, p: AtParams):
+        """Device pointers: u and v hold the initial values and are updated in place."""
+        rep = Report()
+        _check(self._lib.mi_cone_at_reconstruct_dev(self._h, C.byref(g), C.byref(p), C.c_void_p(sino_ptr),
+                                                    C.c_void_p(u_ptr), C.c_void_p(v_ptr), C.byref(rep)))
+        return rep
 
     # -- GoldenC building blocks -----------------------------------------
     def image_step(self, g: Geometry, sino, f, v, alpha, epsilon, iter_image):

@@ -1214,6 +1214,15 @@ int mi_cone_at_reconstruct(mi_context *ctx, const mi_cone_geometry *g, const mi_
     });
 }
 
+int mi_cone_at_reconstruct_dev(mi_context *ctx, const mi_cone_geometry *g, const mi_at_params *p,
+                               const float *sino, float *u, float *v, mi_report *rep) {
+    return guarded([&] {
+        check_ctx(ctx);
+        MI_REQUIRE(g && p, MI_E_ARG, "null argument");
+        cone_at_solve(ctx, *g, *p, sino, u, v, rep, true);
+    });
+}
+
 int mi_cone_config_parse(const char *json, mi_cone_geometry *g, mi_at_params *p) {
     return guarded([&] {
         MI_REQUIRE(json && g && p, MI_E_ARG, "null argument");

@@ -52,10 +52,10 @@ size_t cone_fp_views(hipStream_t st, const ConeTables &t, const int *views, int 
 // Gather back projection over the view list (overwrites vol).
 void cone_bp_views(hipStream_t st, const ConeTables &t, const int *views, int nv, const float *sino, float *vol);
 
-// Ordered-subset AT reconstruction of the volume (cone_at.hip): host sino,
-// u and v (initial values in, results out).
+// Ordered-subset AT reconstruction of the volume (cone_at.hip): sino, u and
+// v (initial values in, results out) on the host, or on the device when dev.
 void cone_at_solve(mi_context *ctx, const mi_cone_geometry &g, const mi_at_params &p, const float *sino,
-                   float *u, float *v, mi_report *rep);
+                   float *u, float *v, mi_report *rep, bool dev = false);
 
 // Traced entries of all rays (counted on the device once per geometry).
 int64_t cone_nnz(hipStream_t st, ConeTables &t);

@@ -233,7 +233,7 @@ __global__ void k_cat_zero(double *slot) {
 }  // namespace
 
 void cone_at_solve(mi_context *ctx, const mi_cone_geometry &g, const mi_at_params &p, const float *sino_h,
-                   float *u_h, float *v_h, mi_report *rep) {
+                   float *u_h, float *v_h, mi_report *rep, bool dev) {
     MI_REQUIRE(sino_h && u_h && v_h, MI_E_ARG, "null argument");
     MI_REQUIRE(p.precision == MI_FP32, MI_E_ARG, "the cone-beam solver runs in FP32 (FP64 reductions)");
     MI_REQUIRE(p.n_subsets >= 1 && p.n_subsets <= g.n_views && p.max_iters >= 0 && p.iter_edge >= 0, MI_E_ARG,
@@ -260,13 +260,20 @@ void cone_at_solve(mi_context *ctx, const mi_cone_geometry &g, const mi_at_param
 
     const size_t vblocks = (nvox + kBlock - 1) / kBlock;
     const size_t fpb = (size_t)((g.n_cols + 15) / 16) * ((g.n_rows + 15) / 16) * nv_max;
-    DevBuf<float> sino(nsino), q(nsino), u(nvox), v(nvox), b(nvox), d(nvox), nn(nvox), dv(nvox);
+    // dev: the caller's device buffers are used in place; otherwise host
+    // copies in device buffers of the solve.
+    DevBuf<float> sino(dev ? 0 : nsino), u(dev ? 0 : nvox), v(dev ? 0 : nvox);
+    DevBuf<float> q(nsino), b(nvox), d(nvox), nn(nvox), dv(nvox);
+    const float *SN = dev ? sino_h : sino.p;
+    float *U = dev ? u_h : u.p, *V = dev ? v_h : v.p;
     DevBuf<double> part0(std::max(vblocks, fpb)), part1(vblocks), sc(8);
     DevBuf<int> dlists(lists.size());
     MI_HIP(hipMemcpyAsync(dlists.p, lists.data(), lists.size() * sizeof(int), hipMemcpyHostToDevice, st));
-    MI_HIP(hipMemcpyAsync(sino.p, sino_h, sino.bytes(), hipMemcpyHostToDevice, st));
-    MI_HIP(hipMemcpyAsync(u.p, u_h, u.bytes(), hipMemcpyHostToDevice, st));
-    MI_HIP(hipMemcpyAsync(v.p, v_h, v.bytes(), hipMemcpyHostToDevice, st));
+    if (!dev) {
+        MI_HIP(hipMemcpyAsync(sino.p, sino_h, sino.bytes(), hipMemcpyHostToDevice, st));
+        MI_HIP(hipMemcpyAsync(u.p, u_h, u.bytes(), hipMemcpyHostToDevice, st));
+        MI_HIP(hipMemcpyAsync(v.p, v_h, v.bytes(), hipMemcpyHostToDevice, st));
+    }
 
     const float eta = (float)(0.01 * p.epsilon * p.epsilon);
     const bool timed_fp = ctx->stats.on && ((ctx->stats.mask >> MI_K_CONE_FP) & 1u);
@@ -292,14 +299,14 @@ void cone_at_solve(mi_context *ctx, const mi_cone_geometry &g, const mi_at_param
             size_t nb;
             {
                 LaunchTimer lt(ctx, MI_K_CONE_FP, timed_fp ? frac * cone_fp_bytes(t) : 0.0, true);
-                nb = cone_fp_views(st, t, views, nv, CFP_RESID, u.p, sino.p, q.p, part0.p);
+                nb = cone_fp_views(st, t, views, nv, CFP_RESID, U, SN, q.p, part0.p);
             }
             reduce(part0.p, nb, 0, 1);
             {
                 LaunchTimer lt(ctx, MI_K_CONE_BP, timed_bp ? frac * cone_bp_bytes(t) : 0.0, true);
                 cone_bp_views(st, t, views, nv, q.p, b.p);
             }
-            hipLaunchKernelGGL(k_cat_grad, vg, blk, 0, st, dm, b.p, u.p, v.p, (float)S, (float)p.alpha, eta, d.p,
+            hipLaunchKernelGGL(k_cat_grad, vg, blk, 0, st, dm, b.p, U, V, (float)S, (float)p.alpha, eta, d.p,
                                part1.p);
             reduce(part1.p, vblocks, 1, 0);
             {
@@ -307,21 +314,21 @@ void cone_at_solve(mi_context *ctx, const mi_cone_geometry &g, const mi_at_param
                 nb = cone_fp_views(st, t, views, nv, CFP_SQ, d.p, nullptr, nullptr, part0.p);
             }
             reduce(part0.p, nb, 2, 0);
-            hipLaunchKernelGGL(k_cat_pairs<true>, vg, blk, 0, st, dm, d.p, v.p, eta, part1.p);
+            hipLaunchKernelGGL(k_cat_pairs<true>, vg, blk, 0, st, dm, d.p, V, eta, part1.p);
             reduce(part1.p, vblocks, 3, 0);
-            hipLaunchKernelGGL(k_cat_update, vg, blk, 0, st, nvox, u.p, d.p, sc.p, (double)S, p.alpha, p.lower,
+            hipLaunchKernelGGL(k_cat_update, vg, blk, 0, st, nvox, U, d.p, sc.p, (double)S, p.alpha, p.lower,
                                p.upper);
         }
         if (p.iter_edge > 0) {
-            hipLaunchKernelGGL(k_cat_edge_n, vg, blk, 0, st, dm, u.p, nn.p);
+            hipLaunchKernelGGL(k_cat_edge_n, vg, blk, 0, st, dm, U, nn.p);
             for (int e = 0; e < p.iter_edge; ++e) {
-                hipLaunchKernelGGL(k_cat_edge_dir, vg, blk, 0, st, dm, v.p, nn.p, (float)p.alpha, (float)p.beta,
+                hipLaunchKernelGGL(k_cat_edge_dir, vg, blk, 0, st, dm, V, nn.p, (float)p.alpha, (float)p.beta,
                                    (float)p.epsilon, dv.p, part0.p, part1.p);
                 reduce(part0.p, vblocks, 4, 0);
                 reduce(part1.p, vblocks, 5, 0);
                 hipLaunchKernelGGL(k_cat_pairs<false>, vg, blk, 0, st, dm, dv.p, nullptr, 0.0f, part0.p);
                 reduce(part0.p, vblocks, 6, 0);
-                hipLaunchKernelGGL(k_cat_edge_update, vg, blk, 0, st, nvox, v.p, dv.p, sc.p, p.alpha, p.beta,
+                hipLaunchKernelGGL(k_cat_edge_update, vg, blk, 0, st, nvox, V, dv.p, sc.p, p.alpha, p.beta,
                                    p.epsilon);
             }
         }
@@ -344,8 +351,10 @@ void cone_at_solve(mi_context *ctx, const mi_cone_geometry &g, const mi_at_param
     MI_HIP(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    MI_HIP(hipMemcpyAsync(u_h, u.p, u.bytes(), hipMemcpyDeviceToHost, st));
-    MI_HIP(hipMemcpyAsync(v_h, v.p, v.bytes(), hipMemcpyDeviceToHost, st));
+    if (!dev) {
+        MI_HIP(hipMemcpyAsync(u_h, u.p, u.bytes(), hipMemcpyDeviceToHost, st));
+        MI_HIP(hipMemcpyAsync(v_h, v.p, v.bytes(), hipMemcpyDeviceToHost, st));
+    }
     MI_HIP(hipStreamSynchronize(st));
     stats_collect(ctx);
     if (rep) {
