@@ -19,8 +19,6 @@
 //
 // Every sum that sets a step is a double partial per workgroup reduced in a
 // fixed order (results do not depend on timing); the volumes are float.
-// One thread per voxel in the stencil kernels: their traffic (a few volume
-// reads) is small next to the projections.
 #include "cone.hpp"
 
 #include <cmath>
@@ -53,19 +51,6 @@ __device__ __forceinline__ bool in3(const Dims &d, int z, int x, int y) {
     return (unsigned)z < (unsigned)d.nz && (unsigned)x < (unsigned)d.nx && (unsigned)y < (unsigned)d.ny;
 }
 
-__device__ __forceinline__ float at3(const float *f, const Dims &d, int z, int x, int y, float pad) {
-    return in3(d, z, x, y) ? f[((size_t)z * d.nx + x) * d.ny + y] : pad;
-}
-
-__device__ __forceinline__ bool voxel(const Dims &d, size_t &i, int &z, int &x, int &y) {
-    i = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    const size_t n = (size_t)d.nz * d.nx * d.ny;
-    y = (int)(i % d.ny);
-    x = (int)((i / d.ny) % d.nx);
-    z = (int)(i / ((size_t)d.ny * d.nx));
-    return i < n;
-}
-
 __device__ __forceinline__ double block_sum(double x) {
     __shared__ double red[kBlock / kWave];
     for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
@@ -74,32 +59,157 @@ __device__ __forceinline__ double block_sum(double x) {
     return (red[0] + red[1]) + (red[2] + red[3]);
 }
 
+// The stencil kernels march over z.  A workgroup owns a tile of kTx x kTy
+// columns (one wave per x row, one lane per y) over up to kZrun slices; each
+// plane of the tile and its one-voxel halo is staged once through LDS, and a
+// lane keeps the 3 x 3 neighbourhood of the last planes in registers
+// (rotating the arrays, no copies), so a volume is read about once per
+// kernel.  A lane accumulates its column's double sums over the slices and
+// the workgroup writes one partial.
+constexpr int kTy = kWave, kTx = kBlock / kWave, kPy = kTy + 2, kPlane = (kTx + 2) * kPy, kZrun = 32, kRun = 64;
+static_assert(kPlane > kBlock && kPlane <= 2 * kBlock, "a lane stages one or two entries of a plane");
+
+// Workgroups of the stencil kernels for the volume (host side).
+size_t tile_blocks(const Dims &d) {
+    return (size_t)((d.ny + kTy - 1) / kTy) * ((d.nx + kTx - 1) / kTx) * ((d.nz + kZrun - 1) / kZrun);
+}
+
+struct Tile {
+    int x0, y0, z0, z1;  // tile origin; slices [z0, z1)
+    int x, y;            // this lane's column
+    bool in;             // the column lies in the volume
+    int e1;              // second LDS entry this lane stages (-1: none; the first is threadIdx.x)
+    int off0, off1;      // in-plane offsets x * ny + y of the two entries (-1: outside the volume)
+};
+
+__device__ __forceinline__ int plane_off(const Dims &d, const Tile &t, int e) {
+    const int r = e / kPy, c = e - r * kPy, x = t.x0 + r - 1, y = t.y0 + c - 1;
+    return ((unsigned)x < (unsigned)d.nx && (unsigned)y < (unsigned)d.ny) ? x * d.ny + y : -1;
+}
+
+__device__ __forceinline__ Tile tile_of(const Dims &d) {
+    const int nty = (d.ny + kTy - 1) / kTy, ntx = (d.nx + kTx - 1) / kTx;
+    // tiles are numbered y fastest, then x, then slices; workgroups go to the
+    // 8 XCDs round-robin, and are dealt to them in runs of kRun consecutive
+    // tiles, so that neighbouring tiles, which share halo lines, mostly run in
+    // one L2 while all XCDs work on the same slices
+    int b = blockIdx.x;
+    if (b < (int)(gridDim.x & ~(8u * kRun - 1))) {
+        const int xcd = b & 7, j = b >> 3;
+        b = ((j / kRun) * 8 + xcd) * kRun + j % kRun;
+    }
+    const int by = b % nty, bx = (b / nty) % ntx, bz = b / (nty * ntx);
+    Tile t;
+    t.y0 = by * kTy;
+    t.x0 = bx * kTx;
+    t.z0 = bz * kZrun;
+    t.z1 = min(t.z0 + kZrun, d.nz);
+    t.y = t.y0 + (int)(threadIdx.x % kTy);
+    t.x = t.x0 + (int)(threadIdx.x / kTy);
+    t.in = t.x < d.nx && t.y < d.ny;
+    t.e1 = (int)threadIdx.x + kBlock < kPlane ? (int)threadIdx.x + kBlock : -1;
+    t.off0 = plane_off(d, t, threadIdx.x);
+    t.off1 = t.e1 >= 0 ? plane_off(d, t, t.e1) : -1;
+    return t;
+}
+
+// The two entries of a plane of the tile that a lane stages, fetched one
+// slice ahead of their use so that their latency overlaps a slice of work.
+struct Ahead {
+    float a, b;
+};
+
+// f(z, ...) at the lane's staged entries, pad outside the volume.
+__device__ __forceinline__ Ahead fetch(const float *__restrict__ f, const Dims &d, const Tile &t, int z, float pad) {
+    const bool zin = (unsigned)z < (unsigned)d.nz;
+    const int zo = zin ? z * d.nx * d.ny : 0;
+    Ahead r;
+    r.a = (zin && t.off0 >= 0) ? f[zo + t.off0] : pad;
+    r.b = (zin && t.off1 >= 0) ? f[zo + t.off1] : pad;
+    return r;
+}
+
+// slot[r][c] = the fetched f(z, x0 + r - 1, y0 + c - 1); SQ stores f^2 + eta.
+template <bool SQ>
+__device__ __forceinline__ void put(float *slot, const Tile &t, const Ahead &r, float eta) {
+    slot[threadIdx.x] = SQ ? r.a * r.a + eta : r.a;
+    if (t.e1 >= 0) slot[t.e1] = SQ ? r.b * r.b + eta : r.b;
+}
+
+// n[b + 1][c + 1] = the staged value at (x + b, y + c) of this lane's column.
+__device__ __forceinline__ void nbhd(float (&n)[3][3], const float *slot) {
+    const float *p = slot + (int)(threadIdx.x / kTy + 1) * kPy + (int)(threadIdx.x % kTy) + 1;
+#pragma unroll
+    for (int b = -1; b <= 1; ++b)
+#pragma unroll
+        for (int c = -1; c <= 1; ++c) n[b + 1][c + 1] = p[b * kPy + c];
+}
+
+__device__ __forceinline__ int vidx(const Dims &d, const Tile &t, int z) { return (z * d.nx + t.x) * d.ny + t.y; }
+
+using Nb = float[3][3];
+
 // d = S b + alpha r(u; v); part = sum d^2.
 __global__ __launch_bounds__(kBlock) void k_cat_grad(Dims dm, const float *__restrict__ b, const float *__restrict__ u,
                                                      const float *__restrict__ v, float S, float alpha, float eta,
                                                      float *__restrict__ dout, double *__restrict__ part) {
-    size_t i;
-    int z, x, y;
-    const bool ok = voxel(dm, i, z, x, y);
+    __shared__ float lu[4][kPlane], ls[4][kPlane];  // plane p of u, v^2 + eta in slot (p + 1) & 3
+    const Tile t = tile_of(dm);
+    put<false>(lu[t.z0 & 3], t, fetch(u, dm, t, t.z0 - 1, 0.0f), eta);
+    put<false>(lu[(t.z0 + 1) & 3], t, fetch(u, dm, t, t.z0, 0.0f), eta);
+    put<true>(ls[t.z0 & 3], t, fetch(v, dm, t, t.z0 - 1, 1.0f), eta);
+    Ahead au = fetch(u, dm, t, t.z0 + 1, 0.0f), av = fetch(v, dm, t, t.z0, 1.0f);
+    float ab = t.in ? b[vidx(dm, t, t.z0)] : 0.0f;
+    __syncthreads();
+    Nb U0, U1, U2, S0, S1, S2;
+    nbhd(U0, lu[t.z0 & 3]);
+    nbhd(U1, lu[(t.z0 + 1) & 3]);
+    nbhd(S0, ls[t.z0 & 3]);
     double sq = 0.0;
-    if (ok) {
-        const float f0 = u[i], v0 = v[i], s0 = v0 * v0 + eta;
-        float acc = 0.0f;
-#pragma unroll
-        for (int k = 0; k < 13; ++k) {
-            int a, bb, c;
-            koff(k, a, bb, c);
-            const float fp = at3(u, dm, z + a, x + bb, y + c, 0.0f);
-            const float fm = at3(u, dm, z - a, x - bb, y - c, 0.0f);
-            const float vm = at3(v, dm, z - a, x - bb, y - c, 1.0f);
-            acc += kwf(a, bb, c) * (s0 * (fp - f0) - (vm * vm + eta) * (f0 - fm));
+    // slice z from u on z - 1, z, z + 1 and v^2 + eta on z - 1, z; stages
+    // u(z + 1) into up and s(z) into sc, fetches the next slice's entries
+    auto step = [&](int z, const Nb &um, const Nb &uc, Nb &up, const Nb &sm, Nb &sc) {
+        put<false>(lu[(z + 2) & 3], t, au, eta);
+        put<true>(ls[(z + 1) & 3], t, av, eta);
+        const float bz = ab;
+        if (z + 1 < t.z1) {
+            au = fetch(u, dm, t, z + 2, 0.0f);
+            av = fetch(v, dm, t, z + 1, 1.0f);
+            ab = t.in ? b[vidx(dm, t, z + 1)] : 0.0f;
         }
-        const float dd = S * b[i] + alpha * acc;
-        dout[i] = dd;
-        sq = (double)dd * (double)dd;
+        __syncthreads();
+        nbhd(up, lu[(z + 2) & 3]);
+        nbhd(sc, ls[(z + 1) & 3]);
+        if (t.in) {
+            const float f0 = uc[1][1], s0 = sc[1][1];
+            // per weight class |k|^2 = 1, 2, 3: sum of f(x + k) - f(x) and of
+            // s(x - k) (f(x) - f(x - k))
+            float A[3] = {0.0f, 0.0f, 0.0f}, B[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int k = 0; k < 13; ++k) {
+                int a, bb, c;
+                koff(k, a, bb, c);
+                const int n = a * a + bb * bb + c * c - 1;
+                const float fp = a ? up[1 + bb][1 + c] : uc[1 + bb][1 + c];
+                const float fm = a ? um[1 - bb][1 - c] : uc[1 - bb][1 - c];
+                const float sk = a ? sm[1 - bb][1 - c] : sc[1 - bb][1 - c];
+                A[n] += fp - f0;
+                B[n] = fmaf(sk, f0 - fm, B[n]);
+            }
+            const float acc = (fmaf(s0, A[0], -B[0]) + 0.5f * fmaf(s0, A[1], -B[1])) +
+                              (1.0f / 3.0f) * fmaf(s0, A[2], -B[2]);
+            const float dd = S * bz + alpha * acc;
+            dout[vidx(dm, t, z)] = dd;
+            sq += (double)dd * (double)dd;
+        }
+    };
+    for (int z = t.z0; z < t.z1; z += 3) {
+        step(z, U0, U1, U2, S0, S1);
+        if (z + 1 < t.z1) step(z + 1, U1, U2, U0, S1, S2);
+        if (z + 2 < t.z1) step(z + 2, U2, U0, U1, S2, S0);
     }
-    const double t = block_sum(sq);
-    if (threadIdx.x == 0) part[blockIdx.x] = t;
+    const double tot = block_sum(sq);
+    if (threadIdx.x == 0) part[blockIdx.x] = tot;
 }
 
 // part = pair sum of p (zero outside) with weights (v^2 + eta) when WV, 1
@@ -108,25 +218,56 @@ template <bool WV>
 __global__ __launch_bounds__(kBlock) void k_cat_pairs(Dims dm, const float *__restrict__ p,
                                                       const float *__restrict__ v, float eta,
                                                       double *__restrict__ part) {
-    size_t i;
-    int z, x, y;
-    const bool ok = voxel(dm, i, z, x, y);
+    __shared__ float lp[2][kPlane];  // plane z in slot z & 1
+    const Tile t = tile_of(dm);
+    put<false>(lp[t.z0 & 1], t, fetch(p, dm, t, t.z0, 0.0f), 0.0f);
+    Ahead ap = fetch(p, dm, t, t.z0 + 1, 0.0f);
+    float av = WV && t.in ? v[vidx(dm, t, t.z0)] : 0.0f;
+    __syncthreads();
+    Nb P0, P1;
+    nbhd(P0, lp[t.z0 & 1]);
     double acc = 0.0;
-    if (ok) {
-        const float p0 = p[i];
-        const double s0 = WV ? (double)v[i] * v[i] + eta : 1.0, so = WV ? 1.0 + eta : 1.0;
-#pragma unroll
-        for (int k = 0; k < 13; ++k) {
-            int a, b, c;
-            koff(k, a, b, c);
-            const double w = kwf(a, b, c);
-            const double q = (double)at3(p, dm, z + a, x + b, y + c, 0.0f) - (double)p0;
-            acc += w * s0 * q * q;
-            if (!in3(dm, z - a, x - b, y - c)) acc += w * so * (double)p0 * p0;
+    // slice z from p on z, z + 1; stages p(z + 1) into pn, fetches the next
+    // slice's entries
+    auto step = [&](int z, const Nb &pc, Nb &pn) {
+        put<false>(lp[(z + 1) & 1], t, ap, 0.0f);
+        const float vz = av;
+        if (z + 1 < t.z1) {
+            ap = fetch(p, dm, t, z + 2, 0.0f);
+            if (WV) av = t.in ? v[vidx(dm, t, z + 1)] : 0.0f;
         }
+        __syncthreads();
+        nbhd(pn, lp[(z + 1) & 1]);
+        if (t.in) {
+            const double p0 = pc[1][1];
+            const double s0 = WV ? (double)vz * vz + eta : 1.0, so = WV ? 1.0 + eta : 1.0;
+            // per weight class |k|^2 = 1, 2, 3: the sum of (p(x + k) - p(x))^2
+            // and the count of offsets whose backward neighbour x - k lies
+            // outside (a = 1 leaves through z = 0, b = +-1 through the x
+            // faces, c = +-1 through the y faces)
+            const bool zlo = z == 0, xlo = t.x == 0, xhi = t.x == dm.nx - 1, ylo = t.y == 0, yhi = t.y == dm.ny - 1;
+            double g[3] = {0.0, 0.0, 0.0};
+            int out[3] = {0, 0, 0};
+#pragma unroll
+            for (int k = 0; k < 13; ++k) {
+                int a, b, c;
+                koff(k, a, b, c);
+                const int n = a * a + b * b + c * c - 1;
+                const double q = (double)(a ? pn[1 + b][1 + c] : pc[1 + b][1 + c]) - p0;
+                g[n] = fma(q, q, g[n]);
+                out[n] += (a == 1 && zlo) || (b == 1 && xlo) || (b == -1 && xhi) || (c == 1 && ylo) ||
+                          (c == -1 && yhi);
+            }
+            acc += s0 * ((g[0] + 0.5 * g[1]) + (1.0 / 3.0) * g[2]) +
+                   so * (p0 * p0) * ((out[0] + 0.5 * out[1]) + (1.0 / 3.0) * out[2]);
+        }
+    };
+    for (int z = t.z0; z < t.z1; z += 2) {
+        step(z, P0, P1);
+        if (z + 1 < t.z1) step(z + 1, P1, P0);
     }
-    const double t = block_sum(acc);
-    if (threadIdx.x == 0) part[blockIdx.x] = t;
+    const double tot = block_sum(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = tot;
 }
 
 // u = clamp(u + c d), c = sc[1] / (S sc[2] + alpha sc[3]).
@@ -145,19 +286,37 @@ __global__ __launch_bounds__(kBlock) void k_cat_update(size_t n, float *__restri
 
 // n(x) = sum_k w_k (u(x + k) - u(x))^2.
 __global__ __launch_bounds__(kBlock) void k_cat_edge_n(Dims dm, const float *__restrict__ u, float *__restrict__ nn) {
-    size_t i;
-    int z, x, y;
-    if (!voxel(dm, i, z, x, y)) return;
-    const float f0 = u[i];
-    float acc = 0.0f;
+    __shared__ float lu[2][kPlane];  // plane z in slot z & 1
+    const Tile t = tile_of(dm);
+    put<false>(lu[t.z0 & 1], t, fetch(u, dm, t, t.z0, 0.0f), 0.0f);
+    Ahead au = fetch(u, dm, t, t.z0 + 1, 0.0f);
+    __syncthreads();
+    Nb U0, U1;
+    nbhd(U0, lu[t.z0 & 1]);
+    // slice z from u on z, z + 1; stages u(z + 1) into un, fetches the next
+    // slice's entries
+    auto step = [&](int z, const Nb &uc, Nb &un) {
+        put<false>(lu[(z + 1) & 1], t, au, 0.0f);
+        if (z + 1 < t.z1) au = fetch(u, dm, t, z + 2, 0.0f);
+        __syncthreads();
+        nbhd(un, lu[(z + 1) & 1]);
+        if (t.in) {
+            const float f0 = uc[1][1];
+            float g[3] = {0.0f, 0.0f, 0.0f};  // per weight class |k|^2 = 1, 2, 3
 #pragma unroll
-    for (int k = 0; k < 13; ++k) {
-        int a, b, c;
-        koff(k, a, b, c);
-        const float q = at3(u, dm, z + a, x + b, y + c, 0.0f) - f0;
-        acc += kwf(a, b, c) * q * q;
+            for (int k = 0; k < 13; ++k) {
+                int a, b, c;
+                koff(k, a, b, c);
+                const float q = (a ? un[1 + b][1 + c] : uc[1 + b][1 + c]) - f0;
+                g[a * a + b * b + c * c - 1] = fmaf(q, q, g[a * a + b * b + c * c - 1]);
+            }
+            nn[vidx(dm, t, z)] = (g[0] + 0.5f * g[1]) + (1.0f / 3.0f) * g[2];
+        }
+    };
+    for (int z = t.z0; z < t.z1; z += 2) {
+        step(z, U0, U1);
+        if (z + 1 < t.z1) step(z + 1, U1, U0);
     }
-    nn[i] = acc;
 }
 
 // dv = -alpha v n + beta eps lap(v) - beta / (4 eps) (v - 1);
@@ -166,30 +325,55 @@ __global__ __launch_bounds__(kBlock) void k_cat_edge_dir(Dims dm, const float *_
                                                          const float *__restrict__ nn, float alpha, float beta,
                                                          float eps, float *__restrict__ dv, double *__restrict__ p0,
                                                          double *__restrict__ p1) {
-    size_t i;
-    int z, x, y;
-    const bool ok = voxel(dm, i, z, x, y);
-    double a0 = 0.0, a1 = 0.0;
-    if (ok) {
-        const float v0 = v[i];
-        float lap = 0.0f;
-#pragma unroll
-        for (int k = 0; k < 13; ++k) {
-            int a, b, c;
-            koff(k, a, b, c);
-            lap += kwf(a, b, c) * ((at3(v, dm, z + a, x + b, y + c, 1.0f) - v0) -
-                                   (v0 - at3(v, dm, z - a, x - b, y - c, 1.0f)));
-        }
-        const float d = -alpha * v0 * nn[i] + beta * eps * lap - beta / (4.0f * eps) * (v0 - 1.0f);
-        dv[i] = d;
-        a0 = (double)d * d;
-        a1 = (double)nn[i] * d * d;
-    }
-    double t = block_sum(a0);
-    if (threadIdx.x == 0) p0[blockIdx.x] = t;
+    __shared__ float lv[4][kPlane];  // plane p in slot (p + 1) & 3
+    const Tile t = tile_of(dm);
+    put<false>(lv[t.z0 & 3], t, fetch(v, dm, t, t.z0 - 1, 1.0f), 0.0f);
+    put<false>(lv[(t.z0 + 1) & 3], t, fetch(v, dm, t, t.z0, 1.0f), 0.0f);
+    Ahead av = fetch(v, dm, t, t.z0 + 1, 1.0f);
+    float an = t.in ? nn[vidx(dm, t, t.z0)] : 0.0f;
     __syncthreads();
-    t = block_sum(a1);
-    if (threadIdx.x == 0) p1[blockIdx.x] = t;
+    Nb V0, V1, V2;
+    nbhd(V0, lv[t.z0 & 3]);
+    nbhd(V1, lv[(t.z0 + 1) & 3]);
+    double a0 = 0.0, a1 = 0.0;
+    // slice z from v on z - 1, z, z + 1; stages v(z + 1) into vp, fetches
+    // the next slice's entries
+    auto step = [&](int z, const Nb &vm, const Nb &vc, Nb &vp) {
+        put<false>(lv[(z + 2) & 3], t, av, 0.0f);
+        const float n = an;
+        if (z + 1 < t.z1) {
+            av = fetch(v, dm, t, z + 2, 1.0f);
+            an = t.in ? nn[vidx(dm, t, z + 1)] : 0.0f;
+        }
+        __syncthreads();
+        nbhd(vp, lv[(z + 2) & 3]);
+        if (t.in) {
+            const float v0 = vc[1][1];
+            float lap = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 13; ++k) {
+                int a, b, c;
+                koff(k, a, b, c);
+                const float fp = a ? vp[1 + b][1 + c] : vc[1 + b][1 + c];
+                const float fm = a ? vm[1 - b][1 - c] : vc[1 - b][1 - c];
+                lap += kwf(a, b, c) * ((fp - v0) - (v0 - fm));
+            }
+            const float d = -alpha * v0 * n + beta * eps * lap - beta / (4.0f * eps) * (v0 - 1.0f);
+            dv[vidx(dm, t, z)] = d;
+            a0 += (double)d * d;
+            a1 += (double)n * d * d;
+        }
+    };
+    for (int z = t.z0; z < t.z1; z += 3) {
+        step(z, V0, V1, V2);
+        if (z + 1 < t.z1) step(z + 1, V1, V2, V0);
+        if (z + 2 < t.z1) step(z + 2, V2, V0, V1);
+    }
+    double s = block_sum(a0);
+    if (threadIdx.x == 0) p0[blockIdx.x] = s;
+    __syncthreads();
+    s = block_sum(a1);
+    if (threadIdx.x == 0) p1[blockIdx.x] = s;
 }
 
 // v = clamp(v + c dv, 0, 1), c = sc[4] / (alpha sc[5] + beta eps sc[6] + beta / (4 eps) sc[4]).
@@ -258,7 +442,7 @@ void cone_at_solve(mi_context *ctx, const mi_cone_geometry &g, const mi_at_param
     int nv_max = 0;
     for (int s = 0; s < S; ++s) nv_max = std::max(nv_max, off[s + 1] - off[s]);
 
-    const size_t vblocks = (nvox + kBlock - 1) / kBlock;
+    const size_t vblocks = (nvox + kBlock - 1) / kBlock, tblocks = tile_blocks(dm);
     const size_t fpb = (size_t)((g.n_cols + 15) / 16) * ((g.n_rows + 15) / 16) * nv_max;
     // dev: the caller's device buffers are used in place; otherwise host
     // copies in device buffers of the solve.
@@ -266,7 +450,7 @@ void cone_at_solve(mi_context *ctx, const mi_cone_geometry &g, const mi_at_param
     DevBuf<float> q(nsino), b(nvox), d(nvox), nn(nvox), dv(nvox);
     const float *SN = dev ? sino_h : sino.p;
     float *U = dev ? u_h : u.p, *V = dev ? v_h : v.p;
-    DevBuf<double> part0(std::max(vblocks, fpb)), part1(vblocks), sc(8);
+    DevBuf<double> part0(std::max(tblocks, fpb)), part1(tblocks), sc(8);
     DevBuf<int> dlists(lists.size());
     MI_HIP(hipMemcpyAsync(dlists.p, lists.data(), lists.size() * sizeof(int), hipMemcpyHostToDevice, st));
     if (!dev) {
@@ -279,7 +463,7 @@ void cone_at_solve(mi_context *ctx, const mi_cone_geometry &g, const mi_at_param
     const bool timed_fp = ctx->stats.on && ((ctx->stats.mask >> MI_K_CONE_FP) & 1u);
     const bool timed_bp = ctx->stats.on && ((ctx->stats.mask >> MI_K_CONE_BP) & 1u);
     if (timed_fp || timed_bp) cone_nnz(st, t);
-    const dim3 vg((unsigned)vblocks), blk(kBlock);
+    const dim3 vg((unsigned)vblocks), tg((unsigned)tblocks), blk(kBlock);
     auto reduce = [&](const double *part, size_t n, int slot, int acc) {
         hipLaunchKernelGGL(k_cat_reduce, dim3(1), blk, 0, st, part, n, sc.p + slot, acc);
     };
@@ -306,28 +490,28 @@ void cone_at_solve(mi_context *ctx, const mi_cone_geometry &g, const mi_at_param
                 LaunchTimer lt(ctx, MI_K_CONE_BP, timed_bp ? frac * cone_bp_bytes(t) : 0.0, true);
                 cone_bp_views(st, t, views, nv, q.p, b.p);
             }
-            hipLaunchKernelGGL(k_cat_grad, vg, blk, 0, st, dm, b.p, U, V, (float)S, (float)p.alpha, eta, d.p,
+            hipLaunchKernelGGL(k_cat_grad, tg, blk, 0, st, dm, b.p, U, V, (float)S, (float)p.alpha, eta, d.p,
                                part1.p);
-            reduce(part1.p, vblocks, 1, 0);
+            reduce(part1.p, tblocks, 1, 0);
             {
                 LaunchTimer lt(ctx, MI_K_CONE_FP, timed_fp ? frac * cone_fp_bytes(t) : 0.0, true);
                 nb = cone_fp_views(st, t, views, nv, CFP_SQ, d.p, nullptr, nullptr, part0.p);
             }
             reduce(part0.p, nb, 2, 0);
-            hipLaunchKernelGGL(k_cat_pairs<true>, vg, blk, 0, st, dm, d.p, V, eta, part1.p);
-            reduce(part1.p, vblocks, 3, 0);
+            hipLaunchKernelGGL(k_cat_pairs<true>, tg, blk, 0, st, dm, d.p, V, eta, part1.p);
+            reduce(part1.p, tblocks, 3, 0);
             hipLaunchKernelGGL(k_cat_update, vg, blk, 0, st, nvox, U, d.p, sc.p, (double)S, p.alpha, p.lower,
                                p.upper);
         }
         if (p.iter_edge > 0) {
-            hipLaunchKernelGGL(k_cat_edge_n, vg, blk, 0, st, dm, U, nn.p);
+            hipLaunchKernelGGL(k_cat_edge_n, tg, blk, 0, st, dm, U, nn.p);
             for (int e = 0; e < p.iter_edge; ++e) {
-                hipLaunchKernelGGL(k_cat_edge_dir, vg, blk, 0, st, dm, V, nn.p, (float)p.alpha, (float)p.beta,
+                hipLaunchKernelGGL(k_cat_edge_dir, tg, blk, 0, st, dm, V, nn.p, (float)p.alpha, (float)p.beta,
                                    (float)p.epsilon, dv.p, part0.p, part1.p);
-                reduce(part0.p, vblocks, 4, 0);
-                reduce(part1.p, vblocks, 5, 0);
-                hipLaunchKernelGGL(k_cat_pairs<false>, vg, blk, 0, st, dm, dv.p, nullptr, 0.0f, part0.p);
-                reduce(part0.p, vblocks, 6, 0);
+                reduce(part0.p, tblocks, 4, 0);
+                reduce(part1.p, tblocks, 5, 0);
+                hipLaunchKernelGGL(k_cat_pairs<false>, tg, blk, 0, st, dm, dv.p, nullptr, 0.0f, part0.p);
+                reduce(part0.p, tblocks, 6, 0);
                 hipLaunchKernelGGL(k_cat_edge_update, vg, blk, 0, st, nvox, V, dv.p, sc.p, p.alpha, p.beta,
                                    p.epsilon);
             }

@@ -39,6 +39,9 @@ SMALL = [
     # odd sizes: a partial slice chunk (13 = 8 + 5), partial 16 x 16 tiles,
     # an odd panel (HALFDET = 8 of 17 rows), two back-projection launches
     mi.cone_geometry(18, 22, 13, 50, 17, 30, 1.0, 1.7, 45.0, 80.0),
+    # two y tiles (70 > 64), a partial x tile (22 = 5 x 4 + 2) and two runs
+    # of slices (40 > 32) in the volume stencils
+    mi.cone_geometry(22, 70, 40, 24, 40, 80, 1.0, 1.6, 60.0, 110.0),
 ]
 
 
@@ -191,7 +194,7 @@ def _cone_case(g, seed, n_ell=6):
 
 
 @pytest.mark.parametrize("prm", [(1e-3, 1e-4, 1e-2, 5, 3, 1), (0.5, 0.02, 0.05, 3, 2, 1),
-                                 (0.5, 0.02, 0.05, 4, 2, 2)])
+                                 (0.5, 0.02, 0.05, 4, 2, 2), (0.5, 0.02, 0.05, 4, 2, 4)])
 def test_cone_at_parity(ctx, prm):
     """Ordered-subset AT on the volume: FP32 device run against the FP64
     oracle (1e-5 relative, u and v), the data fidelity of the last pass, and
