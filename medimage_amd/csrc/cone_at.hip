@@ -218,48 +218,72 @@ template <bool WV>
 __global__ __launch_bounds__(kBlock) void k_cat_pairs(Dims dm, const float *__restrict__ p,
                                                       const float *__restrict__ v, float eta,
                                                       double *__restrict__ part) {
-    __shared__ float lp[2][kPlane];  // plane z in slot z & 1
+    __shared__ double lp[2][kPlane];  // plane z in slot z & 1, converted to double once
     const Tile t = tile_of(dm);
-    put<false>(lp[t.z0 & 1], t, fetch(p, dm, t, t.z0, 0.0f), 0.0f);
+    auto put_d = [&](double *slot, const Ahead &r) {
+        slot[threadIdx.x] = r.a;
+        if (t.e1 >= 0) slot[t.e1] = r.b;
+    };
+    put_d(lp[t.z0 & 1], fetch(p, dm, t, t.z0, 0.0f));
     Ahead ap = fetch(p, dm, t, t.z0 + 1, 0.0f);
     float av = WV && t.in ? v[vidx(dm, t, t.z0)] : 0.0f;
+    // The pair weights of x whose other voxel x - k lies outside: per weight
+    // class |k|^2 = 1, 2, 3 the count of such offsets (b = +-1 leaves through
+    // the x faces, c = +-1 through the y faces, and on slice 0 a = 1 through
+    // z = 0), summed with the weights; wout[1] on slice 0, wout[0] elsewhere.
+    double wout[2];
+    {
+        const bool xlo = t.x == 0, xhi = t.x == dm.nx - 1, ylo = t.y == 0, yhi = t.y == dm.ny - 1;
+        int c0[3] = {0, 0, 0}, c1[3] = {0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < 13; ++k) {
+            int a, b, c;
+            koff(k, a, b, c);
+            const int n = a * a + b * b + c * c - 1;
+            const bool xy = (b == 1 && xlo) || (b == -1 && xhi) || (c == 1 && ylo) || (c == -1 && yhi);
+            c0[n] += xy;
+            c1[n] += xy || a == 1;
+        }
+        wout[0] = (c0[0] + 0.5 * c0[1]) + (1.0 / 3.0) * c0[2];
+        wout[1] = (c1[0] + 0.5 * c1[1]) + (1.0 / 3.0) * c1[2];
+    }
     __syncthreads();
-    Nb P0, P1;
-    nbhd(P0, lp[t.z0 & 1]);
+    double P0[3][3], P1[3][3];
+    auto nbhd_d = [&](double (&n)[3][3], const double *slot) {
+        const double *q = slot + (int)(threadIdx.x / kTy + 1) * kPy + (int)(threadIdx.x % kTy) + 1;
+#pragma unroll
+        for (int b = -1; b <= 1; ++b)
+#pragma unroll
+            for (int c = -1; c <= 1; ++c) n[b + 1][c + 1] = q[b * kPy + c];
+    };
+    nbhd_d(P0, lp[t.z0 & 1]);
     double acc = 0.0;
     // slice z from p on z, z + 1; stages p(z + 1) into pn, fetches the next
     // slice's entries
-    auto step = [&](int z, const Nb &pc, Nb &pn) {
-        put<false>(lp[(z + 1) & 1], t, ap, 0.0f);
+    auto step = [&](int z, const double (&pc)[3][3], double (&pn)[3][3]) {
+        put_d(lp[(z + 1) & 1], ap);
         const float vz = av;
         if (z + 1 < t.z1) {
             ap = fetch(p, dm, t, z + 2, 0.0f);
             if (WV) av = t.in ? v[vidx(dm, t, z + 1)] : 0.0f;
         }
         __syncthreads();
-        nbhd(pn, lp[(z + 1) & 1]);
+        nbhd_d(pn, lp[(z + 1) & 1]);
         if (t.in) {
             const double p0 = pc[1][1];
             const double s0 = WV ? (double)vz * vz + eta : 1.0, so = WV ? 1.0 + eta : 1.0;
-            // per weight class |k|^2 = 1, 2, 3: the sum of (p(x + k) - p(x))^2
-            // and the count of offsets whose backward neighbour x - k lies
-            // outside (a = 1 leaves through z = 0, b = +-1 through the x
-            // faces, c = +-1 through the y faces)
-            const bool zlo = z == 0, xlo = t.x == 0, xhi = t.x == dm.nx - 1, ylo = t.y == 0, yhi = t.y == dm.ny - 1;
+            // per weight class: the sum of (p(x + k) - p(x))^2, exact
+            // differences of floats in double
             double g[3] = {0.0, 0.0, 0.0};
-            int out[3] = {0, 0, 0};
 #pragma unroll
             for (int k = 0; k < 13; ++k) {
                 int a, b, c;
                 koff(k, a, b, c);
                 const int n = a * a + b * b + c * c - 1;
-                const double q = (double)(a ? pn[1 + b][1 + c] : pc[1 + b][1 + c]) - p0;
+                const double q = (a ? pn[1 + b][1 + c] : pc[1 + b][1 + c]) - p0;
                 g[n] = fma(q, q, g[n]);
-                out[n] += (a == 1 && zlo) || (b == 1 && xlo) || (b == -1 && xhi) || (c == 1 && ylo) ||
-                          (c == -1 && yhi);
             }
-            acc += s0 * ((g[0] + 0.5 * g[1]) + (1.0 / 3.0) * g[2]) +
-                   so * (p0 * p0) * ((out[0] + 0.5 * out[1]) + (1.0 / 3.0) * out[2]);
+            acc += s0 * ((g[0] + 0.5 * g[1]) + (1.0 / 3.0) * g[2]) + so * (p0 * p0) * wout[z == 0 ? 1 : 0];
         }
     };
     for (int z = t.z0; z < t.z1; z += 2) {
