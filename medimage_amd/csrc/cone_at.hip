@@ -294,18 +294,38 @@ __global__ __launch_bounds__(kBlock) void k_cat_pairs(Dims dm, const float *__re
     if (threadIdx.x == 0) part[blockIdx.x] = tot;
 }
 
+__device__ __forceinline__ float clamp_step(float u, float d, double c, double lo, double hi) {
+    double x = (double)u + c * (double)d;
+    x = x < lo ? lo : x;
+    x = x > hi ? hi : x;
+    return (float)x;
+}
+
+// u = clamp(u + c d) on four voxels per lane, with 16-byte accesses when both
+// volumes are 16-byte aligned (vec).
+__device__ __forceinline__ void step4(size_t n, float *__restrict__ u, const float *__restrict__ d, double c,
+                                      double lo, double hi, bool vec) {
+    const size_t i = ((size_t)blockIdx.x * kBlock + threadIdx.x) * 4;
+    if (i >= n) return;
+    if (vec && i + 4 <= n) {
+        float4 a = *reinterpret_cast<const float4 *>(u + i);
+        const float4 b = *reinterpret_cast<const float4 *>(d + i);
+        a.x = clamp_step(a.x, b.x, c, lo, hi);
+        a.y = clamp_step(a.y, b.y, c, lo, hi);
+        a.z = clamp_step(a.z, b.z, c, lo, hi);
+        a.w = clamp_step(a.w, b.w, c, lo, hi);
+        *reinterpret_cast<float4 *>(u + i) = a;
+    } else {
+        for (size_t j = i; j < n && j < i + 4; ++j) u[j] = clamp_step(u[j], d[j], c, lo, hi);
+    }
+}
+
 // u = clamp(u + c d), c = sc[1] / (S sc[2] + alpha sc[3]).
 __global__ __launch_bounds__(kBlock) void k_cat_update(size_t n, float *__restrict__ u, const float *__restrict__ d,
                                                        const double *__restrict__ sc, double S, double alpha,
-                                                       double lo, double hi) {
-    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
+                                                       double lo, double hi, bool vec) {
     const double den = S * sc[2] + alpha * sc[3];
-    const double c = den > 0 ? sc[1] / den : 0.0;
-    double x = (double)u[i] + c * (double)d[i];
-    x = x < lo ? lo : x;
-    x = x > hi ? hi : x;
-    u[i] = (float)x;
+    step4(n, u, d, den > 0 ? sc[1] / den : 0.0, lo, hi, vec);
 }
 
 // n(x) = sum_k w_k (u(x + k) - u(x))^2.
@@ -404,15 +424,9 @@ __global__ __launch_bounds__(kBlock) void k_cat_edge_dir(Dims dm, const float *_
 __global__ __launch_bounds__(kBlock) void k_cat_edge_update(size_t n, float *__restrict__ v,
                                                             const float *__restrict__ dv,
                                                             const double *__restrict__ sc, double alpha, double beta,
-                                                            double eps) {
-    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
+                                                            double eps, bool vec) {
     const double den = alpha * sc[5] + beta * eps * sc[6] + beta / (4.0 * eps) * sc[4];
-    const double c = den > 0 ? sc[4] / den : 0.0;
-    double x = (double)v[i] + c * (double)dv[i];
-    x = x < 0.0 ? 0.0 : x;
-    x = x > 1.0 ? 1.0 : x;
-    v[i] = (float)x;
+    step4(n, v, dv, den > 0 ? sc[4] / den : 0.0, 0.0, 1.0, vec);
 }
 
 // slot = (acc ? slot : 0) + sum part[0 .. n), fixed order, one workgroup.
@@ -466,7 +480,7 @@ void cone_at_solve(mi_context *ctx, const mi_cone_geometry &g, const mi_at_param
     int nv_max = 0;
     for (int s = 0; s < S; ++s) nv_max = std::max(nv_max, off[s + 1] - off[s]);
 
-    const size_t vblocks = (nvox + kBlock - 1) / kBlock, tblocks = tile_blocks(dm);
+    const size_t tblocks = tile_blocks(dm);
     const size_t fpb = (size_t)((g.n_cols + 15) / 16) * ((g.n_rows + 15) / 16) * nv_max;
     // dev: the caller's device buffers are used in place; otherwise host
     // copies in device buffers of the solve.
@@ -487,7 +501,8 @@ void cone_at_solve(mi_context *ctx, const mi_cone_geometry &g, const mi_at_param
     const bool timed_fp = ctx->stats.on && ((ctx->stats.mask >> MI_K_CONE_FP) & 1u);
     const bool timed_bp = ctx->stats.on && ((ctx->stats.mask >> MI_K_CONE_BP) & 1u);
     if (timed_fp || timed_bp) cone_nnz(st, t);
-    const dim3 vg((unsigned)vblocks), tg((unsigned)tblocks), blk(kBlock);
+    const dim3 v4g((unsigned)((nvox + 4 * kBlock - 1) / (4 * kBlock))), tg((unsigned)tblocks), blk(kBlock);
+    const bool vec_u = (((uintptr_t)U | (uintptr_t)d.p) & 15) == 0, vec_v = (((uintptr_t)V | (uintptr_t)dv.p) & 15) == 0;
     auto reduce = [&](const double *part, size_t n, int slot, int acc) {
         hipLaunchKernelGGL(k_cat_reduce, dim3(1), blk, 0, st, part, n, sc.p + slot, acc);
     };
@@ -524,8 +539,8 @@ void cone_at_solve(mi_context *ctx, const mi_cone_geometry &g, const mi_at_param
             reduce(part0.p, nb, 2, 0);
             hipLaunchKernelGGL(k_cat_pairs<true>, tg, blk, 0, st, dm, d.p, V, eta, part1.p);
             reduce(part1.p, tblocks, 3, 0);
-            hipLaunchKernelGGL(k_cat_update, vg, blk, 0, st, nvox, U, d.p, sc.p, (double)S, p.alpha, p.lower,
-                               p.upper);
+            hipLaunchKernelGGL(k_cat_update, v4g, blk, 0, st, nvox, U, d.p, sc.p, (double)S, p.alpha, p.lower,
+                               p.upper, vec_u);
         }
         if (p.iter_edge > 0) {
             hipLaunchKernelGGL(k_cat_edge_n, tg, blk, 0, st, dm, U, nn.p);
@@ -536,8 +551,8 @@ void cone_at_solve(mi_context *ctx, const mi_cone_geometry &g, const mi_at_param
                 reduce(part1.p, tblocks, 5, 0);
                 hipLaunchKernelGGL(k_cat_pairs<false>, tg, blk, 0, st, dm, dv.p, nullptr, 0.0f, part0.p);
                 reduce(part0.p, tblocks, 6, 0);
-                hipLaunchKernelGGL(k_cat_edge_update, vg, blk, 0, st, nvox, V, dv.p, sc.p, p.alpha, p.beta,
-                                   p.epsilon);
+                hipLaunchKernelGGL(k_cat_edge_update, v4g, blk, 0, st, nvox, V, dv.p, sc.p, p.alpha, p.beta,
+                                   p.epsilon, vec_v);
             }
         }
         MI_HIP(hipGetLastError());

@@ -126,13 +126,14 @@ def test_cone_tiny_geometries(ctx, g):
 class _DevBuf:
     """Device buffer through the HIP runtime the library itself uses."""
 
-    def __init__(self, arr):
+    def __init__(self, arr, offset=0):
         import ctypes as C
         self.C = C
         self.hip = C.CDLL("libamdhip64.so.7")
         self.n = arr.nbytes
-        self.ptr = C.c_void_p()
-        assert self.hip.hipMalloc(C.byref(self.ptr), C.c_size_t(self.n)) == 0
+        self.base = C.c_void_p()
+        assert self.hip.hipMalloc(C.byref(self.base), C.c_size_t(self.n + 16)) == 0
+        self.ptr = C.c_void_p(self.base.value + offset)
         assert self.hip.hipMemcpy(self.ptr, arr.ctypes.data_as(C.c_void_p), C.c_size_t(self.n), 1) == 0
 
     def get(self, like):
@@ -141,7 +142,7 @@ class _DevBuf:
         return out
 
     def free(self):
-        self.hip.hipFree(self.ptr)
+        self.hip.hipFree(self.base)
 
 
 def test_cone_device_pointer_entry_points(ctx):
@@ -158,6 +159,29 @@ def test_cone_device_pointer_entry_points(ctx):
     u, v, rep = ctx.cone_at_reconstruct(g, sino, p)
     shape = (g.nz, g.nx, g.ny)
     bs, bu, bv = _DevBuf(sino), _DevBuf(np.zeros(shape, np.float32)), _DevBuf(np.ones(shape, np.float32))
+    rep2 = ctx.cone_at_reconstruct_dev(g, bs.ptr.value, bu.ptr.value, bv.ptr.value, p)
+    assert rep2.iterations == rep.iterations and rep2.data_fidelity == rep.data_fidelity
+    assert np.array_equal(bu.get(u).view(np.uint32), u.view(np.uint32))
+    assert np.array_equal(bv.get(v).view(np.uint32), v.view(np.uint32))
+    for b in (bs, bu, bv):
+        b.free()
+
+
+def test_cone_at_odd_volume_and_unaligned_buffers(ctx):
+    """A volume whose voxel count is not a multiple of 4 (the updates' scalar
+    tail) against the FP64 oracle, and the same solve on device buffers 4
+    bytes past a 16-byte boundary (the updates' scalar path) bitwise equal."""
+    g = mi.cone_geometry(17, 21, 11, 30, 20, 28, 1.0, 2.0, 60.0, 110.0)
+    assert (g.nx * g.ny * g.nz) % 4 == 3
+    vol, sino = _cone_case(g, 5)
+    p = mi.at_params(0.5, 0.02, 0.05, n_subsets=3, max_iters=2)
+    u, v, rep = ctx.cone_at_reconstruct(g, sino, p)
+    shape = (g.nz, g.nx, g.ny)
+    ou, ov, n, _ = O.cone_os_at(O.cone_from(g), sino, np.zeros(shape), np.ones(shape), 0.5, 0.02, 0.05, 3, 2)
+    assert rel_l2(u, ou) < 1e-5 and rel_max(u, ou) < 1e-5
+    assert rel_l2(v, ov) < 1e-5 and rel_max(v, ov) < 1e-5
+    bs = _DevBuf(sino)
+    bu, bv = _DevBuf(np.zeros(shape, np.float32), offset=4), _DevBuf(np.ones(shape, np.float32), offset=4)
     rep2 = ctx.cone_at_reconstruct_dev(g, bs.ptr.value, bu.ptr.value, bv.ptr.value, p)
     assert rep2.iterations == rep.iterations and rep2.data_fidelity == rep.data_fidelity
     assert np.array_equal(bu.get(u).view(np.uint32), u.view(np.uint32))
