@@ -145,6 +145,25 @@ class _DevBuf:
         self.hip.hipFree(self.base)
 
 
+@pytest.mark.parametrize("g", [
+    mi.cone_geometry(5, 7, 1, 3, 2, 9, 1.0, 1.0, 20.0, 40.0),       # one slice, less than a tile
+    mi.cone_geometry(3, 70, 2, 4, 3, 40, 1.0, 1.0, 60.0, 110.0),    # three x rows, two y tiles
+])
+def test_cone_at_tiny_volumes(ctx, g):
+    """The volume solver where the stencil tiles are mostly outside the
+    volume: u and v within 1e-5 of the FP64 oracle."""
+    rng = np.random.default_rng(8)
+    shape = (g.nz, g.nx, g.ny)
+    sino = O.cone_fp(O.cone_from(g), 0.05 * rng.random(shape)).astype(np.float32)
+    p = mi.at_params(0.5, 0.02, 0.05, n_subsets=2, max_iters=2)
+    u, v, rep = ctx.cone_at_reconstruct(g, sino, p)
+    ou, ov, n, _ = O.cone_os_at(O.cone_from(g), sino, np.zeros(shape), np.ones(shape), 0.5, 0.02, 0.05, 2, 2)
+    assert rep.iterations == n == 2
+    assert np.abs(ou).max() > 0 and ov.min() < 1
+    assert rel_l2(u, ou) < 1e-5 and rel_max(u, ou) < 1e-5
+    assert rel_l2(v, ov) < 1e-5 and rel_max(v, ov) < 1e-5
+
+
 def test_cone_device_pointer_entry_points(ctx):
     """The _dev entry points on device buffers give the host ones' results
     bitwise."""
