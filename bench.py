@@ -34,24 +34,76 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "AT iters/sec & rays/sec (512², 720 views) plain vs batched; p50/p99 job latency"
 
 
-def cpu_baseline(sino, passes):
+def host_cpus():
+    """CPUs this process may run on, the machine's count and the CPU model.
+    A GPU host may hand each GPU's job a share of its cores (its affinity
+    mask, OMP_NUM_THREADS); the CPU baseline uses that share."""
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    omp = int(os.environ.get("OMP_NUM_THREADS") or 0)
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"threads": min(allowed, omp) if omp else allowed, "affinity_cpus": allowed,
+            "machine_cpus": os.cpu_count(), "omp_num_threads": omp or None, "model": model}
+
+
+def cpu_baseline(sino, passes, sino_c1=None, passes_c1=0):
     """The CPU port (oracle/cpu_at.c: the same AT iteration with the oracle's
     tracer, rays re-traced in every projection as the reference's OpenMP
     programs do, FP32 bulk, OpenMP over rays) on `passes` AT iterations of
-    the same job, on the host threads this job may use."""
+    the C2 job, on every CPU this process may use; then the C1 job (256^2,
+    360 views, 512 bins, one subset) for `passes_c1` iterations."""
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_py as O
     from medimage_amd import configs
-    threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or min(os.cpu_count() or 1, 16)
+    host = host_cpus()
+    threads = host["threads"]
     g = configs.c2_geometry()
     op = O.op_fan(O.fan_from(g))
     t0 = time.perf_counter()
     O.cpu_os_at_f32(op, sino.astype(np.float32), configs.ALPHA, configs.BETA, configs.EPS, 12, passes,
                     iter_edge=1, threads=threads)
     dt = time.perf_counter() - t0
-    return {"value": passes / dt, "unit": "AT iterations/s", "cores": threads, "kind": "port",
-            "sample": f"{passes} AT iterations of the same 512x512 / 720-view / 736-bin / 12-subset job, "
-                      f"FP32 CPU port with OpenMP over rays, {threads} threads, {dt:.1f} s"}
+    out = {"value": passes / dt, "unit": "AT iterations/s", "cores": threads, "kind": "port",
+           "sample": f"{passes} AT iterations of the same 512x512 / 720-view / 736-bin / 12-subset job, "
+                     f"FP32 CPU port with OpenMP over rays, {threads} threads, {dt:.1f} s",
+           "rays_per_s": passes * g.n_views * g.n_bins / dt, "host": host}
+    if sino_c1 is not None and passes_c1 > 0:
+        g1 = configs.c1_geometry()
+        op1 = O.op_fan(O.fan_from(g1))
+        t0 = time.perf_counter()
+        O.cpu_os_at_f32(op1, sino_c1.astype(np.float32), configs.ALPHA, configs.BETA, configs.EPS, 1, passes_c1,
+                        iter_edge=1, threads=threads)
+        dt1 = time.perf_counter() - t0
+        out["c1"] = {"value": passes_c1 / dt1, "unit": "AT iterations/s",
+                     "rays_per_s": passes_c1 * g1.n_views * g1.n_bins / dt1,
+                     "sample": f"{passes_c1} AT iterations of the C1 job (256x256, 1.0 mm, 360 views, 512 bins, "
+                               f"one subset), {threads} threads, {dt1:.1f} s"}
+    return out
+
+
+def c1_gpu(ctx, iters):
+    """C1 (256^2 / 360 views / 512 bins, 50 AT iterations, one subset) as a
+    plain per-job GPU solve: returns (AT iterations/s, sinogram)."""
+    from medimage_amd import configs
+    g = configs.c1_geometry()
+    _, sino = configs.make_case(ctx, g, 1e5, seed=1)
+    p = configs.c1_params(max_iters=iters)
+    ctx.at_reconstruct(g, sino.astype(np.float32), configs.c1_params(max_iters=2))  # warm (matrix, code)
+    u, v, rep = ctx.at_reconstruct(g, sino.astype(np.float32), p)
+    return {"workload": "C1: 256x256 (1.0 mm), 360 views, 512 bins (1.0 mm), one subset, "
+                        f"{rep.iterations} AT iterations, FP32 (FP64 reductions)",
+            "at_iterations_per_s": rep.iterations / (rep.elapsed_ms * 1e-3),
+            "rays_per_s": rep.iterations * g.n_views * g.n_bins / (rep.elapsed_ms * 1e-3),
+            "elapsed_ms": rep.elapsed_ms}, sino
 
 
 def bench_c3(ctx, n_jobs, iters, log, plain_pass=True):
@@ -227,7 +279,9 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-passes", type=int, default=10)
+    ap.add_argument("--cpu-passes-c1", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--c3-jobs", type=int, default=64)
     ap.add_argument("--c3-iters", type=int, default=50)
     ap.add_argument("--no-c3", action="store_true")
@@ -258,7 +312,9 @@ def main():
     ctx.trim()
     nnz = ctx.prepare(g, 12)
     print(f"[bench] nnz {nnz}, setup {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
-    p = configs.c2_params()
+    # every pass below runs (warm-up, instrumented, timed): none is skipped
+    # by the iteration cap
+    p = configs.c2_params(max_iters=args.warmup + max(args.warmup, 5) + args.steps)
     job = ctx.job(g, sino.astype(np.float32), p)
     job.enqueue(args.warmup)
     job.synchronize()
@@ -299,6 +355,8 @@ def main():
     rep = job.report()
     print(f"[bench] {args.steps} AT iterations in {elapsed * 1e3:.1f} ms; data fidelity {rep.data_fidelity:.6e}",
           file=sys.stderr, flush=True)
+    total_passes = rep.iterations
+    u32, v32 = job.read()
     s = ctx.stats(dom)
     avg_ms = s.total_ms / max(s.launches, 1)
     bytes_per = s.bytes / max(s.launches, 1)
@@ -338,6 +396,27 @@ def main():
                      "launches_timed": s.launches},
         "kernels": per_kernel,
     }
+    # The benched result's error: the same job, the same number of passes,
+    # run in FP64 throughout on the device (the full-FP64 GPU oracle of the
+    # C2 form; its agreement with the CPU FP64 oracle is a GPU test).
+    job.close()
+    p64 = configs.c2_params(max_iters=total_passes, precision=mi.MI_FP64)
+    u64, v64, rep64 = ctx.at_reconstruct(g, sino.astype(np.float64), p64)
+
+    def rel(a, b):
+        a = np.asarray(a, np.float64)
+        b = np.asarray(b, np.float64)
+        return (float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)),
+                float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)))
+    eu, ev = rel(u32, u64), rel(v32, v64)
+    out["accuracy"] = {"reference": f"the same job in FP64 on the device, {rep64.iterations} passes",
+                       "u_rel_l2": eu[0], "u_rel_max": eu[1], "v_rel_l2": ev[0], "v_rel_max": ev[1],
+                       "tolerance": 1e-5, "within": max(eu + ev) <= 1e-5}
+    print(f"[bench] FP32 vs FP64 after {total_passes} passes: u {eu[0]:.2e} / {eu[1]:.2e}, "
+          f"v {ev[0]:.2e} / {ev[1]:.2e}", file=sys.stderr, flush=True)
+    sino_c1 = None
+    if not args.no_c1:
+        out["c1"], sino_c1 = c1_gpu(ctx, 50)
     if args.copy_gib > 0:
         nb = int(args.copy_gib * (1 << 30)) // 16 * 16
         ms = ctx.device_copy(nb, 3)
@@ -346,17 +425,14 @@ def main():
     if not args.no_c4:
         out["cone_c4"] = bench_c4(ctx, args.c4_reps, args.c4_iters, lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_c3:
-        job.close()
         out["batched"] = bench_c3(ctx, args.c3_jobs, args.c3_iters,
                                   lambda m: print(m, file=sys.stderr, flush=True), not args.c3_batched_only)
     if not args.no_serve:
-        job.close()
         out["serving"] = bench_serving(ctx, args.serve_jobs, args.serve_rate,
                                        lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_cpu and args.cpu_passes > 0:
         print(f"[bench] CPU baseline: {args.cpu_passes} AT iterations of the CPU port", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(sino.astype(np.float32), args.cpu_passes)
-    job.close()
+        out["cpu_baseline"] = cpu_baseline(sino.astype(np.float32), args.cpu_passes, sino_c1, args.cpu_passes_c1)
     ctx.close()
     print(json.dumps(out))
 

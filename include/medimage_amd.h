@@ -66,6 +66,16 @@ enum mi_beam {
 
 enum mi_precision { MI_FP32 = 0, MI_FP64 = 1 };
 
+/* Summation order of the GoldenC-form drivers (mi_mumford_shah).
+ * MI_ORDER_REFERENCE (default): every sum is added in the order the
+ * reference program adds it (A and AStar per ray / per pixel in the order
+ * origin/GoldenC/src/proj.c:14-397 emits the products, energies and inner
+ * products in the row-major loops of origin/GoldenC/src/main.c:81-360), so
+ * results equal the reference's bit for bit.  MI_ORDER_PARALLEL: the same
+ * algorithm with tree reductions (faster; rounding differs at 1e-16 and the
+ * CG amplifies it, about 20x per alternation). */
+enum mi_order { MI_ORDER_REFERENCE = 0, MI_ORDER_PARALLEL = 1 };
+
 /* Geometry.  Field <- reference name (industry3d Parameter / GoldenC config.h). */
 typedef struct mi_geometry {
     int32_t beam;          /* BEAM ("Parallel"/"Cone") -> enum mi_beam         */
@@ -82,6 +92,11 @@ typedef struct mi_geometry {
                               (double pi/tolerances), 1 blockrecon constants
                               (float, blockrecon/src/config.h:35-38)           */
     int32_t reserved;
+    /* MI_BEAM_FAN_ARC only: the source angle of each view in degrees
+     * (n_views values: the angle column of a rebinned helical slice's _pos
+     * table, helical/rebin/src/ms2d.cpp:40-47), or NULL for span_deg v /
+     * n_views.  Read during the call; the library keeps its own copy. */
+    const double *view_deg;
 } mi_geometry;
 
 /* AT parameters.  alpha/beta/epsilon as in
@@ -97,7 +112,7 @@ typedef struct mi_at_params {
     double  rel_tol;        /* OS form: stop when |df_k - df_{k-1}| < rel_tol*df_{k-1} */
     double  lower, upper;   /* LOWER_X / UPPER_X clamp of u (OS form)          */
     int32_t precision;      /* enum mi_precision                               */
-    int32_t reserved;
+    int32_t order;          /* enum mi_order (GoldenC form), 0 = reference     */
 } mi_at_params;
 
 /* Result report of a reconstruction. */
@@ -164,13 +179,30 @@ int mi_get_tuning(mi_context *ctx, const mi_geometry *g, int n_subsets, mi_tunin
  *              (origin/GoldenC/src/proj.c:14-204, msbeam proj.c:14-32)
  * backproject: img[x][y] = sum over rays through the pixel of sino * len,
  *              gather form, deterministic (proj.c:206-397, msbeam proj.c:34-52)
+ * For MI_BEAM_GOLDENC the _f64 forms add each value's products in the order
+ * proj.c does (one lane per ray / pixel), so they equal the reference's A
+ * and AStar bit for bit.
  */
 int mi_forward(mi_context *ctx, const mi_geometry *g, const float *img, float *sino);
 int mi_backproject(mi_context *ctx, const mi_geometry *g, const float *sino, float *img);
 int mi_forward_f64(mi_context *ctx, const mi_geometry *g, const double *img, double *sino);
 int mi_backproject_f64(mi_context *ctx, const mi_geometry *g, const double *sino, double *img);
 
-/* ---- AT building blocks (GoldenC, origin/GoldenC/src/main.c) ------------- */
+/* Projections in the reference's summation order and type: one lane per
+ * ray (pixel) adds the products in the order of the reference's own loop.
+ * MI_BEAM_GOLDENC with MI_FP64: A / AStar of origin/GoldenC/src/proj.c:14-397
+ * bit for bit.  MI_BEAM_PARALLEL (flavour 0) with MI_FP32: msbeam's A /
+ * AStar (msbeam/CPU_OpenMP/src/proj.c:14-52: rays in [view][bin] order,
+ * float products and sums) bit for bit.  Other beams: the tracer's order
+ * along each ray and ray order at each pixel.  Element type of img / sino:
+ * float for MI_FP32, double for MI_FP64.  Slower than mi_forward (one lane
+ * per sum); meant for drop-in parity. */
+int mi_forward_ordered(mi_context *ctx, const mi_geometry *g, int precision, const void *img, void *sino);
+int mi_backproject_ordered(mi_context *ctx, const mi_geometry *g, int precision, const void *sino, void *img);
+
+/* ---- AT building blocks (GoldenC, origin/GoldenC/src/main.c) -------------
+ * Both steps sum in the reference's order (MI_ORDER_REFERENCE): equal to the
+ * reference's functions bit for bit on the GoldenC geometry. */
 /* minimize_in_IMAGE_variable (main.c:81-257): CG image step, FP64. */
 int mi_image_step_f64(mi_context *ctx, const mi_geometry *g, double *f_out, const double *sino,
                       const double *f, const double *v, double alpha, double epsilon,
@@ -183,7 +215,7 @@ int mi_medfilt2_f64(mi_context *ctx, int nx, int ny, double *f);
 
 /* ---- drivers ------------------------------------------------------------- */
 /* MumfordShah (main.c:35-49): f = 0, v = 1, iter_alt x (image step, edge
- * step), FP64 throughout. */
+ * step), FP64 throughout, in the summation order p->order selects. */
 int mi_mumford_shah(mi_context *ctx, const mi_geometry *g, const mi_at_params *p,
                     const double *sino, double *f, double *v, mi_report *rep);
 /* Ordered-subset AT reconstruction of one job (the plain per-job GPU form).
@@ -192,6 +224,23 @@ int mi_at_reconstruct(mi_context *ctx, const mi_geometry *g, const mi_at_params 
                       const float *sino, float *u, float *v, mi_report *rep);
 int mi_at_reconstruct_f64(mi_context *ctx, const mi_geometry *g, const mi_at_params *p,
                           const double *sino, double *u, double *v, mi_report *rep);
+
+/* The per-slice loop of the helical slice driver
+ * (helical/rebin/src/main.cpp:158-195, ms2d.cpp:251-274): for slice i =
+ * first .. first + count - 1 it reads <prefix>i_pos and <prefix>i_raw (as
+ * the reference builds them from rebined_folder), takes the slice's view
+ * count and per-view source angles from the table (g->n_views, span_deg and
+ * view_deg are not used), and reconstructs it with the ordered-subset AT
+ * form of p (u = 0, v = 1 at the start, as ms2d.cpp:275-276).  The slices
+ * run together in one batch, admitted as device memory allows.  g: the arc
+ * geometry every slice shares (MI_BEAM_FAN_ARC: image size, pixel, bins,
+ * LENGTH_PER_DET, SOURCE_TO_ISO / _DET, OFF_CENTER).  u_stack / v_stack:
+ * count slices of nx * ny, slice-major, float for MI_FP32 and double for
+ * MI_FP64; reps (may be NULL): one report per slice.  The reference runs
+ * its per-ray msbeam-style sweep there (ms2d.cpp:59-184); this runs the
+ * ordered-subset form of the same energy. */
+int mi_helical_reconstruct(mi_context *ctx, const char *prefix, int first, int count, const mi_geometry *g,
+                           const mi_at_params *p, void *u_stack, void *v_stack, mi_report *reps);
 
 /* ---- device-resident jobs (ordered-subset form) ----------------------------
  * A job keeps its sinogram, image u, edge v and scalars in HBM between
@@ -314,6 +363,16 @@ int mi_device_copy(mi_context *ctx, int64_t bytes, int reps, double *ms);
  * ITERATIONS, LOWER_X / UPPER_X).  A missing required key is MI_E_ARG, as a
  * missing key stops the reference's parser. */
 int mi_config_parse(const char *json_text, mi_geometry *g, mi_at_params *p);
+/* Reads a rebinned helical slice's table, <slice>_pos
+ * (helical/rebin/src/ms2d.cpp:33-47): lines "pos,angle" (table position as
+ * written, source angle in degrees) up to the first line that does not
+ * parse.  Up to cap rows go to pos_mm / angle_deg (either may be NULL);
+ * *n = rows in the file, the slice's view count (NPROJ). */
+int mi_read_pos(const char *path, double *pos_mm, double *angle_deg, int cap, int *n);
+/* Reads a slice's sinogram, <slice>_raw (ms2d.cpp:270-274): n_views *
+ * n_bins whitespace-separated numbers in [view][bin] order.  Fewer is
+ * MI_E_ARG. */
+int mi_read_raw(const char *path, int64_t n_views, int64_t n_bins, double *sino);
 /* Reads an industry3d projection file (industry3d/src/ct3d.cpp:199-214,
  * industry3d/readme.txt): a 1024-byte header, nproj * ndx * ndy unsigned
  * shorts in [proj][x][y] order into proj, then (flat != NULL) the ndx * ndy

@@ -38,14 +38,29 @@ class Geometry(C.Structure):
         ("beam", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("pixel_mm", C.c_double),
         ("n_views", C.c_int32), ("span_deg", C.c_double), ("n_bins", C.c_int32), ("bin_mm", C.c_double),
         ("sod_mm", C.c_double), ("sdd_mm", C.c_double), ("det_offset", C.c_double),
-        ("flavour", C.c_int32), ("reserved", C.c_int32),
+        ("flavour", C.c_int32), ("reserved", C.c_int32), ("view_deg", C.POINTER(C.c_double)),
     ]
 
     def key(self):
-        return tuple(getattr(self, f) for f, _ in self._fields_)
+        views = getattr(self, "_views", None)
+        return tuple(getattr(self, f) for f, _ in self._fields_[:-1]) + \
+            ((views.tobytes(),) if views is not None else ())
+
+    def with_views(self, angles_deg):
+        """A copy with per-view source angles (degrees; MI_BEAM_FAN_ARC), as a
+        rebinned helical slice's _pos table gives them."""
+        a = np.array(angles_deg, np.float64)   # an owned copy
+        if a.shape != (self.n_views,):
+            raise ValueError(f"need {self.n_views} angles, got {a.shape}")
+        g = Geometry(self.beam, self.nx, self.ny, self.pixel_mm, self.n_views, self.span_deg, self.n_bins,
+                     self.bin_mm, self.sod_mm, self.sdd_mm, self.det_offset, self.flavour, 0,
+                     a.ctypes.data_as(C.POINTER(C.c_double)))
+        g._views = a
+        return g
 
     def __repr__(self):
-        return "Geometry(" + ", ".join(f"{f}={getattr(self, f)}" for f, _ in self._fields_[:-1]) + ")"
+        extra = ", view_deg=table" if getattr(self, "_views", None) is not None else ""
+        return "Geometry(" + ", ".join(f"{f}={getattr(self, f)}" for f, _ in self._fields_[:-2]) + extra + ")"
 
 
 class ConeGeometry(C.Structure):
@@ -75,7 +90,7 @@ class AtParams(C.Structure):
         ("iter_alt", C.c_int32), ("iter_image", C.c_int32), ("iter_edge", C.c_int32),
         ("n_subsets", C.c_int32), ("max_iters", C.c_int32), ("median_every", C.c_int32),
         ("rel_tol", C.c_double), ("lower", C.c_double), ("upper", C.c_double),
-        ("precision", C.c_int32), ("reserved", C.c_int32),
+        ("precision", C.c_int32), ("order", C.c_int32),
     ]
 
 
@@ -124,9 +139,12 @@ def goldenc_geometry():
 
 
 def at_params(alpha=1e-3, beta=1e-4, epsilon=1e-2, n_subsets=1, max_iters=50, iter_edge=1, rel_tol=0.0,
-              median_every=0, lower=-math.inf, upper=math.inf, iter_alt=10, iter_image=10, precision=MI_FP32):
+              median_every=0, lower=-math.inf, upper=math.inf, iter_alt=10, iter_image=10, precision=MI_FP32,
+              order=0):
+    """order (GoldenC form): MI_ORDER_REFERENCE (0) sums in the reference's
+    order, MI_ORDER_PARALLEL (1) with tree reductions."""
     return AtParams(alpha, beta, epsilon, iter_alt, iter_image, iter_edge, n_subsets, max_iters, median_every,
-                    rel_tol, lower, upper, precision, 0)
+                    rel_tol, lower, upper, precision, order)
 
 
 class MIError(RuntimeError):
@@ -163,6 +181,8 @@ def _load():
         "mi_backproject": [P, G, P, P],
         "mi_forward_f64": [P, G, P, P],
         "mi_backproject_f64": [P, G, P, P],
+        "mi_forward_ordered": [P, G, C.c_int, P, P],
+        "mi_backproject_ordered": [P, G, C.c_int, P, P],
         "mi_image_step_f64": [P, G, P, P, P, P, C.c_double, C.c_double, C.c_int],
         "mi_edge_step_f64": [P, C.c_int, C.c_int, P, P, P, C.c_double, C.c_double, C.c_double, C.c_int],
         "mi_medfilt2_f64": [P, C.c_int, C.c_int, P],
@@ -195,6 +215,9 @@ def _load():
         "mi_device_copy": [P, C.c_int64, C.c_int, C.POINTER(C.c_double)],
         "mi_config_parse": [C.c_char_p, G, C.POINTER(AtParams)],
         "mi_read_dr": [C.c_char_p, C.c_int64, C.c_int64, C.c_int64, P, P],
+        "mi_read_pos": [C.c_char_p, P, P, C.c_int, C.POINTER(C.c_int)],
+        "mi_read_raw": [C.c_char_p, C.c_int64, C.c_int64, P],
+        "mi_helical_reconstruct": [P, C.c_char_p, C.c_int, C.c_int, G, C.POINTER(AtParams), P, P, P],
         "mi_cone_trace_ray": [P, CG, C.c_int, C.c_int, C.c_int, P, P, C.c_int, C.POINTER(C.c_int)],
         "mi_cone_forward": [P, CG, P, P],
         "mi_cone_forward_dev": [P, CG, P, P],
@@ -240,6 +263,23 @@ def read_dr(path, nproj: int, ndx: int, ndy: int, flat: bool = True):
     fl = np.empty((ndx, ndy), np.float32) if flat else None
     _check(_load().mi_read_dr(str(path).encode(), nproj, ndx, ndy, _ptr(proj), _ptr(fl) if flat else None))
     return (proj, fl) if flat else proj
+
+
+def read_pos(path):
+    """(pos, angle_deg) columns of a rebinned helical slice's _pos table (mi_read_pos)."""
+    n = C.c_int()
+    _check(_load().mi_read_pos(str(path).encode(), None, None, 0, C.byref(n)))
+    pos = np.empty(n.value, np.float64)
+    ang = np.empty(n.value, np.float64)
+    _check(_load().mi_read_pos(str(path).encode(), _ptr(pos), _ptr(ang), n.value, C.byref(n)))
+    return pos, ang
+
+
+def read_raw(path, n_views: int, n_bins: int):
+    """[n_views, n_bins] sinogram of a slice's _raw file (mi_read_raw)."""
+    out = np.empty((n_views, n_bins), np.float64)
+    _check(_load().mi_read_raw(str(path).encode(), n_views, n_bins, _ptr(out)))
+    return out
 
 
 def _check(rc):
@@ -344,6 +384,22 @@ class Context:
         _check(fn(self._h, C.byref(g), _ptr(sino), _ptr(out)))
         return out
 
+    def forward_ordered(self, g: Geometry, img, dtype=np.float64):
+        """A in the reference's summation order and type (bitwise GoldenC's A
+        in FP64, msbeam's A in FP32 on its parallel geometry)."""
+        img = _arr(img, dtype, (g.nx, g.ny))
+        out = np.empty((g.n_views, g.n_bins), dtype)
+        prec = MI_FP64 if dtype == np.float64 else MI_FP32
+        _check(self._lib.mi_forward_ordered(self._h, C.byref(g), prec, _ptr(img), _ptr(out)))
+        return out
+
+    def backproject_ordered(self, g: Geometry, sino, dtype=np.float64):
+        sino = _arr(sino, dtype, (g.n_views, g.n_bins))
+        out = np.empty((g.nx, g.ny), dtype)
+        prec = MI_FP64 if dtype == np.float64 else MI_FP32
+        _check(self._lib.mi_backproject_ordered(self._h, C.byref(g), prec, _ptr(sino), _ptr(out)))
+        return out
+
     # -- 3D cone beam (industry3d) -----------------------------------------
     def cone_trace_ray(self, g: ConeGeometry, view: int, row: int, col: int, cap: int = 0):
         """(voxel index, length) sequence of ray (view, row, col) as traced on
@@ -426,6 +482,17 @@ class Context:
         _check(self._lib.mi_mumford_shah(self._h, C.byref(g), C.byref(p), _ptr(sino), _ptr(f), _ptr(v),
                                          C.byref(rep)))
         return f, v, rep
+
+    def helical_reconstruct(self, prefix, first: int, count: int, g: Geometry, p: AtParams):
+        """Per-slice loop over <prefix>i_pos / <prefix>i_raw (mi_helical_reconstruct);
+        returns (u [count, nx, ny], v, reports)."""
+        dt = np.float64 if p.precision == MI_FP64 else np.float32
+        u = np.empty((count, g.nx, g.ny), dt)
+        v = np.empty((count, g.nx, g.ny), dt)
+        reps = (Report * count)()
+        _check(self._lib.mi_helical_reconstruct(self._h, str(prefix).encode(), first, count, C.byref(g), C.byref(p),
+                                                _ptr(u), _ptr(v), reps))
+        return u, v, list(reps)
 
     def at_reconstruct(self, g: Geometry, sino, p: AtParams, u0=None, v0=None):
         dt = np.float64 if p.precision == MI_FP64 else np.float32

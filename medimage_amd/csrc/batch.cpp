@@ -92,6 +92,8 @@ int64_t Batch<T>::submit(const mi_geometry &g, const mi_at_params &p, const T *s
     BatchJob<T> j;
     j.id = next_id++;
     j.geom = g;
+    j.views = view_table(g);
+    j.geom.view_deg = nullptr;
     j.p = p;
     j.p.n_subsets = S;
     const int64_t nrows = (int64_t)g.n_views * g.n_bins, npix = (int64_t)g.nx * g.ny;
@@ -233,22 +235,22 @@ int Batch<T>::admit(bool require) {
         try {
             if (!j.m) {
                 try {
-                    j.m = get_engine_matrix(ctx, j.geom, S, sizeof(T) == 8);
+                    j.m = get_engine_matrix(ctx, j.geometry(), S, sizeof(T) == 8);
                 } catch (const Error &e) {
                     if (e.code != MI_E_NOMEM) throw;
                     (void)hipGetLastError();
                     if (evict_idle(ctx, -1) <= 0) throw;
-                    j.m = get_engine_matrix(ctx, j.geom, S, sizeof(T) == 8);
+                    j.m = get_engine_matrix(ctx, j.geometry(), S, sizeof(T) == 8);
                 }
             }
             j.m->last_use = ++ctx->tick;
             // the budget keeps the device as full as it needs to be and lets
             // later jobs wait in arrival order instead of sharing it
             if (max_entries > 0 && n_active() > 0 && active_entries() + j.m->nnz > max_entries) break;
-            const GeomKey key = make_key(j.geom, S);
+            const GeomKey key = make_key(j.geometry(), S);
             int group = 0;
             for (int64_t q : pending) {
-                const GeomKey kq = make_key(jobs.at(q).geom, S);
+                const GeomKey kq = make_key(jobs.at(q).geometry(), S);
                 if (!(kq < key) && !(key < kq)) ++group;
             }
             place(j, group);

@@ -42,7 +42,13 @@ struct Unit {
 template <typename T>
 struct BatchJob {
     int64_t id = 0;
-    mi_geometry geom{};
+    mi_geometry geom{};           // geom.view_deg unused: see geometry()
+    std::vector<double> views;    // owned copy of the caller's per-view angles
+    mi_geometry geometry() const {
+        mi_geometry g = geom;
+        g.view_deg = views.empty() ? nullptr : views.data();
+        return g;
+    }
     mi_at_params p{};
     int state = BJ_PENDING;
     std::shared_ptr<Matrix> m;

@@ -2,6 +2,7 @@
 // error translation, context and geometry cache, instrumentation.
 #include "batch.hpp"
 #include "cone.hpp"
+#include "ordered.hpp"
 #include "solver.hpp"
 
 #include <cmath>
@@ -43,8 +44,17 @@ void hip_check(hipError_t e, const char *what, const char *file, int line) {
     throw Error(e == hipErrorOutOfMemory ? MI_E_NOMEM : MI_E_DEVICE, m);
 }
 
+std::vector<double> view_table(const mi_geometry &g) {
+    if (!g.view_deg) return {};
+    return std::vector<double>(g.view_deg, g.view_deg + g.n_views);
+}
+
 void validate_geometry(const mi_geometry &g) {
     MI_REQUIRE(g.nx > 0 && g.ny > 0 && g.n_views > 0 && g.n_bins > 0, MI_E_ARG, "geometry sizes must be positive");
+    MI_REQUIRE(!g.view_deg || g.beam == MI_BEAM_FAN_ARC, MI_E_ARG, "per-view angles are for the arc beam only");
+    if (g.view_deg)
+        for (int v = 0; v < g.n_views; ++v)
+            MI_REQUIRE(std::isfinite(g.view_deg[v]), MI_E_ARG, "per-view angles must be finite");
     MI_REQUIRE((int64_t)g.nx * g.ny < (int64_t)1 << 31, MI_E_LIMIT, "image too large");
     switch (g.beam) {
         case MI_BEAM_FAN_FLAT:
@@ -77,8 +87,9 @@ GeomKey make_key(const mi_geometry &g, int S, int layout) {
     GeomKey k;
     k.i = {g.beam, g.nx, g.ny, g.n_views, g.n_bins, g.beam == MI_BEAM_PARALLEL ? g.flavour : 0, S, layout};
     if (g.beam == MI_BEAM_FAN_FLAT || g.beam == MI_BEAM_FAN_ARC)
-        k.d = {g.pixel_mm, g.span_deg, g.bin_mm, g.sod_mm, g.sdd_mm, g.det_offset};
+        k.d = {g.pixel_mm, g.view_deg ? 0.0 : g.span_deg, g.bin_mm, g.sod_mm, g.sdd_mm, g.det_offset};
     else k.d = {0, 0, 0, 0, 0, 0};
+    k.views = view_table(g);
     return k;
 }
 
@@ -296,7 +307,15 @@ void forward_impl(mi_context *ctx, const mi_geometry *g, const T *img, T *sino) 
     const hipStream_t st = ctx->stream;
     DevBuf<T> x(m.npix), rows(m.nrows), user(m.nrows);
     MI_HIP(hipMemcpyAsync(x.p, img, m.npix * sizeof(T), hipMemcpyHostToDevice, st));
-    {
+    if constexpr (sizeof(T) == 8) {
+        // GoldenC's A in the reference's summation order (bitwise its result).
+        if (g->beam == MI_BEAM_GOLDENC) {
+            const SeqMatrix &q = get_seq(st, m);
+            LaunchTimer lt(ctx, MI_K_FP, fp_bytes(m, m.nrows, sizeof(T), FP_WRITE));
+            seq_forward<double>(st, q, x.p, rows.p);
+        }
+    }
+    if (!(f64 && g->beam == MI_BEAM_GOLDENC)) {
         LaunchTimer lt(ctx, MI_K_FP, fp_bytes(m, m.nrows, sizeof(T), FP_WRITE));
         launch_fp<T>(st, FP_WRITE, csr_view(m, f64), 0, m.nrows, x.p, nullptr, nullptr, rows.p, nullptr);
     }
@@ -314,12 +333,97 @@ void backproject_impl(mi_context *ctx, const mi_geometry *g, const T *sino, T *i
     const hipStream_t st = ctx->stream;
     DevBuf<T> x(m.npix), rows(m.nrows);
     load_sino<T>(st, m, sino, rows.p);
-    {
+    if constexpr (sizeof(T) == 8) {
+        // GoldenC's AStar in the reference's summation order.
+        if (g->beam == MI_BEAM_GOLDENC) {
+            const SeqMatrix &q = get_seq(st, m);
+            LaunchTimer lt(ctx, MI_K_BP, bp_bytes(m, m.nrows, sizeof(T), false));
+            seq_backproject<double>(st, q, g->nx, g->ny, rows.p, nullptr, nullptr, 0.0, 0.0, x.p);
+        }
+    }
+    if (!(f64 && g->beam == MI_BEAM_GOLDENC)) {
         LaunchTimer lt(ctx, MI_K_BP, bp_bytes(m, m.nrows, sizeof(T), false));
         launch_bp_grad<T>(st, csc_view(m, 0, f64), g->nx, g->ny, rows.p, 1.0, nullptr, nullptr, 0.0, 0.0, x.p,
                           nullptr, bp_lanes(m));
     }
     MI_HIP(hipMemcpyAsync(img, x.p, m.npix * sizeof(T), hipMemcpyDeviceToHost, st));
+    MI_HIP(hipStreamSynchronize(st));
+    stats_collect(ctx);
+}
+
+template <typename T>
+void helical_slices(mi_context *ctx, const char *prefix, int first, int count, const mi_geometry &g0,
+                    const mi_at_params &p, T *u_stack, T *v_stack, mi_report *reps) {
+    MI_REQUIRE(g0.beam == MI_BEAM_FAN_ARC, MI_E_ARG, "the helical slices use the arc beam");
+    MI_REQUIRE(p.n_subsets >= 1 && p.max_iters >= 1 && p.iter_edge >= 0, MI_E_ARG, "bad AT parameters");
+    const int64_t npix = (int64_t)g0.nx * g0.ny;
+    Batch<T> b(ctx, p.n_subsets);
+    std::vector<int64_t> ids(count);
+    std::vector<double> stage;
+    std::vector<T> sino;
+    for (int k = 0; k < count; ++k) {
+        const std::string base = std::string(prefix) + std::to_string(first + k);
+        // the slice's table fixes its view count and angles (load_table)
+        int nv = 0;
+        if (mi_read_pos((base + "_pos").c_str(), nullptr, nullptr, 0, &nv) != MI_OK) fail(MI_E_ARG, mi_last_error());
+        MI_REQUIRE(nv >= p.n_subsets, MI_E_ARG, base + "_pos: fewer views than subsets");
+        std::vector<double> angle(nv);
+        if (mi_read_pos((base + "_pos").c_str(), nullptr, angle.data(), nv, &nv) != MI_OK)
+            fail(MI_E_ARG, mi_last_error());
+        stage.resize((size_t)nv * g0.n_bins);
+        if (mi_read_raw((base + "_raw").c_str(), nv, g0.n_bins, stage.data()) != MI_OK) fail(MI_E_ARG, mi_last_error());
+        sino.assign(stage.begin(), stage.end());
+        mi_geometry g = g0;
+        g.n_views = nv;
+        g.view_deg = angle.data();
+        ids[k] = b.submit(g, p, sino.data(), nullptr, nullptr);
+    }
+    // Every slice advances in the same passes; finished slices leave (their
+    // result goes to host memory) and pending ones join as device memory
+    // frees up.
+    for (;;) {
+        b.step(1);
+        bool all_done = true;
+        for (int k = 0; k < count; ++k) {
+            const BatchJob<T> &j = b.job(ids[k]);
+            MI_REQUIRE(j.state != BJ_FAILED, MI_E_NOMEM, "slice " + std::to_string(first + k) + ": " + j.error);
+            if (j.state != BJ_DONE) all_done = false;
+        }
+        if (all_done) break;
+    }
+    for (int k = 0; k < count; ++k) {
+        b.read(ids[k], u_stack + (size_t)k * npix, v_stack + (size_t)k * npix);
+        if (reps) fill_report<T>(b, ids[k], &reps[k]);
+    }
+    stats_collect(ctx);
+}
+
+// Projections in the reference's summation order (mi_forward_ordered).
+template <typename T>
+void ordered_impl(mi_context *ctx, const mi_geometry *g, const T *in, T *out, bool forward) {
+    MI_REQUIRE(g && in && out, MI_E_ARG, "null argument");
+    // The parallel beam's tracer works in float; the others' lengths are
+    // double and the lists keep them so.
+    Matrix &m = *get_matrix(ctx, *g, 1, g->beam != MI_BEAM_PARALLEL);
+    const hipStream_t st = ctx->stream;
+    const SeqMatrix &q = get_seq(st, m);
+    DevBuf<T> x(m.npix), rows(m.nrows);
+    if (forward) {
+        MI_HIP(hipMemcpyAsync(x.p, in, m.npix * sizeof(T), hipMemcpyHostToDevice, st));
+        {
+            LaunchTimer lt(ctx, MI_K_FP, fp_bytes(m, m.nrows, sizeof(T), FP_WRITE));
+            seq_forward<T>(st, q, x.p, rows.p);
+        }
+        // one subset: rows are already in [view][bin] order
+        MI_HIP(hipMemcpyAsync(out, rows.p, m.nrows * sizeof(T), hipMemcpyDeviceToHost, st));
+    } else {
+        MI_HIP(hipMemcpyAsync(rows.p, in, m.nrows * sizeof(T), hipMemcpyHostToDevice, st));
+        {
+            LaunchTimer lt(ctx, MI_K_BP, bp_bytes(m, m.nrows, sizeof(T), false));
+            seq_backproject<T>(st, q, g->nx, g->ny, rows.p, nullptr, nullptr, 0.0, 0.0, x.p);
+        }
+        MI_HIP(hipMemcpyAsync(out, x.p, m.npix * sizeof(T), hipMemcpyDeviceToHost, st));
+    }
     MI_HIP(hipStreamSynchronize(st));
     stats_collect(ctx);
 }
@@ -523,6 +627,23 @@ int mi_backproject_f64(mi_context *ctx, const mi_geometry *g, const double *sino
     return guarded([&] { backproject_impl<double>(check_ctx(ctx), g, sino, img); });
 }
 
+int mi_forward_ordered(mi_context *ctx, const mi_geometry *g, int precision, const void *img, void *sino) {
+    return guarded([&] {
+        check_ctx(ctx);
+        MI_REQUIRE(precision == MI_FP32 || precision == MI_FP64, MI_E_ARG, "bad precision");
+        if (precision == MI_FP64) ordered_impl<double>(ctx, g, (const double *)img, (double *)sino, true);
+        else ordered_impl<float>(ctx, g, (const float *)img, (float *)sino, true);
+    });
+}
+int mi_backproject_ordered(mi_context *ctx, const mi_geometry *g, int precision, const void *sino, void *img) {
+    return guarded([&] {
+        check_ctx(ctx);
+        MI_REQUIRE(precision == MI_FP32 || precision == MI_FP64, MI_E_ARG, "bad precision");
+        if (precision == MI_FP64) ordered_impl<double>(ctx, g, (const double *)sino, (double *)img, false);
+        else ordered_impl<float>(ctx, g, (const float *)sino, (float *)img, false);
+    });
+}
+
 int mi_image_step_f64(mi_context *ctx, const mi_geometry *g, double *f_out, const double *sino, const double *f,
                       const double *v, double alpha, double epsilon, int iter_image) {
     return guarded([&] {
@@ -534,7 +655,7 @@ int mi_image_step_f64(mi_context *ctx, const mi_geometry *g, double *f_out, cons
         load_sino<double>(st, m, sino, js.g.p);
         MI_HIP(hipMemcpyAsync(js.u.p, f, js.npix * sizeof(double), hipMemcpyHostToDevice, st));
         MI_HIP(hipMemcpyAsync(js.v.p, v, js.npix * sizeof(double), hipMemcpyHostToDevice, st));
-        gc_image_step<double>(ctx, m, js, alpha, epsilon, iter_image, nullptr);
+        gc_image_step_ordered(ctx, m, js, alpha, epsilon, iter_image, nullptr);
         MI_HIP(hipMemcpyAsync(f_out, js.u.p, js.npix * sizeof(double), hipMemcpyDeviceToHost, st));
         MI_HIP(hipStreamSynchronize(st));
     });
@@ -555,7 +676,7 @@ int mi_edge_step_f64(mi_context *ctx, int nx, int ny, double *v_out, const doubl
         const hipStream_t st = ctx->stream;
         MI_HIP(hipMemcpyAsync(js.u.p, f, js.npix * sizeof(double), hipMemcpyHostToDevice, st));
         MI_HIP(hipMemcpyAsync(js.v.p, v, js.npix * sizeof(double), hipMemcpyHostToDevice, st));
-        gc_edge_step<double>(ctx, js, alpha, beta, epsilon, iter_edge);
+        gc_edge_step_ordered(ctx, js, alpha, beta, epsilon, iter_edge);
         MI_HIP(hipMemcpyAsync(v_out, js.v.p, js.npix * sizeof(double), hipMemcpyDeviceToHost, st));
         MI_HIP(hipStreamSynchronize(st));
     });
@@ -581,7 +702,10 @@ int mi_mumford_shah(mi_context *ctx, const mi_geometry *g, const mi_at_params *p
         check_ctx(ctx);
         MI_REQUIRE(g && p && sino && f && v, MI_E_ARG, "null argument");
         MI_REQUIRE(p->iter_alt >= 0 && p->iter_image >= 1 && p->iter_edge >= 1, MI_E_ARG, "bad iteration counts");
+        MI_REQUIRE(p->order == MI_ORDER_REFERENCE || p->order == MI_ORDER_PARALLEL, MI_E_ARG, "bad order");
+        const bool ref = p->order == MI_ORDER_REFERENCE;
         Matrix &m = *get_matrix(ctx, *g, 1, true);
+        if (ref) get_seq(ctx->stream, m);
         JobState<double> js(m);
         const hipStream_t st = ctx->stream;
         load_sino<double>(st, m, sino, js.g.p);
@@ -592,8 +716,13 @@ int mi_mumford_shah(mi_context *ctx, const mi_geometry *g, const mi_at_params *p
         MI_HIP(hipEventCreate(&e1));
         MI_HIP(hipEventRecord(e0, st));
         for (int i = 1; i <= p->iter_alt; ++i) {
-            gc_image_step<double>(ctx, m, js, p->alpha, p->epsilon, p->iter_image, nullptr);
-            gc_edge_step<double>(ctx, js, p->alpha, p->beta, p->epsilon, p->iter_edge);
+            if (ref) {
+                gc_image_step_ordered(ctx, m, js, p->alpha, p->epsilon, p->iter_image, nullptr);
+                gc_edge_step_ordered(ctx, js, p->alpha, p->beta, p->epsilon, p->iter_edge);
+            } else {
+                gc_image_step<double>(ctx, m, js, p->alpha, p->epsilon, p->iter_image, nullptr);
+                gc_edge_step<double>(ctx, js, p->alpha, p->beta, p->epsilon, p->iter_edge);
+            }
         }
         MI_HIP(hipEventRecord(e1, st));
         MI_HIP(hipEventSynchronize(e1));
@@ -628,6 +757,18 @@ int mi_at_reconstruct_f64(mi_context *ctx, const mi_geometry *g, const mi_at_par
         check_ctx(ctx);
         MI_REQUIRE(g && p && sino && u && v, MI_E_ARG, "null argument");
         at_solve<double>(ctx, *g, *p, sino, u, v, rep);
+    });
+}
+
+int mi_helical_reconstruct(mi_context *ctx, const char *prefix, int first, int count, const mi_geometry *g,
+                           const mi_at_params *p, void *u_stack, void *v_stack, mi_report *reps) {
+    return guarded([&] {
+        check_ctx(ctx);
+        MI_REQUIRE(prefix && g && p && u_stack && v_stack && count >= 1, MI_E_ARG, "bad arguments");
+        if (p->precision == MI_FP64)
+            helical_slices<double>(ctx, prefix, first, count, *g, *p, (double *)u_stack, (double *)v_stack, reps);
+        else
+            helical_slices<float>(ctx, prefix, first, count, *g, *p, (float *)u_stack, (float *)v_stack, reps);
     });
 }
 
@@ -1119,6 +1260,41 @@ int mi_config_parse(const char *json, mi_geometry *g, mi_at_params *p) {
             p->upper = opt(o, "UPPER_X", INFINITY);
         }
         validate_geometry(*g);
+    });
+}
+
+int mi_read_pos(const char *path, double *pos_mm, double *angle_deg, int cap, int *n) {
+    return guarded([&] {
+        MI_REQUIRE(path && n && cap >= 0, MI_E_ARG, "bad arguments");
+        FILE *fp = std::fopen(path, "r");
+        MI_REQUIRE(fp, MI_E_ARG, std::string("cannot open table file ") + path);
+        // load_table: fscanf "%lf,%lf\n" until a line does not parse
+        double pos = 0, angle = 0;
+        int rows = 0;
+        while (std::fscanf(fp, "%lf,%lf\n", &pos, &angle) == 2) {
+            if (rows < cap) {
+                if (pos_mm) pos_mm[rows] = pos;
+                if (angle_deg) angle_deg[rows] = angle;
+            }
+            ++rows;
+        }
+        std::fclose(fp);
+        *n = rows;
+    });
+}
+
+int mi_read_raw(const char *path, int64_t n_views, int64_t n_bins, double *sino) {
+    return guarded([&] {
+        MI_REQUIRE(path && sino && n_views > 0 && n_bins > 0, MI_E_ARG, "bad arguments");
+        FILE *fp = std::fopen(path, "r");
+        MI_REQUIRE(fp, MI_E_ARG, std::string("cannot open sinogram file ") + path);
+        // ms2d.cpp:270-274: whitespace-separated values, [view][bin]
+        const int64_t want = n_views * n_bins;
+        int64_t got = 0;
+        while (got < want && std::fscanf(fp, "%lf", &sino[got]) == 1) ++got;
+        std::fclose(fp);
+        MI_REQUIRE(got == want, MI_E_ARG, std::string(path) + ": " + std::to_string(got) + " values, expected " +
+                                              std::to_string(want));
     });
 }
 

@@ -561,6 +561,10 @@ void validate_cone(const mi_cone_geometry &g) {
     MI_REQUIRE(g.nx > 0 && g.ny > 0 && g.nz > 0 && g.n_views > 0 && g.n_rows > 0 && g.n_cols > 0, MI_E_ARG,
                "cone geometry sizes must be positive");
     MI_REQUIRE((int64_t)g.nx * g.ny * g.nz < ((int64_t)1 << 31), MI_E_LIMIT, "volume beyond 2^31 voxels");
+    // The forward walk forms a voxel's offset as ((z nx + x) ny) + y with
+    // 24-bit multiplies: z nx + x and ny must stay below 2^24.
+    MI_REQUIRE((int64_t)g.nz * g.nx <= ((int64_t)1 << 24) && g.ny < (1 << 24), MI_E_LIMIT,
+               "cone volume needs nz * nx <= 2^24 and ny < 2^24");
     MI_REQUIRE((int64_t)g.n_views * g.n_rows * g.n_cols < ((int64_t)1 << 40), MI_E_LIMIT, "sinogram too large");
     MI_REQUIRE(std::isfinite(g.sample_mm) && std::isfinite(g.pixel_mm) && g.sample_mm > 0 && g.pixel_mm > 0,
                MI_E_ARG, "sample_mm and pixel_mm must be positive");

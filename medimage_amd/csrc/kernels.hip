@@ -4,6 +4,7 @@
 // either Kahan per lane (FP32 runs) or plain FP64, then combined in double in
 // a fixed order, so every result is deterministic.
 #include "kernels.hpp"
+#include "gc_stencil.hpp"
 
 #include <cfloat>
 #include <cmath>
@@ -36,10 +37,6 @@ __device__ __forceinline__ void put_partial_max(double *part, double mine, doubl
     if (part && threadIdx.x == 0) part[blockIdx.x] = b;
 }
 
-template <typename T>
-__device__ __forceinline__ T ld(const T *a, int i, int j, int nx, int ny, T pad) {
-    return (i < 0 || j < 0 || i >= nx || j >= ny) ? pad : a[(int64_t)i * ny + j];
-}
 
 // ---------------------------------------------------------------------------
 // Forward projection: one wave per ray.  Each lane takes four consecutive
@@ -123,19 +120,6 @@ __global__ __launch_bounds__(kBlock) void k_fp(const int64_t *__restrict__ rowpt
     put_partial(part, mine, lds);
 }
 
-// GoldenC image-gradient regulariser at pixel (i, j): f zero padded, v one
-// padded, origin/GoldenC/src/main.c:112-125.
-template <typename T>
-__device__ __forceinline__ double gc_reg(const T *u, const T *v, int i, int j, int nx, int ny, double eps) {
-    const double z = (double)u[(int64_t)i * ny + j];
-    const double zd = (double)ld(u, i + 1, j, nx, ny, T(0)), zr = (double)ld(u, i, j + 1, nx, ny, T(0));
-    const double zu = (double)ld(u, i - 1, j, nx, ny, T(0)), zl = (double)ld(u, i, j - 1, nx, ny, T(0));
-    const double p = (double)v[(int64_t)i * ny + j];
-    const double pu = (double)ld(v, i - 1, j, nx, ny, T(1)), pl = (double)ld(v, i, j - 1, nx, ny, T(1));
-    return +(p * p) * (zd - z) + (p * p) * (zr - z) - (pu * pu) * (z - zu) - (pl * pl) * (z - zl) +
-           0.01 * (eps * eps) * (zd + zr - 4 * z + zu + zl);
-}
-
 // Gather back projection: LPC lanes per pixel column of the subset's CSC,
 // four consecutive entries per lane per trip (16-byte loads, padded arrays),
 // then the fused gradient d = scale * b + alpha * R(u, v).
@@ -191,8 +175,8 @@ __global__ __launch_bounds__(kBlock) void k_nv(int nx, int ny, const T *__restri
     for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < npix; q += (int64_t)gridDim.x * kBlock) {
         const int i = (int)(q / ny), j = (int)(q % ny);
         const double z = (double)p[q];
-        const double a = z - (double)ld(p, i - 1, j, nx, ny, T(0));
-        const double b = z - (double)ld(p, i, j - 1, nx, ny, T(0));
+        const double a = z - (double)gc_ld(p, i - 1, j, nx, ny, T(0));
+        const double b = z - (double)gc_ld(p, i, j - 1, nx, ny, T(0));
         const double w = (double)v[q];
         mine += (w * w + e2) * (a * a + b * b);
     }
@@ -271,7 +255,7 @@ __global__ void k_median(int nx, int ny, const T *__restrict__ in, T *__restrict
         T t[9];
         int k = 0;
         for (int di = -1; di <= 1; ++di)
-            for (int dj = -1; dj <= 1; ++dj) t[k++] = ld(in, i + di, j + dj, nx, ny, T(0));
+            for (int dj = -1; dj <= 1; ++dj) t[k++] = gc_ld(in, i + di, j + dj, nx, ny, T(0));
         // Paeth's 19-exchange median-of-9 network.
         cswap(t[1], t[2]); cswap(t[4], t[5]); cswap(t[7], t[8]);
         cswap(t[0], t[1]); cswap(t[3], t[4]); cswap(t[6], t[7]);
@@ -287,25 +271,26 @@ __global__ void k_median(int nx, int ny, const T *__restrict__ in, T *__restrict
 // Edge step, origin/GoldenC/src/main.c:294-305: gradient and the two sums.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_edge_grad(int nx, int ny, const T *__restrict__ f,
-                                                      const T *__restrict__ v, T *__restrict__ d, double alpha,
-                                                      double beta, double eps, const Scalars *sc, double *part0,
-                                                      double *part1) {
+                                                      const T *__restrict__ v, T *__restrict__ d,
+                                                      T *__restrict__ nout, double alpha, double beta, double eps,
+                                                      const Scalars *sc, double *part0, double *part1) {
     __shared__ double lds[kBlock / kWave];
     if (sc->edge_done || sc->converged) return;
     const int64_t npix = (int64_t)nx * ny;
     double m0 = 0.0, m1 = 0.0;
     for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < npix; q += (int64_t)gridDim.x * kBlock) {
         const int i = (int)(q / ny), j = (int)(q % ny);
-        const double dxf = 0.5 * ((double)ld(f, i + 1, j, nx, ny, T(0)) - (double)ld(f, i - 1, j, nx, ny, T(0)));
-        const double dyf = 0.5 * ((double)ld(f, i, j + 1, nx, ny, T(0)) - (double)ld(f, i, j - 1, nx, ny, T(0)));
+        const double dxf = 0.5 * ((double)gc_ld(f, i + 1, j, nx, ny, T(0)) - (double)gc_ld(f, i - 1, j, nx, ny, T(0)));
+        const double dyf = 0.5 * ((double)gc_ld(f, i, j + 1, nx, ny, T(0)) - (double)gc_ld(f, i, j - 1, nx, ny, T(0)));
         const double n = dxf * dxf + dyf * dyf;
         const double t = (double)v[q];
-        const double lap = 0.25 * ((double)ld(v, i + 1, j, nx, ny, T(1)) + (double)ld(v, i, j + 1, nx, ny, T(1)) -
-                                   4 * t + (double)ld(v, i - 1, j, nx, ny, T(1)) +
-                                   (double)ld(v, i, j - 1, nx, ny, T(1)));
+        const double lap = 0.25 * ((double)gc_ld(v, i + 1, j, nx, ny, T(1)) + (double)gc_ld(v, i, j + 1, nx, ny, T(1)) -
+                                   4 * t + (double)gc_ld(v, i - 1, j, nx, ny, T(1)) +
+                                   (double)gc_ld(v, i, j - 1, nx, ny, T(1)));
         const double gr = alpha * t * n - beta * eps * lap + beta * (t - 1) / (4 * eps);
         const T dt = (T)(-gr);
         d[q] = dt;
+        if (nout) nout[q] = (T)n;
         const double dd = (double)dt * (double)dt;
         m0 += dd;
         m1 += n * dd;
@@ -324,8 +309,8 @@ __global__ __launch_bounds__(kBlock) void k_edge_nabla(int nx, int ny, const T *
     double mine = 0.0;
     for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < npix; q += (int64_t)gridDim.x * kBlock) {
         const int i = (int)(q / ny), j = (int)(q % ny);
-        const double a = 0.5 * ((double)ld(d, i + 1, j, nx, ny, T(0)) - (double)ld(d, i - 1, j, nx, ny, T(0)));
-        const double b = 0.5 * ((double)ld(d, i, j + 1, nx, ny, T(0)) - (double)ld(d, i, j - 1, nx, ny, T(0)));
+        const double a = 0.5 * ((double)gc_ld(d, i + 1, j, nx, ny, T(0)) - (double)gc_ld(d, i - 1, j, nx, ny, T(0)));
+        const double b = 0.5 * ((double)gc_ld(d, i, j + 1, nx, ny, T(0)) - (double)gc_ld(d, i, j - 1, nx, ny, T(0)));
         mine += a * a + b * b;
     }
     put_partial(part, mine, lds);
@@ -564,7 +549,8 @@ template <typename T>
 void launch_edge_iter(hipStream_t st, const RedBuf &rb, int nx, int ny, const T *f, T *v, T *dwork, double alpha,
                       double beta, double eps, Scalars *sc) {
     const unsigned grid = grid_pixels((int64_t)nx * ny);
-    k_edge_grad<T><<<grid, kBlock, 0, st>>>(nx, ny, f, v, dwork, alpha, beta, eps, sc, rb.slot(0), rb.slot(1));
+    k_edge_grad<T><<<grid, kBlock, 0, st>>>(nx, ny, f, v, dwork, nullptr, alpha, beta, eps, sc, rb.slot(0),
+                                            rb.slot(1));
     MI_HIP(hipGetLastError());
     k_edge_nabla<T><<<grid, kBlock, 0, st>>>(nx, ny, dwork, sc, rb.slot(2));
     MI_HIP(hipGetLastError());
@@ -580,16 +566,30 @@ void launch_edge_iter(hipStream_t st, const RedBuf &rb, int nx, int ny, const T 
     c.beta = beta;
     c.eps = eps;
     launch_finalize(st, c);
-    k_edge_update<T><<<grid, kBlock, 0, st>>>(nx, ny, v, dwork, sc, rb.slot(3));
+    launch_edge_update<T>(st, rb, nx, ny, v, dwork, sc);
+}
+
+template <typename T>
+void launch_edge_grad_n(hipStream_t st, int nx, int ny, const T *f, const T *v, T *d, T *n, double alpha,
+                        double beta, double eps, const Scalars *sc) {
+    k_edge_grad<T><<<grid_pixels((int64_t)nx * ny), kBlock, 0, st>>>(nx, ny, f, v, d, n, alpha, beta, eps, sc,
+                                                                     nullptr, nullptr);
     MI_HIP(hipGetLastError());
-    FinArgs d{};
-    d.op = FIN_EDGE_DONE;
-    d.nin = 1;
-    d.in[0] = rb.slot(3);
-    d.n[0] = (int)grid;
-    d.is_max[0] = 1;
-    d.sc = sc;
-    launch_finalize(st, d);
+}
+
+template <typename T>
+void launch_edge_update(hipStream_t st, const RedBuf &rb, int nx, int ny, T *v, const T *d, Scalars *sc) {
+    const unsigned grid = grid_pixels((int64_t)nx * ny);
+    k_edge_update<T><<<grid, kBlock, 0, st>>>(nx, ny, v, d, sc, rb.slot(3));
+    MI_HIP(hipGetLastError());
+    FinArgs f{};
+    f.op = FIN_EDGE_DONE;
+    f.nin = 1;
+    f.in[0] = rb.slot(3);
+    f.n[0] = (int)grid;
+    f.is_max[0] = 1;
+    f.sc = sc;
+    launch_finalize(st, f);
 }
 
 void launch_pass_begin(hipStream_t st, Scalars *sc) {
@@ -630,6 +630,9 @@ void launch_scatter_rows(hipStream_t st, int64_t n, const int64_t *perm, const T
     template void launch_median<T>(hipStream_t, int, int, const T *, T *);                                      \
     template void launch_edge_iter<T>(hipStream_t, const RedBuf &, int, int, const T *, T *, T *, double,       \
                                       double, double, Scalars *);                                               \
+    template void launch_edge_grad_n<T>(hipStream_t, int, int, const T *, const T *, T *, T *, double, double,  \
+                                        double, const Scalars *);                                               \
+    template void launch_edge_update<T>(hipStream_t, const RedBuf &, int, int, T *, const T *, Scalars *);      \
     template void launch_gather_rows<T>(hipStream_t, int64_t, const int64_t *, const T *, T *);                 \
     template void launch_scatter_rows<T>(hipStream_t, int64_t, const int64_t *, const T *, T *);
 

@@ -120,6 +120,15 @@ template <typename T>
 void launch_edge_iter(hipStream_t st, const RedBuf &rb, int nx, int ny, const T *f, T *v, T *dwork, double alpha,
                       double beta, double eps, Scalars *sc);
 
+// The edge step's parts: the descent direction d and n = |central grad f|^2
+// (no sums), and the update v = clamp(v + e_c d, 0, 1) with the early-stop
+// test (partial slot 3).  Both skip once sc->edge_done is set.
+template <typename T>
+void launch_edge_grad_n(hipStream_t st, int nx, int ny, const T *f, const T *v, T *d, T *n, double alpha,
+                        double beta, double eps, const Scalars *sc);
+template <typename T>
+void launch_edge_update(hipStream_t st, const RedBuf &rb, int nx, int ny, T *v, const T *d, Scalars *sc);
+
 // Pass bookkeeping of the ordered-subset form.
 void launch_pass_begin(hipStream_t st, Scalars *sc);
 void launch_pass_end(hipStream_t st, Scalars *sc, double rel_tol);

@@ -52,8 +52,15 @@ struct DevBuf {
 struct GeomKey {
     std::array<int32_t, 8> i;
     std::array<double, 6> d;
-    bool operator<(const GeomKey &o) const { return i != o.i ? i < o.i : d < o.d; }
+    std::vector<double> views;       // per-view source angles (arc beam), if given
+    bool operator<(const GeomKey &o) const {
+        if (i != o.i) return i < o.i;
+        if (d != o.d) return d < o.d;
+        return views < o.views;
+    }
 };
+// The caller's per-view angle table as an owned copy (empty: none).
+std::vector<double> view_table(const mi_geometry &g);
 // layout: 0 = CSR / CSC (projector calls), 1 = sliced (AT engine).
 GeomKey make_key(const mi_geometry &g, int n_subsets, int layout = 0);
 void validate_geometry(const mi_geometry &g);
@@ -165,13 +172,17 @@ __host__ __device__ __forceinline__ void tile_coords(int64_t q, int tny, int &i,
     j = (int)(t % tny) * kTileW + ((w >> 4) << 2) + (w & 3);
 }
 
+struct SeqMatrix;  // reference-order entry lists (ordered.hip)
+size_t seq_bytes(const SeqMatrix *q);
+
 // System matrix of one geometry, rows (rays) grouped by view subset.
 //   Row order: subsets in order; inside a subset its views ascending; inside
 //   a view the bins ascending.  Row r is ray (vlist[r / nbins], r % nbins).
 //   CSR (ray-major) feeds the forward projector; one CSC per subset
 //   (pixel-major, entries ordered by row) feeds the gather back projector.
 struct Matrix {
-    mi_geometry g{};
+    mi_geometry g{};                 // g.view_deg points into view_deg below
+    std::vector<double> view_deg;
     int S = 1;
     int npix = 0;
     int64_t nrows = 0, nnz = 0;
@@ -204,6 +215,7 @@ struct Matrix {
     int64_t npix_t = 0;
     bool has_sell = false;
     bool has_csr = true;             // false once the engine layout replaced it
+    std::shared_ptr<SeqMatrix> seq;  // GoldenC-form drivers, built on first use
 
     size_t bytes() const;
 };

@@ -120,45 +120,40 @@ inline unsigned grid_of(int64_t n, int b = kBlock) { return (unsigned)((n + b - 
 // ---------------------------------------------------------------------------
 // GoldenC symmetric tiled DDA (origin/GoldenC/src/proj.c:14-397), walked on
 // the host: one traversal per (angle 1..45, tile, ray 0..383) feeds up to
-// eight (view, bin) rays through the grid's rotations and mirror.  Entries
-// are kept in the order the reference accumulates them.  The view-90 column
+// eight (view, bin) rays through the grid's rotations and mirror.  put(view,
+// bin, pixel, length) is called in the order the reference accumulates the
+// products (views 0 and 90 first, then the traversals).  The view-90 column
 // differs between A (W/2 + k, proj.c:32) and AStar (W/2 - 1 + k, proj.c:225),
 // so the adjoint list is walked separately.
-struct GcRows {
-    std::vector<std::vector<std::pair<uint32_t, double>>> rows;  // [view*767 + bin]
-};
-
-void goldenc_walk(GcRows &out, bool adjoint) {
-    const int N = 512, NB = 767, NV = 180, MID = 383, TILE = 128, TX = 4, NT = 16;
-    out.rows.assign((size_t)NV * NB, {});
-    auto put = [&](int view, int bin, int r, int c, double w) {
-        out.rows[(size_t)view * NB + bin].emplace_back((uint32_t)(r * N + c), w);
-    };
+template <typename Put>
+void goldenc_emit(bool adjoint, Put &&put) {
+    const int N = 512, NB = 767, MID = 383, TILE = 128, TX = 4, NT = 16;
     for (int ray = 0; ray < NB; ++ray) {
         int k = ray - NB / 2;
         if (k > N / 2 - 1 || k < 1 - N / 2) continue;
-        for (int c = 0; c < N; ++c) put(0, ray, N / 2 - 1 + k, c, 1.0);
+        for (int c = 0; c < N; ++c) put(0, ray, (N / 2 - 1 + k) * N + c, 1.0);
     }
     for (int ray = 0; ray < NB; ++ray) {
         int k = ray - NB / 2;
         if (k > N / 2 - 1 || k < 1 - N / 2) continue;
         int col = adjoint ? N / 2 - 1 + k : N / 2 + k;
-        for (int r = 0; r < N; ++r) put(90, ray, r, col, 1.0);
+        for (int r = 0; r < N; ++r) put(90, ray, r * N + col, 1.0);
     }
     auto eight = [&](int al, int ray, int ix, int iy, int dx, int dy, double w) {
         const int W = N;
-        put(al, ray, iy + dy, ix + dx, w);
-        put(90 + al, ray, W - 1 - ix - dx, iy + dy, w);
+        auto px = [&](int r, int c) { return r * N + c; };
+        put(al, ray, px(iy + dy, ix + dx), w);
+        put(90 + al, ray, px(W - 1 - ix - dx, iy + dy), w);
         if (ray != MID) {
-            put(al, NB - 1 - ray, W - 1 - iy - dy, W - 1 - ix - dx, w);
-            put(90 + al, NB - 1 - ray, ix + dx, W - 1 - iy - dy, w);
+            put(al, NB - 1 - ray, px(W - 1 - iy - dy, W - 1 - ix - dx), w);
+            put(90 + al, NB - 1 - ray, px(ix + dx, W - 1 - iy - dy), w);
         }
         if (al != 45) {
-            put(90 - al, ray, ix + dx, iy + dy, w);
-            put(180 - al, ray, W - 1 - iy - dy, ix + dx, w);
+            put(90 - al, ray, px(ix + dx, iy + dy), w);
+            put(180 - al, ray, px(W - 1 - iy - dy, ix + dx), w);
             if (ray != MID) {
-                put(90 - al, NB - 1 - ray, W - 1 - ix - dx, W - 1 - iy - dy, w);
-                put(180 - al, NB - 1 - ray, iy + dy, W - 1 - ix - dx, w);
+                put(90 - al, NB - 1 - ray, px(W - 1 - ix - dx, W - 1 - iy - dy), w);
+                put(180 - al, NB - 1 - ray, px(iy + dy, W - 1 - ix - dx), w);
             }
         }
     };
@@ -201,6 +196,17 @@ void goldenc_walk(GcRows &out, bool adjoint) {
                 }
             }
     }
+}
+
+struct GcRows {
+    std::vector<std::vector<std::pair<uint32_t, double>>> rows;  // [view*767 + bin]
+};
+
+void goldenc_walk(GcRows &out, bool adjoint) {
+    out.rows.assign((size_t)180 * 767, {});
+    goldenc_emit(adjoint, [&](int view, int bin, int pix, double w) {
+        out.rows[(size_t)view * 767 + bin].emplace_back((uint32_t)pix, w);
+    });
 }
 
 // Host CSR in subset row order from per-(view, bin) lists.
@@ -405,7 +411,8 @@ std::vector<int> subset_order(int S) {
 
 size_t Matrix::bytes() const {
     return otf.bytes() + d_vlist.bytes() + d_sub_row0.bytes() + rowptr.bytes() + idx.bytes() + len32.bytes() + len64.bytes() + colptr.bytes() +
-           crow.bytes() + clen32.bytes() + clen64.bytes() + perm.bytes() + fs.bytes() + bs.bytes();
+           crow.bytes() + clen32.bytes() + clen64.bytes() + perm.bytes() + fs.bytes() + bs.bytes() +
+           seq_bytes(seq.get());
 }
 
 std::unique_ptr<Matrix> build_matrix(const mi_geometry &g, int S, bool want64, hipStream_t st, bool otf) {
@@ -413,6 +420,8 @@ std::unique_ptr<Matrix> build_matrix(const mi_geometry &g, int S, bool want64, h
     MI_REQUIRE(S >= 1 && S <= g.n_views, MI_E_ARG, "n_subsets must be in [1, n_views]");
     auto m = std::make_unique<Matrix>();
     m->g = g;
+    m->view_deg = view_table(g);
+    m->g.view_deg = m->view_deg.empty() ? nullptr : m->view_deg.data();
     m->S = S;
     m->npix = g.nx * g.ny;
     m->nrows = (int64_t)g.n_views * g.n_bins;
@@ -486,9 +495,10 @@ std::unique_ptr<Matrix> build_matrix(const mi_geometry &g, int S, bool want64, h
         // load_table of helical/rebin/src/ms2d.cpp:30-62: source angle in
         // degrees -> cos(angle / 180 * pi); detector element j (from
         // -n_bins/2) -> cos((j + OFF_CENTER) * LENGTH_PER_DET * pi / 180 + pi)
+        // (the angle of view v: the slice's _pos table, or uniform over span_deg)
         std::vector<double> c(g.n_views), s(g.n_views), dc(g.n_bins), ds(g.n_bins);
         for (int v = 0; v < g.n_views; ++v) {
-            const double angle = g.span_deg * (double)v / (double)g.n_views;
+            const double angle = g.view_deg ? g.view_deg[v] : g.span_deg * (double)v / (double)g.n_views;
             c[v] = std::cos(angle / 180.0 * M_PI);
             s[v] = std::sin(angle / 180.0 * M_PI);
         }
@@ -543,6 +553,27 @@ std::unique_ptr<Matrix> build_matrix(const mi_geometry &g, int S, bool want64, h
     subset_counts(*m, st);
     if (otf && (g.beam == MI_BEAM_FAN_FLAT || g.beam == MI_BEAM_FAN_ARC)) build_otf(*m, a, st);
     return m;
+}
+
+void goldenc_lists(bool pixel_major, std::vector<int64_t> &ptr, std::vector<uint32_t> &other,
+                   std::vector<double> &len) {
+    const int64_t nitems = pixel_major ? (int64_t)512 * 512 : (int64_t)180 * 767;
+    ptr.assign(nitems + 1, 0);
+    // Count pass, then a fill pass that appends each item's entries in the
+    // order the walk emits them.
+    goldenc_emit(pixel_major, [&](int view, int bin, int pix, double) {
+        ++ptr[1 + (pixel_major ? pix : (int64_t)view * 767 + bin)];
+    });
+    for (int64_t i = 0; i < nitems; ++i) ptr[i + 1] += ptr[i];
+    other.resize(ptr[nitems]);
+    len.resize(ptr[nitems]);
+    std::vector<int64_t> at(ptr.begin(), ptr.end() - 1);
+    goldenc_emit(pixel_major, [&](int view, int bin, int pix, double w) {
+        const int64_t row = (int64_t)view * 767 + bin;
+        const int64_t k = at[pixel_major ? pix : row]++;
+        other[k] = (uint32_t)(pixel_major ? row : pix);
+        len[k] = w;
+    });
 }
 
 }  // namespace mi

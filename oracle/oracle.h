@@ -95,12 +95,14 @@ typedef struct or_fan_geom {
     int arc;              /* 1: equiangular detector of helical/rebin
                              (ms2d.cpp:50-56,172-183); bin_mm then holds
                              LENGTH_PER_DET in degrees, det_offset OFF_CENTER */
+    const double *view_deg; /* arc: per-view source angles in degrees (the
+                             slice's _pos table, ms2d.cpp:40-47), or NULL */
 } or_fan_geom;
 
 /* Per-view trig tables: theta_a = (span_deg * (pi / 180)) * a / nviews,
  * glibc cos/sin (the expression the device library's host code uses); arc:
- * angle_a = span_deg * a / nviews degrees, cos(angle_a / 180 * pi) as
- * load_table does (helical/rebin/src/ms2d.cpp:44-46). */
+ * angle_a = view_deg[a] (when given) or span_deg * a / nviews degrees,
+ * cos(angle_a / 180 * pi) as load_table does (helical/rebin/src/ms2d.cpp:44-46). */
 void or_fan_tables(const or_fan_geom *g, double *cos_t, double *sin_t);
 void or_fan_endpoints(const or_fan_geom *g, const double *cos_t, const double *sin_t,
                       int view, int bin, double *sx, double *sy, double *dx, double *dy);

@@ -165,8 +165,9 @@ void or_fan_tables(const or_fan_geom *g, double *cos_t, double *sin_t)
 {
     for (int a = 0; a < g->nviews; ++a) {
         if (g->arc) {
-            /* helical/rebin/src/ms2d.cpp:44-46 (angle in degrees) */
-            double angle = g->span_deg * (double)a / (double)g->nviews;
+            /* helical/rebin/src/ms2d.cpp:44-46 (angle in degrees, from the
+             * slice's table when there is one) */
+            double angle = g->view_deg ? g->view_deg[a] : g->span_deg * (double)a / (double)g->nviews;
             cos_t[a] = cos(angle / 180.0 * 3.14159265358979323846);
             sin_t[a] = sin(angle / 180.0 * 3.14159265358979323846);
             continue;

@@ -17,7 +17,8 @@ _lib = None
 class FanGeom(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("vx", C.c_double), ("vy", C.c_double),
                 ("nviews", C.c_int), ("nbins", C.c_int), ("bin_mm", C.c_double), ("sod", C.c_double),
-                ("sdd", C.c_double), ("det_offset", C.c_double), ("span_deg", C.c_double), ("arc", C.c_int)]
+                ("sdd", C.c_double), ("det_offset", C.c_double), ("span_deg", C.c_double), ("arc", C.c_int),
+                ("view_deg", C.POINTER(C.c_double))]
 
 
 class Op(C.Structure):
@@ -99,9 +100,12 @@ def arc_geom(n, pixel_mm, views, bins, deg_per_bin, sod=595.0, sdd=1085.6, off_c
 
 
 def fan_from(g):
-    """FanGeom matching a medimage_amd.Geometry (flat or arc fan kind)."""
-    return FanGeom(g.nx, g.ny, g.pixel_mm, g.pixel_mm, g.n_views, g.n_bins, g.bin_mm, g.sod_mm, g.sdd_mm,
-                   g.det_offset, g.span_deg, 1 if g.beam == 3 else 0)
+    """FanGeom matching a medimage_amd.Geometry (flat or arc fan kind, with
+    the per-view angle table when it has one)."""
+    fg = FanGeom(g.nx, g.ny, g.pixel_mm, g.pixel_mm, g.n_views, g.n_bins, g.bin_mm, g.sod_mm, g.sdd_mm,
+                 g.det_offset, g.span_deg, 1 if g.beam == 3 else 0, g.view_deg)
+    fg._views = getattr(g, "_views", None)   # keeps the table alive
+    return fg
 
 
 def siddon2d(sx, sy, dx, dy, nx, ny, vx, vy, cap=8192):

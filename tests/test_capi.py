@@ -30,6 +30,26 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+@pytest.mark.parametrize("lib,header", [("libmedimage_goldenc.so", "medimage_goldenc.h"),
+                                        ("libmedimage_msbeam.so", "medimage_msbeam.h")])
+def test_compat_libraries_export_the_reference_surface(lib, header):
+    """The GoldenC / msbeam compatibility libraries export every function
+    and global their headers declare, under the reference's own names."""
+    import medimage_amd as mi
+    text = re.sub(r"/\*.*?\*/", "", (ROOT / "include" / header).read_text(), flags=re.S)
+    funcs = re.findall(r"^void\s+(\w+)\s*\(", text, flags=re.M)
+    data = [n for decl in re.findall(r"^extern\s+double\s+([^;]+);", text, flags=re.M)
+            for n in re.findall(r"(\w+)\s*\[", decl)]
+    assert funcs
+    out = subprocess.run(["nm", "-D", "--defined-only", str(mi.LIB_PATH.parent / lib)], capture_output=True,
+                         text=True).stdout
+    exported = {line.split()[-1]: line.split()[-2] for line in out.splitlines() if len(line.split()) >= 3}
+    for n in funcs:
+        assert exported.get(n) == "T", (n, exported.get(n))
+    for n in data:
+        assert exported.get(n) in ("B", "D"), (n, exported.get(n))
+
+
 def test_library_is_gfx950_code_object():
     import medimage_amd as mi
     out = subprocess.run(["/opt/rocm/lib/llvm/bin/llvm-readelf", "-S", str(mi.LIB_PATH)], capture_output=True,

@@ -83,3 +83,45 @@ def test_read_dr_roundtrip(tmp_path):
         mi.read_dr(f, 4, 5, 7)          # file too short
     with pytest.raises(mi.MIError):
         mi.read_dr(tmp_path / "missing.dr", 1, 1, 1)
+
+
+def test_helical_slice_pos_and_raw_files(tmp_path):
+    """A rebinned slice's table and sinogram as helical/rebin writes and reads
+    them (ms2d.cpp:40-47, 270-274): "pos,angle" lines, whitespace-separated
+    sinogram values; the values come back exactly."""
+    rng = np.random.default_rng(3)
+    pos = np.round(rng.uniform(-50, 50, 37), 6)
+    ang = np.sort(rng.uniform(0, 720, 37))
+    with open(tmp_path / "4_pos", "w") as f:
+        for a, b in zip(pos, ang):
+            f.write(f"{float(a)!r},{float(b)!r}\n")
+    p2, a2 = mi.read_pos(tmp_path / "4_pos")
+    assert np.array_equal(p2, pos) and np.array_equal(a2, ang)
+    sino = rng.random((37, 12))
+    with open(tmp_path / "4_raw", "w") as f:
+        for row in sino:
+            f.write(" ".join(repr(float(x)) for x in row) + "\n")
+    assert np.array_equal(mi.read_raw(tmp_path / "4_raw", 37, 12), sino)
+    # a short sinogram is an error, as is a missing table
+    with pytest.raises(mi.MIError) as e:
+        mi.read_raw(tmp_path / "4_raw", 38, 12)
+    assert e.value.code == mi.MI_E_ARG
+    with pytest.raises(mi.MIError):
+        mi.read_pos(tmp_path / "5_pos")
+    # the table stops at the first line that does not parse (load_table's loop)
+    with open(tmp_path / "6_pos", "w") as f:
+        f.write("1.0,2.0\n3.0,4.0\nend\n5.0,6.0\n")
+    assert np.array_equal(mi.read_pos(tmp_path / "6_pos")[1], [2.0, 4.0])
+
+
+def test_geometry_view_table_copy():
+    """Geometry.with_views keeps its own table; the key tells tables apart."""
+    g = mi.arc_geometry(64, 4.0, 6, 96, 0.3)
+    a = np.array([0.0, 10.0, 25.0, 70.0, 200.0, 300.5])
+    h = g.with_views(a)
+    a[0] = 99.0
+    assert h.view_deg[0] == 0.0 and h.view_deg[5] == 300.5
+    assert h.key() != g.key() and h.key() != g.with_views(a).key()
+    assert h.key() == g.with_views([0.0, 10.0, 25.0, 70.0, 200.0, 300.5]).key()
+    with pytest.raises(ValueError):
+        g.with_views([1.0, 2.0])
