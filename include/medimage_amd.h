@@ -174,6 +174,28 @@ int mi_autotune(mi_context *ctx, const mi_geometry *g, int n_subsets, mi_tuning 
 int mi_set_tuning(mi_context *ctx, const mi_geometry *g, int n_subsets, const mi_tuning *t);
 int mi_get_tuning(mi_context *ctx, const mi_geometry *g, int n_subsets, mi_tuning *out);
 
+/* Launch settings of the AT engine (jobs, batches, serving) for one
+ * (geometry, subsets): how many jobs share a unit (one stream of the
+ * geometry's matrix serves K interleaved jobs) and whether the back
+ * projection stages its residual values in LDS.  mi_engine_autotune times
+ * every candidate on the device (synthetic jobs, hipGraph replay) and keeps
+ * the fastest for the geometry; batches then pack its jobs accordingly.
+ * Neither setting changes a job's arithmetic: results are bitwise the same
+ * as with the defaults.  The shapes that fix a job's summation order (rays
+ * per forward block, lanes per back-projection pixel) are not tuned; they
+ * are part of the result's definition.  The reference has no such tuner. */
+typedef struct mi_engine_tuning {
+    int32_t k_cap;            /* jobs per unit: 1, 2, 4 or 8; 0 = the default
+                                 rule (K images within one XCD's L2)          */
+    int32_t bp_lds;           /* -1 default rule (LDS up to 24 KiB of slots),
+                                 0 global gathers, 1 LDS up to 56 KiB         */
+    double  us_per_job_pass;  /* measured: one AT iteration of one job        */
+} mi_engine_tuning;
+int mi_engine_autotune(mi_context *ctx, const mi_geometry *g, int n_subsets, int precision, mi_engine_tuning *out);
+int mi_engine_set_tuning(mi_context *ctx, const mi_geometry *g, int n_subsets, int precision,
+                         const mi_engine_tuning *t);
+int mi_engine_get_tuning(mi_context *ctx, const mi_geometry *g, int n_subsets, int precision, mi_engine_tuning *out);
+
 /* ---- projectors (replace A / AStar) --------------------------------------
  * forward:     sino[view][bin] = sum over the ray of img[x][y] * len
  *              (origin/GoldenC/src/proj.c:14-204, msbeam proj.c:14-32)

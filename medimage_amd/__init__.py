@@ -27,6 +27,7 @@ LIB_PATH = Path(os.environ.get("MEDIMAGE_AMD_LIB") or str(_PKG / "libmedimage_am
 MI_OK, MI_E_ARG, MI_E_DEVICE, MI_E_NOMEM, MI_E_STATE, MI_E_LIMIT = 0, -1, -2, -3, -4, -5
 MI_BEAM_FAN_FLAT, MI_BEAM_PARALLEL, MI_BEAM_GOLDENC, MI_BEAM_FAN_ARC = 0, 1, 2, 3
 MI_FP32, MI_FP64 = 0, 1
+MI_ORDER_REFERENCE, MI_ORDER_PARALLEL = 0, 1
 MI_K_FP, MI_K_BP, MI_K_FPSQ, MI_K_STEP, MI_K_EDGE, MI_K_CONE_FP, MI_K_CONE_BP = 0, 1, 2, 3, 4, 5, 6
 MI_BATCH_GRAPH, MI_BATCH_MIN_K, MI_BATCH_RESERVE_MB, MI_BATCH_MAX_K, MI_BATCH_MAX_ENTRIES, MI_BATCH_BP_LDS = \
     0, 1, 2, 3, 4, 5
@@ -106,6 +107,10 @@ class Tuning(C.Structure):
                 ("fp_us", C.c_double), ("bp_us", C.c_double)]
 
 
+class EngineTuning(C.Structure):
+    _fields_ = [("k_cap", C.c_int32), ("bp_lds", C.c_int32), ("us_per_job_pass", C.c_double)]
+
+
 class KernelStats(C.Structure):
     _fields_ = [("launches", C.c_int64), ("total_ms", C.c_double), ("bytes", C.c_double)]
 
@@ -177,6 +182,9 @@ def _load():
         "mi_set_tuning": [P, G, C.c_int, C.POINTER(Tuning)],
         "mi_get_tuning": [P, G, C.c_int, C.POINTER(Tuning)],
         "mi_matrix_export": [P, G, P, P, P, C.c_int64],
+        "mi_engine_autotune": [P, G, C.c_int, C.c_int, C.POINTER(EngineTuning)],
+        "mi_engine_set_tuning": [P, G, C.c_int, C.c_int, C.POINTER(EngineTuning)],
+        "mi_engine_get_tuning": [P, G, C.c_int, C.c_int, C.POINTER(EngineTuning)],
         "mi_forward": [P, G, P, P],
         "mi_backproject": [P, G, P, P],
         "mi_forward_f64": [P, G, P, P],
@@ -351,6 +359,19 @@ class Context:
         _check(self._lib.mi_trace_ray(self._h, C.byref(g), view, bin, _ptr(ind), _ptr(ln), cap, C.byref(n)))
         k = min(n.value, cap)
         return ind[:k].copy(), ln[:k].copy()
+
+    def engine_autotune(self, g: Geometry, n_subsets: int, precision: int = MI_FP32) -> EngineTuning:
+        t = EngineTuning()
+        _check(self._lib.mi_engine_autotune(self._h, C.byref(g), n_subsets, precision, C.byref(t)))
+        return t
+
+    def engine_tuning(self, g: Geometry, n_subsets: int, precision: int = MI_FP32, set_to: EngineTuning = None):
+        """The geometry's engine settings; with set_to, installs them first."""
+        if set_to is not None:
+            _check(self._lib.mi_engine_set_tuning(self._h, C.byref(g), n_subsets, precision, C.byref(set_to)))
+        t = EngineTuning()
+        _check(self._lib.mi_engine_get_tuning(self._h, C.byref(g), n_subsets, precision, C.byref(t)))
+        return t
 
     def matrix(self, g: Geometry):
         """(rowptr, ind, len) of the one-subset matrix, rays in [view][bin] order."""

@@ -421,8 +421,8 @@ void Batch<T>::rebuild() {
         D.bs_voff = m.bs.voff.p;
         D.bs_vtab0 = m.bs.d_vtab0.p;
         D.nbins = m.g.n_bins;
-        const size_t need = m.bs.slot16.p ? (size_t)m.bs.max_slots * U.K * sizeof(T) : kBpLdsMax + 1;
-        const bool in_lds = use_bp_lds && need <= kBpLdsMax;
+        const size_t need = m.bs.slot16.p ? (size_t)m.bs.max_slots * U.K * sizeof(T) : kBpLdsHw + 1;
+        const bool in_lds = bp_in_lds(m, need);
         if (in_lds) {
             lds_need = std::max(lds_need, need);
             kmax_l = std::max(kmax_l, U.K);
@@ -677,7 +677,70 @@ void Batch<T>::release(int64_t id) {
     cancel(id);
 }
 
+// Launch settings of the engine for one geometry: every (jobs per unit,
+// residual staging) candidate runs passes of a unit of synthetic jobs (a
+// constant sinogram; the kernels' work does not depend on the values),
+// replayed from a hipGraph as in a batch, and the fastest time per job and
+// pass is kept on the matrix.  None of the settings changes a job's
+// arithmetic.
+template <typename T>
+mi_engine_tuning engine_autotune(mi_context *ctx, const mi_geometry &g, int S) {
+    std::shared_ptr<Matrix> m = get_engine_matrix(ctx, g, S, sizeof(T) == 8);
+    const hipStream_t st = ctx->stream;
+    const double inf = std::numeric_limits<double>::infinity();
+    std::vector<T> sino((size_t)m->nrows, T(0.5));
+    mi_at_params p{};
+    p.alpha = 1e-3;
+    p.beta = 1e-4;
+    p.epsilon = 1e-2;
+    p.iter_edge = 1;
+    p.n_subsets = S;
+    p.max_iters = 0;   // no cap: every timed pass does its work
+    p.lower = -inf;
+    p.upper = inf;
+    p.precision = sizeof(T) == 8 ? MI_FP64 : MI_FP32;
+    mi_engine_tuning best{0, -1, inf};
+    hipEvent_t e0, e1;
+    MI_HIP(hipEventCreate(&e0));
+    MI_HIP(hipEventCreate(&e1));
+    const int reps = 3;
+    try {
+        for (int K = 1; K <= kMaxK; K *= 2)
+            for (int lds = 0; lds <= 1; ++lds) {
+                if (lds && (!m->bs.slot16.p || (size_t)m->bs.max_slots * K * sizeof(T) > kBpLdsHw)) continue;
+                m->etune.k_cap = K;
+                m->etune.bp_lds = lds;
+                Batch<T> b(ctx, S);
+                b.min_k = b.max_k = K;
+                for (int k = 0; k < K; ++k) b.submit(g, p, sino.data(), nullptr, nullptr);
+                b.admit(false);
+                if (b.n_active() != K) break;   // does not fit: no larger K either
+                b.enqueue(1);                   // warm, graph capture
+                MI_HIP(hipEventRecord(e0, st));
+                b.enqueue(reps);
+                MI_HIP(hipEventRecord(e1, st));
+                MI_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                MI_HIP(hipEventElapsedTime(&ms, e0, e1));
+                const double us = 1e3 * ms / reps / K;
+                if (us < best.us_per_job_pass) best = mi_engine_tuning{K, lds, us};
+            }
+    } catch (...) {
+        m->etune = Matrix::EngineTune{};
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    MI_REQUIRE(best.k_cap > 0, MI_E_NOMEM, "no engine setting fits in device memory");
+    m->etune = Matrix::EngineTune{best.k_cap, best.bp_lds, best.us_per_job_pass};
+    return best;
+}
+
 template struct Batch<float>;
 template struct Batch<double>;
+template mi_engine_tuning engine_autotune<float>(mi_context *, const mi_geometry &, int);
+template mi_engine_tuning engine_autotune<double>(mi_context *, const mi_geometry &, int);
 
 }  // namespace mi

@@ -71,9 +71,17 @@ struct Batch {
     // projectors gather from, stay within one XCD's 4 MiB L2 (K = 4 at 512^2
     // FP32, 8 at 256^2); at most max_k.
     int k_cap(const Matrix &m) const {
+        if (m.etune.k_cap > 0) return std::min(m.etune.k_cap, max_k);
         int k = max_k;
         while (k > 1 && (int64_t)k * m.npix_t * (int64_t)sizeof(T) > (int64_t)(4 << 20)) k >>= 1;
         return k;
+    }
+    // Back projection staged in LDS for unit U of matrix m needing `need`
+    // bytes of slots (MI_BATCH_BP_LDS off: never).
+    bool bp_in_lds(const Matrix &m, size_t need) const {
+        if (!use_bp_lds) return false;
+        if (m.etune.bp_lds == 0) return false;
+        return need <= (m.etune.bp_lds == 1 ? kBpLdsHw : kBpLdsMax);
     }
     bool use_graph = true;
     bool auto_retire = true;
@@ -150,5 +158,10 @@ struct Batch {
     void drop_graph();
     void ensure_stage(int64_t n);
 };
+
+// Times the engine's launch settings for (g, S) and stores the fastest on
+// the geometry's engine matrix (Matrix::etune).
+template <typename T>
+mi_engine_tuning engine_autotune(mi_context *ctx, const mi_geometry &g, int S);
 
 }  // namespace mi

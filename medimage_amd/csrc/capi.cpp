@@ -566,6 +566,38 @@ int mi_set_tuning(mi_context *ctx, const mi_geometry *g, int n_subsets, const mi
     });
 }
 
+int mi_engine_autotune(mi_context *ctx, const mi_geometry *g, int n_subsets, int precision, mi_engine_tuning *out) {
+    return guarded([&] {
+        check_ctx(ctx);
+        MI_REQUIRE(g && out && n_subsets >= 1, MI_E_ARG, "bad arguments");
+        MI_REQUIRE(precision == MI_FP32 || precision == MI_FP64, MI_E_ARG, "bad precision");
+        *out = precision == MI_FP64 ? engine_autotune<double>(ctx, *g, n_subsets)
+                                    : engine_autotune<float>(ctx, *g, n_subsets);
+    });
+}
+
+int mi_engine_set_tuning(mi_context *ctx, const mi_geometry *g, int n_subsets, int precision,
+                         const mi_engine_tuning *t) {
+    return guarded([&] {
+        check_ctx(ctx);
+        MI_REQUIRE(g && t && n_subsets >= 1, MI_E_ARG, "bad arguments");
+        MI_REQUIRE(t->k_cap == 0 || t->k_cap == 1 || t->k_cap == 2 || t->k_cap == 4 || t->k_cap == 8, MI_E_ARG,
+                   "k_cap must be 0, 1, 2, 4 or 8");
+        MI_REQUIRE(t->bp_lds >= -1 && t->bp_lds <= 1, MI_E_ARG, "bp_lds must be -1, 0 or 1");
+        Matrix &m = *get_engine_matrix(ctx, *g, n_subsets, precision == MI_FP64);
+        m.etune = Matrix::EngineTune{t->k_cap, t->bp_lds, t->us_per_job_pass};
+    });
+}
+
+int mi_engine_get_tuning(mi_context *ctx, const mi_geometry *g, int n_subsets, int precision, mi_engine_tuning *out) {
+    return guarded([&] {
+        check_ctx(ctx);
+        MI_REQUIRE(g && out && n_subsets >= 1, MI_E_ARG, "bad arguments");
+        const Matrix &m = *get_engine_matrix(ctx, *g, n_subsets, precision == MI_FP64);
+        *out = mi_engine_tuning{m.etune.k_cap, m.etune.bp_lds, m.etune.us_per_job_pass};
+    });
+}
+
 int mi_get_tuning(mi_context *ctx, const mi_geometry *g, int n_subsets, mi_tuning *out) {
     return guarded([&] {
         check_ctx(ctx);

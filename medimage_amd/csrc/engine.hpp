@@ -29,6 +29,10 @@ constexpr int kPixBlock = kBlock;    // tiled pixel positions per stencil block
 #define MI_BP_LDS_KB 24
 #endif
 constexpr size_t kBpLdsMax = (size_t)MI_BP_LDS_KB << 10;
+// The most a tuned geometry may stage (within the 64 KiB a launch gets
+// without opting in, with room for the kernel's static LDS).
+constexpr size_t kBpLdsHw = (size_t)56 << 10;
+static_assert(kBpLdsMax <= kBpLdsHw, "default staging limit above the tuned one");
 
 // Partial-sum slots of a unit (each K * cap doubles, see UnitDesc).
 enum EngSlot { SL_DF = 0, SL_DD, SL_AD, SL_NV, SL_E0, SL_E1, SL_E2, SL_E3, SL_COUNT };

@@ -215,6 +215,16 @@ struct Matrix {
     int64_t npix_t = 0;
     bool has_sell = false;
     bool has_csr = true;             // false once the engine layout replaced it
+    // AT engine launch settings of this geometry (mi_engine_autotune):
+    // jobs per unit (0: the L2 rule of Batch::k_cap) and the back
+    // projection's residual staging (-1: LDS when the slots fit the default
+    // limit, 0: global gathers, 1: LDS up to the hardware limit).  Neither
+    // changes a job's arithmetic, so results stay bitwise the same.
+    struct EngineTune {
+        int k_cap = 0;
+        int bp_lds = -1;
+        double us_per_job_pass = 0.0;
+    } etune;
     std::shared_ptr<SeqMatrix> seq;  // GoldenC-form drivers, built on first use
 
     size_t bytes() const;

@@ -287,3 +287,32 @@ def test_batch_admission_refuses_without_memory(ctx):
     assert a == 0 and q == 0
     assert b.status(jid).state == mi.MI_JOB_FAILED
     b.close()
+
+
+def test_engine_autotune_keeps_results_bitwise(ctx):
+    """The engine tuner picks jobs per unit and the residual staging per
+    geometry; every setting it may pick gives the default settings' results
+    bit for bit, and batches pack the geometry's jobs by the chosen K."""
+    jobs = [j for j in _jobs(ctx) if j[0] is GEOMS["c"]]
+    g = GEOMS["c"]
+    base = _run_batch(ctx, jobs)
+    t = ctx.engine_autotune(g, 4)
+    print(f"[engine tune] {g}: K {t.k_cap}, LDS {t.bp_lds}, {t.us_per_job_pass:.1f} us per job and pass",
+          flush=True)
+    assert t.k_cap in (1, 2, 4, 8) and t.bp_lds in (0, 1) and t.us_per_job_pass > 0
+    got = ctx.engine_tuning(g, 4)
+    assert (got.k_cap, got.bp_lds) == (t.k_cap, t.bp_lds)
+    try:
+        for k, lds in [(t.k_cap, t.bp_lds), (1, 0), (2, 1), (8, 0), (8, 1)]:
+            ctx.engine_tuning(g, 4, set_to=mi.EngineTuning(k, lds, 0.0))
+            b = ctx.batch(4)
+            for gj, sino, p in jobs:
+                b.submit(gj, sino, p)
+            b.step(0)
+            assert max(b.units()) <= k
+            b.close()
+            res = _run_batch(ctx, jobs)
+            for i in base:
+                assert _same(res[i][0], base[i][0]) and _same(res[i][1], base[i][1]), (k, lds, i)
+    finally:
+        ctx.engine_tuning(g, 4, set_to=mi.EngineTuning(0, -1, 0.0))
