@@ -445,6 +445,19 @@ int mi_cone_forward_dev(mi_context *ctx, const mi_cone_geometry *g, const float 
  * final piece shorter than 0.05 voxel, tracing.cpp:268). */
 int mi_cone_backproject(mi_context *ctx, const mi_cone_geometry *g, const float *sino, float *vol);
 int mi_cone_backproject_dev(mi_context *ctx, const mi_cone_geometry *g, const float *sino, float *vol);
+/* mi_cone_backproject_dev without any device allocation.  It has no
+ * counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The double sums the back projection carries between its
+ * launches of 36 views live in a fixed array of the library (64 MiB on each
+ * device, held until the process ends): with more than 36 views the volume
+ * may hold at most 8388608 voxels (MI_E_LIMIT above that).  The geometry's
+ * tables must be on the device already (any earlier mi_cone_* call with the
+ * same geometry on this context builds them), else MI_E_STATE.  The call is
+ * queued on the context's stream and returns without waiting; the result is
+ * in vol once the stream has run it.  Calls that overlap on one device
+ * (other contexts, other threads) share the array and race. */
+int mi_cone_backproject_dev_noalloc(mi_context *ctx, const mi_cone_geometry *g, const float *sino, float *vol);
 /* Ordered-subset AT reconstruction of the volume: the OS form of
  * mi_at_reconstruct on [nz][nx][ny] with the projectors above and the
  * 26-neighbourhood widening of GoldenC's stencils (13 forward offsets k,

@@ -231,6 +231,7 @@ def _load():
         "mi_cone_forward_dev": [P, CG, P, P],
         "mi_cone_backproject": [P, CG, P, P],
         "mi_cone_backproject_dev": [P, CG, P, P],
+        "mi_cone_backproject_dev_noalloc": [P, CG, P, P],
         "mi_cone_at_reconstruct": [P, CG, C.POINTER(AtParams), P, P, P, C.POINTER(Report)],
         "mi_cone_at_reconstruct_dev": [P, CG, C.POINTER(AtParams), P, P, P, C.POINTER(Report)],
         "mi_cone_config_parse": [C.c_char_p, CG, C.POINTER(AtParams)],

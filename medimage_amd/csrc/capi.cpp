@@ -458,6 +458,17 @@ void cone_backproject_dev(mi_context *ctx, const mi_cone_geometry *g, const floa
     stats_collect(ctx);
 }
 
+// The same back projection with no device allocation and no wait: the
+// geometry's tables must be cached already, the carried sums use the
+// device's fixed scratch array.
+void cone_backproject_dev_noalloc(mi_context *ctx, const mi_cone_geometry *g, const float *sino, float *vol) {
+    MI_REQUIRE(g && vol && sino, MI_E_ARG, "null argument");
+    validate_cone(*g);
+    MI_REQUIRE(ctx->cone && std::memcmp(&ctx->cone->g, g, sizeof(*g)) == 0, MI_E_STATE,
+               "the geometry's tables are not on the device: make one cone call with this geometry first");
+    cone_bp_views(ctx->stream, *ctx->cone, ctx->cone->all.p, ctx->cone->d.nviews, sino, vol, ctx->device);
+}
+
 size_t cone_voxels(const mi_cone_geometry &g) { return (size_t)g.nx * g.ny * g.nz; }
 size_t cone_rays(const mi_cone_geometry &g) { return (size_t)g.n_views * g.n_rows * g.n_cols; }
 
@@ -482,6 +493,8 @@ int mi_context_create(int device, mi_context **out) {
         MI_HIP(hipGetDeviceProperties(&prop, device));
         MI_REQUIRE(std::string(prop.gcnArchName).rfind("gfx950", 0) == 0, MI_E_DEVICE,
                    std::string("library is built for gfx950, device is ") + prop.gcnArchName);
+        // the device's address of the fixed scratch array (once per device)
+        cone_scratch_init(device);
         auto c = new mi_context();
         c->device = device;
         hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -1410,6 +1423,10 @@ int mi_cone_backproject(mi_context *ctx, const mi_cone_geometry *g, const float 
 
 int mi_cone_backproject_dev(mi_context *ctx, const mi_cone_geometry *g, const float *sino, float *vol) {
     return guarded([&] { cone_backproject_dev(check_ctx(ctx), g, sino, vol); });
+}
+
+int mi_cone_backproject_dev_noalloc(mi_context *ctx, const mi_cone_geometry *g, const float *sino, float *vol) {
+    return guarded([&] { cone_backproject_dev_noalloc(check_ctx(ctx), g, sino, vol); });
 }
 
 

@@ -41,6 +41,8 @@
 // double sums carried between chunks in a scratch volume.
 #include "cone.hpp"
 
+#include <mutex>
+
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -659,29 +661,64 @@ size_t cone_fp_views(hipStream_t st, const ConeTables &t, const int *views, int 
     return nb;
 }
 
-void cone_bp_views(hipStream_t st, const ConeTables &t, const int *views, int nv, const float *sino, float *vol) {
+// The allocation-free back projection's carry (mi_cone_backproject_dev_noalloc):
+// one copy per device, for the life of the process.
+constexpr size_t kScratchCarry = (size_t)8 << 20;   // doubles (64 MiB): volumes up to 8 Mi voxels
+static __device__ __align__(256) double scratch[kScratchCarry];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::once_flag g_scratch_once[kMaxDevices];
+double *g_scratch_ptr[kMaxDevices];
+hipError_t g_scratch_err[kMaxDevices];
+}  // namespace
+
+void cone_scratch_init(int device) {
+    MI_REQUIRE(device >= 0 && device < kMaxDevices, MI_E_LIMIT, "device index beyond the scratch table");
+    // The array's name is a host-side stand-in: its device address (one per
+    // device) comes from the runtime, with that device current.
+    std::call_once(g_scratch_once[device], [device] {
+        g_scratch_err[device] = hipGetSymbolAddress((void **)&g_scratch_ptr[device], HIP_SYMBOL(scratch));
+    });
+    MI_HIP(g_scratch_err[device]);
+}
+
+int64_t cone_scratch_voxels() { return (int64_t)kScratchCarry; }
+
+void cone_bp_views(hipStream_t st, const ConeTables &t, const int *views, int nv, const float *sino, float *vol,
+                   int device_scratch) {
     const size_t blocks = (size_t)((t.d.nx + 15) / 16) * ((t.d.ny + 15) / 16) * ((t.d.nz + kZc - 1) / kZc);
     // Views in chunks whose sinogram rows stay cached while every workgroup
-    // sweeps them; double sums carried between chunks.
+    // sweeps them; double sums carried between chunks (a buffer allocated
+    // here, or the device's fixed scratch array).
     const int chunks = (nv + kBpViewChunk - 1) / kBpViewChunk;
-    DevBuf<double> carry(chunks > 1 ? (size_t)t.d.nx * t.d.ny * t.d.nz : 0);
+    const size_t nvox = (size_t)t.d.nx * t.d.ny * t.d.nz;
+    DevBuf<double> own(chunks > 1 && device_scratch < 0 ? nvox : 0);
+    double *carry = own.p;
+    if (chunks > 1 && device_scratch >= 0) {
+        MI_REQUIRE(nvox <= kScratchCarry, MI_E_LIMIT,
+                   "volume of " + std::to_string(nvox) + " voxels exceeds the allocation-free back projection's " +
+                       std::to_string(kScratchCarry) + "-voxel scratch");
+        carry = g_scratch_ptr[device_scratch];
+        MI_REQUIRE(carry, MI_E_STATE, "device scratch not initialised");
+    }
     for (int c = 0; c < chunks; ++c) {
         const int v0 = c * kBpViewChunk, n = std::min(kBpViewChunk, nv - v0);
         if (t.d.parallel)
             launch(k_cone_bp<true, false>, dim3(grid8(blocks)), dim3(kBlock), 0u, st, t.d, t.cs.p,
-                               t.rows.p, t.vinv.p, t.vl2.p, t.vsrc.p, t.vpos.p, views + v0, n, sino, carry.p, c == 0,
+                               t.rows.p, t.vinv.p, t.vl2.p, t.vsrc.p, t.vpos.p, views + v0, n, sino, carry, c == 0,
                                c == chunks - 1, vol);
         else if (t.d.nrows <= kBpRowLds)
             launch(k_cone_bp<false, true>, dim3(grid8(blocks)), dim3(kBlock),
                                (uint32_t)(t.d.nrows * sizeof(double)), st, t.d, t.cs.p, t.rows.p, t.vinv.p, t.vl2.p,
-                               t.vsrc.p, t.vpos.p, views + v0, n, sino, carry.p, c == 0, c == chunks - 1, vol);
+                               t.vsrc.p, t.vpos.p, views + v0, n, sino, carry, c == 0, c == chunks - 1, vol);
         else
             launch(k_cone_bp<false, false>, dim3(grid8(blocks)), dim3(kBlock), 0u, st, t.d, t.cs.p,
-                               t.rows.p, t.vinv.p, t.vl2.p, t.vsrc.p, t.vpos.p, views + v0, n, sino, carry.p, c == 0,
+                               t.rows.p, t.vinv.p, t.vl2.p, t.vsrc.p, t.vpos.p, views + v0, n, sino, carry, c == 0,
                                c == chunks - 1, vol);
         MI_HIP(hipGetLastError());
     }
-    if (chunks > 1) MI_HIP(hipStreamSynchronize(st));   // before the carry buffer is freed
+    if (own.p) MI_HIP(hipStreamSynchronize(st));   // before the carry buffer is freed
 }
 
 void cone_forward(hipStream_t st, const ConeTables &t, const float *vol, float *sino) {

@@ -50,7 +50,14 @@ enum { CFP_WRITE = 0, CFP_RESID = 1, CFP_SQ = 2 };
 size_t cone_fp_views(hipStream_t st, const ConeTables &t, const int *views, int nv, int mode, const float *vol,
                      const float *meas, float *out, double *part);
 // Gather back projection over the view list (overwrites vol).
-void cone_bp_views(hipStream_t st, const ConeTables &t, const int *views, int nv, const float *sino, float *vol);
+// device_scratch >= 0: carry the sums in that device's fixed scratch array
+// (no allocation, no wait; volumes up to cone_scratch_voxels()) instead of
+// a buffer allocated for the call.
+void cone_bp_views(hipStream_t st, const ConeTables &t, const int *views, int nv, const float *sino, float *vol,
+                   int device_scratch = -1);
+// Looks up the device address of the scratch array of `device` (current).
+void cone_scratch_init(int device);
+int64_t cone_scratch_voxels();
 
 // Ordered-subset AT reconstruction of the volume (cone_at.hip): sino, u and
 // v (initial values in, results out) on the host, or on the device when dev.
