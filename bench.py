@@ -106,7 +106,45 @@ def c1_gpu(ctx, iters):
             "elapsed_ms": rep.elapsed_ms}, sino
 
 
-def bench_c3(ctx, n_jobs, iters, log, plain_pass=True):
+def bench_goldenc(ctx, log, cpu=True):
+    """The GoldenC drop-in at the reference program's own configuration
+    (origin/GoldenC/src/config.h: 512 x 512, 180 views x 767 bins,
+    MumfordShah(1000, 0.002, 1e-4) with 10 x (10 CG + 10 SD)), in the
+    reference's summation order: device time of the whole run, and whether
+    its f and v equal the reference program's (tests/golden/goldenc_ms.npz)
+    bit for bit.  CPU leg: one alternation of the oracle's restatement on
+    this host (the reference's single-threaded loop order)."""
+    import medimage_amd as mi
+    gold = np.load(ROOT / "tests" / "golden" / "goldenc_ms.npz")
+    f0 = np.load(ROOT / "tests" / "golden" / "blockrecon_std.npz")["f"].astype(np.float64)
+    g = mi.goldenc_geometry()
+    sino = ctx.forward(g, f0, np.float64)
+    warm = mi.at_params(1000.0, 0.002, 1e-4, iter_alt=1, iter_image=2, iter_edge=2, precision=mi.MI_FP64)
+    ctx.mumford_shah(g, sino, warm)        # reference-order lists, code
+    p = mi.at_params(1000.0, 0.002, 1e-4, iter_alt=10, iter_image=10, iter_edge=10, precision=mi.MI_FP64)
+    t0 = time.perf_counter()
+    f, v, rep = ctx.mumford_shah(g, sino, p)
+    wall = time.perf_counter() - t0
+    same = bool(np.array_equal(f.view(np.uint64), gold["f"].view(np.uint64)) and
+                np.array_equal(v.view(np.uint64), gold["v"].view(np.uint64)))
+    log(f"[goldenc] 10 alternations: {rep.elapsed_ms:.0f} ms on the device, bitwise = reference: {same}")
+    out = {"workload": "GoldenC MumfordShah(1000, 0.002, 1e-4), 512x512, 180 views x 767 bins, 10 x (10 CG + 10 SD), "
+                       "FP64, reference summation order",
+           "device_ms": rep.elapsed_ms, "wall_s": wall, "alternations_per_s": 10.0 / (rep.elapsed_ms * 1e-3),
+           "bitwise_equal_reference_output": same,
+           "sino_bitwise_equal_reference_gDelta": bool(np.array_equal(sino.view(np.uint64), gold["g"].view(np.uint64)))}
+    if cpu:
+        sys.path.insert(0, str(ROOT / "tests"))
+        import oracle_py as O
+        t0 = time.perf_counter()
+        O.mumford_shah(O.op_goldenc(), sino, 1000.0, 0.002, 1e-4, 1, 10, 10)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": 1.0 / dt, "unit": "alternations/s", "cores": 1, "kind": "port",
+                               "sample": f"one alternation (10 CG + 10 SD) of the oracle's restatement, {dt:.1f} s"}
+    return out
+
+
+def bench_c3(ctx, n_jobs, iters, log, plain_pass=True, tune=True):
     """Batched form: n_jobs mixed fan-beam slices (sizes, views, bins and I0
     drawn from the seed), 8 subsets, `iters` AT iterations each, run (a) one
     at a time through the plain per-job form and (b) all together in one
@@ -127,6 +165,15 @@ def bench_c3(ctx, n_jobs, iters, log, plain_pass=True):
     for k, g in geoms.items():
         nnz[k] = ctx.prepare(g, 8)
     log(f"[c3] {n_jobs} jobs over {len(geoms)} geometries prepared in {time.perf_counter() - t:.1f} s")
+    # launch settings per geometry from the engine tuner (both passes use them)
+    t = time.perf_counter()
+    tuned = {}
+    if tune:
+        for k, g in geoms.items():
+            tt = ctx.engine_autotune(g, 8)
+            tuned[f"{g.nx}x{g.n_views}x{g.n_bins}"] = {"k_cap": tt.k_cap, "bp_lds": tt.bp_lds,
+                                                       "us_per_job_pass": tt.us_per_job_pass}
+        log(f"[c3] tuned {len(geoms)} geometries in {time.perf_counter() - t:.1f} s")
     p = configs.at_params(configs.ALPHA, configs.BETA, configs.EPS, n_subsets=8, max_iters=iters)
     rays = sum(g.n_views * g.n_bins for g, _, _ in jobs)
 
@@ -169,6 +216,7 @@ def bench_c3(ctx, n_jobs, iters, log, plain_pass=True):
         "batched_rays_per_s": rays * iters / t_batch,
         "jobs_bitwise_identical": same if plain_pass else None,
         "max_rel_diff": worst,
+        "tuning": tuned,
     }
 
 
@@ -210,6 +258,91 @@ def bench_serving(ctx, n_jobs, rate, log):
                               res["batched"]["results"][i][0].view(np.uint32)) for i in range(n_jobs))
     out["jobs_bitwise_identical"] = int(same)
     out["throughput_ratio"] = res["batched"]["iters_per_s"] / res["plain"]["iters_per_s"]
+    return out
+
+
+def _c5_sino(args):
+    """Worker: one request's noisy analytic sinogram (float32)."""
+    from medimage_amd import configs
+    n, views, bins, i0, seed = args
+    g = configs.mixed_geometry(n, views, bins)
+    return configs.analytic_case(g, i0, seed).astype(np.float32)
+
+
+def bench_serving_full(ctx, n_jobs, rate, log, tune=True, workers=16):
+    """The whole C5 stream (n_jobs Poisson arrivals at `rate` jobs/s) served
+    live by the continuous-batching request loop.  The plain one-job-at-a-
+    time form of the same stream is not run (it would take as long as its
+    total work); its first-come-first-served latencies are derived from each
+    geometry's measured per-pass time of the plain form (one job alone, the
+    same kernels) times each job's iteration count, which is the same alone
+    and in the mix (results are bitwise identical), queued in arrival
+    order.  Geometries are tuned first (mi_engine_autotune) and their
+    matrices built before the clock starts; sinograms are exact ellipse line
+    integrals with Poisson noise, made on `workers` host processes."""
+    from concurrent.futures import ProcessPoolExecutor
+    import medimage_amd as mi
+    from medimage_amd import configs, serve
+    spec = configs.c5_requests(n_jobs, seed=5, rate=rate)
+    t = time.perf_counter()
+    geoms = {}
+    for r in spec:
+        g = configs.mixed_geometry(r["n"], r["views"], r["bins"])
+        geoms.setdefault(g.key(), g)
+    nnz = sum(ctx.prepare(g, 8) for g in geoms.values())
+    log(f"[serve-full] {len(geoms)} geometries ({nnz / 1e9:.2f}e9 entries) prepared in {time.perf_counter() - t:.1f} s")
+    tuned = {}
+    if tune:
+        for k, g in geoms.items():
+            tt = ctx.engine_autotune(g, 8)
+            tuned[k] = (tt.k_cap, tt.bp_lds, tt.us_per_job_pass)
+        log(f"[serve-full] tuned: " + ", ".join(f"{g.nx}^2/{g.n_views}: K{v[0]} L{v[1]}"
+                                               for k, g in geoms.items() for v in [tuned[k]]))
+    # plain per-pass time of each geometry: one job alone, 3 passes
+    plain_pass_ms = {}
+    for k, g in geoms.items():
+        s0 = np.zeros((g.n_views, g.n_bins), np.float32) + np.float32(0.5)
+        _, _, rep = ctx.at_reconstruct(g, s0, configs.at_params(configs.ALPHA, configs.BETA, configs.EPS,
+                                                                n_subsets=8, max_iters=3))
+        plain_pass_ms[k] = rep.elapsed_ms / 3.0
+    t = time.perf_counter()
+    with ProcessPoolExecutor(max_workers=workers) as ex:
+        sinos = list(ex.map(_c5_sino, [(r["n"], r["views"], r["bins"], r["i0"], r["seed"]) for r in spec],
+                            chunksize=8))
+    log(f"[serve-full] {n_jobs} sinograms in {time.perf_counter() - t:.1f} s")
+    reqs = []
+    for r, sino in zip(spec, sinos):
+        g = geoms[configs.mixed_geometry(r["n"], r["views"], r["bins"]).key()]
+        p = configs.at_params(configs.ALPHA, configs.BETA, configs.EPS, n_subsets=8, max_iters=200, rel_tol=1e-4)
+        reqs.append(dict(t=r["t"], geom=g, sino=sino, params=p))
+    del sinos
+    res = serve.run_stream(ctx, reqs, n_subsets=8, mode="batched", log=log, keep_results=False)
+    # plain FCFS derived: service = iterations x the geometry's plain pass time
+    svc = np.array([res["iterations"][i] * plain_pass_ms[reqs[i]["geom"].key()] * 1e-3 for i in range(n_jobs)])
+    arr = np.array([r["t"] for r in reqs])
+    order = np.argsort(arr, kind="stable")
+    free, lat = 0.0, np.zeros(n_jobs)
+    for i in order:
+        start = max(free, arr[i])
+        free = start + svc[i]
+        lat[i] = free - arr[i]
+    iters = float(res["iterations"].sum())
+    out = {"workload": f"the C5 stream: {n_jobs} Poisson arrivals at {rate:g} jobs/s, N in {{256,384,512,768,1024}}, "
+                       "views in {360,720,1152,2304}, bins 1.5N, I0 log-uniform [1e4,1e5], 8 subsets, rel. "
+                       "data-fidelity change < 1e-4 or 200 AT iterations",
+           "geometries": len(geoms), "matrix_entries": nnz,
+           "tuning": {f"{g.nx}x{g.n_views}x{g.n_bins}": {"k_cap": v[0], "bp_lds": v[1], "us_per_job_pass": v[2]}
+                      for k, g in geoms.items() for v in [tuned.get(k, (0, -1, 0.0))]},
+           "batched": {k: res[k] for k in ("makespan_s", "jobs_per_s", "iters_per_s", "latency_p50_s", "latency_p99_s",
+                                          "latency_mean_s", "mean_iterations")},
+           "plain_derived": {"makespan_s": float(free), "jobs_per_s": n_jobs / float(free),
+                             "iters_per_s": iters / float(svc.sum()),
+                             "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
+                             "latency_mean_s": float(lat.mean()),
+                             "how": "first come first served over the measured per-pass time of each geometry's "
+                                    "plain form times each job's iteration count"},
+           "plain_pass_ms": {f"{g.nx}x{g.n_views}x{g.n_bins}": plain_pass_ms[k] for k, g in geoms.items()}}
+    out["throughput_ratio"] = out["batched"]["iters_per_s"] / out["plain_derived"]["iters_per_s"]
     return out
 
 
@@ -282,6 +415,8 @@ def main():
     ap.add_argument("--cpu-passes-c1", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c1", action="store_true")
+    ap.add_argument("--no-goldenc", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="C3: keep the engine's default launch settings")
     ap.add_argument("--c3-jobs", type=int, default=64)
     ap.add_argument("--c3-iters", type=int, default=50)
     ap.add_argument("--no-c3", action="store_true")
@@ -290,6 +425,8 @@ def main():
     ap.add_argument("--serve-jobs", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-full", action="store_true",
+                    help="only the whole C5 stream (--serve-jobs 2000 by default here): batched live, plain derived")
     ap.add_argument("--c4-reps", type=int, default=2)
     ap.add_argument("--c4-iters", type=int, default=2, help="AT passes of the C4 cone-beam solve (0: projectors only)")
     ap.add_argument("--no-c4", action="store_true")
@@ -305,6 +442,12 @@ def main():
     from medimage_amd import configs
 
     ctx = mi.Context(0)
+    if args.serve_full:
+        n = args.serve_jobs if args.serve_jobs != 48 else 2000
+        out = bench_serving_full(ctx, n, args.serve_rate, lambda m: print(m, file=sys.stderr, flush=True))
+        print(json.dumps({"c5_full_stream": out}))
+        ctx.close()
+        return
     g = configs.c2_geometry()
     print(f"[bench] tracing C2 geometry and generating the phantom", file=sys.stderr, flush=True)
     t = time.perf_counter()
@@ -417,6 +560,8 @@ def main():
     sino_c1 = None
     if not args.no_c1:
         out["c1"], sino_c1 = c1_gpu(ctx, 50)
+    if not args.no_goldenc:
+        out["goldenc"] = bench_goldenc(ctx, lambda m: print(m, file=sys.stderr, flush=True), not args.no_cpu)
     if args.copy_gib > 0:
         nb = int(args.copy_gib * (1 << 30)) // 16 * 16
         ms = ctx.device_copy(nb, 3)
@@ -426,7 +571,8 @@ def main():
         out["cone_c4"] = bench_c4(ctx, args.c4_reps, args.c4_iters, lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_c3:
         out["batched"] = bench_c3(ctx, args.c3_jobs, args.c3_iters,
-                                  lambda m: print(m, file=sys.stderr, flush=True), not args.c3_batched_only)
+                                  lambda m: print(m, file=sys.stderr, flush=True), not args.c3_batched_only,
+                                  not args.no_tune)
     if not args.no_serve:
         out["serving"] = bench_serving(ctx, args.serve_jobs, args.serve_rate,
                                        lambda m: print(m, file=sys.stderr, flush=True))

@@ -65,6 +65,9 @@ Batch<T>::Batch(mi_context *c, int n_subsets) : ctx(c), S(n_subsets) {
     MI_HIP(hipMemsetAsync(d_sc.p, 0, d_sc.bytes(), ctx->stream));
     MI_HIP(hipMemcpyAsync(d_jp.p, h_jp.data(), d_jp.bytes(), hipMemcpyHostToDevice, ctx->stream));
     MI_HIP(hipStreamSynchronize(ctx->stream));
+    MI_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    MI_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    MI_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     t0 = std::chrono::steady_clock::now();
 }
 
@@ -72,6 +75,12 @@ template <typename T>
 Batch<T>::~Batch() {
     (void)hipStreamSynchronize(ctx->stream);
     drop_graph();
+    if (side) {
+        (void)hipStreamSynchronize(side);
+        (void)hipStreamDestroy(side);
+    }
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
 }
 
 template <typename T>
@@ -123,10 +132,17 @@ int Batch<T>::n_active() const {
 
 template <typename T>
 int64_t Batch<T>::active_entries() const {
+    // one stream of a unit's matrix serves all of the unit's jobs
     int64_t e = 0;
-    for (const auto &kv : jobs)
-        if (kv.second.state == BJ_ACTIVE && kv.second.m) e += kv.second.m->nnz;
+    for (int uid : live) e += units[uid]->m->nnz;
     return e;
+}
+
+template <typename T>
+bool Batch<T>::has_free_slot(const Matrix *m) const {
+    for (int uid : live)
+        if (units[uid]->m.get() == m && units[uid]->used() < units[uid]->K) return true;
+    return false;
 }
 
 template <typename T>
@@ -229,9 +245,21 @@ void Batch<T>::load_job(BatchJob<T> &j) {
 template <typename T>
 int Batch<T>::admit(bool require) {
     int n = 0;
-    while (!pending.empty()) {
-        const int64_t id = pending.front();
+    // Arrival order, until a job that needs a new unit does not fit (the
+    // entry budget or memory); after that only jobs that fill a free slot
+    // of a unit already on the device join: they share its matrix stream
+    // and its allocated state, so they cost the others little.
+    bool blocked = false;
+    for (auto it = pending.begin(); it != pending.end();) {
+        const int64_t id = *it;
         BatchJob<T> &j = job(id);
+        if (blocked) {
+            if (!j.m) j.m = find_engine_matrix(ctx, j.geometry(), S, sizeof(T) == 8);
+            if (!j.m || !has_free_slot(j.m.get())) {
+                ++it;
+                continue;
+            }
+        }
         try {
             if (!j.m) {
                 try {
@@ -246,7 +274,12 @@ int Batch<T>::admit(bool require) {
             j.m->last_use = ++ctx->tick;
             // the budget keeps the device as full as it needs to be and lets
             // later jobs wait in arrival order instead of sharing it
-            if (max_entries > 0 && n_active() > 0 && active_entries() + j.m->nnz > max_entries) break;
+            if (!has_free_slot(j.m.get()) && max_entries > 0 && n_active() > 0 &&
+                active_entries() + j.m->nnz > max_entries) {
+                blocked = true;
+                ++it;
+                continue;
+            }
             const GeomKey key = make_key(j.geometry(), S);
             int group = 0;
             for (int64_t q : pending) {
@@ -259,18 +292,22 @@ int Batch<T>::admit(bool require) {
             (void)hipGetLastError();
             if (j.uid >= 0) free_slot(j);
             const bool wait = !require && (e.code == MI_E_NOMEM || e.code == MI_E_LIMIT) && n_active() > 0;
-            if (wait) break;
+            if (wait) {
+                blocked = true;
+                ++it;
+                continue;
+            }
             if (require) {
-                pending.pop_front();
+                pending.erase(it);
                 jobs.erase(id);
                 throw;
             }
             j.state = BJ_FAILED;
             j.error = e.what();
-            pending.pop_front();
+            it = pending.erase(it);
             continue;
         }
-        pending.pop_front();
+        it = pending.erase(it);
         ++n;
     }
     if (n) compact();
@@ -364,6 +401,8 @@ void Batch<T>::compact() {
 
 template <typename T>
 void Batch<T>::drop_graph() {
+    // no replay of the graph may still be running when it is destroyed
+    if (gexec) (void)hipStreamSynchronize(ctx->stream);
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
     gexec = nullptr;
@@ -379,12 +418,15 @@ void Batch<T>::rebuild() {
     int64_t off = 0;
     kmax = 1;
     all_walk = true;
-    size_t lds_need = 0;
     edge_iters = 0;
     any_median = false;
-    rows.nunits = bpl.nunits = bpg.nunits = pix.nunits = (int)live.size();
-    rows.start[0] = bpl.start[0] = bpg.start[0] = pix.start[0] = 0;
-    kmax_l = kmax_g = 1;
+    pix.nunits = (int)live.size();
+    pix.start[0] = 0;
+    for (Launch *L : {&rows[0], &rows[1], &bpl[0], &bpl[1], &bpg[0], &bpg[1]}) {
+        *L = Launch{};
+        L->map.nunits = (int)live.size();
+        L->map.start[0] = 0;
+    }
     const bool f64 = sizeof(T) == 8;
     for (size_t i = 0; i < live.size(); ++i) {
         Unit<T> &U = *units[live[i]];
@@ -423,12 +465,11 @@ void Batch<T>::rebuild() {
         D.nbins = m.g.n_bins;
         const size_t need = m.bs.slot16.p ? (size_t)m.bs.max_slots * U.K * sizeof(T) : kBpLdsHw + 1;
         const bool in_lds = bp_in_lds(m, need);
-        if (in_lds) {
-            lds_need = std::max(lds_need, need);
-            kmax_l = std::max(kmax_l, U.K);
-        } else {
-            kmax_g = std::max(kmax_g, U.K);
-        }
+        const int rc = U.K > 1 ? 1 : 0, bc = U.K > 4 ? 1 : 0;   // the unit's classes
+        Launch &B = in_lds ? bpl[bc] : bpg[bc];
+        B.kmax = std::max(B.kmax, U.K);
+        if (in_lds) B.lds = std::max(B.lds, need);
+        rows[rc].kmax = std::max(rows[rc].kmax, U.K);
         D.sub_row0 = m.d_sub_row0.p;
         D.g = U.g.p;
         D.r = U.r.p;
@@ -449,9 +490,11 @@ void Batch<T>::rebuild() {
         D.bs_P = m.bs.P;
         D.job0 = U.uid * kMaxK;
         MI_REQUIRE(m.has_sell && (!f64 || m.has64), MI_E_STATE, "matrix lacks the engine's sliced layout");
-        rows.start[i + 1] = rows.start[i] + (int)fp_blocks(m);
-        bpl.start[i + 1] = bpl.start[i] + (in_lds ? (int)bp_blocks(m) : 0);
-        bpg.start[i + 1] = bpg.start[i] + (in_lds ? 0 : (int)bp_blocks(m));
+        for (int c = 0; c < 2; ++c) {
+            rows[c].map.start[i + 1] = rows[c].map.start[i] + (c == rc ? (int)fp_blocks(m) : 0);
+            bpl[c].map.start[i + 1] = bpl[c].map.start[i] + (in_lds && c == bc ? (int)bp_blocks(m) : 0);
+            bpg[c].map.start[i + 1] = bpg[c].map.start[i] + (!in_lds && c == bc ? (int)bp_blocks(m) : 0);
+        }
         pix.start[i + 1] = pix.start[i] + (int)px_blocks(m);
         kmax = std::max(kmax, U.K);
         all_walk = all_walk && m.has_otf;
@@ -462,7 +505,6 @@ void Batch<T>::rebuild() {
             any_median = any_median || P.median_every > 0;
         }
     }
-    bp_lds = lds_need;
     MI_HIP(hipStreamSynchronize(st));
     if ((int64_t)d_part.n < off) d_part.alloc((size_t)std::max<int64_t>(off, 1));
     if (!desc.empty())
@@ -490,28 +532,58 @@ void Batch<T>::record_pass(bool timed) {
         for (int uid : live) b += f(*units[uid]);
         return b;
     };
+    // The two classes of a phase: class 1 on the side stream, class 0 on
+    // the context's stream, joined before the next phase.  Timed passes run
+    // on one stream (the per-kernel events time whole phases).
+    auto branches = [&](auto &&launch0, auto &&launch1, bool has0, bool has1) {
+        if (has0 && has1 && !timed) {
+            MI_HIP(hipEventRecord(ev_fork, st));
+            MI_HIP(hipStreamWaitEvent(side, ev_fork, 0));
+            launch1(side);
+            launch0(st);
+            MI_HIP(hipEventRecord(ev_join, side));
+            MI_HIP(hipStreamWaitEvent(st, ev_join, 0));
+        } else {
+            if (has0) launch0(st);
+            if (has1) launch1(st);
+        }
+    };
     eng_finalize<T>(st, args, EF_BEGIN, 0);
     for (int s = 0; s < S; ++s) {
         auto nnz_s = [&](const Unit<T> &U) { return (double)U.m->sub_nnz[s]; };
         auto nr_s = [&](const Unit<T> &U) { return (double)(U.m->sub_row0[s + 1] - U.m->sub_row0[s]); };
+        auto fp = [&](bool resid) {
+            auto one = [&](int c) {
+                return [&, c](hipStream_t q) {
+                    if (resid) eng_fp_resid<T>(q, args, rows[c].map, s, rows[c].kmax, all_walk);
+                    else eng_fp_sq<T>(q, args, rows[c].map, s, rows[c].kmax, all_walk);
+                };
+            };
+            branches(one(0), one(1), rows[0].any(), rows[1].any());
+        };
         {
             LaunchTimer lt(ctx, MI_K_FP, sum([&](const Unit<T> &U) {
                 return nnz_s(U) * (4 + sz) + U.m->npix * sz * U.K + 2 * nr_s(U) * sz * U.K;
             }), true);
-            eng_fp_resid<T>(st, args, rows, s, kmax, all_walk);
+            fp(true);
         }
         {
             LaunchTimer lt(ctx, MI_K_BP, sum([&](const Unit<T> &U) {
                 return nnz_s(U) * (4 + sz) + nr_s(U) * sz * U.K + 3.0 * U.m->npix * sz * U.K;
             }), true);
-            if (bpl.start[bpl.nunits] > 0) eng_bp_grad<T>(st, args, bpl, s, kmax_l, bp_lds);
-            if (bpg.start[bpg.nunits] > 0) eng_bp_grad<T>(st, args, bpg, s, kmax_g, 0);
+            auto one = [&](int c) {
+                return [&, c](hipStream_t q) {
+                    if (bpl[c].any()) eng_bp_grad<T>(q, args, bpl[c].map, s, bpl[c].kmax, bpl[c].lds);
+                    if (bpg[c].any()) eng_bp_grad<T>(q, args, bpg[c].map, s, bpg[c].kmax, 0);
+                };
+            };
+            branches(one(0), one(1), bpl[0].any() || bpg[0].any(), bpl[1].any() || bpg[1].any());
         }
         {
             LaunchTimer lt(ctx, MI_K_FPSQ, sum([&](const Unit<T> &U) {
                 return nnz_s(U) * (4 + sz) + U.m->npix * sz * U.K;
             }), true);
-            eng_fp_sq<T>(st, args, rows, s, kmax, all_walk);
+            fp(false);
         }
         {
             LaunchTimer lt(ctx, MI_K_STEP, sum([&](const Unit<T> &U) { return 5.0 * U.m->npix * sz * U.K; }));

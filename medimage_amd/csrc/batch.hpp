@@ -99,16 +99,28 @@ struct Batch {
     std::vector<JobParams> h_jp;
     std::vector<Scalars> h_sc;
     std::vector<int> live;         // uids in table order
-    BlockMap rows{}, pix{};
-    // back projection: units whose residual slots fit kBpLdsMax in the LDS
-    // launch (bpl), the others in the global-gather launch (bpg); a unit
-    // has blocks in exactly one of the two maps
-    BlockMap bpl{}, bpg{};
-    int kmax_l = 1, kmax_g = 1;
+    BlockMap pix{};
+    // Projector launches, each split in two classes of units by K so that
+    // small units do not run with the register budget (and occupancy) of
+    // code compiled for K = 8: forward rows by K = 1 / K >= 2, back
+    // projection by K <= 4 / K = 8.  Back projection: units whose residual
+    // slots fit the staging limit in the LDS launches (bpl), the others in
+    // the global-gather launches (bpg); a unit has blocks in exactly one map
+    // of each phase.  The two classes of a phase run as concurrent branches
+    // (second stream, forked and joined by events; captured as parallel
+    // graph nodes).
+    struct Launch {
+        BlockMap map{};
+        int kmax = 1;
+        size_t lds = 0;
+        bool any() const { return map.start[map.nunits] > 0; }
+    };
+    Launch rows[2], bpl[2], bpg[2];
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     EngArgs<T> args{};
     int kmax = 1;
     bool all_walk = true;   // every unit's forward projector is the checkpointed walk
-    size_t bp_lds = 0;      // dynamic LDS of the back projection (0: global gathers)
     bool use_bp_lds = true; // MI_BATCH_BP_LDS
     int edge_iters = 0;            // max iter_edge over live jobs
     bool any_median = false;
@@ -143,7 +155,8 @@ struct Batch {
     Scalars scalars(int64_t id);
     void read(int64_t id, T *u, T *v);
     int n_active() const;
-    int64_t active_entries() const;   // sum over active jobs of their matrix entries
+    int64_t active_entries() const;   // sum over the units on the device of their matrix entries
+    bool has_free_slot(const Matrix *m) const;   // a unit of m has an empty slot
     int n_pending() const { return (int)pending.size(); }
     int n_units() const { return (int)live.size(); }
 

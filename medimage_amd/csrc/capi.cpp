@@ -107,6 +107,12 @@ std::shared_ptr<Matrix> get_matrix(mi_context *ctx, const mi_geometry &g, int S,
     return m;
 }
 
+std::shared_ptr<Matrix> find_engine_matrix(mi_context *ctx, const mi_geometry &g, int S, bool want64) {
+    auto it = ctx->matrices.find(make_key(g, S, 1));
+    if (it == ctx->matrices.end() || (want64 && !it->second->has64)) return nullptr;
+    return it->second;
+}
+
 std::shared_ptr<Matrix> get_engine_matrix(mi_context *ctx, const mi_geometry &g, int S, bool want64) {
     const GeomKey key = make_key(g, S, 1);
     auto it = ctx->matrices.find(key);

@@ -150,16 +150,31 @@ __device__ __forceinline__ int vidx(const Dims &d, const Tile &t, int z) { retur
 using Nb = float[3][3];
 
 // d = S b + alpha r(u; v); part = sum d^2.
+// The planes a step stages are fetched three steps ahead (one register ring
+// slot per unrolled step), so that their latency overlaps three slices of
+// work.
 __global__ __launch_bounds__(kBlock) void k_cat_grad(Dims dm, const float *__restrict__ b, const float *__restrict__ u,
                                                      const float *__restrict__ v, float S, float alpha, float eta,
                                                      float *__restrict__ dout, double *__restrict__ part) {
     __shared__ float lu[4][kPlane], ls[4][kPlane];  // plane p of u, v^2 + eta in slot (p + 1) & 3
     const Tile t = tile_of(dm);
+    // ring slot of step z: u(z + 1), v(z), b(z)
+    struct Ring {
+        Ahead u, v;
+        float b;
+    };
+    auto load = [&](Ring &r, int z) {
+        r.u = fetch(u, dm, t, z + 1, 0.0f);
+        r.v = fetch(v, dm, t, z, 1.0f);
+        r.b = t.in ? b[vidx(dm, t, z)] : 0.0f;
+    };
     put<false>(lu[t.z0 & 3], t, fetch(u, dm, t, t.z0 - 1, 0.0f), eta);
     put<false>(lu[(t.z0 + 1) & 3], t, fetch(u, dm, t, t.z0, 0.0f), eta);
     put<true>(ls[t.z0 & 3], t, fetch(v, dm, t, t.z0 - 1, 1.0f), eta);
-    Ahead au = fetch(u, dm, t, t.z0 + 1, 0.0f), av = fetch(v, dm, t, t.z0, 1.0f);
-    float ab = t.in ? b[vidx(dm, t, t.z0)] : 0.0f;
+    Ring R[3];
+    load(R[0], t.z0);
+    if (t.z0 + 1 < t.z1) load(R[1], t.z0 + 1);
+    if (t.z0 + 2 < t.z1) load(R[2], t.z0 + 2);
     __syncthreads();
     Nb U0, U1, U2, S0, S1, S2;
     nbhd(U0, lu[t.z0 & 3]);
@@ -167,16 +182,12 @@ __global__ __launch_bounds__(kBlock) void k_cat_grad(Dims dm, const float *__res
     nbhd(S0, ls[t.z0 & 3]);
     double sq = 0.0;
     // slice z from u on z - 1, z, z + 1 and v^2 + eta on z - 1, z; stages
-    // u(z + 1) into up and s(z) into sc, fetches the next slice's entries
-    auto step = [&](int z, const Nb &um, const Nb &uc, Nb &up, const Nb &sm, Nb &sc) {
-        put<false>(lu[(z + 2) & 3], t, au, eta);
-        put<true>(ls[(z + 1) & 3], t, av, eta);
-        const float bz = ab;
-        if (z + 1 < t.z1) {
-            au = fetch(u, dm, t, z + 2, 0.0f);
-            av = fetch(v, dm, t, z + 1, 1.0f);
-            ab = t.in ? b[vidx(dm, t, z + 1)] : 0.0f;
-        }
+    // u(z + 1) into up and s(z) into sc, refills the ring slot for z + 3
+    auto step = [&](int z, const Nb &um, const Nb &uc, Nb &up, const Nb &sm, Nb &sc, Ring &r) {
+        put<false>(lu[(z + 2) & 3], t, r.u, eta);
+        put<true>(ls[(z + 1) & 3], t, r.v, eta);
+        const float bz = r.b;
+        if (z + 3 < t.z1) load(r, z + 3);
         __syncthreads();
         nbhd(up, lu[(z + 2) & 3]);
         nbhd(sc, ls[(z + 1) & 3]);
@@ -204,29 +215,37 @@ __global__ __launch_bounds__(kBlock) void k_cat_grad(Dims dm, const float *__res
         }
     };
     for (int z = t.z0; z < t.z1; z += 3) {
-        step(z, U0, U1, U2, S0, S1);
-        if (z + 1 < t.z1) step(z + 1, U1, U2, U0, S1, S2);
-        if (z + 2 < t.z1) step(z + 2, U2, U0, U1, S2, S0);
+        step(z, U0, U1, U2, S0, S1, R[0]);
+        if (z + 1 < t.z1) step(z + 1, U1, U2, U0, S1, S2, R[1]);
+        if (z + 2 < t.z1) step(z + 2, U2, U0, U1, S2, S0, R[2]);
     }
     const double tot = block_sum(sq);
     if (threadIdx.x == 0) part[blockIdx.x] = tot;
 }
 
 // part = pair sum of p (zero outside) with weights (v^2 + eta) when WV, 1
-// otherwise.
+// otherwise.  The squared differences of a weight class are summed in float
+// (13 terms per voxel), the voxel's total and the workgroup's sum in
+// double.  Planes are fetched two steps ahead.
 template <bool WV>
 __global__ __launch_bounds__(kBlock) void k_cat_pairs(Dims dm, const float *__restrict__ p,
                                                       const float *__restrict__ v, float eta,
                                                       double *__restrict__ part) {
-    __shared__ double lp[2][kPlane];  // plane z in slot z & 1, converted to double once
+    __shared__ float lp[2][kPlane];  // plane z in slot z & 1
     const Tile t = tile_of(dm);
-    auto put_d = [&](double *slot, const Ahead &r) {
-        slot[threadIdx.x] = r.a;
-        if (t.e1 >= 0) slot[t.e1] = r.b;
+    // ring slot of step z: p(z + 1) and v(z)
+    struct Ring {
+        Ahead p;
+        float v;
     };
-    put_d(lp[t.z0 & 1], fetch(p, dm, t, t.z0, 0.0f));
-    Ahead ap = fetch(p, dm, t, t.z0 + 1, 0.0f);
-    float av = WV && t.in ? v[vidx(dm, t, t.z0)] : 0.0f;
+    auto load = [&](Ring &r, int z) {
+        r.p = fetch(p, dm, t, z + 1, 0.0f);
+        r.v = WV && t.in ? v[vidx(dm, t, z)] : 0.0f;
+    };
+    put<false>(lp[t.z0 & 1], t, fetch(p, dm, t, t.z0, 0.0f), 0.0f);
+    Ring R[2];
+    load(R[0], t.z0);
+    if (t.z0 + 1 < t.z1) load(R[1], t.z0 + 1);
     // The pair weights of x whose other voxel x - k lies outside: per weight
     // class |k|^2 = 1, 2, 3 the count of such offsets (b = +-1 leaves through
     // the x faces, c = +-1 through the y faces, and on slice 0 a = 1 through
@@ -248,47 +267,36 @@ __global__ __launch_bounds__(kBlock) void k_cat_pairs(Dims dm, const float *__re
         wout[1] = (c1[0] + 0.5 * c1[1]) + (1.0 / 3.0) * c1[2];
     }
     __syncthreads();
-    double P0[3][3], P1[3][3];
-    auto nbhd_d = [&](double (&n)[3][3], const double *slot) {
-        const double *q = slot + (int)(threadIdx.x / kTy + 1) * kPy + (int)(threadIdx.x % kTy) + 1;
-#pragma unroll
-        for (int b = -1; b <= 1; ++b)
-#pragma unroll
-            for (int c = -1; c <= 1; ++c) n[b + 1][c + 1] = q[b * kPy + c];
-    };
-    nbhd_d(P0, lp[t.z0 & 1]);
+    Nb P0, P1;
+    nbhd(P0, lp[t.z0 & 1]);
     double acc = 0.0;
-    // slice z from p on z, z + 1; stages p(z + 1) into pn, fetches the next
-    // slice's entries
-    auto step = [&](int z, const double (&pc)[3][3], double (&pn)[3][3]) {
-        put_d(lp[(z + 1) & 1], ap);
-        const float vz = av;
-        if (z + 1 < t.z1) {
-            ap = fetch(p, dm, t, z + 2, 0.0f);
-            if (WV) av = t.in ? v[vidx(dm, t, z + 1)] : 0.0f;
-        }
+    // slice z from p on z, z + 1; stages p(z + 1) into pn, refills the ring
+    // slot for z + 2
+    auto step = [&](int z, const Nb &pc, Nb &pn, Ring &r) {
+        put<false>(lp[(z + 1) & 1], t, r.p, 0.0f);
+        const float vz = r.v;
+        if (z + 2 < t.z1) load(r, z + 2);
         __syncthreads();
-        nbhd_d(pn, lp[(z + 1) & 1]);
+        nbhd(pn, lp[(z + 1) & 1]);
         if (t.in) {
-            const double p0 = pc[1][1];
+            const float p0 = pc[1][1];
             const double s0 = WV ? (double)vz * vz + eta : 1.0, so = WV ? 1.0 + eta : 1.0;
-            // per weight class: the sum of (p(x + k) - p(x))^2, exact
-            // differences of floats in double
-            double g[3] = {0.0, 0.0, 0.0};
+            float g[3] = {0.0f, 0.0f, 0.0f};   // per weight class: sum of (p(x + k) - p(x))^2
 #pragma unroll
             for (int k = 0; k < 13; ++k) {
                 int a, b, c;
                 koff(k, a, b, c);
                 const int n = a * a + b * b + c * c - 1;
-                const double q = (a ? pn[1 + b][1 + c] : pc[1 + b][1 + c]) - p0;
-                g[n] = fma(q, q, g[n]);
+                const float q = (a ? pn[1 + b][1 + c] : pc[1 + b][1 + c]) - p0;
+                g[n] = fmaf(q, q, g[n]);
             }
-            acc += s0 * ((g[0] + 0.5 * g[1]) + (1.0 / 3.0) * g[2]) + so * (p0 * p0) * wout[z == 0 ? 1 : 0];
+            acc += s0 * (((double)g[0] + 0.5 * (double)g[1]) + (1.0 / 3.0) * (double)g[2]) +
+                   so * ((double)p0 * p0) * wout[z == 0 ? 1 : 0];
         }
     };
     for (int z = t.z0; z < t.z1; z += 2) {
-        step(z, P0, P1);
-        if (z + 1 < t.z1) step(z + 1, P1, P0);
+        step(z, P0, P1, R[0]);
+        if (z + 1 < t.z1) step(z + 1, P1, P0, R[1]);
     }
     const double tot = block_sum(acc);
     if (threadIdx.x == 0) part[blockIdx.x] = tot;
@@ -333,15 +341,16 @@ __global__ __launch_bounds__(kBlock) void k_cat_edge_n(Dims dm, const float *__r
     __shared__ float lu[2][kPlane];  // plane z in slot z & 1
     const Tile t = tile_of(dm);
     put<false>(lu[t.z0 & 1], t, fetch(u, dm, t, t.z0, 0.0f), 0.0f);
-    Ahead au = fetch(u, dm, t, t.z0 + 1, 0.0f);
+    // ring slot of step z: u(z + 1), fetched two steps ahead
+    Ahead R[2] = {fetch(u, dm, t, t.z0 + 1, 0.0f), fetch(u, dm, t, t.z0 + 2, 0.0f)};
     __syncthreads();
     Nb U0, U1;
     nbhd(U0, lu[t.z0 & 1]);
-    // slice z from u on z, z + 1; stages u(z + 1) into un, fetches the next
-    // slice's entries
-    auto step = [&](int z, const Nb &uc, Nb &un) {
-        put<false>(lu[(z + 1) & 1], t, au, 0.0f);
-        if (z + 1 < t.z1) au = fetch(u, dm, t, z + 2, 0.0f);
+    // slice z from u on z, z + 1; stages u(z + 1) into un, refills the ring
+    // slot for z + 2
+    auto step = [&](int z, const Nb &uc, Nb &un, Ahead &r) {
+        put<false>(lu[(z + 1) & 1], t, r, 0.0f);
+        if (z + 2 < t.z1) r = fetch(u, dm, t, z + 3, 0.0f);
         __syncthreads();
         nbhd(un, lu[(z + 1) & 1]);
         if (t.in) {
@@ -358,8 +367,8 @@ __global__ __launch_bounds__(kBlock) void k_cat_edge_n(Dims dm, const float *__r
         }
     };
     for (int z = t.z0; z < t.z1; z += 2) {
-        step(z, U0, U1);
-        if (z + 1 < t.z1) step(z + 1, U1, U0);
+        step(z, U0, U1, R[0]);
+        if (z + 1 < t.z1) step(z + 1, U1, U0, R[1]);
     }
 }
 
@@ -371,24 +380,32 @@ __global__ __launch_bounds__(kBlock) void k_cat_edge_dir(Dims dm, const float *_
                                                          double *__restrict__ p1) {
     __shared__ float lv[4][kPlane];  // plane p in slot (p + 1) & 3
     const Tile t = tile_of(dm);
+    // ring slot of step z: v(z + 1) and n(z), fetched three steps ahead
+    struct Ring {
+        Ahead v;
+        float n;
+    };
+    auto load = [&](Ring &r, int z) {
+        r.v = fetch(v, dm, t, z + 1, 1.0f);
+        r.n = t.in ? nn[vidx(dm, t, z)] : 0.0f;
+    };
     put<false>(lv[t.z0 & 3], t, fetch(v, dm, t, t.z0 - 1, 1.0f), 0.0f);
     put<false>(lv[(t.z0 + 1) & 3], t, fetch(v, dm, t, t.z0, 1.0f), 0.0f);
-    Ahead av = fetch(v, dm, t, t.z0 + 1, 1.0f);
-    float an = t.in ? nn[vidx(dm, t, t.z0)] : 0.0f;
+    Ring R[3];
+    load(R[0], t.z0);
+    if (t.z0 + 1 < t.z1) load(R[1], t.z0 + 1);
+    if (t.z0 + 2 < t.z1) load(R[2], t.z0 + 2);
     __syncthreads();
     Nb V0, V1, V2;
     nbhd(V0, lv[t.z0 & 3]);
     nbhd(V1, lv[(t.z0 + 1) & 3]);
     double a0 = 0.0, a1 = 0.0;
-    // slice z from v on z - 1, z, z + 1; stages v(z + 1) into vp, fetches
-    // the next slice's entries
-    auto step = [&](int z, const Nb &vm, const Nb &vc, Nb &vp) {
-        put<false>(lv[(z + 2) & 3], t, av, 0.0f);
-        const float n = an;
-        if (z + 1 < t.z1) {
-            av = fetch(v, dm, t, z + 2, 1.0f);
-            an = t.in ? nn[vidx(dm, t, z + 1)] : 0.0f;
-        }
+    // slice z from v on z - 1, z, z + 1; stages v(z + 1) into vp, refills
+    // the ring slot for z + 3
+    auto step = [&](int z, const Nb &vm, const Nb &vc, Nb &vp, Ring &r) {
+        put<false>(lv[(z + 2) & 3], t, r.v, 0.0f);
+        const float n = r.n;
+        if (z + 3 < t.z1) load(r, z + 3);
         __syncthreads();
         nbhd(vp, lv[(z + 2) & 3]);
         if (t.in) {
@@ -409,9 +426,9 @@ __global__ __launch_bounds__(kBlock) void k_cat_edge_dir(Dims dm, const float *_
         }
     };
     for (int z = t.z0; z < t.z1; z += 3) {
-        step(z, V0, V1, V2);
-        if (z + 1 < t.z1) step(z + 1, V1, V2, V0);
-        if (z + 2 < t.z1) step(z + 2, V2, V0, V1);
+        step(z, V0, V1, V2, R[0]);
+        if (z + 1 < t.z1) step(z + 1, V1, V2, V0, R[1]);
+        if (z + 2 < t.z1) step(z + 2, V2, V0, V1, R[2]);
     }
     double s = block_sum(a0);
     if (threadIdx.x == 0) p0[blockIdx.x] = s;

@@ -270,6 +270,8 @@ namespace mi {
 std::shared_ptr<Matrix> get_matrix(mi_context *ctx, const mi_geometry &g, int S, bool want64);
 // The same geometry in the AT engine's sliced layout only (CSR / CSC freed).
 std::shared_ptr<Matrix> get_engine_matrix(mi_context *ctx, const mi_geometry &g, int S, bool want64);
+// The cached engine matrix of (g, S), or null (nothing is built).
+std::shared_ptr<Matrix> find_engine_matrix(mi_context *ctx, const mi_geometry &g, int S, bool want64);
 // Frees cached matrices that no job holds, least recently used first, until
 // at least `bytes` were freed (all idle ones when bytes < 0).  Returns bytes freed.
 int64_t evict_idle(mi_context *ctx, int64_t bytes);

@@ -16,7 +16,8 @@ import numpy as np
 from . import MI_BATCH_MAX_ENTRIES, MI_FP32
 
 
-def run_stream(ctx, requests, n_subsets=8, mode="batched", passes_per_step=1, log=None, max_entries=400_000_000):
+def run_stream(ctx, requests, n_subsets=8, mode="batched", passes_per_step=1, log=None, max_entries=400_000_000,
+               keep_results=True):
     """requests: list of dicts with keys t (arrival, seconds from start),
     geom, sino, params.  Returns a dict with per-job latency, iterations and
     the totals.  max_entries (batched mode): admission budget in matrix
@@ -54,7 +55,8 @@ def run_stream(ctx, requests, n_subsets=8, mode="batched", passes_per_step=1, lo
             st = b.status(jid)
             lat[i] = done_at - requests[i]["t"]
             iters[i] = st.iterations
-            results[i] = b.read(jid)
+            if keep_results:
+                results[i] = b.read(jid)
             b.release(jid)
             finished += 1
             if log and finished % 10 == 0:

@@ -187,21 +187,29 @@ def test_batch_walk_and_sliced_units_mixed_bitwise(ctx):
 
 
 def test_batch_entry_budget_and_max_k(ctx):
-    """MI_BATCH_MAX_ENTRIES admits one job at a time when every matrix is
-    over the budget (the rest wait in arrival order); MI_BATCH_MAX_K caps the
-    unit size.  Results stay bitwise those of the job alone."""
-    jobs = _jobs(ctx)[4:8]                   # four jobs of geometry c
+    """MI_BATCH_MAX_ENTRIES: a job that needs a new unit waits while the
+    units on the device already stream that many matrix entries, and the
+    jobs behind it wait in arrival order, except those that fill a free slot
+    of a unit already on the device (they share its matrix stream).
+    MI_BATCH_MAX_K caps the unit size.  Results stay bitwise those of the
+    job alone."""
+    allj = _jobs(ctx)
+    a0, b0, c0, c1, c2 = allj[0], allj[3], allj[4], allj[5], allj[7]
+    for jobs, want_active in (([a0, b0, c0, c1], 1),      # a0 alone; b0 and then c need new units
+                              ([c0, a0, c1, c2], 3)):     # c's unit (K=4) takes c1, c2 past a0
+        alone = _alone(ctx, jobs)
+        b = ctx.batch(4)
+        b.set(mi.MI_BATCH_MAX_ENTRIES, 1)
+        ids = [b.submit(*j) for j in jobs]
+        a, q = b.step(1)
+        assert (a, q) == (want_active, len(jobs) - want_active)
+        b.run()
+        for i, jid in enumerate(ids):
+            u, v = b.read(jid)
+            assert _same(u, alone[i][0]) and _same(v, alone[i][1]), f"budget job {i}"
+        b.close()
+    jobs = allj[4:8]                          # four jobs of geometry c
     alone = _alone(ctx, jobs)
-    b = ctx.batch(4)
-    b.set(mi.MI_BATCH_MAX_ENTRIES, 1)
-    ids = [b.submit(*j) for j in jobs]
-    a, q = b.step(1)
-    assert a == 1 and q == 3
-    b.run()
-    for i, jid in enumerate(ids):
-        u, v = b.read(jid)
-        assert _same(u, alone[i][0]) and _same(v, alone[i][1]), f"budget job {i}"
-    b.close()
     b = ctx.batch(4)
     b.set(mi.MI_BATCH_MAX_K, 2)
     ids = [b.submit(*j) for j in jobs]

@@ -13,6 +13,8 @@ small geometries the gather is also compared voxel by voxel with the
 transpose of the float walk itself (the reference's scatter), within the
 bound the dropped pieces set.
 """
+from pathlib import Path
+
 import numpy as np
 import pytest
 
@@ -257,15 +259,20 @@ def test_cone_at_odd_volume_and_unaligned_buffers(ctx):
 
 def test_cone_geometry_beyond_index_width_rejected(ctx):
     """A volume under 2^31 voxels whose z nx exceeds 2^24 (the forward walk's
-    24-bit offset multiplies) is refused with MI_E_LIMIT before any device
-    work; the pointers are never dereferenced."""
+    24-bit offset multiplies) is refused with MI_E_LIMIT.  The call is
+    mi_cone_trace_ray, which works on buffers of its own, so no caller
+    pointer reaches the device whatever the check decides; at z nx = 2^24
+    exactly (the largest offset is 2^24 - 1) the geometry is accepted."""
     import ctypes as C
-    g = mi.cone_geometry(4096, 64, 4096, 4, 4, 4, 0.1, 1.0, 600.0, 1000.0)
-    dummy = np.zeros(4, np.float32)
-    ptr = dummy.ctypes.data
-    rc = ctx._lib.mi_cone_forward_dev(ctx._h, C.byref(g), C.c_void_p(ptr), C.c_void_p(ptr))
+    n = C.c_int()
+    g = mi.cone_geometry(4097, 64, 4096, 4, 4, 4, 0.1, 1.0, 600.0, 1000.0)
+    rc = ctx._lib.mi_cone_trace_ray(ctx._h, C.byref(g), 0, 0, 0, None, None, 0, C.byref(n))
     assert rc == mi.MI_E_LIMIT
     assert b"2^24" in ctx._lib.mi_last_error()
+    g = mi.cone_geometry(4096, 64, 4096, 4, 4, 4, 0.1, 1.0, 600.0, 1000.0)
+    rc = ctx._lib.mi_cone_trace_ray(ctx._h, C.byref(g), 0, 0, 0, None, None, 0, C.byref(n))
+    assert rc == mi.MI_OK, ctx._lib.mi_last_error()
+    ctx.trim()
 
 
 def test_cone_tables_in_context_memory(ctx):
@@ -378,3 +385,44 @@ def test_cone_c4_full_size(ctx):
     rhs = float(np.dot(vol_h.astype(np.float64), bty.reshape(-1).astype(np.float64)))
     _print(f"C4 adjoint: <Ax,y> {lhs:.9e} <x,Bty> {rhs:.9e} rel {abs(lhs - rhs) / abs(rhs):.2e}")
     assert abs(lhs - rhs) / abs(rhs) < 1e-5
+
+
+def test_cone_config_and_dr_file_into_device_solve(ctx, tmp_path):
+    """industry3d's own configuration (tests/golden/industry3d_cone_taiji.json,
+    BEAM "Cone") scaled down by four on every axis with the field of view
+    kept, parsed by mi_cone_config_parse; its projections written as a .dr
+    file (1024-byte header, ushort [NPROJ][NDX][NDY], then the blank scan;
+    industry3d/src/ct3d.cpp:199-214) and read back by mi_read_dr; then the
+    device's ordered-subset AT solve from them (ALPHA, BETA, EPSILON and
+    LOWER_X / UPPER_X from the file) against the FP64 oracle, within 1e-5."""
+    import json
+    cfg = json.loads((Path(__file__).parent / "golden" / "industry3d_cone_taiji.json").read_text())
+    for k in ("NX", "NY", "NZ", "NDX", "NDY"):
+        cfg[k] //= 4
+    cfg["NPROJ"] = 24
+    cfg["SAMPLESIZE"] *= 4
+    cfg["PIXELSIZE"] *= 4
+    g, p = mi.cone_config_parse(json.dumps(cfg))
+    assert (g.nx, g.nz, g.n_views, g.n_rows, g.n_cols) == (32, 32, 24, 64, 64)
+    assert (p.alpha, p.beta, p.epsilon, p.lower, p.upper) == (2.0, 0.001, 0.125, 0.0, 10.0)
+    rng = np.random.default_rng(12)
+    vol = np.zeros((g.nz, g.nx, g.ny))
+    vol[8:24, 6:26, 10:22] = 0.02 * (1.0 + rng.random((16, 20, 12)))
+    line = O.cone_fp(O.cone_from(g), vol)
+    counts = np.round(line * 1000.0).astype(np.uint16)          # the file's integer projections
+    flat = np.full((g.n_rows, g.n_cols), 1000.0, np.float32)
+    dr = tmp_path / "taiji_small.dr"
+    dr.write_bytes(bytes(1024) + counts.tobytes() + flat.tobytes())
+    proj, fl = mi.read_dr(dr, g.n_views, g.n_rows, g.n_cols)
+    assert np.array_equal(proj, counts) and np.array_equal(fl, flat)
+    sino = (proj.astype(np.float64) / 1000.0).astype(np.float32)   # the caller's unit conversion
+    p.n_subsets = 4
+    p.max_iters = 2
+    u, v, rep = ctx.cone_at_reconstruct(g, sino, p)
+    shape = (g.nz, g.nx, g.ny)
+    ou, ov, n, _ = O.cone_os_at(O.cone_from(g), sino, np.zeros(shape), np.ones(shape), p.alpha, p.beta, p.epsilon,
+                                4, 2, lower=p.lower, upper=p.upper)
+    _print(f"taiji scaled: {rep.iterations} passes, rel L2 u {rel_l2(u, ou):.2e} v {rel_l2(v, ov):.2e}")
+    assert rep.iterations == n == 2
+    assert rel_l2(u, ou) < 1e-5 and rel_max(u, ou) < 1e-5
+    assert rel_l2(v, ov) < 1e-5 and rel_max(v, ov) < 1e-5

@@ -349,6 +349,23 @@ def test_c2_fp32_vs_fp64(ctx):
     assert rel_l2(v32, v64) < 1e-5
 
 
+def test_c2_fp64_matches_cpu_oracle(ctx):
+    """The full-FP64 device run that the benchmark's accuracy statement uses
+    as its reference, checked at the C2 size itself (512^2, 720 views, 736
+    bins, 12 subsets) against the CPU FP64 oracle: two AT iterations, within
+    1e-10."""
+    g = configs.c2_geometry()
+    _, sino = _case(ctx, g, seed=2, i0=2e4)
+    s64 = sino.astype(np.float32).astype(np.float64)
+    p = configs.c2_params(max_iters=2, precision=mi.MI_FP64)
+    u, v, rep = ctx.at_reconstruct(g, s64, p)
+    ou, ov, n, hist = O.os_at(O.op_fan(O.fan_from(g)), s64, configs.ALPHA, configs.BETA, configs.EPS, 12, 2)
+    print(f"[c2 fp64 vs oracle] rel_l2 u {rel_l2(u, ou):.3e} v {rel_l2(v, ov):.3e}", flush=True)
+    assert rep.iterations == n == 2
+    assert rel_l2(u, ou) < 1e-10 and rel_max(u, ou) < 1e-10 and rel_l2(v, ov) < 1e-10
+    assert abs(rep.data_fidelity - hist[-1]) <= 1e-10 * hist[-1]
+
+
 def test_errors_are_reported(ctx):
     bad = mi.fan_geometry(64, 1.0, 10, 16, 1.0, sod=500.0, sdd=400.0)
     with pytest.raises(mi.MIError) as e:
