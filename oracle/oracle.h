@@ -25,9 +25,14 @@
  *   or_medfilt2 / or_gc_*         scipy median_filter and finite-difference
  *                                 checks of the energy/gradient pair.
  *   or_msbeam_solve               compiled reference msbeam/CPU_OpenMP (1 thread).
- *   or_os_at                      reduces to or_gc_image_step's first step at
- *                                 one subset; exact line search checked against
- *                                 a brute-force minimisation along the direction.
+ *   or_os_at / or_os_step         reduces to or_gc_image_step's first step at
+ *                                 one subset; at S = 4 the direction is the
+ *                                 steepest descent of S|g_s - A_s u|^2 + alpha R
+ *                                 (central differences of a dense-matrix numpy
+ *                                 energy) and the step its exact minimiser along
+ *                                 d (parabola vertex; alpha = 0 in any state,
+ *                                 alpha > 0 where the reference's nV is R's
+ *                                 curvature along d); a pass is its subset steps.
  *   or_cone_trace / or_cone_walk  closed forms (axis-aligned rays, chord length,
  *                                 crossing count) and exact-rational enumeration
  *                                 of the voxel sequence; the reference's
@@ -192,6 +197,13 @@ typedef struct or_at_params {
 void or_subset_order(int S, int *order);
 /* Views of subset s (in the order above): v with v % S == order[s]. */
 int or_subset_views(int nviews, int S, int s, int *views);
+/* One subset step of or_os_at on subset s (views as or_subset_views):
+ * r = g_s - A_s u, d = S A_s^T r + the regulariser term of or_gc_ugrad_reg,
+ * c = |d|^2 / (S |A_s d|^2 + alpha nV(d)), u <- clamp(u + c d).  d_out
+ * (ni values, may be NULL) receives d; |r|^2 is added to *df (may be NULL).
+ * Returns c. */
+double or_os_step(const or_op *op, const double *g, double *u, const double *v,
+                  const or_at_params *p, int s, double *d_out, double *df);
 /* Runs up to max_iters AT iterations from u, v.  df_hist[k] receives the
  * data fidelity sum_s ||g_s - A_s u||^2 of pass k.  Returns passes run. */
 int or_os_at(const or_op *op, const double *g, double *u, double *v,

@@ -274,53 +274,62 @@ int or_subset_views(int nviews, int S, int s, int *views)
     return n;
 }
 
-int or_os_at(const or_op *op, const double *g, double *u, double *v,
-             const or_at_params *p, double *df_hist)
+double or_os_step(const or_op *op, const double *g, double *u, const double *v,
+                  const or_at_params *p, int s, double *d_out, double *df)
 {
     const int h = op->h, w = op->w, S = p->n_subsets, nb = op->nbins;
     const size_t ni = (size_t)h * w, ns = (size_t)op->nviews * nb;
     int *views = (int *)malloc(sizeof(int) * op->nviews);
     double *q = (double *)malloc(sizeof(double) * ns);
     double *b = (double *)malloc(sizeof(double) * ni);
-    double *d = (double *)malloc(sizeof(double) * ni);
+    double *d = d_out ? d_out : (double *)malloc(sizeof(double) * ni);
+    const int nv = or_subset_views(op->nviews, S, s, views);
+    double num = 0, ad2 = 0, den, c, fid = 0;
+    double *acc = df ? df : &fid;    /* added in view order, as the pass sums it */
+
+    or_op_fp(op, views, nv, u, q);
+    for (int k = 0; k < nv; ++k)
+        for (int j = 0; j < nb; ++j) {
+            size_t i = (size_t)views[k] * nb + j;
+            q[i] = g[i] - q[i];
+            *acc += dsq(q[i]);
+        }
+    or_op_bp(op, views, nv, q, b);
+    for (size_t i = 0; i < ni; ++i) d[i] = S * b[i];
+    or_gc_ugrad_reg(d, u, v, p->alpha, p->eps, h, w);
+    or_op_fp(op, views, nv, d, q);
+    for (int k = 0; k < nv; ++k)
+        for (int j = 0; j < nb; ++j) ad2 += dsq(q[(size_t)views[k] * nb + j]);
+    for (size_t i = 0; i < ni; ++i) num += dsq(d[i]);
+    den = S * ad2 + p->alpha * or_gc_nV(d, v, p->eps, h, w);
+    c = (den > 0) ? num / den : 0.0;
+    for (size_t i = 0; i < ni; ++i) {
+        double x = u[i] + c * d[i];
+        if (x < p->lower) x = p->lower;
+        if (x > p->upper) x = p->upper;
+        u[i] = x;
+    }
+    free(views); free(q); free(b);
+    if (!d_out) free(d);
+    return c;
+}
+
+int or_os_at(const or_op *op, const double *g, double *u, double *v,
+             const or_at_params *p, double *df_hist)
+{
+    const int h = op->h, w = op->w, S = p->n_subsets;
     double df_prev = 0;
     int it;
 
     for (it = 1; it <= p->max_iters; ++it) {
         double df = 0;
-        for (int s = 0; s < S; ++s) {
-            int nv = or_subset_views(op->nviews, S, s, views);
-            double num = 0, ad2 = 0, den, c;
-            or_op_fp(op, views, nv, u, q);
-            for (int k = 0; k < nv; ++k)
-                for (int j = 0; j < nb; ++j) {
-                    size_t i = (size_t)views[k] * nb + j;
-                    q[i] = g[i] - q[i];
-                    df += dsq(q[i]);
-                }
-            or_op_bp(op, views, nv, q, b);
-            for (size_t i = 0; i < ni; ++i) d[i] = S * b[i];
-            or_gc_ugrad_reg(d, u, v, p->alpha, p->eps, h, w);
-            or_op_fp(op, views, nv, d, q);
-            for (int k = 0; k < nv; ++k)
-                for (int j = 0; j < nb; ++j) ad2 += dsq(q[(size_t)views[k] * nb + j]);
-            for (size_t i = 0; i < ni; ++i) num += dsq(d[i]);
-            den = S * ad2 + p->alpha * or_gc_nV(d, v, p->eps, h, w);
-            c = (den > 0) ? num / den : 0.0;
-            for (size_t i = 0; i < ni; ++i) {
-                double x = u[i] + c * d[i];
-                if (x < p->lower) x = p->lower;
-                if (x > p->upper) x = p->upper;
-                u[i] = x;
-            }
-        }
+        for (int s = 0; s < S; ++s) or_os_step(op, g, u, v, p, s, NULL, &df);
         if (p->median_every > 0 && it % p->median_every == 0) or_medfilt2(u, h, w);
         if (p->iter_edge > 0) or_gc_edge_step(v, u, v, p->alpha, p->beta, p->eps, p->iter_edge, h, w);
         if (df_hist) df_hist[it - 1] = df;
         if (it > 1 && fabs(df - df_prev) < p->rel_tol * df_prev) break;
         df_prev = df;
     }
-    free(views); free(q); free(b); free(d);
     return (it > p->max_iters) ? p->max_iters : it;
 }
 

@@ -262,6 +262,20 @@ def subset_views(nviews, S, s):
     return out[:n].copy()
 
 
+def os_step(op, g, u, v, alpha, eps, S, s, lower=-math.inf, upper=math.inf):
+    """One subset step of the OS iteration: returns (u after the step, d, c)."""
+    g = np.ascontiguousarray(g, np.float64)
+    u = np.array(u, np.float64)
+    v = np.ascontiguousarray(v, np.float64)
+    d = np.zeros_like(u)
+    p = AtParams(alpha, 0.0, eps, S, 1, 0, 0, 0.0, lower, upper)
+    L = lib()
+    L.or_os_step.restype = C.c_double
+    L.or_os_step.argtypes = [C.c_void_p] * 5 + [C.c_int, C.c_void_p, C.c_void_p]
+    c = L.or_os_step(C.byref(op), P(g), P(u), P(v), C.byref(p), s, P(d), None)
+    return u, d, c
+
+
 def os_at(op, g, alpha, beta, eps, S, iters, iter_edge=1, rel_tol=0.0, median_every=0, lower=-math.inf,
           upper=math.inf, u0=None, v0=None):
     g = np.ascontiguousarray(g, np.float64)

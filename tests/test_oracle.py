@@ -479,6 +479,123 @@ def test_os_subsets_partition_gradient():
     assert np.allclose(full, parts, rtol=1e-13, atol=1e-13)
 
 
+def _dense(fg):
+    rp, ind, ln = O.fan_csr(fg)
+    A = np.zeros((fg.nviews * fg.nbins, fg.nx * fg.ny))
+    for r in range(len(rp) - 1):
+        np.add.at(A[r], ind[rp[r]:rp[r + 1]], ln[rp[r]:rp[r + 1]])
+    return A
+
+
+def _subset_energy(A, g, rows, S, alpha, eps, v):
+    """S |g_s - A_s u|^2 plus alpha times the regulariser whose gradient the
+    reference's image step uses: every grid edge p -> q (q the lower or right
+    neighbour; u zero and v one outside the image) weighted by
+    v_p^2 + 0.01 eps^2."""
+    h, w = v.shape
+    W = np.ones((h + 2, w + 2))
+    W[1:-1, 1:-1] = v
+    W = W ** 2 + 0.01 * eps ** 2
+
+    def E(u):
+        U = np.zeros((h + 2, w + 2))
+        U[1:-1, 1:-1] = u
+        r = g.ravel()[rows] - A[rows] @ u.ravel()
+        dn = U[1:, :-1] - U[:-1, :-1]
+        rt = U[:-1, 1:] - U[:-1, :-1]
+        return S * r @ r + alpha * np.sum(W[:-1, :-1] * (dn ** 2 + rt ** 2))
+    return E
+
+
+def _rows(fg, S, s):
+    return np.concatenate([np.arange(k * fg.nbins, (k + 1) * fg.nbins) for k in O.subset_views(fg.nviews, S, s)])
+
+
+def test_os_step_direction_is_the_steepest_descent_of_the_subset_energy():
+    """d = -1/2 grad(S |g_s - A_s u|^2 + alpha R(u; v)) at S = 4 for a random
+    u and v (central differences are exact for a quadratic): a dropped S, a
+    wrong alpha or sign, or a wrong subset fails it."""
+    fg = O.fan_geom(12, 1.0, 16, 20, 1.0)
+    op = O.op_fan(fg)
+    A = _dense(fg)
+    rs = np.random.RandomState(21)
+    g = O.fan_fp(fg, rs.rand(12, 12))
+    u = rs.rand(12, 12)
+    v = 0.3 + 0.7 * rs.rand(12, 12)
+    S, alpha, eps = 4, 0.8, 0.3
+    for s in range(S):
+        E = _subset_energy(A, g, _rows(fg, S, s), S, alpha, eps, v)
+        grad = np.zeros(u.size)
+        for k in range(u.size):
+            e = np.zeros(u.size)
+            e[k] = 1.0
+            grad[k] = (E(u + e.reshape(u.shape)) - E(u - e.reshape(u.shape))) / 2.0
+        _, d, _ = O.os_step(op, g, u, v, alpha, eps, S, s)
+        assert np.allclose(d.ravel(), -0.5 * grad, rtol=1e-9, atol=1e-9 * np.abs(grad).max())
+
+
+def _vertex(E, u, d, t):
+    e0, e1, e2 = E(u), E(u + t * d), E(u - t * d)
+    return t * (e2 - e1) / (2.0 * (e1 + e2 - 2.0 * e0))
+
+
+def test_os_step_is_the_exact_minimiser_along_d():
+    """The S-scaled step c = |d|^2 / (S |A_s d|^2 + alpha nV(d)) against a
+    brute-force minimisation of the subset energy along d (the vertex of the
+    parabola through three points; the energy is quadratic in c).
+    (a) alpha = 0, random u and v: exact for any state.
+    (b) alpha > 0: the reference's nV (backward differences, no edge to the
+    zero padding below the last row or right of the last column) equals the
+    regulariser's curvature along d when v = 1 and d vanishes on the last row
+    and column.  That holds for g = A u with u zero on its last two rows and
+    columns: the residual is zero and d is the regulariser term alone.  Every
+    subset is checked, and the subset energy with S = 1 in place of S = 4
+    has a different minimiser."""
+    fg = O.fan_geom(12, 1.0, 16, 20, 1.0)
+    op = O.op_fan(fg)
+    A = _dense(fg)
+    rs = np.random.RandomState(22)
+    S, eps = 4, 0.3
+    # (a)
+    g = O.fan_fp(fg, rs.rand(12, 12))
+    u = rs.rand(12, 12)
+    v = 0.3 + 0.7 * rs.rand(12, 12)
+    for s in range(S):
+        E = _subset_energy(A, g, _rows(fg, S, s), S, 0.0, eps, v)
+        un, d, c = O.os_step(op, g, u, v, 0.0, eps, S, s)
+        assert abs(c - _vertex(E, u, d, c)) < 1e-9 * c
+        assert np.allclose(un, u + _vertex(E, u, d, c) * d, rtol=1e-12, atol=1e-12)
+    # (b)
+    u = np.zeros((12, 12))
+    u[:10, :10] = rs.rand(10, 10)
+    g = O.fan_fp(fg, u)
+    v = np.ones((12, 12))
+    for alpha in (0.5, 5.0):
+        for s in range(S):
+            E = _subset_energy(A, g, _rows(fg, S, s), S, alpha, eps, v)
+            un, d, c = O.os_step(op, g, u, v, alpha, eps, S, s)
+            assert not d[-1, :].any() and not d[:, -1].any() and np.abs(d).max() > 0
+            cv = _vertex(E, u, d, c)
+            assert abs(c - cv) < 1e-9 * c, (alpha, s, c, cv)
+            assert np.allclose(un, u + cv * d, rtol=1e-12, atol=1e-12)
+            E1 = _subset_energy(A, g, _rows(fg, S, s), 1, alpha, eps, v)
+            assert abs(c - _vertex(E1, u, d, c)) > 1e-3 * c
+
+
+def test_os_at_pass_is_its_subset_steps():
+    """or_os_at runs or_os_step over the subsets in order (no median filter,
+    no edge step): bitwise the same u."""
+    fg = O.fan_geom(16, 1.0, 24, 24, 1.0)
+    op = O.op_fan(fg)
+    rs = np.random.RandomState(23)
+    g = O.fan_fp(fg, rs.rand(16, 16))
+    u, v, n, _ = O.os_at(op, g, 1e-2, 1e-4, 1e-2, 4, 1, iter_edge=0)
+    w = np.zeros((16, 16))
+    for s in range(4):
+        w, _, _ = O.os_step(op, g, w, np.ones((16, 16)), 1e-2, 1e-2, 4, s)
+    assert np.array_equal(u, w)
+
+
 def test_os_decreases_data_fidelity():
     fg = O.fan_geom(32, 1.0, 32, 48, 1.0)
     op = O.op_fan(fg)
