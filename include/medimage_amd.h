@@ -351,7 +351,7 @@ enum mi_kernel_id {
     MI_K_FP = 0,          /* forward projection of a subset (residual form)  */
     MI_K_BP = 1,          /* gather back projection + image gradient         */
     MI_K_FPSQ = 2,        /* forward projection of the direction, ||A d||^2  */
-    MI_K_STEP = 3,        /* line-search step + update                       */
+    MI_K_STEP = 3,        /* nV, line-search step + update                   */
     MI_K_EDGE = 4,        /* edge (v) step kernels                           */
     MI_K_CONE_FP = 5,     /* 3D cone-beam forward projection (mi_cone_*)     */
     MI_K_CONE_BP = 6,     /* 3D cone-beam gather back projection             */

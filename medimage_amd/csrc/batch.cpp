@@ -552,20 +552,14 @@ void Batch<T>::record_pass(bool timed) {
     for (int s = 0; s < S; ++s) {
         auto nnz_s = [&](const Unit<T> &U) { return (double)U.m->sub_nnz[s]; };
         auto nr_s = [&](const Unit<T> &U) { return (double)(U.m->sub_row0[s + 1] - U.m->sub_row0[s]); };
-        auto fp = [&](bool resid) {
-            auto one = [&](int c) {
-                return [&, c](hipStream_t q) {
-                    if (resid) eng_fp_resid<T>(q, args, rows[c].map, s, rows[c].kmax, all_walk);
-                    else eng_fp_sq<T>(q, args, rows[c].map, s, rows[c].kmax, all_walk);
-                };
-            };
-            branches(one(0), one(1), rows[0].any(), rows[1].any());
-        };
         {
             LaunchTimer lt(ctx, MI_K_FP, sum([&](const Unit<T> &U) {
                 return nnz_s(U) * (4 + sz) + U.m->npix * sz * U.K + 2 * nr_s(U) * sz * U.K;
             }), true);
-            fp(true);
+            auto one = [&](int c) {
+                return [&, c](hipStream_t q) { eng_fp_resid<T>(q, args, rows[c].map, s, rows[c].kmax, all_walk); };
+            };
+            branches(one(0), one(1), rows[0].any(), rows[1].any());
         }
         {
             LaunchTimer lt(ctx, MI_K_BP, sum([&](const Unit<T> &U) {
@@ -583,7 +577,10 @@ void Batch<T>::record_pass(bool timed) {
             LaunchTimer lt(ctx, MI_K_FPSQ, sum([&](const Unit<T> &U) {
                 return nnz_s(U) * (4 + sz) + U.m->npix * sz * U.K;
             }), true);
-            fp(false);
+            auto one = [&](int c) {
+                return [&, c](hipStream_t q) { eng_fp_sq<T>(q, args, rows[c].map, s, rows[c].kmax, all_walk); };
+            };
+            branches(one(0), one(1), rows[0].any(), rows[1].any());
         }
         {
             LaunchTimer lt(ctx, MI_K_STEP, sum([&](const Unit<T> &U) { return 5.0 * U.m->npix * sz * U.K; }));

@@ -116,7 +116,7 @@ struct Batch {
         bool any() const { return map.start[map.nunits] > 0; }
     };
     Launch rows[2], bpl[2], bpg[2];
-    hipStream_t side = nullptr;
+    hipStream_t side = nullptr;   // the second class of a phase
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     EngArgs<T> args{};
     int kmax = 1;

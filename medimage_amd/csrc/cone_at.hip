@@ -225,8 +225,9 @@ __global__ __launch_bounds__(kBlock) void k_cat_grad(Dims dm, const float *__res
 
 // part = pair sum of p (zero outside) with weights (v^2 + eta) when WV, 1
 // otherwise.  The squared differences of a weight class are summed in float
-// (13 terms per voxel), the voxel's total and the workgroup's sum in
-// double.  Planes are fetched two steps ahead.
+// and weighted in float (13 terms per voxel); the lane's sum over its
+// slices and the workgroup's sum are double.  Planes are fetched two steps
+// ahead.
 template <bool WV>
 __global__ __launch_bounds__(kBlock) void k_cat_pairs(Dims dm, const float *__restrict__ p,
                                                       const float *__restrict__ v, float eta,
@@ -269,7 +270,7 @@ __global__ __launch_bounds__(kBlock) void k_cat_pairs(Dims dm, const float *__re
     __syncthreads();
     Nb P0, P1;
     nbhd(P0, lp[t.z0 & 1]);
-    double acc = 0.0;
+    double acc = 0.0, acco = 0.0;
     // slice z from p on z, z + 1; stages p(z + 1) into pn, refills the ring
     // slot for z + 2
     auto step = [&](int z, const Nb &pc, Nb &pn, Ring &r) {
@@ -280,7 +281,6 @@ __global__ __launch_bounds__(kBlock) void k_cat_pairs(Dims dm, const float *__re
         nbhd(pn, lp[(z + 1) & 1]);
         if (t.in) {
             const float p0 = pc[1][1];
-            const double s0 = WV ? (double)vz * vz + eta : 1.0, so = WV ? 1.0 + eta : 1.0;
             float g[3] = {0.0f, 0.0f, 0.0f};   // per weight class: sum of (p(x + k) - p(x))^2
 #pragma unroll
             for (int k = 0; k < 13; ++k) {
@@ -290,15 +290,19 @@ __global__ __launch_bounds__(kBlock) void k_cat_pairs(Dims dm, const float *__re
                 const float q = (a ? pn[1 + b][1 + c] : pc[1 + b][1 + c]) - p0;
                 g[n] = fmaf(q, q, g[n]);
             }
-            acc += s0 * (((double)g[0] + 0.5 * (double)g[1]) + (1.0 / 3.0) * (double)g[2]) +
-                   so * ((double)p0 * p0) * wout[z == 0 ? 1 : 0];
+            // the voxel's weighted pair sum in float (13 terms), added in double
+            const float gs = fmaf(1.0f / 3.0f, g[2], fmaf(0.5f, g[1], g[0]));
+            acc += WV ? (double)(fmaf(vz, vz, eta) * gs) : (double)gs;
+            const double wz = wout[z == 0 ? 1 : 0];
+            if (wz != 0.0) acco += wz * ((double)p0 * p0);   // the pairs with an outside voxel
         }
     };
     for (int z = t.z0; z < t.z1; z += 2) {
         step(z, P0, P1, R[0]);
         if (z + 1 < t.z1) step(z + 1, P1, P0, R[1]);
     }
-    const double tot = block_sum(acc);
+    // outside voxels have v = 1: weight 1 + eta
+    const double tot = block_sum(acc + (WV ? 1.0 + (double)eta : 1.0) * acco);
     if (threadIdx.x == 0) part[blockIdx.x] = tot;
 }
 
@@ -341,16 +345,16 @@ __global__ __launch_bounds__(kBlock) void k_cat_edge_n(Dims dm, const float *__r
     __shared__ float lu[2][kPlane];  // plane z in slot z & 1
     const Tile t = tile_of(dm);
     put<false>(lu[t.z0 & 1], t, fetch(u, dm, t, t.z0, 0.0f), 0.0f);
-    // ring slot of step z: u(z + 1), fetched two steps ahead
-    Ahead R[2] = {fetch(u, dm, t, t.z0 + 1, 0.0f), fetch(u, dm, t, t.z0 + 2, 0.0f)};
+    // ring slot of step z: u(z + 1), fetched three steps ahead
+    Ahead R[3] = {fetch(u, dm, t, t.z0 + 1, 0.0f), fetch(u, dm, t, t.z0 + 2, 0.0f), fetch(u, dm, t, t.z0 + 3, 0.0f)};
     __syncthreads();
     Nb U0, U1;
     nbhd(U0, lu[t.z0 & 1]);
     // slice z from u on z, z + 1; stages u(z + 1) into un, refills the ring
-    // slot for z + 2
+    // slot for z + 3
     auto step = [&](int z, const Nb &uc, Nb &un, Ahead &r) {
         put<false>(lu[(z + 1) & 1], t, r, 0.0f);
-        if (z + 2 < t.z1) r = fetch(u, dm, t, z + 3, 0.0f);
+        if (z + 3 < t.z1) r = fetch(u, dm, t, z + 4, 0.0f);
         __syncthreads();
         nbhd(un, lu[(z + 1) & 1]);
         if (t.in) {
@@ -366,9 +370,14 @@ __global__ __launch_bounds__(kBlock) void k_cat_edge_n(Dims dm, const float *__r
             nn[vidx(dm, t, z)] = (g[0] + 0.5f * g[1]) + (1.0f / 3.0f) * g[2];
         }
     };
-    for (int z = t.z0; z < t.z1; z += 2) {
+    // the planes rotate through U0 and U1 (two slots), the ring through three
+    for (int z = t.z0; z < t.z1; z += 6) {
         step(z, U0, U1, R[0]);
         if (z + 1 < t.z1) step(z + 1, U1, U0, R[1]);
+        if (z + 2 < t.z1) step(z + 2, U0, U1, R[2]);
+        if (z + 3 < t.z1) step(z + 3, U1, U0, R[0]);
+        if (z + 4 < t.z1) step(z + 4, U0, U1, R[1]);
+        if (z + 5 < t.z1) step(z + 5, U1, U0, R[2]);
     }
 }
 

@@ -563,32 +563,60 @@ __device__ __forceinline__ void fp_unit(const EngArgs<T> &a, const UG<T> &U, int
     block_store<T, K, false>(a, U, RESID ? SL_DF : SL_AD, lb, m, lds);
 }
 
-// GoldenC image-gradient regulariser of K interleaved jobs at pixel (i, j):
-// u zero padded, v one padded (origin/GoldenC/src/main.c:112-125).
+// K interleaved values at pixel (i, j), or `pad` outside the image.
 template <typename T, int K>
-__device__ __forceinline__ void gc_reg(const UG<T> &U, int i, int j, const double (&eps)[K], double (&out)[K]) {
-    T z[K], zd[K], zr[K], zu[K], zl[K], p[K], pu[K], pl[K];
-    auto ld = [&](const GL T *arr, int ii, int jj, T (&o)[K], T pad) {
-        const int64_t q = U.at(ii, jj);
-        if (q < 0) {
+__device__ __forceinline__ void ldn(const UG<T> &U, const GL T *arr, int i, int j, T (&o)[K], T pad) {
+    const int64_t q = U.at(i, j);
+    if (q < 0) {
 #pragma unroll
-            for (int k = 0; k < K; ++k) o[k] = pad;
-        } else {
-            loadk<T, K>(arr + q * K, o);
-        }
-    };
-    ld(U.u, i, j, z, T(0));
-    ld(U.u, i + 1, j, zd, T(0));
-    ld(U.u, i, j + 1, zr, T(0));
-    ld(U.u, i - 1, j, zu, T(0));
-    ld(U.u, i, j - 1, zl, T(0));
-    ld(U.v, i, j, p, T(1));
-    ld(U.v, i - 1, j, pu, T(1));
-    ld(U.v, i, j - 1, pl, T(1));
+        for (int k = 0; k < K; ++k) o[k] = pad;
+    } else {
+        loadk<T, K>(arr + q * K, o);
+    }
+}
+
+// The back projection's 16 x 16 pixel square (rows i0 .. i0 + 15, columns
+// j0 .. j0 + 15) and its one-pixel halo, 18 x 18 pixels of K values, for the
+// regulariser of its epilogue: staged through LDS once instead of eight
+// neighbour loads per pixel.
+constexpr int kSqW = kBpBlockTC * kTileW + 2, kSqN = kSqW * kSqW;
+static_assert(kBpBlockTR * kTileH == kBpBlockTC * kTileW, "the back-projection block is square");
+__device__ __forceinline__ int sq_at(int r, int c) { return (r + 1) * kSqW + c + 1; }
+
+// L[sq_at(r, c) * K + k] = arr at (i0 + r, j0 + c), `pad` outside the image.
+// Thread (w, lane) stages pixel (tile row w, tile column lane / 32, tile
+// position lane % 32) of the square; threads 0 .. 67 the halo.  Every thread
+// calls it; ends with a barrier.
+template <typename T, int K>
+__device__ __forceinline__ void stage_square(const UG<T> &U, const GL T *arr, int i0, int j0, T pad, LDSP T *L) {
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave, w = lane & 31;
+    const int r = wv * kTileH + ((w >> 2) & 3), c = (lane >> 5) * kTileW + ((w >> 4) << 2) + (w & 3);
+    T x[K];
+    ldn<T, K>(U, arr, i0 + r, j0 + c, x, pad);
+#pragma unroll
+    for (int k = 0; k < K; ++k) L[sq_at(r, c) * K + k] = x[k];
+    constexpr int E = kSqW - 2, kHalo = 2 * kSqW + 2 * E;   // 68
+    if (threadIdx.x < kHalo) {
+        const int e = threadIdx.x;
+        const int hr = e < kSqW ? -1 : (e < 2 * kSqW ? E : (e - 2 * kSqW) % E);
+        const int hc = e < 2 * kSqW ? e % kSqW - 1 : (e < 2 * kSqW + E ? -1 : E);
+        ldn<T, K>(U, arr, i0 + hr, j0 + hc, x, pad);
+#pragma unroll
+        for (int k = 0; k < K; ++k) L[sq_at(hr, hc) * K + k] = x[k];
+    }
+    __syncthreads();
+}
+
+// GoldenC image-gradient regulariser of K interleaved jobs at square pixel
+// (r, c): u zero padded, v one padded (origin/GoldenC/src/main.c:112-125).
+template <typename T, int K>
+__device__ __forceinline__ void gc_reg(const LDSP T *Lu, const LDSP T *Lv, int r, int c, const double (&eps)[K],
+                                       double (&out)[K]) {
+    const LDSP T *u = Lu + sq_at(r, c) * K, *v = Lv + sq_at(r, c) * K;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const double Z = z[k], Zd = zd[k], Zr = zr[k], Zu = zu[k], Zl = zl[k];
-        const double Pp = p[k], Pu = pu[k], Pl = pl[k];
+        const double Z = u[k], Zd = u[kSqW * K + k], Zr = u[K + k], Zu = u[-kSqW * K + k], Zl = u[-K + k];
+        const double Pp = v[k], Pu = v[-kSqW * K + k], Pl = v[-K + k];
         out[k] = +(Pp * Pp) * (Zd - Z) + (Pp * Pp) * (Zr - Z) - (Pu * Pu) * (Z - Zu) - (Pl * Pl) * (Z - Zl) +
                  0.01 * (eps[k] * eps[k]) * (Zd + Zr - 4 * Z + Zu + Zl);
     }
@@ -643,7 +671,11 @@ __device__ __forceinline__ void bp_unit(const EngArgs<T> &a, const UG<T> &U, int
             }
         __syncthreads();
     }
-    if (li < U.bs_slice0[s + 1] - sl0) {
+    const bool has = li < U.bs_slice0[s + 1] - sl0;
+    double tot[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) tot[j] = 0.0;
+    if (has) {
         const int64_t sl = sl0 + li;
         Acc<T> acc[K];
         if (LDS) {
@@ -654,9 +686,18 @@ __device__ __forceinline__ void bp_unit(const EngArgs<T> &a, const UG<T> &U, int
             walk_slice<T, K, MI_NT_BP>(U.bs_base, U.bs_T, U.bs_idx, U.bs_first, U.bs_d16, U.bs_len, sl, lane, U.r,
                                        U.nrows, acc);
         }
-        double tot[K];
 #pragma unroll
         for (int j = 0; j < K; ++j) tot[j] = acc[j].value();
+    }
+    // u and v of the square for the regulariser, in the LDS the residual
+    // slots used (every wave has finished its walk)
+    const int nbc = (U.tny + kBpBlockTC - 1) / kBpBlockTC;
+    const int i0 = (lb / nbc) * (kBpBlockTR * kTileH), j0 = (lb % nbc) * (kBpBlockTC * kTileW);
+    LDSP T *Lu = ldr, *Lv = ldr + kSqN * K;
+    __syncthreads();
+    stage_square<T, K>(U, U.u, i0, j0, T(0), Lu);
+    stage_square<T, K>(U, U.v, i0, j0, T(1), Lv);
+    if (has) {
         const int64_t q = bp_item(lb, wv, lane, tnx, U.tny);
         if (q >= 0) {
             int i, j;
@@ -666,7 +707,7 @@ __device__ __forceinline__ void bp_unit(const EngArgs<T> &a, const UG<T> &U, int
                 double eps[K], reg[K];
 #pragma unroll
                 for (int k = 0; k < K; ++k) eps[k] = jpar(a, U.job0 + k).eps;
-                gc_reg<T, K>(U, i, j, eps, reg);
+                gc_reg<T, K>(Lu, Lv, i - i0, j - j0, eps, reg);
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
                     const double d = (double)a.S * tot[k] + jpar(a, U.job0 + k).alpha * reg[k];
@@ -685,34 +726,88 @@ __device__ __forceinline__ void bp_unit(const EngArgs<T> &a, const UG<T> &U, int
 // Stencil kernels: one thread per tiled position q (256 per block); the
 // padding positions of the tiles do no work.
 
-// K interleaved values at pixel (i, j), or `pad` outside the image.
+// Stencil strips.  Tile rows hold a multiple of kStripTiles tiles, so the 256
+// tiled positions of stencil block lb are the 4 x 64 pixels of one strip
+// (rows i0 .. i0 + 3, columns j0 .. j0 + 63).  A stencil stages the strip of
+// each array it reads around, with a one-pixel halo (6 x 66 pixels, K values
+// each), through LDS once, and reads the neighbours from there.
+constexpr int kStripW = kStripTiles * kTileW, kHaloW = kStripW + 2, kHaloN = (kTileH + 2) * kHaloW;
+static_assert(kPixBlock == kStripTiles * kTileH * kTileW, "a stencil block is one strip");
+
+struct Strip {
+    int64_t q;        // this thread's tiled position
+    int i0, j0;       // strip origin
+    int li, lj;       // this thread's pixel, relative to the origin
+    bool real;        // the pixel lies in the image
+};
+template <typename T>
+__device__ __forceinline__ Strip strip_of(const UG<T> &U, int lb) {
+    const int t0 = lb * kStripTiles;
+    Strip s;
+    s.q = (int64_t)lb * kPixBlock + threadIdx.x;
+    s.i0 = (t0 / U.tny) * kTileH;
+    s.j0 = (t0 % U.tny) * kTileW;
+    int i, j;
+    tile_coords(s.q, U.tny, i, j);
+    s.li = i - s.i0;
+    s.lj = j - s.j0;
+    s.real = i < U.nx && j < U.ny;
+    return s;
+}
+__device__ __forceinline__ int halo_at(int li, int lj) { return (li + 1) * kHaloW + lj + 1; }
+
+// L[halo_at(r, c) * K + k] = arr at pixel (i0 + r, j0 + c) for the strip and
+// its halo, `pad` outside the image.  Every thread calls it; ends with a
+// barrier.
 template <typename T, int K>
-__device__ __forceinline__ void ldn(const UG<T> &U, const GL T *arr, int i, int j, T (&o)[K], T pad) {
-    const int64_t q = U.at(i, j);
-    if (q < 0) {
-#pragma unroll
-        for (int k = 0; k < K; ++k) o[k] = pad;
+__device__ __forceinline__ void stage_strip(const UG<T> &U, const GL T *arr, const Strip &s, T pad, T *L) {
+    T x[K];
+    if (s.real) {
+        loadk<T, K>(arr + s.q * K, x);
     } else {
-        loadk<T, K>(arr + q * K, o);
+#pragma unroll
+        for (int k = 0; k < K; ++k) x[k] = pad;
     }
+#pragma unroll
+    for (int k = 0; k < K; ++k) L[halo_at(s.li, s.lj) * K + k] = x[k];
+    // the halo: rows -1 and 4 (66 pixels each, corners included), columns
+    // -1 and 64 of rows 0 .. 3
+    constexpr int kHalo = 2 * kHaloW + 2 * kTileH;
+    static_assert(kHalo <= kBlock, "one halo pixel per thread");
+    if (threadIdx.x < kHalo) {
+        const int e = threadIdx.x, f = e - 2 * kHaloW;
+        const int li = e < 2 * kHaloW ? (e < kHaloW ? -1 : kTileH) : f % kTileH;
+        const int lj = e < 2 * kHaloW ? e % kHaloW - 1 : (f < kTileH ? -1 : kStripW);
+        ldn<T, K>(U, arr, s.i0 + li, s.j0 + lj, x, pad);
+#pragma unroll
+        for (int k = 0; k < K; ++k) L[halo_at(li, lj) * K + k] = x[k];
+    }
+    __syncthreads();
+}
+
+// The staged values at offset (di, dj) from this thread's pixel.
+template <typename T, int K>
+__device__ __forceinline__ void ldl(const T *L, const Strip &s, int di, int dj, T (&o)[K]) {
+    const T *p = L + halo_at(s.li + di, s.lj + dj) * K;
+#pragma unroll
+    for (int k = 0; k < K; ++k) o[k] = p[k];
 }
 
 // nV(d) = sum (v^2 + 0.01 eps^2)((d - d_up)^2 + (d - d_left)^2), d zero
 // padded (origin/GoldenC/src/main.c:152-158).
 template <typename T, int K>
-__device__ __forceinline__ void nv_unit(const EngArgs<T> &a, const UG<T> &U, int lb, double *lds) {
-    const int64_t q = (int64_t)lb * kPixBlock + threadIdx.x;
+__device__ __forceinline__ void nv_unit(const EngArgs<T> &a, const UG<T> &U, int lb, double *lds, T *L) {
+    const Strip st = strip_of(U, lb);
+    stage_strip<T, K>(U, U.d, st, T(0), L);
     double m[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) m[j] = 0.0;
-    int i = 0, jj = 0;
-    if (q < U.npix_t) tile_coords(q, U.tny, i, jj);
-    if (q < U.npix_t && i < U.nx && jj < U.ny) {
+    if (st.real) {
         T dq[K], du[K], dl[K], vv[K];
-        loadk<T, K>(U.d + q * K, dq);
-        loadk<T, K>(U.v + q * K, vv);
-        ldn<T, K>(U, U.d, i - 1, jj, du, T(0));
-        ldn<T, K>(U, U.d, i, jj - 1, dl, T(0));
+        ldl<T, K>(L, st, 0, 0, dq);
+        ldl<T, K>(L, st, -1, 0, du);
+        ldl<T, K>(L, st, 0, -1, dl);
+        loadk<T, K>(U.v + st.q * K, vv);
 #pragma unroll
         for (int j = 0; j < K; ++j) {
             const double eps = jpar(a, U.job0 + j).eps;
@@ -770,17 +865,17 @@ __device__ __forceinline__ bool median_due(const EngArgs<T> &a, int js) {
 // w = 3x3 median of u (zero padding, origin/GoldenC/src/main.c:366-390) for
 // jobs whose median is due this pass, else w = u.
 template <typename T, int K>
-__device__ __forceinline__ void median_unit(const EngArgs<T> &a, const UG<T> &U, int lb) {
-    const int64_t q = (int64_t)lb * kPixBlock + threadIdx.x;
-    if (!real_pixel(q, U.npix_t, U.tny, U.nx, U.ny)) return;
-    int i, jj;
-    tile_coords(q, U.tny, i, jj);
+__device__ __forceinline__ void median_unit(const EngArgs<T> &a, const UG<T> &U, int lb, T *L) {
+    const Strip st = strip_of(U, lb);
+    stage_strip<T, K>(U, U.u, st, T(0), L);
+    if (!st.real) return;
+    const int64_t q = st.q;
     T nb[9][K];
     int n = 0;
 #pragma unroll
     for (int di = -1; di <= 1; ++di)
 #pragma unroll
-        for (int dj = -1; dj <= 1; ++dj, ++n) ldn<T, K>(U, U.u, i + di, jj + dj, nb[n], T(0));
+        for (int dj = -1; dj <= 1; ++dj, ++n) ldl<T, K>(L, st, di, dj, nb[n]);
     T out[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) {
@@ -822,24 +917,26 @@ __device__ __forceinline__ bool edge_live(const EngArgs<T> &a, int js, int it) {
 // ||w||^2 and sum n w^2 (f = u zero padded, v one padded).
 template <typename T, int K>
 __device__ __forceinline__ void edge_grad_unit(const EngArgs<T> &a, const UG<T> &U, int lb, int it,
-                                               double *lds) {
-    const int64_t q = (int64_t)lb * kPixBlock + threadIdx.x;
+                                               double *lds, T *L) {
+    const Strip st = strip_of(U, lb);
+    T *Lu = L, *Lv = L + kHaloN * K;
+    stage_strip<T, K>(U, U.u, st, T(0), Lu);
+    stage_strip<T, K>(U, U.v, st, T(1), Lv);
+    const int64_t q = st.q;
     double m0[K], m1[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) m0[j] = m1[j] = 0.0;
-    if (real_pixel(q, U.npix_t, U.tny, U.nx, U.ny)) {
-        int i, jj;
-        tile_coords(q, U.tny, i, jj);
+    if (st.real) {
         T fd[K], fu[K], fr[K], fl[K], vc[K], vd[K], vu[K], vr[K], vl[K];
-        ldn<T, K>(U, U.u, i + 1, jj, fd, T(0));
-        ldn<T, K>(U, U.u, i - 1, jj, fu, T(0));
-        ldn<T, K>(U, U.u, i, jj + 1, fr, T(0));
-        ldn<T, K>(U, U.u, i, jj - 1, fl, T(0));
-        loadk<T, K>(U.v + q * K, vc);
-        ldn<T, K>(U, U.v, i + 1, jj, vd, T(1));
-        ldn<T, K>(U, U.v, i - 1, jj, vu, T(1));
-        ldn<T, K>(U, U.v, i, jj + 1, vr, T(1));
-        ldn<T, K>(U, U.v, i, jj - 1, vl, T(1));
+        ldl<T, K>(Lu, st, 1, 0, fd);
+        ldl<T, K>(Lu, st, -1, 0, fu);
+        ldl<T, K>(Lu, st, 0, 1, fr);
+        ldl<T, K>(Lu, st, 0, -1, fl);
+        ldl<T, K>(Lv, st, 0, 0, vc);
+        ldl<T, K>(Lv, st, 1, 0, vd);
+        ldl<T, K>(Lv, st, -1, 0, vu);
+        ldl<T, K>(Lv, st, 0, 1, vr);
+        ldl<T, K>(Lv, st, 0, -1, vl);
         T out[K];
         zerok<T, K>(out);
 #pragma unroll
@@ -869,19 +966,18 @@ __device__ __forceinline__ void edge_grad_unit(const EngArgs<T> &a, const UG<T> 
 // main.c:311-329: |central gradient of w|^2.
 template <typename T, int K>
 __device__ __forceinline__ void edge_nabla_unit(const EngArgs<T> &a, const UG<T> &U, int lb, int it,
-                                                double *lds) {
-    const int64_t q = (int64_t)lb * kPixBlock + threadIdx.x;
+                                                double *lds, T *L) {
+    const Strip st = strip_of(U, lb);
+    stage_strip<T, K>(U, U.w, st, T(0), L);
     double m[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) m[j] = 0.0;
-    if (real_pixel(q, U.npix_t, U.tny, U.nx, U.ny)) {
-        int i, jj;
-        tile_coords(q, U.tny, i, jj);
+    if (st.real) {
         T d1[K], d2[K], d3[K], d4[K];
-        ldn<T, K>(U, U.w, i + 1, jj, d1, T(0));
-        ldn<T, K>(U, U.w, i - 1, jj, d2, T(0));
-        ldn<T, K>(U, U.w, i, jj + 1, d3, T(0));
-        ldn<T, K>(U, U.w, i, jj - 1, d4, T(0));
+        ldl<T, K>(L, st, 1, 0, d1);
+        ldl<T, K>(L, st, -1, 0, d2);
+        ldl<T, K>(L, st, 0, 1, d3);
+        ldl<T, K>(L, st, 0, -1, d4);
 #pragma unroll
         for (int j = 0; j < K; ++j) {
             if (!edge_live(a, U.job0 + j, it)) continue;
@@ -969,26 +1065,6 @@ __global__ __launch_bounds__(kBlock) void k_eng_bp(EngArgs<T> a, BlockMap m, int
     MI_KSWITCH(KMAX, (bp_unit<T, K, LDS>(a, U, s, lb, lds, (LDSP T *)ldr_raw)))
 }
 
-enum PixOp { PX_AXPY = 0, PX_MEDIAN, PX_COPY_WU, PX_EDGE_GRAD, PX_EDGE_NABLA, PX_EDGE_UPDATE, PX_NV };
-
-template <typename T, int KMAX, int OP>
-__global__ __launch_bounds__(kBlock) void k_eng_pix(EngArgs<T> a, BlockMap m, int it) {
-    __shared__ double lds[kBlock / kWave * kMaxK];
-    const int b = (int)xcd_block();
-    if (b >= m.start[m.nunits]) return;
-    const int ui = find_unit(m, b);
-    const UG<T> U(a.units[ui]);
-    if (unit_idle(a, U)) return;
-    const int lb = b - m.start[ui];
-    if constexpr (OP == PX_AXPY) { MI_KSWITCH(KMAX, (axpy_unit<T, K>(a, U, lb))) }
-    else if constexpr (OP == PX_MEDIAN) { MI_KSWITCH(KMAX, (median_unit<T, K>(a, U, lb))) }
-    else if constexpr (OP == PX_COPY_WU) { MI_KSWITCH(KMAX, (copy_wu_unit<T, K>(U, lb))) }
-    else if constexpr (OP == PX_NV) { MI_KSWITCH(KMAX, (nv_unit<T, K>(a, U, lb, lds))) }
-    else if constexpr (OP == PX_EDGE_GRAD) { MI_KSWITCH(KMAX, (edge_grad_unit<T, K>(a, U, lb, it, lds))) }
-    else if constexpr (OP == PX_EDGE_NABLA) { MI_KSWITCH(KMAX, (edge_nabla_unit<T, K>(a, U, lb, it, lds))) }
-    else { MI_KSWITCH(KMAX, (edge_update_unit<T, K>(a, U, lb, it, lds))) }
-}
-
 // Fixed-order block reduction of n partials (sum or max), 256 threads.
 __device__ __forceinline__ double fin_reduce(const GL double *p, int64_t n, bool mx, double *lds) {
     double acc = 0.0;
@@ -1055,13 +1131,37 @@ __device__ __forceinline__ void fin_reduce_n(const GL double *const (&p)[NA], co
 
 __host__ __device__ __forceinline__ int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
-// One block per job slot (units listed in m; m.nunits * kMaxK blocks).
+// The subset step's sums of job j of unit U (fixed order over the blocks of
+// the forward walks, back projection and nV) and its line search
+// c = <d,d> / (S |A_s d|^2 + alpha nV(d)).  All threads of the block.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_eng_fin(EngArgs<T> a, int op, int s) {
-    __shared__ double lds[4 * (kBlock / kWave)];
-    const int ui = blockIdx.x / kMaxK, j = blockIdx.x % kMaxK;
-    const UG<T> U(a.units[ui]);
-    if (j >= U.K) return;
+__device__ __forceinline__ void fin_step(const EngArgs<T> &a, const UG<T> &U, int j, int s, double *lds) {
+    const int js = U.job0 + j;
+    if (!job_live(a, js)) return;
+    GL Scalars *sc = &jsc(a, js);
+    const GL JobParams &P = jpar(a, js);
+    const int64_t nr = cdiv(U.fs_slice0[s + 1] - U.fs_slice0[s], kBlock / kWave);
+    const int64_t nb = cdiv(U.bs_slice0[s + 1] - U.bs_slice0[s], kBlock / kWave);
+    const int64_t npx = cdiv(U.npix_t, kPixBlock);
+    const GL double *const p[4] = {part_at(a, U, SL_DD, j), part_at(a, U, SL_AD, j), part_at(a, U, SL_NV, j),
+                                   part_at(a, U, SL_DF, j)};
+    const int64_t n[4] = {nb, nr, npx, nr};
+    double r[4];
+    fin_reduce_n<4>(p, n, r, lds);
+    const double dd = r[0], ad2 = r[1], nv = r[2], df = r[3];
+    if (threadIdx.x == 0) {
+        sc->df_pass += df;
+        sc->dd = dd;
+        sc->ad2 = ad2;
+        sc->nv = nv;
+        const double den = (double)a.S * ad2 + P.alpha * nv;
+        sc->c = (den > 0) ? dd / den : 0.0;
+    }
+}
+
+// Finalize op `op` of job j of unit U (every thread of the block calls it).
+template <typename T>
+__device__ __forceinline__ void fin_job(const EngArgs<T> &a, const UG<T> &U, int j, int op, int s, double *lds) {
     const int js = U.job0 + j;
     if (!job_live(a, js)) return;
     GL Scalars *sc = &jsc(a, js);
@@ -1074,25 +1174,6 @@ __global__ __launch_bounds__(kBlock) void k_eng_fin(EngArgs<T> a, int op, int s)
                 sc->edge_done = 0;
             }
             break;
-        case EF_STEP: {
-            const int64_t nr = cdiv(U.fs_slice0[s + 1] - U.fs_slice0[s], kBlock / kWave);
-            const int64_t nb = cdiv(U.bs_slice0[s + 1] - U.bs_slice0[s], kBlock / kWave);
-            const GL double *const p[4] = {part_at(a, U, SL_DD, j), part_at(a, U, SL_AD, j),
-                                           part_at(a, U, SL_NV, j), part_at(a, U, SL_DF, j)};
-            const int64_t n[4] = {nb, nr, npx, nr};
-            double r[4];
-            fin_reduce_n<4>(p, n, r, lds);
-            const double dd = r[0], ad2 = r[1], nv = r[2], df = r[3];
-            if (threadIdx.x == 0) {
-                sc->df_pass += df;
-                sc->dd = dd;
-                sc->ad2 = ad2;
-                sc->nv = nv;
-                const double den = (double)a.S * ad2 + P.alpha * nv;
-                sc->c = (den > 0) ? dd / den : 0.0;
-            }
-            break;
-        }
         case EF_EDGE_C: {
             if (sc->edge_done || s >= P.iter_edge) return;
             const GL double *const p[3] = {part_at(a, U, SL_E0, j), part_at(a, U, SL_E1, j),
@@ -1133,6 +1214,42 @@ __global__ __launch_bounds__(kBlock) void k_eng_fin(EngArgs<T> a, int op, int s)
             break;
         }
     }
+}
+
+// One block per job slot (units listed in m; m.nunits * kMaxK blocks).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_eng_fin(EngArgs<T> a, int op, int s) {
+    __shared__ double lds[4 * (kBlock / kWave)];
+    const int ui = blockIdx.x / kMaxK, j = blockIdx.x % kMaxK;
+    const UG<T> U(a.units[ui]);
+    if (j >= U.K) return;
+    if (op == EF_STEP) fin_step(a, U, j, s, lds);
+    else fin_job(a, U, j, op, s, lds);
+}
+
+enum PixOp { PX_AXPY = 0, PX_MEDIAN, PX_COPY_WU, PX_EDGE_GRAD, PX_EDGE_NABLA, PX_EDGE_UPDATE, PX_NV };
+
+template <typename T, int KMAX, int OP>
+__global__ __launch_bounds__(kBlock) void k_eng_pix(EngArgs<T> a, BlockMap m, int it) {
+    __shared__ double lds[kBlock / kWave * kMaxK];
+    // strips of the arrays the stencil reads around (u and v for the edge
+    // gradient, one array for the median, nV and the gradient of the edge
+    // direction)
+    constexpr int kStrips = OP == PX_EDGE_GRAD ? 2 : (OP == PX_MEDIAN || OP == PX_EDGE_NABLA || OP == PX_NV ? 1 : 0);
+    __shared__ __align__(16) T L[kStrips ? kStrips * kHaloN * KMAX : 1];
+    const int b = (int)xcd_block();
+    if (b >= m.start[m.nunits]) return;
+    const int ui = find_unit(m, b);
+    const UG<T> U(a.units[ui]);
+    if (unit_idle(a, U)) return;
+    const int lb = b - m.start[ui];
+    if constexpr (OP == PX_AXPY) { MI_KSWITCH(KMAX, (axpy_unit<T, K>(a, U, lb))) }
+    else if constexpr (OP == PX_MEDIAN) { MI_KSWITCH(KMAX, (median_unit<T, K>(a, U, lb, L))) }
+    else if constexpr (OP == PX_NV) { MI_KSWITCH(KMAX, (nv_unit<T, K>(a, U, lb, lds, L))) }
+    else if constexpr (OP == PX_COPY_WU) { MI_KSWITCH(KMAX, (copy_wu_unit<T, K>(U, lb))) }
+    else if constexpr (OP == PX_EDGE_GRAD) { MI_KSWITCH(KMAX, (edge_grad_unit<T, K>(a, U, lb, it, lds, L))) }
+    else if constexpr (OP == PX_EDGE_NABLA) { MI_KSWITCH(KMAX, (edge_nabla_unit<T, K>(a, U, lb, it, lds, L))) }
+    else { MI_KSWITCH(KMAX, (edge_update_unit<T, K>(a, U, lb, it, lds))) }
 }
 
 template <typename T>
@@ -1193,6 +1310,13 @@ void eng_fp_resid(hipStream_t st, const EngArgs<T> &a, const BlockMap &m, int s,
     MI_HIP(hipGetLastError());
 }
 
+// Dynamic LDS of a back-projection launch: the staged residual slots, and
+// after the walk the u and v squares of the regulariser (K <= kmax values).
+template <typename T>
+uint32_t bp_lds(size_t slot_bytes, int kmax) {
+    return (uint32_t)std::max(slot_bytes, (size_t)2 * kSqN * kmax * sizeof(T));
+}
+
 template <typename T>
 void eng_bp_grad(hipStream_t st, const EngArgs<T> &a, const BlockMap &m, int s, int kmax, size_t lds_bytes) {
     const unsigned grid = grid8(m.start[m.nunits]);
@@ -1209,13 +1333,13 @@ void eng_bp_grad(hipStream_t st, const EngArgs<T> &a, const BlockMap &m, int s, 
             }();
             (void)once;
         }
-        if (kmax == 1) launch(k_eng_bp<T, 1, true>, dim3(grid), dim3(kBlock), (uint32_t)(lds_bytes), st, a, m, s);
-        else if (kmax <= 4) launch(k_eng_bp<T, 4, true>, dim3(grid), dim3(kBlock), (uint32_t)(lds_bytes), st, a, m, s);
-        else launch(k_eng_bp<T, kMaxK, true>, dim3(grid), dim3(kBlock), (uint32_t)(lds_bytes), st, a, m, s);
+        if (kmax == 1) launch(k_eng_bp<T, 1, true>, dim3(grid), dim3(kBlock), bp_lds<T>(lds_bytes, 1), st, a, m, s);
+        else if (kmax <= 4) launch(k_eng_bp<T, 4, true>, dim3(grid), dim3(kBlock), bp_lds<T>(lds_bytes, 4), st, a, m, s);
+        else launch(k_eng_bp<T, kMaxK, true>, dim3(grid), dim3(kBlock), bp_lds<T>(lds_bytes, kMaxK), st, a, m, s);
     } else {
-        if (kmax == 1) launch(k_eng_bp<T, 1, false>, dim3(grid), dim3(kBlock), (uint32_t)(0), st, a, m, s);
-        else if (kmax <= 4) launch(k_eng_bp<T, 4, false>, dim3(grid), dim3(kBlock), (uint32_t)(0), st, a, m, s);
-        else launch(k_eng_bp<T, kMaxK, false>, dim3(grid), dim3(kBlock), (uint32_t)(0), st, a, m, s);
+        if (kmax == 1) launch(k_eng_bp<T, 1, false>, dim3(grid), dim3(kBlock), bp_lds<T>(0, 1), st, a, m, s);
+        else if (kmax <= 4) launch(k_eng_bp<T, 4, false>, dim3(grid), dim3(kBlock), bp_lds<T>(0, 4), st, a, m, s);
+        else launch(k_eng_bp<T, kMaxK, false>, dim3(grid), dim3(kBlock), bp_lds<T>(0, kMaxK), st, a, m, s);
     }
     MI_HIP(hipGetLastError());
 }
@@ -1245,12 +1369,13 @@ static void launch_pix(hipStream_t st, const EngArgs<T> &a, const BlockMap &m, i
 }
 
 template <typename T>
-void eng_axpy(hipStream_t st, const EngArgs<T> &a, const BlockMap &pix, int kmax) {
-    launch_pix<T, PX_AXPY>(st, a, pix, 0, kmax);
-}
-template <typename T>
 void eng_nv(hipStream_t st, const EngArgs<T> &a, const BlockMap &pix, int kmax) {
     launch_pix<T, PX_NV>(st, a, pix, 0, kmax);
+}
+
+template <typename T>
+void eng_axpy(hipStream_t st, const EngArgs<T> &a, const BlockMap &pix, int kmax) {
+    launch_pix<T, PX_AXPY>(st, a, pix, 0, kmax);
 }
 
 template <typename T>

@@ -161,6 +161,7 @@ __host__ __device__ __forceinline__ int64_t sell_pos(int64_t base, int t, int la
 // follow each other (pixel (i & 3, j & 7) at (j >> 2) * 16 + (i & 3) * 4 +
 // (j & 3)), so each 64-byte half line is a square.
 constexpr int kTileH = 4, kTileW = 8;
+constexpr int kStripTiles = 8;   // tiles of a stencil strip; each tile row holds a multiple of it
 __host__ __device__ __forceinline__ int64_t tile_index(int i, int j, int tny) {
     return ((int64_t)(i >> 2) * tny + (j >> 3)) * (kTileH * kTileW) + (((j >> 2) & 1) << 4) + ((i & 3) << 2) +
            (j & 3);

@@ -344,7 +344,9 @@ void build_slots(Matrix &m, const BItems &bi, int64_t nblk, hipStream_t st) {
 void build_sell(Matrix &m, hipStream_t st, bool drop_csr) {
     if (m.has_sell) return;
     const mi_geometry &g = m.g;
-    m.tny = (g.ny + kTileW - 1) / kTileW;
+    // tile rows padded to a multiple of 8 tiles: a stencil block (8 tiles) is
+    // then a 4 x 64 pixel strip of one tile row
+    m.tny = (g.ny + kStripTiles * kTileW - 1) / (kStripTiles * kTileW) * kStripTiles;
     m.npix_t = (int64_t)((g.nx + kTileH - 1) / kTileH) * kTileH * m.tny * kTileW;
     const bool f64 = m.has64;
     // forward
