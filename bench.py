@@ -204,6 +204,20 @@ def bench_c3(ctx, n_jobs, iters, log, plain_pass=True, tune=True):
         worst = max(worst, float(np.abs(u1 - u0).max() / max(np.abs(u0).max(), 1e-30)))
     b.close()
     work = n_jobs * iters
+    # Matrix-sharing bound of the mix: a unit walks its matrix once for its K
+    # jobs, so at equal per-entry rates the batch gains at most
+    # sum_jobs nnz / sum_units nnz over running the jobs one at a time (one
+    # unit per ceil(jobs / K cap) of a geometry; with the tuned caps), and
+    # sum_jobs nnz / sum_geometries nnz with every geometry's jobs in one unit.
+    per_geom = {}
+    for g, _, _ in jobs:
+        per_geom[g.key()] = per_geom.get(g.key(), 0) + 1
+    job_nnz = sum(nnz[k] * c for k, c in per_geom.items())
+    bound = {"sum_jobs_nnz": job_nnz, "one_unit_per_geometry": job_nnz / sum(nnz.values())}
+    if tune:
+        caps = {g.key(): tuned[f"{g.nx}x{g.n_views}x{g.n_bins}"]["k_cap"] for g in geoms.values()}
+        unit_nnz = sum(nnz[k] * -(-c // caps[k]) for k, c in per_geom.items())
+        bound["tuned_units"] = job_nnz / unit_nnz
     return {
         "workload": f"{n_jobs} fan-beam slices, N in {{256,384,512}}, views in {{360,720,1152}}, bins in "
                     f"{{512,736}}, I0 U[1e4,1e5], 8 subsets, {iters} AT iterations each",
@@ -217,6 +231,7 @@ def bench_c3(ctx, n_jobs, iters, log, plain_pass=True, tune=True):
         "jobs_bitwise_identical": same if plain_pass else None,
         "max_rel_diff": worst,
         "tuning": tuned,
+        "sharing_bound": bound,
     }
 
 
@@ -242,9 +257,10 @@ def bench_serving(ctx, n_jobs, rate, log):
     nnz = sum(ctx.prepare(g, 8) for g in geoms.values())
     log(f"[serve] {n_jobs} requests over {len(geoms)} geometries ({nnz / 1e9:.2f}e9 matrix entries) "
         f"prepared in {time.perf_counter() - t:.1f} s")
-    out = {"workload": f"{n_jobs} Poisson arrivals at {rate:g} jobs/s, N in {{256,384,512,768,1024}}, views in "
-                       f"{{360,720,1152,2304}}, bins 1.5N, I0 log-uniform [1e4,1e5], 8 subsets, "
-                       f"rel. data-fidelity change < 1e-4 or 200 AT iterations",
+    out = {"workload": f"the first {n_jobs} requests of the C5 stream (not C5 itself: the whole 2000-request "
+                       f"stream is bench.py --serve-full): Poisson arrivals at {rate:g} jobs/s, N in "
+                       f"{{256,384,512,768,1024}}, views in {{360,720,1152,2304}}, bins 1.5N, I0 log-uniform "
+                       f"[1e4,1e5], 8 subsets, rel. data-fidelity change < 1e-4 or 200 AT iterations",
            "geometries": len(geoms)}
     res = {}
     for mode in ("plain", "batched"):

@@ -345,16 +345,16 @@ __global__ __launch_bounds__(kBlock) void k_cat_edge_n(Dims dm, const float *__r
     __shared__ float lu[2][kPlane];  // plane z in slot z & 1
     const Tile t = tile_of(dm);
     put<false>(lu[t.z0 & 1], t, fetch(u, dm, t, t.z0, 0.0f), 0.0f);
-    // ring slot of step z: u(z + 1), fetched three steps ahead
-    Ahead R[3] = {fetch(u, dm, t, t.z0 + 1, 0.0f), fetch(u, dm, t, t.z0 + 2, 0.0f), fetch(u, dm, t, t.z0 + 3, 0.0f)};
+    // ring slot of step z: u(z + 1), fetched two steps ahead
+    Ahead R[2] = {fetch(u, dm, t, t.z0 + 1, 0.0f), fetch(u, dm, t, t.z0 + 2, 0.0f)};
     __syncthreads();
     Nb U0, U1;
     nbhd(U0, lu[t.z0 & 1]);
     // slice z from u on z, z + 1; stages u(z + 1) into un, refills the ring
-    // slot for z + 3
+    // slot for z + 2
     auto step = [&](int z, const Nb &uc, Nb &un, Ahead &r) {
         put<false>(lu[(z + 1) & 1], t, r, 0.0f);
-        if (z + 3 < t.z1) r = fetch(u, dm, t, z + 4, 0.0f);
+        if (z + 2 < t.z1) r = fetch(u, dm, t, z + 3, 0.0f);
         __syncthreads();
         nbhd(un, lu[(z + 1) & 1]);
         if (t.in) {
@@ -370,14 +370,9 @@ __global__ __launch_bounds__(kBlock) void k_cat_edge_n(Dims dm, const float *__r
             nn[vidx(dm, t, z)] = (g[0] + 0.5f * g[1]) + (1.0f / 3.0f) * g[2];
         }
     };
-    // the planes rotate through U0 and U1 (two slots), the ring through three
-    for (int z = t.z0; z < t.z1; z += 6) {
+    for (int z = t.z0; z < t.z1; z += 2) {
         step(z, U0, U1, R[0]);
         if (z + 1 < t.z1) step(z + 1, U1, U0, R[1]);
-        if (z + 2 < t.z1) step(z + 2, U0, U1, R[2]);
-        if (z + 3 < t.z1) step(z + 3, U1, U0, R[0]);
-        if (z + 4 < t.z1) step(z + 4, U0, U1, R[1]);
-        if (z + 5 < t.z1) step(z + 5, U1, U0, R[2]);
     }
 }
 
