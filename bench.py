@@ -255,8 +255,10 @@ def bench_serving(ctx, n_jobs, rate, log):
         p = configs.at_params(configs.ALPHA, configs.BETA, configs.EPS, n_subsets=8, max_iters=200, rel_tol=1e-4)
         reqs.append(dict(t=r["t"], geom=g, sino=sino, params=p))
     nnz = sum(ctx.prepare(g, 8) for g in geoms.values())
+    for g in geoms.values():        # launch settings per geometry, as the full stream uses them
+        ctx.engine_autotune(g, 8)
     log(f"[serve] {n_jobs} requests over {len(geoms)} geometries ({nnz / 1e9:.2f}e9 matrix entries) "
-        f"prepared in {time.perf_counter() - t:.1f} s")
+        f"prepared and tuned in {time.perf_counter() - t:.1f} s")
     out = {"workload": f"the first {n_jobs} requests of the C5 stream (not C5 itself: the whole 2000-request "
                        f"stream is bench.py --serve-full): Poisson arrivals at {rate:g} jobs/s, N in "
                        f"{{256,384,512,768,1024}}, views in {{360,720,1152,2304}}, bins 1.5N, I0 log-uniform "

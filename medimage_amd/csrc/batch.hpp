@@ -67,14 +67,17 @@ struct Batch {
     int min_k = 1;                 // smallest K of a new unit
     int max_k = kMaxK;             // largest K of a unit
     int64_t max_entries = 0;       // admission budget: matrix entries of the active jobs (0: none)
-    // Largest K of a unit over m: the unit's K interleaved images, which the
-    // projectors gather from, stay within one XCD's 4 MiB L2 (K = 4 at 512^2
-    // FP32, 8 at 256^2); at most max_k.
+    // Largest K of a unit over m (at most max_k): the tuned value, else 8 for
+    // images of up to 2^17 pixels and 4 above.  A K-wide FP32 gather is one
+    // 16-byte load up to K = 4 and two at K = 8, so K = 4 costs a large image
+    // little more per matrix entry than K = 1 while sharing the entry four
+    // ways; small images also amortise the launches at K = 8.  Measured with
+    // the engine tuner on the 20 C5 geometries (N = 256 .. 1024): K = 8 is
+    // fastest per job for 256^2, K = 4 for 512^2 to 1024^2 (1024^2 x 2304
+    // views: 5.1 ms per job pass against 14.1 ms at K = 1).
     int k_cap(const Matrix &m) const {
         if (m.etune.k_cap > 0) return std::min(m.etune.k_cap, max_k);
-        int k = max_k;
-        while (k > 1 && (int64_t)k * m.npix_t * (int64_t)sizeof(T) > (int64_t)(4 << 20)) k >>= 1;
-        return k;
+        return std::min(m.npix <= (int64_t)1 << 17 ? 8 : 4, max_k);
     }
     // Back projection staged in LDS for unit U of matrix m needing `need`
     // bytes of slots (MI_BATCH_BP_LDS off: never).
