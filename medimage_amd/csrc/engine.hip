@@ -643,32 +643,35 @@ __device__ __forceinline__ void bp_unit(const EngArgs<T> &a, const UG<T> &U, int
         const GL int32_t *vlo = U.bs_vlo + U.bs_vtab0[s] + (int64_t)lb * nv;
         // the same offsets as voff_at (runtime.hpp)
         const GL int32_t *voff = U.bs_voff + (U.bs_vtab0[s] + (int64_t)s * nblk + (int64_t)lb * (nv + 1));
-        // wave w copies views w, w + 4, ...; eight views' loads in flight
-        constexpr int VU = 8, NW = kBlock / kWave;
-        for (int v0 = wv; v0 < nv; v0 += NW * VU)
-            for (int k0 = 0;; k0 += kWave) {
-                T t[VU][K];
-                int o[VU], c[VU];
-                bool more = false;
-#pragma unroll
-                for (int u = 0; u < VU; ++u) {
-                    const int v = v0 + NW * u;
-                    o[u] = c[u] = 0;
-                    if (v < nv) {
-                        o[u] = voff[v];
-                        c[u] = voff[v + 1] - o[u];
-                        more = more || c[u] > k0 + kWave;
-                        if (k0 + lane < c[u])
-                            loadk<T, K>(U.r + (U.sub_row0[s] + (int64_t)v * U.nbins + vlo[v] + k0 + lane) * K, t[u]);
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < VU; ++u)
-                    if (k0 + lane < c[u])
-#pragma unroll
-                        for (int j = 0; j < K; ++j) ldr[(o[u] + k0 + lane) * K + j] = t[u][j];
-                if (!__any(more)) break;   // wave-uniform exit
+        // Wave w copies views w, w + 4, ...: a view's slots are c consecutive
+        // residual rows (K values each), so the copy is one contiguous run of
+        // dwords, moved by direct global-to-LDS loads (no register round
+        // trip; every load of the block in flight until the one wait).
+        // The views' slot offsets and first rows are read 64 views at a time,
+        // one view per lane, and taken from the lanes in turn, so that no
+        // load waits between the direct loads.
+        constexpr int NW = kBlock / kWave, DW = (int)(sizeof(T) / 4) * K;
+        for (int vb = wv; vb < nv; vb += NW * kWave) {
+            const int vl = vb + NW * lane;
+            int o_l = 0, c_l = 0, row_l = 0;   // row within the subset's residual rows
+            if (vl < nv) {
+                o_l = voff[vl];
+                c_l = voff[vl + 1] - o_l;
+                row_l = vl * U.nbins + vlo[vl];
             }
+            const int nb = min(kWave, (nv - vb + NW - 1) / NW);
+            const GL uint32_t *r0 = (const GL uint32_t *)(U.r + U.sub_row0[s] * K);
+            for (int i = 0; i < nb; ++i) {
+                const int o = __builtin_amdgcn_readlane(o_l, i), n = __builtin_amdgcn_readlane(c_l, i) * DW;
+                const GL uint32_t *src = r0 + (int64_t)__builtin_amdgcn_readlane(row_l, i) * DW;
+                LDSP uint32_t *dst = (LDSP uint32_t *)(ldr + (int64_t)o * K);
+                for (int e0 = 0; e0 < n; e0 += kWave)
+                    if (e0 + lane < n)
+                        __builtin_amdgcn_global_load_lds((const GL void *)(src + e0 + lane), (LDSP void *)(dst + e0), 4,
+                                                         0, 0);
+            }
+        }
+        __builtin_amdgcn_s_waitcnt(0);   // the block's direct loads have landed in LDS
         __syncthreads();
     }
     const bool has = li < U.bs_slice0[s + 1] - sl0;
