@@ -19,14 +19,14 @@ constexpr int kMaxK = 8;             // jobs per unit
 constexpr int kMaxUnits = 64;        // units per batch
 constexpr int kPixBlock = kBlock;    // tiled pixel positions per stencil block
 // Largest dynamic LDS of a back-projection block (the staged residual
-// slots); units above it run in the global-gather launch.  Measured: the
-// staged form wins where the slots are few (512^2 x 720 views, K = 1:
-// ~8 KiB, 48.5 -> 42.5 us per launch); for K = 4 units of 1152-view
-// geometries (tens of KiB) the copy and the lower occupancy cost more than
-// the gathers save (64-job batch: 2632 it/s all global, 2256 with 64 KiB,
-// 1851 with 96 KiB, 2627 with 16 KiB).
+// slots); units above it run in the global-gather launch.  Measured with
+// the slots staged by direct global-to-LDS loads: the 64-job batch (no
+// tuning) runs 2829 AT iterations/s with a 24 KiB limit, 2903 with 40 KiB,
+// 2911 with 56 KiB.  (With the slots staged through registers the copy cost
+// more than the gathers saved above ~24 KiB: 2632 it/s all global, 2256 with
+// 64 KiB.)
 #ifndef MI_BP_LDS_KB
-#define MI_BP_LDS_KB 24
+#define MI_BP_LDS_KB 56
 #endif
 constexpr size_t kBpLdsMax = (size_t)MI_BP_LDS_KB << 10;
 // The most a tuned geometry may stage (within the 64 KiB a launch gets
