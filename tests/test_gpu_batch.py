@@ -238,6 +238,40 @@ def test_batch_bp_lds_equals_global_gathers(ctx, precision):
         assert _same(a[i][0], b[i][0]) and _same(a[i][1], b[i][1]), f"job {i}"
 
 
+@pytest.mark.parametrize("precision", [mi.MI_FP32, mi.MI_FP64])
+def test_batch_bp_lds_many_views_per_subset(ctx, precision):
+    """300 views per subset: each wave of a back-projection block stages its
+    views' residual slots in more than one round of 64 views (a 16 x 16
+    block of 2 mm pixels covers at most 21 of the 4 mm bins of a view, so
+    the slots take at most 25 KiB per job in FP32 and stay staged for a unit
+    of one job, and of two in FP32).  Staged and global gathers give the same
+    bits, and FP64 matches the oracle."""
+    g = mi.fan_geometry(48, 2.0, 600, 32, 4.0)
+    dt = np.float64 if precision == mi.MI_FP64 else np.float32
+    jobs = []
+    for i in range(2):
+        _, sino = configs.make_case(ctx, g, 3e4, seed=300 + i)
+        jobs.append((g, sino.astype(dt), mi.at_params(1e-3, 1e-4, 1e-2, n_subsets=2, max_iters=3,
+                                                        precision=precision)))
+    for sel in (jobs[:1], jobs):
+        runs = []
+        for lds in (1, 0):
+            b = ctx.batch(2, precision)
+            b.set(mi.MI_BATCH_BP_LDS, lds)
+            ids = [b.submit(*j) for j in sel]
+            b.run()
+            runs.append([b.read(i) for i in ids])
+            b.close()
+        for (ua, va), (ub, vb) in zip(*runs):
+            assert _same(ua, ub) and _same(va, vb)
+        if precision == mi.MI_FP64 and len(sel) == 1:
+            gs, sino, p = sel[0]
+            u, v, n, _ = O.os_at(O.op_fan(O.fan_from(gs)), sino, 1e-3, 1e-4, 1e-2, 2, 3)
+            ud, vd = runs[0][0]
+            assert np.linalg.norm(ud - u) <= 1e-10 * np.linalg.norm(u)
+            assert np.linalg.norm(vd - v) <= 1e-10 * np.linalg.norm(v)
+
+
 def test_batch_release_and_errors(ctx):
     g = GEOMS["a"]
     _, sino = configs.make_case(ctx, g, 3e4, seed=9)
