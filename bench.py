@@ -364,6 +364,45 @@ def bench_serving_full(ctx, n_jobs, rate, log, tune=True, workers=16):
     return out
 
 
+def bench_k_scaling(ctx, log):
+    """Jobs per unit: for geometries of the C5 mix, the time per job and
+    pass of a unit of K identical jobs (K = 1, 2, 4, 8, set through the
+    engine's tuning setting, residual staged in LDS where it fits) and the
+    per-phase times of an instrumented pass (HIP events per phase)."""
+    import medimage_amd as mi
+    from medimage_amd import configs
+    out = {}
+    for (n, v, b) in [(256, 720, 384), (512, 720, 768), (768, 1152, 1152), (1024, 2304, 1536)]:
+        g = configs.mixed_geometry(n, v, b)
+        ctx.prepare(g, 8)
+        sino = np.full((g.n_views, g.n_bins), 0.5, np.float32)
+        p = configs.at_params(configs.ALPHA, configs.BETA, configs.EPS, n_subsets=8, max_iters=0)
+        rows = {}
+        for K in (1, 2, 4, 8):
+            ctx.engine_tuning(g, 8, set_to=mi.EngineTuning(K, -1, 0.0))
+            bt = ctx.batch(8, min_k=K)
+            for _ in range(K):
+                bt.submit(g, sino, p)
+            bt.step(2)
+            t0 = time.perf_counter()
+            bt.step(4)
+            t1 = time.perf_counter()
+            ctx.stats_reset()
+            ctx.stats_enable(True)
+            bt.step(2)
+            ctx.stats_enable(False)
+            ph = {name: ctx.stats(k).total_ms / 2 for name, k in
+                  (("forward_walk", mi.MI_K_FP), ("back_projection", mi.MI_K_BP), ("direction_walk", mi.MI_K_FPSQ),
+                   ("nv_step_update", mi.MI_K_STEP), ("edge", mi.MI_K_EDGE))}
+            rows[K] = {"ms_per_job_pass": (t1 - t0) * 1e3 / 4 / K, "units": bt.units(), "phase_ms_per_pass": ph}
+            log(f"[k-scaling] {n}^2 x {v} x {b} K={K}: {rows[K]['ms_per_job_pass']:.3f} ms per job pass")
+            bt.close()
+        ctx.engine_tuning(g, 8, set_to=mi.EngineTuning(0, -1, 0.0))
+        out[f"{n}x{v}x{b}"] = rows
+        ctx.trim()
+    return out
+
+
 def bench_c4(ctx, reps, at_iters, log):
     """C4 cone beam (512^3 volume, 720 views of the 1024 x 768 panel): the
     forward and gather back projection kernels (HIP-event time; host copies
@@ -445,6 +484,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-full", action="store_true",
                     help="only the whole C5 stream (--serve-jobs 2000 by default here): batched live, plain derived")
+    ap.add_argument("--k-scaling", action="store_true",
+                    help="only the jobs-per-unit scaling of four C5 geometries (K = 1, 2, 4, 8)")
     ap.add_argument("--c4-reps", type=int, default=2)
     ap.add_argument("--c4-iters", type=int, default=2, help="AT passes of the C4 cone-beam solve (0: projectors only)")
     ap.add_argument("--no-c4", action="store_true")
@@ -460,6 +501,10 @@ def main():
     from medimage_amd import configs
 
     ctx = mi.Context(0)
+    if args.k_scaling:
+        print(json.dumps({"k_scaling": bench_k_scaling(ctx, lambda m: print(m, file=sys.stderr, flush=True))}))
+        ctx.close()
+        return
     if args.serve_full:
         n = args.serve_jobs if args.serve_jobs != 48 else 2000
         out = bench_serving_full(ctx, n, args.serve_rate, lambda m: print(m, file=sys.stderr, flush=True))
