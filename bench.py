@@ -526,7 +526,7 @@ def main():
     job.synchronize()
 
     names = {mi.MI_K_FP: "fp_residual", mi.MI_K_BP: "bp_gather_grad", mi.MI_K_FPSQ: "fp_direction_norm"}
-    kernel_symbol = {mi.MI_K_FP: "k_eng_fp<float, 1, true, 1>", mi.MI_K_BP: "k_eng_bp<float, 1, true>",
+    kernel_symbol = {mi.MI_K_FP: "k_eng_fp<float, 1, true, 1>", mi.MI_K_BP: "k_eng_bp<float, 1, true, true>",
                      mi.MI_K_FPSQ: "k_eng_fp<float, 1, false, 1>"}
 
     def per_kernel_stats():

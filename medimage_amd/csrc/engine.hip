@@ -96,6 +96,13 @@ __device__ __forceinline__ GL Scalars &jsc(const EngArgs<T> &a, int js) {
 #ifndef MI_NT_BP
 #define MI_NT_BP 1      // ... and in the back projector
 #endif
+// Staged back projection: the walk's first stream loads issued with the
+// residual copies, before their wait (0: never, 1: K = 1 only, 2: every K).
+// Measured on the 64-job batch: with every K the K <= 4 staged launch took
+// 678 us against 617 us unprimed; K = 1 only: 622 us.
+#ifndef MI_BP_PRIME
+#define MI_BP_PRIME 1
+#endif
 #ifndef MI_STEPS
 #define MI_STEPS 8      // steps of a projector lane in flight (K <= 2; 4 above)
 #endif
@@ -250,51 +257,74 @@ enum IdxMode { IM_ABS32 = 0, IM_DELTA16 = 1, IM_SLOT16 = 2 };
 // IM: absolute 32-bit indices (ip), 16-bit differences (dp, from `cur`), or
 // 16-bit LDS slots (dp read unsigned).  LDSG: gather from `lx` (LDS) instead
 // of the global array x.
-template <typename T, int K, bool NT, int IM, bool LDSG = false>
+// Stream registers of one pipeline stage of walk: GR groups of kGroup
+// entries (index word and length) per lane.
+template <typename T, int K>
+struct Stage {
+    static constexpr int GR = (K <= 2) ? MI_STEPS / kGroup : 1;   // groups per pipeline stage
+    uint32_t sv[GR][kGroup];
+    T lv[GR][kGroup];
+};
+
+// Issues the stream loads of groups g0 .. g0 + GR - 1 of a lane into b.
+template <typename T, int K, bool NT, int IM>
+__device__ __forceinline__ void stage_fetch(Stage<T, K> &b, const GL uint32_t *__restrict__ ip,
+                                            const GL int16_t *__restrict__ dp, const GL T *__restrict__ lp, int g0) {
+    constexpr bool D16 = IM != IM_ABS32;
+    constexpr int64_t kStride = kGroup * kWave;   // entries between a lane's groups
+#pragma unroll
+    for (int r = 0; r < Stage<T, K>::GR; ++r) {
+        const int64_t o = (int64_t)(g0 + r) * kStride;
+        if constexpr (IM == IM_SLOT16) {
+            const u32x2 a = stream_ld<NT>((const GL u32x2 *)(dp + o));
+            b.sv[r][0] = a.x & 0xffffu;
+            b.sv[r][1] = a.x >> 16;
+            b.sv[r][2] = a.y & 0xffffu;
+            b.sv[r][3] = a.y >> 16;
+        } else if constexpr (D16) {
+            const u32x2 a = stream_ld<NT>((const GL u32x2 *)(dp + o));
+            b.sv[r][0] = (uint32_t)(int32_t)(int16_t)(a.x & 0xffffu);
+            b.sv[r][1] = (uint32_t)(int32_t)(int16_t)(a.x >> 16);
+            b.sv[r][2] = (uint32_t)(int32_t)(int16_t)(a.y & 0xffffu);
+            b.sv[r][3] = (uint32_t)(int32_t)(int16_t)(a.y >> 16);
+        } else {
+            const u32x4 a = stream_ld<NT>((const GL u32x4 *)(ip + o));
+            b.sv[r][0] = a.x; b.sv[r][1] = a.y; b.sv[r][2] = a.z; b.sv[r][3] = a.w;
+        }
+        if constexpr (sizeof(T) == 4) {
+            const u32x4 a = stream_ld<NT>((const GL u32x4 *)(lp + o));
+            __builtin_memcpy(&b.lv[r][0], &a, 16);
+        } else {
+            const u32x4 a = stream_ld<NT>((const GL u32x4 *)(lp + o));
+            const u32x4 c = stream_ld<NT>((const GL u32x4 *)(lp + o) + 1);
+            __builtin_memcpy(&b.lv[r][0], &a, 16);
+            __builtin_memcpy(&b.lv[r][2], &c, 16);
+        }
+    }
+}
+
+// The first stage of a walk over n entries (none when the walk has no full
+// stage); a walk with PRIMED set starts from it.
+template <typename T, int K, bool NT, int IM>
+__device__ __forceinline__ void walk_prime(Stage<T, K> &b, const GL uint32_t *ip, const GL int16_t *dp,
+                                           const GL T *lp, int n) {
+    if (n / kGroup / Stage<T, K>::GR > 0) stage_fetch<T, K, NT, IM>(b, ip, dp, lp, 0);
+}
+
+template <typename T, int K, bool NT, int IM, bool LDSG = false, bool PRIMED = false>
 __device__ __forceinline__ void walk(const GL uint32_t *__restrict__ ip, const GL int16_t *__restrict__ dp,
                                      uint32_t cur, const GL T *__restrict__ lp, int n, const GL T *__restrict__ x,
-                                     uint32_t xbytes, Acc<T> (&acc)[K], const LDSP T *lx = nullptr) {
+                                     uint32_t xbytes, Acc<T> (&acc)[K], Stage<T, K> &b,
+                                     const LDSP T *lx = nullptr) {
     constexpr bool D16 = IM != IM_ABS32;
-    constexpr int GR = (K <= 2) ? MI_STEPS / kGroup : 1;   // groups per pipeline stage
+    constexpr int GR = Stage<T, K>::GR;
     constexpr int64_t kStride = kGroup * kWave;             // entries between a lane's groups
 #if MI_BUF_GATHER
     const __amdgpu_buffer_rsrc_t xr = make_rsrc((const void *)x, xbytes);
 #else
     (void)xbytes;
 #endif
-    uint32_t sv[GR][kGroup];
-    T lv[GR][kGroup];
-    auto fetch = [&](int g0) {
-#pragma unroll
-        for (int r = 0; r < GR; ++r) {
-            const int64_t o = (int64_t)(g0 + r) * kStride;
-            if constexpr (IM == IM_SLOT16) {
-                const u32x2 a = stream_ld<NT>((const GL u32x2 *)(dp + o));
-                sv[r][0] = a.x & 0xffffu;
-                sv[r][1] = a.x >> 16;
-                sv[r][2] = a.y & 0xffffu;
-                sv[r][3] = a.y >> 16;
-            } else if constexpr (D16) {
-                const u32x2 a = stream_ld<NT>((const GL u32x2 *)(dp + o));
-                sv[r][0] = (uint32_t)(int32_t)(int16_t)(a.x & 0xffffu);
-                sv[r][1] = (uint32_t)(int32_t)(int16_t)(a.x >> 16);
-                sv[r][2] = (uint32_t)(int32_t)(int16_t)(a.y & 0xffffu);
-                sv[r][3] = (uint32_t)(int32_t)(int16_t)(a.y >> 16);
-            } else {
-                const u32x4 a = stream_ld<NT>((const GL u32x4 *)(ip + o));
-                sv[r][0] = a.x; sv[r][1] = a.y; sv[r][2] = a.z; sv[r][3] = a.w;
-            }
-            if constexpr (sizeof(T) == 4) {
-                const u32x4 a = stream_ld<NT>((const GL u32x4 *)(lp + o));
-                __builtin_memcpy(&lv[r][0], &a, 16);
-            } else {
-                const u32x4 a = stream_ld<NT>((const GL u32x4 *)(lp + o));
-                const u32x4 b = stream_ld<NT>((const GL u32x4 *)(lp + o) + 1);
-                __builtin_memcpy(&lv[r][0], &a, 16);
-                __builtin_memcpy(&lv[r][2], &b, 16);
-            }
-        }
-    };
+    auto fetch = [&](int g0) { stage_fetch<T, K, NT, IM>(b, ip, dp, lp, g0); };
     auto gather = [&](uint32_t id, T (&o)[K]) {
         if constexpr (LDSG) {
 #pragma unroll
@@ -311,7 +341,7 @@ __device__ __forceinline__ void walk(const GL uint32_t *__restrict__ ip, const G
     // Software pipeline: the next stage's stream loads are issued after the
     // current stage's gathers (the barriers keep that order) and before they
     // are consumed, so a lane waits for one memory round trip per stage.
-    if (full > 0) fetch(0);
+    if (!PRIMED && full > 0) fetch(0);
     int g = 0;
     for (int c = 0; c < full; ++c, g += GR) {
         uint32_t id[GR][kGroup];
@@ -321,12 +351,12 @@ __device__ __forceinline__ void walk(const GL uint32_t *__restrict__ ip, const G
 #pragma unroll
             for (int q = 0; q < kGroup; ++q) {
                 if constexpr (IM == IM_DELTA16) {
-                    cur += sv[r][q];
+                    cur += b.sv[r][q];
                     id[r][q] = cur;
                 } else {
-                    id[r][q] = sv[r][q];
+                    id[r][q] = b.sv[r][q];
                 }
-                ln[r][q] = lv[r][q];
+                ln[r][q] = b.lv[r][q];
             }
 #pragma unroll
         for (int r = 0; r < GR; ++r)
@@ -383,8 +413,9 @@ __device__ __forceinline__ void walk_slice(const GL int64_t *base, const GL int3
     const int64_t p0 = sell_pos(base[sl], 0, lane);
     const int n = Tn[sl];
     const uint32_t xb = (uint32_t)(xn * K * (int64_t)sizeof(T));
-    if (d16) walk<T, K, NT, IM_DELTA16>(nullptr, d16 + p0, first[sl * kWave + lane], len + p0, n, x, xb, acc);
-    else walk<T, K, NT, IM_ABS32>(idx + p0, nullptr, 0u, len + p0, n, x, xb, acc);
+    Stage<T, K> b;
+    if (d16) walk<T, K, NT, IM_DELTA16>(nullptr, d16 + p0, first[sl * kWave + lane], len + p0, n, x, xb, acc, b);
+    else walk<T, K, NT, IM_ABS32>(idx + p0, nullptr, 0u, len + p0, n, x, xb, acc, b);
 }
 
 // One lane of the checkpointed forward walk: n Siddon steps from the stored
@@ -607,6 +638,32 @@ __device__ __forceinline__ void stage_square(const UG<T> &U, const GL T *arr, in
     __syncthreads();
 }
 
+// The same square as stage_square, in the same layout, moved by direct
+// global-to-LDS loads (positions outside the image get `pad` by LDS stores).
+// Nothing waits here: the caller waits for every load (s_waitcnt 0) and
+// synchronises the block before reading L.
+template <typename T, int K>
+__device__ __forceinline__ void dma_square(const UG<T> &U, const GL T *arr, int i0, int j0, T pad, LDSP T *L) {
+    constexpr int DW = (int)(sizeof(T) / 4), PD = K * DW, N = kSqN * PD;   // dwords per pixel, per square
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    uint32_t pw[DW];
+    __builtin_memcpy(pw, &pad, sizeof(T));
+    const GL uint32_t *src = (const GL uint32_t *)arr;
+    LDSP uint32_t *dst = (LDSP uint32_t *)L;
+    for (int b = wv * kWave; b < N; b += kBlock) {
+        const int t = b + lane;
+        if (t < N) {
+            const int p = t / PD, e = t - p * PD;
+            const int i = i0 + p / kSqW - 1, j = j0 + p % kSqW - 1;
+            if (i >= 0 && j >= 0 && i < U.nx && j < U.ny)
+                __builtin_amdgcn_global_load_lds((const GL void *)(src + tile_index(i, j, U.tny) * PD + e),
+                                                 (LDSP void *)(dst + b), 4, 0, 0);
+            else
+                dst[t] = (DW == 1 || (e & 1) == 0) ? pw[0] : pw[DW - 1];
+        }
+    }
+}
+
 // GoldenC image-gradient regulariser of K interleaved jobs at square pixel
 // (r, c): u zero padded, v one padded (origin/GoldenC/src/main.c:112-125).
 template <typename T, int K>
@@ -628,16 +685,34 @@ __device__ __forceinline__ void gc_reg(const LDSP T *Lu, const LDSP T *Lv, int r
 // wave w holds tile row w of the block).  LDS: the block first copies the
 // residual values of every (view, bin) its entries use into `ldr` (slots of
 // the block's table, `vlo` / `voff`), then gathers from there.
-template <typename T, int K, bool LDS>
+template <typename T, int K, bool LDS, bool SQ>
 __device__ __forceinline__ void bp_unit(const EngArgs<T> &a, const UG<T> &U, int s, int lb, double *lds,
-                                        LDSP T *ldr) {
+                                        LDSP T *ldr, int sq_off) {
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
-    const int64_t sl0 = U.bs_slice0[s], li = (int64_t)lb * (kBlock / kWave) + wv;
+    const int64_t sl0 = U.bs_slice0[s], li = (int64_t)lb * (kBlock / kWave) + wv, sl = sl0 + li;
+    const bool has = li < U.bs_slice0[s + 1] - sl0;
     const int tnx = (U.nx + kTileH - 1) / kTileH;
+    const int nbc = (U.tny + kBpBlockTC - 1) / kBpBlockTC;
+    const int i0 = (lb / nbc) * (kBpBlockTR * kTileH), j0 = (lb % nbc) * (kBpBlockTC * kTileW);
     double m[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) m[j] = 0.0;
+    // u and v of the square for the regulariser: with SQ in their own LDS
+    // region at sq_off, loaded now, behind the walk; otherwise after the
+    // walk, in the LDS the residual slots used.
+    LDSP T *Lu = ldr + (SQ ? sq_off : 0), *Lv = Lu + kSqN * K;
+    if constexpr (SQ) {
+        dma_square<T, K>(U, U.u, i0, j0, T(0), Lu);
+        dma_square<T, K>(U, U.v, i0, j0, T(1), Lv);
+    }
+    Stage<T, K> st;
+    int64_t p0 = 0;
+    int n = 0;
     if (LDS) {
+        if (has) {
+            p0 = sell_pos(U.bs_base[sl], 0, lane);
+            n = U.bs_T[sl];
+        }
         const int nv = (int)((U.sub_row0[s + 1] - U.sub_row0[s]) / U.nbins);
         const int64_t nblk = bp_blocks_of(tnx, U.tny);
         const GL int32_t *vlo = U.bs_vlo + U.bs_vtab0[s] + (int64_t)lb * nv;
@@ -662,29 +737,35 @@ __device__ __forceinline__ void bp_unit(const EngArgs<T> &a, const UG<T> &U, int
             const int nb = min(kWave, (nv - vb + NW - 1) / NW);
             const GL uint32_t *r0 = (const GL uint32_t *)(U.r + U.sub_row0[s] * K);
             for (int i = 0; i < nb; ++i) {
-                const int o = __builtin_amdgcn_readlane(o_l, i), n = __builtin_amdgcn_readlane(c_l, i) * DW;
+                const int o = __builtin_amdgcn_readlane(o_l, i), nd = __builtin_amdgcn_readlane(c_l, i) * DW;
                 const GL uint32_t *src = r0 + (int64_t)__builtin_amdgcn_readlane(row_l, i) * DW;
                 LDSP uint32_t *dst = (LDSP uint32_t *)(ldr + (int64_t)o * K);
-                for (int e0 = 0; e0 < n; e0 += kWave)
-                    if (e0 + lane < n)
+                for (int e0 = 0; e0 < nd; e0 += kWave)
+                    if (e0 + lane < nd)
                         __builtin_amdgcn_global_load_lds((const GL void *)(src + e0 + lane), (LDSP void *)(dst + e0), 4,
                                                          0, 0);
             }
         }
+        // The walk's first stream loads go out with the copies, so that the
+        // one wait covers both.
+        if (has && (MI_BP_PRIME == 2 || (MI_BP_PRIME == 1 && K == 1)))
+            walk_prime<T, K, MI_NT_BP, IM_SLOT16>(st, nullptr, (const GL int16_t *)(U.bs_slot16 + p0),
+                                                  U.bs_len + p0, n);
         __builtin_amdgcn_s_waitcnt(0);   // the block's direct loads have landed in LDS
         __syncthreads();
     }
-    const bool has = li < U.bs_slice0[s + 1] - sl0;
     double tot[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) tot[j] = 0.0;
     if (has) {
-        const int64_t sl = sl0 + li;
         Acc<T> acc[K];
         if (LDS) {
-            const int64_t p0 = sell_pos(U.bs_base[sl], 0, lane);
-            walk<T, K, MI_NT_BP, IM_SLOT16, true>(nullptr, (const GL int16_t *)(U.bs_slot16 + p0), 0u,
-                                                  U.bs_len + p0, U.bs_T[sl], U.r, 0u, acc, ldr);
+            if (MI_BP_PRIME == 2 || (MI_BP_PRIME == 1 && K == 1))
+                walk<T, K, MI_NT_BP, IM_SLOT16, true, true>(nullptr, (const GL int16_t *)(U.bs_slot16 + p0), 0u,
+                                                            U.bs_len + p0, n, U.r, 0u, acc, st, ldr);
+            else
+                walk<T, K, MI_NT_BP, IM_SLOT16, true, false>(nullptr, (const GL int16_t *)(U.bs_slot16 + p0), 0u,
+                                                             U.bs_len + p0, n, U.r, 0u, acc, st, ldr);
         } else {
             walk_slice<T, K, MI_NT_BP>(U.bs_base, U.bs_T, U.bs_idx, U.bs_first, U.bs_d16, U.bs_len, sl, lane, U.r,
                                        U.nrows, acc);
@@ -692,14 +773,14 @@ __device__ __forceinline__ void bp_unit(const EngArgs<T> &a, const UG<T> &U, int
 #pragma unroll
         for (int j = 0; j < K; ++j) tot[j] = acc[j].value();
     }
-    // u and v of the square for the regulariser, in the LDS the residual
-    // slots used (every wave has finished its walk)
-    const int nbc = (U.tny + kBpBlockTC - 1) / kBpBlockTC;
-    const int i0 = (lb / nbc) * (kBpBlockTR * kTileH), j0 = (lb % nbc) * (kBpBlockTC * kTileW);
-    LDSP T *Lu = ldr, *Lv = ldr + kSqN * K;
-    __syncthreads();
-    stage_square<T, K>(U, U.u, i0, j0, T(0), Lu);
-    stage_square<T, K>(U, U.v, i0, j0, T(1), Lv);
+    if constexpr (!SQ) {
+        __syncthreads();   // every wave has finished its walk
+        stage_square<T, K>(U, U.u, i0, j0, T(0), Lu);
+        stage_square<T, K>(U, U.v, i0, j0, T(1), Lv);
+    } else if constexpr (!LDS) {
+        __builtin_amdgcn_s_waitcnt(0);   // the squares' direct loads
+        __syncthreads();
+    }
     if (has) {
         const int64_t q = bp_item(lb, wv, lane, tnx, U.tny);
         if (q >= 0) {
@@ -1055,8 +1136,8 @@ __global__ __launch_bounds__(kBlock) void k_eng_fp(EngArgs<T> a, BlockMap m, int
     MI_KSWITCH(KMAX, (fp_unit<T, K, RESID, FM>(a, U, s, lb, lds, ldr)))
 }
 
-template <typename T, int KMAX, bool LDS>
-__global__ __launch_bounds__(kBlock) void k_eng_bp(EngArgs<T> a, BlockMap m, int s) {
+template <typename T, int KMAX, bool LDS, bool SQ>
+__global__ __launch_bounds__(kBlock) void k_eng_bp(EngArgs<T> a, BlockMap m, int s, int sq_off) {
     __shared__ double lds[kBlock / kWave * kMaxK];
     extern __shared__ __align__(16) unsigned char ldr_raw[];   // LDS: the block's residual slots
     const int b = (int)xcd_block();
@@ -1065,7 +1146,7 @@ __global__ __launch_bounds__(kBlock) void k_eng_bp(EngArgs<T> a, BlockMap m, int
     const UG<T> U(a.units[ui]);
     if (unit_idle(a, U)) return;
     const int lb = b - m.start[ui];
-    MI_KSWITCH(KMAX, (bp_unit<T, K, LDS>(a, U, s, lb, lds, (LDSP T *)ldr_raw)))
+    MI_KSWITCH(KMAX, (bp_unit<T, K, LDS, SQ>(a, U, s, lb, lds, (LDSP T *)ldr_raw, sq_off)))
 }
 
 // Fixed-order block reduction of n partials (sum or max), 256 threads.
@@ -1313,36 +1394,50 @@ void eng_fp_resid(hipStream_t st, const EngArgs<T> &a, const BlockMap &m, int s,
     MI_HIP(hipGetLastError());
 }
 
-// Dynamic LDS of a back-projection launch: the staged residual slots, and
-// after the walk the u and v squares of the regulariser (K <= kmax values).
-template <typename T>
-uint32_t bp_lds(size_t slot_bytes, int kmax) {
-    return (uint32_t)std::max(slot_bytes, (size_t)2 * kSqN * kmax * sizeof(T));
+// Dynamic LDS of a back-projection launch.  The u / v squares of the
+// regulariser either take a region of their own behind the residual slots,
+// loaded while the slots are (the kernel's SQ form, squares at element sq_off),
+// or reuse the slots' LDS after the walk.  The staged form takes its own
+// region when both fit the launch's LDS and that kernel keeps as many blocks
+// per CU (its registers and LDS, by the occupancy calculator, up to the
+// blocks per CU the grid has); the global-gather form holds only the squares.
+template <typename T, int KM, bool LDS>
+void bp_launch(hipStream_t st, unsigned grid, const EngArgs<T> &a, const BlockMap &m, int s, size_t slot_bytes) {
+    const size_t sq = (size_t)2 * kSqN * KM * sizeof(T), at = (slot_bytes + 15) & ~(size_t)15;
+    const uint32_t own = (uint32_t)(at + sq), shared = (uint32_t)std::max(slot_bytes, sq);
+    const int sq_off = (int)(at / sizeof(T));
+    if constexpr (!LDS) {
+        launch(k_eng_bp<T, KM, false, true>, dim3(grid), dim3(kBlock), own, st, a, m, s, sq_off);
+    } else {
+        bool sep = false;
+        if (at + sq <= kBpLdsLaunch) {
+            int nb_own = 0, nb_shared = 0;
+            MI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb_own, (const void *)k_eng_bp<T, KM, true, true>,
+                                                                kBlock, own));
+            MI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &nb_shared, (const void *)k_eng_bp<T, KM, true, false>, kBlock, shared));
+            int dev = 0, ncu = 1;
+            MI_HIP(hipGetDevice(&dev));
+            MI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+            const int used = (int)((grid + ncu - 1) / ncu);   // blocks per CU the launch can place
+            sep = std::min(nb_own, used) >= std::min(nb_shared, used);
+        }
+        if (sep) launch(k_eng_bp<T, KM, true, true>, dim3(grid), dim3(kBlock), own, st, a, m, s, sq_off);
+        else launch(k_eng_bp<T, KM, true, false>, dim3(grid), dim3(kBlock), shared, st, a, m, s, 0);
+    }
 }
 
 template <typename T>
 void eng_bp_grad(hipStream_t st, const EngArgs<T> &a, const BlockMap &m, int s, int kmax, size_t lds_bytes) {
     const unsigned grid = grid8(m.start[m.nunits]);
     if (lds_bytes) {
-        if (kBpLdsMax > (64 << 10)) {   // dynamic LDS above 64 KiB is opted into once per kernel
-            static bool once = [] {
-                MI_HIP(hipFuncSetAttribute((const void *)k_eng_bp<T, 1, true>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kBpLdsMax));
-                MI_HIP(hipFuncSetAttribute((const void *)k_eng_bp<T, 4, true>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kBpLdsMax));
-                MI_HIP(hipFuncSetAttribute((const void *)k_eng_bp<T, kMaxK, true>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kBpLdsMax));
-                return true;
-            }();
-            (void)once;
-        }
-        if (kmax == 1) launch(k_eng_bp<T, 1, true>, dim3(grid), dim3(kBlock), bp_lds<T>(lds_bytes, 1), st, a, m, s);
-        else if (kmax <= 4) launch(k_eng_bp<T, 4, true>, dim3(grid), dim3(kBlock), bp_lds<T>(lds_bytes, 4), st, a, m, s);
-        else launch(k_eng_bp<T, kMaxK, true>, dim3(grid), dim3(kBlock), bp_lds<T>(lds_bytes, kMaxK), st, a, m, s);
+        if (kmax == 1) bp_launch<T, 1, true>(st, grid, a, m, s, lds_bytes);
+        else if (kmax <= 4) bp_launch<T, 4, true>(st, grid, a, m, s, lds_bytes);
+        else bp_launch<T, kMaxK, true>(st, grid, a, m, s, lds_bytes);
     } else {
-        if (kmax == 1) launch(k_eng_bp<T, 1, false>, dim3(grid), dim3(kBlock), bp_lds<T>(0, 1), st, a, m, s);
-        else if (kmax <= 4) launch(k_eng_bp<T, 4, false>, dim3(grid), dim3(kBlock), bp_lds<T>(0, 4), st, a, m, s);
-        else launch(k_eng_bp<T, kMaxK, false>, dim3(grid), dim3(kBlock), bp_lds<T>(0, kMaxK), st, a, m, s);
+        if (kmax == 1) bp_launch<T, 1, false>(st, grid, a, m, s, 0);
+        else if (kmax <= 4) bp_launch<T, 4, false>(st, grid, a, m, s, 0);
+        else bp_launch<T, kMaxK, false>(st, grid, a, m, s, 0);
     }
     MI_HIP(hipGetLastError());
 }

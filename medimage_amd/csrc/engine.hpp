@@ -33,6 +33,9 @@ constexpr size_t kBpLdsMax = (size_t)MI_BP_LDS_KB << 10;
 // without opting in, with room for the kernel's static LDS).
 constexpr size_t kBpLdsHw = (size_t)56 << 10;
 static_assert(kBpLdsMax <= kBpLdsHw, "default staging limit above the tuned one");
+// Dynamic LDS of a back-projection launch, slots and u / v squares together
+// (the 64 KiB less the kernel's static LDS and a margin).
+constexpr size_t kBpLdsLaunch = (size_t)63 << 10;
 
 // Partial-sum slots of a unit (each K * cap doubles, see UnitDesc).
 enum EngSlot { SL_DF = 0, SL_DD, SL_AD, SL_NV, SL_E0, SL_E1, SL_E2, SL_E3, SL_COUNT };
