@@ -309,9 +309,15 @@ enum mi_batch_option {
     MI_BATCH_MAX_ENTRIES = 4, /* admission stops while the active jobs'
                                  matrices hold this many entries (0: no
                                  limit, the default); one job always runs  */
-    MI_BATCH_BP_LDS = 5       /* 1 (default): back projection gathers the
+    MI_BATCH_BP_LDS = 5,      /* 1 (default): back projection gathers the
                                  residual from LDS when it fits; 0: from
                                  global memory (same results, bit for bit) */
+    MI_BATCH_BP_SQ = 6        /* staged back projection: where the u / v
+                                 squares of the regulariser go; -1
+                                 (default): chosen by occupancy, 0: the
+                                 residual's LDS after the walk, 1: an LDS
+                                 region of their own, loaded before the walk
+                                 where it fits (same results, bit for bit) */
 };
 int mi_batch_set(mi_batch *b, int option, int64_t value);
 /* Queues a job (inputs are copied; u0 / v0 may be NULL).  It is admitted to

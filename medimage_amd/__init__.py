@@ -29,8 +29,8 @@ MI_BEAM_FAN_FLAT, MI_BEAM_PARALLEL, MI_BEAM_GOLDENC, MI_BEAM_FAN_ARC = 0, 1, 2, 
 MI_FP32, MI_FP64 = 0, 1
 MI_ORDER_REFERENCE, MI_ORDER_PARALLEL = 0, 1
 MI_K_FP, MI_K_BP, MI_K_FPSQ, MI_K_STEP, MI_K_EDGE, MI_K_CONE_FP, MI_K_CONE_BP = 0, 1, 2, 3, 4, 5, 6
-MI_BATCH_GRAPH, MI_BATCH_MIN_K, MI_BATCH_RESERVE_MB, MI_BATCH_MAX_K, MI_BATCH_MAX_ENTRIES, MI_BATCH_BP_LDS = \
-    0, 1, 2, 3, 4, 5
+MI_BATCH_GRAPH, MI_BATCH_MIN_K, MI_BATCH_RESERVE_MB, MI_BATCH_MAX_K, MI_BATCH_MAX_ENTRIES, MI_BATCH_BP_LDS, \
+    MI_BATCH_BP_SQ = 0, 1, 2, 3, 4, 5, 6
 MI_JOB_PENDING, MI_JOB_ACTIVE, MI_JOB_DONE, MI_JOB_FAILED = 0, 1, 2, 3
 
 

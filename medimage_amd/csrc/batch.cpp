@@ -567,7 +567,7 @@ void Batch<T>::record_pass(bool timed) {
             }), true);
             auto one = [&](int c) {
                 return [&, c](hipStream_t q) {
-                    if (bpl[c].any()) eng_bp_grad<T>(q, args, bpl[c].map, s, bpl[c].kmax, bpl[c].lds);
+                    if (bpl[c].any()) eng_bp_grad<T>(q, args, bpl[c].map, s, bpl[c].kmax, bpl[c].lds, bp_sq);
                     if (bpg[c].any()) eng_bp_grad<T>(q, args, bpg[c].map, s, bpg[c].kmax, 0);
                 };
             };

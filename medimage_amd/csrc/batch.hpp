@@ -125,6 +125,7 @@ struct Batch {
     int kmax = 1;
     bool all_walk = true;   // every unit's forward projector is the checkpointed walk
     bool use_bp_lds = true; // MI_BATCH_BP_LDS
+    int bp_sq = -1;         // MI_BATCH_BP_SQ
     int edge_iters = 0;            // max iter_edge over live jobs
     bool any_median = false;
     bool dirty = true;

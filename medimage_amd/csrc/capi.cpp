@@ -950,6 +950,11 @@ int mi_batch_set(mi_batch *b, int option, int64_t value) {
                     B.use_bp_lds = value != 0;
                     B.dirty = true;
                     break;
+                case MI_BATCH_BP_SQ:
+                    MI_REQUIRE(value >= -1 && value <= 1, MI_E_ARG, "BP_SQ must be -1, 0 or 1");
+                    B.bp_sq = (int)value;
+                    B.dirty = true;
+                    break;
                 case MI_BATCH_MAX_ENTRIES:
                     MI_REQUIRE(value >= 0, MI_E_ARG, "negative entry budget");
                     B.max_entries = value;

@@ -1402,7 +1402,8 @@ void eng_fp_resid(hipStream_t st, const EngArgs<T> &a, const BlockMap &m, int s,
 // per CU (its registers and LDS, by the occupancy calculator, up to the
 // blocks per CU the grid has); the global-gather form holds only the squares.
 template <typename T, int KM, bool LDS>
-void bp_launch(hipStream_t st, unsigned grid, const EngArgs<T> &a, const BlockMap &m, int s, size_t slot_bytes) {
+void bp_launch(hipStream_t st, unsigned grid, const EngArgs<T> &a, const BlockMap &m, int s, size_t slot_bytes,
+               int sq_mode) {
     const size_t sq = (size_t)2 * kSqN * KM * sizeof(T), at = (slot_bytes + 15) & ~(size_t)15;
     const uint32_t own = (uint32_t)(at + sq), shared = (uint32_t)std::max(slot_bytes, sq);
     const int sq_off = (int)(at / sizeof(T));
@@ -1410,7 +1411,11 @@ void bp_launch(hipStream_t st, unsigned grid, const EngArgs<T> &a, const BlockMa
         launch(k_eng_bp<T, KM, false, true>, dim3(grid), dim3(kBlock), own, st, a, m, s, sq_off);
     } else {
         bool sep = false;
-        if (at + sq <= kBpLdsLaunch) {
+        if (at + sq > kBpLdsLaunch) {
+            sep = false;
+        } else if (sq_mode >= 0) {
+            sep = sq_mode == 1;
+        } else {
             int nb_own = 0, nb_shared = 0;
             MI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb_own, (const void *)k_eng_bp<T, KM, true, true>,
                                                                 kBlock, own));
@@ -1428,16 +1433,17 @@ void bp_launch(hipStream_t st, unsigned grid, const EngArgs<T> &a, const BlockMa
 }
 
 template <typename T>
-void eng_bp_grad(hipStream_t st, const EngArgs<T> &a, const BlockMap &m, int s, int kmax, size_t lds_bytes) {
+void eng_bp_grad(hipStream_t st, const EngArgs<T> &a, const BlockMap &m, int s, int kmax, size_t lds_bytes,
+                 int sq_mode) {
     const unsigned grid = grid8(m.start[m.nunits]);
     if (lds_bytes) {
-        if (kmax == 1) bp_launch<T, 1, true>(st, grid, a, m, s, lds_bytes);
-        else if (kmax <= 4) bp_launch<T, 4, true>(st, grid, a, m, s, lds_bytes);
-        else bp_launch<T, kMaxK, true>(st, grid, a, m, s, lds_bytes);
+        if (kmax == 1) bp_launch<T, 1, true>(st, grid, a, m, s, lds_bytes, sq_mode);
+        else if (kmax <= 4) bp_launch<T, 4, true>(st, grid, a, m, s, lds_bytes, sq_mode);
+        else bp_launch<T, kMaxK, true>(st, grid, a, m, s, lds_bytes, sq_mode);
     } else {
-        if (kmax == 1) bp_launch<T, 1, false>(st, grid, a, m, s, 0);
-        else if (kmax <= 4) bp_launch<T, 4, false>(st, grid, a, m, s, 0);
-        else bp_launch<T, kMaxK, false>(st, grid, a, m, s, 0);
+        if (kmax == 1) bp_launch<T, 1, false>(st, grid, a, m, s, 0, sq_mode);
+        else if (kmax <= 4) bp_launch<T, 4, false>(st, grid, a, m, s, 0, sq_mode);
+        else bp_launch<T, kMaxK, false>(st, grid, a, m, s, 0, sq_mode);
     }
     MI_HIP(hipGetLastError());
 }
@@ -1535,7 +1541,7 @@ void eng_copy(hipStream_t st, void *dst, const void *src, int64_t bytes) {
 
 #define MI_ENG_INST(T)                                                                                     \
     template void eng_fp_resid<T>(hipStream_t, const EngArgs<T> &, const BlockMap &, int, int, bool);       \
-    template void eng_bp_grad<T>(hipStream_t, const EngArgs<T> &, const BlockMap &, int, int, size_t);      \
+    template void eng_bp_grad<T>(hipStream_t, const EngArgs<T> &, const BlockMap &, int, int, size_t, int); \
     template void eng_fp_sq<T>(hipStream_t, const EngArgs<T> &, const BlockMap &, int, int, bool);         \
     template void eng_nv<T>(hipStream_t, const EngArgs<T> &, const BlockMap &, int);                        \
     template void eng_axpy<T>(hipStream_t, const EngArgs<T> &, const BlockMap &, int);                      \

@@ -104,8 +104,11 @@ template <typename T>
 void eng_fp_resid(hipStream_t st, const EngArgs<T> &a, const BlockMap &m, int s, int kmax, bool all_walk);
 template <typename T>
 // lds_bytes: 0, or the dynamic LDS of the staged-residual form (every unit
-// has slot codes and max_slots * K * sizeof(T) fits).
-void eng_bp_grad(hipStream_t st, const EngArgs<T> &a, const BlockMap &m, int s, int kmax, size_t lds_bytes);
+// has slot codes and max_slots * K * sizeof(T) fits).  sq_mode, staged form
+// only: where the u / v squares of the regulariser go (-1: by occupancy, 0:
+// the slots' LDS after the walk, 1: a region of their own where it fits).
+void eng_bp_grad(hipStream_t st, const EngArgs<T> &a, const BlockMap &m, int s, int kmax, size_t lds_bytes,
+                 int sq_mode = -1);
 // ||A_s d||^2 (rows map; the walk of the residual launch without the
 // residual) and nV(d) (pixel map), as two launches.
 template <typename T>

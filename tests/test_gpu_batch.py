@@ -230,12 +230,16 @@ def test_batch_entry_budget_and_max_k(ctx):
 def test_batch_bp_lds_equals_global_gathers(ctx, precision):
     """The back projection that stages each block's residual values in LDS
     and the one that gathers them from global memory sum the same values in
-    the same order: bitwise equal results."""
+    the same order: bitwise equal results.  So do the two places of the
+    regulariser's u / v squares in the staged form (their own LDS region,
+    loaded before the walk, or the residual's LDS after it), forced for
+    every staged launch (units of K = 1, 4 and 8)."""
     jobs = _jobs(ctx, precision)
     a = _run_batch(ctx, jobs, precision)
-    b = _run_batch(ctx, jobs, precision, options=[(mi.MI_BATCH_BP_LDS, 0)])
-    for i in range(len(jobs)):
-        assert _same(a[i][0], b[i][0]) and _same(a[i][1], b[i][1]), f"job {i}"
+    for opts in ([(mi.MI_BATCH_BP_LDS, 0)], [(mi.MI_BATCH_BP_SQ, 0)], [(mi.MI_BATCH_BP_SQ, 1)]):
+        b = _run_batch(ctx, jobs, precision, options=opts)
+        for i in range(len(jobs)):
+            assert _same(a[i][0], b[i][0]) and _same(a[i][1], b[i][1]), f"job {i}, options {opts}"
 
 
 @pytest.mark.parametrize("precision", [mi.MI_FP32, mi.MI_FP64])
@@ -287,6 +291,9 @@ def test_batch_release_and_errors(ctx):
     assert e.value.code == mi.MI_E_ARG
     with pytest.raises(mi.MIError) as e:
         b.submit(g, sino, mi.at_params(n_subsets=3))
+    assert e.value.code == mi.MI_E_ARG
+    with pytest.raises(mi.MIError) as e:
+        b.set(mi.MI_BATCH_BP_SQ, 2)
     assert e.value.code == mi.MI_E_ARG
     b.close()
 
