@@ -53,7 +53,15 @@ namespace {
 
 constexpr int kConeTile = 16;   // forward: 16 x 16 panel pixels per workgroup
 constexpr int kConeGroup = 8;   // forward: walk steps per load group
-constexpr int kZc = 8;          // back projection: slices per lane
+// Back projection: slices per lane.  A lane's per-view work on its column
+// (footprint, panel columns, x / y clip) is shared by its kZc slices, whose
+// double sums live in LDS (kZc x 2 KiB per workgroup, beside the rows' 1 / w):
+// C4 (512^3, 720 views) back projection 644 ms with 6 (18 KiB, 8 workgroups
+// per CU), 663 with 7, 674 with 5, 680 with 8 (22 KiB, 7 per CU), 689 with 4.
+#ifndef MI_CONE_ZC
+#define MI_CONE_ZC 6
+#endif
+constexpr int kZc = MI_CONE_ZC;
 constexpr int kBpViewChunk = 36; // back projection: views per launch (their rows stay in the caches)
 constexpr int kBpRowLds = 4096;  // back projection: panels up to this many rows keep 1 / w in LDS
 
@@ -529,11 +537,11 @@ __global__ __launch_bounds__(kBlock) void k_cone_bp(ConeDev g, const float *cs, 
                 const float whi = fmaxf(zb * imin, zb * imax) * kcol;
                 const int r0 = max(0, (int)ceilf(h5 - whi - margin));
                 const int r1 = min(g.nrows - 1, (int)floorf(h5 - wlo + margin));
-                const double dz0 = (double)z - sz;
+                const double dz0 = (double)z - sz, dz1 = dz0 + 1.0;
                 double part = 0.0;
                 for (int r = r0; r <= r1; ++r) {
                     const double iw = RLDS ? riw[r] : rowtab[2 * r + 1];
-                    const float ta = (float)fma(dz0, iw, nlo), tb = (float)fma(dz0 + 1.0, iw, nlo);
+                    const float ta = (float)fma(dz0, iw, nlo), tb = (float)fma(dz1, iw, nlo);
                     const float a = fmaxf(0.0f, fminf(ta, tb)), e = fminf(span, fmaxf(ta, tb));
                     if (!(e > a)) continue;
                     // w = 1 / iw and the ray length to about one float ulp
