@@ -422,7 +422,7 @@ void Batch<T>::rebuild() {
     any_median = false;
     pix.nunits = (int)live.size();
     pix.start[0] = 0;
-    for (Launch *L : {&rows[0], &rows[1], &bpl[0], &bpl[1], &bpg[0], &bpg[1]}) {
+    for (Launch *L : {&rows[0], &rows[1], &bpl[0][0], &bpl[0][1], &bpl[1][0], &bpl[1][1], &bpg[0], &bpg[1]}) {
         *L = Launch{};
         L->map.nunits = (int)live.size();
         L->map.start[0] = 0;
@@ -466,7 +466,8 @@ void Batch<T>::rebuild() {
         const size_t need = m.bs.slot16.p ? (size_t)m.bs.max_slots * U.K * sizeof(T) : kBpLdsHw + 1;
         const bool in_lds = bp_in_lds(m, need);
         const int rc = U.K > 1 ? 1 : 0, bc = U.K > 4 ? 1 : 0;   // the unit's classes
-        Launch &B = in_lds ? bpl[bc] : bpg[bc];
+        const int lc = need > kBpLdsSplit ? 1 : 0;
+        Launch &B = in_lds ? bpl[bc][lc] : bpg[bc];
         B.kmax = std::max(B.kmax, U.K);
         if (in_lds) B.lds = std::max(B.lds, need);
         rows[rc].kmax = std::max(rows[rc].kmax, U.K);
@@ -492,7 +493,9 @@ void Batch<T>::rebuild() {
         MI_REQUIRE(m.has_sell && (!f64 || m.has64), MI_E_STATE, "matrix lacks the engine's sliced layout");
         for (int c = 0; c < 2; ++c) {
             rows[c].map.start[i + 1] = rows[c].map.start[i] + (c == rc ? (int)fp_blocks(m) : 0);
-            bpl[c].map.start[i + 1] = bpl[c].map.start[i] + (in_lds && c == bc ? (int)bp_blocks(m) : 0);
+            for (int l = 0; l < 2; ++l)
+                bpl[c][l].map.start[i + 1] =
+                    bpl[c][l].map.start[i] + (in_lds && c == bc && l == lc ? (int)bp_blocks(m) : 0);
             bpg[c].map.start[i + 1] = bpg[c].map.start[i] + (!in_lds && c == bc ? (int)bp_blocks(m) : 0);
         }
         pix.start[i + 1] = pix.start[i] + (int)px_blocks(m);
@@ -567,11 +570,13 @@ void Batch<T>::record_pass(bool timed) {
             }), true);
             auto one = [&](int c) {
                 return [&, c](hipStream_t q) {
-                    if (bpl[c].any()) eng_bp_grad<T>(q, args, bpl[c].map, s, bpl[c].kmax, bpl[c].lds, bp_sq);
+                    for (const Launch &L : bpl[c])
+                        if (L.any()) eng_bp_grad<T>(q, args, L.map, s, L.kmax, L.lds, bp_sq);
                     if (bpg[c].any()) eng_bp_grad<T>(q, args, bpg[c].map, s, bpg[c].kmax, 0);
                 };
             };
-            branches(one(0), one(1), bpl[0].any() || bpg[0].any(), bpl[1].any() || bpg[1].any());
+            branches(one(0), one(1), bpl[0][0].any() || bpl[0][1].any() || bpg[0].any(),
+                     bpl[1][0].any() || bpl[1][1].any() || bpg[1].any());
         }
         {
             LaunchTimer lt(ctx, MI_K_FPSQ, sum([&](const Unit<T> &U) {

@@ -109,7 +109,9 @@ struct Batch {
     // projection by K <= 4 / K = 8.  Back projection: units whose residual
     // slots fit the staging limit in the LDS launches (bpl), the others in
     // the global-gather launches (bpg); a unit has blocks in exactly one map
-    // of each phase.  The two classes of a phase run as concurrent branches
+    // of each phase.  The LDS launches are split once more by the units'
+    // staging bytes (up to kBpLdsSplit, above), as a launch gives every block
+    // the LDS of its largest unit and with it the blocks per CU.  The two classes of a phase run as concurrent branches
     // (second stream, forked and joined by events; captured as parallel
     // graph nodes).
     struct Launch {
@@ -118,7 +120,7 @@ struct Batch {
         size_t lds = 0;
         bool any() const { return map.start[map.nunits] > 0; }
     };
-    Launch rows[2], bpl[2], bpg[2];
+    Launch rows[2], bpl[2][2], bpg[2];   // bpl[K class][LDS class]
     hipStream_t side = nullptr;   // the second class of a phase
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     EngArgs<T> args{};

@@ -33,6 +33,13 @@ constexpr size_t kBpLdsMax = (size_t)MI_BP_LDS_KB << 10;
 // without opting in, with room for the kernel's static LDS).
 constexpr size_t kBpLdsHw = (size_t)56 << 10;
 static_assert(kBpLdsMax <= kBpLdsHw, "default staging limit above the tuned one");
+// Staged back-projection units needing more than this many bytes of slots
+// run in a launch of their own (a launch gives each block its largest unit's
+// LDS).
+#ifndef MI_BP_LDS_SPLIT_KB
+#define MI_BP_LDS_SPLIT_KB 20
+#endif
+constexpr size_t kBpLdsSplit = (size_t)MI_BP_LDS_SPLIT_KB << 10;
 // Dynamic LDS of a back-projection launch, slots and u / v squares together
 // (the 64 KiB less the kernel's static LDS and a margin).
 constexpr size_t kBpLdsLaunch = (size_t)63 << 10;
