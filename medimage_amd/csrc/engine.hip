@@ -434,7 +434,7 @@ __device__ __forceinline__ void walk_slice(const GL int64_t *base, const GL int3
 template <typename T, int K>
 __device__ __forceinline__ void walk_otf(const UG<T> &U, int64_t q, const GL T *__restrict__ x, Acc<T> (&acc)[K]) {
 #ifndef MI_OTF_OS8
-#define MI_OTF_OS8 2
+#define MI_OTF_OS8 2   // steps in flight at K = 8 (4: 116 VGPRs, slower at every image size)
 #endif
     constexpr int OS = (K <= 2) ? 8 : (K == 4 ? 4 : MI_OTF_OS8);
     constexpr uint32_t E = K * sizeof(T), LE = __builtin_ctz(E);
