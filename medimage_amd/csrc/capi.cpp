@@ -163,8 +163,10 @@ LaunchTimer::LaunchTimer(mi_context *c, int k, double by, bool disp) : ctx(c), k
     auto &pool = ctx->stats.pool;
     auto take = [&]() {
         hipEvent_t e;
+        // timing only, read after a stream synchronize: no system-scope
+        // fence (cache write-back) when the event is recorded
         if (!pool.empty()) { e = pool.back(); pool.pop_back(); }
-        else MI_HIP(hipEventCreate(&e));
+        else MI_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
         return e;
     };
     a = take();
