@@ -254,9 +254,6 @@ typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 #define LDSP __attribute__((address_space(3)))
 enum IdxMode { IM_ABS32 = 0, IM_DELTA16 = 1, IM_SLOT16 = 2 };
 
-// IM: absolute 32-bit indices (ip), 16-bit differences (dp, from `cur`), or
-// 16-bit LDS slots (dp read unsigned).  LDSG: gather from `lx` (LDS) instead
-// of the global array x.
 // Stream registers of one pipeline stage of walk: GR groups of kGroup
 // entries (index word and length) per lane.
 template <typename T, int K>
@@ -311,6 +308,10 @@ __device__ __forceinline__ void walk_prime(Stage<T, K> &b, const GL uint32_t *ip
     if (n / kGroup / Stage<T, K>::GR > 0) stage_fetch<T, K, NT, IM>(b, ip, dp, lp, 0);
 }
 
+// IM: absolute 32-bit indices (ip), 16-bit differences (dp, from `cur`), or
+// 16-bit LDS slots (dp read unsigned).  LDSG: gather from `lx` (LDS) instead
+// of the global array x.  b holds the stage in flight; with PRIMED the caller
+// has issued the first one (walk_prime).
 template <typename T, int K, bool NT, int IM, bool LDSG = false, bool PRIMED = false>
 __device__ __forceinline__ void walk(const GL uint32_t *__restrict__ ip, const GL int16_t *__restrict__ dp,
                                      uint32_t cur, const GL T *__restrict__ lp, int n, const GL T *__restrict__ x,
