@@ -468,8 +468,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--cpu-passes", type=int, default=10)
-    ap.add_argument("--cpu-passes-c1", type=int, default=5)
+    ap.add_argument("--cpu-passes", type=int, default=80)      # about 10 s of the CPU port at C2
+    ap.add_argument("--cpu-passes-c1", type=int, default=500)  # about 10 s at C1
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--no-goldenc", action="store_true")
