@@ -13,7 +13,7 @@ Times, on this host:
   msbeam/CPU_OpenMP/src/main.c:196-202) at 1 and 8 threads, both builds;
   against the oracle's one-thread restatement (or_msbeam_solve).
 
-Prints one JSON object; the figures go to BASELINE.md.
+Prints one JSON object with the timings.
 """
 import ctypes as C
 import json
