@@ -364,6 +364,97 @@ def bench_serving_full(ctx, n_jobs, rate, log, tune=True, workers=16):
     return out
 
 
+def bench_helical(ctx, n_slices, iters, log, cpu_passes=20):
+    """The helical per-slice path at the reference's clinical geometry
+    (helical/rebin/json/UCLA_PHANTOM/SCAN1_1.5mm.json: 874 x 874 at 0.5859 mm,
+    736 arc detector elements of 0.06786 degrees, SOD 595 mm, SDD 1085.6 mm,
+    off-centre 1.75 elements; 1152 views per slice): n_slices rebinned slices,
+    each with its own uneven source-angle table, written as <prefix>i_pos /
+    <prefix>i_raw, then (a) every slice through mi_at_reconstruct one at a
+    time and (b) all of them through the per-slice driver
+    (mi_helical_reconstruct, slices batched).  Matrices are built before the
+    clock starts.  Each slice's matrix is its own (its angle table), so the
+    batch shares no matrix: (b) gains only from running the slices together."""
+    import tempfile
+    import medimage_amd as mi
+    from medimage_amd import phantom
+    views, n = 1152, 874
+    g0 = mi.arc_geometry(n, 0.5859, 1, 736, 0.06786112, sod=595.0, sdd=1085.6, off_center=1.75)
+    p = mi.at_params(0.1, 0.01, 1e-3, n_subsets=8, max_iters=iters)
+    rng = np.random.default_rng(11)
+    tmp = tempfile.mkdtemp(prefix="helical_")
+    prefix = os.path.join(tmp, "slice")
+    t = time.perf_counter()
+    slices = []
+    nnz = 0
+    for i in range(n_slices):
+        steps = rng.uniform(0.6, 1.4, views) * (360.0 / views)
+        angles = np.cumsum(steps) + rng.uniform(0.0, 360.0)
+        g = mi.arc_geometry(n, 0.5859, views, 736, 0.06786112, sod=595.0, sdd=1085.6,
+                            off_center=1.75).with_views(angles)
+        ell = phantom.random_ellipses(10, n * 0.5859, 500 + i, mu_max=0.02)
+        img = phantom.rasterize(ell, n, 0.5859)
+        sino = phantom.noisy_sinogram(ctx.forward(g, img, np.float32), 5e4, 500 + i).astype(np.float32)
+        # the files hold exactly these values: the angles and the float32
+        # sinogram as the doubles they are (17 significant digits)
+        with open(f"{prefix}{i}_pos", "w") as f:
+            for v, a in enumerate(angles):
+                f.write(f"{float(v) * 0.5!r},{float(a)!r}\n")
+        np.savetxt(f"{prefix}{i}_raw", sino.astype(np.float64), fmt="%.17g")
+        ctx.trim()   # the projector's matrix of this slice
+        slices.append((g, sino))
+        log(f"[helical] slice {i}: {views} views written ({time.perf_counter() - t:.1f} s)")
+    for g, _ in slices:
+        nnz += ctx.prepare(g, 8)
+    log(f"[helical] engine matrices of {n_slices} slices ready ({time.perf_counter() - t:.1f} s)")
+    ctx.at_reconstruct(slices[0][0], slices[0][1], mi.at_params(0.1, 0.01, 1e-3, n_subsets=8, max_iters=1))  # warm
+    t0 = time.perf_counter()
+    alone = [ctx.at_reconstruct(g, s, p) for g, s in slices]
+    t_plain = time.perf_counter() - t0
+    log(f"[helical] one at a time: {n_slices} x {iters} iterations in {t_plain:.2f} s")
+    # the driver's own file reading, timed alone with the same readers
+    t0 = time.perf_counter()
+    for i in range(n_slices):
+        _, ang = mi.read_pos(f"{prefix}{i}_pos")
+        mi.read_raw(f"{prefix}{i}_raw", len(ang), 736)
+    t_read = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    u, v, reps = ctx.helical_reconstruct(prefix, 0, n_slices, g0, p)
+    t_drv = time.perf_counter() - t0
+    log(f"[helical] per-slice driver (files read, slices batched): {t_drv:.2f} s, of which reading the files "
+        f"alone takes {t_read:.2f} s")
+    same = sum(1 for k, (ua, va, _) in enumerate(alone)
+               if np.array_equal(u[k].view(np.uint32), ua.view(np.uint32))
+               and np.array_equal(v[k].view(np.uint32), va.view(np.uint32)))
+    for i in range(n_slices):
+        os.remove(f"{prefix}{i}_pos")
+        os.remove(f"{prefix}{i}_raw")
+    os.rmdir(tmp)
+    work = n_slices * iters
+    out = {"workload": f"{n_slices} rebinned helical slices at the clinical geometry: 874x874 (0.5859 mm), "
+                       f"1152 views each with its own source-angle table, 736 arc elements (0.06786 deg), "
+                       f"SOD 595 mm, SDD 1085.6 mm, 8 subsets, {iters} AT iterations, alpha 0.1, beta 0.01, eps 1e-3",
+           "matrix_entries_per_slice": nnz / n_slices,
+           "plain_iters_per_s": work / t_plain, "driver_iters_per_s": work / t_drv,
+           "driver_wall_s": t_drv, "read_files_s": t_read,
+           "driver_iters_per_s_without_reads": work / max(t_drv - t_read, 1e-9),
+           "slices_bitwise_identical": same,
+           "driver_includes": "reading the slice files (text) and submitting every slice"}
+    if cpu_passes > 0:
+        sys.path.insert(0, str(ROOT / "tests"))
+        import oracle_py as O
+        host = host_cpus()
+        g, s = slices[0]
+        t0 = time.perf_counter()
+        O.cpu_os_at_f32(O.op_fan(O.fan_from(g)), s, 0.1, 0.01, 1e-3, 8, cpu_passes, iter_edge=1,
+                        threads=host["threads"])
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": cpu_passes / dt, "unit": "AT iterations/s", "cores": host["threads"],
+                               "kind": "port", "sample": f"{cpu_passes} AT iteration(s) of slice 0, FP32 CPU port, "
+                                                         f"{host['threads']} threads, {dt:.1f} s"}
+    return out
+
+
 def bench_k_scaling(ctx, log):
     """Jobs per unit: for geometries of the C5 mix, the time per job and
     pass of a unit of K identical jobs (K = 1, 2, 4, 8, set through the
@@ -484,6 +575,10 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-full", action="store_true",
                     help="only the whole C5 stream (--serve-jobs 2000 by default here): batched live, plain derived")
+    ap.add_argument("--helical", action="store_true",
+                    help="the helical per-slice path at the clinical arc geometry (one JSON line of its own)")
+    ap.add_argument("--helical-slices", type=int, default=8)
+    ap.add_argument("--helical-iters", type=int, default=10)
     ap.add_argument("--k-scaling", action="store_true",
                     help="only the jobs-per-unit scaling of four C5 geometries (K = 1, 2, 4, 8)")
     ap.add_argument("--c4-reps", type=int, default=2)
@@ -503,6 +598,12 @@ def main():
     ctx = mi.Context(0)
     if args.k_scaling:
         print(json.dumps({"k_scaling": bench_k_scaling(ctx, lambda m: print(m, file=sys.stderr, flush=True))}))
+        ctx.close()
+        return
+    if args.helical:
+        out = bench_helical(ctx, args.helical_slices, args.helical_iters,
+                            lambda m: print(m, file=sys.stderr, flush=True), 0 if args.no_cpu else 20)
+        print(json.dumps({"helical_slices": out}))
         ctx.close()
         return
     if args.serve_full:
