@@ -106,6 +106,26 @@ def c1_gpu(ctx, iters):
             "elapsed_ms": rep.elapsed_ms}, sino
 
 
+def msbeam_gpu(ctx, iters):
+    """The parallel beam at msbeam's own configuration (msbeam/CPU_OpenMP/src/
+    proto.h: 512 x 512, 180 views over 180 degrees, 768 rays; its tracer,
+    bit-exact): the plain per-job ordered-subset AT solve with one subset
+    (every ray each iteration, as msbeam's sweeps), FP32 (FP64 reductions)."""
+    import medimage_amd as mi
+    from medimage_amd import configs
+    g = mi.parallel_geometry(512, 180, 768, flavour=0)
+    _, sino = configs.make_case(ctx, g, 1e5, seed=6)
+    p = mi.at_params(configs.ALPHA, configs.BETA, configs.EPS, n_subsets=1, max_iters=iters)
+    ctx.at_reconstruct(g, sino.astype(np.float32), mi.at_params(configs.ALPHA, configs.BETA, configs.EPS,
+                                                                n_subsets=1, max_iters=2))  # warm
+    u, v, rep = ctx.at_reconstruct(g, sino.astype(np.float32), p)
+    return {"workload": f"msbeam parallel beam: 512x512, 180 views over 180 degrees, 768 rays, one subset, "
+                        f"{rep.iterations} AT iterations, FP32 (FP64 reductions)",
+            "at_iterations_per_s": rep.iterations / (rep.elapsed_ms * 1e-3),
+            "rays_per_s": rep.iterations * g.n_views * g.n_bins / (rep.elapsed_ms * 1e-3),
+            "elapsed_ms": rep.elapsed_ms}
+
+
 def bench_goldenc(ctx, log, cpu=True):
     """The GoldenC drop-in at the reference program's own configuration
     (origin/GoldenC/src/config.h: 512 x 512, 180 views x 767 bins,
@@ -724,6 +744,7 @@ def main():
     sino_c1 = None
     if not args.no_c1:
         out["c1"], sino_c1 = c1_gpu(ctx, 50)
+        out["msbeam_parallel"] = msbeam_gpu(ctx, 50)
     if not args.no_goldenc:
         out["goldenc"] = bench_goldenc(ctx, lambda m: print(m, file=sys.stderr, flush=True), not args.no_cpu)
     if args.copy_gib > 0:
