@@ -5,6 +5,8 @@
 #include "ordered.hpp"
 #include "solver.hpp"
 
+#include <cctype>
+#include <charconv>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -12,6 +14,7 @@
 #include <cstring>
 #include <limits>
 #include <string>
+#include <vector>
 
 namespace mi {
 
@@ -1344,13 +1347,39 @@ int mi_read_pos(const char *path, double *pos_mm, double *angle_deg, int cap, in
 int mi_read_raw(const char *path, int64_t n_views, int64_t n_bins, double *sino) {
     return guarded([&] {
         MI_REQUIRE(path && sino && n_views > 0 && n_bins > 0, MI_E_ARG, "bad arguments");
-        FILE *fp = std::fopen(path, "r");
+        FILE *fp = std::fopen(path, "rb");
         MI_REQUIRE(fp, MI_E_ARG, std::string("cannot open sinogram file ") + path);
-        // ms2d.cpp:270-274: whitespace-separated values, [view][bin]
+        // ms2d.cpp:270-274: whitespace-separated values, [view][bin].  The
+        // file is read whole and parsed with from_chars (correctly rounded,
+        // the values fscanf's %lf gives, several times faster); a token it
+        // does not take (a leading '+', a value out of range) goes through
+        // strtod, as fscanf would.
+        std::vector<char> buf;
+        {
+            char chunk[1 << 16];
+            size_t k;
+            while ((k = std::fread(chunk, 1, sizeof chunk, fp)) > 0) buf.insert(buf.end(), chunk, chunk + k);
+            std::fclose(fp);
+        }
+        buf.push_back('\0');
+        const char *q = buf.data(), *end = buf.data() + buf.size() - 1;
         const int64_t want = n_views * n_bins;
         int64_t got = 0;
-        while (got < want && std::fscanf(fp, "%lf", &sino[got]) == 1) ++got;
-        std::fclose(fp);
+        while (got < want) {
+            while (q < end && std::isspace((unsigned char)*q)) ++q;
+            if (q == end) break;
+            double x;
+            const auto r = std::from_chars(q, end, x);
+            if (r.ec == std::errc()) {
+                q = r.ptr;
+            } else {
+                char *e = nullptr;
+                x = std::strtod(q, &e);
+                if (e == q) break;   // not a number
+                q = e;
+            }
+            sino[got++] = x;
+        }
         MI_REQUIRE(got == want, MI_E_ARG, std::string(path) + ": " + std::to_string(got) + " values, expected " +
                                               std::to_string(want));
     });

@@ -102,6 +102,13 @@ def test_helical_slice_pos_and_raw_files(tmp_path):
         for row in sino:
             f.write(" ".join(repr(float(x)) for x in row) + "\n")
     assert np.array_equal(mi.read_raw(tmp_path / "4_raw", 37, 12), sino)
+    # the forms fscanf's %lf takes: a leading '+', tabs, exponents, a value
+    # out of range (infinity)
+    with open(tmp_path / "7_raw", "w") as f:
+        f.write("+1.5\t-2e-3\n  1E+2 0.1\n1e400 -0\n")
+    r7 = mi.read_raw(tmp_path / "7_raw", 3, 2)
+    assert np.array_equal(r7, np.array([[1.5, -2e-3], [100.0, 0.1], [np.inf, -0.0]]))
+    assert np.signbit(r7[2, 1])
     # a short sinogram is an error, as is a missing table
     with pytest.raises(mi.MIError) as e:
         mi.read_raw(tmp_path / "4_raw", 38, 12)
