@@ -5,6 +5,8 @@
 #include "ordered.hpp"
 #include "solver.hpp"
 
+#include <algorithm>
+#include <atomic>
 #include <cctype>
 #include <charconv>
 #include <cmath>
@@ -14,6 +16,7 @@
 #include <cstring>
 #include <limits>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace mi {
@@ -370,24 +373,60 @@ void helical_slices(mi_context *ctx, const char *prefix, int first, int count, c
     const int64_t npix = (int64_t)g0.nx * g0.ny;
     Batch<T> b(ctx, p.n_subsets);
     std::vector<int64_t> ids(count);
-    std::vector<double> stage;
-    std::vector<T> sino;
+    // The slices' files are read on host threads (up to 16 at once), then the
+    // slices are submitted in order; the first slice (in order) whose files
+    // do not read gives the error.
+    struct SliceIn {
+        std::vector<double> angle;
+        std::vector<T> sino;
+        std::string err;
+    };
+    std::vector<SliceIn> in(count);
+    std::atomic<int> next{0};
+    auto load = [&] {
+        std::vector<double> stage;
+        for (int k; (k = next.fetch_add(1)) < count;) {
+            SliceIn &S = in[k];
+            try {
+                const std::string base = std::string(prefix) + std::to_string(first + k);
+                // the slice's table fixes its view count and angles (load_table)
+                int nv = 0;
+                if (mi_read_pos((base + "_pos").c_str(), nullptr, nullptr, 0, &nv) != MI_OK) {
+                    S.err = mi_last_error();
+                    continue;
+                }
+                if (nv < p.n_subsets) {
+                    S.err = base + "_pos: fewer views than subsets";
+                    continue;
+                }
+                S.angle.resize(nv);
+                if (mi_read_pos((base + "_pos").c_str(), nullptr, S.angle.data(), nv, &nv) != MI_OK) {
+                    S.err = mi_last_error();
+                    continue;
+                }
+                stage.resize((size_t)nv * g0.n_bins);
+                if (mi_read_raw((base + "_raw").c_str(), nv, g0.n_bins, stage.data()) != MI_OK) {
+                    S.err = mi_last_error();
+                    continue;
+                }
+                S.sino.assign(stage.begin(), stage.end());
+            } catch (const std::exception &e) {
+                S.err = e.what();
+            }
+        }
+    };
+    const int nt = std::max(1, std::min({count, 16, (int)std::thread::hardware_concurrency()}));
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back(load);
+    load();
+    for (auto &t : pool) t.join();
     for (int k = 0; k < count; ++k) {
-        const std::string base = std::string(prefix) + std::to_string(first + k);
-        // the slice's table fixes its view count and angles (load_table)
-        int nv = 0;
-        if (mi_read_pos((base + "_pos").c_str(), nullptr, nullptr, 0, &nv) != MI_OK) fail(MI_E_ARG, mi_last_error());
-        MI_REQUIRE(nv >= p.n_subsets, MI_E_ARG, base + "_pos: fewer views than subsets");
-        std::vector<double> angle(nv);
-        if (mi_read_pos((base + "_pos").c_str(), nullptr, angle.data(), nv, &nv) != MI_OK)
-            fail(MI_E_ARG, mi_last_error());
-        stage.resize((size_t)nv * g0.n_bins);
-        if (mi_read_raw((base + "_raw").c_str(), nv, g0.n_bins, stage.data()) != MI_OK) fail(MI_E_ARG, mi_last_error());
-        sino.assign(stage.begin(), stage.end());
+        MI_REQUIRE(in[k].err.empty(), MI_E_ARG, in[k].err);
         mi_geometry g = g0;
-        g.n_views = nv;
-        g.view_deg = angle.data();
-        ids[k] = b.submit(g, p, sino.data(), nullptr, nullptr);
+        g.n_views = (int)in[k].angle.size();
+        g.view_deg = in[k].angle.data();
+        ids[k] = b.submit(g, p, in[k].sino.data(), nullptr, nullptr);
+        std::vector<T>().swap(in[k].sino);   // the batch holds its own copy
     }
     // Every slice advances in the same passes; finished slices leave (their
     // result goes to host memory) and pending ones join as device memory

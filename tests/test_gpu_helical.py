@@ -80,3 +80,24 @@ def test_helical_per_slice_driver(ctx, tmp_path):
         assert reps[k].iterations == ra.iterations == 4
         assert np.array_equal(u[k].view(np.uint32), ua.view(np.uint32))
         assert np.array_equal(v[k].view(np.uint32), va.view(np.uint32))
+
+
+def test_helical_driver_reports_the_first_unreadable_slice(ctx, tmp_path):
+    """The slices' files are read together; the error names the first slice,
+    in slice order, whose files do not read (here slice 1's missing _raw,
+    although slice 2's table is malformed too)."""
+    g0 = mi.arc_geometry(96, 3.0, 1, 128, 0.2, off_center=1.75)
+    prefix = str(tmp_path / "s")
+    for i in range(3):
+        g = _slice_geometry(60, 20 + i)
+        with open(f"{prefix}{i}_pos", "w") as f:
+            for v, a in enumerate(g._views):
+                f.write(f"{float(v) * 0.5!r},{float(a)!r}\n")
+        if i != 1:
+            np.savetxt(f"{prefix}{i}_raw", np.zeros((60, 128)), fmt="%.17g")
+    with open(f"{prefix}2_pos", "w") as f:
+        f.write("0.0,1.0\n")   # one view: fewer than the subsets
+    p = mi.at_params(1e-3, 1e-4, 1e-2, n_subsets=4, max_iters=2)
+    with pytest.raises(mi.MIError) as e:
+        ctx.helical_reconstruct(prefix, 0, 3, g0, p)
+    assert e.value.code == mi.MI_E_ARG and "s1_raw" in str(e.value), str(e.value)
