@@ -253,8 +253,10 @@ int mi_at_reconstruct_f64(mi_context *ctx, const mi_geometry *g, const mi_at_par
  * the reference builds them from rebined_folder), takes the slice's view
  * count and per-view source angles from the table (g->n_views, span_deg and
  * view_deg are not used), and reconstructs it with the ordered-subset AT
- * form of p (u = 0, v = 1 at the start, as ms2d.cpp:275-276).  The slices
- * run together in one batch, admitted as device memory allows.  g: the arc
+ * form of p (u = 0, v = 1 at the start, as ms2d.cpp:275-276).  The files
+ * are read on up to 16 host threads; an unreadable slice fails the call with
+ * MI_E_ARG, naming the first such slice in order.  The slices run together
+ * in one batch, admitted as device memory allows.  g: the arc
  * geometry every slice shares (MI_BEAM_FAN_ARC: image size, pixel, bins,
  * LENGTH_PER_DET, SOURCE_TO_ISO / _DET, OFF_CENTER).  u_stack / v_stack:
  * count slices of nx * ny, slice-major, float for MI_FP32 and double for
